@@ -1,0 +1,859 @@
+// fdga.hip -- C-ABI of the MI355X parquet hot path (include/fdga.h).
+//
+// A context owns every device array of one ParquetSolver / NL_ParquetSolver
+// (src/ParquetSolver.jl:5-157, src/nonlocal/ParquetSolver.jl:1-153) and one HIP stream.
+// Host arrays cross the boundary only in fdga_set / fdga_get / fdga_(un)flatten.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fdga.h"
+#include "fdga_kernels.h"
+
+using namespace fdga;
+
+namespace {
+
+struct DevChan {
+    double2 *K1 = nullptr, *K2 = nullptr, *K3 = nullptr;
+    double2 *A1 = nullptr, *A2 = nullptr, *A3 = nullptr;   // averages (alias K* unless own_avg)
+    bool own_avg = false;                                   // A* allocated separately
+};
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+struct fdga_ctx {
+    fdga_dims d{};
+    hipStream_t st = nullptr;
+    std::string err;
+    int NP = 1, NkG = 1, NG = 0;
+    int NW1 = 0, NW2 = 0, NV2 = 0, NW3 = 0, NV3 = 0, NWP = 0, NVP = 0, nVP = 0;
+    size_t nGel = 0, nPiel = 0, n1 = 0, n2 = 0, n3 = 0, nL = 0;
+    double2 *Gbare = nullptr, *G0 = nullptr, *Sig0 = nullptr, *G = nullptr, *Sig = nullptr;
+    double2 *Pi0pp = nullptr, *Pi0ph = nullptr, *Pipp = nullptr, *Piph = nullptr;
+    double2 *Lpp = nullptr, *Lph = nullptr, *W = nullptr;
+    double2 *sde0 = nullptr;
+    int sde0_valid = 0, sde0_h = -1;
+    double2 *tab[4] = {nullptr, nullptr, nullptr, nullptr};
+    int tabs_nonzero = 0;
+    double2 U{};
+    DevChan F[3], Fb[3], FL[3], R0[3];
+    double2 *cache[10] = {};
+    double2 *tmpG1 = nullptr, *tmpG2 = nullptr, *tmpG3 = nullptr;   // [NG, NkG] scratch
+    double2 *tmpL1 = nullptr, *tmpL2 = nullptr;                     // [NW2*NV2, NP] scratch
+    double2 *u2C = nullptr;                                         // [4nG-1, NkG]
+    double *scal = nullptr;                                         // device scalars
+    double2 *stage = nullptr;                                       // staging for layout changes
+    size_t stage_n = 0;
+    // timing
+    int timing = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double stage_ms[9] = {};
+};
+
+namespace {
+
+int fail(fdga_ctx* c, const std::string& msg, int code = -1) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) return fail((c), std::string(#call) + ": " + hipGetErrorString(e_), -2); \
+    } while (0)
+
+#define LAUNCHCHK(c)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess) return fail((c), std::string("kernel launch: ") + hipGetErrorString(e_), -3); \
+    } while (0)
+
+int dalloc(fdga_ctx* c, double2** p, size_t n) {
+    if (n == 0) n = 1;
+    HIPCHK(c, hipMalloc((void**)p, n * sizeof(double2)));
+    k_zero<<<cdiv(n, 256), 256, 0, c->st>>>(*p, n);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+ChanV chan_view(const fdga_ctx* c, const DevChan& dc, int n1, int n2b, int n2f, int n3b, int n3f, int NP) {
+    ChanV v;
+    v.K1 = dc.K1; v.K2 = dc.K2; v.K3 = dc.K3;
+    v.A1 = dc.A1; v.A2 = dc.A2; v.A3 = dc.A3;
+    v.n1 = n1; v.n2b = n2b; v.n2f = n2f; v.n3b = n3b; v.n3f = n3f;
+    v.NP = NP;
+    (void)c;
+    return v;
+}
+ChanV chanF(const fdga_ctx* c, const DevChan& dc) {
+    return chan_view(c, dc, c->d.nK1, c->d.nK2b, c->d.nK2f, c->d.nK3b, c->d.nK3f, c->NP);
+}
+
+RefV ref_view(const fdga_ctx* c, bool zero) {
+    RefV r;
+    if (zero) {
+        r.U = cmk(0.0, 0.0);
+        r.tb = 0; r.tf = 0;
+        for (auto& t : r.tab) t = nullptr;
+        return r;
+    }
+    r.U = c->U;
+    r.tb = c->tabs_nonzero ? c->d.tab_nb : 0;
+    r.tf = c->d.tab_nf;
+    for (int k = 0; k < 4; ++k) r.tab[k] = c->tab[k];
+    return r;
+}
+
+F0V f0_view(const fdga_ctx* c) {
+    F0V f;
+    f.kind = c->d.ref_kind;
+    f.ref = ref_view(c, false);
+    for (int r = 0; r < 3; ++r)
+        f.g[r] = (c->d.ref_kind == 1) ? chan_view(c, c->R0[r], c->d.r_nK1, c->d.r_nK2b, c->d.r_nK2f, c->d.r_nK3b, c->d.r_nK3f, 1)
+                                      : ChanV{};
+    return f;
+}
+F0V zero_f0() {
+    F0V f{};
+    f.kind = 0;
+    f.ref.U = cmk(0.0, 0.0);
+    f.ref.tb = 0;
+    return f;
+}
+
+VtxV vtx_F(const fdga_ctx* c) {
+    VtxV v;
+    for (int r = 0; r < 3; ++r) v.g[r] = chanF(c, c->F[r]);
+    v.f0 = f0_view(c);
+    return v;
+}
+VtxV vtx_FL(const fdga_ctx* c) {
+    VtxV v;
+    for (int r = 0; r < 3; ++r) v.g[r] = chanF(c, c->FL[r]);
+    v.f0 = zero_f0();
+    return v;
+}
+
+int alloc_chan(fdga_ctx* c, DevChan& dc, size_t s1, size_t s2, size_t s3, int NP, bool avg) {
+    if (dalloc(c, &dc.K1, s1 * NP) || dalloc(c, &dc.K2, s2 * NP) || dalloc(c, &dc.K3, s3 * NP)) return -2;
+    if (avg && NP > 1) {
+        if (dalloc(c, &dc.A1, s1) || dalloc(c, &dc.A2, s2) || dalloc(c, &dc.A3, s3)) return -2;
+        dc.own_avg = true;
+    } else {
+        dc.A1 = dc.K1; dc.A2 = dc.K2; dc.A3 = dc.K3;
+    }
+    return 0;
+}
+void free_chan(DevChan& dc) {
+    if (dc.own_avg) {
+        (void)hipFree(dc.A1);
+        (void)hipFree(dc.A2);
+        (void)hipFree(dc.A3);
+    }
+    (void)hipFree(dc.K1); (void)hipFree(dc.K2); (void)hipFree(dc.K3);
+    dc = DevChan{};
+}
+
+// P averages of the target vertex F (s-wave cross-channel lookups)
+int refresh_avg(fdga_ctx* c) {
+    if (c->NP == 1) return 0;
+    for (int r = 0; r < 3; ++r) {
+        k_pavg<<<cdiv(c->n1, 256), 256, 0, c->st>>>(c->F[r].A1, c->F[r].K1, c->n1, c->NP);
+        k_pavg<<<cdiv(c->n2, 256), 256, 0, c->st>>>(c->F[r].A2, c->F[r].K2, c->n2, c->NP);
+        k_pavg<<<cdiv(c->n3, 256), 256, 0, c->st>>>(c->F[r].A3, c->F[r].K3, c->n3, c->NP);
+    }
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int ensure_stage(fdga_ctx* c, size_t n) {
+    if (c->stage_n >= n) return 0;
+    if (c->stage) (void)hipFree(c->stage);
+    c->stage = nullptr;
+    c->stage_n = 0;
+    HIPCHK(c, hipMalloc((void**)&c->stage, n * sizeof(double2)));
+    c->stage_n = n;
+    return 0;
+}
+
+int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale) {
+    if (L == 1) {
+        if (scale != 1.0) {
+            k_axpby<<<cdiv(nrows, 256), 256, 0, c->st>>>(data, data, scale, 0.0, nrows);
+            LAUNCHCHK(c);
+        }
+        return 0;
+    }
+    const int LL = L * L;
+    int RB = std::max(1, std::min(16, 1024 / LL));
+    size_t smem = (size_t)(2 * RB * LL + L) * sizeof(double2);
+    k_dft2_rows<<<cdiv(nrows, RB), 256, smem, c->st>>>(data, nrows, L, RB, sign, scale);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+// bubbles for G (target, into Πpp/Πph) or G0 (reference, into Π0pp/Π0ph)
+int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
+    const fdga_dims& d = c->d;
+    if (!d.nonlocal) {
+        dim3 g(cdiv(c->NVP, 256), c->NWP);
+        k_bubble_local<<<g, 256, 0, c->st>>>(pp, ph, Gsrc, d.nG, d.nK1, c->nVP, d.T);
+        LAUNCHCHK(c);
+        return 0;
+    }
+    // G(R) = fft(G)/LG^2 (src/nonlocal/bubble.jl:110)
+    HIPCHK(c, hipMemcpyAsync(c->tmpG1, Gsrc, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG))) return -3;
+    dim3 g(cdiv(c->NVP, 256), c->NWP, c->NP);
+    k_bubble_nl_R<<<g, 256, 0, c->st>>>(pp, ph, c->tmpG1, d.nG, d.LG, d.L, d.nK1, c->nVP, d.T);
+    LAUNCHCHK(c);
+    size_t nrow = (size_t)c->NVP * c->NWP;
+    if (dft2(c, pp, nrow, d.L, +1, 1.0)) return -3;
+    if (dft2(c, ph, nrow, d.L, +1, 1.0)) return -3;
+    return 0;
+}
+
+void stage_begin(fdga_ctx* c) {
+    if (c->timing) (void)hipEventRecord(c->ev[0], c->st);
+}
+void stage_end(fdga_ctx* c, int s) {
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev[1], c->st);
+    (void)hipEventSynchronize(c->ev[1]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->stage_ms[s] += ms;
+}
+
+const int kSp[3] = {PSP, DSP, PSP};
+const double kSign[3] = {+1.0, -1.0, +1.0};
+
+double2* pi_of(fdga_ctx* c, int ch, bool zero) {
+    if (zero) return ch == PCH ? c->Pi0pp : c->Pi0ph;
+    return ch == PCH ? c->Pipp : c->Piph;
+}
+
+// ---------------------------------------------------------------------------------------
+// SDE levels (src/SDE.jl:35-48): F, F.F0, F.F0.F0 ... accumulated into S
+struct Level { int kind; ChanV g[3]; int chanNP; double scale; };
+
+int sde_sigma(fdga_ctx* c, double2* S, const double2* G, double scale) {
+    const fdga_dims& d = c->d;
+    if (!d.nonlocal) {
+        k_sde_sigma_local<<<cdiv(c->NG, 128), 128, 0, c->st>>>(S, G, c->Lpp, c->Lph, d.nG, d.nK2b, d.nK2f, d.T, scale);
+        LAUNCHCHK(c);
+        return 0;
+    }
+    // real-space evaluation (src/nonlocal/SDE.jl:209-280)
+    HIPCHK(c, hipMemcpyAsync(c->tmpG1, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG))) return -3;
+    HIPCHK(c, hipMemcpyAsync(c->tmpL1, c->Lpp, c->nL * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->tmpL2, c->Lph, c->nL * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    size_t nrow = (size_t)c->NW2 * c->NV2;
+    if (dft2(c, c->tmpL1, nrow, d.L, -1, 1.0 / ((double)d.L * d.L))) return -3;
+    if (dft2(c, c->tmpL2, nrow, d.L, -1, 1.0 / ((double)d.L * d.L))) return -3;
+    dim3 g(cdiv(c->NG, 128), c->NkG);
+    k_sde_sigma_nl_R<<<g, 128, 0, c->st>>>(c->tmpG2, c->tmpG1, c->tmpL1, c->tmpL2, d.nG, d.nK2b, d.nK2f, d.LG, d.L, d.T);
+    LAUNCHCHK(c);
+    if (dft2(c, c->tmpG2, c->NG, d.LG, +1, 1.0)) return -3;
+    k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(S, c->tmpG2, scale, 1.0, c->nGel);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+// Σ += U² T² Σ G G G  (src/SDE.jl:328-364; NL src/nonlocal/SDE.jl:400-447)
+int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
+    const fdga_dims& d = c->d;
+    const double2 fac = cscale(cmul(c->U, c->U), d.T * d.T);
+    const int NB = 4 * d.nG - 1;
+    if (!d.nonlocal) {
+        k_u2_C<<<dim3(cdiv(NB, 128), 1), 128, 0, c->st>>>(c->u2C, G, G, d.nG, 1);
+        k_u2_S<<<dim3(cdiv(c->NG, 128), 1), 128, 0, c->st>>>(S, c->u2C, G, d.nG, 1, fac, 1);
+        LAUNCHCHK(c);
+        return 0;
+    }
+    const int LG = d.LG;
+    HIPCHK(c, hipMemcpyAsync(c->tmpG1, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->tmpG2, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    if (dft2(c, c->tmpG1, c->NG, LG, -1, 1.0 / ((double)LG * LG))) return -3;   // G(+R)
+    if (dft2(c, c->tmpG2, c->NG, LG, +1, 1.0 / ((double)LG * LG))) return -3;   // G(-R)
+    k_u2_C<<<dim3(cdiv(NB, 128), c->NkG), 128, 0, c->st>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
+    k_u2_S<<<dim3(cdiv(c->NG, 128), c->NkG), 128, 0, c->st>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac, 0);
+    LAUNCHCHK(c);
+    if (dft2(c, c->tmpG3, c->NG, LG, +1, 1.0)) return -3;
+    k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(S, c->tmpG3, 1.0, 1.0, c->nGel);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int sde_hartree(fdga_ctx* c, double2* S, const double2* G, double sign) {
+    k_occupation<<<1, 256, 0, c->st>>>(c->scal, G, c->nGel, c->d.T, (double)c->NkG);
+    k_hartree<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(S, c->nGel, c->scal, c->U, sign);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* Ph) {
+    const fdga_dims& d = c->d;
+    const size_t nout = (size_t)c->NW2 * c->NV2 * c->NP;
+    RefV ref = ref_view(c, false);
+    k_sde_L<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref, lv.kind,
+                                                       d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, lv.chanNP, cscale(c->U, d.T));
+    LAUNCHCHK(c);
+    return 0;
+}
+
+// SDE!(Σ, G, Πpp, Πph, Lpp, Lph, F; include_U², include_Hartree) over the levels starting at F
+// (target: start = 0) or at F0 (reference system: start = 1)
+int sde_levels(fdga_ctx* c, double2* S, const double2* G, const double2* Pp, const double2* Ph, int start, int incH) {
+    std::vector<Level> lv;
+    if (start == 0) {
+        Level l{0, {}, c->NP, 1.0};
+        for (int r = 0; r < 3; ++r) l.g[r] = chanF(c, c->F[r]);
+        lv.push_back(l);
+    }
+    if (c->d.ref_kind == 1) {
+        Level l{0, {}, 1, 1.0};
+        F0V f = f0_view(c);
+        for (int r = 0; r < 3; ++r) l.g[r] = f.g[r];
+        lv.push_back(l);
+    }
+    Level lr{1, {}, 1, 1.0 / 3.0};
+    lv.push_back(lr);
+    for (size_t i = 0; i < lv.size(); ++i) {
+        const bool refl = lv[i].kind == 1;
+        // a RefVertex with null tables contributes (U + 0) - U = 0 exactly
+        if (!(refl && !c->tabs_nonzero)) {
+            if (sde_level_L(c, lv[i], Pp, Ph)) return -3;
+            if (sde_sigma(c, S, G, lv[i].scale)) return -3;
+        }
+        if (i == 0) {
+            if (sde_u2(c, S, G)) return -3;
+            if (incH && sde_hartree(c, S, G, +1.0)) return -3;
+        }
+    }
+    return 0;
+}
+
+// K3-class operands
+K3Common k3common(fdga_ctx* c, int ch) {
+    K3Common k;
+    k.n3b = c->d.nK3b; k.n3f = c->d.nK3f; k.NP = c->NP;
+    k.nWP = c->d.nK1; k.nVP = c->nVP;
+    k.Pi = pi_of(c, ch, false);
+    k.Pi0 = pi_of(c, ch, true);
+    k.T = c->d.T;
+    return k;
+}
+
+int idx_gamma(int ch) { return ch == PCH ? 4 : ch == ACH ? 5 : 6; }
+int idx_f0(int ch) { return ch == PCH ? 1 : ch == ACH ? 2 : 3; }
+int idx_fbase(int ch) { return ch == PCH ? 7 : ch == ACH ? 8 : 9; }
+
+}  // namespace
+
+// =========================================================================================
+extern "C" {
+
+const char* fdga_last_error(const fdga_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
+    if (!out || !dims) return -1;
+    *out = nullptr;
+    auto* c = new fdga_ctx();
+    c->d = *dims;
+    fdga_dims& d = c->d;
+    if (!d.nonlocal) { d.L = 1; d.LG = 1; }
+    // constructor assertions (src/channel.jl:33-40) and box relations the kernels rely on
+    if (d.nK1 < d.nK2b || d.nK1 < d.nK2f || d.nK2b < d.nK3b || d.nK2f < d.nK3f || d.nK3b < 1 || d.nK3f < 1 ||
+        d.mPinu < 1 || d.nG < 1 || d.L < 1 || d.LG < 1 || d.T <= 0.0) {
+        fail(c, "invalid dimensions (need nK1 >= nK2 >= nK3 >= 1, mPinu >= 1, nG >= 1, T > 0)");
+        *out = c;
+        return -1;
+    }
+    if (d.nK1 * d.mPinu < d.nK3f) {
+        fail(c, "bubble mesh nK1*mPinu must contain the K3 fermionic box");
+        *out = c;
+        return -1;
+    }
+    if (d.ref_kind == 1 && (d.r_nK1 < d.r_nK2b || d.r_nK1 < d.r_nK2f || d.r_nK2b < d.r_nK3b || d.r_nK2f < d.r_nK3f || d.r_nK3b < 1)) {
+        fail(c, "invalid reference Vertex boxes");
+        *out = c;
+        return -1;
+    }
+    if (d.tab_nb < 1) d.tab_nb = 1;
+    if (d.tab_nf < 1) d.tab_nf = 1;
+    (void)hipGetLastError();
+    if (hipStreamCreate(&c->st) != hipSuccess) {
+        fail(c, "hipStreamCreate failed (no HIP device?)", -2);
+        *out = c;
+        return -2;
+    }
+    c->NP = d.L * d.L;
+    c->NkG = d.LG * d.LG;
+    c->NG = 2 * d.nG;
+    c->NW1 = 2 * d.nK1 - 1;
+    c->NW2 = 2 * d.nK2b - 1;
+    c->NV2 = 2 * d.nK2f;
+    c->NW3 = 2 * d.nK3b - 1;
+    c->NV3 = 2 * d.nK3f;
+    c->NWP = 2 * d.nK1 - 1;
+    c->nVP = d.nK1 * d.mPinu;
+    c->NVP = 2 * c->nVP;
+    c->nGel = (size_t)c->NG * c->NkG;
+    c->nPiel = (size_t)c->NVP * c->NWP * c->NP;
+    c->n1 = (size_t)c->NW1;
+    c->n2 = (size_t)c->NW2 * c->NV2;
+    c->n3 = (size_t)c->NV3 * c->NV3 * c->NW3;
+    c->nL = c->n2 * c->NP;
+    c->U = cmk(d.U, 0.0);
+    int rc = 0;
+    for (double2** p : {&c->Gbare, &c->G0, &c->Sig0, &c->G, &c->Sig, &c->tmpG1, &c->tmpG2, &c->tmpG3})
+        rc |= dalloc(c, p, c->nGel);
+    for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph, &c->W}) rc |= dalloc(c, p, c->nPiel);
+    for (double2** p : {&c->Lpp, &c->Lph, &c->tmpL1, &c->tmpL2}) rc |= dalloc(c, p, c->nL);
+    rc |= dalloc(c, &c->sde0, c->nGel);
+    rc |= dalloc(c, &c->u2C, (size_t)(4 * d.nG - 1) * c->NkG);
+    size_t ntab = (size_t)(2 * d.tab_nb - 1) * (2 * d.tab_nf) * (2 * d.tab_nf);
+    for (auto& t : c->tab) rc |= dalloc(c, &t, ntab);
+    for (int r = 0; r < 3; ++r) {
+        rc |= alloc_chan(c, c->F[r], c->n1, c->n2, c->n3, c->NP, true);
+        rc |= alloc_chan(c, c->Fb[r], c->n1, c->n2, c->n3, c->NP, false);
+        rc |= alloc_chan(c, c->FL[r], c->n1, c->n2, c->n3, c->NP, false);
+        if (d.ref_kind == 1) {
+            size_t r1 = (size_t)(2 * d.r_nK1 - 1), r2 = (size_t)(2 * d.r_nK2b - 1) * (2 * d.r_nK2f);
+            size_t r3 = (size_t)(2 * d.r_nK3f) * (2 * d.r_nK3f) * (2 * d.r_nK3b - 1);
+            rc |= alloc_chan(c, c->R0[r], r1, r2, r3, 1, false);
+        }
+    }
+    for (auto& p : c->cache) rc |= dalloc(c, &p, c->n3);
+    if (hipMalloc((void**)&c->scal, 8 * sizeof(double)) != hipSuccess) rc = -2;
+    if (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess) rc = -2;
+    if (rc) {
+        if (c->err.empty()) c->err = "device allocation failed";
+        *out = c;
+        return -2;
+    }
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+        *out = c;
+        return fail(c, "stream synchronisation failed", -2);
+    }
+    *out = c;
+    return 0;
+}
+
+int fdga_destroy(fdga_ctx* c) {
+    if (!c) return 0;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (double2* p : {c->Gbare, c->G0, c->Sig0, c->G, c->Sig, c->tmpG1, c->tmpG2, c->tmpG3, c->Pi0pp, c->Pi0ph,
+                       c->Pipp, c->Piph, c->W, c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C, c->stage})
+        if (p) (void)hipFree(p);
+    for (auto* t : c->tab) if (t) (void)hipFree(t);
+    for (auto* p : c->cache) if (p) (void)hipFree(p);
+    for (int r = 0; r < 3; ++r) {
+        free_chan(c->F[r]); free_chan(c->Fb[r]); free_chan(c->FL[r]);
+        if (c->R0[r].K1) free_chan(c->R0[r]);
+    }
+    if (c->scal) (void)hipFree(c->scal);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return 0;
+}
+
+// array table: pointer, element count, layout kind (0 plain, 1 K3, 2 bubble, 3 cache, 4 F-cache)
+struct ArrInfo { double2* p; size_t n; int kind; int NW, NV, NP; int ch; };
+
+static int arr_info(fdga_ctx* c, int which, ArrInfo& a) {
+    const fdga_dims& d = c->d;
+    a = ArrInfo{nullptr, 0, 0, 0, 0, 1, 0};
+    switch (which) {
+        case FDGA_GBARE: a.p = c->Gbare; a.n = c->nGel; return 0;
+        case FDGA_G0: a.p = c->G0; a.n = c->nGel; return 0;
+        case FDGA_SIGMA0: a.p = c->Sig0; a.n = c->nGel; return 0;
+        case FDGA_G: a.p = c->G; a.n = c->nGel; return 0;
+        case FDGA_SIGMA: a.p = c->Sig; a.n = c->nGel; return 0;
+        case FDGA_PI0PP: case FDGA_PI0PH: case FDGA_PIPP: case FDGA_PIPH: {
+            double2* ps[4] = {c->Pi0pp, c->Pi0ph, c->Pipp, c->Piph};
+            a.p = ps[which - FDGA_PI0PP]; a.n = c->nPiel; a.kind = 2; a.NW = c->NWP; a.NV = c->NVP; a.NP = c->NP;
+            return 0;
+        }
+        case FDGA_LPP: a.p = c->Lpp; a.n = c->nL; return 0;
+        case FDGA_LPH: a.p = c->Lph; a.n = c->nL; return 0;
+        case FDGA_TAB_FPP: case FDGA_TAB_FPX: case FDGA_TAB_FTP: case FDGA_TAB_FTX:
+            a.p = c->tab[which - FDGA_TAB_FPP];
+            a.n = (size_t)(2 * d.tab_nb - 1) * (2 * d.tab_nf) * (2 * d.tab_nf);
+            return 0;
+        default: break;
+    }
+    if (which >= FDGA_CACHE0 && which < FDGA_CACHE0 + 10) {
+        int k = which - FDGA_CACHE0;
+        a.p = c->cache[k]; a.n = c->n3 * c->NP; a.kind = (k >= 7) ? 4 : 3;
+        a.NW = c->NW3; a.NV = c->NV3; a.NP = c->NP; a.ch = k - 7;
+        return 0;
+    }
+    if (which >= FDGA_VERTEX0 && which < FDGA_VERTEX0 + 64) {
+        int k = which - FDGA_VERTEX0, v = k / 16, ch = (k % 16) / 3, cls = (k % 16) % 3 + 1;
+        if (ch > 2) return -1;
+        if (v == 3 && d.ref_kind != 1) return -1;
+        DevChan& dc = v == 0 ? c->F[ch] : v == 1 ? c->Fb[ch] : v == 2 ? c->FL[ch] : c->R0[ch];
+        int NP = (v == 3) ? 1 : c->NP;
+        if (cls == 1) { a.p = dc.K1; a.n = (size_t)(v == 3 ? 2 * d.r_nK1 - 1 : c->NW1) * NP; return 0; }
+        if (cls == 2) {
+            a.p = dc.K2;
+            a.n = (v == 3 ? (size_t)(2 * d.r_nK2b - 1) * (2 * d.r_nK2f) : c->n2) * NP;
+            return 0;
+        }
+        a.p = dc.K3; a.kind = 1; a.NP = NP;
+        a.NW = v == 3 ? 2 * d.r_nK3b - 1 : c->NW3;
+        a.NV = v == 3 ? 2 * d.r_nK3f : c->NV3;
+        a.n = (size_t)a.NW * a.NV * a.NV * NP;
+        return 0;
+    }
+    return -1;
+}
+
+int64_t fdga_size(const fdga_ctx* c, int32_t which) {
+    ArrInfo a;
+    if (!c || arr_info(const_cast<fdga_ctx*>(c), which, a)) return -1;
+    return (int64_t)a.n;
+}
+
+static void invalidate(fdga_ctx* c, int which) {
+    if (which == FDGA_G0 || which == FDGA_PI0PP || which == FDGA_PI0PH || (which >= FDGA_TAB_FPP && which <= FDGA_TAB_FTX) ||
+        (which >= FDGA_VERTEX(3, 0, 1) && which <= FDGA_VERTEX(3, 2, 3)))
+        c->sde0_valid = 0;
+}
+
+int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
+    if (!c || !host) return -1;
+    ArrInfo a;
+    if (arr_info(c, which, a)) return fail(c, "unknown array id " + std::to_string(which));
+    if (which == FDGA_LPP || which == FDGA_LPH || a.kind == 3 || a.kind == 4) return fail(c, "array is read-only");
+    const size_t bytes = a.n * sizeof(double2);
+    if (a.kind == 0) {
+        HIPCHK(c, hipMemcpyAsync(a.p, host, bytes, hipMemcpyHostToDevice, c->st));
+    } else {
+        if (ensure_stage(c, a.n)) return -2;
+        HIPCHK(c, hipMemcpyAsync(c->stage, host, bytes, hipMemcpyHostToDevice, c->st));
+        if (a.kind == 1) k_k3_to_dev<<<cdiv(a.n, 256), 256, 0, c->st>>>(a.p, c->stage, a.NW, a.NV, a.NP);
+        else k_pi_to_dev<<<cdiv(a.n, 256), 256, 0, c->st>>>(a.p, c->stage, a.NW, a.NV, a.NP);
+        LAUNCHCHK(c);
+    }
+    if (which >= FDGA_TAB_FPP && which <= FDGA_TAB_FTX) {
+        for (size_t i = 0; i < 2 * a.n; ++i)
+            if (host[i] != 0.0) { c->tabs_nonzero = 1; break; }
+    }
+    invalidate(c, which);
+    if (which >= FDGA_VERTEX(0, 0, 1) && which <= FDGA_VERTEX(0, 2, 3) && refresh_avg(c)) return -3;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int fdga_get(fdga_ctx* c, int32_t which, double* host) {
+    if (!c || !host) return -1;
+    ArrInfo a;
+    if (arr_info(c, which, a)) return fail(c, "unknown array id " + std::to_string(which));
+    const size_t bytes = a.n * sizeof(double2);
+    if (a.kind == 0) {
+        HIPCHK(c, hipMemcpyAsync(host, a.p, bytes, hipMemcpyDeviceToHost, c->st));
+    } else {
+        if (ensure_stage(c, a.n)) return -2;
+        if (a.kind == 1) k_k3_to_ref<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, a.NW, a.NV, a.NP);
+        else if (a.kind == 2) k_pi_to_ref<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, a.NW, a.NV, a.NP);
+        else if (a.kind == 3) k_cache_to_ref<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, a.NW, a.NV, a.NP);
+        else {
+            int ch = a.ch == 0 ? PCH : a.ch == 1 ? ACH : TCH;
+            k_cache_F_julia<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, chanF(c, c->F[ch]), chanF(c, c->F[ACH]),
+                                                               ch, c->d.nK3b, c->d.nK3f, c->NP);
+        }
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(host, c->stage, bytes, hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int fdga_init(fdga_ctx* c) {
+    if (!c) return -1;
+    if (bubbles_into(c, c->Pi0pp, c->Pi0ph, c->G0)) return -3;
+    HIPCHK(c, hipMemcpyAsync(c->Sig, c->Sig0, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->G, c->Sig, c->Gbare, c->nGel);
+    LAUNCHCHK(c);
+    if (bubbles_into(c, c->Pipp, c->Piph, c->G)) return -3;
+    if (refresh_avg(c)) return -3;
+    c->sde0_valid = 0;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int fdga_dyson(fdga_ctx* c) {
+    k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->G, c->Sig, c->Gbare, c->nGel);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int fdga_bubbles(fdga_ctx* c) { return bubbles_into(c, c->Pipp, c->Piph, c->G); }
+
+int fdga_build_k3_cache(fdga_ctx* c) {
+    CachePtrs cp;
+    for (int k = 0; k < 10; ++k) cp.c[k] = c->cache[k];
+    dim3 g(cdiv(c->NV3, 64), c->NV3, c->NW3);
+    k_cache<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int fdga_bse_l_k2(fdga_ctx* c, int32_t ch) {
+    if (ch < 0 || ch > 2) return fail(c, "bad channel");
+    const size_t nout = c->n2 * c->NP;
+    k_gemv_lk2<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->FL[ch].K2, c->FL[ACH].K2, pi_of(c, ch, true), vtx_F(c),
+                                                          c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP, ch, kSp[ch],
+                                                          c->d.nonlocal, c->d.T * kSign[ch], ch == TCH);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int fdga_bse_l_k3(fdga_ctx* c, int32_t ch) {
+    if (ch < 0 || ch > 2) return fail(c, "bad channel");
+    const int NV = c->NV3, NW = c->NW3;
+    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
+    LK3_Aop a{c->cache[idx_gamma(ch)], NV, NW};
+    LK3_Bop b{k3common(c, ch), c->cache[idx_f0(ch)]};
+    const double s = (ch == ACH) ? 1.0 : -1.0;   // src/BSE_templates.jl:122,130,138
+    LK3_Ep e{c->FL[ch].K3, c->FL[ACH].K3, c->d.T * s, NV, ch == TCH};
+    const size_t nb = (size_t)NW * c->NP * tm * tn;
+    zgemm_tiles<<<(unsigned)nb, 256, 0, c->st>>>(NV, NV, NV, tm, tn, a, b, e);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int fdga_bse_k1(fdga_ctx* c, int32_t ch) {
+    if (ch < 0 || ch > 2) return fail(c, "bad channel");
+    dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
+    k_bse_w<<<gw, 256, 0, c->st>>>(c->W, pi_of(c, ch, false), pi_of(c, ch, true), vtx_FL(c), f0_view(c), c->d.nK1, c->nVP,
+                                   ch, kSp[ch]);
+    const size_t nout = c->n1 * c->NP;
+    k_gemv_k1<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[ch].K1, c->Fb[ACH].K1, c->W, vtx_F(c), c->d.nK1, c->d.nK1,
+                                                         c->nVP, c->NP, ch, kSp[ch], c->d.T * kSign[ch], ch == TCH);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int fdga_bse_k2(fdga_ctx* c, int32_t ch) {
+    if (ch < 0 || ch > 2) return fail(c, "bad channel");
+    dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
+    k_bse_w<<<gw, 256, 0, c->st>>>(c->W, pi_of(c, ch, false), pi_of(c, ch, true), vtx_FL(c), f0_view(c), c->d.nK1, c->nVP,
+                                   ch, kSp[ch]);
+    const size_t nout = c->n2 * c->NP;
+    k_gemv_k2<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[ch].K2, c->Fb[ACH].K2, c->W, vtx_F(c), chanF(c, c->FL[ch]),
+                                                         chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
+                                                         c->NP, ch, kSp[ch], c->d.T * kSign[ch]);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int fdga_bse_k3(fdga_ctx* c, int32_t ch) {
+    if (ch < 0 || ch > 2) return fail(c, "bad channel");
+    const int NV = c->NV3, NW = c->NW3;
+    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
+    K3Common kc = k3common(c, ch);
+    K3_Aop a{kc, c->cache[idx_fbase(ch)], chanF(c, c->F[ch]), chanF(c, c->F[ACH]), ch};
+    const int ig = (ch == PCH) ? 0 : idx_gamma(ch);
+    const double s1 = (ch == ACH) ? 1.0 : -1.0, s2 = (ch == TCH) ? -1.0 : 1.0;   // src/BSE_templates.jl:156,164,172
+    K3_Bop b{kc, c->cache[idx_f0(ch)], c->cache[ig], c->FL[ch].K3, c->FL[ACH].K3, ch, s1, s2};
+    K3_Ep e{c->Fb[ch].K3, c->Fb[ACH].K3, c->FL[ch].K3, c->FL[ACH].K3, c->d.T, NV, ch};
+    const size_t nb = (size_t)NW * c->NP * tm * tn;
+    zgemm_tiles<<<(unsigned)nb, 256, 0, c->st>>>(NV, NV, NV, tm, tn, a, b, e);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+int fdga_update_F(fdga_ctx* c) {
+    // Fbuff is overwritten in full by every iteration, so set!(S.F, S.Fbuff) is a buffer swap
+    for (int r = 0; r < 3; ++r) {
+        std::swap(c->F[r].K1, c->Fb[r].K1);
+        std::swap(c->F[r].K2, c->Fb[r].K2);
+        std::swap(c->F[r].K3, c->Fb[r].K3);
+        if (!c->F[r].own_avg) { c->F[r].A1 = c->F[r].K1; c->F[r].A2 = c->F[r].K2; c->F[r].A3 = c->F[r].K3; }
+        c->Fb[r].A1 = c->Fb[r].K1; c->Fb[r].A2 = c->Fb[r].K2; c->Fb[r].A3 = c->Fb[r].K3;
+    }
+    return refresh_avg(c);
+}
+
+int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
+    k_zero<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->nGel);
+    if (sde_levels(c, c->Sig, c->G, c->Pipp, c->Piph, 0, include_hartree)) return -3;
+    if (strategy == FDGA_FDPA) {
+        // SDE(F0, G0, Π0) does not change between iterations: computed once (src/SDE.jl:19-20)
+        if (!c->sde0_valid || c->sde0_h != include_hartree) {
+            k_zero<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->sde0, c->nGel);
+            if (sde_levels(c, c->sde0, c->G0, c->Pi0pp, c->Pi0ph, 1, include_hartree)) return -3;
+            c->sde0_valid = 1;
+            c->sde0_h = include_hartree;
+        }
+        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->sde0, -1.0, 1.0, c->nGel);
+        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->Sig0, 1.0, 1.0, c->nGel);
+        LAUNCHCHK(c);
+        if (include_hartree && c->d.fdpa_h0 && sde_hartree(c, c->Sig, c->G0, -1.0)) return -3;
+    }
+    return 0;
+}
+
+int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
+    if (!c) return -1;
+    if (strategy != FDGA_SCPA && strategy != FDGA_FDPA) return fail(c, "Calculation strategy unknown");
+    stage_begin(c);
+    if (update_sigma) {
+        if (fdga_dyson(c) || fdga_bubbles(c)) return -3;
+    }
+    stage_end(c, 0);
+    stage_begin(c);
+    if (fdga_build_k3_cache(c)) return -3;
+    stage_end(c, 1);
+    if (strategy == FDGA_FDPA) {
+        stage_begin(c);
+        for (int ch : {PCH, ACH, TCH}) if (fdga_bse_l_k2(c, ch)) return -3;
+        stage_end(c, 2);
+        stage_begin(c);
+        for (int ch : {PCH, ACH, TCH}) if (fdga_bse_l_k3(c, ch)) return -3;
+        stage_end(c, 3);
+    }
+    stage_begin(c);
+    for (int ch : {PCH, ACH, TCH}) if (fdga_bse_k1(c, ch)) return -3;
+    stage_end(c, 4);
+    stage_begin(c);
+    for (int ch : {PCH, ACH, TCH}) if (fdga_bse_k2(c, ch)) return -3;
+    stage_end(c, 5);
+    stage_begin(c);
+    for (int ch : {PCH, ACH, TCH}) if (fdga_bse_k3(c, ch)) return -3;
+    stage_end(c, 6);
+    stage_begin(c);
+    if (fdga_update_F(c)) return -3;
+    stage_end(c, 7);
+    stage_begin(c);
+    if (update_sigma && fdga_sde(c, strategy, 1)) return -3;
+    stage_end(c, 8);
+    return 0;
+}
+
+int64_t fdga_flat_size(const fdga_ctx* c) {
+    return (int64_t)(3 * (c->n1 + c->n2 + c->n3) * c->NP + c->nGel);
+}
+
+int fdga_unflatten(fdga_ctx* c, const double* x) {
+    const size_t n = (size_t)fdga_flat_size(c);
+    if (ensure_stage(c, n)) return -2;
+    HIPCHK(c, hipMemcpyAsync(c->stage, x, n * sizeof(double2), hipMemcpyHostToDevice, c->st));
+    size_t off = 0;
+    for (int r = 0; r < 3; ++r) {
+        HIPCHK(c, hipMemcpyAsync(c->F[r].K1, c->stage + off, c->n1 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+        off += c->n1 * c->NP;
+        HIPCHK(c, hipMemcpyAsync(c->F[r].K2, c->stage + off, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+        off += c->n2 * c->NP;
+        k_k3_to_dev<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->F[r].K3, c->stage + off, c->NW3, c->NV3, c->NP);
+        LAUNCHCHK(c);
+        off += c->n3 * c->NP;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->Sig, c->stage + off, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    if (refresh_avg(c)) return -3;
+    return 0;
+}
+
+int fdga_flatten(fdga_ctx* c, double* x) {
+    const size_t n = (size_t)fdga_flat_size(c);
+    if (ensure_stage(c, n)) return -2;
+    size_t off = 0;
+    for (int r = 0; r < 3; ++r) {
+        HIPCHK(c, hipMemcpyAsync(c->stage + off, c->F[r].K1, c->n1 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+        off += c->n1 * c->NP;
+        HIPCHK(c, hipMemcpyAsync(c->stage + off, c->F[r].K2, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+        off += c->n2 * c->NP;
+        k_k3_to_ref<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->stage + off, c->F[r].K3, c->NW3, c->NV3, c->NP);
+        LAUNCHCHK(c);
+        off += c->n3 * c->NP;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->stage + off, c->Sig, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(x, c->stage, n * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int fdga_fixed_point(fdga_ctx* c, double* R, const double* x, int32_t strategy) {
+    if (fdga_unflatten(c, x)) return -3;
+    if (fdga_iterate(c, strategy, 1)) return -3;
+    if (fdga_flatten(c, R)) return -3;
+    const size_t n = 2 * (size_t)fdga_flat_size(c);
+    for (size_t i = 0; i < n; ++i) R[i] -= x[i];
+    return 0;
+}
+
+int fdga_sync(fdga_ctx* c) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int fdga_time_iterations(fdga_ctx* c, int32_t strategy, int32_t n, double* ms) {
+    if (!c || !ms || n < 1) return -1;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipEventRecord(a, c->st));
+    for (int i = 0; i < n; ++i)
+        if (fdga_iterate(c, strategy, 1)) return -3;
+    HIPCHK(c, hipEventRecord(b, c->st));
+    HIPCHK(c, hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, a, b));
+    *ms = t;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return 0;
+}
+
+int fdga_stage_ms(const fdga_ctx* c, int32_t stage, double* ms) {
+    if (!c || !ms || stage < 0 || stage > 8) return -1;
+    *ms = c->stage_ms[stage];
+    return 0;
+}
+
+// C = A B for dense column-major complex matrices through the BSE GEMM engine
+int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, const double* A, const double* B, double* C) {
+    if (M < 1 || N < 1 || K < 1 || !A || !B || !C) return -1;
+    double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
+    const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
+    if (hipMalloc((void**)&dA, na * sizeof(double2)) != hipSuccess) return -2;
+    if (hipMalloc((void**)&dB, nb * sizeof(double2)) != hipSuccess) { (void)hipFree(dA); return -2; }
+    if (hipMalloc((void**)&dC, nc * sizeof(double2)) != hipSuccess) { (void)hipFree(dA); (void)hipFree(dB); return -2; }
+    int rc = 0;
+    if (hipMemcpy(dA, A, na * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dB, B, nb * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+    if (!rc) {
+        const int tm = cdiv(M, GBM), tn = cdiv(N, GBN);
+        zgemm_tiles<<<(unsigned)(tm * tn), 256>>>(M, N, K, tm, tn, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M});
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = -3;
+    }
+    if (!rc && hipMemcpy(C, dC, nc * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = -2;
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);
+    return rc;
+}
+
+// enable (1) / disable (0) per-stage event timing; resets the accumulated stage times
+int fdga_set_stage_timing(fdga_ctx* c, int32_t on) {
+    if (!c) return -1;
+    c->timing = on;
+    for (auto& s : c->stage_ms) s = 0.0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,731 @@
+// fdga_kernels.h -- gfx950 kernels of the parquet iteration (FP64 path).
+//
+// Kernel map (reference function -> kernel):
+//   Dyson!                 k_dyson
+//   bubbles! (local)       k_bubble_local        elementwise, HBM bound
+//   bubbles_real_space!    k_dft2_rows + k_bubble_nl_R + k_dft2_rows
+//   s-wave averages        k_pavg
+//   build_K3_cache!        k_cache               gather: Γ_irr / F0 / F-base caches
+//   BSE_L_K3!, BSE_K3!     zgemm_tiles<...>      batched complex GEMM on v_mfma_f64_16x16x4f64
+//   BSE_K1!, BSE_K2!,      k_bse_w + k_gemv_*    gather-GEMV over the bubble mesh
+//   BSE_L_K2!
+//   SDE!                   k_sde_L, k_sde_sigma_local, k_u2_*, k_occupation
+#pragma once
+#include "fdga_device.h"
+
+namespace fdga {
+
+constexpr int WAVE = 64;
+
+// ---------------------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// Dyson!: G = 1 / (1/Gbare + Σ)  (src/dyson.jl:5-31)
+__global__ void k_dyson(double2* __restrict__ G, const double2* __restrict__ S, const double2* __restrict__ Gb, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) G[i] = cinv(cadd(cinv(Gb[i]), S[i]));
+}
+
+// bubbles! local with 1/ν tail (src/bubble.jl:9-36); output [ω, Ω] (ω fastest)
+__global__ void k_bubble_local(double2* __restrict__ pp, double2* __restrict__ ph, const double2* __restrict__ G,
+                               int nG, int nW, int nV, double T) {
+    int iw = blockIdx.x * blockDim.x + threadIdx.x;
+    int iW = blockIdx.y;
+    if (iw >= 2 * nV) return;
+    int m = iW - (nW - 1), n = iw - nV;
+    auto g = [&](int k) -> double2 {
+        return in_f(k, nG) ? G[k + nG] : cmk(1.0 / (M_PI * T * (2.0 * k + 1.0)), 0.0);
+    };
+    double2 gv = g(n);
+    size_t o = (size_t)iw + (size_t)(2 * nV) * iW;
+    pp[o] = cmul(gv, g(bmf(m, n)));
+    ph[o] = cmul(gv, g(bpf(m, n)));
+}
+
+// ---------------------------------------------------------------------------------------
+// 2D DFT over the trailing L x L plane of rows: data[r + nrows*(j1 + L*j2)], in place.
+// One workgroup per block of RB rows, two LDS planes (in/out) plus the L twiddles;
+// sign -1 = fft, +1 = bfft (FFTW conventions of src/nonlocal/bubble.jl:110,154); `scale`
+// multiplies the result.
+__global__ void k_dft2_rows(double2* __restrict__ data, size_t nrows, int L, int RB, int sign, double scale) {
+    extern __shared__ double2 sm[];   // 2 * RB * L * L + L
+    const int LL = L * L;
+    double2* s0 = sm;
+    double2* s1 = sm + RB * LL;
+    double2* tw = sm + 2 * RB * LL;
+    const size_t r0 = (size_t)blockIdx.x * RB;
+    const int nrb = (int)min((size_t)RB, nrows - r0);
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        double sn, cs;
+        sincospi(sign * 2.0 * k / L, &sn, &cs);
+        tw[k] = cmk(cs, sn);
+    }
+    for (int e = threadIdx.x; e < RB * LL; e += blockDim.x) {
+        int r = e % RB, j = e / RB;
+        s0[e] = (r < nrb) ? data[r0 + r + nrows * (size_t)j] : czero();
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < RB * LL; e += blockDim.x) {   // along j1
+        int r = e % RB, j = e / RB, k1 = j % L, j2 = j / L;
+        double2 s = czero();
+        for (int j1 = 0; j1 < L; ++j1) s = cadd(s, cmul(s0[r + RB * (j1 + L * j2)], tw[(j1 * k1) % L]));
+        s1[e] = s;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < RB * LL; e += blockDim.x) {   // along j2
+        int r = e % RB, j = e / RB, j1 = j % L, k2 = j / L;
+        double2 s = czero();
+        for (int j2 = 0; j2 < L; ++j2) s = cadd(s, cmul(s1[r + RB * (j1 + L * j2)], tw[(j2 * k2) % L]));
+        s0[e] = s;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < RB * LL; e += blockDim.x) {
+        int r = e % RB, j = e / RB;
+        if (r < nrb) data[r0 + r + nrows * (size_t)j] = cscale(s0[e], scale);
+    }
+}
+
+// bubbles_real_space! product step (src/nonlocal/bubble.jl:110-150): Π_R[ω, Ω, R'] accumulated
+// over the R in [-L/2, L/2]^2 that fold onto R' = mod(R, L); GR = G(ν, R) on the LG mesh.
+__global__ void k_bubble_nl_R(double2* __restrict__ pp, double2* __restrict__ ph, const double2* __restrict__ GR,
+                              int nG, int LG, int L, int nW, int nV, double T) {
+    int iw = blockIdx.x * blockDim.x + threadIdx.x;
+    int iW = blockIdx.y;
+    int iR = blockIdx.z;
+    if (iw >= 2 * nV) return;
+    const int NG = 2 * nG;
+    const int m = iW - (nW - 1), n = iw - nV;
+    const int r1p = iR % L, r2p = iR / L;
+    const int h = L / 2;
+    double2 spp = czero(), sph = czero();
+    for (int R2 = -h; R2 <= h; ++R2) {
+        if (((R2 % L) + L) % L != r2p) continue;
+        for (int R1 = -h; R1 <= h; ++R1) {
+            if (((R1 % L) + L) % L != r1p) continue;
+            double w = 1.0;
+            if (LG % 2 == 0) {
+                if (abs(R1) == LG / 2) w *= 0.5;
+                if (abs(R2) == LG / 2) w *= 0.5;
+            }
+            const int kp = (((R1 % LG) + LG) % LG) + LG * (((R2 % LG) + LG) % LG);
+            const int km = (((-R1 % LG) + LG) % LG) + LG * (((-R2 % LG) + LG) % LG);
+            const bool origin = (R1 == 0 && R2 == 0);
+            auto gat = [&](int k, int kk) -> double2 {
+                if (in_f(k, nG)) return GR[(size_t)(k + nG) + (size_t)NG * kk];
+                return origin ? cmk(1.0 / (M_PI * T * (2.0 * k + 1.0)), 0.0) : czero();
+            };
+            spp = cadd(spp, cscale(cmul(gat(bmf(m, n), kp), gat(n, kp)), w));
+            sph = cadd(sph, cscale(cmul(gat(bpf(m, n), kp), gat(n, km)), w));
+        }
+    }
+    size_t o = (size_t)iw + (size_t)(2 * nV) * (iW + (size_t)(2 * nW - 1) * iR);
+    pp[o] = spp;
+    ph[o] = sph;
+}
+
+// s-wave average over P of one array of `n` elements per P
+__global__ void k_pavg(double2* __restrict__ A, const double2* __restrict__ K, size_t n, int NP) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double2 s = czero();
+    for (int p = 0; p < NP; ++p) s = cadd(s, K[i + n * p]);
+    A[i] = cscale(s, 1.0 / NP);
+}
+
+// ---------------------------------------------------------------------------------------
+// build_K3_cache! (src/build_K3_cache.jl:20-95; NL src/nonlocal/build_K3_cache.jl:18-94).
+// The P-independent caches are stored once; the P-dependent part of cache_F* (the same-channel
+// K2 + K3 of the target vertex) is added inside the BSE GEMM operand loader.
+//   c[0] Γpx [ω, νp]   c[1] F0p [ω, νp]   c[2] F0a [ω, νp]   c[3] F0t(d) [ω, νp]
+//   c[4] Γpp [ν, ω]    c[5] Γa  [ν, ω]    c[6] Γt(d) [ν, ω]
+//   c[7] Fbase_p [ν, ω]  c[8] Fbase_a [ν, ω]  c[9] Fbase_t(d) [ν, ω]
+struct CachePtrs { double2* c[10]; };
+
+__global__ void k_cache(CachePtrs out, VtxV F, int n3b, int n3f) {
+    const int NV = 2 * n3f;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    int iW = blockIdx.z;
+    if (i >= NV) return;
+    const int m = iW - (n3b - 1);
+    const int a = i - n3f, b = j - n3f;
+    const size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * iW);
+    const F0V& f0 = F.f0;
+    {   // loop 1 at (Ω, ω = a, νp = b)
+        out.c[0][o] = F.eval(m, a, b, -1, PCH, XSP, FL_GT | FL_GA);
+        out.c[1][o] = csub(f0.eval(m, a, b, PCH, XSP), f0.eval(m, a, FDGA_INF, PCH, XSP));
+        double2 f0a = csub(f0.eval(m, a, b, ACH, PSP), f0.eval(m, a, FDGA_INF, ACH, PSP));
+        double2 f0t = csub(f0.eval(m, a, b, TCH, PSP), f0.eval(m, a, FDGA_INF, TCH, PSP));
+        out.c[2][o] = f0a;
+        out.c[3][o] = csub(cscale(f0t, 2.0), f0a);
+    }
+    {   // loop 2 at (Ω, ν = a, ω = b)
+        double2 gpp = F.eval_psp(m, a, b, -1, PCH, FL_GT | FL_GA);
+        double2 ga = F.eval_psp(m, a, b, -1, ACH, FL_GP | FL_GT);
+        double2 gt = F.eval_psp(m, a, b, -1, TCH, FL_GP | FL_GA);
+        double2 dp = csub(f0.eval_psp(m, a, b, PCH), f0.eval_psp(m, FDGA_INF, b, PCH));
+        double2 da = csub(f0.eval_psp(m, a, b, ACH), f0.eval_psp(m, FDGA_INF, b, ACH));
+        double2 dt = csub(f0.eval_psp(m, a, b, TCH), f0.eval_psp(m, FDGA_INF, b, TCH));
+        out.c[4][o] = gpp;
+        out.c[5][o] = ga;
+        out.c[6][o] = csub(cscale(gt, 2.0), ga);
+        out.c[7][o] = cadd(dp, gpp);
+        double2 fba = cadd(da, ga);
+        out.c[8][o] = fba;
+        out.c[9][o] = csub(cscale(cadd(dt, gt), 2.0), fba);
+    }
+}
+
+// materialise cache_F*[Ω, ν, ω, P] in the reference layout (ABI read-back only)
+__global__ void k_cache_F_julia(double2* __restrict__ out, const double2* __restrict__ base, ChanV gr, ChanV ga, int ch,
+                                int n3b, int n3f, int NP) {
+    const int NV = 2 * n3f, NW = 2 * n3b - 1;
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t tot = (size_t)NW * NV * NV * NP;
+    if (e >= tot) return;
+    int iW = (int)(e % NW);
+    size_t r = e / NW;
+    int i = (int)(r % NV); r /= NV;
+    int j = (int)(r % NV);
+    int p = (int)(r / NV);
+    const int m = iW - (n3b - 1), v = i - n3f, w = j - n3f;
+    double2 x = base[(size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * iW)];
+    double2 dr = cadd(gr.k2(m, v, p), gr.k3(m, v, w, p));
+    if (ch == TCH) dr = csub(cscale(dr, 2.0), cadd(ga.k2(m, v, p), ga.k3(m, v, w, p)));
+    out[e] = cadd(x, dr);
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched complex GEMM on v_mfma_f64_16x16x4f64.  C[b](m, n) = Σ_k A[b](m, k) B[b](k, n),
+// operands produced element-wise by loader functors (the gather is fused into the load),
+// results consumed by an epilogue functor.  256 threads = 4 waves, 64 x 64 complex tile per
+// workgroup, each wave a 32 x 32 sub-tile = 2 x 2 MFMA tiles with separate real/imag
+// accumulators (C_r = A_r B_r - A_i B_i, C_i = A_r B_i + A_i B_r).
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
+
+template <class AL, class BL, class EP>
+__global__ __launch_bounds__(256) void zgemm_tiles(int M, int N, int K, int tiles_m, int tiles_n, AL al, BL bl, EP ep) {
+    __shared__ double As[2][GBK][GBM + GPAD];
+    __shared__ double Bs[2][GBK][GBN + GPAD];
+    const int tiles = tiles_m * tiles_n;
+    const int bid = blockIdx.x;
+    const int batch = bid / tiles;
+    const int t = bid - batch * tiles;
+    const int m0 = (t % tiles_m) * GBM, n0 = (t / tiles_m) * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    v4d accr[2][2], acci[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            accr[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+            acci[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+#pragma unroll
+        for (int q = 0; q < (GBM * GBK) / 256; ++q) {
+            const int e = tid + 256 * q;
+            const int mm = e % GBM, kk = e / GBM;
+            const int gm = m0 + mm, gk = k0 + kk;
+            double2 v = (gm < M && gk < K) ? al(batch, gm, gk) : czero();
+            As[0][kk][mm] = v.x;
+            As[1][kk][mm] = v.y;
+        }
+#pragma unroll
+        for (int q = 0; q < (GBN * GBK) / 256; ++q) {
+            const int e = tid + 256 * q;
+            const int nn = e % GBN, kk = e / GBN;
+            const int gn = n0 + nn, gk = k0 + kk;
+            double2 v = (gn < N && gk < K) ? bl(batch, gk, gn) : czero();
+            Bs[0][kk][nn] = v.x;
+            Bs[1][kk][nn] = v.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 4) {
+            const int kr = kk + (lane >> 4), c = lane & 15;
+            double ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ar[i] = As[0][kr][wm + 16 * i + c];
+                ai[i] = As[1][kr][wm + 16 * i + c];
+                nai[i] = -ai[i];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                br[j] = Bs[0][kr][wn + 16 * j + c];
+                bi[j] = Bs[1][kr][wn + 16 * j + c];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], accr[i][j], 0, 0, 0);
+                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
+                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    // v_mfma_f64_16x16x4f64 accumulator: element r of lane l is D(row = l/16 + 4 r, col = l % 16)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
+                const int col = n0 + wn + 16 * j + (lane & 15);
+                if (row < M && col < N) ep(batch, row, col, cmk(accr[i][j][r], acci[i][j][r]));
+            }
+}
+
+// ---------------------------------------------------------------------------------------
+// operands of the K3-class BSE for one channel; batch = iΩ + NW3 * P
+struct K3Common {
+    int n3b, n3f, NP;
+    int nWP, nVP;                 // bubble mesh N's
+    const double2* Pi;            // [ω, Ω, P] bubble mesh
+    const double2* Pi0;
+    double T;
+    __device__ int NV() const { return 2 * n3f; }
+    __device__ int NW() const { return 2 * n3b - 1; }
+    __device__ size_t pidx(int m, int w, int p) const {
+        return (size_t)(w + nVP) + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    }
+};
+
+// BSE_L_K3! (src/BSEa/BSEa_K3.jl:1-41): A = Γr[ν, ω, Ω], B = Π0(Ω, ω, P) F0r[ω, νp, Ω]
+struct LK3_Aop {
+    const double2* G; int NV, NW;
+    __device__ double2 operator()(int b, int i, int k) const {
+        const int iW = b % NW;
+        return G[(size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
+    }
+};
+struct LK3_Bop {
+    K3Common c;
+    const double2* F0r;
+    __device__ double2 operator()(int b, int k, int j) const {
+        const int NW = c.NW(), NV = c.NV();
+        const int iW = b % NW, p = b / NW;
+        const int m = iW - (c.n3b - 1), w = k - c.n3f;
+        return cmul(c.Pi0[c.pidx(m, w, p)], F0r[(size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW)]);
+    }
+};
+// FL.γr.K3 = T s acc; t channel: (T s acc + FL.γa.K3) / 2 (src/BSE_templates.jl:136-145)
+struct LK3_Ep {
+    double2* out; const double2* outa; double scale; int NV; int tpost;
+    __device__ void operator()(int b, int i, int j, double2 v) const {
+        size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * b);
+        double2 x = cscale(v, scale);
+        if (tpost) x = cscale(cadd(x, outa[o]), 0.5);
+        out[o] = x;
+    }
+};
+
+// BSE_K3! (src/BSEa/BSEa_K3.jl:43-128):
+//   A(ν, ω)  = cache_Fr = Fbase_r[ν, ω, Ω] + Δγ_r(Ω, ν, ω, P)     (t: 2 Δγ_t - Δγ_a)
+//   B(ω, νp) = s1 [(Π-Π0)(ω) F0r[ω, νp] + Π(ω) Γ~(ω, νp)] + s2 Π(ω) L~(ω, νp) 1(ω~ in box)
+struct K3_Aop {
+    K3Common c;
+    const double2* Fbase;
+    ChanV gr, ga;   // target vertex channel r (and a, for t)
+    int ch;
+    __device__ double2 operator()(int b, int i, int k) const {
+        const int NW = c.NW(), NV = c.NV();
+        const int iW = b % NW, p = b / NW;
+        const int m = iW - (c.n3b - 1), v = i - c.n3f, w = k - c.n3f;
+        double2 x = Fbase[(size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
+        double2 dr = cadd(gr.k2(m, v, p), gr.k3(m, v, w, p));
+        if (ch == TCH) dr = csub(cscale(dr, 2.0), cadd(ga.k2(m, v, p), ga.k3(m, v, w, p)));
+        return cadd(x, dr);
+    }
+};
+struct K3_Bop {
+    K3Common c;
+    const double2* F0r;   // [ω, νp, Ω]
+    const double2* Gam;   // p: Γpx [ω, νp, Ω]; a, t: Γr [ν, ω, Ω] read transposed
+    const double2* Lr;    // FL.γr.K3 [ν, νp, Ω, P]
+    const double2* La;    // FL.γa.K3 (t channel)
+    int ch;
+    double s1, s2;
+    __device__ double2 operator()(int b, int k, int j) const {
+        const int NW = c.NW(), NV = c.NV();
+        const int iW = b % NW, p = b / NW;
+        const int m = iW - (c.n3b - 1), w = k - c.n3f;
+        const size_t pi = c.pidx(m, w, p);
+        const double2 P1 = c.Pi[pi], P0 = c.Pi0[pi];
+        const size_t kj = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW);
+        const double2 f0 = F0r[kj];
+        const double2 g = (ch == PCH) ? Gam[kj] : Gam[(size_t)j + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
+        double2 x = cscale(cadd(cmul(csub(P1, P0), f0), cmul(P1, g)), s1);
+        const int w_ = crossing(m, w, ch);
+        if (in_f(w_, c.n3f)) {
+            const size_t lo = (size_t)(w_ + c.n3f) + (size_t)NV * ((size_t)j + (size_t)NV * ((size_t)iW + (size_t)NW * p));
+            double2 L = Lr[lo];
+            if (ch == TCH) L = csub(cscale(L, 2.0), La[lo]);
+            x = cadd(x, cscale(cmul(P1, L), s2));
+        }
+        return x;
+    }
+};
+// Fbuff.γr.K3 = T acc + L_self; t: then (γt + γa) / 2
+struct K3_Ep {
+    double2* out; const double2* outa; const double2* Lr; const double2* La; double T; int NV; int ch;
+    __device__ void operator()(int b, int i, int j, double2 v) const {
+        size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * b);
+        double2 self = Lr[o];
+        if (ch == TCH) self = csub(cscale(self, 2.0), La[o]);
+        double2 x = cadd(cscale(v, T), self);
+        if (ch == TCH) x = cscale(cadd(x, outa[o]), 0.5);
+        out[o] = x;
+    }
+};
+
+// dense column-major operands (library self-test of the GEMM engine)
+struct Dense_A { const double2* A; int lda; __device__ double2 operator()(int b, int i, int k) const { return A[(size_t)i + (size_t)lda * k]; } };
+struct Dense_B { const double2* B; int ldb; __device__ double2 operator()(int b, int k, int j) const { return B[(size_t)k + (size_t)ldb * j]; } };
+struct Dense_C { double2* C; int ldc; __device__ void operator()(int b, int i, int j, double2 v) const { C[(size_t)i + (size_t)ldc * j] = v; } };
+
+// ---------------------------------------------------------------------------------------
+// BSE weight on the bubble mesh for K1/K2 (src/BSEa/BSEa_K1.jl:38-43, BSEa_K2.jl:77-85):
+//   W(ω, Ω, P) = (Π - Π0) F0(Ω, ω~, ∞, Ch, Sp) + Π FL(Ω, ω~, ∞, P, Ch, Sp),  ω~ = crossing(Ω, ω)
+__global__ void k_bse_w(double2* __restrict__ W, const double2* __restrict__ Pi, const double2* __restrict__ Pi0,
+                        VtxV FL, F0V f0, int nWP, int nVP, int ch, int sp) {
+    int iw = blockIdx.x * blockDim.x + threadIdx.x;
+    int iW = blockIdx.y, p = blockIdx.z;
+    if (iw >= 2 * nVP) return;
+    const int m = iW - (nWP - 1), w = iw - nVP;
+    const size_t o = (size_t)iw + (size_t)(2 * nVP) * ((size_t)iW + (size_t)(2 * nWP - 1) * p);
+    const int wc = crossing(m, w, ch);
+    double2 f0r = f0.eval(m, wc, FDGA_INF, ch, sp);
+    double2 flr = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
+    W[o] = cadd(cmul(csub(Pi[o], Pi0[o]), f0r), cmul(Pi[o], flr));
+}
+
+// BSE_K1!: K1(Ω, P) = T s Σ_ω F(Ω, ∞, ω, P, Ch, Sp) W(ω)   (one wave per output)
+__global__ void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a, const double2* __restrict__ W,
+                          VtxV F, int n1, int nWP, int nVP, int NP, int ch, int sp, double scale, int tpost) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW1 = 2 * n1 - 1;
+    if (wid >= NW1 * NP) return;
+    const int iW = wid % NW1, p = wid / NW1;
+    const int m = iW - (n1 - 1);
+    const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    double sr = 0.0, si = 0.0;
+    for (int iw = lane; iw < 2 * nVP; iw += WAVE) {
+        const int w = iw - nVP;
+        double2 x = cmul(F.eval(m, FDGA_INF, w, p, ch, sp, FL_ALL), Wr[iw]);
+        sr += x.x;
+        si += x.y;
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) {
+        size_t o = (size_t)iW + (size_t)NW1 * p;
+        double2 v = cscale(cmk(sr, si), scale);
+        if (tpost) v = cscale(cadd(v, K1a[o]), 0.5);
+        K1[o] = v;
+    }
+}
+
+// BSE_K2!: K2(Ω, ν, P) = T s Σ_ω [F(Ω,ν,ω) - F(Ω,∞,ω)] W(ω) + FL.K2 (t: 2 FLt - FLa)
+__global__ void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a, const double2* __restrict__ W,
+                          VtxV F, ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int nVP, int NP, int ch, int sp,
+                          double scale) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
+    if (wid >= NW2 * NV2 * NP) return;
+    const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
+    const int m = iW - (n2b - 1), v = iv - n2f;
+    const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    double sr = 0.0, si = 0.0;
+    for (int iw = lane; iw < 2 * nVP; iw += WAVE) {
+        const int w = iw - nVP;
+        double2 fl = csub(F.eval(m, v, w, p, ch, sp, FL_ALL), F.eval(m, FDGA_INF, w, p, ch, sp, FL_ALL));
+        double2 x = cmul(fl, Wr[iw]);
+        sr += x.x;
+        si += x.y;
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) {
+        const size_t o = (size_t)wid;
+        double2 val = cscale(cmk(sr, si), scale);
+        if (ch == TCH) {
+            val = cadd(val, cscale(Lr.K2[o], 2.0));
+            val = cadd(val, cscale(La.K2[o], -1.0));
+            val = cscale(cadd(val, K2a[o]), 0.5);
+        } else {
+            val = cadd(val, Lr.K2[o]);
+        }
+        K2[o] = val;
+    }
+}
+
+// BSE_L_K2!: local (src/BSEa/BSEa_K2.jl:1-40): Σ over the bubble mesh of
+//   Γ(Ω, ν, ω) Π0(ω) F0(Ω, ω~, ∞);  NL (src/nonlocal/BSEa/BSEa_K2.jl:1-41): Σ over the K2 mesh of
+//   Γ(Ω, ν, ω~) Π0(ω) F0(Ω, ω, ∞)
+__global__ void k_gemv_lk2(double2* __restrict__ out, const double2* __restrict__ outa, const double2* __restrict__ Pi0,
+                           VtxV F, int n2b, int n2f, int nWP, int nVP, int NP, int ch, int sp, int nonlocal,
+                           double scale, int tpost) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
+    if (wid >= NW2 * NV2 * NP) return;
+    const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
+    const int m = iW - (n2b - 1), v = iv - n2f;
+    const int fl = FL_ALL & ~FL_F0 & ~(FL_GP << ch);
+    const double2* P0 = Pi0 + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    double sr = 0.0, si = 0.0;
+    const int cnt = nonlocal ? NV2 : 2 * nVP;
+    for (int q = lane; q < cnt; q += WAVE) {
+        double2 x;
+        if (!nonlocal) {
+            const int w = q - nVP;
+            x = cmul(cmul(F.eval(m, v, w, p, ch, sp, fl), P0[q]), F.f0.eval(m, crossing(m, w, ch), FDGA_INF, ch, sp));
+        } else {
+            const int w = q - n2f;
+            x = cmul(cmul(F.eval(m, v, crossing(m, w, ch), p, ch, sp, fl), P0[w + nVP]), F.f0.eval(m, w, FDGA_INF, ch, sp));
+        }
+        sr += x.x;
+        si += x.y;
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) {
+        double2 val = cscale(cmk(sr, si), scale);
+        if (tpost) val = cscale(cadd(val, outa[wid]), 0.5);
+        out[wid] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// SDE_channel_L_pp! / _ph! (src/SDE.jl:79-217; NL src/nonlocal/SDE.jl:3-142).
+// kind 0: Vertex channels g (momentum p if NP > 1); kind 1: RefVertex tables.
+__global__ void k_sde_L(double2* __restrict__ Lpp, double2* __restrict__ Lph, const double2* __restrict__ Pipp,
+                        const double2* __restrict__ Piph, ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
+                        int n2b, int n2f, int nWP, int nVP, int NP, int chanNP, double2 TU) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
+    if (wid >= NW2 * NV2 * NP) return;
+    const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
+    const int m = iW - (n2b - 1), v = iv - n2f;
+    const int pc = chanNP > 1 ? p : 0;
+    const size_t base = (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+    for (int iw = lane; iw < 2 * nVP; iw += WAVE) {
+        const int w = iw - nVP;
+        double2 xp, xh;
+        if (kind == 0) {
+            xp = gp.eval(m, bmf(m, w), v, pc);
+            xh = cadd(ga.eval(m, v, w, pc), gt.eval(m, v, w, pc));
+        } else {
+            xp = csub(ref.eval(m, bmf(m, w), v, PCH, PSP), ref.bare(PSP));
+            xh = csub(csub(cadd(ref.eval(m, v, w, ACH, PSP), ref.eval(m, v, w, TCH, PSP)), ref.bare(PSP)), ref.bare(PSP));
+        }
+        double2 a = cmul(Pipp[base + iw], xp);
+        double2 b = cmul(Piph[base + iw], xh);
+        ar += a.x; ai += a.y; br += b.x; bi += b.y;
+    }
+    ar = wave_sum(ar); ai = wave_sum(ai); br = wave_sum(br); bi = wave_sum(bi);
+    if (lane == 0) {
+        Lpp[wid] = cmul(cmk(ar, ai), TU);
+        Lph[wid] = cmul(cmk(br, bi), TU);
+    }
+}
+
+// Σ(ν) += scale T Σ_Ω [G(Ω-ν) Lpp(Ω,ν) + G(Ω+ν) Lph(Ω,ν)]  (src/SDE.jl:242-263)
+__global__ void k_sde_sigma_local(double2* __restrict__ S, const double2* __restrict__ G, const double2* __restrict__ Lpp,
+                                  const double2* __restrict__ Lph, int nG, int n2b, int n2f, double T, double scale) {
+    int iv = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iv >= 2 * nG) return;
+    const int v = iv - nG;
+    if (!in_f(v, n2f)) return;
+    const int NW2 = 2 * n2b - 1;
+    double2 s = czero();
+    for (int iW = 0; iW < NW2; ++iW) {
+        const int m = iW - (n2b - 1);
+        const size_t k = (size_t)iW + (size_t)NW2 * (v + n2f);
+        const int a = bmf(m, v), b = bpf(m, v);
+        if (in_f(a, nG)) s = cadd(s, cmul(G[a + nG], Lpp[k]));
+        if (in_f(b, nG)) s = cadd(s, cmul(G[b + nG], Lph[k]));
+    }
+    S[iv] = cadd(S[iv], cscale(s, T * scale));
+}
+
+// NL SDE_compute_inner! real-space step (src/nonlocal/SDE.jl:209-276): for output column q of
+// Σ_R[ν, q] (q on the LG mesh) collect Σ_R( R) += G(-R) Lpp(R) and Σ_R(-R) += G(-R) Lph(R)
+// over R in [-L/2, L/2]^2 (half weights at |R_i| = L/2 for even L); the result is T Σ_R.
+__global__ void k_sde_sigma_nl_R(double2* __restrict__ SR, const double2* __restrict__ GR, const double2* __restrict__ LppR,
+                                 const double2* __restrict__ LphR, int nG, int n2b, int n2f, int LG, int L, double T) {
+    const int NG = 2 * nG;
+    int iv = blockIdx.x * blockDim.x + threadIdx.x;
+    int q = blockIdx.y;
+    if (iv >= NG) return;
+    const int v = iv - nG;
+    double2 s = czero();
+    if (in_f(v, n2f)) {
+        const int q1 = q % LG, q2 = q / LG, h = L / 2;
+        const int NW2 = 2 * n2b - 1;
+        const size_t nrow = (size_t)NW2 * (2 * n2f);
+        auto md = [](int a, int n) { return ((a % n) + n) % n; };
+        for (int R2 = -h; R2 <= h; ++R2)
+            for (int R1 = -h; R1 <= h; ++R1) {
+                const bool tpp = md(R1, LG) == q1 && md(R2, LG) == q2;
+                const bool tph = md(-R1, LG) == q1 && md(-R2, LG) == q2;
+                if (!tpp && !tph) continue;
+                double w = 1.0;
+                if (L % 2 == 0) {
+                    if (abs(R1) == L / 2) w *= 0.5;
+                    if (abs(R2) == L / 2) w *= 0.5;
+                }
+                const size_t iRL = (size_t)(md(R1, L) + L * md(R2, L));
+                const size_t imRG = (size_t)(md(-R1, LG) + LG * md(-R2, LG));
+                for (int iW = 0; iW < NW2; ++iW) {
+                    const int m = iW - (n2b - 1);
+                    const size_t k = (size_t)iW + (size_t)NW2 * (v + n2f) + nrow * iRL;
+                    if (tpp) {
+                        const int a = bmf(m, v);
+                        if (in_f(a, nG)) s = cadd(s, cscale(cmul(GR[(size_t)(a + nG) + (size_t)NG * imRG], LppR[k]), w));
+                    }
+                    if (tph) {
+                        const int b = bpf(m, v);
+                        if (in_f(b, nG)) s = cadd(s, cscale(cmul(GR[(size_t)(b + nG) + (size_t)NG * imRG], LphR[k]), w));
+                    }
+                }
+            }
+    }
+    SR[(size_t)iv + (size_t)NG * q] = cscale(s, T);
+}
+
+// SDE_U2_using_G (src/SDE.jl:328-364) via C(m) = Σ_n G(n) G(n+m):
+//   Σ_U²(ν) = U² T² Σ_m C(m) G(m + ν); for the NL solver one column per R of the real-space G.
+__global__ void k_u2_C(double2* __restrict__ C, const double2* __restrict__ Gm, const double2* __restrict__ Gp, int nG, int nR) {
+    const int NG = 2 * nG, NB = 4 * nG - 1;
+    int im = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (im >= NB || r >= nR) return;
+    const int m = im - (2 * nG - 1);
+    double2 s = czero();
+    for (int i2 = 0; i2 < NG; ++i2) {
+        const int n2 = i2 - nG, n1 = n2 + m;
+        if (in_f(n1, nG)) s = cadd(s, cmul(Gm[(size_t)(n1 + nG) + (size_t)NG * r], Gp[(size_t)i2 + (size_t)NG * r]));
+    }
+    C[(size_t)im + (size_t)NB * r] = s;
+}
+__global__ void k_u2_S(double2* __restrict__ S, const double2* __restrict__ C, const double2* __restrict__ Gp, int nG, int nR,
+                       double2 fac, int accumulate) {
+    const int NG = 2 * nG, NB = 4 * nG - 1;
+    int iv = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (iv >= NG || r >= nR) return;
+    const int v = iv - nG;
+    double2 s = czero();
+    for (int im = 0; im < NB; ++im) {
+        const int m = im - (2 * nG - 1), x = m + v;
+        if (in_f(x, nG)) s = cadd(s, cmul(C[(size_t)im + (size_t)NB * r], Gp[(size_t)(x + nG) + (size_t)NG * r]));
+    }
+    s = cmul(s, fac);
+    size_t o = (size_t)iv + (size_t)NG * r;
+    S[o] = accumulate ? cadd(S[o], s) : s;
+}
+
+// occupation n = 1/2 + T Im Σ G / Nk (src/dyson.jl:33-41): single-workgroup reduction
+__global__ void k_occupation(double* __restrict__ out, const double2* __restrict__ G, size_t n, double T, double nk) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) s += G[i].y;
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = 0.5 + red[0] * T / nk;
+}
+// Σ += sign (n - 1/2) U i
+__global__ void k_hartree(double2* __restrict__ S, size_t n, const double* __restrict__ occ, double2 U, double sign) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double f = (occ[0] - 0.5) * sign;
+    S[i] = cadd(S[i], cmul(cmk(0.0, f), U));
+}
+
+// zero fill (used instead of hipMemsetAsync so that a captured iteration holds kernel nodes only)
+__global__ void k_zero(double2* __restrict__ y, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = czero();
+}
+
+// y = a x + b y (real scalars a, b)
+__global__ void k_axpby(double2* __restrict__ y, const double2* __restrict__ x, double a, double b, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = cadd(cscale(x[i], a), cscale(y[i], b));
+}
+
+// layout permutations between the reference layout and the device layout
+// K3: ref [Ω, ν, νp, P] <-> dev [ν, νp, Ω, P]
+__global__ void k_k3_to_dev(double2* __restrict__ out, const double2* __restrict__ in, int NW, int NV, int NP) {
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t tot = (size_t)NW * NV * NV * NP;
+    if (e >= tot) return;
+    int iW = (int)(e % NW);
+    size_t r = e / NW;
+    int i = (int)(r % NV); r /= NV;
+    int j = (int)(r % NV);
+    size_t p = r / NV;
+    out[(size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * ((size_t)iW + (size_t)NW * p))] = in[e];
+}
+__global__ void k_k3_to_ref(double2* __restrict__ out, const double2* __restrict__ in, int NW, int NV, int NP) {
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t tot = (size_t)NW * NV * NV * NP;
+    if (e >= tot) return;
+    int iW = (int)(e % NW);
+    size_t r = e / NW;
+    int i = (int)(r % NV); r /= NV;
+    int j = (int)(r % NV);
+    size_t p = r / NV;
+    out[e] = in[(size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * ((size_t)iW + (size_t)NW * p))];
+}
+// bubble: ref [Ω, ν, P] <-> dev [ν, Ω, P]
+__global__ void k_pi_to_dev(double2* __restrict__ out, const double2* __restrict__ in, int NW, int NV, int NP) {
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t tot = (size_t)NW * NV * NP;
+    if (e >= tot) return;
+    int iW = (int)(e % NW);
+    size_t r = e / NW;
+    int iv = (int)(r % NV);
+    size_t p = r / NV;
+    out[(size_t)iv + (size_t)NV * ((size_t)iW + (size_t)NW * p)] = in[e];
+}
+__global__ void k_pi_to_ref(double2* __restrict__ out, const double2* __restrict__ in, int NW, int NV, int NP) {
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t tot = (size_t)NW * NV * NP;
+    if (e >= tot) return;
+    int iW = (int)(e % NW);
+    size_t r = e / NW;
+    int iv = (int)(r % NV);
+    size_t p = r / NV;
+    out[e] = in[(size_t)iv + (size_t)NV * ((size_t)iW + (size_t)NW * p)];
+}
+// P-independent cache [i, j, Ω] -> ref [Ω, i, j, P] (replicated over P)
+__global__ void k_cache_to_ref(double2* __restrict__ out, const double2* __restrict__ in, int NW, int NV, int NP) {
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t tot = (size_t)NW * NV * NV * NP;
+    if (e >= tot) return;
+    int iW = (int)(e % NW);
+    size_t r = e / NW;
+    int i = (int)(r % NV); r /= NV;
+    int j = (int)(r % NV);
+    out[e] = in[(size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * iW)];
+}
+
+}  // namespace fdga

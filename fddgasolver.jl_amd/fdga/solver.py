@@ -1,0 +1,338 @@
+"""Host side of the MI355X parquet solver, mirroring fdDGAsolver.jl's solver API.
+
+Names follow the reference (src/fdDGAsolver.jl:180-205) with Julia's `!` dropped:
+parquet_solver_siam_parquet_approximation, parquet_solver_hubbard_parquet_approximation,
+ParquetSolver, NL_ParquetSolver, iterate_solver, fixed_point, solve, Dyson, bubbles,
+build_K3_cache, BSE_L_K2, BSE_L_K3, BSE_K1, BSE_K2, BSE_K3, SDE, flatten, unflatten.
+All numerical work runs on the GPU through libfdga.so; this module only allocates the
+context, converts arguments and drives the fixed-point loop.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+pCh, tCh, aCh = L.PCH, L.TCH, L.ACH
+_STRATEGIES = {"scPA": L.SCPA, "fdPA": L.FDPA}
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class FdgaError(RuntimeError):
+    pass
+
+
+class Solver:
+    """Device-resident ParquetSolver (local) or NL_ParquetSolver (s-wave lattice).
+
+    Constructor arguments follow ParquetSolver (src/ParquetSolver.jl:81-97) and
+    NL_ParquetSolver (src/nonlocal/ParquetSolver.jl:77-92): boxes nK1, nK2, nK3, the fermionic
+    G mesh nG, the bubble factor mΠν_factor, the k-meshes LG (G, Σ) and L (vertex, bubble)."""
+
+    def __init__(self, *, nG, nK1, nK2, nK3, T, U, mPinu, nonlocal_=False, LG=1, L_=1,
+                 ref_kind=0, ref_boxes=None, tab=(1, 1), fdpa_h0=True):
+        if nK1 < max(nK2) or any(a < b for a, b in zip(nK2, nK3)):
+            raise AssertionError("Number of frequencies in K1 >= K2 >= K3 required")
+        d = L.Dims()
+        d.nonlocal_ = int(nonlocal_)
+        d.nG, d.nK1 = nG, nK1
+        d.nK2b, d.nK2f = nK2
+        d.nK3b, d.nK3f = nK3
+        d.mPinu = mPinu
+        d.LG, d.L = (LG, L_) if nonlocal_ else (1, 1)
+        d.T, d.U = T, U
+        d.ref_kind = ref_kind
+        rb = ref_boxes or (1, (1, 1), (1, 1))
+        d.r_nK1 = rb[0]
+        d.r_nK2b, d.r_nK2f = rb[1]
+        d.r_nK3b, d.r_nK3f = rb[2]
+        d.tab_nb, d.tab_nf = tab
+        d.fdpa_h0 = int(fdpa_h0)
+        d.precision = L.FP64
+        self.dims = d
+        self.T, self.U = T, U
+        self.nonlocal_ = bool(nonlocal_)
+        self.nG, self.nK1, self.nK2, self.nK3, self.mPinu = nG, nK1, tuple(nK2), tuple(nK3), mPinu
+        self.LG, self.L = d.LG, d.L
+        self.NP, self.NkG = d.L * d.L, d.LG * d.LG
+        self.lib = L.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.fdga_create(ctypes.byref(h), ctypes.byref(d))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.fdga_last_error(h).decode() if h.value else "fdga_create failed"
+            if h.value:
+                self.lib.fdga_destroy(h)
+            self.h = None
+            raise FdgaError(msg)
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.lib.fdga_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FdgaError(self.lib.fdga_last_error(self.h).decode())
+
+    # ------------------------------------------------------------------ arrays
+    def size(self, which):
+        return self.lib.fdga_size(self.h, which)
+
+    def get(self, which):
+        n = self.size(which)
+        if n < 0:
+            raise KeyError(which)
+        out = np.empty(n, dtype=np.complex128)
+        self._check(self.lib.fdga_get(self.h, which, _dp(out.view(np.float64))))
+        return out
+
+    def set(self, which, data):
+        n = self.size(which)
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.complex128).reshape(-1, order="F"))
+        if a.size != n:
+            raise ValueError(f"array {which}: expected {n} elements, got {a.size}")
+        self._check(self.lib.fdga_set(self.h, which, _dp(a.view(np.float64))))
+
+    def vertex(self, vtx, ch, cls):
+        return self.get(L.vertex_id(vtx, ch, cls))
+
+    def set_vertex(self, vtx, ch, cls, data):
+        self.set(L.vertex_id(vtx, ch, cls), data)
+
+    @property
+    def Σ(self):
+        return self.get(L.SIGMA).reshape(2 * self.nG, self.NkG, order="F")
+
+    def init(self):
+        """Constructor tail: Π0 = bubbles(G0), G = Dyson(Σ0, Gbare), Π = bubbles(G)."""
+        self._check(self.lib.fdga_init(self.h))
+
+    # ------------------------------------------------------------------ timing
+    def time_iterations(self, strategy, n):
+        ms = ctypes.c_double(0.0)
+        self._check(self.lib.fdga_time_iterations(self.h, _strategy(strategy), n, ctypes.byref(ms)))
+        return ms.value
+
+    def set_stage_timing(self, on):
+        self._check(self.lib.fdga_set_stage_timing(self.h, int(on)))
+
+    def stage_ms(self):
+        out = []
+        for s in range(9):
+            ms = ctypes.c_double(0.0)
+            self._check(self.lib.fdga_stage_ms(self.h, s, ctypes.byref(ms)))
+            out.append(ms.value)
+        return out
+
+    def sync(self):
+        self._check(self.lib.fdga_sync(self.h))
+
+
+def _strategy(strategy):
+    if isinstance(strategy, int):
+        return strategy
+    if strategy not in _STRATEGIES:
+        raise AssertionError("Calculation strategy unknown")
+    return _STRATEGIES[strategy]
+
+
+# ------------------------------------------------------------------------------ models
+def siam_bare_Green(nG, T, *, e, Δ, D):
+    """i*G0 of the SIAM on MatsubaraMesh(T, nG, Fermion) (src/models/siam.jl:9-29); D=None: D = ∞."""
+    n = np.arange(-nG, nG)
+    v = np.pi * T * (2 * n + 1)
+    if D is None or D == np.inf:
+        return 1.0 / (v + 1j * e + Δ * np.sign(v))
+    return 1.0 / (v + 1j * e + 2 * Δ / np.pi * np.arctan(D / v))
+
+
+def hubbard_band(k1, k2, *, t1, t2=0.0, t3=0.0):
+    """src/models/hubbard.jl:31-44"""
+    return (-2 * t1 * (np.cos(k1) + np.cos(k2)) - 4 * t2 * np.cos(k1) * np.cos(k2)
+            - 2 * t3 * (np.cos(2 * k1) + np.cos(2 * k2)))
+
+
+def hubbard_bare_Green(nG, T, LG, *, μ, t1, t2=0.0, t3=0.0):
+    """i*G0 of the square-lattice Hubbard model, [ν, k] with k = 2π(j1, j2)/LG, j1 fastest
+    (src/models/hubbard.jl:8-29)."""
+    n = np.arange(-nG, nG)
+    v = np.pi * T * (2 * n + 1)
+    j = np.arange(LG)
+    k1 = np.tile(2 * np.pi * j / LG, LG)
+    k2 = np.repeat(2 * np.pi * j / LG, LG)
+    ek = hubbard_band(k1, k2, t1=t1, t2=t2, t3=t3)
+    return 1j / (1j * v[:, None] + μ - ek[None, :])
+
+
+def compute_occupation(G, T, Nk=1):
+    """src/dyson.jl:33-41"""
+    return 0.5 + np.imag(np.sum(G)) * T / Nk
+
+
+# ------------------------------------------------------------------------------ constructors
+def ParquetSolver(nK1, nK2, nK3, Gbare, G0, Σ0, F0=None, *, T, U, mΠν_factor=6, fdpa_h0=True,
+                  F0_boxes=None, F0_tables=None):
+    """ParquetSolver(nK1, nK2, nK3, Gbare, G0, Σ0, F0) (src/ParquetSolver.jl:81-156).
+
+    F0 = None: RefVertex(T, U) with null tables.  F0 = dict {(ch, cls): array} of a local
+    reference Vertex around RefVertex(T, U) with boxes F0_boxes = (nK1, nK2, nK3).
+    F0_tables = (tb, tf, {'Fp_p': .., 'Fp_x': .., 'Ft_p': .., 'Ft_x': ..}) optional RefVertex tables."""
+    nG = len(Gbare) // 2
+    return _make(False, nG, nK1, nK2, nK3, T, U, mΠν_factor, 1, 1, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0)
+
+
+def NL_ParquetSolver(nK1, nK2, nK3, L, Gbare, G0, Σ0, F0=None, *, T, U, mΠν_factor=32, fdpa_h0=True,
+                     F0_boxes=None, F0_tables=None):
+    """NL_ParquetSolver(nK1, nK2, nK3, mK_Γ, Gbare, G0, Σ0, F0) (src/nonlocal/ParquetSolver.jl:77-152);
+    Gbare, G0, Σ0 are [ν, k] on an LG x LG mesh, the vertex lives on an L x L mesh."""
+    Gbare = np.asarray(Gbare)
+    nG = Gbare.shape[0] // 2
+    LG = int(round(np.sqrt(Gbare.shape[1])))
+    return _make(True, nG, nK1, nK2, nK3, T, U, mΠν_factor, LG, L, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0)
+
+
+def _make(nonlocal_, nG, nK1, nK2, nK3, T, U, mp, LG, L_, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0):
+    tab = (1, 1)
+    if F0_tables is not None:
+        tab = (F0_tables[0], F0_tables[1])
+    S = Solver(nG=nG, nK1=nK1, nK2=nK2, nK3=nK3, T=T, U=U, mPinu=mp, nonlocal_=nonlocal_, LG=LG, L_=L_,
+               ref_kind=0 if F0 is None else 1, ref_boxes=F0_boxes, tab=tab, fdpa_h0=fdpa_h0)
+    S.set(L.GBARE, Gbare)
+    S.set(L.G0, G0)
+    S.set(L.SIGMA0, Σ0)
+    if F0 is not None:
+        for (ch, cls), arr in F0.items():
+            S.set_vertex(3, ch, cls, arr)
+    if F0_tables is not None:
+        for name, which in (("Fp_p", L.TAB_FPP), ("Fp_x", L.TAB_FPX), ("Ft_p", L.TAB_FTP), ("Ft_x", L.TAB_FTX)):
+            if name in F0_tables[2]:
+                S.set(which, F0_tables[2][name])
+    S.init()
+    return S
+
+
+def parquet_solver_siam_parquet_approximation(nG, nK1, nK2, nK3, *, e, Δ, D, T, U, mΠν_factor=6, fdpa_h0=True):
+    """src/ParquetSolver.jl:170-200: SIAM with G0 = Σ0 = 0 and F0 = RefVertex(T, U)."""
+    Gbare = siam_bare_Green(nG, T, e=e, Δ=Δ, D=D)
+    z = np.zeros_like(Gbare)
+    return ParquetSolver(nK1, nK2, nK3, Gbare, z, z, T=T, U=U, mΠν_factor=mΠν_factor, fdpa_h0=fdpa_h0)
+
+
+def parquet_solver_hubbard_parquet_approximation(nG, nK1, nK2, nK3, LG, L, *, T, U, μ, t1, t2=0.0, t3=0.0,
+                                                 mΠν_factor=32, fdpa_h0=True):
+    """src/nonlocal/ParquetSolver.jl:166-196: Hubbard with G0 = Σ0 = 0 and F0 = RefVertex(T, U)."""
+    Gbare = hubbard_bare_Green(nG, T, LG, μ=μ, t1=t1, t2=t2, t3=t3)
+    z = np.zeros_like(Gbare)
+    return NL_ParquetSolver(nK1, nK2, nK3, L, Gbare, z, z, T=T, U=U, mΠν_factor=mΠν_factor, fdpa_h0=fdpa_h0)
+
+
+# ------------------------------------------------------------------------------ stages
+def Dyson(S):
+    S._check(S.lib.fdga_dyson(S.h))
+
+
+def bubbles(S):
+    S._check(S.lib.fdga_bubbles(S.h))
+
+
+def build_K3_cache(S):
+    S._check(S.lib.fdga_build_k3_cache(S.h))
+
+
+def BSE_L_K2(S, Ch):
+    S._check(S.lib.fdga_bse_l_k2(S.h, Ch))
+
+
+def BSE_L_K3(S, Ch):
+    S._check(S.lib.fdga_bse_l_k3(S.h, Ch))
+
+
+def BSE_K1(S, Ch):
+    S._check(S.lib.fdga_bse_k1(S.h, Ch))
+
+
+def BSE_K2(S, Ch):
+    S._check(S.lib.fdga_bse_k2(S.h, Ch))
+
+
+def BSE_K3(S, Ch):
+    S._check(S.lib.fdga_bse_k3(S.h, Ch))
+
+
+def update_F(S):
+    S._check(S.lib.fdga_update_F(S.h))
+
+
+def SDE(S, strategy="scPA", include_Hartree=True):
+    S._check(S.lib.fdga_sde(S.h, _strategy(strategy), int(include_Hartree)))
+
+
+def iterate_solver(S, strategy="fdPA", update_Σ=True):
+    """iterate_solver!(S; strategy, update_Σ) (src/solve.jl:4-116)."""
+    S._check(S.lib.fdga_iterate(S.h, _strategy(strategy), int(update_Σ)))
+
+
+def flatten(S):
+    n = S.lib.fdga_flat_size(S.h)
+    x = np.empty(n, dtype=np.complex128)
+    S._check(S.lib.fdga_flatten(S.h, _dp(x.view(np.float64))))
+    return x
+
+
+def unflatten(S, x):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    S._check(S.lib.fdga_unflatten(S.h, _dp(x.view(np.float64))))
+
+
+def fixed_point(R, x, S, strategy="fdPA"):
+    """fixed_point!(R, x, S) (src/solve.jl:119-157): R = G(x) - x."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    S._check(S.lib.fdga_fixed_point(S.h, _dp(R.view(np.float64)), _dp(x.view(np.float64)), _strategy(strategy)))
+    return R
+
+
+def solve(S, *, strategy="fdPA", maxiter=100, tol=1e-4, δ=0.85, mem=8, verbose=False):
+    """solve!(S; maxiter, tol, δ, mem) (src/solve.jl:160-196): Anderson acceleration as
+    NLsolve 4.5.1 method=:anderson (beta = δ, m = mem, ftol = tol, xtol = 0, aa_start = 1).
+    The iterate is x_{k+1} = g_k - Σ_i γ_i Δg_i with g = x + δ f(x) and
+    γ = argmin ||f_k - ΔF γ||; converged when max|f| < tol.  Returns (iterations, residual)."""
+    strategy = _strategy(strategy)
+    x = flatten(S)
+    f = np.empty_like(x)
+    dG, dF = [], []
+    fold = gold = None
+    res = np.inf
+    it = 0
+    for it in range(1, maxiter + 1):
+        fixed_point(f, x, S, strategy)
+        g = x + δ * f
+        res = float(np.max(np.abs(f)))
+        if verbose:
+            print(f"iter {it:4d}  max|f| = {res:.3e}")
+        if res < tol:
+            break
+        x = g
+        if mem == 0:
+            continue
+        if it == 1:
+            fold, gold = f.copy(), g.copy()
+            continue
+        dG.append(g - gold)
+        dF.append(f - fold)
+        gold, fold = g.copy(), f.copy()
+        if len(dG) > min(mem, x.size):
+            dG.pop(0)
+            dF.pop(0)
+        Fm = np.stack(dF, axis=1)
+        Q, R = np.linalg.qr(Fm)
+        gam = np.linalg.solve(R, Q.conj().T @ f)
+        for i in range(len(dG)):
+            x = x - gam[i] * dG[i]
+    return it, res

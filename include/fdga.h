@@ -1,0 +1,103 @@
+/*
+ * fdga.h -- C-ABI of the MI355X (gfx950) Bethe-Salpeter hot path of fdDGAsolver.jl.
+ *
+ * The reference has no FFI; its seam is Julia multiple dispatch on the solver and the
+ * `mode` symbol (src/solve.jl:173-175).  Each entry point below replaces one reference
+ * function for a solver whose state lives on the device; a Julia method for `S.mode === :hip`
+ * binds it with `ccall`.  Host arrays use the reference's MeshFunction
+ * storage (column-major, first index fastest, ComplexF64 = two interleaved doubles) and are
+ * borrowed only for the duration of a call.  Every function returns 0 on success and a
+ * negative code on error; fdga_last_error() returns the message.  A context is not
+ * re-entrant; separate contexts may be used from separate threads.
+ */
+#ifndef FDGA_H
+#define FDGA_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdga_ctx fdga_ctx;
+
+/* Solver dimensions: the constructor arguments of ParquetSolver (src/ParquetSolver.jl:81-156)
+ * and NL_ParquetSolver (src/nonlocal/ParquetSolver.jl:77-152). */
+typedef struct fdga_dims {
+    int32_t nonlocal;   /* 0 ParquetSolver (local), 1 NL_ParquetSolver (s-wave, momentum P) */
+    int32_t nG;         /* fermionic mesh of G, Σ (MatsubaraMesh(T, nG, Fermion)) */
+    int32_t nK1, nK2b, nK2f, nK3b, nK3f;
+    int32_t mPinu;      /* mΠν_factor: bubble ν-mesh = nK1 * mPinu */
+    int32_t LG, L;      /* k-mesh of G/Σ and of the vertex/bubble (L x L); 1, 1 when local */
+    double  T;
+    double  U;          /* bare interaction of the innermost RefVertex */
+    int32_t ref_kind;   /* F0 = RefVertex (0) or local Vertex around the RefVertex (1) */
+    int32_t r_nK1, r_nK2b, r_nK2f, r_nK3b, r_nK3f;  /* boxes of the local reference Vertex */
+    int32_t tab_nb, tab_nf;                         /* RefVertex table box; (1, 1) = null tables */
+    int32_t fdpa_h0;    /* 1: fdPA SDE subtracts (n0-1/2) U i after Σ0 (src/SDE.jl:26-31) */
+    int32_t precision;  /* FDGA_FP64 */
+} fdga_dims;
+
+enum { FDGA_FP64 = 0 };
+enum { FDGA_PCH = 0, FDGA_TCH = 1, FDGA_ACH = 2 };
+enum { FDGA_SCPA = 0, FDGA_FDPA = 1 };
+
+/* array identifiers for fdga_set / fdga_get (reference layouts) */
+enum {
+    FDGA_GBARE = 0, FDGA_G0, FDGA_SIGMA0, FDGA_G, FDGA_SIGMA,   /* [NG, NkG]        S.Gbare S.G0 S.Σ0 S.G S.Σ */
+    FDGA_PI0PP, FDGA_PI0PH, FDGA_PIPP, FDGA_PIPH,               /* [NΩ1, NνΠ, NP]   S.Π0pp S.Π0ph S.Πpp S.Πph */
+    FDGA_LPP, FDGA_LPH,                                         /* [NΩ2, Nν2, NP]   S.Lpp S.Lph (get only) */
+    FDGA_TAB_FPP, FDGA_TAB_FPX, FDGA_TAB_FTP, FDGA_TAB_FTX,     /* RefVertex tables [tb, tf, tf] */
+    FDGA_CACHE0 = 32,   /* + c: S.cache_Γpx F0p F0a F0t Γpp Γa Γt Fp Fa Ft [NΩ3,Nν3,Nν3,NP] (get only) */
+    FDGA_VERTEX0 = 64   /* + 16*v + 3*ch + (cls-1); v: 0 S.F, 1 S.Fbuff, 2 S.FL, 3 S.F0 (local Vertex) */
+};
+#define FDGA_VERTEX(v, ch, cls) (FDGA_VERTEX0 + 16 * (v) + 3 * (ch) + ((cls) - 1))
+
+/* lifetime (ParquetSolver constructor, src/ParquetSolver.jl:81-156) */
+int         fdga_create(fdga_ctx** ctx, const fdga_dims* dims);
+int         fdga_destroy(fdga_ctx* ctx);
+const char* fdga_last_error(const fdga_ctx* ctx);
+int64_t     fdga_size(const fdga_ctx* ctx, int32_t which);        /* complex elements, -1 if unknown */
+int         fdga_set(fdga_ctx* ctx, int32_t which, const double* host);
+int         fdga_get(fdga_ctx* ctx, int32_t which, double* host);
+/* Π0 = bubbles(G0); Σ = Σ0; G = Dyson(Σ, Gbare); Π = bubbles(G)   (src/ParquetSolver.jl:97-114) */
+int         fdga_init(fdga_ctx* ctx);
+
+/* stages of iterate_solver! (src/solve.jl:4-116) */
+int fdga_dyson(fdga_ctx* ctx);                          /* Dyson!(S)          src/dyson.jl:1-31 */
+int fdga_bubbles(fdga_ctx* ctx);                        /* bubbles!(S)        src/bubble.jl:1-36, src/nonlocal/bubble.jl:87-158 */
+int fdga_build_k3_cache(fdga_ctx* ctx);                 /* build_K3_cache!(S) src/build_K3_cache.jl:20-95 */
+int fdga_bse_l_k2(fdga_ctx* ctx, int32_t ch);           /* BSE_L_K2!(S, Ch)   src/BSE_templates.jl:40-71 */
+int fdga_bse_l_k3(fdga_ctx* ctx, int32_t ch);           /* BSE_L_K3!(S, Ch)   src/BSE_templates.jl:116-145 */
+int fdga_bse_k1(fdga_ctx* ctx, int32_t ch);             /* BSE_K1!(S, Ch)     src/BSE_templates.jl:12-35 */
+int fdga_bse_k2(fdga_ctx* ctx, int32_t ch);             /* BSE_K2!(S, Ch)     src/BSE_templates.jl:77-110 */
+int fdga_bse_k3(fdga_ctx* ctx, int32_t ch);             /* BSE_K3!(S, Ch)     src/BSE_templates.jl:151-180 */
+int fdga_update_F(fdga_ctx* ctx);                       /* set!(S.F, S.Fbuff) src/solve.jl:87 */
+int fdga_sde(fdga_ctx* ctx, int32_t strategy, int32_t include_hartree);   /* SDE!(S) src/SDE.jl:3-33 */
+int fdga_iterate(fdga_ctx* ctx, int32_t strategy, int32_t update_sigma);  /* iterate_solver! src/solve.jl:4-116 */
+
+/* fixed-point layer: x = [γp(K1,K2,K3); γt; γa; Σ] (src/ParquetSolver.jl:277-306) */
+int64_t fdga_flat_size(const fdga_ctx* ctx);
+int     fdga_flatten(fdga_ctx* ctx, double* x);
+int     fdga_unflatten(fdga_ctx* ctx, const double* x);
+int     fdga_fixed_point(fdga_ctx* ctx, double* R, const double* x, int32_t strategy);  /* src/solve.jl:119-157 */
+
+/* synchronisation and timing on the context's stream */
+int fdga_sync(fdga_ctx* ctx);
+/* runs `n` iterate_solver! passes and returns the elapsed milliseconds measured with HIP events */
+int fdga_time_iterations(fdga_ctx* ctx, int32_t strategy, int32_t n, double* ms);
+/* per-stage device time of the last fdga_time_iterations call (ms, summed over its passes);
+ * stage: 0 dyson+bubbles, 1 cache, 2 L_K2, 3 L_K3, 4 K1, 5 K2, 6 K3, 7 update, 8 SDE */
+int fdga_stage_ms(const fdga_ctx* ctx, int32_t stage, double* ms);
+/* enable (1) / disable (0) per-stage event timing inside fdga_iterate (adds a host sync per
+ * stage); resets the accumulated stage times */
+int fdga_set_stage_timing(fdga_ctx* ctx, int32_t on);
+
+/* Self-test of the batched complex GEMM engine used by BSE_L_K3!/BSE_K3!: C = A B for dense
+ * column-major M x K and K x N ComplexF64 matrices.  No counterpart in the reference; it exists
+ * so that the library's tests can check the MFMA tiling directly. */
+int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, const double* A, const double* B, double* C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
