@@ -1,0 +1,128 @@
+#!/usr/bin/env python3
+"""Benchmark: parquet iterations/s and BSE TFLOP/s of the FP64 GPU path on one MI355X.
+
+Workload (one "step" = one iterate_solver! pass, src/solve.jl:4-116, strategy :fdPA):
+SIAM with G0(iν) = 1/(iν + iΔ sign ν), Δ = π/5, U = 2, β = 20 (T = 0.05), nK1 = 128,
+nK2 = (64, 64), nK3 = (32, 64), mΠν_factor = 6, nG = 256.  The solver is first brought to
+its fixed point with the Anderson loop (untimed), then `warmup` passes run and `steps` passes
+are timed with HIP events on the solver's stream, all inputs resident in HBM.
+
+Prints one JSON line.  `roofline` is for the BSE contraction kernel (BSE_K3! + BSE_L_K3!,
+batched complex GEMM on FP64 MFMA): algorithmic flops per launch 8 NΩ3 Nν3^3 (one channel, one
+GEMM) over its measured average launch time.  `cpu_baseline` times the CPU restatement
+(oracle/, one thread) on one iteration of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fddgasolver.jl_amd"))
+
+WORKLOAD = dict(T=0.05, U=2.0, Delta=3.141592653589793 / 5, nK1=128, nK2=(64, 64), nK3=(32, 64), mPinu=6, nG=256)
+FP64_MFMA_PEAK = 78.6e12
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flops_bse_k3(w, NP=1):
+    NW3 = 2 * w["nK3"][0] - 1
+    NV3 = 2 * w["nK3"][1]
+    return 8.0 * NW3 * NP * NV3 ** 3   # one channel, one GEMM
+
+
+def build_solver(w):
+    import fdga
+    S = fdga.parquet_solver_siam_parquet_approximation(w["nG"], w["nK1"], w["nK2"], w["nK3"], e=0.0, Δ=w["Delta"],
+                                                       D=None, T=w["T"], U=w["U"], mΠν_factor=w["mPinu"])
+    return S
+
+
+def cpu_baseline():
+    """The CPU restatement of the reference loops (oracle/, one thread) timed on one fdPA
+    iteration of the same workload (full box), from the solver's initial state."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    w = WORKLOAD
+    o = po.Oracle(nG=w["nG"], nK1=w["nK1"], nK2=w["nK2"], nK3=w["nK3"], mPinu=w["mPinu"], T=w["T"], U=w["U"])
+    o.siam(0.0, w["Delta"], None)
+    o.init()
+    t0 = time.perf_counter()
+    o.iterate(po.FDPA)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "iters/s", "cores": 1, "kind": "port",
+            "sample": f"one fdPA iterate_solver! pass of the full workload box on the CPU restatement "
+                      f"(oracle/fdga_oracle.cpp, 1 thread): {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: the parquet hot path is run on a single GPU in this build "
+              "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import fdga
+    w = WORKLOAD
+    log(f"bench: building SIAM fdPA solver nK1={w['nK1']} nK2={w['nK2']} nK3={w['nK3']} nG={w['nG']}")
+    S = build_solver(w)
+    t0 = time.perf_counter()
+    it, res = fdga.solve(S, strategy="fdPA", tol=1e-8, maxiter=80)
+    log(f"bench: fixed point reached in {it} Anderson iterations (max|f| = {res:.2e}, {time.perf_counter() - t0:.1f} s)")
+    for _ in range(args.warmup):
+        fdga.iterate_solver(S, "fdPA")
+    S.sync()
+    ms = S.time_iterations("fdPA", args.steps)
+    ms_step = ms / args.steps
+    log(f"bench: {args.steps} iterations in {ms:.2f} ms ({ms_step:.3f} ms/iteration)")
+    # per-stage breakdown (separate pass with per-stage events)
+    nst = max(1, min(args.steps, 5))
+    S.set_stage_timing(True)
+    for _ in range(nst):
+        fdga.iterate_solver(S, "fdPA")
+    S.sync()
+    st = [x / nst for x in S.stage_ms()]
+    S.set_stage_timing(False)
+    names = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE"]
+    log("bench: stage ms/iteration: " + ", ".join(f"{n}={v:.3f}" for n, v in zip(names, st)))
+    f_gemm = flops_bse_k3(w)
+    t_gemm = (st[3] + st[6]) / 6.0 * 1e-3   # 6 GEMM launches per fdPA iteration
+    achieved = f_gemm / t_gemm
+    bse_tflops = 6 * f_gemm / ((st[3] + st[6]) * 1e-3) / 1e12
+    out = {
+        "metric": "parquet iters/s + BSE TFLOP/s, FP32-MFMA vs FP64 GPU @ rel.err<=1e-6 vs FP64",
+        "value": 1000.0 / ms_step,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SIAM model inputs; solver state at its fdPA fixed point)",
+        "config": {"workload": "local SIAM fdPA, U=2, beta=20, nK1=128, nK2=(64,64), nK3=(32,64), mPinu=6, nG=256",
+                   "precision_path": "plain FP64", "bse_tflops": bse_tflops,
+                   "stage_ms": dict(zip(names, st))},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK, "traffic": None,
+                     "kernel": "zgemm_tiles (BSE_K3!/BSE_L_K3! contraction)"},
+    }
+    if not args.no_cpu_baseline:
+        log("bench: timing the CPU baseline (oracle, one thread)")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
