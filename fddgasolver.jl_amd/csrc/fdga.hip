@@ -242,6 +242,14 @@ double2* pi_of(fdga_ctx* c, int ch, bool zero) {
     return ch == PCH ? c->Pipp : c->Piph;
 }
 
+// bound of every box of F, F0 and the RefVertex tables (frequency window of the gather-GEMVs)
+int box_bound(const fdga_ctx* c) {
+    const fdga_dims& d = c->d;
+    int nk = std::max({d.nK1, d.nK2b, d.nK2f, d.nK3b, d.nK3f, d.tab_nb, d.tab_nf});
+    if (d.ref_kind == 1) nk = std::max({nk, d.r_nK1, d.r_nK2b, d.r_nK2f, d.r_nK3b, d.r_nK3f});
+    return nk + 1;
+}
+
 // ---------------------------------------------------------------------------------------
 // SDE levels (src/SDE.jl:35-48): F, F.F0, F.F0.F0 ... accumulated into S
 struct Level { int kind; ChanV g[3]; int chanNP; double scale; };
@@ -307,7 +315,7 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
     const size_t nout = (size_t)c->NW2 * c->NV2 * c->NP;
     RefV ref = ref_view(c, false);
     k_sde_L<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref, lv.kind,
-                                                       d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, lv.chanNP, cscale(c->U, d.T));
+                                                       d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, lv.chanNP, cscale(c->U, d.T), box_bound(c));
     LAUNCHCHK(c);
     return 0;
 }
@@ -360,6 +368,45 @@ int idx_f0(int ch) { return ch == PCH ? 1 : ch == ACH ? 2 : 3; }
 int idx_fbase(int ch) { return ch == PCH ? 7 : ch == ACH ? 8 : 9; }
 
 }  // namespace
+
+template <int CH>
+static int launch_lk2(fdga_ctx* c) {
+    const size_t nout = c->n2 * c->NP;
+    k_gemv_lk2<CH><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->FL[CH].K2, c->FL[ACH].K2, pi_of(c, CH, true), vtx_F(c),
+                                                              c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP, c->d.nonlocal,
+                                                              c->d.T * kSign[CH], box_bound(c));
+    LAUNCHCHK(c);
+    return 0;
+}
+
+template <int CH>
+static int launch_w(fdga_ctx* c) {
+    dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
+    k_bse_w<CH><<<gw, 256, 0, c->st>>>(c->W, pi_of(c, CH, false), pi_of(c, CH, true), vtx_FL(c), f0_view(c), c->d.nK1, c->nVP);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+template <int CH>
+static int launch_k1(fdga_ctx* c) {
+    if (launch_w<CH>(c)) return -3;
+    const size_t nout = c->n1 * c->NP;
+    k_gemv_k1<CH><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->W, vtx_F(c), c->d.nK1,
+                                                             c->d.nK1, c->nVP, c->NP, c->d.T * kSign[CH], box_bound(c));
+    LAUNCHCHK(c);
+    return 0;
+}
+
+template <int CH>
+static int launch_k2(fdga_ctx* c) {
+    if (launch_w<CH>(c)) return -3;
+    const size_t nout = c->n2 * c->NP;
+    k_gemv_k2<CH><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->W, vtx_F(c), chanF(c, c->FL[CH]),
+                                                             chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
+                                                             c->NP, c->d.T * kSign[CH], box_bound(c));
+    LAUNCHCHK(c);
+    return 0;
+}
 
 // =========================================================================================
 extern "C" {
@@ -613,14 +660,12 @@ int fdga_build_k3_cache(fdga_ctx* c) {
     return 0;
 }
 
+
 int fdga_bse_l_k2(fdga_ctx* c, int32_t ch) {
-    if (ch < 0 || ch > 2) return fail(c, "bad channel");
-    const size_t nout = c->n2 * c->NP;
-    k_gemv_lk2<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->FL[ch].K2, c->FL[ACH].K2, pi_of(c, ch, true), vtx_F(c),
-                                                          c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP, ch, kSp[ch],
-                                                          c->d.nonlocal, c->d.T * kSign[ch], ch == TCH);
-    LAUNCHCHK(c);
-    return 0;
+    if (ch == PCH) return launch_lk2<PCH>(c);
+    if (ch == TCH) return launch_lk2<TCH>(c);
+    if (ch == ACH) return launch_lk2<ACH>(c);
+    return fail(c, "bad channel");
 }
 
 int fdga_bse_l_k3(fdga_ctx* c, int32_t ch) {
@@ -637,29 +682,21 @@ int fdga_bse_l_k3(fdga_ctx* c, int32_t ch) {
     return 0;
 }
 
+
+
+
 int fdga_bse_k1(fdga_ctx* c, int32_t ch) {
-    if (ch < 0 || ch > 2) return fail(c, "bad channel");
-    dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
-    k_bse_w<<<gw, 256, 0, c->st>>>(c->W, pi_of(c, ch, false), pi_of(c, ch, true), vtx_FL(c), f0_view(c), c->d.nK1, c->nVP,
-                                   ch, kSp[ch]);
-    const size_t nout = c->n1 * c->NP;
-    k_gemv_k1<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[ch].K1, c->Fb[ACH].K1, c->W, vtx_F(c), c->d.nK1, c->d.nK1,
-                                                         c->nVP, c->NP, ch, kSp[ch], c->d.T * kSign[ch], ch == TCH);
-    LAUNCHCHK(c);
-    return 0;
+    if (ch == PCH) return launch_k1<PCH>(c);
+    if (ch == TCH) return launch_k1<TCH>(c);
+    if (ch == ACH) return launch_k1<ACH>(c);
+    return fail(c, "bad channel");
 }
 
 int fdga_bse_k2(fdga_ctx* c, int32_t ch) {
-    if (ch < 0 || ch > 2) return fail(c, "bad channel");
-    dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
-    k_bse_w<<<gw, 256, 0, c->st>>>(c->W, pi_of(c, ch, false), pi_of(c, ch, true), vtx_FL(c), f0_view(c), c->d.nK1, c->nVP,
-                                   ch, kSp[ch]);
-    const size_t nout = c->n2 * c->NP;
-    k_gemv_k2<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[ch].K2, c->Fb[ACH].K2, c->W, vtx_F(c), chanF(c, c->FL[ch]),
-                                                         chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
-                                                         c->NP, ch, kSp[ch], c->d.T * kSign[ch]);
-    LAUNCHCHK(c);
-    return 0;
+    if (ch == PCH) return launch_k2<PCH>(c);
+    if (ch == TCH) return launch_k2<TCH>(c);
+    if (ch == ACH) return launch_k2<ACH>(c);
+    return fail(c, "bad channel");
 }
 
 int fdga_bse_k3(fdga_ctx* c, int32_t ch) {

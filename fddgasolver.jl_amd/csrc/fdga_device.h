@@ -25,30 +25,30 @@ enum { FL_F0 = 1, FL_GP = 2, FL_GT = 4, FL_GA = 8, FL_ALL = 15 };
 
 // ---------------------------------------------------------------------------------------
 // complex FP64 helpers on double2 (x = re, y = im)
-__host__ __device__ inline double2 cmk(double r, double i) { double2 z; z.x = r; z.y = i; return z; }
-__host__ __device__ inline double2 cadd(double2 a, double2 b) { return cmk(a.x + b.x, a.y + b.y); }
-__host__ __device__ inline double2 csub(double2 a, double2 b) { return cmk(a.x - b.x, a.y - b.y); }
-__host__ __device__ inline double2 cmul(double2 a, double2 b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-__host__ __device__ inline double2 cscale(double2 a, double s) { return cmk(a.x * s, a.y * s); }
-__host__ __device__ inline double2 cneg(double2 a) { return cmk(-a.x, -a.y); }
-__host__ __device__ inline double2 czero() { return cmk(0.0, 0.0); }
-__host__ __device__ inline double2 cinv(double2 a) {
+__host__ __device__ __forceinline__ double2 cmk(double r, double i) { double2 z; z.x = r; z.y = i; return z; }
+__host__ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return cmk(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return cmk(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ double2 cmul(double2 a, double2 b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ __forceinline__ double2 cscale(double2 a, double s) { return cmk(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ double2 cneg(double2 a) { return cmk(-a.x, -a.y); }
+__host__ __device__ __forceinline__ double2 czero() { return cmk(0.0, 0.0); }
+__host__ __device__ __forceinline__ double2 cinv(double2 a) {
     double d = a.x * a.x + a.y * a.y;
     return cmk(a.x / d, -a.y / d);
 }
 
 // ---------------------------------------------------------------------------------------
 // Matsubara index arithmetic; νInf absorbs
-__host__ __device__ inline bool in_b(int m, int N) { return m != FDGA_INF && m >= -(N - 1) && m <= N - 1; }
-__host__ __device__ inline bool in_f(int n, int N) { return n != FDGA_INF && n >= -N && n <= N - 1; }
-__host__ __device__ inline int bpf(int m, int n) { return (m == FDGA_INF || n == FDGA_INF) ? FDGA_INF : m + n; }      // Ω + ν
-__host__ __device__ inline int bmf(int m, int n) { return (m == FDGA_INF || n == FDGA_INF) ? FDGA_INF : m - n - 1; }  // Ω - ν
-__host__ __device__ inline int fmf(int a, int b) { return (a == FDGA_INF || b == FDGA_INF) ? FDGA_INF : a - b; }      // ν - ν'
-__host__ __device__ inline int fpf(int a, int b) { return (a == FDGA_INF || b == FDGA_INF) ? FDGA_INF : a + b + 1; }  // ν + ν'
-__host__ __device__ inline int crossing(int m, int w, int ch) { return ch == PCH ? bmf(m, w) : w; }  // src/BSE_templates.jl:4-6
+__host__ __device__ __forceinline__ bool in_b(int m, int N) { return m != FDGA_INF && m >= -(N - 1) && m <= N - 1; }
+__host__ __device__ __forceinline__ bool in_f(int n, int N) { return n != FDGA_INF && n >= -N && n <= N - 1; }
+__host__ __device__ __forceinline__ int bpf(int m, int n) { return (m == FDGA_INF || n == FDGA_INF) ? FDGA_INF : m + n; }      // Ω + ν
+__host__ __device__ __forceinline__ int bmf(int m, int n) { return (m == FDGA_INF || n == FDGA_INF) ? FDGA_INF : m - n - 1; }  // Ω - ν
+__host__ __device__ __forceinline__ int fmf(int a, int b) { return (a == FDGA_INF || b == FDGA_INF) ? FDGA_INF : a - b; }      // ν - ν'
+__host__ __device__ __forceinline__ int fpf(int a, int b) { return (a == FDGA_INF || b == FDGA_INF) ? FDGA_INF : a + b + 1; }  // ν + ν'
+__host__ __device__ __forceinline__ int crossing(int m, int w, int ch) { return ch == PCH ? bmf(m, w) : w; }  // src/BSE_templates.jl:4-6
 
 // _convert_channel (src/convention.jl:4-37)
-__host__ __device__ inline void convert(int from, int to, int A, int b, int bp, int& o0, int& o1, int& o2) {
+__host__ __device__ __forceinline__ void convert(int from, int to, int A, int b, int bp, int& o0, int& o1, int& o2) {
     if (from == to) { o0 = A; o1 = b; o2 = bp; return; }
     if (from == PCH) {
         if (to == TCH) { o0 = fmf(bmf(A, b), bp); o1 = bp; o2 = b; return; }
@@ -74,20 +74,20 @@ struct ChanV {
     int n1, n2b, n2f, n3b, n3f;
     int NP;
 
-    __host__ __device__ size_t s1() const { return (size_t)(2 * n1 - 1); }
-    __host__ __device__ size_t s2() const { return (size_t)(2 * n2b - 1) * (2 * n2f); }
-    __host__ __device__ size_t s3() const { return (size_t)(2 * n3f) * (2 * n3f) * (2 * n3b - 1); }
-    __host__ __device__ size_t i1(int m) const { return (size_t)(m + n1 - 1); }
-    __host__ __device__ size_t i2(int m, int n) const { return (size_t)(m + n2b - 1) + (size_t)(2 * n2b - 1) * (n + n2f); }
-    __host__ __device__ size_t i3(int m, int n, int np) const {
+    __host__ __device__ __forceinline__ size_t s1() const { return (size_t)(2 * n1 - 1); }
+    __host__ __device__ __forceinline__ size_t s2() const { return (size_t)(2 * n2b - 1) * (2 * n2f); }
+    __host__ __device__ __forceinline__ size_t s3() const { return (size_t)(2 * n3f) * (2 * n3f) * (2 * n3b - 1); }
+    __host__ __device__ __forceinline__ size_t i1(int m) const { return (size_t)(m + n1 - 1); }
+    __host__ __device__ __forceinline__ size_t i2(int m, int n) const { return (size_t)(m + n2b - 1) + (size_t)(2 * n2b - 1) * (n + n2f); }
+    __host__ __device__ __forceinline__ size_t i3(int m, int n, int np) const {
         return (size_t)(n + n3f) + (size_t)(2 * n3f) * ((size_t)(np + n3f) + (size_t)(2 * n3f) * (m + n3b - 1));
     }
-    __device__ double2 k1(int m, int p) const { return p < 0 ? A1[i1(m)] : K1[i1(m) + s1() * p]; }
-    __device__ double2 k2(int m, int n, int p) const { return p < 0 ? A2[i2(m, n)] : K2[i2(m, n) + s2() * p]; }
-    __device__ double2 k3(int m, int n, int np, int p) const { return p < 0 ? A3[i3(m, n, np)] : K3[i3(m, n, np) + s3() * p]; }
+    __device__ __forceinline__ double2 k1(int m, int p) const { return p < 0 ? A1[i1(m)] : K1[i1(m) + s1() * p]; }
+    __device__ __forceinline__ double2 k2(int m, int n, int p) const { return p < 0 ? A2[i2(m, n)] : K2[i2(m, n) + s2() * p]; }
+    __device__ __forceinline__ double2 k3(int m, int n, int np, int p) const { return p < 0 ? A3[i3(m, n, np)] : K3[i3(m, n, np) + s3() * p]; }
 
     // γ(Ω, ν, νp[, P]) with all classes (src/channel.jl:220-339; src/nonlocal/channel.jl:90-226)
-    __device__ double2 eval(int m, int n, int np, int p) const {
+    __device__ __forceinline__ double2 eval(int m, int n, int np, int p) const {
         double2 v = czero();
         if (!in_b(m, n1)) return v;
         v = k1(m, p);
@@ -118,23 +118,23 @@ struct RefV {
     int tb, tf;                   // table box (tb == 0: no tables)
     const double2* tab[4];        // Fp_p, Fp_x, Ft_p, Ft_x  [Ω, ν, νp]
 
-    __device__ double2 t(int k, int m, int n, int np) const {
+    __device__ __forceinline__ double2 t(int k, int m, int n, int np) const {
         if (tb == 0 || !(in_b(m, tb) && in_f(n, tf) && in_f(np, tf))) return czero();
         size_t i = (size_t)(m + tb - 1) + (size_t)(2 * tb - 1) * ((size_t)(n + tf) + (size_t)(2 * tf) * (np + tf));
         return tab[k][i];
     }
-    __device__ double2 bare(int sp) const { return sp == XSP ? cneg(U) : U; }
-    __device__ double2 eval_p(int m, int n, int np, int ch) const {  // pSp, finite
+    __device__ __forceinline__ double2 bare(int sp) const { return sp == XSP ? cneg(U) : U; }
+    __device__ __forceinline__ double2 eval_p(int m, int n, int np, int ch) const {  // pSp, finite
         if (ch == PCH) return cadd(t(0, m, n, np), U);
         if (ch == TCH) return cadd(t(2, m, n, np), U);
         return cadd(cneg(t(3, m, np, n)), U);
     }
-    __device__ double2 eval_x(int m, int n, int np, int ch) const {  // xSp, finite
+    __device__ __forceinline__ double2 eval_x(int m, int n, int np, int ch) const {  // xSp, finite
         if (ch == PCH) return csub(t(1, m, n, np), U);
         if (ch == TCH) return csub(t(3, m, n, np), U);
         return csub(cneg(t(2, m, np, n)), U);
     }
-    __device__ double2 eval(int m, int n, int np, int ch, int sp) const {
+    __device__ __forceinline__ double2 eval(int m, int n, int np, int ch, int sp) const {
         if (n == FDGA_INF || np == FDGA_INF) return bare(sp);
         if (sp == PSP) return eval_p(m, n, np, ch);
         if (sp == XSP) return eval_x(m, n, np, ch);
@@ -142,20 +142,25 @@ struct RefV {
     }
 };
 
+// channel selection by a run-time index without indexing the descriptor array dynamically
+// (a dynamic index would move the whole kernel-argument descriptor to scratch memory)
+__device__ __forceinline__ double2 chan_eval(const ChanV* g, int r, int m, int n, int np, int p) {
+    if (r == PCH) return g[0].eval(m, n, np, p);
+    if (r == TCH) return g[1].eval(m, n, np, p);
+    return g[2].eval(m, n, np, p);
+}
+
 // γ part of the parallel-spin Vertex evaluator (src/vertex.jl:209-284; NL s-wave
 // src/nonlocal/vertex.jl:216-379): same channel at momentum p, other channels averaged
-__device__ inline double2 gamma_psp(const ChanV* g, int m, int n, int np, int p, int ch, int fl) {
+__device__ __forceinline__ double2 gamma_psp(const ChanV* g, int m, int n, int np, int p, int ch, int fl) {
     double2 v = czero();
     if (n != FDGA_INF && np != FDGA_INF) {
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            if (!(fl & (FL_GP << r))) continue;
-            int a, b, c;
-            convert(ch, r, m, n, np, a, b, c);
-            v = cadd(v, g[r].eval(a, b, c, r == ch ? p : -1));
-        }
+        int a, b, c;
+        if (fl & FL_GP) { convert(ch, PCH, m, n, np, a, b, c); v = cadd(v, g[0].eval(a, b, c, ch == PCH ? p : -1)); }
+        if (fl & FL_GT) { convert(ch, TCH, m, n, np, a, b, c); v = cadd(v, g[1].eval(a, b, c, ch == TCH ? p : -1)); }
+        if (fl & FL_GA) { convert(ch, ACH, m, n, np, a, b, c); v = cadd(v, g[2].eval(a, b, c, ch == ACH ? p : -1)); }
     } else if (fl & (FL_GP << ch)) {
-        v = cadd(v, g[ch].eval(m, n, np, p));
+        v = cadd(v, chan_eval(g, ch, m, n, np, p));
     }
     return v;
 }
@@ -165,13 +170,13 @@ struct F0V {
     int kind;
     RefV ref;
     ChanV g[3];
-    __device__ double2 eval_psp(int m, int n, int np, int ch) const {
+    __device__ __forceinline__ double2 eval_psp(int m, int n, int np, int ch) const {
         double2 v = ref.eval(m, n, np, ch, PSP);
         if (kind == 1) v = cadd(v, gamma_psp(g, m, n, np, 0, ch, FL_GP | FL_GT | FL_GA));
         return v;
     }
     // full evaluator with spin (src/vertex.jl:288-336 for kind 1, src/refvertex.jl for kind 0)
-    __device__ double2 eval(int m, int n, int np, int ch, int sp) const {
+    __device__ __forceinline__ double2 eval(int m, int n, int np, int ch, int sp) const {
         if (kind == 0) return ref.eval(m, n, np, ch, sp);
         if (sp == PSP) return eval_psp(m, n, np, ch);
         double2 x;
@@ -187,21 +192,21 @@ struct F0V {
 struct VtxV {
     ChanV g[3];
     F0V f0;
-    __device__ double2 eval_psp(int m, int n, int np, int p, int ch, int fl) const {
+    __device__ __forceinline__ double2 eval_psp(int m, int n, int np, int p, int ch, int fl) const {
         double2 v = czero();
         if (fl & FL_F0) v = f0.eval_psp(m, n, np, ch);
         return cadd(v, gamma_psp(g, m, n, np, p, ch, fl));
     }
-    __device__ static int swap_ta(int fl) {
+    __device__ __forceinline__ static int swap_ta(int fl) {
         return (fl & (FL_F0 | FL_GP)) | ((fl & FL_GT) ? FL_GA : 0) | ((fl & FL_GA) ? FL_GT : 0);
     }
-    __device__ double2 eval_xsp(int m, int n, int np, int p, int ch, int fl) const {
+    __device__ __forceinline__ double2 eval_xsp(int m, int n, int np, int p, int ch, int fl) const {
         const int f2 = swap_ta(fl);
         if (ch == PCH) return cneg(eval_psp(m, n, bmf(m, np), p, PCH, f2));
         if (ch == TCH) return cneg(eval_psp(m, n, np, p, ACH, f2));
         return cneg(eval_psp(m, n, np, p, TCH, f2));
     }
-    __device__ double2 eval(int m, int n, int np, int p, int ch, int sp, int fl) const {
+    __device__ __forceinline__ double2 eval(int m, int n, int np, int p, int ch, int sp, int fl) const {
         if (sp == PSP) return eval_psp(m, n, np, p, ch, fl);
         if (sp == XSP) return eval_xsp(m, n, np, p, ch, fl);
         double2 v = cscale(eval_psp(m, n, np, p, ch, fl), 2.0);

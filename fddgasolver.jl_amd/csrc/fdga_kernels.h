@@ -17,8 +17,12 @@ namespace fdga {
 
 constexpr int WAVE = 64;
 
+// spin component of the channel's BSE: p, a parallel spin; t density (src/BSE_templates.jl:12-35)
+template <int CH> struct ChSpin { static constexpr int SP = (CH == TCH) ? DSP : PSP; };
+
+
 // ---------------------------------------------------------------------------------------
-__device__ inline double wave_sum(double v) {
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
     return v;
@@ -398,8 +402,10 @@ struct Dense_C { double2* C; int ldc; __device__ void operator()(int b, int i, i
 // ---------------------------------------------------------------------------------------
 // BSE weight on the bubble mesh for K1/K2 (src/BSEa/BSEa_K1.jl:38-43, BSEa_K2.jl:77-85):
 //   W(ω, Ω, P) = (Π - Π0) F0(Ω, ω~, ∞, Ch, Sp) + Π FL(Ω, ω~, ∞, P, Ch, Sp),  ω~ = crossing(Ω, ω)
-__global__ void k_bse_w(double2* __restrict__ W, const double2* __restrict__ Pi, const double2* __restrict__ Pi0,
-                        VtxV FL, F0V f0, int nWP, int nVP, int ch, int sp) {
+template <int CH>
+__global__ __launch_bounds__(256) void k_bse_w(double2* __restrict__ W, const double2* __restrict__ Pi,
+                                               const double2* __restrict__ Pi0, VtxV FL, F0V f0, int nWP, int nVP) {
+    constexpr int ch = CH, sp = ChSpin<CH>::SP;
     int iw = blockIdx.x * blockDim.x + threadIdx.x;
     int iW = blockIdx.y, p = blockIdx.z;
     if (iw >= 2 * nVP) return;
@@ -411,56 +417,95 @@ __global__ void k_bse_w(double2* __restrict__ W, const double2* __restrict__ Pi,
     W[o] = cadd(cmul(csub(Pi[o], Pi0[o]), f0r), cmul(Pi[o], flr));
 }
 
+// Frequency window of the gather-GEMVs.  Every vertex term that depends on the summed
+// frequency ω is supported on |ω| <= Wc = NK + |m| + |n| + 2, where NK bounds every box of the
+// vertex (K1 boxes reach furthest: their bosonic argument is a sum or difference of Ω, ν, ω).
+// Beyond the window the summand is T(ω_far) W(ω) with T(ω_far) the same value for every ω on
+// that side, so Σ_far T W = T(ω_far) Σ_far W: only the weights are summed there.
+__device__ __forceinline__ int window(int NK, int m, int n) { return NK + abs(m) + abs(n) + 2; }
+
+// the two far-side summands (ω = -Wc-1 for lane 0, ω = Wc+1 for lane 1) are evaluated by one
+// inlined evaluation on every lane and gathered with shuffles
+__device__ __forceinline__ void far_pair(double2 t, double2& left, double2& right) {
+    left = cmk(__shfl(t.x, 0, WAVE), __shfl(t.y, 0, WAVE));
+    right = cmk(__shfl(t.x, 1, WAVE), __shfl(t.y, 1, WAVE));
+}
+
 // BSE_K1!: K1(Ω, P) = T s Σ_ω F(Ω, ∞, ω, P, Ch, Sp) W(ω)   (one wave per output)
-__global__ void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a, const double2* __restrict__ W,
-                          VtxV F, int n1, int nWP, int nVP, int NP, int ch, int sp, double scale, int tpost) {
+template <int CH>
+__global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a,
+                                                 const double2* __restrict__ W, VtxV F, int n1, int nWP, int nVP, int NP,
+                                                 double scale, int NK) {
+    constexpr int SP = ChSpin<CH>::SP;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW1 = 2 * n1 - 1;
     if (wid >= NW1 * NP) return;
     const int iW = wid % NW1, p = wid / NW1;
     const int m = iW - (n1 - 1);
+    const int Wc = window(NK, m, 0);
     const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
-    double sr = 0.0, si = 0.0;
-    for (int iw = lane; iw < 2 * nVP; iw += WAVE) {
-        const int w = iw - nVP;
-        double2 x = cmul(F.eval(m, FDGA_INF, w, p, ch, sp, FL_ALL), Wr[iw]);
-        sr += x.x;
-        si += x.y;
+    const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
+    double sr = 0.0, si = 0.0, lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+    for (int iw = lo + lane; iw <= hi; iw += WAVE) {
+        double2 x = cmul(F.eval(m, FDGA_INF, iw - nVP, p, CH, SP, FL_ALL), Wr[iw]);
+        sr += x.x; si += x.y;
     }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
+    for (int iw = lane; iw < lo; iw += WAVE) { lr += Wr[iw].x; li += Wr[iw].y; }
+    for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) { rr += Wr[iw].x; ri += Wr[iw].y; }
+    double2 tl, tr;
+    far_pair(F.eval(m, FDGA_INF, lane == 0 ? -Wc - 1 : Wc + 1, p, CH, SP, FL_ALL), tl, tr);
+    sr = wave_sum(sr); si = wave_sum(si);
+    lr = wave_sum(lr); li = wave_sum(li);
+    rr = wave_sum(rr); ri = wave_sum(ri);
     if (lane == 0) {
+        double2 acc = cmk(sr, si);
+        if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
+        if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
         size_t o = (size_t)iW + (size_t)NW1 * p;
-        double2 v = cscale(cmk(sr, si), scale);
-        if (tpost) v = cscale(cadd(v, K1a[o]), 0.5);
+        double2 v = cscale(acc, scale);
+        if (CH == TCH) v = cscale(cadd(v, K1a[o]), 0.5);
         K1[o] = v;
     }
 }
 
 // BSE_K2!: K2(Ω, ν, P) = T s Σ_ω [F(Ω,ν,ω) - F(Ω,∞,ω)] W(ω) + FL.K2 (t: 2 FLt - FLa)
-__global__ void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a, const double2* __restrict__ W,
-                          VtxV F, ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int nVP, int NP, int ch, int sp,
-                          double scale) {
+template <int CH>
+__device__ __forceinline__ double2 bse_k2_term(const VtxV& F, int m, int v, int w, int p) {
+    constexpr int SP = ChSpin<CH>::SP;
+    return csub(F.eval(m, v, w, p, CH, SP, FL_ALL), F.eval(m, FDGA_INF, w, p, CH, SP, FL_ALL));
+}
+template <int CH>
+__global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a,
+                                                 const double2* __restrict__ W, VtxV F, ChanV Lr, ChanV La, int n2b,
+                                                 int n2f, int nWP, int nVP, int NP, double scale, int NK) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= NW2 * NV2 * NP) return;
     const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
+    const int Wc = window(NK, m, v);
     const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
-    double sr = 0.0, si = 0.0;
-    for (int iw = lane; iw < 2 * nVP; iw += WAVE) {
-        const int w = iw - nVP;
-        double2 fl = csub(F.eval(m, v, w, p, ch, sp, FL_ALL), F.eval(m, FDGA_INF, w, p, ch, sp, FL_ALL));
-        double2 x = cmul(fl, Wr[iw]);
-        sr += x.x;
-        si += x.y;
+    // only the window is iterated for the summand; the far weights are summed separately
+    const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
+    double sr = 0.0, si = 0.0, lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+    for (int iw = lo + lane; iw <= hi; iw += WAVE) {
+        double2 x = cmul(bse_k2_term<CH>(F, m, v, iw - nVP, p), Wr[iw]);
+        sr += x.x; si += x.y;
     }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
+    for (int iw = lane; iw < lo; iw += WAVE) { lr += Wr[iw].x; li += Wr[iw].y; }
+    for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) { rr += Wr[iw].x; ri += Wr[iw].y; }
+    double2 tl, tr;
+    far_pair(bse_k2_term<CH>(F, m, v, lane == 0 ? -Wc - 1 : Wc + 1, p), tl, tr);
+    sr = wave_sum(sr); si = wave_sum(si);
+    lr = wave_sum(lr); li = wave_sum(li);
+    rr = wave_sum(rr); ri = wave_sum(ri);
     if (lane == 0) {
+        double2 acc = cmk(sr, si);
+        if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
+        if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
         const size_t o = (size_t)wid;
-        double2 val = cscale(cmk(sr, si), scale);
-        if (ch == TCH) {
+        double2 val = cscale(acc, scale);
+        if (CH == TCH) {
             val = cadd(val, cscale(Lr.K2[o], 2.0));
             val = cadd(val, cscale(La.K2[o], -1.0));
             val = cscale(cadd(val, K2a[o]), 0.5);
@@ -473,36 +518,41 @@ __global__ void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ 
 
 // BSE_L_K2!: local (src/BSEa/BSEa_K2.jl:1-40): Σ over the bubble mesh of
 //   Γ(Ω, ν, ω) Π0(ω) F0(Ω, ω~, ∞);  NL (src/nonlocal/BSEa/BSEa_K2.jl:1-41): Σ over the K2 mesh of
-//   Γ(Ω, ν, ω~) Π0(ω) F0(Ω, ω, ∞)
-__global__ void k_gemv_lk2(double2* __restrict__ out, const double2* __restrict__ outa, const double2* __restrict__ Pi0,
-                           VtxV F, int n2b, int n2f, int nWP, int nVP, int NP, int ch, int sp, int nonlocal,
-                           double scale, int tpost) {
+//   Γ(Ω, ν, ω~) Π0(ω) F0(Ω, ω, ∞).  Γ excludes F0 and the own channel, so it vanishes outside
+//   the frequency window.
+template <int CH>
+__global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, const double2* __restrict__ outa,
+                                                  const double2* __restrict__ Pi0, VtxV F, int n2b, int n2f, int nWP,
+                                                  int nVP, int NP, int nonlocal, double scale, int NK) {
+    constexpr int SP = ChSpin<CH>::SP;
+    constexpr int FLG = FL_ALL & ~FL_F0 & ~(FL_GP << CH);
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= NW2 * NV2 * NP) return;
     const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
-    const int fl = FL_ALL & ~FL_F0 & ~(FL_GP << ch);
     const double2* P0 = Pi0 + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
     double sr = 0.0, si = 0.0;
-    const int cnt = nonlocal ? NV2 : 2 * nVP;
-    for (int q = lane; q < cnt; q += WAVE) {
-        double2 x;
-        if (!nonlocal) {
+    if (!nonlocal) {
+        const int Wc = window(NK, m, v);
+        const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
+        for (int q = lo + lane; q <= hi; q += WAVE) {
             const int w = q - nVP;
-            x = cmul(cmul(F.eval(m, v, w, p, ch, sp, fl), P0[q]), F.f0.eval(m, crossing(m, w, ch), FDGA_INF, ch, sp));
-        } else {
-            const int w = q - n2f;
-            x = cmul(cmul(F.eval(m, v, crossing(m, w, ch), p, ch, sp, fl), P0[w + nVP]), F.f0.eval(m, w, FDGA_INF, ch, sp));
+            double2 x = cmul(cmul(F.eval(m, v, w, p, CH, SP, FLG), P0[q]), F.f0.eval(m, crossing(m, w, CH), FDGA_INF, CH, SP));
+            sr += x.x; si += x.y;
         }
-        sr += x.x;
-        si += x.y;
+    } else {
+        for (int q = lane; q < NV2; q += WAVE) {
+            const int w = q - n2f;
+            double2 x = cmul(cmul(F.eval(m, v, crossing(m, w, CH), p, CH, SP, FLG), P0[w + nVP]), F.f0.eval(m, w, FDGA_INF, CH, SP));
+            sr += x.x; si += x.y;
+        }
     }
     sr = wave_sum(sr);
     si = wave_sum(si);
     if (lane == 0) {
         double2 val = cscale(cmk(sr, si), scale);
-        if (tpost) val = cscale(cadd(val, outa[wid]), 0.5);
+        if (CH == TCH) val = cscale(cadd(val, outa[wid]), 0.5);
         out[wid] = val;
     }
 }
@@ -510,35 +560,64 @@ __global__ void k_gemv_lk2(double2* __restrict__ out, const double2* __restrict_
 // ---------------------------------------------------------------------------------------
 // SDE_channel_L_pp! / _ph! (src/SDE.jl:79-217; NL src/nonlocal/SDE.jl:3-142).
 // kind 0: Vertex channels g (momentum p if NP > 1); kind 1: RefVertex tables.
-__global__ void k_sde_L(double2* __restrict__ Lpp, double2* __restrict__ Lph, const double2* __restrict__ Pipp,
+__device__ __forceinline__ void sde_terms(const ChanV& gp, const ChanV& gt, const ChanV& ga, const RefV& ref, int kind,
+                                 int m, int v, int w, int pc, double2& xp, double2& xh) {
+    if (kind == 0) {
+        xp = gp.eval(m, bmf(m, w), v, pc);
+        xh = cadd(ga.eval(m, v, w, pc), gt.eval(m, v, w, pc));
+    } else {
+        xp = csub(ref.eval(m, bmf(m, w), v, PCH, PSP), ref.bare(PSP));
+        xh = csub(csub(cadd(ref.eval(m, v, w, ACH, PSP), ref.eval(m, v, w, TCH, PSP)), ref.bare(PSP)), ref.bare(PSP));
+    }
+}
+__global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double2* __restrict__ Lph, const double2* __restrict__ Pipp,
                         const double2* __restrict__ Piph, ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
-                        int n2b, int n2f, int nWP, int nVP, int NP, int chanNP, double2 TU) {
+                        int n2b, int n2f, int nWP, int nVP, int NP, int chanNP, double2 TU, int NK) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= NW2 * NV2 * NP) return;
     const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
     const int pc = chanNP > 1 ? p : 0;
+    const int Wc = window(NK, m, v);
+    const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
     const size_t base = (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
     double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
-    for (int iw = lane; iw < 2 * nVP; iw += WAVE) {
-        const int w = iw - nVP;
+    double plr = 0.0, pli = 0.0, prr = 0.0, pri = 0.0, hlr = 0.0, hli = 0.0, hrr = 0.0, hri = 0.0;
+    for (int iw = lo + lane; iw <= hi; iw += WAVE) {
         double2 xp, xh;
-        if (kind == 0) {
-            xp = gp.eval(m, bmf(m, w), v, pc);
-            xh = cadd(ga.eval(m, v, w, pc), gt.eval(m, v, w, pc));
-        } else {
-            xp = csub(ref.eval(m, bmf(m, w), v, PCH, PSP), ref.bare(PSP));
-            xh = csub(csub(cadd(ref.eval(m, v, w, ACH, PSP), ref.eval(m, v, w, TCH, PSP)), ref.bare(PSP)), ref.bare(PSP));
-        }
+        sde_terms(gp, gt, ga, ref, kind, m, v, iw - nVP, pc, xp, xh);
         double2 a = cmul(Pipp[base + iw], xp);
         double2 b = cmul(Piph[base + iw], xh);
         ar += a.x; ai += a.y; br += b.x; bi += b.y;
     }
+    for (int iw = lane; iw < lo; iw += WAVE) {
+        plr += Pipp[base + iw].x; pli += Pipp[base + iw].y;
+        hlr += Piph[base + iw].x; hli += Piph[base + iw].y;
+    }
+    for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) {
+        prr += Pipp[base + iw].x; pri += Pipp[base + iw].y;
+        hrr += Piph[base + iw].x; hri += Piph[base + iw].y;
+    }
+    double2 fxp, fxh, xpl, xpr, xhl, xhr;
+    sde_terms(gp, gt, ga, ref, kind, m, v, lane == 0 ? -Wc - 1 : Wc + 1, pc, fxp, fxh);
+    far_pair(fxp, xpl, xpr);
+    far_pair(fxh, xhl, xhr);
     ar = wave_sum(ar); ai = wave_sum(ai); br = wave_sum(br); bi = wave_sum(bi);
+    plr = wave_sum(plr); pli = wave_sum(pli); prr = wave_sum(prr); pri = wave_sum(pri);
+    hlr = wave_sum(hlr); hli = wave_sum(hli); hrr = wave_sum(hrr); hri = wave_sum(hri);
     if (lane == 0) {
-        Lpp[wid] = cmul(cmk(ar, ai), TU);
-        Lph[wid] = cmul(cmk(br, bi), TU);
+        double2 A = cmk(ar, ai), B = cmk(br, bi);
+        if (lo > 0) {
+            A = cadd(A, cmul(cmk(plr, pli), xpl));
+            B = cadd(B, cmul(cmk(hlr, hli), xhl));
+        }
+        if (hi < 2 * nVP - 1) {
+            A = cadd(A, cmul(cmk(prr, pri), xpr));
+            B = cadd(B, cmul(cmk(hrr, hri), xhr));
+        }
+        Lpp[wid] = cmul(A, TU);
+        Lph[wid] = cmul(B, TU);
     }
 }
 
