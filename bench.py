@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-8, help="Anderson tolerance of the untimed set-up solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the parquet hot path is run on a single GPU in this build "
@@ -76,7 +77,7 @@ def main():
     log(f"bench: building SIAM fdPA solver nK1={w['nK1']} nK2={w['nK2']} nK3={w['nK3']} nG={w['nG']}")
     S = build_solver(w)
     t0 = time.perf_counter()
-    it, res = fdga.solve(S, strategy="fdPA", tol=1e-8, maxiter=80)
+    it, res = fdga.solve(S, strategy="fdPA", tol=args.tol, maxiter=80)
     log(f"bench: fixed point reached in {it} Anderson iterations (max|f| = {res:.2e}, {time.perf_counter() - t0:.1f} s)")
     for _ in range(args.warmup):
         fdga.iterate_solver(S, "fdPA")
