@@ -46,6 +46,8 @@ struct fdga_ctx {
     double2 U{};
     DevChan F[3], Fb[3], FL[3], R0[3];
     double2 *cache[10] = {};
+    double2 *R[3] = {nullptr, nullptr, nullptr};                    // BSE_K3! right operands [ω, νp, Ω, P]
+    double2 *csum[3] = {nullptr, nullptr, nullptr};                 // their column sums [νp, Ω, P]
     double2 *tmpG1 = nullptr, *tmpG2 = nullptr, *tmpG3 = nullptr;   // [NG, NkG] scratch
     double2 *tmpL1 = nullptr, *tmpL2 = nullptr;                     // [NW2*NV2, NP] scratch
     double2 *u2C = nullptr;                                         // [4nG-1, NkG]
@@ -257,7 +259,7 @@ struct Level { int kind; ChanV g[3]; int chanNP; double scale; };
 int sde_sigma(fdga_ctx* c, double2* S, const double2* G, double scale) {
     const fdga_dims& d = c->d;
     if (!d.nonlocal) {
-        k_sde_sigma_local<<<cdiv(c->NG, 128), 128, 0, c->st>>>(S, G, c->Lpp, c->Lph, d.nG, d.nK2b, d.nK2f, d.T, scale);
+        k_sde_sigma_local<<<cdiv((size_t)c->NG * WAVE, 256), 256, 0, c->st>>>(S, G, c->Lpp, c->Lph, d.nG, d.nK2b, d.nK2f, d.T, scale);
         LAUNCHCHK(c);
         return 0;
     }
@@ -284,8 +286,8 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
     const double2 fac = cscale(cmul(c->U, c->U), d.T * d.T);
     const int NB = 4 * d.nG - 1;
     if (!d.nonlocal) {
-        k_u2_C<<<dim3(cdiv(NB, 128), 1), 128, 0, c->st>>>(c->u2C, G, G, d.nG, 1);
-        k_u2_S<<<dim3(cdiv(c->NG, 128), 1), 128, 0, c->st>>>(S, c->u2C, G, d.nG, 1, fac, 1);
+        k_u2_C<<<cdiv((size_t)NB * WAVE, 256), 256, 0, c->st>>>(c->u2C, G, G, d.nG, 1);
+        k_u2_S<<<cdiv((size_t)c->NG * WAVE, 256), 256, 0, c->st>>>(S, c->u2C, G, d.nG, 1, fac, 1);
         LAUNCHCHK(c);
         return 0;
     }
@@ -294,8 +296,8 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
     HIPCHK(c, hipMemcpyAsync(c->tmpG2, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
     if (dft2(c, c->tmpG1, c->NG, LG, -1, 1.0 / ((double)LG * LG))) return -3;   // G(+R)
     if (dft2(c, c->tmpG2, c->NG, LG, +1, 1.0 / ((double)LG * LG))) return -3;   // G(-R)
-    k_u2_C<<<dim3(cdiv(NB, 128), c->NkG), 128, 0, c->st>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
-    k_u2_S<<<dim3(cdiv(c->NG, 128), c->NkG), 128, 0, c->st>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac, 0);
+    k_u2_C<<<cdiv((size_t)NB * c->NkG * WAVE, 256), 256, 0, c->st>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
+    k_u2_S<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->st>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac, 0);
     LAUNCHCHK(c);
     if (dft2(c, c->tmpG3, c->NG, LG, +1, 1.0)) return -3;
     k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(S, c->tmpG3, 1.0, 1.0, c->nGel);
@@ -352,15 +354,13 @@ int sde_levels(fdga_ctx* c, double2* S, const double2* G, const double2* Pp, con
     return 0;
 }
 
-// K3-class operands
-K3Common k3common(fdga_ctx* c, int ch) {
-    K3Common k;
-    k.n3b = c->d.nK3b; k.n3f = c->d.nK3f; k.NP = c->NP;
-    k.nWP = c->d.nK1; k.nVP = c->nVP;
-    k.Pi = pi_of(c, ch, false);
-    k.Pi0 = pi_of(c, ch, true);
-    k.T = c->d.T;
-    return k;
+// K3-class geometry
+K3Geom k3geom(const fdga_ctx* c) {
+    K3Geom g;
+    g.n3b = c->d.nK3b; g.n3f = c->d.nK3f; g.NP = c->NP;
+    g.nWP = c->d.nK1; g.nVP = c->nVP;
+    g.n2b = c->d.nK2b; g.n2f = c->d.nK2f;
+    return g;
 }
 
 int idx_gamma(int ch) { return ch == PCH ? 4 : ch == ACH ? 5 : 6; }
@@ -405,6 +405,88 @@ static int launch_k2(fdga_ctx* c) {
                                                              chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
                                                              c->NP, c->d.T * kSign[CH], box_bound(c));
     LAUNCHCHK(c);
+    return 0;
+}
+
+// BSE_L_K3! for `n` channels in one launch.  A single t channel is post-processed in the
+// epilogue (its a channel is complete); in a multi-channel launch t is post-processed afterwards.
+static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
+    const int NV = c->NV3, NW = c->NW3;
+    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
+    LK3Chan cc[3] = {};
+    for (int s = 0; s < n; ++s) {
+        const int ch = chans[s];
+        cc[s].G = c->cache[idx_gamma(ch)];
+        cc[s].F0r = c->cache[idx_f0(ch)];
+        cc[s].Pi0 = pi_of(c, ch, true);
+        cc[s].out = c->FL[ch].K3;
+        cc[s].outa = c->FL[ACH].K3;
+        cc[s].scale = c->d.T * ((ch == ACH) ? 1.0 : -1.0);   // src/BSE_templates.jl:122,130,138
+        cc[s].tpost = (ch == TCH && n == 1);
+    }
+    const K3Geom g = k3geom(c);
+    LK3_Aop a{g, cc[0], cc[1], cc[2]};
+    LK3_Bop b{g, cc[0], cc[1], cc[2]};
+    LK3_Ep e{g, cc[0], cc[1], cc[2]};
+    const size_t nb = (size_t)n * NW * c->NP * tm * tn;
+    zgemm_tiles<<<(unsigned)nb, 256, 0, c->st>>>(NV, NV, NV, tm, tn, a, b, e);
+    LAUNCHCHK(c);
+    if (n > 1) {
+        for (int s = 0; s < n; ++s)
+            if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->FL[TCH].K3, c->FL[ACH].K3, c->n3 * c->NP);
+        LAUNCHCHK(c);
+    }
+    return 0;
+}
+
+// BSE_K3! for `n` channels: right operands and column sums, then one GEMM launch
+static int launch_k3(fdga_ctx* c, const int* chans, int n) {
+    const int NV = c->NV3, NW = c->NW3;
+    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
+    const K3Geom g = k3geom(c);
+    K3RChan rc[3] = {};
+    K3Chan kc[3] = {};
+    for (int s = 0; s < n; ++s) {
+        const int ch = chans[s];
+        rc[s].ch = ch;
+        rc[s].s1 = (ch == ACH) ? 1.0 : -1.0;   // src/BSE_templates.jl:156,164,172
+        rc[s].s2 = (ch == TCH) ? -1.0 : 1.0;
+        rc[s].Pi = pi_of(c, ch, false);
+        rc[s].Pi0 = pi_of(c, ch, true);
+        rc[s].F0r = c->cache[idx_f0(ch)];
+        rc[s].Gam = c->cache[(ch == PCH) ? 0 : idx_gamma(ch)];
+        rc[s].Lr = c->FL[ch].K3;
+        rc[s].La = c->FL[ACH].K3;
+        rc[s].R = c->R[ch];
+        rc[s].csum = c->csum[ch];
+        kc[s].ch = ch;
+        kc[s].Fbase = c->cache[idx_fbase(ch)];
+        kc[s].K3r = c->F[ch].K3;
+        kc[s].K3a = c->F[ACH].K3;
+        kc[s].K2r = c->F[ch].K2;
+        kc[s].K2a = c->F[ACH].K2;
+        kc[s].R = c->R[ch];
+        kc[s].csum = c->csum[ch];
+        kc[s].Lr = c->FL[ch].K3;
+        kc[s].La = c->FL[ACH].K3;
+        kc[s].out = c->Fb[ch].K3;
+        kc[s].outa = c->Fb[ACH].K3;
+        kc[s].tpost = (ch == TCH && n == 1);
+    }
+    const size_t ncol = (size_t)NV * NW * c->NP * n;
+    k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->st>>>(g, rc[0], rc[1], rc[2], n);
+    LAUNCHCHK(c);
+    K3_Aop a{g, c->d.T, kc[0], kc[1], kc[2]};
+    K3_Bop b{g, c->d.T, kc[0], kc[1], kc[2]};
+    K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
+    const size_t nb = (size_t)n * NW * c->NP * tm * tn;
+    zgemm_tiles<<<(unsigned)nb, 256, 0, c->st>>>(NV, NV, NV, tm, tn, a, b, e);
+    LAUNCHCHK(c);
+    if (n > 1) {
+        for (int s = 0; s < n; ++s)
+            if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->Fb[TCH].K3, c->Fb[ACH].K3, c->n3 * c->NP);
+        LAUNCHCHK(c);
+    }
     return 0;
 }
 
@@ -483,6 +565,10 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
         }
     }
     for (auto& p : c->cache) rc |= dalloc(c, &p, c->n3);
+    for (int r = 0; r < 3; ++r) {
+        rc |= dalloc(c, &c->R[r], c->n3 * c->NP);
+        rc |= dalloc(c, &c->csum[r], (size_t)c->NV3 * c->NW3 * c->NP);
+    }
     if (hipMalloc((void**)&c->scal, 8 * sizeof(double)) != hipSuccess) rc = -2;
     if (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess) rc = -2;
     if (rc) {
@@ -506,6 +592,10 @@ int fdga_destroy(fdga_ctx* c) {
         if (p) (void)hipFree(p);
     for (auto* t : c->tab) if (t) (void)hipFree(t);
     for (auto* p : c->cache) if (p) (void)hipFree(p);
+    for (int r = 0; r < 3; ++r) {
+        if (c->R[r]) (void)hipFree(c->R[r]);
+        if (c->csum[r]) (void)hipFree(c->csum[r]);
+    }
     for (int r = 0; r < 3; ++r) {
         free_chan(c->F[r]); free_chan(c->Fb[r]); free_chan(c->FL[r]);
         if (c->R0[r].K1) free_chan(c->R0[r]);
@@ -655,7 +745,8 @@ int fdga_build_k3_cache(fdga_ctx* c) {
     CachePtrs cp;
     for (int k = 0; k < 10; ++k) cp.c[k] = c->cache[k];
     dim3 g(cdiv(c->NV3, 64), c->NV3, c->NW3);
-    k_cache<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
+    k_cache1<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
+    k_cache2<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
     LAUNCHCHK(c);
     return 0;
 }
@@ -670,20 +761,9 @@ int fdga_bse_l_k2(fdga_ctx* c, int32_t ch) {
 
 int fdga_bse_l_k3(fdga_ctx* c, int32_t ch) {
     if (ch < 0 || ch > 2) return fail(c, "bad channel");
-    const int NV = c->NV3, NW = c->NW3;
-    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
-    LK3_Aop a{c->cache[idx_gamma(ch)], NV, NW};
-    LK3_Bop b{k3common(c, ch), c->cache[idx_f0(ch)]};
-    const double s = (ch == ACH) ? 1.0 : -1.0;   // src/BSE_templates.jl:122,130,138
-    LK3_Ep e{c->FL[ch].K3, c->FL[ACH].K3, c->d.T * s, NV, ch == TCH};
-    const size_t nb = (size_t)NW * c->NP * tm * tn;
-    zgemm_tiles<<<(unsigned)nb, 256, 0, c->st>>>(NV, NV, NV, tm, tn, a, b, e);
-    LAUNCHCHK(c);
-    return 0;
+    const int chans[1] = {ch};
+    return launch_lk3(c, chans, 1);
 }
-
-
-
 
 int fdga_bse_k1(fdga_ctx* c, int32_t ch) {
     if (ch == PCH) return launch_k1<PCH>(c);
@@ -701,18 +781,8 @@ int fdga_bse_k2(fdga_ctx* c, int32_t ch) {
 
 int fdga_bse_k3(fdga_ctx* c, int32_t ch) {
     if (ch < 0 || ch > 2) return fail(c, "bad channel");
-    const int NV = c->NV3, NW = c->NW3;
-    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
-    K3Common kc = k3common(c, ch);
-    K3_Aop a{kc, c->cache[idx_fbase(ch)], chanF(c, c->F[ch]), chanF(c, c->F[ACH]), ch};
-    const int ig = (ch == PCH) ? 0 : idx_gamma(ch);
-    const double s1 = (ch == ACH) ? 1.0 : -1.0, s2 = (ch == TCH) ? -1.0 : 1.0;   // src/BSE_templates.jl:156,164,172
-    K3_Bop b{kc, c->cache[idx_f0(ch)], c->cache[ig], c->FL[ch].K3, c->FL[ACH].K3, ch, s1, s2};
-    K3_Ep e{c->Fb[ch].K3, c->Fb[ACH].K3, c->FL[ch].K3, c->FL[ACH].K3, c->d.T, NV, ch};
-    const size_t nb = (size_t)NW * c->NP * tm * tn;
-    zgemm_tiles<<<(unsigned)nb, 256, 0, c->st>>>(NV, NV, NV, tm, tn, a, b, e);
-    LAUNCHCHK(c);
-    return 0;
+    const int chans[1] = {ch};
+    return launch_k3(c, chans, 1);
 }
 
 int fdga_update_F(fdga_ctx* c) {
@@ -762,7 +832,8 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
         for (int ch : {PCH, ACH, TCH}) if (fdga_bse_l_k2(c, ch)) return -3;
         stage_end(c, 2);
         stage_begin(c);
-        for (int ch : {PCH, ACH, TCH}) if (fdga_bse_l_k3(c, ch)) return -3;
+        const int all[3] = {PCH, ACH, TCH};
+        if (launch_lk3(c, all, 3)) return -3;
         stage_end(c, 3);
     }
     stage_begin(c);
@@ -772,7 +843,10 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     for (int ch : {PCH, ACH, TCH}) if (fdga_bse_k2(c, ch)) return -3;
     stage_end(c, 5);
     stage_begin(c);
-    for (int ch : {PCH, ACH, TCH}) if (fdga_bse_k3(c, ch)) return -3;
+    {
+        const int all[3] = {PCH, ACH, TCH};
+        if (launch_k3(c, all, 3)) return -3;
+    }
     stage_end(c, 6);
     stage_begin(c);
     if (fdga_update_F(c)) return -3;

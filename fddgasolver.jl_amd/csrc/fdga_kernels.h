@@ -149,7 +149,8 @@ __global__ void k_pavg(double2* __restrict__ A, const double2* __restrict__ K, s
 //   c[7] Fbase_p [ν, ω]  c[8] Fbase_a [ν, ω]  c[9] Fbase_t(d) [ν, ω]
 struct CachePtrs { double2* c[10]; };
 
-__global__ void k_cache(CachePtrs out, VtxV F, int n3b, int n3f) {
+// loop 1 of build_K3_cache! at (Ω, ω, νp): Γpx and the F0 caches
+__global__ __launch_bounds__(64) void k_cache1(CachePtrs out, VtxV F, int n3b, int n3f) {
     const int NV = 2 * n3f;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
@@ -158,30 +159,37 @@ __global__ void k_cache(CachePtrs out, VtxV F, int n3b, int n3f) {
     const int m = iW - (n3b - 1);
     const int a = i - n3f, b = j - n3f;
     const size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * iW);
-    const F0V& f0 = F.f0;
-    {   // loop 1 at (Ω, ω = a, νp = b)
-        out.c[0][o] = F.eval(m, a, b, -1, PCH, XSP, FL_GT | FL_GA);
-        out.c[1][o] = csub(f0.eval(m, a, b, PCH, XSP), f0.eval(m, a, FDGA_INF, PCH, XSP));
-        double2 f0a = csub(f0.eval(m, a, b, ACH, PSP), f0.eval(m, a, FDGA_INF, ACH, PSP));
-        double2 f0t = csub(f0.eval(m, a, b, TCH, PSP), f0.eval(m, a, FDGA_INF, TCH, PSP));
-        out.c[2][o] = f0a;
-        out.c[3][o] = csub(cscale(f0t, 2.0), f0a);
-    }
-    {   // loop 2 at (Ω, ν = a, ω = b)
-        double2 gpp = F.eval_psp(m, a, b, -1, PCH, FL_GT | FL_GA);
-        double2 ga = F.eval_psp(m, a, b, -1, ACH, FL_GP | FL_GT);
-        double2 gt = F.eval_psp(m, a, b, -1, TCH, FL_GP | FL_GA);
-        double2 dp = csub(f0.eval_psp(m, a, b, PCH), f0.eval_psp(m, FDGA_INF, b, PCH));
-        double2 da = csub(f0.eval_psp(m, a, b, ACH), f0.eval_psp(m, FDGA_INF, b, ACH));
-        double2 dt = csub(f0.eval_psp(m, a, b, TCH), f0.eval_psp(m, FDGA_INF, b, TCH));
-        out.c[4][o] = gpp;
-        out.c[5][o] = ga;
-        out.c[6][o] = csub(cscale(gt, 2.0), ga);
-        out.c[7][o] = cadd(dp, gpp);
-        double2 fba = cadd(da, ga);
-        out.c[8][o] = fba;
-        out.c[9][o] = csub(cscale(cadd(dt, gt), 2.0), fba);
-    }
+    out.c[0][o] = F.eval(m, a, b, -1, PCH, XSP, FL_GT | FL_GA);
+    out.c[1][o] = csub(F.f0.eval(m, a, b, PCH, XSP), F.f0.eval(m, a, FDGA_INF, PCH, XSP));
+    double2 f0a = csub(F.f0.eval(m, a, b, ACH, PSP), F.f0.eval(m, a, FDGA_INF, ACH, PSP));
+    double2 f0t = csub(F.f0.eval(m, a, b, TCH, PSP), F.f0.eval(m, a, FDGA_INF, TCH, PSP));
+    out.c[2][o] = f0a;
+    out.c[3][o] = csub(cscale(f0t, 2.0), f0a);
+}
+
+// loop 2 of build_K3_cache! at (Ω, ν, ω): Γ_irr of each channel and the P-independent F caches
+__global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, int n3f) {
+    const int NV = 2 * n3f;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    int iW = blockIdx.z;
+    if (i >= NV) return;
+    const int m = iW - (n3b - 1);
+    const int a = i - n3f, b = j - n3f;
+    const size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * iW);
+    double2 gpp = F.eval_psp(m, a, b, -1, PCH, FL_GT | FL_GA);
+    double2 ga = F.eval_psp(m, a, b, -1, ACH, FL_GP | FL_GT);
+    double2 gt = F.eval_psp(m, a, b, -1, TCH, FL_GP | FL_GA);
+    double2 dp = csub(F.f0.eval_psp(m, a, b, PCH), F.f0.eval_psp(m, FDGA_INF, b, PCH));
+    double2 da = csub(F.f0.eval_psp(m, a, b, ACH), F.f0.eval_psp(m, FDGA_INF, b, ACH));
+    double2 dt = csub(F.f0.eval_psp(m, a, b, TCH), F.f0.eval_psp(m, FDGA_INF, b, TCH));
+    out.c[4][o] = gpp;
+    out.c[5][o] = ga;
+    out.c[6][o] = csub(cscale(gt, 2.0), ga);
+    out.c[7][o] = cadd(dp, gpp);
+    double2 fba = cadd(da, ga);
+    out.c[8][o] = fba;
+    out.c[9][o] = csub(cscale(cadd(dt, gt), 2.0), fba);
 }
 
 // materialise cache_F*[Ω, ν, ω, P] in the reference layout (ABI read-back only)
@@ -292,107 +300,203 @@ __global__ __launch_bounds__(256) void zgemm_tiles(int M, int N, int K, int tile
 }
 
 // ---------------------------------------------------------------------------------------
-// operands of the K3-class BSE for one channel; batch = iΩ + NW3 * P
-struct K3Common {
+// K3-class BSE (src/BSEa/BSEa_K3.jl; NL src/nonlocal/BSEa/BSEa_K3.jl).  One GEMM launch covers
+// up to three channels; batch b = ((chan slot) * NP + P) * NW3 + iΩ.  The right operand R of the
+// full BSE is materialised by k_k3_R (with its column sums), so both loaders are plain loads.
+struct K3Geom {
     int n3b, n3f, NP;
-    int nWP, nVP;                 // bubble mesh N's
-    const double2* Pi;            // [ω, Ω, P] bubble mesh
-    const double2* Pi0;
-    double T;
-    __device__ int NV() const { return 2 * n3f; }
-    __device__ int NW() const { return 2 * n3b - 1; }
-    __device__ size_t pidx(int m, int w, int p) const {
+    int nWP, nVP;   // bubble mesh N's
+    int n2b, n2f;   // K2 box (rank-1 same-channel K2 term)
+    __device__ __forceinline__ int NV() const { return 2 * n3f; }
+    __device__ __forceinline__ int NW() const { return 2 * n3b - 1; }
+    __device__ __forceinline__ size_t pidx(int m, int w, int p) const {
         return (size_t)(w + nVP) + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
     }
 };
 
-// BSE_L_K3! (src/BSEa/BSEa_K3.jl:1-41): A = Γr[ν, ω, Ω], B = Π0(Ω, ω, P) F0r[ω, νp, Ω]
+// --- BSE_L_K3!: A = Γr[ν, ω, Ω], B = Π0(Ω, ω, P) F0r[ω, νp, Ω], FL.γr.K3 = T s A B
+struct LK3Chan {
+    const double2* G;      // Γr cache
+    const double2* F0r;    // F0r cache
+    const double2* Pi0;    // Π0pp or Π0ph
+    double2* out;          // FL.γr.K3
+    const double2* outa;   // FL.γa.K3 (t channel, in-launch post-processing)
+    double scale;          // T * s
+    int tpost;             // t channel post-processing inside the epilogue
+};
 struct LK3_Aop {
-    const double2* G; int NV, NW;
-    __device__ double2 operator()(int b, int i, int k) const {
-        const int iW = b % NW;
-        return G[(size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
+    K3Geom g; LK3Chan c0, c1, c2;
+    __device__ __forceinline__ double2 op(const LK3Chan& c, int bb, int i, int k) const {
+        const int NV = g.NV(), NW = g.NW();
+        const int iW = bb % NW;
+        return c.G[(size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
+    }
+    __device__ __forceinline__ double2 operator()(int b, int i, int k) const {
+        const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
+        return s == 0 ? op(c0, bb, i, k) : s == 1 ? op(c1, bb, i, k) : op(c2, bb, i, k);
     }
 };
 struct LK3_Bop {
-    K3Common c;
-    const double2* F0r;
-    __device__ double2 operator()(int b, int k, int j) const {
-        const int NW = c.NW(), NV = c.NV();
-        const int iW = b % NW, p = b / NW;
-        const int m = iW - (c.n3b - 1), w = k - c.n3f;
-        return cmul(c.Pi0[c.pidx(m, w, p)], F0r[(size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW)]);
+    K3Geom g; LK3Chan c0, c1, c2;
+    __device__ __forceinline__ double2 op(const LK3Chan& c, int bb, int k, int j) const {
+        const int NV = g.NV(), NW = g.NW();
+        const int iW = bb % NW, p = bb / NW;
+        const int m = iW - (g.n3b - 1), w = k - g.n3f;
+        return cmul(c.Pi0[g.pidx(m, w, p)], c.F0r[(size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW)]);
+    }
+    __device__ __forceinline__ double2 operator()(int b, int k, int j) const {
+        const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
+        return s == 0 ? op(c0, bb, k, j) : s == 1 ? op(c1, bb, k, j) : op(c2, bb, k, j);
     }
 };
-// FL.γr.K3 = T s acc; t channel: (T s acc + FL.γa.K3) / 2 (src/BSE_templates.jl:136-145)
 struct LK3_Ep {
-    double2* out; const double2* outa; double scale; int NV; int tpost;
-    __device__ void operator()(int b, int i, int j, double2 v) const {
-        size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * b);
-        double2 x = cscale(v, scale);
-        if (tpost) x = cscale(cadd(x, outa[o]), 0.5);
-        out[o] = x;
+    K3Geom g; LK3Chan c0, c1, c2;
+    __device__ __forceinline__ void op(const LK3Chan& c, int bb, int i, int j, double2 v) const {
+        const int NV = g.NV();
+        size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * bb);
+        double2 x = cscale(v, c.scale);
+        if (c.tpost) x = cscale(cadd(x, c.outa[o]), 0.5);
+        c.out[o] = x;
+    }
+    __device__ __forceinline__ void operator()(int b, int i, int j, double2 v) const {
+        const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
+        if (s == 0) op(c0, bb, i, j, v);
+        else if (s == 1) op(c1, bb, i, j, v);
+        else op(c2, bb, i, j, v);
     }
 };
 
-// BSE_K3! (src/BSEa/BSEa_K3.jl:43-128):
-//   A(ν, ω)  = cache_Fr = Fbase_r[ν, ω, Ω] + Δγ_r(Ω, ν, ω, P)     (t: 2 Δγ_t - Δγ_a)
-//   B(ω, νp) = s1 [(Π-Π0)(ω) F0r[ω, νp] + Π(ω) Γ~(ω, νp)] + s2 Π(ω) L~(ω, νp) 1(ω~ in box)
-struct K3_Aop {
-    K3Common c;
-    const double2* Fbase;
-    ChanV gr, ga;   // target vertex channel r (and a, for t)
+// --- BSE_K3!:
+//   A(ν, ω)  = cache_Fr = Fbase_r[ν, ω, Ω] + K2r(Ω, ν, P) + K3r(Ω, ν, ω, P)   (t: 2·(t) - (a))
+//   R(ω, νp) = s1 [(Π-Π0)(ω) F0r[ω, νp] + Π(ω) Γ~(ω, νp)] + s2 Π(ω) L~(ω, νp) 1(ω~ in box)
+//   K3 = T (A_3 R + K2r(ν) Σ_ω R(ω, νp)) + L_self      (A_3: A without the ω-independent K2 part)
+struct K3Chan {
     int ch;
-    __device__ double2 operator()(int b, int i, int k) const {
-        const int NW = c.NW(), NV = c.NV();
-        const int iW = b % NW, p = b / NW;
-        const int m = iW - (c.n3b - 1), v = i - c.n3f, w = k - c.n3f;
-        double2 x = Fbase[(size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
-        double2 dr = cadd(gr.k2(m, v, p), gr.k3(m, v, w, p));
-        if (ch == TCH) dr = csub(cscale(dr, 2.0), cadd(ga.k2(m, v, p), ga.k3(m, v, w, p)));
-        return cadd(x, dr);
+    const double2* Fbase;  // [ν, ω, Ω]
+    const double2* K3r;    // target vertex K3 of this channel [ν, νp, Ω, P]
+    const double2* K3a;    // target vertex K3 of the a channel (t channel combination)
+    const double2* K2r;    // target vertex K2 [Ω, ν, P]
+    const double2* K2a;
+    const double2* R;      // [ω, νp, Ω, P]
+    const double2* csum;   // [νp, Ω, P]
+    const double2* Lr;     // FL.γr.K3
+    const double2* La;     // FL.γa.K3
+    double2* out;          // Fbuff.γr.K3
+    const double2* outa;   // Fbuff.γa.K3 (in-launch t post-processing)
+    int tpost;
+};
+struct K3_Aop {
+    K3Geom g; double T; K3Chan c0, c1, c2;
+    __device__ __forceinline__ double2 op(const K3Chan& c, int bb, int i, int k) const {
+        const int NV = g.NV(), NW = g.NW();
+        const int iW = bb % NW;
+        const size_t cij = (size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW);
+        const size_t vij = (size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * bb);
+        double2 x = c.Fbase[cij];
+        if (c.ch == TCH) return cadd(x, csub(cscale(c.K3r[vij], 2.0), c.K3a[vij]));
+        return cadd(x, c.K3r[vij]);
+    }
+    __device__ __forceinline__ double2 operator()(int b, int i, int k) const {
+        const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
+        return s == 0 ? op(c0, bb, i, k) : s == 1 ? op(c1, bb, i, k) : op(c2, bb, i, k);
     }
 };
 struct K3_Bop {
-    K3Common c;
+    K3Geom g; double T; K3Chan c0, c1, c2;
+    __device__ __forceinline__ double2 op(const K3Chan& c, int bb, int k, int j) const {
+        const int NV = g.NV();
+        return c.R[(size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb)];
+    }
+    __device__ __forceinline__ double2 operator()(int b, int k, int j) const {
+        const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
+        return s == 0 ? op(c0, bb, k, j) : s == 1 ? op(c1, bb, k, j) : op(c2, bb, k, j);
+    }
+};
+struct K3_Ep {
+    K3Geom g; double T; K3Chan c0, c1, c2;
+    __device__ __forceinline__ void op(const K3Chan& c, int bb, int i, int j, double2 v) const {
+        const int NV = g.NV(), NW = g.NW();
+        const int iW = bb % NW, p = bb / NW;
+        const int m = iW - (g.n3b - 1), nu = i - g.n3f;
+        const size_t k2 = (size_t)(m + g.n2b - 1) + (size_t)(2 * g.n2b - 1) * ((size_t)(nu + g.n2f) + (size_t)(2 * g.n2f) * p);
+        double2 k2v = c.K2r[k2];
+        if (c.ch == TCH) k2v = csub(cscale(k2v, 2.0), c.K2a[k2]);
+        v = cadd(v, cmul(k2v, c.csum[(size_t)j + (size_t)NV * bb]));
+        const size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * bb);
+        double2 self = c.Lr[o];
+        if (c.ch == TCH) self = csub(cscale(self, 2.0), c.La[o]);
+        double2 x = cadd(cscale(v, T), self);
+        if (c.tpost) x = cscale(cadd(x, c.outa[o]), 0.5);
+        c.out[o] = x;
+    }
+    __device__ __forceinline__ void operator()(int b, int i, int j, double2 v) const {
+        const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
+        if (s == 0) op(c0, bb, i, j, v);
+        else if (s == 1) op(c1, bb, i, j, v);
+        else op(c2, bb, i, j, v);
+    }
+};
+
+// right operand of BSE_K3! and its column sums; one wave per (νp, Ω, P), lanes over ω
+struct K3RChan {
+    int ch;
+    double s1, s2;
+    const double2* Pi;
+    const double2* Pi0;
     const double2* F0r;   // [ω, νp, Ω]
     const double2* Gam;   // p: Γpx [ω, νp, Ω]; a, t: Γr [ν, ω, Ω] read transposed
     const double2* Lr;    // FL.γr.K3 [ν, νp, Ω, P]
-    const double2* La;    // FL.γa.K3 (t channel)
-    int ch;
-    double s1, s2;
-    __device__ double2 operator()(int b, int k, int j) const {
-        const int NW = c.NW(), NV = c.NV();
-        const int iW = b % NW, p = b / NW;
-        const int m = iW - (c.n3b - 1), w = k - c.n3f;
-        const size_t pi = c.pidx(m, w, p);
+    const double2* La;    // FL.γa.K3
+    double2* R;           // [ω, νp, Ω, P]
+    double2* csum;        // [νp, Ω, P]
+};
+template <int CH>
+__device__ __forceinline__ void k3_R_column(const K3Geom& g, const K3RChan& c, int j, int bb, int lane) {
+    const int NV = g.NV(), NW = g.NW();
+    const int iW = bb % NW, p = bb / NW;
+    const int m = iW - (g.n3b - 1);
+    double cr = 0.0, ci = 0.0;
+    for (int k = lane; k < NV; k += WAVE) {
+        const int w = k - g.n3f;
+        const size_t pi = g.pidx(m, w, p);
         const double2 P1 = c.Pi[pi], P0 = c.Pi0[pi];
         const size_t kj = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW);
-        const double2 f0 = F0r[kj];
-        const double2 g = (ch == PCH) ? Gam[kj] : Gam[(size_t)j + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
-        double2 x = cscale(cadd(cmul(csub(P1, P0), f0), cmul(P1, g)), s1);
-        const int w_ = crossing(m, w, ch);
-        if (in_f(w_, c.n3f)) {
-            const size_t lo = (size_t)(w_ + c.n3f) + (size_t)NV * ((size_t)j + (size_t)NV * ((size_t)iW + (size_t)NW * p));
-            double2 L = Lr[lo];
-            if (ch == TCH) L = csub(cscale(L, 2.0), La[lo]);
-            x = cadd(x, cscale(cmul(P1, L), s2));
+        const double2 f0 = c.F0r[kj];
+        const double2 gm = (CH == PCH) ? c.Gam[kj] : c.Gam[(size_t)j + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
+        double2 x = cscale(cadd(cmul(csub(P1, P0), f0), cmul(P1, gm)), c.s1);
+        const int w_ = crossing(m, w, CH);
+        if (in_f(w_, g.n3f)) {
+            const size_t lo = (size_t)(w_ + g.n3f) + (size_t)NV * ((size_t)j + (size_t)NV * bb);
+            double2 L = c.Lr[lo];
+            if (CH == TCH) L = csub(cscale(L, 2.0), c.La[lo]);
+            x = cadd(x, cscale(cmul(P1, L), c.s2));
         }
-        return x;
+        c.R[(size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb)] = x;
+        cr += x.x; ci += x.y;
     }
-};
-// Fbuff.γr.K3 = T acc + L_self; t: then (γt + γa) / 2
-struct K3_Ep {
-    double2* out; const double2* outa; const double2* Lr; const double2* La; double T; int NV; int ch;
-    __device__ void operator()(int b, int i, int j, double2 v) const {
-        size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * b);
-        double2 self = Lr[o];
-        if (ch == TCH) self = csub(cscale(self, 2.0), La[o]);
-        double2 x = cadd(cscale(v, T), self);
-        if (ch == TCH) x = cscale(cadd(x, outa[o]), 0.5);
-        out[o] = x;
-    }
-};
+    cr = wave_sum(cr); ci = wave_sum(ci);
+    if (lane == 0) c.csum[(size_t)j + (size_t)NV * bb] = cmk(cr, ci);
+}
+__device__ __forceinline__ void k3_R_dispatch(const K3Geom& g, const K3RChan& c, int j, int bb, int lane) {
+    if (c.ch == PCH) k3_R_column<PCH>(g, c, j, bb, lane);
+    else if (c.ch == TCH) k3_R_column<TCH>(g, c, j, bb, lane);
+    else k3_R_column<ACH>(g, c, j, bb, lane);
+}
+__global__ __launch_bounds__(256) void k_k3_R(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NV = 2 * g.n3f, nb = (2 * g.n3b - 1) * g.NP;
+    if (wid >= NV * nb * nchan) return;
+    const int j = wid % NV, r = wid / NV, bb = r % nb, s = r / nb;
+    if (s == 0) k3_R_dispatch(g, c0, j, bb, lane);
+    else if (s == 1) k3_R_dispatch(g, c1, j, bb, lane);
+    else k3_R_dispatch(g, c2, j, bb, lane);
+}
+
+// t channel post-processing after a multi-channel launch: γt = (γt_d + γa) / 2
+__global__ void k_post_t(double2* __restrict__ t, const double2* __restrict__ a, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) t[i] = cscale(cadd(t[i], a[i]), 0.5);
+}
 
 // dense column-major operands (library self-test of the GEMM engine)
 struct Dense_A { const double2* A; int lda; __device__ double2 operator()(int b, int i, int k) const { return A[(size_t)i + (size_t)lda * k]; } };
@@ -621,23 +725,27 @@ __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double
     }
 }
 
-// Σ(ν) += scale T Σ_Ω [G(Ω-ν) Lpp(Ω,ν) + G(Ω+ν) Lph(Ω,ν)]  (src/SDE.jl:242-263)
-__global__ void k_sde_sigma_local(double2* __restrict__ S, const double2* __restrict__ G, const double2* __restrict__ Lpp,
-                                  const double2* __restrict__ Lph, int nG, int n2b, int n2f, double T, double scale) {
-    int iv = blockIdx.x * blockDim.x + threadIdx.x;
+// Σ(ν) += scale T Σ_Ω [G(Ω-ν) Lpp(Ω,ν) + G(Ω+ν) Lph(Ω,ν)]  (src/SDE.jl:242-263); one wave per ν
+__global__ __launch_bounds__(256) void k_sde_sigma_local(double2* __restrict__ S, const double2* __restrict__ G,
+                                                         const double2* __restrict__ Lpp, const double2* __restrict__ Lph,
+                                                         int nG, int n2b, int n2f, double T, double scale) {
+    const int iv = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     if (iv >= 2 * nG) return;
     const int v = iv - nG;
     if (!in_f(v, n2f)) return;
     const int NW2 = 2 * n2b - 1;
-    double2 s = czero();
-    for (int iW = 0; iW < NW2; ++iW) {
+    double sr = 0.0, si = 0.0;
+    for (int iW = lane; iW < NW2; iW += WAVE) {
         const int m = iW - (n2b - 1);
         const size_t k = (size_t)iW + (size_t)NW2 * (v + n2f);
         const int a = bmf(m, v), b = bpf(m, v);
+        double2 s = czero();
         if (in_f(a, nG)) s = cadd(s, cmul(G[a + nG], Lpp[k]));
         if (in_f(b, nG)) s = cadd(s, cmul(G[b + nG], Lph[k]));
+        sr += s.x; si += s.y;
     }
-    S[iv] = cadd(S[iv], cscale(s, T * scale));
+    sr = wave_sum(sr); si = wave_sum(si);
+    if (lane == 0) S[iv] = cadd(S[iv], cscale(cmk(sr, si), T * scale));
 }
 
 // NL SDE_compute_inner! real-space step (src/nonlocal/SDE.jl:209-276): for output column q of
@@ -687,34 +795,46 @@ __global__ void k_sde_sigma_nl_R(double2* __restrict__ SR, const double2* __rest
 
 // SDE_U2_using_G (src/SDE.jl:328-364) via C(m) = Σ_n G(n) G(n+m):
 //   Σ_U²(ν) = U² T² Σ_m C(m) G(m + ν); for the NL solver one column per R of the real-space G.
-__global__ void k_u2_C(double2* __restrict__ C, const double2* __restrict__ Gm, const double2* __restrict__ Gp, int nG, int nR) {
+// One wave per output, lanes over the summed index.
+__global__ __launch_bounds__(256) void k_u2_C(double2* __restrict__ C, const double2* __restrict__ Gm,
+                                              const double2* __restrict__ Gp, int nG, int nR) {
     const int NG = 2 * nG, NB = 4 * nG - 1;
-    int im = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (im >= NB || r >= nR) return;
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid >= NB * nR) return;
+    const int im = wid % NB, r = wid / NB;
     const int m = im - (2 * nG - 1);
-    double2 s = czero();
-    for (int i2 = 0; i2 < NG; ++i2) {
+    double sr = 0.0, si = 0.0;
+    for (int i2 = lane; i2 < NG; i2 += WAVE) {
         const int n2 = i2 - nG, n1 = n2 + m;
-        if (in_f(n1, nG)) s = cadd(s, cmul(Gm[(size_t)(n1 + nG) + (size_t)NG * r], Gp[(size_t)i2 + (size_t)NG * r]));
+        if (in_f(n1, nG)) {
+            double2 x = cmul(Gm[(size_t)(n1 + nG) + (size_t)NG * r], Gp[(size_t)i2 + (size_t)NG * r]);
+            sr += x.x; si += x.y;
+        }
     }
-    C[(size_t)im + (size_t)NB * r] = s;
+    sr = wave_sum(sr); si = wave_sum(si);
+    if (lane == 0) C[(size_t)im + (size_t)NB * r] = cmk(sr, si);
 }
-__global__ void k_u2_S(double2* __restrict__ S, const double2* __restrict__ C, const double2* __restrict__ Gp, int nG, int nR,
-                       double2 fac, int accumulate) {
+__global__ __launch_bounds__(256) void k_u2_S(double2* __restrict__ S, const double2* __restrict__ C,
+                                              const double2* __restrict__ Gp, int nG, int nR, double2 fac, int accumulate) {
     const int NG = 2 * nG, NB = 4 * nG - 1;
-    int iv = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (iv >= NG || r >= nR) return;
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid >= NG * nR) return;
+    const int iv = wid % NG, r = wid / NG;
     const int v = iv - nG;
-    double2 s = czero();
-    for (int im = 0; im < NB; ++im) {
+    double sr = 0.0, si = 0.0;
+    for (int im = lane; im < NB; im += WAVE) {
         const int m = im - (2 * nG - 1), x = m + v;
-        if (in_f(x, nG)) s = cadd(s, cmul(C[(size_t)im + (size_t)NB * r], Gp[(size_t)(x + nG) + (size_t)NG * r]));
+        if (in_f(x, nG)) {
+            double2 y = cmul(C[(size_t)im + (size_t)NB * r], Gp[(size_t)(x + nG) + (size_t)NG * r]);
+            sr += y.x; si += y.y;
+        }
     }
-    s = cmul(s, fac);
-    size_t o = (size_t)iv + (size_t)NG * r;
-    S[o] = accumulate ? cadd(S[o], s) : s;
+    sr = wave_sum(sr); si = wave_sum(si);
+    if (lane == 0) {
+        double2 s = cmul(cmk(sr, si), fac);
+        size_t o = (size_t)iv + (size_t)NG * r;
+        S[o] = accumulate ? cadd(S[o], s) : s;
+    }
 }
 
 // occupation n = 1/2 + T Im Σ G / Nk (src/dyson.jl:33-41): single-workgroup reduction
