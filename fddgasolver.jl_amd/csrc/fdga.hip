@@ -38,7 +38,7 @@ struct fdga_ctx {
     size_t nGel = 0, nPiel = 0, n1 = 0, n2 = 0, n3 = 0, nL = 0;
     double2 *Gbare = nullptr, *G0 = nullptr, *Sig0 = nullptr, *G = nullptr, *Sig = nullptr;
     double2 *Pi0pp = nullptr, *Pi0ph = nullptr, *Pipp = nullptr, *Piph = nullptr;
-    double2 *Lpp = nullptr, *Lph = nullptr, *W = nullptr;
+    double2 *Lpp = nullptr, *Lph = nullptr, *W = nullptr, *W2 = nullptr;
     double2 *sde0 = nullptr;
     int sde0_valid = 0, sde0_h = -1;
     double2 *tab[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -380,16 +380,16 @@ static int launch_lk2(fdga_ctx* c) {
 }
 
 template <int CH>
-static int launch_w(fdga_ctx* c) {
+static int launch_w(fdga_ctx* c, double2* W) {
     dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
-    k_bse_w<CH><<<gw, 256, 0, c->st>>>(c->W, pi_of(c, CH, false), pi_of(c, CH, true), vtx_FL(c), f0_view(c), c->d.nK1, c->nVP);
+    k_bse_w<CH><<<gw, 256, 0, c->st>>>(W, pi_of(c, CH, false), pi_of(c, CH, true), vtx_FL(c), f0_view(c), c->d.nK1, c->nVP);
     LAUNCHCHK(c);
     return 0;
 }
 
 template <int CH>
 static int launch_k1(fdga_ctx* c) {
-    if (launch_w<CH>(c)) return -3;
+    if (launch_w<CH>(c, c->W)) return -3;
     const size_t nout = c->n1 * c->NP;
     k_gemv_k1<CH><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->W, vtx_F(c), c->d.nK1,
                                                              c->d.nK1, c->nVP, c->NP, c->d.T * kSign[CH], box_bound(c));
@@ -399,11 +399,22 @@ static int launch_k1(fdga_ctx* c) {
 
 template <int CH>
 static int launch_k2(fdga_ctx* c) {
-    if (launch_w<CH>(c)) return -3;
+    if (launch_w<CH>(c, c->W)) return -3;
     const size_t nout = c->n2 * c->NP;
     k_gemv_k2<CH><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->W, vtx_F(c), chanF(c, c->FL[CH]),
                                                              chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
                                                              c->NP, c->d.T * kSign[CH], box_bound(c));
+    LAUNCHCHK(c);
+    return 0;
+}
+
+// BSE_K2! of the a and t channels in one kernel (t needs the finished a channel)
+static int launch_k2_at(fdga_ctx* c) {
+    if (launch_w<ACH>(c, c->W) || launch_w<TCH>(c, c->W2)) return -3;
+    const size_t nout = c->n2 * c->NP;
+    k_gemv_k2_at<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->W, c->W2, vtx_F(c),
+                                                            chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]), c->d.nK2b, c->d.nK2f,
+                                                            c->d.nK1, c->nVP, c->NP, c->d.T, box_bound(c));
     LAUNCHCHK(c);
     return 0;
 }
@@ -548,7 +559,7 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     int rc = 0;
     for (double2** p : {&c->Gbare, &c->G0, &c->Sig0, &c->G, &c->Sig, &c->tmpG1, &c->tmpG2, &c->tmpG3})
         rc |= dalloc(c, p, c->nGel);
-    for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph, &c->W}) rc |= dalloc(c, p, c->nPiel);
+    for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph, &c->W, &c->W2}) rc |= dalloc(c, p, c->nPiel);
     for (double2** p : {&c->Lpp, &c->Lph, &c->tmpL1, &c->tmpL2}) rc |= dalloc(c, p, c->nL);
     rc |= dalloc(c, &c->sde0, c->nGel);
     rc |= dalloc(c, &c->u2C, (size_t)(4 * d.nG - 1) * c->NkG);
@@ -588,7 +599,7 @@ int fdga_destroy(fdga_ctx* c) {
     if (!c) return 0;
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (double2* p : {c->Gbare, c->G0, c->Sig0, c->G, c->Sig, c->tmpG1, c->tmpG2, c->tmpG3, c->Pi0pp, c->Pi0ph,
-                       c->Pipp, c->Piph, c->W, c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C, c->stage})
+                       c->Pipp, c->Piph, c->W, c->W2, c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C, c->stage})
         if (p) (void)hipFree(p);
     for (auto* t : c->tab) if (t) (void)hipFree(t);
     for (auto* p : c->cache) if (p) (void)hipFree(p);
@@ -840,7 +851,7 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     for (int ch : {PCH, ACH, TCH}) if (fdga_bse_k1(c, ch)) return -3;
     stage_end(c, 4);
     stage_begin(c);
-    for (int ch : {PCH, ACH, TCH}) if (fdga_bse_k2(c, ch)) return -3;
+    if (fdga_bse_k2(c, PCH) || launch_k2_at(c)) return -3;
     stage_end(c, 5);
     stage_begin(c);
     {

@@ -620,6 +620,66 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
     }
 }
 
+// BSE_K2! of the a and t channels in one pass.  The t channel runs in the density component,
+// F(t, dSp) = 2 F(t, pSp) + F(t, xSp) = 2 F(t, pSp) - F(a, pSp) (src/vertex.jl:288-336), so the
+// a-channel evaluation is shared:  K2a = T Σ fa Wa + FL.a.K2,
+// K2t = (T (-1) Σ (2 ft - fa) Wt + 2 FL.t.K2 - FL.a.K2 + K2a) / 2   (src/BSE_templates.jl:95-110)
+__global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, double2* __restrict__ K2t,
+                                                    const double2* __restrict__ Wa, const double2* __restrict__ Wt,
+                                                    VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP,
+                                                    int NP, double T, int NK) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
+    if (wid >= NW2 * NV2 * NP) return;
+    const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
+    const int m = iW - (n2b - 1), v = iv - n2f;
+    const int Wc = window(NK, m, v);
+    const size_t roff = (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    const double2* wa = Wa + roff;
+    const double2* wt = Wt + roff;
+    const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
+    double ar = 0.0, ai = 0.0, tr = 0.0, ti = 0.0;
+    for (int iw = lo + lane; iw <= hi; iw += WAVE) {
+        const int w = iw - nVP;
+        const double2 fa = csub(F.eval_psp(m, v, w, p, ACH, FL_ALL), F.eval_psp(m, FDGA_INF, w, p, ACH, FL_ALL));
+        const double2 ft = csub(F.eval_psp(m, v, w, p, TCH, FL_ALL), F.eval_psp(m, FDGA_INF, w, p, TCH, FL_ALL));
+        const double2 xa = cmul(fa, wa[iw]);
+        const double2 xt = cmul(csub(cscale(ft, 2.0), fa), wt[iw]);
+        ar += xa.x; ai += xa.y; tr += xt.x; ti += xt.y;
+    }
+    double alr = 0.0, ali = 0.0, arr = 0.0, ari = 0.0, tlr = 0.0, tli = 0.0, trr = 0.0, tri = 0.0;
+    for (int iw = lane; iw < lo; iw += WAVE) { alr += wa[iw].x; ali += wa[iw].y; tlr += wt[iw].x; tli += wt[iw].y; }
+    for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) { arr += wa[iw].x; ari += wa[iw].y; trr += wt[iw].x; tri += wt[iw].y; }
+    // far summands: lane 0 at ω = -Wc-1, lane 1 at ω = Wc+1
+    const int wf = lane == 0 ? -Wc - 1 : Wc + 1;
+    const double2 fa = csub(F.eval_psp(m, v, wf, p, ACH, FL_ALL), F.eval_psp(m, FDGA_INF, wf, p, ACH, FL_ALL));
+    const double2 ft = csub(F.eval_psp(m, v, wf, p, TCH, FL_ALL), F.eval_psp(m, FDGA_INF, wf, p, TCH, FL_ALL));
+    double2 fal, far_, fdl, fdr;
+    far_pair(fa, fal, far_);
+    far_pair(csub(cscale(ft, 2.0), fa), fdl, fdr);
+    ar = wave_sum(ar); ai = wave_sum(ai); tr = wave_sum(tr); ti = wave_sum(ti);
+    alr = wave_sum(alr); ali = wave_sum(ali); arr = wave_sum(arr); ari = wave_sum(ari);
+    tlr = wave_sum(tlr); tli = wave_sum(tli); trr = wave_sum(trr); tri = wave_sum(tri);
+    if (lane == 0) {
+        double2 A = cmk(ar, ai), D = cmk(tr, ti);
+        if (lo > 0) {
+            A = cadd(A, cmul(fal, cmk(alr, ali)));
+            D = cadd(D, cmul(fdl, cmk(tlr, tli)));
+        }
+        if (hi < 2 * nVP - 1) {
+            A = cadd(A, cmul(far_, cmk(arr, ari)));
+            D = cadd(D, cmul(fdr, cmk(trr, tri)));
+        }
+        const size_t o = (size_t)wid;
+        const double2 ka = cadd(cscale(A, T), La.K2[o]);
+        double2 kt = cscale(D, -T);
+        kt = cadd(kt, cscale(Lt.K2[o], 2.0));
+        kt = cadd(kt, cscale(La.K2[o], -1.0));
+        K2a[o] = ka;
+        K2t[o] = cscale(cadd(kt, ka), 0.5);
+    }
+}
+
 // BSE_L_K2!: local (src/BSEa/BSEa_K2.jl:1-40): Σ over the bubble mesh of
 //   Γ(Ω, ν, ω) Π0(ω) F0(Ω, ω~, ∞);  NL (src/nonlocal/BSEa/BSEa_K2.jl:1-41): Σ over the K2 mesh of
 //   Γ(Ω, ν, ω~) Π0(ω) F0(Ω, ω, ∞).  Γ excludes F0 and the own channel, so it vanishes outside
