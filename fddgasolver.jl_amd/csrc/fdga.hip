@@ -440,8 +440,7 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     LK3_Bop b{g, cc[0], cc[1], cc[2]};
     LK3_Ep e{g, cc[0], cc[1], cc[2]};
     const size_t nb = (size_t)n * NW * c->NP * tm * tn;
-    zgemm_tiles<<<(unsigned)nb, 256, 0, c->st>>>(NV, NV, NV, tm, tn, a, b, e);
-    LAUNCHCHK(c);
+    if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, a, b, e) != hipSuccess) return fail(c, "BSE_L_K3 GEMM launch failed", -3);
     if (n > 1) {
         for (int s = 0; s < n; ++s)
             if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->FL[TCH].K3, c->FL[ACH].K3, c->n3 * c->NP);
@@ -491,8 +490,7 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     K3_Bop b{g, c->d.T, kc[0], kc[1], kc[2]};
     K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
     const size_t nb = (size_t)n * NW * c->NP * tm * tn;
-    zgemm_tiles<<<(unsigned)nb, 256, 0, c->st>>>(NV, NV, NV, tm, tn, a, b, e);
-    LAUNCHCHK(c);
+    if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, a, b, e) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     if (n > 1) {
         for (int s = 0; s < n; ++s)
             if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->Fb[TCH].K3, c->Fb[ACH].K3, c->n3 * c->NP);
@@ -527,6 +525,11 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     }
     if (d.ref_kind == 1 && (d.r_nK1 < d.r_nK2b || d.r_nK1 < d.r_nK2f || d.r_nK2b < d.r_nK3b || d.r_nK2f < d.r_nK3f || d.r_nK3b < 1)) {
         fail(c, "invalid reference Vertex boxes");
+        *out = c;
+        return -1;
+    }
+    if (d.precision != FDGA_FP64 && d.precision != FDGA_FP32) {
+        fail(c, "unknown precision (FDGA_FP64 = 0, FDGA_FP32 = 1)");
         *out = c;
         return -1;
     }
@@ -950,7 +953,7 @@ int fdga_stage_ms(const fdga_ctx* c, int32_t stage, double* ms) {
 }
 
 // C = A B for dense column-major complex matrices through the BSE GEMM engine
-int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, const double* A, const double* B, double* C) {
+int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, const double* A, const double* B, double* C) {
     if (M < 1 || N < 1 || K < 1 || !A || !B || !C) return -1;
     double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
@@ -962,8 +965,8 @@ int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, const double* A, const 
         hipMemcpy(dB, B, nb * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
     if (!rc) {
         const int tm = cdiv(M, GBM), tn = cdiv(N, GBN);
-        zgemm_tiles<<<(unsigned)(tm * tn), 256>>>(M, N, K, tm, tn, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M});
-        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = -3;
+        if (zgemm_launch(precision, (size_t)(tm * tn), 0, M, N, K, tm, tn, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M}) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) rc = -3;
     }
     if (!rc && hipMemcpy(C, dC, nc * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = -2;
     (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);

@@ -212,18 +212,40 @@ __global__ void k_cache_F_julia(double2* __restrict__ out, const double2* __rest
 }
 
 // ---------------------------------------------------------------------------------------
-// Batched complex GEMM on v_mfma_f64_16x16x4f64.  C[b](m, n) = Σ_k A[b](m, k) B[b](k, n),
-// operands produced element-wise by loader functors (the gather is fused into the load),
-// results consumed by an epilogue functor.  256 threads = 4 waves, 64 x 64 complex tile per
-// workgroup, each wave a 32 x 32 sub-tile = 2 x 2 MFMA tiles with separate real/imag
-// accumulators (C_r = A_r B_r - A_i B_i, C_i = A_r B_i + A_i B_r).
+// Batched complex GEMM on the matrix cores.  C[b](m, n) = Σ_k A[b](m, k) B[b](k, n), operands
+// produced element-wise by loader functors (the gather is fused into the load), results consumed
+// by an epilogue functor.  256 threads = 4 waves, 64 x 64 complex tile per workgroup, each wave a
+// 32 x 32 sub-tile = 2 x 2 MFMA tiles with separate real/imag accumulators
+// (C_r = A_r B_r - A_i B_i, C_i = A_r B_i + A_i B_r).
+//   F32 = false: v_mfma_f64_16x16x4f64 (FP64 in, FP64 accumulate)
+//   F32 = true : v_mfma_f32_16x16x4f32 (operands rounded to FP32, FP32 accumulate)
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
 
-template <class AL, class BL, class EP>
-__global__ __launch_bounds__(256) void zgemm_tiles(int M, int N, int K, int tiles_m, int tiles_n, AL al, BL bl, EP ep) {
-    __shared__ double As[2][GBK][GBM + GPAD];
-    __shared__ double Bs[2][GBK][GBN + GPAD];
+template <bool F32> struct MfmaT;
+template <> struct MfmaT<false> {
+    typedef double S;
+    typedef v4d Acc;
+    __device__ __forceinline__ static Acc mfma(S a, S b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    // accumulator element r of lane l is D(row = l/16 + 4 r, col = l % 16)
+    __device__ __forceinline__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct MfmaT<true> {
+    typedef float S;
+    typedef v4f Acc;
+    __device__ __forceinline__ static Acc mfma(S a, S b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    // accumulator element r of lane l is D(row = 4 (l/16) + r, col = l % 16)
+    __device__ __forceinline__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <bool F32, class AL, class BL, class EP>
+__global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int tiles_m, int tiles_n, AL al, BL bl, EP ep) {
+    typedef MfmaT<F32> MT;
+    typedef typename MT::S S;
+    typedef typename MT::Acc Acc;
+    __shared__ S As[2][GBK][GBM + GPAD];
+    __shared__ S Bs[2][GBK][GBN + GPAD];
     const int tiles = tiles_m * tiles_n;
     const int bid = blockIdx.x;
     const int batch = bid / tiles;
@@ -231,13 +253,13 @@ __global__ __launch_bounds__(256) void zgemm_tiles(int M, int N, int K, int tile
     const int m0 = (t % tiles_m) * GBM, n0 = (t / tiles_m) * GBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    v4d accr[2][2], acci[2][2];
+    Acc accr[2][2], acci[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            accr[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-            acci[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+            accr[i][j] = Acc{0, 0, 0, 0};
+            acci[i][j] = Acc{0, 0, 0, 0};
         }
     for (int k0 = 0; k0 < K; k0 += GBK) {
 #pragma unroll
@@ -246,8 +268,8 @@ __global__ __launch_bounds__(256) void zgemm_tiles(int M, int N, int K, int tile
             const int mm = e % GBM, kk = e / GBM;
             const int gm = m0 + mm, gk = k0 + kk;
             double2 v = (gm < M && gk < K) ? al(batch, gm, gk) : czero();
-            As[0][kk][mm] = v.x;
-            As[1][kk][mm] = v.y;
+            As[0][kk][mm] = (S)v.x;
+            As[1][kk][mm] = (S)v.y;
         }
 #pragma unroll
         for (int q = 0; q < (GBN * GBK) / 256; ++q) {
@@ -255,14 +277,14 @@ __global__ __launch_bounds__(256) void zgemm_tiles(int M, int N, int K, int tile
             const int nn = e % GBN, kk = e / GBN;
             const int gn = n0 + nn, gk = k0 + kk;
             double2 v = (gn < N && gk < K) ? bl(batch, gk, gn) : czero();
-            Bs[0][kk][nn] = v.x;
-            Bs[1][kk][nn] = v.y;
+            Bs[0][kk][nn] = (S)v.x;
+            Bs[1][kk][nn] = (S)v.y;
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 4) {
             const int kr = kk + (lane >> 4), c = lane & 15;
-            double ar[2], ai[2], nai[2], br[2], bi[2];
+            S ar[2], ai[2], nai[2], br[2], bi[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 ar[i] = As[0][kr][wm + 16 * i + c];
@@ -278,25 +300,33 @@ __global__ __launch_bounds__(256) void zgemm_tiles(int M, int N, int K, int tile
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
-                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], accr[i][j], 0, 0, 0);
-                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
-                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+                    accr[i][j] = MT::mfma(ar[i], br[j], accr[i][j]);
+                    accr[i][j] = MT::mfma(nai[i], bi[j], accr[i][j]);
+                    acci[i][j] = MT::mfma(ar[i], bi[j], acci[i][j]);
+                    acci[i][j] = MT::mfma(ai[i], br[j], acci[i][j]);
                 }
         }
         __syncthreads();
     }
-    // v_mfma_f64_16x16x4f64 accumulator: element r of lane l is D(row = l/16 + 4 r, col = l % 16)
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
+                const int row = m0 + wm + 16 * i + MT::row(lane, r);
                 const int col = n0 + wn + 16 * j + (lane & 15);
-                if (row < M && col < N) ep(batch, row, col, cmk(accr[i][j][r], acci[i][j][r]));
+                if (row < M && col < N) ep(batch, row, col, cmk((double)accr[i][j][r], (double)acci[i][j][r]));
             }
+}
+
+// launch helper: precision 0 = FP64 MFMA, 1 = FP32 MFMA
+template <class AL, class BL, class EP>
+inline hipError_t zgemm_launch(int precision, size_t nblocks, hipStream_t st, int M, int N, int K, int tm, int tn,
+                               const AL& a, const BL& b, const EP& e) {
+    if (precision == 1) zgemm_tiles_t<true><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, a, b, e);
+    else zgemm_tiles_t<false><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, a, b, e);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------------

@@ -21,7 +21,7 @@ TAB_FPP, TAB_FPX, TAB_FTP, TAB_FTX = 11, 12, 13, 14
 CACHE0 = 32
 PCH, TCH, ACH = 0, 1, 2
 SCPA, FDPA = 0, 1
-FP64 = 0
+FP64, FP32 = 0, 1
 
 
 def vertex_id(vtx, ch, cls):
@@ -73,7 +73,7 @@ SIGNATURES = {
     "fdga_time_iterations": (ctypes.c_int, [_vp, _i32, _i32, _dp]),
     "fdga_stage_ms": (ctypes.c_int, [_vp, _i32, _dp]),
     "fdga_set_stage_timing": (ctypes.c_int, [_vp, _i32]),
-    "fdga_selftest_zgemm": (ctypes.c_int, [_i32, _i32, _i32, _dp, _dp, _dp]),
+    "fdga_selftest_zgemm": (ctypes.c_int, [_i32, _i32, _i32, _i32, _dp, _dp, _dp]),
 }
 
 _lib = None

@@ -33,7 +33,7 @@ class Solver:
     G mesh nG, the bubble factor mΠν_factor, the k-meshes LG (G, Σ) and L (vertex, bubble)."""
 
     def __init__(self, *, nG, nK1, nK2, nK3, T, U, mPinu, nonlocal_=False, LG=1, L_=1,
-                 ref_kind=0, ref_boxes=None, tab=(1, 1), fdpa_h0=True):
+                 ref_kind=0, ref_boxes=None, tab=(1, 1), fdpa_h0=True, precision="fp64"):
         if nK1 < max(nK2) or any(a < b for a, b in zip(nK2, nK3)):
             raise AssertionError("Number of frequencies in K1 >= K2 >= K3 required")
         d = L.Dims()
@@ -51,7 +51,10 @@ class Solver:
         d.r_nK3b, d.r_nK3f = rb[2]
         d.tab_nb, d.tab_nf = tab
         d.fdpa_h0 = int(fdpa_h0)
-        d.precision = L.FP64
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        d.precision = L.FP64 if precision == "fp64" else L.FP32
+        self.precision = precision
         self.dims = d
         self.T, self.U = T, U
         self.nonlocal_ = bool(nonlocal_)
@@ -178,32 +181,34 @@ def compute_occupation(G, T, Nk=1):
 
 # ------------------------------------------------------------------------------ constructors
 def ParquetSolver(nK1, nK2, nK3, Gbare, G0, Σ0, F0=None, *, T, U, mΠν_factor=6, fdpa_h0=True,
-                  F0_boxes=None, F0_tables=None):
+                  F0_boxes=None, F0_tables=None, precision="fp64"):
     """ParquetSolver(nK1, nK2, nK3, Gbare, G0, Σ0, F0) (src/ParquetSolver.jl:81-156).
 
     F0 = None: RefVertex(T, U) with null tables.  F0 = dict {(ch, cls): array} of a local
     reference Vertex around RefVertex(T, U) with boxes F0_boxes = (nK1, nK2, nK3).
     F0_tables = (tb, tf, {'Fp_p': .., 'Fp_x': .., 'Ft_p': .., 'Ft_x': ..}) optional RefVertex tables."""
     nG = len(Gbare) // 2
-    return _make(False, nG, nK1, nK2, nK3, T, U, mΠν_factor, 1, 1, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0)
+    return _make(False, nG, nK1, nK2, nK3, T, U, mΠν_factor, 1, 1, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0,
+                 precision)
 
 
 def NL_ParquetSolver(nK1, nK2, nK3, L, Gbare, G0, Σ0, F0=None, *, T, U, mΠν_factor=32, fdpa_h0=True,
-                     F0_boxes=None, F0_tables=None):
+                     F0_boxes=None, F0_tables=None, precision="fp64"):
     """NL_ParquetSolver(nK1, nK2, nK3, mK_Γ, Gbare, G0, Σ0, F0) (src/nonlocal/ParquetSolver.jl:77-152);
     Gbare, G0, Σ0 are [ν, k] on an LG x LG mesh, the vertex lives on an L x L mesh."""
     Gbare = np.asarray(Gbare)
     nG = Gbare.shape[0] // 2
     LG = int(round(np.sqrt(Gbare.shape[1])))
-    return _make(True, nG, nK1, nK2, nK3, T, U, mΠν_factor, LG, L, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0)
+    return _make(True, nG, nK1, nK2, nK3, T, U, mΠν_factor, LG, L, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0,
+                 precision)
 
 
-def _make(nonlocal_, nG, nK1, nK2, nK3, T, U, mp, LG, L_, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0):
+def _make(nonlocal_, nG, nK1, nK2, nK3, T, U, mp, LG, L_, Gbare, G0, Σ0, F0, F0_boxes, F0_tables, fdpa_h0, precision):
     tab = (1, 1)
     if F0_tables is not None:
         tab = (F0_tables[0], F0_tables[1])
     S = Solver(nG=nG, nK1=nK1, nK2=nK2, nK3=nK3, T=T, U=U, mPinu=mp, nonlocal_=nonlocal_, LG=LG, L_=L_,
-               ref_kind=0 if F0 is None else 1, ref_boxes=F0_boxes, tab=tab, fdpa_h0=fdpa_h0)
+               ref_kind=0 if F0 is None else 1, ref_boxes=F0_boxes, tab=tab, fdpa_h0=fdpa_h0, precision=precision)
     S.set(L.GBARE, Gbare)
     S.set(L.G0, G0)
     S.set(L.SIGMA0, Σ0)
@@ -218,19 +223,22 @@ def _make(nonlocal_, nG, nK1, nK2, nK3, T, U, mp, LG, L_, Gbare, G0, Σ0, F0, F0
     return S
 
 
-def parquet_solver_siam_parquet_approximation(nG, nK1, nK2, nK3, *, e, Δ, D, T, U, mΠν_factor=6, fdpa_h0=True):
+def parquet_solver_siam_parquet_approximation(nG, nK1, nK2, nK3, *, e, Δ, D, T, U, mΠν_factor=6, fdpa_h0=True,
+                                              precision="fp64"):
     """src/ParquetSolver.jl:170-200: SIAM with G0 = Σ0 = 0 and F0 = RefVertex(T, U)."""
     Gbare = siam_bare_Green(nG, T, e=e, Δ=Δ, D=D)
     z = np.zeros_like(Gbare)
-    return ParquetSolver(nK1, nK2, nK3, Gbare, z, z, T=T, U=U, mΠν_factor=mΠν_factor, fdpa_h0=fdpa_h0)
+    return ParquetSolver(nK1, nK2, nK3, Gbare, z, z, T=T, U=U, mΠν_factor=mΠν_factor, fdpa_h0=fdpa_h0,
+                         precision=precision)
 
 
 def parquet_solver_hubbard_parquet_approximation(nG, nK1, nK2, nK3, LG, L, *, T, U, μ, t1, t2=0.0, t3=0.0,
-                                                 mΠν_factor=32, fdpa_h0=True):
+                                                 mΠν_factor=32, fdpa_h0=True, precision="fp64"):
     """src/nonlocal/ParquetSolver.jl:166-196: Hubbard with G0 = Σ0 = 0 and F0 = RefVertex(T, U)."""
     Gbare = hubbard_bare_Green(nG, T, LG, μ=μ, t1=t1, t2=t2, t3=t3)
     z = np.zeros_like(Gbare)
-    return NL_ParquetSolver(nK1, nK2, nK3, L, Gbare, z, z, T=T, U=U, mΠν_factor=mΠν_factor, fdpa_h0=fdpa_h0)
+    return NL_ParquetSolver(nK1, nK2, nK3, L, Gbare, z, z, T=T, U=U, mΠν_factor=mΠν_factor, fdpa_h0=fdpa_h0,
+                            precision=precision)
 
 
 # ------------------------------------------------------------------------------ stages

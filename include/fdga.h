@@ -34,10 +34,11 @@ typedef struct fdga_dims {
     int32_t r_nK1, r_nK2b, r_nK2f, r_nK3b, r_nK3f;  /* boxes of the local reference Vertex */
     int32_t tab_nb, tab_nf;                         /* RefVertex table box; (1, 1) = null tables */
     int32_t fdpa_h0;    /* 1: fdPA SDE subtracts (n0-1/2) U i after Σ0 (src/SDE.jl:26-31) */
-    int32_t precision;  /* FDGA_FP64 */
+    int32_t precision;  /* FDGA_FP64, or FDGA_FP32: BSE_L_K3!/BSE_K3! contraction on FP32 MFMA (operands
+                           rounded to FP32, FP32 accumulation); everything else stays FP64 */
 } fdga_dims;
 
-enum { FDGA_FP64 = 0 };
+enum { FDGA_FP64 = 0, FDGA_FP32 = 1 };
 enum { FDGA_PCH = 0, FDGA_TCH = 1, FDGA_ACH = 2 };
 enum { FDGA_SCPA = 0, FDGA_FDPA = 1 };
 
@@ -93,9 +94,9 @@ int fdga_stage_ms(const fdga_ctx* ctx, int32_t stage, double* ms);
 int fdga_set_stage_timing(fdga_ctx* ctx, int32_t on);
 
 /* Self-test of the batched complex GEMM engine used by BSE_L_K3!/BSE_K3!: C = A B for dense
- * column-major M x K and K x N ComplexF64 matrices.  No counterpart in the reference; it exists
+ * column-major M x K and K x N ComplexF64 matrices on the FP64 (precision 0) or FP32 (1) MFMA path.  No counterpart in the reference; it exists
  * so that the library's tests can check the MFMA tiling directly. */
-int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, const double* A, const double* B, double* C);
+int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, const double* A, const double* B, double* C);
 
 #ifdef __cplusplus
 }

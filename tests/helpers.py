@@ -22,7 +22,7 @@ def rand_c(rng, n, scale=0.2):
     return scale * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
 
 
-def make_pair(name, seed=0, fdpa_h0=True, box=None):
+def make_pair(name, seed=0, fdpa_h0=True, box=None, precision="fp64"):
     """Returns (hip_solver, oracle) with identical random inputs for case `name`."""
     import fdga
     from fdga import _lib as L
@@ -32,7 +32,7 @@ def make_pair(name, seed=0, fdpa_h0=True, box=None):
     tab = TAB if tables else (1, 1)
     S = fdga.Solver(nG=b["nG"], nK1=b["nK1"], nK2=b["nK2"], nK3=b["nK3"], T=b["T"], U=b["U"], mPinu=b["mPinu"],
                     nonlocal_=nonlocal_, LG=LG, L_=Lv, ref_kind=ref_kind, ref_boxes=REF_BOX if ref_kind else None,
-                    tab=tab, fdpa_h0=fdpa_h0)
+                    tab=tab, fdpa_h0=fdpa_h0, precision=precision)
     O = po.Oracle(nonlocal_=nonlocal_, nG=b["nG"], nK1=b["nK1"], nK2=b["nK2"], nK3=b["nK3"], mPinu=b["mPinu"],
                   T=b["T"], U=b["U"], LG=LG, L=Lv, ref_kind=ref_kind, ref_boxes=REF_BOX if ref_kind else None,
                   tab=tab, fdpa_h0=fdpa_h0)

@@ -157,3 +157,15 @@ def test_errors_are_raised(fdga_mod):
         fdga_mod.Solver(nG=4, nK1=2, nK2=(3, 3), nK3=(1, 1), T=0.1, U=1.0, mPinu=1)
     with pytest.raises(fdga_mod.FdgaError, match="invalid dimensions"):
         fdga_mod.Solver(nG=4, nK1=3, nK2=(2, 2), nK3=(1, 1), T=-0.1, U=1.0, mPinu=1)
+
+
+@pytest.mark.parametrize("case", ["local_vertexref", "nl_L2"])
+def test_fp32_contraction_iteration(case, fdga_mod):
+    """The FP32-MFMA contraction path: one fdPA iteration stays within FP32 rounding of the
+    FP64 oracle (everything but the BSE_L_K3!/BSE_K3! GEMMs is FP64), and the FP32 GEMMs are
+    the ones that ran (the error is well above FP64 rounding)."""
+    S, O = make_pair(case, seed=5, precision="fp32")
+    fdga_mod.iterate_solver(S, "fdPA")
+    O.iterate(po.FDPA)
+    e = relerr(fdga_mod.flatten(S), O.flatten())
+    assert 1e-12 < e < 1e-5, e
