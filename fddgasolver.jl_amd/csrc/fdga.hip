@@ -14,6 +14,7 @@
 
 #include "../../include/fdga.h"
 #include "fdga_kernels.h"
+#include "fdga_sym.h"
 
 using namespace fdga;
 
@@ -26,6 +27,16 @@ struct DevChan {
 };
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// device copy of one symmetry group's classes (fdga_sym.h); ncls = 0: identity group
+struct SymDev {
+    int* off = nullptr;
+    int* mem = nullptr;
+    int ncls = 0;
+};
+// groups of init_sym_grp! (src/ParquetSolver.jl:203-231): SGΣ; SGpp[1] = SGph[1] (same generator);
+// SGpp[2] = SGppL[2]; SGph[2] = SGphL[2]; SGpp[3], SGph[3], SGppL[3], SGphL[3]; SG0pp2, SG0ph2
+enum { SG_S, SG_K1, SG_PP2, SG_PH2, SG_PP3, SG_PH3, SG_PPL3, SG_PHL3, SG_0PP2, SG_0PH2, SG_N };
 
 }  // namespace
 
@@ -54,6 +65,7 @@ struct fdga_ctx {
     double *scal = nullptr;                                         // device scalars
     double2 *stage = nullptr;                                       // staging for layout changes
     size_t stage_n = 0;
+    SymDev sg[SG_N];                                                // installed symmetry groups
     // timing
     int timing = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -78,6 +90,15 @@ int fail(fdga_ctx* c, const std::string& msg, int code = -1) {
         hipError_t e_ = hipGetLastError();                                                 \
         if (e_ != hipSuccess) return fail((c), std::string("kernel launch: ") + hipGetErrorString(e_), -3); \
     } while (0)
+
+SymCls symcls(const fdga_ctx* c, int id) { return SymCls{c->sg[id].off, c->sg[id].mem, c->sg[id].ncls}; }
+// number of waves of a one-wave-per-output-class launch
+size_t nclass(const SymCls& s, size_t nout) { return s.ncls ? (size_t)s.ncls : nout; }
+void free_sym(SymDev& g) {
+    if (g.off) (void)hipFree(g.off);
+    if (g.mem) (void)hipFree(g.mem);
+    g = SymDev{};
+}
 
 int dalloc(fdga_ctx* c, double2** p, size_t n) {
     if (n == 0) n = 1;
@@ -236,7 +257,6 @@ void stage_end(fdga_ctx* c, int s) {
     c->stage_ms[s] += ms;
 }
 
-const int kSp[3] = {PSP, DSP, PSP};
 const double kSign[3] = {+1.0, -1.0, +1.0};
 
 double2* pi_of(fdga_ctx* c, int ch, bool zero) {
@@ -259,7 +279,9 @@ struct Level { int kind; ChanV g[3]; int chanNP; double scale; };
 int sde_sigma(fdga_ctx* c, double2* S, const double2* G, double scale) {
     const fdga_dims& d = c->d;
     if (!d.nonlocal) {
-        k_sde_sigma_local<<<cdiv((size_t)c->NG * WAVE, 256), 256, 0, c->st>>>(S, G, c->Lpp, c->Lph, d.nG, d.nK2b, d.nK2f, d.T, scale);
+        const SymCls ss = symcls(c, SG_S);
+        k_sde_sigma_local<<<cdiv(nclass(ss, c->NG) * WAVE, 256), 256, 0, c->st>>>(S, G, c->Lpp, c->Lph, d.nG, d.nK2b, d.nK2f,
+                                                                                  d.T, scale, ss);
         LAUNCHCHK(c);
         return 0;
     }
@@ -287,7 +309,8 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
     const int NB = 4 * d.nG - 1;
     if (!d.nonlocal) {
         k_u2_C<<<cdiv((size_t)NB * WAVE, 256), 256, 0, c->st>>>(c->u2C, G, G, d.nG, 1);
-        k_u2_S<<<cdiv((size_t)c->NG * WAVE, 256), 256, 0, c->st>>>(S, c->u2C, G, d.nG, 1, fac, 1);
+        const SymCls ss = symcls(c, SG_S);
+        k_u2_S<<<cdiv(nclass(ss, c->NG) * WAVE, 256), 256, 0, c->st>>>(S, c->u2C, G, d.nG, 1, fac, 1, ss);
         LAUNCHCHK(c);
         return 0;
     }
@@ -297,7 +320,8 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
     if (dft2(c, c->tmpG1, c->NG, LG, -1, 1.0 / ((double)LG * LG))) return -3;   // G(+R)
     if (dft2(c, c->tmpG2, c->NG, LG, +1, 1.0 / ((double)LG * LG))) return -3;   // G(-R)
     k_u2_C<<<cdiv((size_t)NB * c->NkG * WAVE, 256), 256, 0, c->st>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
-    k_u2_S<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->st>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac, 0);
+    k_u2_S<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->st>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac, 0,
+                                                                         SymCls{nullptr, nullptr, 0});
     LAUNCHCHK(c);
     if (dft2(c, c->tmpG3, c->NG, LG, +1, 1.0)) return -3;
     k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(S, c->tmpG3, 1.0, 1.0, c->nGel);
@@ -312,12 +336,25 @@ int sde_hartree(fdga_ctx* c, double2* S, const double2* G, double sign) {
     return 0;
 }
 
-int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* Ph) {
+// Lpp/Lph of one level; `zero` selects the reference system's groups SG0pp2/SG0ph2
+int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* Ph, bool zero) {
     const fdga_dims& d = c->d;
     const size_t nout = (size_t)c->NW2 * c->NV2 * c->NP;
     RefV ref = ref_view(c, false);
-    k_sde_L<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref, lv.kind,
-                                                       d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, lv.chanNP, cscale(c->U, d.T), box_bound(c));
+    const SymCls spp = symcls(c, zero ? SG_0PP2 : SG_PP2), sph = symcls(c, zero ? SG_0PH2 : SG_PH2);
+    const double2 TU = cscale(c->U, d.T);
+    const int NK = box_bound(c);
+    if (!spp.ncls && !sph.ncls) {
+        k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref, lv.kind,
+                                                              d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, lv.chanNP, TU, NK, spp);
+    } else {
+        k_sde_L<1><<<cdiv(nclass(spp, nout) * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref,
+                                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP,
+                                                                           lv.chanNP, TU, NK, spp);
+        k_sde_L<2><<<cdiv(nclass(sph, nout) * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref,
+                                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP,
+                                                                           lv.chanNP, TU, NK, sph);
+    }
     LAUNCHCHK(c);
     return 0;
 }
@@ -343,7 +380,7 @@ int sde_levels(fdga_ctx* c, double2* S, const double2* G, const double2* Pp, con
         const bool refl = lv[i].kind == 1;
         // a RefVertex with null tables contributes (U + 0) - U = 0 exactly
         if (!(refl && !c->tabs_nonzero)) {
-            if (sde_level_L(c, lv[i], Pp, Ph)) return -3;
+            if (sde_level_L(c, lv[i], Pp, Ph, start == 1)) return -3;
             if (sde_sigma(c, S, G, lv[i].scale)) return -3;
         }
         if (i == 0) {
@@ -369,12 +406,16 @@ int idx_fbase(int ch) { return ch == PCH ? 7 : ch == ACH ? 8 : 9; }
 
 }  // namespace
 
+// classes of the channel's K2 (SGpp[2] = SGppL[2] for p, SGph[2] = SGphL[2] for a and t)
+static SymCls sym_k2(const fdga_ctx* c, int ch) { return symcls(c, ch == PCH ? SG_PP2 : SG_PH2); }
+
 template <int CH>
 static int launch_lk2(fdga_ctx* c) {
-    const size_t nout = c->n2 * c->NP;
-    k_gemv_lk2<CH><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->FL[CH].K2, c->FL[ACH].K2, pi_of(c, CH, true), vtx_F(c),
-                                                              c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP, c->d.nonlocal,
-                                                              c->d.T * kSign[CH], box_bound(c));
+    const SymCls sym = sym_k2(c, CH);
+    const size_t nw = nclass(sym, c->n2 * c->NP);
+    k_gemv_lk2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->FL[CH].K2, c->FL[ACH].K2, pi_of(c, CH, true), vtx_F(c),
+                                                            c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP, c->d.nonlocal,
+                                                            c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
 }
@@ -390,9 +431,10 @@ static int launch_w(fdga_ctx* c, double2* W) {
 template <int CH>
 static int launch_k1(fdga_ctx* c) {
     if (launch_w<CH>(c, c->W)) return -3;
-    const size_t nout = c->n1 * c->NP;
-    k_gemv_k1<CH><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->W, vtx_F(c), c->d.nK1,
-                                                             c->d.nK1, c->nVP, c->NP, c->d.T * kSign[CH], box_bound(c));
+    const SymCls sym = symcls(c, SG_K1);
+    const size_t nw = nclass(sym, c->n1 * c->NP);
+    k_gemv_k1<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->W, vtx_F(c), c->d.nK1,
+                                                           c->d.nK1, c->nVP, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
 }
@@ -400,10 +442,11 @@ static int launch_k1(fdga_ctx* c) {
 template <int CH>
 static int launch_k2(fdga_ctx* c) {
     if (launch_w<CH>(c, c->W)) return -3;
-    const size_t nout = c->n2 * c->NP;
-    k_gemv_k2<CH><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->W, vtx_F(c), chanF(c, c->FL[CH]),
-                                                             chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
-                                                             c->NP, c->d.T * kSign[CH], box_bound(c));
+    const SymCls sym = sym_k2(c, CH);
+    const size_t nw = nclass(sym, c->n2 * c->NP);
+    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->W, vtx_F(c), chanF(c, c->FL[CH]),
+                                                           chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
+                                                           c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
 }
@@ -411,19 +454,42 @@ static int launch_k2(fdga_ctx* c) {
 // BSE_K2! of the a and t channels in one kernel (t needs the finished a channel)
 static int launch_k2_at(fdga_ctx* c) {
     if (launch_w<ACH>(c, c->W) || launch_w<TCH>(c, c->W2)) return -3;
-    const size_t nout = c->n2 * c->NP;
-    k_gemv_k2_at<<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->W, c->W2, vtx_F(c),
-                                                            chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]), c->d.nK2b, c->d.nK2f,
-                                                            c->d.nK1, c->nVP, c->NP, c->d.T, box_bound(c));
+    const SymCls sym = symcls(c, SG_PH2);   // a and t share SGph[2]
+    const size_t nw = nclass(sym, c->n2 * c->NP);
+    k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->W, c->W2, vtx_F(c),
+                                                          chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]), c->d.nK2b, c->d.nK2f,
+                                                          c->d.nK1, c->nVP, c->NP, c->d.T, box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
 }
 
-// BSE_L_K3! for `n` channels in one launch.  A single t channel is post-processed in the
-// epilogue (its a channel is complete); in a multi-channel launch t is post-processed afterwards.
+// classes of the channel's K3 (BSE_K3!: SGpp[3] / SGph[3]; BSE_L_K3!: SGppL[3] / SGphL[3]),
+// stored with device (K3 [ν, νp, Ω]) indices
+static SymCls sym_k3(const fdga_ctx* c, int ch, bool left) {
+    return symcls(c, ch == PCH ? (left ? SG_PPL3 : SG_PP3) : (left ? SG_PHL3 : SG_PH3));
+}
+
+// Symmetry fill of the K3 outputs (the GEMM evaluates every element; members are then replaced
+// by op(representative) as SG(K3, InitFunction(diagram)) does) and the t post-processing.
+static int k3_finish(fdga_ctx* c, DevChan* out, const int* chans, int n, bool left, bool t_in_epilogue) {
+    for (int s = 0; s < n; ++s) {
+        const SymCls sym = sym_k3(c, chans[s], left);
+        if (sym.ncls) k_sym_fill<<<cdiv(sym.ncls, 256), 256, 0, c->st>>>(out[chans[s]].K3, sym);
+    }
+    if (!t_in_epilogue)
+        for (int s = 0; s < n; ++s)
+            if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(out[TCH].K3, out[ACH].K3, c->n3 * c->NP);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+// BSE_L_K3! for `n` channels in one launch.  A single t channel without a symmetry group is
+// post-processed in the epilogue (its a channel is complete); otherwise t is post-processed
+// after the launch (and after the symmetry fill).
 static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     const int NV = c->NV3, NW = c->NW3;
     const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
+    const bool t_epi = (n == 1 && chans[0] == TCH && !sym_k3(c, TCH, true).ncls);
     LK3Chan cc[3] = {};
     for (int s = 0; s < n; ++s) {
         const int ch = chans[s];
@@ -433,7 +499,7 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
         cc[s].out = c->FL[ch].K3;
         cc[s].outa = c->FL[ACH].K3;
         cc[s].scale = c->d.T * ((ch == ACH) ? 1.0 : -1.0);   // src/BSE_templates.jl:122,130,138
-        cc[s].tpost = (ch == TCH && n == 1);
+        cc[s].tpost = (ch == TCH && t_epi);
     }
     const K3Geom g = k3geom(c);
     LK3_Aop a{g, cc[0], cc[1], cc[2]};
@@ -441,18 +507,14 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     LK3_Ep e{g, cc[0], cc[1], cc[2]};
     const size_t nb = (size_t)n * NW * c->NP * tm * tn;
     if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, a, b, e) != hipSuccess) return fail(c, "BSE_L_K3 GEMM launch failed", -3);
-    if (n > 1) {
-        for (int s = 0; s < n; ++s)
-            if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->FL[TCH].K3, c->FL[ACH].K3, c->n3 * c->NP);
-        LAUNCHCHK(c);
-    }
-    return 0;
+    return k3_finish(c, c->FL, chans, n, true, t_epi);
 }
 
 // BSE_K3! for `n` channels: right operands and column sums, then one GEMM launch
 static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     const int NV = c->NV3, NW = c->NW3;
     const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
+    const bool t_epi = (n == 1 && chans[0] == TCH && !sym_k3(c, TCH, false).ncls);
     const K3Geom g = k3geom(c);
     K3RChan rc[3] = {};
     K3Chan kc[3] = {};
@@ -481,7 +543,7 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
         kc[s].La = c->FL[ACH].K3;
         kc[s].out = c->Fb[ch].K3;
         kc[s].outa = c->Fb[ACH].K3;
-        kc[s].tpost = (ch == TCH && n == 1);
+        kc[s].tpost = (ch == TCH && t_epi);
     }
     const size_t ncol = (size_t)NV * NW * c->NP * n;
     k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->st>>>(g, rc[0], rc[1], rc[2], n);
@@ -491,12 +553,7 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
     const size_t nb = (size_t)n * NW * c->NP * tm * tn;
     if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, a, b, e) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
-    if (n > 1) {
-        for (int s = 0; s < n; ++s)
-            if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->Fb[TCH].K3, c->Fb[ACH].K3, c->n3 * c->NP);
-        LAUNCHCHK(c);
-    }
-    return 0;
+    return k3_finish(c, c->Fb, chans, n, false, t_epi);
 }
 
 // =========================================================================================
@@ -615,6 +672,7 @@ int fdga_destroy(fdga_ctx* c) {
         if (c->R0[r].K1) free_chan(c->R0[r]);
     }
     if (c->scal) (void)hipFree(c->scal);
+    for (auto& g : c->sg) free_sym(g);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -747,6 +805,64 @@ int fdga_init(fdga_ctx* c) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+int upload_sym(fdga_ctx* c, int id, const fdga_sym::Classes& cl) {
+    free_sym(c->sg[id]);
+    SymDev& g = c->sg[id];
+    HIPCHK(c, hipMalloc((void**)&g.off, cl.off.size() * sizeof(int)));
+    HIPCHK(c, hipMalloc((void**)&g.mem, cl.mem.size() * sizeof(int)));
+    HIPCHK(c, hipMemcpy(g.off, cl.off.data(), cl.off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(g.mem, cl.mem.data(), cl.mem.size() * sizeof(int), hipMemcpyHostToDevice));
+    g.ncls = cl.ncls();
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int fdga_init_sym_grp(fdga_ctx* c) {
+    if (!c) return -1;
+    const fdga_dims& d = c->d;
+    if (d.nonlocal)
+        return fail(c, "init_sym_grp!: the lattice symmetry groups of NL_ParquetSolver are not provided "
+                       "(the NL solver evaluates every element)");
+    using namespace fdga_sym;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const Axis aS{false, d.nG}, a1{true, d.nK1}, b2{true, d.nK2b}, f2{false, d.nK2f}, b3{true, d.nK3b}, f3{false, d.nK3f};
+    const int64_t NW = c->NW3, NV = c->NV3;
+    // reference K3 index iΩ + NW (iν + NV iν') -> device index iν + NV (iν' + NV iΩ)
+    auto k3dev = [NW, NV](int64_t r) {
+        const int64_t iW = r % NW, iv = (r / NW) % NV, ivp = r / (NW * NV);
+        return iv + NV * (ivp + NV * iW);
+    };
+    const Classes pp2 = build({b2, f2}, {g_k2_1, g_k2pp_2});
+    const Classes ph2 = build({b2, f2}, {g_k2_1, g_k2ph_2});
+    int rc = 0;
+    rc |= upload_sym(c, SG_S, build({aS}, {g_sigma}));
+    rc |= upload_sym(c, SG_K1, build({a1}, {g_k1}));
+    rc |= upload_sym(c, SG_PP2, pp2);
+    rc |= upload_sym(c, SG_PH2, ph2);
+    rc |= upload_sym(c, SG_PP3, build({b3, f3, f3}, {g_k3_1, g_k3_2, g_k3pp_3}, k3dev));
+    rc |= upload_sym(c, SG_PH3, build({b3, f3, f3}, {g_k3_1, g_k3_2, g_k3ph_3}, k3dev));
+    rc |= upload_sym(c, SG_PPL3, build({b3, f3, f3}, {g_k3_1, g_k3pp_3}, k3dev));
+    rc |= upload_sym(c, SG_PHL3, build({b3, f3, f3}, {g_k3_1, g_k3ph_3}, k3dev));
+    rc |= upload_sym(c, SG_0PP2, pp2);
+    rc |= upload_sym(c, SG_0PH2, ph2);
+    c->sde0_valid = 0;
+    return rc ? -2 : 0;
+}
+
+int fdga_reset_sym_grp(fdga_ctx* c) {
+    if (!c) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    // SGΣ, SGpp, SGph, SGppL, SGphL become identity groups; SG0pp2 / SG0ph2 are left as they are
+    for (int id : {SG_S, SG_K1, SG_PP2, SG_PH2, SG_PP3, SG_PH3, SG_PPL3, SG_PHL3}) free_sym(c->sg[id]);
+    c->sde0_valid = 0;
+    return 0;
+}
+
 int fdga_dyson(fdga_ctx* c) {
     k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->G, c->Sig, c->Gbare, c->nGel);
     LAUNCHCHK(c);
@@ -812,16 +928,17 @@ int fdga_update_F(fdga_ctx* c) {
 }
 
 int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
+    // SDE(F0, G0, Π0) of fdPA does not change between iterations: computed once (src/SDE.jl:19-20),
+    // before the target levels so that Lpp/Lph end up holding the target's (S.Lpp, S.Lph)
+    if (strategy == FDGA_FDPA && (!c->sde0_valid || c->sde0_h != include_hartree)) {
+        k_zero<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->sde0, c->nGel);
+        if (sde_levels(c, c->sde0, c->G0, c->Pi0pp, c->Pi0ph, 1, include_hartree)) return -3;
+        c->sde0_valid = 1;
+        c->sde0_h = include_hartree;
+    }
     k_zero<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->nGel);
     if (sde_levels(c, c->Sig, c->G, c->Pipp, c->Piph, 0, include_hartree)) return -3;
     if (strategy == FDGA_FDPA) {
-        // SDE(F0, G0, Π0) does not change between iterations: computed once (src/SDE.jl:19-20)
-        if (!c->sde0_valid || c->sde0_h != include_hartree) {
-            k_zero<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->sde0, c->nGel);
-            if (sde_levels(c, c->sde0, c->G0, c->Pi0pp, c->Pi0ph, 1, include_hartree)) return -3;
-            c->sde0_valid = 1;
-            c->sde0_h = include_hartree;
-        }
         k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->sde0, -1.0, 1.0, c->nGel);
         k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->Sig0, 1.0, 1.0, c->nGel);
         LAUNCHCHK(c);

@@ -28,6 +28,40 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// ---------------------------------------------------------------------------------------
+// Symmetry classes of an output array (SymmetryGroup of init_sym_grp!, src/ParquetSolver.jl:203-231).
+// Class k holds mem[off[k] .. off[k+1]) with mem = (linear index << 2) | op; its first member is
+// the representative (op 0).  op bit 0 negates, bit 1 conjugates.  ncls == 0 is the identity
+// group: every output is its own class, evaluated densely.
+struct SymCls {
+    const int* off;
+    const int* mem;
+    int ncls;
+    __device__ __forceinline__ int count(int nout) const { return ncls ? ncls : nout; }
+    __device__ __forceinline__ int rep(int w) const { return ncls ? (mem[off[w]] >> 2) : w; }
+    __device__ __forceinline__ void range(int w, int& b, int& e) const {
+        if (ncls) { b = off[w]; e = off[w + 1]; } else { b = 0; e = 1; }
+    }
+    __device__ __forceinline__ int member(int w, int k) const { return ncls ? mem[k] : (w << 2); }
+};
+__device__ __forceinline__ double2 sym_apply(int op, double2 v) {
+    if (op & 1) v = cmk(-v.x, -v.y);
+    if (op & 2) v.y = -v.y;
+    return v;
+}
+
+// out[member] = op(out[representative]) for every class (one thread per class)
+__global__ void k_sym_fill(double2* __restrict__ out, SymCls sym) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= sym.ncls) return;
+    const int b = sym.off[k], e = sym.off[k + 1];
+    const double2 v = out[sym.mem[b] >> 2];
+    for (int i = b + 1; i < e; ++i) {
+        const int code = sym.mem[i];
+        out[code >> 2] = sym_apply(code & 3, v);
+    }
+}
+
 // Dyson!: G = 1 / (1/Gbare + Σ)  (src/dyson.jl:5-31)
 __global__ void k_dyson(double2* __restrict__ G, const double2* __restrict__ S, const double2* __restrict__ Gb, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -565,16 +599,17 @@ __device__ __forceinline__ void far_pair(double2 t, double2& left, double2& righ
     right = cmk(__shfl(t.x, 1, WAVE), __shfl(t.y, 1, WAVE));
 }
 
-// BSE_K1!: K1(Ω, P) = T s Σ_ω F(Ω, ∞, ω, P, Ch, Sp) W(ω)   (one wave per output)
+// BSE_K1!: K1(Ω, P) = T s Σ_ω F(Ω, ∞, ω, P, Ch, Sp) W(ω)   (one wave per output class)
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a,
                                                  const double2* __restrict__ W, VtxV F, int n1, int nWP, int nVP, int NP,
-                                                 double scale, int NK) {
+                                                 double scale, int NK, SymCls sym) {
     constexpr int SP = ChSpin<CH>::SP;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW1 = 2 * n1 - 1;
-    if (wid >= NW1 * NP) return;
-    const int iW = wid % NW1, p = wid / NW1;
+    if (wid >= sym.count(NW1 * NP)) return;
+    const int o0 = sym.rep(wid);
+    const int iW = o0 % NW1, p = o0 / NW1;
     const int m = iW - (n1 - 1);
     const int Wc = window(NK, m, 0);
     const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
@@ -591,12 +626,16 @@ __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const
     sr = wave_sum(sr); si = wave_sum(si);
     lr = wave_sum(lr); li = wave_sum(li);
     rr = wave_sum(rr); ri = wave_sum(ri);
-    if (lane == 0) {
-        double2 acc = cmk(sr, si);
-        if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
-        if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
-        size_t o = (size_t)iW + (size_t)NW1 * p;
-        double2 v = cscale(acc, scale);
+    // every lane holds the same sums; lanes write the class members
+    double2 acc = cmk(sr, si);
+    if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
+    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
+    const double2 raw = cscale(acc, scale);
+    int b, e;
+    sym.range(wid, b, e);
+    for (int k = b + lane; k < e; k += WAVE) {
+        const int code = sym.member(wid, k), o = code >> 2;
+        double2 v = sym_apply(code & 3, raw);
         if (CH == TCH) v = cscale(cadd(v, K1a[o]), 0.5);
         K1[o] = v;
     }
@@ -611,11 +650,12 @@ __device__ __forceinline__ double2 bse_k2_term(const VtxV& F, int m, int v, int 
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a,
                                                  const double2* __restrict__ W, VtxV F, ChanV Lr, ChanV La, int n2b,
-                                                 int n2f, int nWP, int nVP, int NP, double scale, int NK) {
+                                                 int n2f, int nWP, int nVP, int NP, double scale, int NK, SymCls sym) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
-    if (wid >= NW2 * NV2 * NP) return;
-    const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
+    if (wid >= sym.count(NW2 * NV2 * NP)) return;
+    const int o0 = sym.rep(wid);
+    const int iW = o0 % NW2, iv = (o0 / NW2) % NV2, p = o0 / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
     const int Wc = window(NK, m, v);
     const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
@@ -633,12 +673,15 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
     sr = wave_sum(sr); si = wave_sum(si);
     lr = wave_sum(lr); li = wave_sum(li);
     rr = wave_sum(rr); ri = wave_sum(ri);
-    if (lane == 0) {
-        double2 acc = cmk(sr, si);
-        if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
-        if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
-        const size_t o = (size_t)wid;
-        double2 val = cscale(acc, scale);
+    double2 acc = cmk(sr, si);
+    if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
+    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
+    const double2 raw = cscale(acc, scale);
+    int b, e;
+    sym.range(wid, b, e);
+    for (int k = b + lane; k < e; k += WAVE) {
+        const int code = sym.member(wid, k), o = code >> 2;
+        double2 val = sym_apply(code & 3, raw);
         if (CH == TCH) {
             val = cadd(val, cscale(Lr.K2[o], 2.0));
             val = cadd(val, cscale(La.K2[o], -1.0));
@@ -657,11 +700,12 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
 __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, double2* __restrict__ K2t,
                                                     const double2* __restrict__ Wa, const double2* __restrict__ Wt,
                                                     VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP,
-                                                    int NP, double T, int NK) {
+                                                    int NP, double T, int NK, SymCls sym) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
-    if (wid >= NW2 * NV2 * NP) return;
-    const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
+    if (wid >= sym.count(NW2 * NV2 * NP)) return;
+    const int o0 = sym.rep(wid);
+    const int iW = o0 % NW2, iv = (o0 / NW2) % NV2, p = o0 / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
     const int Wc = window(NK, m, v);
     const size_t roff = (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
@@ -690,19 +734,22 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
     ar = wave_sum(ar); ai = wave_sum(ai); tr = wave_sum(tr); ti = wave_sum(ti);
     alr = wave_sum(alr); ali = wave_sum(ali); arr = wave_sum(arr); ari = wave_sum(ari);
     tlr = wave_sum(tlr); tli = wave_sum(tli); trr = wave_sum(trr); tri = wave_sum(tri);
-    if (lane == 0) {
-        double2 A = cmk(ar, ai), D = cmk(tr, ti);
-        if (lo > 0) {
-            A = cadd(A, cmul(fal, cmk(alr, ali)));
-            D = cadd(D, cmul(fdl, cmk(tlr, tli)));
-        }
-        if (hi < 2 * nVP - 1) {
-            A = cadd(A, cmul(far_, cmk(arr, ari)));
-            D = cadd(D, cmul(fdr, cmk(trr, tri)));
-        }
-        const size_t o = (size_t)wid;
-        const double2 ka = cadd(cscale(A, T), La.K2[o]);
-        double2 kt = cscale(D, -T);
+    double2 A = cmk(ar, ai), D = cmk(tr, ti);
+    if (lo > 0) {
+        A = cadd(A, cmul(fal, cmk(alr, ali)));
+        D = cadd(D, cmul(fdl, cmk(tlr, tli)));
+    }
+    if (hi < 2 * nVP - 1) {
+        A = cadd(A, cmul(far_, cmk(arr, ari)));
+        D = cadd(D, cmul(fdr, cmk(trr, tri)));
+    }
+    const double2 rawa = cscale(A, T), rawt = cscale(D, -T);
+    int b, e;
+    sym.range(wid, b, e);
+    for (int k = b + lane; k < e; k += WAVE) {
+        const int code = sym.member(wid, k), o = code >> 2;
+        const double2 ka = cadd(sym_apply(code & 3, rawa), La.K2[o]);
+        double2 kt = sym_apply(code & 3, rawt);
         kt = cadd(kt, cscale(Lt.K2[o], 2.0));
         kt = cadd(kt, cscale(La.K2[o], -1.0));
         K2a[o] = ka;
@@ -717,13 +764,14 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, const double2* __restrict__ outa,
                                                   const double2* __restrict__ Pi0, VtxV F, int n2b, int n2f, int nWP,
-                                                  int nVP, int NP, int nonlocal, double scale, int NK) {
+                                                  int nVP, int NP, int nonlocal, double scale, int NK, SymCls sym) {
     constexpr int SP = ChSpin<CH>::SP;
     constexpr int FLG = FL_ALL & ~FL_F0 & ~(FL_GP << CH);
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
-    if (wid >= NW2 * NV2 * NP) return;
-    const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
+    if (wid >= sym.count(NW2 * NV2 * NP)) return;
+    const int o0 = sym.rep(wid);
+    const int iW = o0 % NW2, iv = (o0 / NW2) % NV2, p = o0 / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
     const double2* P0 = Pi0 + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
     double sr = 0.0, si = 0.0;
@@ -744,33 +792,45 @@ __global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, con
     }
     sr = wave_sum(sr);
     si = wave_sum(si);
-    if (lane == 0) {
-        double2 val = cscale(cmk(sr, si), scale);
-        if (CH == TCH) val = cscale(cadd(val, outa[wid]), 0.5);
-        out[wid] = val;
+    const double2 raw = cscale(cmk(sr, si), scale);
+    int b, e;
+    sym.range(wid, b, e);
+    for (int k = b + lane; k < e; k += WAVE) {
+        const int code = sym.member(wid, k), o = code >> 2;
+        double2 val = sym_apply(code & 3, raw);
+        if (CH == TCH) val = cscale(cadd(val, outa[o]), 0.5);
+        out[o] = val;
     }
 }
 
 // ---------------------------------------------------------------------------------------
 // SDE_channel_L_pp! / _ph! (src/SDE.jl:79-217; NL src/nonlocal/SDE.jl:3-142).
 // kind 0: Vertex channels g (momentum p if NP > 1); kind 1: RefVertex tables.
+template <bool DP, bool DH>
 __device__ __forceinline__ void sde_terms(const ChanV& gp, const ChanV& gt, const ChanV& ga, const RefV& ref, int kind,
                                  int m, int v, int w, int pc, double2& xp, double2& xh) {
+    xp = czero();
+    xh = czero();
     if (kind == 0) {
-        xp = gp.eval(m, bmf(m, w), v, pc);
-        xh = cadd(ga.eval(m, v, w, pc), gt.eval(m, v, w, pc));
+        if (DP) xp = gp.eval(m, bmf(m, w), v, pc);
+        if (DH) xh = cadd(ga.eval(m, v, w, pc), gt.eval(m, v, w, pc));
     } else {
-        xp = csub(ref.eval(m, bmf(m, w), v, PCH, PSP), ref.bare(PSP));
-        xh = csub(csub(cadd(ref.eval(m, v, w, ACH, PSP), ref.eval(m, v, w, TCH, PSP)), ref.bare(PSP)), ref.bare(PSP));
+        if (DP) xp = csub(ref.eval(m, bmf(m, w), v, PCH, PSP), ref.bare(PSP));
+        if (DH) xh = csub(csub(cadd(ref.eval(m, v, w, ACH, PSP), ref.eval(m, v, w, TCH, PSP)), ref.bare(PSP)), ref.bare(PSP));
     }
 }
+// WHICH: bit 0 computes Lpp, bit 1 Lph.  Under a symmetry group the two outputs have their own
+// classes (SGpp2, SGph2), so each is launched separately over its classes.
+template <int WHICH>
 __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double2* __restrict__ Lph, const double2* __restrict__ Pipp,
                         const double2* __restrict__ Piph, ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
-                        int n2b, int n2f, int nWP, int nVP, int NP, int chanNP, double2 TU, int NK) {
+                        int n2b, int n2f, int nWP, int nVP, int NP, int chanNP, double2 TU, int NK, SymCls sym) {
+    constexpr bool DP = WHICH & 1, DH = WHICH & 2;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
-    if (wid >= NW2 * NV2 * NP) return;
-    const int iW = wid % NW2, iv = (wid / NW2) % NV2, p = wid / (NW2 * NV2);
+    if (wid >= sym.count(NW2 * NV2 * NP)) return;
+    const int o0 = sym.rep(wid);
+    const int iW = o0 % NW2, iv = (o0 / NW2) % NV2, p = o0 / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
     const int pc = chanNP > 1 ? p : 0;
     const int Wc = window(NK, m, v);
@@ -780,47 +840,51 @@ __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double
     double plr = 0.0, pli = 0.0, prr = 0.0, pri = 0.0, hlr = 0.0, hli = 0.0, hrr = 0.0, hri = 0.0;
     for (int iw = lo + lane; iw <= hi; iw += WAVE) {
         double2 xp, xh;
-        sde_terms(gp, gt, ga, ref, kind, m, v, iw - nVP, pc, xp, xh);
-        double2 a = cmul(Pipp[base + iw], xp);
-        double2 b = cmul(Piph[base + iw], xh);
-        ar += a.x; ai += a.y; br += b.x; bi += b.y;
+        sde_terms<DP, DH>(gp, gt, ga, ref, kind, m, v, iw - nVP, pc, xp, xh);
+        if (DP) { double2 a = cmul(Pipp[base + iw], xp); ar += a.x; ai += a.y; }
+        if (DH) { double2 b = cmul(Piph[base + iw], xh); br += b.x; bi += b.y; }
     }
     for (int iw = lane; iw < lo; iw += WAVE) {
-        plr += Pipp[base + iw].x; pli += Pipp[base + iw].y;
-        hlr += Piph[base + iw].x; hli += Piph[base + iw].y;
+        if (DP) { plr += Pipp[base + iw].x; pli += Pipp[base + iw].y; }
+        if (DH) { hlr += Piph[base + iw].x; hli += Piph[base + iw].y; }
     }
     for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) {
-        prr += Pipp[base + iw].x; pri += Pipp[base + iw].y;
-        hrr += Piph[base + iw].x; hri += Piph[base + iw].y;
+        if (DP) { prr += Pipp[base + iw].x; pri += Pipp[base + iw].y; }
+        if (DH) { hrr += Piph[base + iw].x; hri += Piph[base + iw].y; }
     }
     double2 fxp, fxh, xpl, xpr, xhl, xhr;
-    sde_terms(gp, gt, ga, ref, kind, m, v, lane == 0 ? -Wc - 1 : Wc + 1, pc, fxp, fxh);
+    sde_terms<DP, DH>(gp, gt, ga, ref, kind, m, v, lane == 0 ? -Wc - 1 : Wc + 1, pc, fxp, fxh);
     far_pair(fxp, xpl, xpr);
     far_pair(fxh, xhl, xhr);
     ar = wave_sum(ar); ai = wave_sum(ai); br = wave_sum(br); bi = wave_sum(bi);
     plr = wave_sum(plr); pli = wave_sum(pli); prr = wave_sum(prr); pri = wave_sum(pri);
     hlr = wave_sum(hlr); hli = wave_sum(hli); hrr = wave_sum(hrr); hri = wave_sum(hri);
-    if (lane == 0) {
-        double2 A = cmk(ar, ai), B = cmk(br, bi);
-        if (lo > 0) {
-            A = cadd(A, cmul(cmk(plr, pli), xpl));
-            B = cadd(B, cmul(cmk(hlr, hli), xhl));
-        }
-        if (hi < 2 * nVP - 1) {
-            A = cadd(A, cmul(cmk(prr, pri), xpr));
-            B = cadd(B, cmul(cmk(hrr, hri), xhr));
-        }
-        Lpp[wid] = cmul(A, TU);
-        Lph[wid] = cmul(B, TU);
+    double2 A = cmk(ar, ai), B = cmk(br, bi);
+    if (lo > 0) {
+        A = cadd(A, cmul(cmk(plr, pli), xpl));
+        B = cadd(B, cmul(cmk(hlr, hli), xhl));
+    }
+    if (hi < 2 * nVP - 1) {
+        A = cadd(A, cmul(cmk(prr, pri), xpr));
+        B = cadd(B, cmul(cmk(hrr, hri), xhr));
+    }
+    const double2 rawp = cmul(A, TU), rawh = cmul(B, TU);
+    int b, e;
+    sym.range(wid, b, e);
+    for (int k = b + lane; k < e; k += WAVE) {
+        const int code = sym.member(wid, k), o = code >> 2;
+        if (DP) Lpp[o] = sym_apply(code & 3, rawp);
+        if (DH) Lph[o] = sym_apply(code & 3, rawh);
     }
 }
 
 // Σ(ν) += scale T Σ_Ω [G(Ω-ν) Lpp(Ω,ν) + G(Ω+ν) Lph(Ω,ν)]  (src/SDE.jl:242-263); one wave per ν
 __global__ __launch_bounds__(256) void k_sde_sigma_local(double2* __restrict__ S, const double2* __restrict__ G,
                                                          const double2* __restrict__ Lpp, const double2* __restrict__ Lph,
-                                                         int nG, int n2b, int n2f, double T, double scale) {
-    const int iv = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
-    if (iv >= 2 * nG) return;
+                                                         int nG, int n2b, int n2f, double T, double scale, SymCls sym) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid >= sym.count(2 * nG)) return;
+    const int iv = sym.rep(wid);
     const int v = iv - nG;
     if (!in_f(v, n2f)) return;
     const int NW2 = 2 * n2b - 1;
@@ -835,7 +899,14 @@ __global__ __launch_bounds__(256) void k_sde_sigma_local(double2* __restrict__ S
         sr += s.x; si += s.y;
     }
     sr = wave_sum(sr); si = wave_sum(si);
-    if (lane == 0) S[iv] = cadd(S[iv], cscale(cmk(sr, si), T * scale));
+    // a class of the Σ group ({ν, -ν}) lies inside or outside the K2 box as a whole
+    const double2 raw = cscale(cmk(sr, si), T * scale);
+    int b, e;
+    sym.range(wid, b, e);
+    for (int k = b + lane; k < e; k += WAVE) {
+        const int code = sym.member(wid, k), o = code >> 2;
+        S[o] = cadd(S[o], sym_apply(code & 3, raw));
+    }
 }
 
 // NL SDE_compute_inner! real-space step (src/nonlocal/SDE.jl:209-276): for output column q of
@@ -905,11 +976,13 @@ __global__ __launch_bounds__(256) void k_u2_C(double2* __restrict__ C, const dou
     if (lane == 0) C[(size_t)im + (size_t)NB * r] = cmk(sr, si);
 }
 __global__ __launch_bounds__(256) void k_u2_S(double2* __restrict__ S, const double2* __restrict__ C,
-                                              const double2* __restrict__ Gp, int nG, int nR, double2 fac, int accumulate) {
+                                              const double2* __restrict__ Gp, int nG, int nR, double2 fac, int accumulate,
+                                              SymCls sym) {
     const int NG = 2 * nG, NB = 4 * nG - 1;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
-    if (wid >= NG * nR) return;
-    const int iv = wid % NG, r = wid / NG;
+    if (wid >= sym.count(NG * nR)) return;
+    const int o0 = sym.rep(wid);
+    const int iv = o0 % NG, r = o0 / NG;
     const int v = iv - nG;
     double sr = 0.0, si = 0.0;
     for (int im = lane; im < NB; im += WAVE) {
@@ -920,9 +993,12 @@ __global__ __launch_bounds__(256) void k_u2_S(double2* __restrict__ S, const dou
         }
     }
     sr = wave_sum(sr); si = wave_sum(si);
-    if (lane == 0) {
-        double2 s = cmul(cmk(sr, si), fac);
-        size_t o = (size_t)iv + (size_t)NG * r;
+    const double2 raw = cmul(cmk(sr, si), fac);
+    int b, e;
+    sym.range(wid, b, e);
+    for (int k = b + lane; k < e; k += WAVE) {
+        const int code = sym.member(wid, k), o = code >> 2;
+        const double2 s = sym_apply(code & 3, raw);
         S[o] = accumulate ? cadd(S[o], s) : s;
     }
 }

@@ -54,6 +54,8 @@ SIGNATURES = {
     "fdga_set": (ctypes.c_int, [_vp, _i32, _dp]),
     "fdga_get": (ctypes.c_int, [_vp, _i32, _dp]),
     "fdga_init": (ctypes.c_int, [_vp]),
+    "fdga_init_sym_grp": (ctypes.c_int, [_vp]),
+    "fdga_reset_sym_grp": (ctypes.c_int, [_vp]),
     "fdga_dyson": (ctypes.c_int, [_vp]),
     "fdga_bubbles": (ctypes.c_int, [_vp]),
     "fdga_build_k3_cache": (ctypes.c_int, [_vp]),

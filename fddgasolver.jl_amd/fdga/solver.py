@@ -118,6 +118,14 @@ class Solver:
         self._check(self.lib.fdga_init(self.h))
 
     # ------------------------------------------------------------------ timing
+    def init_sym_grp(self):
+        """init_sym_grp!(S) (src/ParquetSolver.jl:203-231): frequency symmetry groups (local solver)."""
+        self._check(self.lib.fdga_init_sym_grp(self.h))
+
+    def reset_sym_grp(self):
+        """reset_sym_grp!(S) (src/ParquetSolver.jl:233-243)."""
+        self._check(self.lib.fdga_reset_sym_grp(self.h))
+
     def time_iterations(self, strategy, n):
         ms = ctypes.c_double(0.0)
         self._check(self.lib.fdga_time_iterations(self.h, _strategy(strategy), n, ctypes.byref(ms)))
@@ -280,6 +288,14 @@ def update_F(S):
 
 def SDE(S, strategy="scPA", include_Hartree=True):
     S._check(S.lib.fdga_sde(S.h, _strategy(strategy), int(include_Hartree)))
+
+
+def init_sym_grp(S):
+    S.init_sym_grp()
+
+
+def reset_sym_grp(S):
+    S.reset_sym_grp()
 
 
 def iterate_solver(S, strategy="fdPA", update_Σ=True):

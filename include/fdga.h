@@ -63,6 +63,15 @@ int         fdga_get(fdga_ctx* ctx, int32_t which, double* host);
 /* Π0 = bubbles(G0); Σ = Σ0; G = Dyson(Σ, Gbare); Π = bubbles(G)   (src/ParquetSolver.jl:97-114) */
 int         fdga_init(fdga_ctx* ctx);
 
+/* symmetry groups of the local solver: init_sym_grp!(S) (src/ParquetSolver.jl:203-231) installs the
+ * frequency symmetry groups of src/symmetries.jl:1-94; afterwards the BSE, L and SDE stages evaluate
+ * one representative per class and write op(value) to the other members, as SG(f, InitFunction)
+ * does.  reset_sym_grp!(S) (src/ParquetSolver.jl:233-243) restores the identity groups (SG0pp2 and
+ * SG0ph2 are left installed, as there).  The NL solver's lattice groups are not provided: the call
+ * fails for NL_ParquetSolver. */
+int fdga_init_sym_grp(fdga_ctx* ctx);
+int fdga_reset_sym_grp(fdga_ctx* ctx);
+
 /* stages of iterate_solver! (src/solve.jl:4-116) */
 int fdga_dyson(fdga_ctx* ctx);                          /* Dyson!(S)          src/dyson.jl:1-31 */
 int fdga_bubbles(fdga_ctx* ctx);                        /* bubbles!(S)        src/bubble.jl:1-36, src/nonlocal/bubble.jl:87-158 */
