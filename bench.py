@@ -14,8 +14,10 @@ fixed point; the max relative error of Σ, χ_sp, χ_ch and K3 between the two i
 (`config.value_path` says which).
 
 Prints one JSON line.  `roofline` is for the BSE contraction kernel of the headline path
-(BSE_K3! + BSE_L_K3!, batched complex GEMM on MFMA): algorithmic flops per launch
-8 NΩ3 Nν3^3 (one channel, one GEMM) over its measured average launch time.  `cpu_baseline` times the CPU restatement
+(BSE_K3! + BSE_L_K3!, batched complex GEMM on MFMA): flops per launch = 8 * 64 * 64 * Nν3 per
+computed 64 x 64 output tile (all tiles, 3 NΩ3 (Nν3/64)^2, with identity groups; the tiles that
+hold a class representative under init_sym_grp!) over the measured average launch time.
+`bse_tflops_dense_equiv` divides the dense count 8 NΩ3 Nν3^3 per channel by the same time.  `cpu_baseline` times the CPU restatement
 (oracle/, one thread) on one iteration of the same workload.
 """
 import argparse
@@ -108,6 +110,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8, help="Anderson tolerance of the untimed FP64 set-up solve")
+    ap.add_argument("--no-sym", action="store_true",
+                    help="identity symmetry groups (every element evaluated) instead of init_sym_grp!")
     ap.add_argument("--tol32", type=float, default=1e-7,
                     help="Anderson tolerance of the FP32-MFMA solve (its residual floor is ~5e-8 at this workload)")
     args = ap.parse_args()
@@ -118,12 +122,14 @@ def main():
     import fdga
     w = WORKLOAD
     names = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE"]
-    f_gemm = flops_bse_k3(w)
+    f_gemm = flops_bse_k3(w)   # algorithmic flops of one dense channel GEMM
     paths = {}
     x64 = None
     for prec in ("fp64", "fp32"):
         log(f"bench[{prec}]: building SIAM fdPA solver nK1={w['nK1']} nK2={w['nK2']} nK3={w['nK3']} nG={w['nG']}")
         S = build_solver(w, prec)
+        if not args.no_sym:
+            S.init_sym_grp()
         if x64 is not None:
             fdga.unflatten(S, x64)   # the FP32 path starts from the FP64 fixed point
         t0 = time.perf_counter()
@@ -136,8 +142,18 @@ def main():
         ms_step, st = time_path(S, args, names)
         log(f"bench[{prec}]: {ms_step:.3f} ms/iteration; stages " + ", ".join(f"{n}={v:.3f}" for n, v in st.items()))
         t_bse = (st["L_K3"] + st["K3"]) * 1e-3          # 6 GEMM launches per fdPA iteration
-        paths[prec] = dict(ms=ms_step, st=st, obs=obs, it=it, res=res,
-                           gemm_avg_s=t_bse / 6.0, bse_tflops=6 * f_gemm / t_bse / 1e12)
+        # flops the GEMM launches execute: computed 64 x 64 tiles over the full contraction length
+        nblk = S.lib.fdga_gemm_blocks(S.h, 1) + S.lib.fdga_gemm_blocks(S.h, 0)
+        f_exec = nblk * 8.0 * 64 * 64 * (2 * w["nK3"][1])
+        paths[prec] = dict(ms=ms_step, st=st, obs=obs, it=it, res=res, gemm_avg_s=t_bse / 6.0,
+                           f_exec_launch=f_exec / 6.0, bse_tflops=f_exec / t_bse / 1e12,
+                           bse_tflops_dense_equiv=6 * f_gemm / t_bse / 1e12)
+        if prec == "fp64" and not args.no_sym:
+            # the same state with identity groups: every element evaluated
+            S.reset_sym_grp()
+            ms_d, st_d = time_path(S, args, names)
+            log(f"bench[fp64, dense]: {ms_d:.3f} ms/iteration")
+            paths["fp64_dense"] = dict(ms=ms_d, st=st_d)
         S.close()
     errs = {k: max_relerr(paths["fp32"]["obs"][k], paths["fp64"]["obs"][k]) for k in paths["fp64"]["obs"]}
     log("bench: FP32-MFMA vs FP64 max rel.err " + ", ".join(f"{k}={v:.2e}" for k, v in errs.items()))
@@ -145,7 +161,7 @@ def main():
     head = "fp32" if fp32_ok else "fp64"
     P = paths[head]
     peak = FP32_MFMA_PEAK if head == "fp32" else FP64_MFMA_PEAK
-    achieved = f_gemm / P["gemm_avg_s"]
+    achieved = P["f_exec_launch"] / P["gemm_avg_s"]
     out = {
         "metric": "parquet iters/s + BSE TFLOP/s, FP32-MFMA vs FP64 GPU @ rel.err<=1e-6 vs FP64",
         "value": 1000.0 / P["ms"],
@@ -160,11 +176,14 @@ def main():
         "dtype": "f32-mfma contraction, f64 elsewhere" if head == "fp32" else "f64",
         "data": "synthetic (SIAM model inputs; solver state at its fdPA fixed point)",
         "config": {"workload": "local SIAM fdPA, U=2, beta=20, nK1=128, nK2=(64,64), nK3=(32,64), mPinu=6, nG=256",
+                   "symmetry_groups": "identity" if args.no_sym else "init_sym_grp! (frequency symmetries)",
                    "value_path": head,
                    "fp32_mfma": {"iters_per_s": 1000.0 / paths["fp32"]["ms"], "bse_tflops": paths["fp32"]["bse_tflops"],
+                                 "bse_tflops_dense_equiv": paths["fp32"]["bse_tflops_dense_equiv"],
                                  "stage_ms": paths["fp32"]["st"], "anderson_iterations": paths["fp32"]["it"],
                                  "max_relerr_vs_fp64": errs},
                    "fp64": {"iters_per_s": 1000.0 / paths["fp64"]["ms"], "bse_tflops": paths["fp64"]["bse_tflops"],
+                            "bse_tflops_dense_equiv": paths["fp64"]["bse_tflops_dense_equiv"],
                             "stage_ms": paths["fp64"]["st"], "anderson_iterations": paths["fp64"]["it"]},
                    "speedup_fp32_over_fp64": paths["fp64"]["ms"] / paths["fp32"]["ms"]},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
@@ -174,6 +193,9 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline (oracle, one thread)")
         out["cpu_baseline"] = cpu_baseline()
+    if "fp64_dense" in paths:
+        out["config"]["fp64_dense"] = {"iters_per_s": 1000.0 / paths["fp64_dense"]["ms"],
+                                       "stage_ms": paths["fp64_dense"]["st"]}
     print(json.dumps(out), flush=True)
 
 

@@ -33,6 +33,7 @@ struct SymDev {
     int* off = nullptr;
     int* mem = nullptr;
     int ncls = 0;
+    std::vector<int> tiles;   // K3 groups: GEMM blocks (iΩ * tiles + tile) holding a representative
 };
 // groups of init_sym_grp! (src/ParquetSolver.jl:203-231): SGΣ; SGpp[1] = SGph[1] (same generator);
 // SGpp[2] = SGppL[2]; SGph[2] = SGphL[2]; SGpp[3], SGph[3], SGppL[3], SGphL[3]; SG0pp2, SG0ph2
@@ -66,6 +67,10 @@ struct fdga_ctx {
     double2 *stage = nullptr;                                       // staging for layout changes
     size_t stage_n = 0;
     SymDev sg[SG_N];                                                // installed symmetry groups
+    // GEMM block lists under the K3 groups: [0..3] one channel of SG_PP3, SG_PH3, SG_PPL3, SG_PHL3;
+    // [4] BSE_L_K3! and [5] BSE_K3! over the channels (p, a, t) in one launch
+    int* blist[6] = {};
+    int blist_n[6] = {};
     // timing
     int timing = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -98,6 +103,14 @@ void free_sym(SymDev& g) {
     if (g.off) (void)hipFree(g.off);
     if (g.mem) (void)hipFree(g.mem);
     g = SymDev{};
+}
+
+void free_blists(fdga_ctx* c) {
+    for (int k = 0; k < 6; ++k) {
+        if (c->blist[k]) (void)hipFree(c->blist[k]);
+        c->blist[k] = nullptr;
+        c->blist_n[k] = 0;
+    }
 }
 
 int dalloc(fdga_ctx* c, double2** p, size_t n) {
@@ -469,6 +482,19 @@ static SymCls sym_k3(const fdga_ctx* c, int ch, bool left) {
     return symcls(c, ch == PCH ? (left ? SG_PPL3 : SG_PP3) : (left ? SG_PHL3 : SG_PH3));
 }
 
+// GEMM block list of a K3 launch over `n` channels: the precomputed list when every slot has a
+// group and the channels are one channel or (p, a, t); null (all blocks) otherwise
+static const int* k3_blocks(const fdga_ctx* c, const int* chans, int n, bool left, size_t& nblocks) {
+    for (int s = 0; s < n; ++s)
+        if (!sym_k3(c, chans[s], left).ncls) return nullptr;
+    int k = -1;
+    if (n == 1) k = (chans[0] == PCH ? 0 : 1) + (left ? 2 : 0);
+    else if (n == 3 && chans[0] == PCH && chans[1] == ACH && chans[2] == TCH) k = left ? 4 : 5;
+    if (k < 0 || !c->blist[k]) return nullptr;
+    nblocks = (size_t)c->blist_n[k];
+    return c->blist[k];
+}
+
 // Symmetry fill of the K3 outputs (the GEMM evaluates every element; members are then replaced
 // by op(representative) as SG(K3, InitFunction(diagram)) does) and the t post-processing.
 static int k3_finish(fdga_ctx* c, DevChan* out, const int* chans, int n, bool left, bool t_in_epilogue) {
@@ -505,8 +531,9 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     LK3_Aop a{g, cc[0], cc[1], cc[2]};
     LK3_Bop b{g, cc[0], cc[1], cc[2]};
     LK3_Ep e{g, cc[0], cc[1], cc[2]};
-    const size_t nb = (size_t)n * NW * c->NP * tm * tn;
-    if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, a, b, e) != hipSuccess) return fail(c, "BSE_L_K3 GEMM launch failed", -3);
+    size_t nb = (size_t)n * NW * c->NP * tm * tn;
+    const int* bl = k3_blocks(c, chans, n, true, nb);
+    if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, bl, a, b, e) != hipSuccess) return fail(c, "BSE_L_K3 GEMM launch failed", -3);
     return k3_finish(c, c->FL, chans, n, true, t_epi);
 }
 
@@ -551,8 +578,9 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     K3_Aop a{g, c->d.T, kc[0], kc[1], kc[2]};
     K3_Bop b{g, c->d.T, kc[0], kc[1], kc[2]};
     K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
-    const size_t nb = (size_t)n * NW * c->NP * tm * tn;
-    if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, a, b, e) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
+    size_t nb = (size_t)n * NW * c->NP * tm * tn;
+    const int* bl = k3_blocks(c, chans, n, false, nb);
+    if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, bl, a, b, e) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     return k3_finish(c, c->Fb, chans, n, false, t_epi);
 }
 
@@ -673,6 +701,7 @@ int fdga_destroy(fdga_ctx* c) {
     }
     if (c->scal) (void)hipFree(c->scal);
     for (auto& g : c->sg) free_sym(g);
+    free_blists(c);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -850,6 +879,42 @@ int fdga_init_sym_grp(fdga_ctx* c) {
     rc |= upload_sym(c, SG_PHL3, build({b3, f3, f3}, {g_k3_1, g_k3ph_3}, k3dev));
     rc |= upload_sym(c, SG_0PP2, pp2);
     rc |= upload_sym(c, SG_0PH2, ph2);
+    if (rc) return -2;
+    // GEMM blocks holding a representative (block = iΩ * tiles + ti + tm * tj, rows ν, columns ν')
+    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN), tiles = tm * tn;
+    for (int id : {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3}) {
+        std::vector<char> hit((size_t)NW * tiles, 0);
+        SymDev& g = c->sg[id];
+        std::vector<int> off(g.ncls + 1), mem;
+        HIPCHK(c, hipMemcpy(off.data(), g.off, off.size() * sizeof(int), hipMemcpyDeviceToHost));
+        mem.resize(off.back());
+        HIPCHK(c, hipMemcpy(mem.data(), g.mem, mem.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int k = 0; k < g.ncls; ++k) {
+            const int64_t dv = mem[off[k]] >> 2;
+            const int iv = (int)(dv % NV), ivp = (int)((dv / NV) % NV), iW = (int)(dv / (NV * NV));
+            hit[(size_t)iW * tiles + iv / GBM + tm * (ivp / GBN)] = 1;
+        }
+        g.tiles.clear();
+        for (size_t b = 0; b < hit.size(); ++b)
+            if (hit[b]) g.tiles.push_back((int)b);
+    }
+    free_blists(c);
+    auto upload_list = [&](int k, const std::vector<int>& v) -> int {
+        HIPCHK(c, hipMalloc((void**)&c->blist[k], std::max<size_t>(1, v.size()) * sizeof(int)));
+        HIPCHK(c, hipMemcpy(c->blist[k], v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+        c->blist_n[k] = (int)v.size();
+        return 0;
+    };
+    const int ids1[4] = {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3};
+    for (int k = 0; k < 4; ++k) rc |= upload_list(k, c->sg[ids1[k]].tiles);
+    for (int left = 1; left >= 0; --left) {
+        // slots (p, a, t) of the batched launches
+        const int ids3[3] = {left ? SG_PPL3 : SG_PP3, left ? SG_PHL3 : SG_PH3, left ? SG_PHL3 : SG_PH3};
+        std::vector<int> all;
+        for (int sl = 0; sl < 3; ++sl)
+            for (int b : c->sg[ids3[sl]].tiles) all.push_back(b + sl * (int)NW * tiles);
+        rc |= upload_list(left ? 4 : 5, all);
+    }
     c->sde0_valid = 0;
     return rc ? -2 : 0;
 }
@@ -859,6 +924,7 @@ int fdga_reset_sym_grp(fdga_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     // SGΣ, SGpp, SGph, SGppL, SGphL become identity groups; SG0pp2 / SG0ph2 are left as they are
     for (int id : {SG_S, SG_K1, SG_PP2, SG_PH2, SG_PP3, SG_PH3, SG_PPL3, SG_PHL3}) free_sym(c->sg[id]);
+    free_blists(c);
     c->sde0_valid = 0;
     return 0;
 }
@@ -1082,12 +1148,22 @@ int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, cons
         hipMemcpy(dB, B, nb * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
     if (!rc) {
         const int tm = cdiv(M, GBM), tn = cdiv(N, GBN);
-        if (zgemm_launch(precision, (size_t)(tm * tn), 0, M, N, K, tm, tn, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M}) != hipSuccess ||
+        if (zgemm_launch(precision, (size_t)(tm * tn), 0, M, N, K, tm, tn, nullptr, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M}) != hipSuccess ||
             hipDeviceSynchronize() != hipSuccess) rc = -3;
     }
     if (!rc && hipMemcpy(C, dC, nc * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = -2;
     (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);
     return rc;
+}
+
+// GEMM blocks (GBM x GBN output tiles over the full contraction length) of the batched (p, a, t)
+// BSE_L_K3! (left = 1) or BSE_K3! (left = 0) launch of one iteration
+int64_t fdga_gemm_blocks(const fdga_ctx* c, int32_t left) {
+    if (!c) return -1;
+    const int chans[3] = {PCH, ACH, TCH};
+    size_t nb = (size_t)3 * c->NW3 * c->NP * cdiv(c->NV3, GBM) * cdiv(c->NV3, GBN);
+    (void)k3_blocks(c, chans, 3, left != 0, nb);
+    return (int64_t)nb;
 }
 
 // enable (1) / disable (0) per-stage event timing; resets the accumulated stage times

@@ -274,14 +274,16 @@ template <> struct MfmaT<true> {
 };
 
 template <bool F32, class AL, class BL, class EP>
-__global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int tiles_m, int tiles_n, AL al, BL bl, EP ep) {
+__global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int tiles_m, int tiles_n,
+                                                    const int* __restrict__ blist, AL al, BL bl, EP ep) {
     typedef MfmaT<F32> MT;
     typedef typename MT::S S;
     typedef typename MT::Acc Acc;
     __shared__ S As[2][GBK][GBM + GPAD];
     __shared__ S Bs[2][GBK][GBN + GPAD];
     const int tiles = tiles_m * tiles_n;
-    const int bid = blockIdx.x;
+    // block = batch * tiles + tile; with a block list only the listed tiles are computed
+    const int bid = blist ? blist[blockIdx.x] : (int)blockIdx.x;
     const int batch = bid / tiles;
     const int t = bid - batch * tiles;
     const int m0 = (t % tiles_m) * GBM, n0 = (t / tiles_m) * GBN;
@@ -354,12 +356,14 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
             }
 }
 
-// launch helper: precision 0 = FP64 MFMA, 1 = FP32 MFMA
+// launch helper: precision 0 = FP64 MFMA, 1 = FP32 MFMA; blist (nblocks entries) selects the
+// blocks to compute, null = all
 template <class AL, class BL, class EP>
 inline hipError_t zgemm_launch(int precision, size_t nblocks, hipStream_t st, int M, int N, int K, int tm, int tn,
-                               const AL& a, const BL& b, const EP& e) {
-    if (precision == 1) zgemm_tiles_t<true><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, a, b, e);
-    else zgemm_tiles_t<false><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, a, b, e);
+                               const int* blist, const AL& a, const BL& b, const EP& e) {
+    if (nblocks == 0) return hipSuccess;
+    if (precision == 1) zgemm_tiles_t<true><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, blist, a, b, e);
+    else zgemm_tiles_t<false><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, blist, a, b, e);
     return hipGetLastError();
 }
 

@@ -101,6 +101,10 @@ int fdga_stage_ms(const fdga_ctx* ctx, int32_t stage, double* ms);
 /* enable (1) / disable (0) per-stage event timing inside fdga_iterate (adds a host sync per
  * stage); resets the accumulated stage times */
 int fdga_set_stage_timing(fdga_ctx* ctx, int32_t on);
+/* number of 64 x 64 output tiles (each over the full contraction length) that the batched (p, a, t)
+ * BSE_L_K3! (left = 1) or BSE_K3! (left = 0) GEMM launch computes; fewer than the dense count
+ * when symmetry groups are installed.  No counterpart in the reference (measurement). */
+int64_t fdga_gemm_blocks(const fdga_ctx* ctx, int32_t left);
 
 /* Self-test of the batched complex GEMM engine used by BSE_L_K3!/BSE_K3!: C = A B for dense
  * column-major M x K and K x N ComplexF64 matrices on the FP64 (precision 0) or FP32 (1) MFMA path.  No counterpart in the reference; it exists
