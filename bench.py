@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8, help="Anderson tolerance of the untimed FP64 set-up solve")
     ap.add_argument("--no-sym", action="store_true",
                     help="identity symmetry groups (every element evaluated) instead of init_sym_grp!")
+    ap.add_argument("--paths", default="fp64,fp32", help="precision paths to run (profiling runs select one)")
+    ap.add_argument("--no-dense", action="store_true", help="skip the identity-group FP64 timing")
     ap.add_argument("--tol32", type=float, default=1e-7,
                     help="Anderson tolerance of the FP32-MFMA solve (its residual floor is ~5e-8 at this workload)")
     args = ap.parse_args()
@@ -125,7 +127,8 @@ def main():
     f_gemm = flops_bse_k3(w)   # algorithmic flops of one dense channel GEMM
     paths = {}
     x64 = None
-    for prec in ("fp64", "fp32"):
+    run_paths = [q for q in ("fp64", "fp32") if q in args.paths.split(",")]
+    for prec in run_paths:
         log(f"bench[{prec}]: building SIAM fdPA solver nK1={w['nK1']} nK2={w['nK2']} nK3={w['nK3']} nG={w['nG']}")
         S = build_solver(w, prec)
         if not args.no_sym:
@@ -148,13 +151,18 @@ def main():
         paths[prec] = dict(ms=ms_step, st=st, obs=obs, it=it, res=res, gemm_avg_s=t_bse / 6.0,
                            f_exec_launch=f_exec / 6.0, bse_tflops=f_exec / t_bse / 1e12,
                            bse_tflops_dense_equiv=6 * f_gemm / t_bse / 1e12)
-        if prec == "fp64" and not args.no_sym:
+        if prec == "fp64" and not args.no_sym and not args.no_dense:
             # the same state with identity groups: every element evaluated
             S.reset_sym_grp()
             ms_d, st_d = time_path(S, args, names)
             log(f"bench[fp64, dense]: {ms_d:.3f} ms/iteration")
             paths["fp64_dense"] = dict(ms=ms_d, st=st_d)
         S.close()
+    if len(run_paths) < 2:
+        # profiling run of one path: per-stage line only
+        q = run_paths[0]
+        print(json.dumps({"path": q, "ms_per_step": paths[q]["ms"], "stage_ms": paths[q]["st"]}), flush=True)
+        return
     errs = {k: max_relerr(paths["fp32"]["obs"][k], paths["fp64"]["obs"][k]) for k in paths["fp64"]["obs"]}
     log("bench: FP32-MFMA vs FP64 max rel.err " + ", ".join(f"{k}={v:.2e}" for k, v in errs.items()))
     fp32_ok = max(errs.values()) <= 1e-6

@@ -34,6 +34,7 @@ struct SymDev {
     int* mem = nullptr;
     int ncls = 0;
     std::vector<int> tiles;   // K3 groups: GEMM blocks (iΩ * tiles + tile) holding a representative
+    int* repop = nullptr;     // K3 groups: per element (representative << 2) | op
 };
 // groups of init_sym_grp! (src/ParquetSolver.jl:203-231): SGΣ; SGpp[1] = SGph[1] (same generator);
 // SGpp[2] = SGppL[2]; SGph[2] = SGphL[2]; SGpp[3], SGph[3], SGppL[3], SGphL[3]; SG0pp2, SG0ph2
@@ -50,7 +51,9 @@ struct fdga_ctx {
     size_t nGel = 0, nPiel = 0, n1 = 0, n2 = 0, n3 = 0, nL = 0;
     double2 *Gbare = nullptr, *G0 = nullptr, *Sig0 = nullptr, *G = nullptr, *Sig = nullptr;
     double2 *Pi0pp = nullptr, *Pi0ph = nullptr, *Pipp = nullptr, *Piph = nullptr;
-    double2 *Lpp = nullptr, *Lph = nullptr, *W = nullptr, *W2 = nullptr;
+    double2 *Lpp = nullptr, *Lph = nullptr;
+    double2* Wc[3] = {nullptr, nullptr, nullptr};                  // BSE_K1!/BSE_K2! weights per channel
+    int w_fresh = 0;                                                // Wc valid for the current iteration
     double2 *sde0 = nullptr;
     int sde0_valid = 0, sde0_h = -1;
     double2 *tab[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -70,7 +73,11 @@ struct fdga_ctx {
     // GEMM block lists under the K3 groups: [0..3] one channel of SG_PP3, SG_PH3, SG_PPL3, SG_PHL3;
     // [4] BSE_L_K3! and [5] BSE_K3! over the channels (p, a, t) in one launch
     int* blist[6] = {};
-    int blist_n[6] = {};
+    int blist_n[6] = {};      // workgroups of the list (computed tiles and idle padding)
+    int blist_tiles[6] = {};  // computed tiles
+    int* finmap = nullptr;    // k_k3_finish chunk map (XCD-paired Ω blocks)
+    int finmap_n = 0, fin_nchunk = 0;
+    double2* k3raw = nullptr;                                       // raw t-channel K3 under groups [n3]
     // timing
     int timing = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -102,6 +109,7 @@ size_t nclass(const SymCls& s, size_t nout) { return s.ncls ? (size_t)s.ncls : n
 void free_sym(SymDev& g) {
     if (g.off) (void)hipFree(g.off);
     if (g.mem) (void)hipFree(g.mem);
+    if (g.repop) (void)hipFree(g.repop);
     g = SymDev{};
 }
 
@@ -110,7 +118,25 @@ void free_blists(fdga_ctx* c) {
         if (c->blist[k]) (void)hipFree(c->blist[k]);
         c->blist[k] = nullptr;
         c->blist_n[k] = 0;
+        c->blist_tiles[k] = 0;
     }
+    if (c->finmap) (void)hipFree(c->finmap);
+    c->finmap = nullptr;
+    c->finmap_n = 0;
+}
+
+// Orders work items for XCD locality: item i goes to XCD key(i) % 8, and workgroup 8 r + x runs
+// the r-th item of XCD x (workgroups are dispatched round-robin over the 8 XCDs); -1 pads.
+template <class Key>
+std::vector<int> xcd_order(const std::vector<int>& items, Key key) {
+    std::vector<std::vector<int>> q(8);
+    for (int it : items) q[key(it) & 7].push_back(it);
+    size_t rows = 0;
+    for (auto& v : q) rows = std::max(rows, v.size());
+    std::vector<int> out(rows * 8, -1);
+    for (int x = 0; x < 8; ++x)
+        for (size_t r = 0; r < q[x].size(); ++r) out[r * 8 + x] = q[x][r];
+    return out;
 }
 
 int dalloc(fdga_ctx* c, double2** p, size_t n) {
@@ -443,10 +469,10 @@ static int launch_w(fdga_ctx* c, double2* W) {
 
 template <int CH>
 static int launch_k1(fdga_ctx* c) {
-    if (launch_w<CH>(c, c->W)) return -3;
+    if (!c->w_fresh && launch_w<CH>(c, c->Wc[CH])) return -3;
     const SymCls sym = symcls(c, SG_K1);
     const size_t nw = nclass(sym, c->n1 * c->NP);
-    k_gemv_k1<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->W, vtx_F(c), c->d.nK1,
+    k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], vtx_F(c), c->d.nK1,
                                                            c->d.nK1, c->nVP, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
@@ -454,10 +480,10 @@ static int launch_k1(fdga_ctx* c) {
 
 template <int CH>
 static int launch_k2(fdga_ctx* c) {
-    if (launch_w<CH>(c, c->W)) return -3;
+    if (!c->w_fresh && launch_w<CH>(c, c->Wc[CH])) return -3;
     const SymCls sym = sym_k2(c, CH);
     const size_t nw = nclass(sym, c->n2 * c->NP);
-    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->W, vtx_F(c), chanF(c, c->FL[CH]),
+    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->Wc[CH], vtx_F(c), chanF(c, c->FL[CH]),
                                                            chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
                                                            c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
@@ -466,10 +492,10 @@ static int launch_k2(fdga_ctx* c) {
 
 // BSE_K2! of the a and t channels in one kernel (t needs the finished a channel)
 static int launch_k2_at(fdga_ctx* c) {
-    if (launch_w<ACH>(c, c->W) || launch_w<TCH>(c, c->W2)) return -3;
+    if (!c->w_fresh && (launch_w<ACH>(c, c->Wc[ACH]) || launch_w<TCH>(c, c->Wc[TCH]))) return -3;
     const SymCls sym = symcls(c, SG_PH2);   // a and t share SGph[2]
     const size_t nw = nclass(sym, c->n2 * c->NP);
-    k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->W, c->W2, vtx_F(c),
+    k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->Wc[ACH], c->Wc[TCH], vtx_F(c),
                                                           chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]), c->d.nK2b, c->d.nK2f,
                                                           c->d.nK1, c->nVP, c->NP, c->d.T, box_bound(c), sym);
     LAUNCHCHK(c);
@@ -484,7 +510,7 @@ static SymCls sym_k3(const fdga_ctx* c, int ch, bool left) {
 
 // GEMM block list of a K3 launch over `n` channels: the precomputed list when every slot has a
 // group and the channels are one channel or (p, a, t); null (all blocks) otherwise
-static const int* k3_blocks(const fdga_ctx* c, const int* chans, int n, bool left, size_t& nblocks) {
+static const int* k3_blocks(const fdga_ctx* c, const int* chans, int n, bool left, size_t& nblocks, size_t* ntiles = nullptr) {
     for (int s = 0; s < n; ++s)
         if (!sym_k3(c, chans[s], left).ncls) return nullptr;
     int k = -1;
@@ -492,21 +518,49 @@ static const int* k3_blocks(const fdga_ctx* c, const int* chans, int n, bool lef
     else if (n == 3 && chans[0] == PCH && chans[1] == ACH && chans[2] == TCH) k = left ? 4 : 5;
     if (k < 0 || !c->blist[k]) return nullptr;
     nblocks = (size_t)c->blist_n[k];
+    if (ntiles) *ntiles = (size_t)c->blist_tiles[k];
     return c->blist[k];
 }
 
-// Symmetry fill of the K3 outputs (the GEMM evaluates every element; members are then replaced
-// by op(representative) as SG(K3, InitFunction(diagram)) does) and the t post-processing.
-static int k3_finish(fdga_ctx* c, DevChan* out, const int* chans, int n, bool left, bool t_in_epilogue) {
-    for (int s = 0; s < n; ++s) {
-        const SymCls sym = sym_k3(c, chans[s], left);
-        if (sym.ncls) k_sym_fill<<<cdiv(sym.ncls, 256), 256, 0, c->st>>>(out[chans[s]].K3, sym);
+// Finish of a K3 launch: with symmetry groups the class fill and the t post-processing in one
+// pass (k_k3_finish; the t slot's GEMM output went to k3raw); without, the t post-processing
+// unless the epilogue did it.
+static int k3_finish(fdga_ctx* c, DevChan* out, const int* chans, int n, bool left, bool sym, bool t_in_epilogue) {
+    if (!sym) {
+        if (!t_in_epilogue)
+            for (int s = 0; s < n; ++s)
+                if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(out[TCH].K3, out[ACH].K3, c->n3 * c->NP);
+        LAUNCHCHK(c);
+        return 0;
     }
-    if (!t_in_epilogue)
-        for (int s = 0; s < n; ++s)
-            if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(out[TCH].K3, out[ACH].K3, c->n3 * c->NP);
+    K3Fin f{};
+    double2* o[3] = {nullptr, nullptr, nullptr};
+    const double2* r[3] = {nullptr, nullptr, nullptr};
+    const int* rp[3] = {nullptr, nullptr, nullptr};
+    f.n = n;
+    f.tslot = f.aslot = -1;
+    for (int s = 0; s < n; ++s) {
+        const int ch = chans[s];
+        o[s] = out[ch].K3;
+        r[s] = (ch == TCH) ? c->k3raw : out[ch].K3;
+        rp[s] = c->sg[ch == PCH ? (left ? SG_PPL3 : SG_PP3) : (left ? SG_PHL3 : SG_PH3)].repop;
+        if (ch == TCH) f.tslot = s;
+        if (ch == ACH) f.aslot = s;
+    }
+    f.out0 = o[0]; f.out1 = o[1]; f.out2 = o[2];
+    f.raw0 = r[0]; f.raw1 = r[1]; f.raw2 = r[2];
+    f.rep0 = rp[0]; f.rep1 = rp[1]; f.rep2 = rp[2];
+    f.afinal = out[ACH].K3;
+    k_k3_finish<<<(unsigned)c->finmap_n, 256, 0, c->st>>>(f, c->finmap, c->fin_nchunk, c->NV3 * c->NV3);
     LAUNCHCHK(c);
     return 0;
+}
+
+// true when every channel of the launch has a K3 symmetry group (local solver, NP = 1)
+static bool k3_sym(const fdga_ctx* c, const int* chans, int n, bool left) {
+    for (int s = 0; s < n; ++s)
+        if (!sym_k3(c, chans[s], left).ncls) return false;
+    return n > 0;
 }
 
 // BSE_L_K3! for `n` channels in one launch.  A single t channel without a symmetry group is
@@ -515,14 +569,15 @@ static int k3_finish(fdga_ctx* c, DevChan* out, const int* chans, int n, bool le
 static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     const int NV = c->NV3, NW = c->NW3;
     const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
-    const bool t_epi = (n == 1 && chans[0] == TCH && !sym_k3(c, TCH, true).ncls);
+    const bool sym = k3_sym(c, chans, n, true);
+    const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     LK3Chan cc[3] = {};
     for (int s = 0; s < n; ++s) {
         const int ch = chans[s];
         cc[s].G = c->cache[idx_gamma(ch)];
         cc[s].F0r = c->cache[idx_f0(ch)];
         cc[s].Pi0 = pi_of(c, ch, true);
-        cc[s].out = c->FL[ch].K3;
+        cc[s].out = (sym && ch == TCH) ? c->k3raw : c->FL[ch].K3;
         cc[s].outa = c->FL[ACH].K3;
         cc[s].scale = c->d.T * ((ch == ACH) ? 1.0 : -1.0);   // src/BSE_templates.jl:122,130,138
         cc[s].tpost = (ch == TCH && t_epi);
@@ -531,17 +586,19 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     LK3_Aop a{g, cc[0], cc[1], cc[2]};
     LK3_Bop b{g, cc[0], cc[1], cc[2]};
     LK3_Ep e{g, cc[0], cc[1], cc[2]};
-    size_t nb = (size_t)n * NW * c->NP * tm * tn;
-    const int* bl = k3_blocks(c, chans, n, true, nb);
-    if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, bl, a, b, e) != hipSuccess) return fail(c, "BSE_L_K3 GEMM launch failed", -3);
-    return k3_finish(c, c->FL, chans, n, true, t_epi);
+    size_t nl = 0;
+    const int* bl = k3_blocks(c, chans, n, true, nl);
+    if (zgemm_launch(c->d.precision, n * NW * c->NP, c->st, NV, NV, NV, tm, tn, bl, nl, a, b, e) != hipSuccess)
+        return fail(c, "BSE_L_K3 GEMM launch failed", -3);
+    return k3_finish(c, c->FL, chans, n, true, sym, t_epi);
 }
 
 // BSE_K3! for `n` channels: right operands and column sums, then one GEMM launch
 static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     const int NV = c->NV3, NW = c->NW3;
     const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
-    const bool t_epi = (n == 1 && chans[0] == TCH && !sym_k3(c, TCH, false).ncls);
+    const bool sym = k3_sym(c, chans, n, false);
+    const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const K3Geom g = k3geom(c);
     K3RChan rc[3] = {};
     K3Chan kc[3] = {};
@@ -568,7 +625,7 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
         kc[s].csum = c->csum[ch];
         kc[s].Lr = c->FL[ch].K3;
         kc[s].La = c->FL[ACH].K3;
-        kc[s].out = c->Fb[ch].K3;
+        kc[s].out = (sym && ch == TCH) ? c->k3raw : c->Fb[ch].K3;
         kc[s].outa = c->Fb[ACH].K3;
         kc[s].tpost = (ch == TCH && t_epi);
     }
@@ -578,10 +635,11 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     K3_Aop a{g, c->d.T, kc[0], kc[1], kc[2]};
     K3_Bop b{g, c->d.T, kc[0], kc[1], kc[2]};
     K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
-    size_t nb = (size_t)n * NW * c->NP * tm * tn;
-    const int* bl = k3_blocks(c, chans, n, false, nb);
-    if (zgemm_launch(c->d.precision, nb, c->st, NV, NV, NV, tm, tn, bl, a, b, e) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
-    return k3_finish(c, c->Fb, chans, n, false, t_epi);
+    size_t nl = 0;
+    const int* bl = k3_blocks(c, chans, n, false, nl);
+    if (zgemm_launch(c->d.precision, n * NW * c->NP, c->st, NV, NV, NV, tm, tn, bl, nl, a, b, e) != hipSuccess)
+        return fail(c, "BSE_K3 GEMM launch failed", -3);
+    return k3_finish(c, c->Fb, chans, n, false, sym, t_epi);
 }
 
 // =========================================================================================
@@ -647,7 +705,7 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     int rc = 0;
     for (double2** p : {&c->Gbare, &c->G0, &c->Sig0, &c->G, &c->Sig, &c->tmpG1, &c->tmpG2, &c->tmpG3})
         rc |= dalloc(c, p, c->nGel);
-    for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph, &c->W, &c->W2}) rc |= dalloc(c, p, c->nPiel);
+    for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph, &c->Wc[0], &c->Wc[1], &c->Wc[2]}) rc |= dalloc(c, p, c->nPiel);
     for (double2** p : {&c->Lpp, &c->Lph, &c->tmpL1, &c->tmpL2}) rc |= dalloc(c, p, c->nL);
     rc |= dalloc(c, &c->sde0, c->nGel);
     rc |= dalloc(c, &c->u2C, (size_t)(4 * d.nG - 1) * c->NkG);
@@ -687,7 +745,8 @@ int fdga_destroy(fdga_ctx* c) {
     if (!c) return 0;
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (double2* p : {c->Gbare, c->G0, c->Sig0, c->G, c->Sig, c->tmpG1, c->tmpG2, c->tmpG3, c->Pi0pp, c->Pi0ph,
-                       c->Pipp, c->Piph, c->W, c->W2, c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C, c->stage})
+                       c->Pipp, c->Piph, c->Wc[0], c->Wc[1], c->Wc[2], c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C,
+                       c->stage})
         if (p) (void)hipFree(p);
     for (auto* t : c->tab) if (t) (void)hipFree(t);
     for (auto* p : c->cache) if (p) (void)hipFree(p);
@@ -702,6 +761,7 @@ int fdga_destroy(fdga_ctx* c) {
     if (c->scal) (void)hipFree(c->scal);
     for (auto& g : c->sg) free_sym(g);
     free_blists(c);
+    if (c->k3raw) (void)hipFree(c->k3raw);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -889,20 +949,28 @@ int fdga_init_sym_grp(fdga_ctx* c) {
         HIPCHK(c, hipMemcpy(off.data(), g.off, off.size() * sizeof(int), hipMemcpyDeviceToHost));
         mem.resize(off.back());
         HIPCHK(c, hipMemcpy(mem.data(), g.mem, mem.size() * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<int> repop(c->n3);
         for (int k = 0; k < g.ncls; ++k) {
             const int64_t dv = mem[off[k]] >> 2;
             const int iv = (int)(dv % NV), ivp = (int)((dv / NV) % NV), iW = (int)(dv / (NV * NV));
             hit[(size_t)iW * tiles + iv / GBM + tm * (ivp / GBN)] = 1;
+            for (int q = off[k]; q < off[k + 1]; ++q) repop[mem[q] >> 2] = (int)((dv << 2) | (mem[q] & 3));
         }
+        HIPCHK(c, hipMalloc((void**)&g.repop, repop.size() * sizeof(int)));
+        HIPCHK(c, hipMemcpy(g.repop, repop.data(), repop.size() * sizeof(int), hipMemcpyHostToDevice));
         g.tiles.clear();
         for (size_t b = 0; b < hit.size(); ++b)
             if (hit[b]) g.tiles.push_back((int)b);
     }
     free_blists(c);
+    if (!c->k3raw && dalloc(c, &c->k3raw, c->n3)) return -2;
+    // block lists in XCD order: all tiles of one batch (channel slot, Ω) on one XCD, batch b on XCD b % 8
     auto upload_list = [&](int k, const std::vector<int>& v) -> int {
-        HIPCHK(c, hipMalloc((void**)&c->blist[k], std::max<size_t>(1, v.size()) * sizeof(int)));
-        HIPCHK(c, hipMemcpy(c->blist[k], v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
-        c->blist_n[k] = (int)v.size();
+        const std::vector<int> o = xcd_order(v, [tiles](int b) { return b / tiles; });
+        HIPCHK(c, hipMalloc((void**)&c->blist[k], std::max<size_t>(1, o.size()) * sizeof(int)));
+        HIPCHK(c, hipMemcpy(c->blist[k], o.data(), o.size() * sizeof(int), hipMemcpyHostToDevice));
+        c->blist_n[k] = (int)o.size();
+        c->blist_tiles[k] = (int)v.size();
         return 0;
     };
     const int ids1[4] = {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3};
@@ -914,6 +982,25 @@ int fdga_init_sym_grp(fdga_ctx* c) {
         for (int sl = 0; sl < 3; ++sl)
             for (int b : c->sg[ids3[sl]].tiles) all.push_back(b + sl * (int)NW * tiles);
         rc |= upload_list(left ? 4 : 5, all);
+    }
+    // k_k3_finish chunks: 256 elements of one Ω block; the blocks Ω and -Ω on one XCD
+    {
+        const int nchunk = cdiv((size_t)NV * NV, 256);
+        // per pair (Ω, -Ω) the chunks of both blocks alternate, so that an XCD works on both
+        // blocks of a pair at the same time
+        std::vector<int> items;
+        const int n3b0 = d.nK3b - 1;
+        for (int m = 0; m <= n3b0; ++m)
+            for (int ch = 0; ch < nchunk; ++ch) {
+                items.push_back((n3b0 - m) * nchunk + ch);
+                if (m > 0) items.push_back((n3b0 + m) * nchunk + ch);
+            }
+        const int n3b = d.nK3b;
+        const std::vector<int> o = xcd_order(items, [nchunk, n3b](int it) { return std::abs(it / nchunk - (n3b - 1)); });
+        HIPCHK(c, hipMalloc((void**)&c->finmap, o.size() * sizeof(int)));
+        HIPCHK(c, hipMemcpy(c->finmap, o.data(), o.size() * sizeof(int), hipMemcpyHostToDevice));
+        c->finmap_n = (int)o.size();
+        c->fin_nchunk = nchunk;
     }
     c->sde0_valid = 0;
     return rc ? -2 : 0;
@@ -1034,10 +1121,16 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
         stage_end(c, 3);
     }
     stage_begin(c);
-    for (int ch : {PCH, ACH, TCH}) if (fdga_bse_k1(c, ch)) return -3;
+    // the K1/K2 weights depend on Π, Π0, FL and F0 only: computed once for both stages
+    if (launch_w<PCH>(c, c->Wc[PCH]) || launch_w<ACH>(c, c->Wc[ACH]) || launch_w<TCH>(c, c->Wc[TCH])) return -3;
+    c->w_fresh = 1;
+    int rc = 0;
+    for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
     stage_end(c, 4);
     stage_begin(c);
-    if (fdga_bse_k2(c, PCH) || launch_k2_at(c)) return -3;
+    rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));
+    c->w_fresh = 0;
+    if (rc) return -3;
     stage_end(c, 5);
     stage_begin(c);
     {
@@ -1148,7 +1241,7 @@ int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, cons
         hipMemcpy(dB, B, nb * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
     if (!rc) {
         const int tm = cdiv(M, GBM), tn = cdiv(N, GBN);
-        if (zgemm_launch(precision, (size_t)(tm * tn), 0, M, N, K, tm, tn, nullptr, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M}) != hipSuccess ||
+        if (zgemm_launch(precision, 1, 0, M, N, K, tm, tn, nullptr, 0, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M}) != hipSuccess ||
             hipDeviceSynchronize() != hipSuccess) rc = -3;
     }
     if (!rc && hipMemcpy(C, dC, nc * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = -2;
@@ -1161,9 +1254,9 @@ int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, cons
 int64_t fdga_gemm_blocks(const fdga_ctx* c, int32_t left) {
     if (!c) return -1;
     const int chans[3] = {PCH, ACH, TCH};
-    size_t nb = (size_t)3 * c->NW3 * c->NP * cdiv(c->NV3, GBM) * cdiv(c->NV3, GBN);
-    (void)k3_blocks(c, chans, 3, left != 0, nb);
-    return (int64_t)nb;
+    size_t nl = 0, nt = (size_t)3 * c->NW3 * c->NP * cdiv(c->NV3, GBM) * cdiv(c->NV3, GBN);
+    (void)k3_blocks(c, chans, 3, left != 0, nl, &nt);
+    return (int64_t)nt;
 }
 
 // enable (1) / disable (0) per-stage event timing; resets the accumulated stage times

@@ -50,16 +50,46 @@ __device__ __forceinline__ double2 sym_apply(int op, double2 v) {
     return v;
 }
 
-// out[member] = op(out[representative]) for every class (one thread per class)
-__global__ void k_sym_fill(double2* __restrict__ out, SymCls sym) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= sym.ncls) return;
-    const int b = sym.off[k], e = sym.off[k + 1];
-    const double2 v = out[sym.mem[b] >> 2];
-    for (int i = b + 1; i < e; ++i) {
-        const int code = sym.mem[i];
-        out[code >> 2] = sym_apply(code & 3, v);
+// Finish of a K3 launch under symmetry groups (one thread per element, coalesced stores):
+//   x_s[j] = op(raw_s[rep(j)]) for each slot s (rep_s[j] = (rep << 2) | op; null: x_s[j] = raw_s[j]),
+//   then for the t slot x_t[j] = (x_t[j] + x_a[j]) / 2 (src/BSE_templates.jl:176-177).
+// The p and a slots are finished in place (raw_s = out_s: representatives keep their value, and
+// only representatives are read); the t slot's raw values come from a separate array, because
+// its post-processing also rewrites representatives.  Representatives of a K3 element lie in the
+// element's own Ω block or in the -Ω block: workgroup w works on chunk wgmap[w] = iΩ * nchunk +
+// chunk, and the map places both blocks of a pair (Ω, -Ω) on one XCD (workgroups are dispatched
+// round-robin over the 8 XCDs), so the gathers hit that XCD's L2.  -1 entries are idle.
+struct K3Fin {
+    double2* out0; double2* out1; double2* out2;
+    const double2* raw0; const double2* raw1; const double2* raw2;
+    const int* rep0; const int* rep1; const int* rep2;
+    int n;                    // slots in the launch
+    int tslot, aslot;         // slot of t and of a (-1: absent)
+    const double2* afinal;    // final a-channel K3 when a is not in the launch
+};
+__device__ __forceinline__ double2 fin_load(const double2* raw, const int* rep, size_t j) {
+    if (!rep) return raw[j];
+    const int code = rep[j];
+    return sym_apply(code & 3, raw[code >> 2]);
+}
+__global__ __launch_bounds__(256) void k_k3_finish(K3Fin f, const int* __restrict__ wgmap, int nchunk, int blk) {
+    const int code = wgmap[blockIdx.x];
+    if (code < 0) return;
+    const int iW = code / nchunk, e = (code - iW * nchunk) * 256 + (int)threadIdx.x;
+    if (e >= blk) return;
+    const size_t j = (size_t)e + (size_t)blk * iW;
+    double2 v0 = fin_load(f.raw0, f.rep0, j);
+    double2 v1 = f.n > 1 ? fin_load(f.raw1, f.rep1, j) : czero();
+    double2 v2 = f.n > 2 ? fin_load(f.raw2, f.rep2, j) : czero();
+    if (f.tslot >= 0) {
+        const double2 a = f.aslot == 0 ? v0 : f.aslot == 1 ? v1 : f.aslot == 2 ? v2 : f.afinal[j];
+        if (f.tslot == 0) v0 = cscale(cadd(v0, a), 0.5);
+        else if (f.tslot == 1) v1 = cscale(cadd(v1, a), 0.5);
+        else v2 = cscale(cadd(v2, a), 0.5);
     }
+    f.out0[j] = v0;
+    if (f.n > 1) f.out1[j] = v1;
+    if (f.n > 2) f.out2[j] = v2;
 }
 
 // Dyson!: G = 1 / (1/Gbare + Σ)  (src/dyson.jl:5-31)
@@ -262,7 +292,8 @@ template <> struct MfmaT<false> {
     typedef double S;
     typedef v4d Acc;
     __device__ __forceinline__ static Acc mfma(S a, S b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-    // accumulator element r of lane l is D(row = l/16 + 4 r, col = l % 16)
+    // accumulator element r of lane l is D(row = l/16 + 4 r, col = l % 16); D is the transposed
+    // output tile (operands swapped), so row() gives the output column
     __device__ __forceinline__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct MfmaT<true> {
@@ -273,8 +304,12 @@ template <> struct MfmaT<true> {
     __device__ __forceinline__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// Operand loaders may opt out of the register prefetch (a loader with many loads per element
+// raises the register count to one wave per SIMD): a loader type with `static constexpr bool
+// prefetch = false` is staged synchronously.
+template <class L> struct Prefetch { static constexpr bool on = true; };
 template <bool F32, class AL, class BL, class EP>
-__global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int tiles_m, int tiles_n,
+__global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int tiles_m, int tiles_n, int nbatch,
                                                     const int* __restrict__ blist, AL al, BL bl, EP ep) {
     typedef MfmaT<F32> MT;
     typedef typename MT::S S;
@@ -282,8 +317,21 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
     __shared__ S As[2][GBK][GBM + GPAD];
     __shared__ S Bs[2][GBK][GBN + GPAD];
     const int tiles = tiles_m * tiles_n;
-    // block = batch * tiles + tile; with a block list only the listed tiles are computed
-    const int bid = blist ? blist[blockIdx.x] : (int)blockIdx.x;
+    // block = batch * tiles + tile.  Workgroups go round-robin to the 8 XCDs; the tiles of one
+    // batch share their A rows and B columns, so they are placed on one XCD: workgroup w runs
+    // batch 8 (w / 8 / tiles) + w % 8, tile (w / 8) % tiles (the grid covers a multiple of 8
+    // batches; blocks past the last batch are idle).  A block list (built in the same order,
+    // -1 = idle) selects the computed tiles instead.
+    int bid;
+    if (blist) {
+        bid = blist[blockIdx.x];
+    } else {
+        const int w = (int)blockIdx.x, r = w >> 3;
+        const int bt = (r / tiles) * 8 + (w & 7);
+        bid = bt * tiles + r % tiles;
+        if (bt >= nbatch) bid = -1;
+    }
+    if (bid < 0) return;
     const int batch = bid / tiles;
     const int t = bid - batch * tiles;
     const int m0 = (t % tiles_m) * GBM, n0 = (t / tiles_m) * GBN;
@@ -297,26 +345,61 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
             accr[i][j] = Acc{0, 0, 0, 0};
             acci[i][j] = Acc{0, 0, 0, 0};
         }
-    for (int k0 = 0; k0 < K; k0 += GBK) {
+    // software pipeline: the operands of the next K-tile are loaded into registers while the
+    // current one is multiplied from LDS
+    constexpr int QA = (GBM * GBK) / 256, QB = (GBN * GBK) / 256;
+    double2 pa[QA], pb[QB];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < (GBM * GBK) / 256; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int e = tid + 256 * q;
-            const int mm = e % GBM, kk = e / GBM;
-            const int gm = m0 + mm, gk = k0 + kk;
-            double2 v = (gm < M && gk < K) ? al(batch, gm, gk) : czero();
-            As[0][kk][mm] = (S)v.x;
-            As[1][kk][mm] = (S)v.y;
+            const int gm = m0 + e % GBM, gk = k0 + e / GBM;
+            pa[q] = (gm < M && gk < K) ? al(batch, gm, gk) : czero();
         }
 #pragma unroll
-        for (int q = 0; q < (GBN * GBK) / 256; ++q) {
+        for (int q = 0; q < QB; ++q) {
             const int e = tid + 256 * q;
-            const int nn = e % GBN, kk = e / GBN;
-            const int gn = n0 + nn, gk = k0 + kk;
-            double2 v = (gn < N && gk < K) ? bl(batch, gk, gn) : czero();
-            Bs[0][kk][nn] = (S)v.x;
-            Bs[1][kk][nn] = (S)v.y;
+            const int gn = n0 + e % GBN, gk = k0 + e / GBN;
+            pb[q] = (gn < N && gk < K) ? bl(batch, gk, gn) : czero();
+        }
+    };
+    constexpr bool PF = Prefetch<AL>::on && Prefetch<BL>::on;
+    if (PF) fetch(0);
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+        if (PF) {
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                const int e = tid + 256 * q;
+                As[0][e / GBM][e % GBM] = (S)pa[q].x;
+                As[1][e / GBM][e % GBM] = (S)pa[q].y;
+            }
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                const int e = tid + 256 * q;
+                Bs[0][e / GBN][e % GBN] = (S)pb[q].x;
+                Bs[1][e / GBN][e % GBN] = (S)pb[q].y;
+            }
+        } else {
+            // element by element: load and store to LDS without holding the tile in registers
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                const int e = tid + 256 * q;
+                const int gm = m0 + e % GBM, gk = k0 + e / GBM;
+                const double2 v = (gm < M && gk < K) ? al(batch, gm, gk) : czero();
+                As[0][e / GBM][e % GBM] = (S)v.x;
+                As[1][e / GBM][e % GBM] = (S)v.y;
+            }
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                const int e = tid + 256 * q;
+                const int gn = n0 + e % GBN, gk = k0 + e / GBN;
+                const double2 v = (gn < N && gk < K) ? bl(batch, gk, gn) : czero();
+                Bs[0][e / GBN][e % GBN] = (S)v.x;
+                Bs[1][e / GBN][e % GBN] = (S)v.y;
+            }
         }
         __syncthreads();
+        if (PF && k0 + GBK < K) fetch(k0 + GBK);
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 4) {
             const int kr = kk + (lane >> 4), c = lane & 15;
@@ -336,10 +419,12 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    accr[i][j] = MT::mfma(ar[i], br[j], accr[i][j]);
-                    accr[i][j] = MT::mfma(nai[i], bi[j], accr[i][j]);
-                    acci[i][j] = MT::mfma(ar[i], bi[j], acci[i][j]);
-                    acci[i][j] = MT::mfma(ai[i], br[j], acci[i][j]);
+                    // B fragment as the MFMA's left operand: the accumulator is the transposed
+                    // tile, so lane % 16 runs along the output rows (contiguous in memory)
+                    accr[i][j] = MT::mfma(br[j], ar[i], accr[i][j]);
+                    accr[i][j] = MT::mfma(bi[j], nai[i], accr[i][j]);
+                    acci[i][j] = MT::mfma(bi[j], ar[i], acci[i][j]);
+                    acci[i][j] = MT::mfma(br[j], ai[i], acci[i][j]);
                 }
         }
         __syncthreads();
@@ -350,20 +435,22 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + 16 * i + MT::row(lane, r);
-                const int col = n0 + wn + 16 * j + (lane & 15);
+                const int row = m0 + wm + 16 * i + (lane & 15);
+                const int col = n0 + wn + 16 * j + MT::row(lane, r);
                 if (row < M && col < N) ep(batch, row, col, cmk((double)accr[i][j][r], (double)acci[i][j][r]));
             }
 }
 
-// launch helper: precision 0 = FP64 MFMA, 1 = FP32 MFMA; blist (nblocks entries) selects the
-// blocks to compute, null = all
+// launch helper: precision 0 = FP64 MFMA, 1 = FP32 MFMA.  Without a block list the grid covers
+// nbatch batches of tm x tn tiles (rounded up to a multiple of 8 batches, see zgemm_tiles_t);
+// with one, nlist workgroups run the listed blocks.
 template <class AL, class BL, class EP>
-inline hipError_t zgemm_launch(int precision, size_t nblocks, hipStream_t st, int M, int N, int K, int tm, int tn,
-                               const int* blist, const AL& a, const BL& b, const EP& e) {
+inline hipError_t zgemm_launch(int precision, int nbatch, hipStream_t st, int M, int N, int K, int tm, int tn,
+                               const int* blist, size_t nlist, const AL& a, const BL& b, const EP& e) {
+    const size_t nblocks = blist ? nlist : (size_t)((nbatch + 7) / 8) * 8 * tm * tn;
     if (nblocks == 0) return hipSuccess;
-    if (precision == 1) zgemm_tiles_t<true><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, blist, a, b, e);
-    else zgemm_tiles_t<false><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, blist, a, b, e);
+    if (precision == 1) zgemm_tiles_t<true><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, nbatch, blist, a, b, e);
+    else zgemm_tiles_t<false><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, nbatch, blist, a, b, e);
     return hipGetLastError();
 }
 
@@ -469,6 +556,7 @@ struct K3_Aop {
         return s == 0 ? op(c0, bb, i, k) : s == 1 ? op(c1, bb, i, k) : op(c2, bb, i, k);
     }
 };
+template <> struct Prefetch<struct K3_Aop> { static constexpr bool on = false; };
 struct K3_Bop {
     K3Geom g; double T; K3Chan c0, c1, c2;
     __device__ __forceinline__ double2 op(const K3Chan& c, int bb, int k, int j) const {
@@ -603,13 +691,16 @@ __device__ __forceinline__ void far_pair(double2 t, double2& left, double2& righ
     right = cmk(__shfl(t.x, 1, WAVE), __shfl(t.y, 1, WAVE));
 }
 
-// BSE_K1!: K1(Ω, P) = T s Σ_ω F(Ω, ∞, ω, P, Ch, Sp) W(ω)   (one wave per output class)
+// BSE_K1!: K1(Ω, P) = T s Σ_ω F(Ω, ∞, ω, P, Ch, Sp) W(ω).  One 256-thread block per output
+// class (the K1 mesh has few outputs; four waves per output shorten each one's chain of gathers).
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a,
                                                  const double2* __restrict__ W, VtxV F, int n1, int nWP, int nVP, int NP,
                                                  double scale, int NK, SymCls sym) {
     constexpr int SP = ChSpin<CH>::SP;
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    constexpr int NT = 256;
+    __shared__ double red[NT / WAVE][6];
+    const int wid = blockIdx.x, tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const int NW1 = 2 * n1 - 1;
     if (wid >= sym.count(NW1 * NP)) return;
     const int o0 = sym.rep(wid);
@@ -619,21 +710,30 @@ __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const
     const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
     double sr = 0.0, si = 0.0, lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
-    for (int iw = lo + lane; iw <= hi; iw += WAVE) {
+    for (int iw = lo + tid; iw <= hi; iw += NT) {
         double2 x = cmul(F.eval(m, FDGA_INF, iw - nVP, p, CH, SP, FL_ALL), Wr[iw]);
         sr += x.x; si += x.y;
     }
-    for (int iw = lane; iw < lo; iw += WAVE) { lr += Wr[iw].x; li += Wr[iw].y; }
-    for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) { rr += Wr[iw].x; ri += Wr[iw].y; }
-    double2 tl, tr;
-    far_pair(F.eval(m, FDGA_INF, lane == 0 ? -Wc - 1 : Wc + 1, p, CH, SP, FL_ALL), tl, tr);
+    for (int iw = tid; iw < lo; iw += NT) { lr += Wr[iw].x; li += Wr[iw].y; }
+    for (int iw = hi + 1 + tid; iw < 2 * nVP; iw += NT) { rr += Wr[iw].x; ri += Wr[iw].y; }
     sr = wave_sum(sr); si = wave_sum(si);
     lr = wave_sum(lr); li = wave_sum(li);
     rr = wave_sum(rr); ri = wave_sum(ri);
-    // every lane holds the same sums; lanes write the class members
-    double2 acc = cmk(sr, si);
-    if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
-    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
+    if (lane == 0) {
+        red[wv][0] = sr; red[wv][1] = si; red[wv][2] = lr;
+        red[wv][3] = li; red[wv][4] = rr; red[wv][5] = ri;
+    }
+    __syncthreads();
+    if (wv != 0) return;
+    double t[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t[q] = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+    double2 tl, tr;
+    far_pair(F.eval(m, FDGA_INF, lane == 0 ? -Wc - 1 : Wc + 1, p, CH, SP, FL_ALL), tl, tr);
+    // every lane of wave 0 holds the same sums; lanes write the class members
+    double2 acc = cmk(t[0], t[1]);
+    if (lo > 0) acc = cadd(acc, cmul(tl, cmk(t[2], t[3])));
+    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(t[4], t[5])));
     const double2 raw = cscale(acc, scale);
     int b, e;
     sym.range(wid, b, e);
@@ -645,11 +745,32 @@ __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const
     }
 }
 
-// BSE_K2!: K2(Ω, ν, P) = T s Σ_ω [F(Ω,ν,ω) - F(Ω,∞,ω)] W(ω) + FL.K2 (t: 2 FLt - FLa)
+// BSE_K2!: K2(Ω, ν, P) = T s Σ_ω [F(Ω,ν,ω) - F(Ω,∞,ω)] W(ω) + FL.K2 (t: 2 FLt - FLa).
+// In the difference the channel's own K1(Ω) and K2(Ω, ω) cancel (the νInf evaluation keeps
+// exactly those two classes, src/channel.jl:220-339), the channel's K2(Ω, ν) does not depend on
+// ω and factors out as K2(Ω, ν) Σ_ω W(ω), and what depends on ω is the reference vertex part,
+// the channel's own K3 and the two other channels at the converted arguments.
 template <int CH>
-__device__ __forceinline__ double2 bse_k2_term(const VtxV& F, int m, int v, int w, int p) {
-    constexpr int SP = ChSpin<CH>::SP;
-    return csub(F.eval(m, v, w, p, CH, SP, FL_ALL), F.eval(m, FDGA_INF, w, p, CH, SP, FL_ALL));
+__device__ __forceinline__ double2 k2d_psp(const VtxV& F, int m, int v, int w, int p) {
+    double2 x = csub(F.f0.eval_psp(m, v, w, CH), F.f0.eval_psp(m, FDGA_INF, w, CH));
+    const ChanV& g = F.g[CH];
+    if (in_b(m, g.n3b) && in_f(v, g.n3f) && in_f(w, g.n3f)) x = cadd(x, g.k3(m, v, w, p));
+    int a, b, c;
+    if (CH != PCH) { convert(CH, PCH, m, v, w, a, b, c); x = cadd(x, F.g[PCH].eval(a, b, c, -1)); }
+    if (CH != TCH) { convert(CH, TCH, m, v, w, a, b, c); x = cadd(x, F.g[TCH].eval(a, b, c, -1)); }
+    if (CH != ACH) { convert(CH, ACH, m, v, w, a, b, c); x = cadd(x, F.g[ACH].eval(a, b, c, -1)); }
+    return x;
+}
+// the ω-dependent summand in the channel's spin component (t: density, 2 F(t) - F(a))
+template <int CH>
+__device__ __forceinline__ double2 k2d_term(const VtxV& F, int m, int v, int w, int p) {
+    if (CH == TCH) return csub(cscale(k2d_psp<TCH>(F, m, v, w, p), 2.0), k2d_psp<ACH>(F, m, v, w, p));
+    return k2d_psp<CH>(F, m, v, w, p);
+}
+template <int CH>
+__device__ __forceinline__ double2 k2d_rank1(const VtxV& F, int m, int v, int p) {
+    if (CH == TCH) return csub(cscale(F.g[TCH].k2(m, v, p), 2.0), F.g[ACH].k2(m, v, p));
+    return F.g[CH].k2(m, v, p);
 }
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a,
@@ -665,21 +786,25 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
     const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
     // only the window is iterated for the summand; the far weights are summed separately
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
-    double sr = 0.0, si = 0.0, lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+    double sr = 0.0, si = 0.0, mr = 0.0, mi = 0.0, lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
     for (int iw = lo + lane; iw <= hi; iw += WAVE) {
-        double2 x = cmul(bse_k2_term<CH>(F, m, v, iw - nVP, p), Wr[iw]);
+        const double2 wv = Wr[iw];
+        double2 x = cmul(k2d_term<CH>(F, m, v, iw - nVP, p), wv);
         sr += x.x; si += x.y;
+        mr += wv.x; mi += wv.y;
     }
     for (int iw = lane; iw < lo; iw += WAVE) { lr += Wr[iw].x; li += Wr[iw].y; }
     for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) { rr += Wr[iw].x; ri += Wr[iw].y; }
     double2 tl, tr;
-    far_pair(bse_k2_term<CH>(F, m, v, lane == 0 ? -Wc - 1 : Wc + 1, p), tl, tr);
+    far_pair(k2d_term<CH>(F, m, v, lane == 0 ? -Wc - 1 : Wc + 1, p), tl, tr);
     sr = wave_sum(sr); si = wave_sum(si);
+    mr = wave_sum(mr); mi = wave_sum(mi);
     lr = wave_sum(lr); li = wave_sum(li);
     rr = wave_sum(rr); ri = wave_sum(ri);
     double2 acc = cmk(sr, si);
     if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
     if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
+    acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), cmk(mr + lr + rr, mi + li + ri)));
     const double2 raw = cscale(acc, scale);
     int b, e;
     sym.range(wid, b, e);
@@ -717,25 +842,29 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
     const double2* wt = Wt + roff;
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
     double ar = 0.0, ai = 0.0, tr = 0.0, ti = 0.0;
+    double amr = 0.0, ami = 0.0, tmr = 0.0, tmi = 0.0;
     for (int iw = lo + lane; iw <= hi; iw += WAVE) {
         const int w = iw - nVP;
-        const double2 fa = csub(F.eval_psp(m, v, w, p, ACH, FL_ALL), F.eval_psp(m, FDGA_INF, w, p, ACH, FL_ALL));
-        const double2 ft = csub(F.eval_psp(m, v, w, p, TCH, FL_ALL), F.eval_psp(m, FDGA_INF, w, p, TCH, FL_ALL));
-        const double2 xa = cmul(fa, wa[iw]);
-        const double2 xt = cmul(csub(cscale(ft, 2.0), fa), wt[iw]);
+        const double2 fa = k2d_psp<ACH>(F, m, v, w, p);
+        const double2 ft = k2d_psp<TCH>(F, m, v, w, p);
+        const double2 wav = wa[iw], wtv = wt[iw];
+        const double2 xa = cmul(fa, wav);
+        const double2 xt = cmul(csub(cscale(ft, 2.0), fa), wtv);
         ar += xa.x; ai += xa.y; tr += xt.x; ti += xt.y;
+        amr += wav.x; ami += wav.y; tmr += wtv.x; tmi += wtv.y;
     }
     double alr = 0.0, ali = 0.0, arr = 0.0, ari = 0.0, tlr = 0.0, tli = 0.0, trr = 0.0, tri = 0.0;
     for (int iw = lane; iw < lo; iw += WAVE) { alr += wa[iw].x; ali += wa[iw].y; tlr += wt[iw].x; tli += wt[iw].y; }
     for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) { arr += wa[iw].x; ari += wa[iw].y; trr += wt[iw].x; tri += wt[iw].y; }
     // far summands: lane 0 at ω = -Wc-1, lane 1 at ω = Wc+1
     const int wf = lane == 0 ? -Wc - 1 : Wc + 1;
-    const double2 fa = csub(F.eval_psp(m, v, wf, p, ACH, FL_ALL), F.eval_psp(m, FDGA_INF, wf, p, ACH, FL_ALL));
-    const double2 ft = csub(F.eval_psp(m, v, wf, p, TCH, FL_ALL), F.eval_psp(m, FDGA_INF, wf, p, TCH, FL_ALL));
+    const double2 fa = k2d_psp<ACH>(F, m, v, wf, p);
+    const double2 ft = k2d_psp<TCH>(F, m, v, wf, p);
     double2 fal, far_, fdl, fdr;
     far_pair(fa, fal, far_);
     far_pair(csub(cscale(ft, 2.0), fa), fdl, fdr);
     ar = wave_sum(ar); ai = wave_sum(ai); tr = wave_sum(tr); ti = wave_sum(ti);
+    amr = wave_sum(amr); ami = wave_sum(ami); tmr = wave_sum(tmr); tmi = wave_sum(tmi);
     alr = wave_sum(alr); ali = wave_sum(ali); arr = wave_sum(arr); ari = wave_sum(ari);
     tlr = wave_sum(tlr); tli = wave_sum(tli); trr = wave_sum(trr); tri = wave_sum(tri);
     double2 A = cmk(ar, ai), D = cmk(tr, ti);
@@ -747,6 +876,8 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
         A = cadd(A, cmul(far_, cmk(arr, ari)));
         D = cadd(D, cmul(fdr, cmk(trr, tri)));
     }
+    A = cadd(A, cmul(k2d_rank1<ACH>(F, m, v, p), cmk(amr + alr + arr, ami + ali + ari)));
+    D = cadd(D, cmul(k2d_rank1<TCH>(F, m, v, p), cmk(tmr + tlr + trr, tmi + tli + tri)));
     const double2 rawa = cscale(A, T), rawt = cscale(D, -T);
     int b, e;
     sym.range(wid, b, e);
