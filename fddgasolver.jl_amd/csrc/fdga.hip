@@ -53,6 +53,9 @@ struct fdga_ctx {
     double2 *Pi0pp = nullptr, *Pi0ph = nullptr, *Pipp = nullptr, *Piph = nullptr;
     double2 *Lpp = nullptr, *Lph = nullptr;
     double2* Wc[3] = {nullptr, nullptr, nullptr};                  // BSE_K1!/BSE_K2! weights per channel
+    double2* Wtail[3] = {nullptr, nullptr, nullptr};               // their tail sums (k_tail_sums)
+    double2* Ptail[4] = {nullptr, nullptr, nullptr, nullptr};      // tail sums of Π0pp, Π0ph, Πpp, Πph
+    size_t ntail = 0;                                               // elements of one tail-sum array
     int w_fresh = 0;                                                // Wc valid for the current iteration
     double2 *sde0 = nullptr;
     int sde0_valid = 0, sde0_h = -1;
@@ -263,14 +266,28 @@ int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale
     return 0;
 }
 
-// bubbles for G (target, into Πpp/Πph) or G0 (reference, into Π0pp/Π0ph)
+// tail sums of one array on the bubble mesh (rows of 2 nVP entries)
+int tail_sums(fdga_ctx* c, const double2* X, double2* T) {
+    const size_t nrows = (size_t)c->NWP * c->NP;
+    k_tail_sums<<<dim3((unsigned)nrows, 2), 256, 0, c->st>>>(X, T, c->nVP, nrows);
+    LAUNCHCHK(c);
+    return 0;
+}
+double2* pi_tail(fdga_ctx* c, const double2* Pi) {
+    if (Pi == c->Pi0pp) return c->Ptail[0];
+    if (Pi == c->Pi0ph) return c->Ptail[1];
+    if (Pi == c->Pipp) return c->Ptail[2];
+    return c->Ptail[3];
+}
+
+// bubbles for G (target, into Πpp/Πph) or G0 (reference, into Π0pp/Π0ph), with their tail sums
 int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     const fdga_dims& d = c->d;
     if (!d.nonlocal) {
         dim3 g(cdiv(c->NVP, 256), c->NWP);
         k_bubble_local<<<g, 256, 0, c->st>>>(pp, ph, Gsrc, d.nG, d.nK1, c->nVP, d.T);
         LAUNCHCHK(c);
-        return 0;
+        return tail_sums(c, pp, pi_tail(c, pp)) || tail_sums(c, ph, pi_tail(c, ph));
     }
     // G(R) = fft(G)/LG^2 (src/nonlocal/bubble.jl:110)
     HIPCHK(c, hipMemcpyAsync(c->tmpG1, Gsrc, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
@@ -281,7 +298,7 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     size_t nrow = (size_t)c->NVP * c->NWP;
     if (dft2(c, pp, nrow, d.L, +1, 1.0)) return -3;
     if (dft2(c, ph, nrow, d.L, +1, 1.0)) return -3;
-    return 0;
+    return tail_sums(c, pp, pi_tail(c, pp)) || tail_sums(c, ph, pi_tail(c, ph));
 }
 
 void stage_begin(fdga_ctx* c) {
@@ -377,20 +394,26 @@ int sde_hartree(fdga_ctx* c, double2* S, const double2* G, double sign) {
 
 // Lpp/Lph of one level; `zero` selects the reference system's groups SG0pp2/SG0ph2
 int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* Ph, bool zero) {
+    // the bubbles' tail sums are kept current by bubbles_into and fdga_set
     const fdga_dims& d = c->d;
     const size_t nout = (size_t)c->NW2 * c->NV2 * c->NP;
     RefV ref = ref_view(c, false);
     const SymCls spp = symcls(c, zero ? SG_0PP2 : SG_PP2), sph = symcls(c, zero ? SG_0PH2 : SG_PH2);
     const double2 TU = cscale(c->U, d.T);
     const int NK = box_bound(c);
+    const double2* Tpp = pi_tail(c, Pp);
+    const double2* Tph = pi_tail(c, Ph);
     if (!spp.ncls && !sph.ncls) {
-        k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref, lv.kind,
+        k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
+                                                              lv.kind,
                                                               d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, lv.chanNP, TU, NK, spp);
     } else {
-        k_sde_L<1><<<cdiv(nclass(spp, nout) * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref,
+        k_sde_L<1><<<cdiv(nclass(spp, nout) * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
+                                                                           lv.g[2], ref,
                                                                            lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP,
                                                                            lv.chanNP, TU, NK, spp);
-        k_sde_L<2><<<cdiv(nclass(sph, nout) * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, lv.g[0], lv.g[1], lv.g[2], ref,
+        k_sde_L<2><<<cdiv(nclass(sph, nout) * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
+                                                                           lv.g[2], ref,
                                                                            lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP,
                                                                            lv.chanNP, TU, NK, sph);
     }
@@ -459,20 +482,22 @@ static int launch_lk2(fdga_ctx* c) {
     return 0;
 }
 
+// K1/K2 weights of channel CH into Wc[CH] and their tail sums into Wtail[CH]
 template <int CH>
-static int launch_w(fdga_ctx* c, double2* W) {
+static int launch_w(fdga_ctx* c) {
     dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
-    k_bse_w<CH><<<gw, 256, 0, c->st>>>(W, pi_of(c, CH, false), pi_of(c, CH, true), vtx_FL(c), f0_view(c), c->d.nK1, c->nVP);
+    k_bse_w<CH><<<gw, 256, 0, c->st>>>(c->Wc[CH], pi_of(c, CH, false), pi_of(c, CH, true), vtx_FL(c), f0_view(c), c->d.nK1,
+                                       c->nVP);
     LAUNCHCHK(c);
-    return 0;
+    return tail_sums(c, c->Wc[CH], c->Wtail[CH]);
 }
 
 template <int CH>
 static int launch_k1(fdga_ctx* c) {
-    if (!c->w_fresh && launch_w<CH>(c, c->Wc[CH])) return -3;
+    if (!c->w_fresh && launch_w<CH>(c)) return -3;
     const SymCls sym = symcls(c, SG_K1);
     const size_t nw = nclass(sym, c->n1 * c->NP);
-    k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], vtx_F(c), c->d.nK1,
+    k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], c->Wtail[CH], vtx_F(c), c->d.nK1,
                                                            c->d.nK1, c->nVP, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
@@ -480,10 +505,11 @@ static int launch_k1(fdga_ctx* c) {
 
 template <int CH>
 static int launch_k2(fdga_ctx* c) {
-    if (!c->w_fresh && launch_w<CH>(c, c->Wc[CH])) return -3;
+    if (!c->w_fresh && launch_w<CH>(c)) return -3;
     const SymCls sym = sym_k2(c, CH);
     const size_t nw = nclass(sym, c->n2 * c->NP);
-    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->Wc[CH], vtx_F(c), chanF(c, c->FL[CH]),
+    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->Wc[CH], c->Wtail[CH], vtx_F(c),
+                                                           chanF(c, c->FL[CH]),
                                                            chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
                                                            c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
@@ -492,10 +518,11 @@ static int launch_k2(fdga_ctx* c) {
 
 // BSE_K2! of the a and t channels in one kernel (t needs the finished a channel)
 static int launch_k2_at(fdga_ctx* c) {
-    if (!c->w_fresh && (launch_w<ACH>(c, c->Wc[ACH]) || launch_w<TCH>(c, c->Wc[TCH]))) return -3;
+    if (!c->w_fresh && (launch_w<ACH>(c) || launch_w<TCH>(c))) return -3;
     const SymCls sym = symcls(c, SG_PH2);   // a and t share SGph[2]
     const size_t nw = nclass(sym, c->n2 * c->NP);
-    k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->Wc[ACH], c->Wc[TCH], vtx_F(c),
+    k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->Wc[ACH], c->Wc[TCH],
+                                                          c->Wtail[ACH], c->Wtail[TCH], vtx_F(c),
                                                           chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]), c->d.nK2b, c->d.nK2f,
                                                           c->d.nK1, c->nVP, c->NP, c->d.T, box_bound(c), sym);
     LAUNCHCHK(c);
@@ -706,6 +733,9 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     for (double2** p : {&c->Gbare, &c->G0, &c->Sig0, &c->G, &c->Sig, &c->tmpG1, &c->tmpG2, &c->tmpG3})
         rc |= dalloc(c, p, c->nGel);
     for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph, &c->Wc[0], &c->Wc[1], &c->Wc[2]}) rc |= dalloc(c, p, c->nPiel);
+    c->ntail = (size_t)c->NWP * c->NP * (2 * (size_t)c->nVP + 2);
+    for (double2** p : {&c->Wtail[0], &c->Wtail[1], &c->Wtail[2], &c->Ptail[0], &c->Ptail[1], &c->Ptail[2], &c->Ptail[3]})
+        rc |= dalloc(c, p, c->ntail);
     for (double2** p : {&c->Lpp, &c->Lph, &c->tmpL1, &c->tmpL2}) rc |= dalloc(c, p, c->nL);
     rc |= dalloc(c, &c->sde0, c->nGel);
     rc |= dalloc(c, &c->u2C, (size_t)(4 * d.nG - 1) * c->NkG);
@@ -746,7 +776,7 @@ int fdga_destroy(fdga_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (double2* p : {c->Gbare, c->G0, c->Sig0, c->G, c->Sig, c->tmpG1, c->tmpG2, c->tmpG3, c->Pi0pp, c->Pi0ph,
                        c->Pipp, c->Piph, c->Wc[0], c->Wc[1], c->Wc[2], c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C,
-                       c->stage})
+                       c->stage, c->Wtail[0], c->Wtail[1], c->Wtail[2], c->Ptail[0], c->Ptail[1], c->Ptail[2], c->Ptail[3]})
         if (p) (void)hipFree(p);
     for (auto* t : c->tab) if (t) (void)hipFree(t);
     for (auto* p : c->cache) if (p) (void)hipFree(p);
@@ -846,6 +876,7 @@ int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
         if (a.kind == 1) k_k3_to_dev<<<cdiv(a.n, 256), 256, 0, c->st>>>(a.p, c->stage, a.NW, a.NV, a.NP);
         else k_pi_to_dev<<<cdiv(a.n, 256), 256, 0, c->st>>>(a.p, c->stage, a.NW, a.NV, a.NP);
         LAUNCHCHK(c);
+        if (a.kind == 2 && tail_sums(c, a.p, pi_tail(c, a.p))) return -3;
     }
     if (which >= FDGA_TAB_FPP && which <= FDGA_TAB_FTX) {
         for (size_t i = 0; i < 2 * a.n; ++i)
@@ -1122,7 +1153,7 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     }
     stage_begin(c);
     // the K1/K2 weights depend on Π, Π0, FL and F0 only: computed once for both stages
-    if (launch_w<PCH>(c, c->Wc[PCH]) || launch_w<ACH>(c, c->Wc[ACH]) || launch_w<TCH>(c, c->Wc[TCH])) return -3;
+    if (launch_w<PCH>(c) || launch_w<ACH>(c) || launch_w<TCH>(c)) return -3;
     c->w_fresh = 1;
     int rc = 0;
     for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);

@@ -78,11 +78,15 @@ __global__ __launch_bounds__(256) void k_k3_finish(K3Fin f, const int* __restric
     const int iW = code / nchunk, e = (code - iW * nchunk) * 256 + (int)threadIdx.x;
     if (e >= blk) return;
     const size_t j = (size_t)e + (size_t)blk * iW;
-    double2 v0 = fin_load(f.raw0, f.rep0, j);
-    double2 v1 = f.n > 1 ? fin_load(f.raw1, f.rep1, j) : czero();
-    double2 v2 = f.n > 2 ? fin_load(f.raw2, f.rep2, j) : czero();
+    double2 v0 = fin_load(f.raw0, f.rep0, j), v1 = czero(), v2 = czero();
+    if (f.n > 1) v1 = fin_load(f.raw1, f.rep1, j);
+    if (f.n > 2) v2 = fin_load(f.raw2, f.rep2, j);
     if (f.tslot >= 0) {
-        const double2 a = f.aslot == 0 ? v0 : f.aslot == 1 ? v1 : f.aslot == 2 ? v2 : f.afinal[j];
+        double2 a;
+        if (f.aslot == 0) a = v0;
+        else if (f.aslot == 1) a = v1;
+        else if (f.aslot == 2) a = v2;
+        else a = f.afinal[j];
         if (f.tslot == 0) v0 = cscale(cadd(v0, a), 0.5);
         else if (f.tslot == 1) v1 = cscale(cadd(v1, a), 0.5);
         else v2 = cscale(cadd(v2, a), 0.5);
@@ -677,6 +681,62 @@ __global__ __launch_bounds__(256) void k_bse_w(double2* __restrict__ W, const do
     W[o] = cadd(cmul(csub(Pi[o], Pi0[o]), f0r), cmul(Pi[o], flr));
 }
 
+// Tail sums of rows on the bubble ν-mesh (2 nVP entries, ω index iw = ω + nVP): per row r,
+//   left[i]  = Σ_{k < i} X[k]           (i = 0 .. nVP)
+//   right[j] = Σ_{k >= nVP + j} X[k]    (j = 0 .. nVP)
+// stored as T[r * 2 (nVP + 1) + i] and T[r * 2 (nVP + 1) + nVP + 1 + j].  Each side is accumulated
+// from its outer end, so a far tail is summed from its smallest entries.
+__device__ __forceinline__ void wave_incl_scan(double& r, double& i, int lane) {
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const double yr = __shfl_up(r, o, WAVE), yi = __shfl_up(i, o, WAVE);
+        if (lane >= o) { r += yr; i += yi; }
+    }
+}
+// one 256-thread block per (row, side): thread t sums a contiguous segment (from the outer end
+// inwards), the segment totals are scanned across the block, then each thread writes its
+// segment's running sums
+__global__ __launch_bounds__(256) void k_tail_sums(const double2* __restrict__ X, double2* __restrict__ T, int nVP,
+                                                   size_t nrows) {
+    __shared__ double pr[256 / WAVE], pi[256 / WAVE];
+    const size_t row = blockIdx.x;
+    const int side = blockIdx.y, tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    if (row >= nrows) return;
+    const double2* x = X + row * (size_t)(2 * nVP);
+    double2* tl = T + row * (size_t)(2 * (nVP + 1));
+    double2* tr = tl + (nVP + 1);
+    // q-th element from the outer end of this side: left k = q, right k = 2 nVP - 1 - q
+    auto at = [&](int q) { return side == 0 ? x[q] : x[2 * nVP - 1 - q]; };
+    const int seg = (nVP + 255) / 256, q0 = tid * seg, q1 = min(nVP, q0 + seg);
+    double sr = 0.0, si = 0.0;
+    for (int q = q0; q < q1; ++q) { const double2 v = at(q); sr += v.x; si += v.y; }
+    // exclusive prefix of the segment totals: wave scan, then the preceding waves' totals
+    double ir = sr, ii = si;
+    wave_incl_scan(ir, ii, lane);
+    if (lane == WAVE - 1) { pr[wv] = ir; pi[wv] = ii; }
+    __syncthreads();
+    double br = ir - sr, bi = ii - si;
+    for (int k = 0; k < wv; ++k) { br += pr[k]; bi += pi[k]; }
+    // running sums: left tl[q + 1] = Σ_{q' <= q}; right tr[k - nVP] = Σ_{k' >= k} with k = 2 nVP - 1 - q
+    if (tid == 0) {
+        if (side == 0) tl[0] = czero();
+        else tr[nVP] = czero();
+    }
+    for (int q = q0; q < q1; ++q) {
+        const double2 v = at(q);
+        br += v.x; bi += v.y;
+        if (side == 0) tl[q + 1] = cmk(br, bi);
+        else tr[nVP - 1 - q] = cmk(br, bi);
+    }
+}
+// far sums of one row outside the window [lo, hi]: Σ_{k < lo} and Σ_{k > hi} (lo <= nVP < hi + 1)
+__device__ __forceinline__ double2 tail_left(const double2* T, size_t row, int nVP, int lo) {
+    return T[row * (size_t)(2 * (nVP + 1)) + lo];
+}
+__device__ __forceinline__ double2 tail_right(const double2* T, size_t row, int nVP, int hi) {
+    return T[row * (size_t)(2 * (nVP + 1)) + (nVP + 1) + (hi + 1 - nVP)];
+}
+
 // Frequency window of the gather-GEMVs.  Every vertex term that depends on the summed
 // frequency ω is supported on |ω| <= Wc = NK + |m| + |n| + 2, where NK bounds every box of the
 // vertex (K1 boxes reach furthest: their bosonic argument is a sum or difference of Ω, ν, ω).
@@ -695,11 +755,11 @@ __device__ __forceinline__ void far_pair(double2 t, double2& left, double2& righ
 // class (the K1 mesh has few outputs; four waves per output shorten each one's chain of gathers).
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a,
-                                                 const double2* __restrict__ W, VtxV F, int n1, int nWP, int nVP, int NP,
-                                                 double scale, int NK, SymCls sym) {
+                                                 const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
+                                                 int n1, int nWP, int nVP, int NP, double scale, int NK, SymCls sym) {
     constexpr int SP = ChSpin<CH>::SP;
     constexpr int NT = 256;
-    __shared__ double red[NT / WAVE][6];
+    __shared__ double red[NT / WAVE][2];
     const int wid = blockIdx.x, tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const int NW1 = 2 * n1 - 1;
     if (wid >= sym.count(NW1 * NP)) return;
@@ -707,33 +767,26 @@ __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const
     const int iW = o0 % NW1, p = o0 / NW1;
     const int m = iW - (n1 - 1);
     const int Wc = window(NK, m, 0);
-    const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
+    const double2* Wr = W + (size_t)(2 * nVP) * row;
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
-    double sr = 0.0, si = 0.0, lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+    double sr = 0.0, si = 0.0;
     for (int iw = lo + tid; iw <= hi; iw += NT) {
         double2 x = cmul(F.eval(m, FDGA_INF, iw - nVP, p, CH, SP, FL_ALL), Wr[iw]);
         sr += x.x; si += x.y;
     }
-    for (int iw = tid; iw < lo; iw += NT) { lr += Wr[iw].x; li += Wr[iw].y; }
-    for (int iw = hi + 1 + tid; iw < 2 * nVP; iw += NT) { rr += Wr[iw].x; ri += Wr[iw].y; }
     sr = wave_sum(sr); si = wave_sum(si);
-    lr = wave_sum(lr); li = wave_sum(li);
-    rr = wave_sum(rr); ri = wave_sum(ri);
-    if (lane == 0) {
-        red[wv][0] = sr; red[wv][1] = si; red[wv][2] = lr;
-        red[wv][3] = li; red[wv][4] = rr; red[wv][5] = ri;
-    }
+    if (lane == 0) { red[wv][0] = sr; red[wv][1] = si; }
     __syncthreads();
     if (wv != 0) return;
-    double t[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) t[q] = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+    const double tsr = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    const double tsi = red[0][1] + red[1][1] + red[2][1] + red[3][1];
     double2 tl, tr;
     far_pair(F.eval(m, FDGA_INF, lane == 0 ? -Wc - 1 : Wc + 1, p, CH, SP, FL_ALL), tl, tr);
     // every lane of wave 0 holds the same sums; lanes write the class members
-    double2 acc = cmk(t[0], t[1]);
-    if (lo > 0) acc = cadd(acc, cmul(tl, cmk(t[2], t[3])));
-    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(t[4], t[5])));
+    double2 acc = cmk(tsr, tsi);
+    if (lo > 0) acc = cadd(acc, cmul(tl, tail_left(Wtail, row, nVP, lo)));
+    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, tail_right(Wtail, row, nVP, hi)));
     const double2 raw = cscale(acc, scale);
     int b, e;
     sym.range(wid, b, e);
@@ -774,8 +827,9 @@ __device__ __forceinline__ double2 k2d_rank1(const VtxV& F, int m, int v, int p)
 }
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a,
-                                                 const double2* __restrict__ W, VtxV F, ChanV Lr, ChanV La, int n2b,
-                                                 int n2f, int nWP, int nVP, int NP, double scale, int NK, SymCls sym) {
+                                                 const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
+                                                 ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int nVP, int NP,
+                                                 double scale, int NK, SymCls sym) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= sym.count(NW2 * NV2 * NP)) return;
@@ -783,28 +837,25 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
     const int iW = o0 % NW2, iv = (o0 / NW2) % NV2, p = o0 / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
     const int Wc = window(NK, m, v);
-    const double2* Wr = W + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
-    // only the window is iterated for the summand; the far weights are summed separately
+    const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
+    const double2* Wr = W + (size_t)(2 * nVP) * row;
+    // only the window is iterated for the summand; the far weights come from the tail sums
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
-    double sr = 0.0, si = 0.0, mr = 0.0, mi = 0.0, lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+    double sr = 0.0, si = 0.0, mr = 0.0, mi = 0.0;
     for (int iw = lo + lane; iw <= hi; iw += WAVE) {
         const double2 wv = Wr[iw];
         double2 x = cmul(k2d_term<CH>(F, m, v, iw - nVP, p), wv);
         sr += x.x; si += x.y;
         mr += wv.x; mi += wv.y;
     }
-    for (int iw = lane; iw < lo; iw += WAVE) { lr += Wr[iw].x; li += Wr[iw].y; }
-    for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) { rr += Wr[iw].x; ri += Wr[iw].y; }
     double2 tl, tr;
     far_pair(k2d_term<CH>(F, m, v, lane == 0 ? -Wc - 1 : Wc + 1, p), tl, tr);
     sr = wave_sum(sr); si = wave_sum(si);
     mr = wave_sum(mr); mi = wave_sum(mi);
-    lr = wave_sum(lr); li = wave_sum(li);
-    rr = wave_sum(rr); ri = wave_sum(ri);
-    double2 acc = cmk(sr, si);
-    if (lo > 0) acc = cadd(acc, cmul(tl, cmk(lr, li)));
-    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, cmk(rr, ri)));
-    acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), cmk(mr + lr + rr, mi + li + ri)));
+    const double2 wl = lo > 0 ? tail_left(Wtail, row, nVP, lo) : czero();
+    const double2 wr = hi < 2 * nVP - 1 ? tail_right(Wtail, row, nVP, hi) : czero();
+    double2 acc = cadd(cadd(cmk(sr, si), cmul(tl, wl)), cmul(tr, wr));
+    acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), cadd(cmk(mr, mi), cadd(wl, wr))));
     const double2 raw = cscale(acc, scale);
     int b, e;
     sym.range(wid, b, e);
@@ -828,6 +879,7 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
 // K2t = (T (-1) Σ (2 ft - fa) Wt + 2 FL.t.K2 - FL.a.K2 + K2a) / 2   (src/BSE_templates.jl:95-110)
 __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, double2* __restrict__ K2t,
                                                     const double2* __restrict__ Wa, const double2* __restrict__ Wt,
+                                                    const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
                                                     VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP,
                                                     int NP, double T, int NK, SymCls sym) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
@@ -837,7 +889,8 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
     const int iW = o0 % NW2, iv = (o0 / NW2) % NV2, p = o0 / (NW2 * NV2);
     const int m = iW - (n2b - 1), v = iv - n2f;
     const int Wc = window(NK, m, v);
-    const size_t roff = (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
+    const size_t roff = (size_t)(2 * nVP) * row;
     const double2* wa = Wa + roff;
     const double2* wt = Wt + roff;
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
@@ -853,9 +906,6 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
         ar += xa.x; ai += xa.y; tr += xt.x; ti += xt.y;
         amr += wav.x; ami += wav.y; tmr += wtv.x; tmi += wtv.y;
     }
-    double alr = 0.0, ali = 0.0, arr = 0.0, ari = 0.0, tlr = 0.0, tli = 0.0, trr = 0.0, tri = 0.0;
-    for (int iw = lane; iw < lo; iw += WAVE) { alr += wa[iw].x; ali += wa[iw].y; tlr += wt[iw].x; tli += wt[iw].y; }
-    for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) { arr += wa[iw].x; ari += wa[iw].y; trr += wt[iw].x; tri += wt[iw].y; }
     // far summands: lane 0 at ω = -Wc-1, lane 1 at ω = Wc+1
     const int wf = lane == 0 ? -Wc - 1 : Wc + 1;
     const double2 fa = k2d_psp<ACH>(F, m, v, wf, p);
@@ -865,19 +915,13 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
     far_pair(csub(cscale(ft, 2.0), fa), fdl, fdr);
     ar = wave_sum(ar); ai = wave_sum(ai); tr = wave_sum(tr); ti = wave_sum(ti);
     amr = wave_sum(amr); ami = wave_sum(ami); tmr = wave_sum(tmr); tmi = wave_sum(tmi);
-    alr = wave_sum(alr); ali = wave_sum(ali); arr = wave_sum(arr); ari = wave_sum(ari);
-    tlr = wave_sum(tlr); tli = wave_sum(tli); trr = wave_sum(trr); tri = wave_sum(tri);
-    double2 A = cmk(ar, ai), D = cmk(tr, ti);
-    if (lo > 0) {
-        A = cadd(A, cmul(fal, cmk(alr, ali)));
-        D = cadd(D, cmul(fdl, cmk(tlr, tli)));
-    }
-    if (hi < 2 * nVP - 1) {
-        A = cadd(A, cmul(far_, cmk(arr, ari)));
-        D = cadd(D, cmul(fdr, cmk(trr, tri)));
-    }
-    A = cadd(A, cmul(k2d_rank1<ACH>(F, m, v, p), cmk(amr + alr + arr, ami + ali + ari)));
-    D = cadd(D, cmul(k2d_rank1<TCH>(F, m, v, p), cmk(tmr + tlr + trr, tmi + tli + tri)));
+    const bool hl = lo > 0, hr = hi < 2 * nVP - 1;
+    const double2 awl = hl ? tail_left(Wat, row, nVP, lo) : czero(), awr = hr ? tail_right(Wat, row, nVP, hi) : czero();
+    const double2 twl = hl ? tail_left(Wtt, row, nVP, lo) : czero(), twr = hr ? tail_right(Wtt, row, nVP, hi) : czero();
+    double2 A = cadd(cadd(cmk(ar, ai), cmul(fal, awl)), cmul(far_, awr));
+    double2 D = cadd(cadd(cmk(tr, ti), cmul(fdl, twl)), cmul(fdr, twr));
+    A = cadd(A, cmul(k2d_rank1<ACH>(F, m, v, p), cadd(cmk(amr, ami), cadd(awl, awr))));
+    D = cadd(D, cmul(k2d_rank1<TCH>(F, m, v, p), cadd(cmk(tmr, tmi), cadd(twl, twr))));
     const double2 rawa = cscale(A, T), rawt = cscale(D, -T);
     int b, e;
     sym.range(wid, b, e);
@@ -958,7 +1002,8 @@ __device__ __forceinline__ void sde_terms(const ChanV& gp, const ChanV& gt, cons
 // classes (SGpp2, SGph2), so each is launched separately over its classes.
 template <int WHICH>
 __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double2* __restrict__ Lph, const double2* __restrict__ Pipp,
-                        const double2* __restrict__ Piph, ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
+                        const double2* __restrict__ Piph, const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
+                        ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
                         int n2b, int n2f, int nWP, int nVP, int NP, int chanNP, double2 TU, int NK, SymCls sym) {
     constexpr bool DP = WHICH & 1, DH = WHICH & 2;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
@@ -970,38 +1015,28 @@ __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double
     const int pc = chanNP > 1 ? p : 0;
     const int Wc = window(NK, m, v);
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
-    const size_t base = (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
+    const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
+    const size_t base = (size_t)(2 * nVP) * row;
     double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
-    double plr = 0.0, pli = 0.0, prr = 0.0, pri = 0.0, hlr = 0.0, hli = 0.0, hrr = 0.0, hri = 0.0;
     for (int iw = lo + lane; iw <= hi; iw += WAVE) {
         double2 xp, xh;
         sde_terms<DP, DH>(gp, gt, ga, ref, kind, m, v, iw - nVP, pc, xp, xh);
         if (DP) { double2 a = cmul(Pipp[base + iw], xp); ar += a.x; ai += a.y; }
         if (DH) { double2 b = cmul(Piph[base + iw], xh); br += b.x; bi += b.y; }
     }
-    for (int iw = lane; iw < lo; iw += WAVE) {
-        if (DP) { plr += Pipp[base + iw].x; pli += Pipp[base + iw].y; }
-        if (DH) { hlr += Piph[base + iw].x; hli += Piph[base + iw].y; }
-    }
-    for (int iw = hi + 1 + lane; iw < 2 * nVP; iw += WAVE) {
-        if (DP) { prr += Pipp[base + iw].x; pri += Pipp[base + iw].y; }
-        if (DH) { hrr += Piph[base + iw].x; hri += Piph[base + iw].y; }
-    }
     double2 fxp, fxh, xpl, xpr, xhl, xhr;
     sde_terms<DP, DH>(gp, gt, ga, ref, kind, m, v, lane == 0 ? -Wc - 1 : Wc + 1, pc, fxp, fxh);
     far_pair(fxp, xpl, xpr);
     far_pair(fxh, xhl, xhr);
     ar = wave_sum(ar); ai = wave_sum(ai); br = wave_sum(br); bi = wave_sum(bi);
-    plr = wave_sum(plr); pli = wave_sum(pli); prr = wave_sum(prr); pri = wave_sum(pri);
-    hlr = wave_sum(hlr); hli = wave_sum(hli); hrr = wave_sum(hrr); hri = wave_sum(hri);
     double2 A = cmk(ar, ai), B = cmk(br, bi);
     if (lo > 0) {
-        A = cadd(A, cmul(cmk(plr, pli), xpl));
-        B = cadd(B, cmul(cmk(hlr, hli), xhl));
+        if (DP) A = cadd(A, cmul(tail_left(Tpp, row, nVP, lo), xpl));
+        if (DH) B = cadd(B, cmul(tail_left(Tph, row, nVP, lo), xhl));
     }
     if (hi < 2 * nVP - 1) {
-        A = cadd(A, cmul(cmk(prr, pri), xpr));
-        B = cadd(B, cmul(cmk(hrr, hri), xhr));
+        if (DP) A = cadd(A, cmul(tail_right(Tpp, row, nVP, hi), xpr));
+        if (DH) B = cadd(B, cmul(tail_right(Tph, row, nVP, hi), xhr));
     }
     const double2 rawp = cmul(A, TU), rawh = cmul(B, TU);
     int b, e;
