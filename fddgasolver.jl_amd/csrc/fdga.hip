@@ -83,8 +83,8 @@ struct fdga_ctx {
     double2* k3raw = nullptr;                                       // raw t-channel K3 under groups [n3]
     // timing
     int timing = 0;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    double stage_ms[9] = {};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double stage_ms[11] = {};   // 0..8 stages of fdga_iterate, 9 / 10 the BSE_L_K3! / BSE_K3! GEMM kernels
 };
 
 namespace {
@@ -310,6 +310,18 @@ void stage_end(fdga_ctx* c, int s) {
     (void)hipEventSynchronize(c->ev[1]);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->stage_ms[s] += ms;
+}
+// event timing of one kernel launch inside a stage (ev[2], ev[3])
+void kernel_begin(fdga_ctx* c) {
+    if (c->timing) (void)hipEventRecord(c->ev[2], c->st);
+}
+void kernel_end(fdga_ctx* c, int s) {
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev[3], c->st);
+    (void)hipEventSynchronize(c->ev[3]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
     c->stage_ms[s] += ms;
 }
 
@@ -615,8 +627,10 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     LK3_Ep e{g, cc[0], cc[1], cc[2]};
     size_t nl = 0;
     const int* bl = k3_blocks(c, chans, n, true, nl);
+    kernel_begin(c);
     if (zgemm_launch(c->d.precision, n * NW * c->NP, c->st, NV, NV, NV, tm, tn, bl, nl, a, b, e) != hipSuccess)
         return fail(c, "BSE_L_K3 GEMM launch failed", -3);
+    kernel_end(c, 9);
     return k3_finish(c, c->FL, chans, n, true, sym, t_epi);
 }
 
@@ -664,8 +678,10 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
     size_t nl = 0;
     const int* bl = k3_blocks(c, chans, n, false, nl);
+    kernel_begin(c);
     if (zgemm_launch(c->d.precision, n * NW * c->NP, c->st, NV, NV, NV, tm, tn, bl, nl, a, b, e) != hipSuccess)
         return fail(c, "BSE_K3 GEMM launch failed", -3);
+    kernel_end(c, 10);
     return k3_finish(c, c->Fb, chans, n, false, sym, t_epi);
 }
 
@@ -757,7 +773,8 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
         rc |= dalloc(c, &c->csum[r], (size_t)c->NV3 * c->NW3 * c->NP);
     }
     if (hipMalloc((void**)&c->scal, 8 * sizeof(double)) != hipSuccess) rc = -2;
-    if (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess) rc = -2;
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = -2;
     if (rc) {
         if (c->err.empty()) c->err = "device allocation failed";
         *out = c;
@@ -1254,7 +1271,7 @@ int fdga_time_iterations(fdga_ctx* c, int32_t strategy, int32_t n, double* ms) {
 }
 
 int fdga_stage_ms(const fdga_ctx* c, int32_t stage, double* ms) {
-    if (!c || !ms || stage < 0 || stage > 8) return -1;
+    if (!c || !ms || stage < 0 || stage > 10) return -1;
     *ms = c->stage_ms[stage];
     return 0;
 }

@@ -11,7 +11,7 @@ module FdgaHIP
 using fdDGAsolver
 using MatsubaraFunctions
 import fdDGAsolver: iterate_solver!, bubbles!, build_K3_cache!, BSE_K1!, BSE_K2!, BSE_K3!, BSE_L_K2!, BSE_L_K3!,
-                    Dyson!, SDE!
+                    Dyson!, SDE!, init_sym_grp!, reset_sym_grp!
 
 const libfdga = joinpath(@__DIR__, "..", "lib", "libfdga.so")
 
@@ -20,7 +20,7 @@ struct FdgaDims
     nonlocal::Int32; nG::Int32; nK1::Int32; nK2b::Int32; nK2f::Int32; nK3b::Int32; nK3f::Int32
     mPinu::Int32; LG::Int32; L::Int32; T::Float64; U::Float64; ref_kind::Int32
     r_nK1::Int32; r_nK2b::Int32; r_nK2f::Int32; r_nK3b::Int32; r_nK3f::Int32
-    tab_nb::Int32; tab_nf::Int32; fdpa_h0::Int32; precision::Int32
+    tab_nb::Int32; tab_nf::Int32; fdpa_h0::Int32; precision::Int32   # precision: 0 FP64, 1 FP32-MFMA contraction
 end
 
 const PCH, TCH, ACH = Int32(0), Int32(1), Int32(2)
@@ -66,6 +66,19 @@ function context(S::ParquetSolver{ComplexF64})
         finalizer(_ -> ccall((:fdga_destroy, libfdga), Cint, (Ptr{Cvoid},), ctx[]), S)
         ctx[]
     end
+end
+
+# init_sym_grp! / reset_sym_grp! (src/ParquetSolver.jl:203-243): the host groups for the Julia
+# side and the device class tables for the iteration
+function init_sym_grp!(S::ParquetSolver{ComplexF64}, ::Val{:hip})
+    invoke(init_sym_grp!, Tuple{ParquetSolver}, S)
+    ctx = context(S)
+    check(ctx, ccall((:fdga_init_sym_grp, libfdga), Cint, (Ptr{Cvoid},), ctx))
+end
+function reset_sym_grp!(S::ParquetSolver{ComplexF64}, ::Val{:hip})
+    invoke(reset_sym_grp!, Tuple{fdDGAsolver.AbstractSolver}, S)
+    ctx = context(S)
+    check(ctx, ccall((:fdga_reset_sym_grp, libfdga), Cint, (Ptr{Cvoid},), ctx))
 end
 
 # iterate_solver! for mode :hip: one device pass between flatten/unflatten of the solver state
