@@ -111,6 +111,69 @@ def time_path(S, args, names):
     return ms / args.steps, dict(zip(names, st))
 
 
+C2 = dict(L=16, T=0.2, U=2.0, mu=0.0, t1=1.0, nK1=64, nK2=(32, 32), nK3=(16, 32), mPinu=32, nG=256)
+
+
+def build_c2(precision, w=C2):
+    """Nonlocal 2D Hubbard fdPA solver (NL_ParquetSolver, src/nonlocal/ParquetSolver.jl:77-152) on a
+    16 x 16 lattice with a synthetic local reference Vertex: K1 = K2 = 0 and K3 a seed-0 complex
+    Gaussian times 0.1 U / (1 + |ν| + |ν'|) (ν in units of πT), G0 the bare lattice G, Σ0 = 0."""
+    import numpy as np
+    import fdga
+    L = w["L"]
+    Gb = fdga.hubbard_bare_Green(w["nG"], w["T"], L, μ=w["mu"], t1=w["t1"])
+    nb, nf = w["nK3"]
+    rng = np.random.default_rng(0)
+    NW, NV = 2 * nb - 1, 2 * nf
+    n = np.arange(-nf, nf)
+    amp = 0.1 * w["U"] / (1 + np.abs(2 * n[None, :] + 1) + np.abs(2 * n[:, None] + 1))   # [ν', ν] -> (ν, ν') symmetric
+    F0 = {}
+    for ch in range(3):
+        k3 = (rng.standard_normal((NW, NV, NV)) + 1j * rng.standard_normal((NW, NV, NV))) * amp[None, :, :]
+        F0[(ch, 1)] = np.zeros(2 * w["nK1"] - 1, complex)
+        F0[(ch, 2)] = np.zeros((2 * w["nK2"][0] - 1) * 2 * w["nK2"][1], complex)
+        F0[(ch, 3)] = k3.reshape(-1, order="F")
+    S = fdga.NL_ParquetSolver(w["nK1"], w["nK2"], w["nK3"], L, Gb, Gb, np.zeros_like(Gb), F0, T=w["T"], U=w["U"],
+                              mΠν_factor=w["mPinu"], F0_boxes=(w["nK1"], w["nK2"], w["nK3"]), precision=precision)
+    return S
+
+
+def main_c2(args):
+    """One-GPU timing of the C2 lattice iteration in both precisions, from the same state; the
+    FP32-MFMA path's relative error against FP64 after one iteration."""
+    import numpy as np
+    import fdga
+    names = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE", "gemm_L_K3", "gemm_K3"]
+    w = C2
+    NW3, NV3, NP = 2 * w["nK3"][0] - 1, 2 * w["nK3"][1], w["L"] ** 2
+    f_dense = 2 * 3 * 8.0 * NW3 * NP * NV3 ** 3
+    res, x0, first = {}, None, {}
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        S = build_c2(prec)
+        log(f"bench c2[{prec}]: solver built ({time.perf_counter() - t0:.1f} s)")
+        if x0 is None:
+            x0 = fdga.flatten(S)
+        else:
+            fdga.unflatten(S, x0)
+        fdga.iterate_solver(S, "fdPA")
+        S.sync()
+        first[prec] = fdga.flatten(S)
+        fdga.unflatten(S, x0)
+        ms, st = time_path(S, args, names)
+        log(f"bench c2[{prec}]: {ms:.2f} ms/iteration; stages " + ", ".join(f"{k}={v:.2f}" for k, v in st.items()))
+        t_bse = (st["gemm_L_K3"] + st["gemm_K3"]) * 1e-3
+        res[prec] = {"ms_per_step": ms, "iters_per_s": 1000.0 / ms, "stage_ms": st,
+                     "bse_tflops": f_dense / t_bse / 1e12}
+        S.close()
+    a, b = first["fp32"], first["fp64"]
+    rel = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    print(json.dumps({"workload": "c2: NL Hubbard fdPA, L=16, beta=5, U=2, nK1=64, nK2=(32,32), nK3=(16,32), mPinu=32, "
+                                  "nG=256, synthetic local reference vertex", "steps": args.steps, "warmup": args.warmup,
+                      "fp64": res["fp64"], "fp32_mfma": res["fp32"],
+                      "fp32_vs_fp64_state_relerr_after_one_iteration": rel}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,11 +187,16 @@ def main():
     ap.add_argument("--no-dense", action="store_true", help="skip the identity-group FP64 timing")
     ap.add_argument("--tol32", type=float, default=1e-7,
                     help="Anderson tolerance of the FP32-MFMA solve (its residual floor is ~5e-8 at this workload)")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2"],
+                    help="c1: the headline SIAM workload (one JSON line of the metric); c2: the NL Hubbard lattice "
+                         "iteration (timing and FP32 error, separate JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the parquet hot path is run on a single GPU in this build "
               "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c2":
+        return main_c2(args)
     import fdga
     w = WORKLOAD
     names = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE", "gemm_L_K3", "gemm_K3"]

@@ -658,6 +658,7 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
         rc[s].csum = c->csum[ch];
         kc[s].ch = ch;
         kc[s].Fbase = c->cache[idx_fbase(ch)];
+        kc[s].full = (c->NP == 1);
         kc[s].K3r = c->F[ch].K3;
         kc[s].K3a = c->F[ACH].K3;
         kc[s].K2r = c->F[ch].K2;
@@ -673,14 +674,17 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     const size_t ncol = (size_t)NV * NW * c->NP * n;
     k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->st>>>(g, rc[0], rc[1], rc[2], n);
     LAUNCHCHK(c);
-    K3_Aop a{g, c->d.T, kc[0], kc[1], kc[2]};
     K3_Bop b{g, c->d.T, kc[0], kc[1], kc[2]};
     K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
     size_t nl = 0;
     const int* bl = k3_blocks(c, chans, n, false, nl);
     kernel_begin(c);
-    if (zgemm_launch(c->d.precision, n * NW * c->NP, c->st, NV, NV, NV, tm, tn, bl, nl, a, b, e) != hipSuccess)
-        return fail(c, "BSE_K3 GEMM launch failed", -3);
+    hipError_t ge;
+    if (c->NP == 1) ge = zgemm_launch(c->d.precision, n * NW, c->st, NV, NV, NV, tm, tn, bl, nl,
+                                      K3_AopT<true>{g, c->d.T, kc[0], kc[1], kc[2]}, b, e);
+    else ge = zgemm_launch(c->d.precision, n * NW * c->NP, c->st, NV, NV, NV, tm, tn, bl, nl,
+                           K3_AopT<false>{g, c->d.T, kc[0], kc[1], kc[2]}, b, e);
+    if (ge != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
     return k3_finish(c, c->Fb, chans, n, false, sym, t_epi);
 }
@@ -920,7 +924,7 @@ int fdga_get(fdga_ctx* c, int32_t which, double* host) {
         else {
             int ch = a.ch == 0 ? PCH : a.ch == 1 ? ACH : TCH;
             k_cache_F_julia<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, chanF(c, c->F[ch]), chanF(c, c->F[ACH]),
-                                                               ch, c->d.nK3b, c->d.nK3f, c->NP);
+                                                               ch, c->d.nK3b, c->d.nK3f, c->NP, c->NP == 1);
         }
         LAUNCHCHK(c);
         HIPCHK(c, hipMemcpyAsync(host, c->stage, bytes, hipMemcpyDeviceToHost, c->st));
@@ -1077,7 +1081,7 @@ int fdga_build_k3_cache(fdga_ctx* c) {
     for (int k = 0; k < 10; ++k) cp.c[k] = c->cache[k];
     dim3 g(cdiv(c->NV3, 64), c->NV3, c->NW3);
     k_cache1<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
-    k_cache2<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
+    k_cache2<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f, c->NP == 1);
     LAUNCHCHK(c);
     return 0;
 }

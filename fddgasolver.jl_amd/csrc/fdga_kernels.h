@@ -235,8 +235,11 @@ __global__ __launch_bounds__(64) void k_cache1(CachePtrs out, VtxV F, int n3b, i
     out.c[3][o] = csub(cscale(f0t, 2.0), f0a);
 }
 
-// loop 2 of build_K3_cache! at (Ω, ν, ω): Γ_irr of each channel and the P-independent F caches
-__global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, int n3f) {
+// loop 2 of build_K3_cache! at (Ω, ν, ω): Γ_irr of each channel and the F caches.  With `full`
+// (one momentum) the F caches are the reference's cache_Fr = F0 part + Γr + K2r(Ω,ν) + K3r(Ω,ν,ω)
+// (t: 2 (t) - (a)); otherwise (NL) they hold only the P-independent part Fbase_r and the same-channel
+// K2r + K3r(P) are added by the BSE_K3! loader and epilogue.
+__global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, int n3f, int full) {
     const int NV = 2 * n3f;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
@@ -254,15 +257,21 @@ __global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, i
     out.c[4][o] = gpp;
     out.c[5][o] = ga;
     out.c[6][o] = csub(cscale(gt, 2.0), ga);
-    out.c[7][o] = cadd(dp, gpp);
-    double2 fba = cadd(da, ga);
+    double2 fbp = cadd(dp, gpp), fba = cadd(da, ga), fbt = cadd(dt, gt);
+    if (full) {
+        // within the K3 box F(ν) - F(νInf) of the channel's own γ is K2(Ω, ν) + K3(Ω, ν, ω)
+        fbp = cadd(fbp, cadd(F.g[PCH].k2(m, a, 0), F.g[PCH].k3(m, a, b, 0)));
+        fba = cadd(fba, cadd(F.g[ACH].k2(m, a, 0), F.g[ACH].k3(m, a, b, 0)));
+        fbt = cadd(fbt, cadd(F.g[TCH].k2(m, a, 0), F.g[TCH].k3(m, a, b, 0)));
+    }
+    out.c[7][o] = fbp;
     out.c[8][o] = fba;
-    out.c[9][o] = csub(cscale(cadd(dt, gt), 2.0), fba);
+    out.c[9][o] = csub(cscale(fbt, 2.0), fba);
 }
 
 // materialise cache_F*[Ω, ν, ω, P] in the reference layout (ABI read-back only)
 __global__ void k_cache_F_julia(double2* __restrict__ out, const double2* __restrict__ base, ChanV gr, ChanV ga, int ch,
-                                int n3b, int n3f, int NP) {
+                                int n3b, int n3f, int NP, int full) {
     const int NV = 2 * n3f, NW = 2 * n3b - 1;
     size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     size_t tot = (size_t)NW * NV * NV * NP;
@@ -274,6 +283,7 @@ __global__ void k_cache_F_julia(double2* __restrict__ out, const double2* __rest
     int p = (int)(r / NV);
     const int m = iW - (n3b - 1), v = i - n3f, w = j - n3f;
     double2 x = base[(size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * iW)];
+    if (full) { out[e] = x; return; }
     double2 dr = cadd(gr.k2(m, v, p), gr.k3(m, v, w, p));
     if (ch == TCH) dr = csub(cscale(dr, 2.0), cadd(ga.k2(m, v, p), ga.k3(m, v, w, p)));
     out[e] = cadd(x, dr);
@@ -419,15 +429,21 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
                 br[j] = Bs[0][kr][wn + 16 * j + c];
                 bi[j] = Bs[1][kr][wn + 16 * j + c];
             }
+            // B fragment as the MFMA's left operand: the accumulator is the transposed tile, so
+            // lane % 16 runs along the output rows (contiguous in memory).  The eight first terms
+            // are independent; each accumulator's second term follows eight MFMAs later.
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    // B fragment as the MFMA's left operand: the accumulator is the transposed
-                    // tile, so lane % 16 runs along the output rows (contiguous in memory)
                     accr[i][j] = MT::mfma(br[j], ar[i], accr[i][j]);
-                    accr[i][j] = MT::mfma(bi[j], nai[i], accr[i][j]);
                     acci[i][j] = MT::mfma(bi[j], ar[i], acci[i][j]);
+                }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = MT::mfma(bi[j], nai[i], accr[i][j]);
                     acci[i][j] = MT::mfma(br[j], ai[i], acci[i][j]);
                 }
         }
@@ -531,6 +547,7 @@ struct LK3_Ep {
 //   K3 = T (A_3 R + K2r(ν) Σ_ω R(ω, νp)) + L_self      (A_3: A without the ω-independent K2 part)
 struct K3Chan {
     int ch;
+    int full;              // Fbase is the whole cache_Fr (one momentum): no K2r/K3r terms here
     const double2* Fbase;  // [ν, ω, Ω]
     const double2* K3r;    // target vertex K3 of this channel [ν, νp, Ω, P]
     const double2* K3a;    // target vertex K3 of the a channel (t channel combination)
@@ -544,14 +561,17 @@ struct K3Chan {
     const double2* outa;   // Fbuff.γa.K3 (in-launch t post-processing)
     int tpost;
 };
-struct K3_Aop {
+// FULL: the cache holds the whole cache_Fr (one momentum, a single load per element)
+template <bool FULL>
+struct K3_AopT {
     K3Geom g; double T; K3Chan c0, c1, c2;
     __device__ __forceinline__ double2 op(const K3Chan& c, int bb, int i, int k) const {
         const int NV = g.NV(), NW = g.NW();
         const int iW = bb % NW;
         const size_t cij = (size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW);
+        const double2 x = c.Fbase[cij];
+        if (FULL) return x;
         const size_t vij = (size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * bb);
-        double2 x = c.Fbase[cij];
         if (c.ch == TCH) return cadd(x, csub(cscale(c.K3r[vij], 2.0), c.K3a[vij]));
         return cadd(x, c.K3r[vij]);
     }
@@ -560,7 +580,7 @@ struct K3_Aop {
         return s == 0 ? op(c0, bb, i, k) : s == 1 ? op(c1, bb, i, k) : op(c2, bb, i, k);
     }
 };
-template <> struct Prefetch<struct K3_Aop> { static constexpr bool on = false; };
+template <> struct Prefetch<K3_AopT<false>> { static constexpr bool on = false; };
 struct K3_Bop {
     K3Geom g; double T; K3Chan c0, c1, c2;
     __device__ __forceinline__ double2 op(const K3Chan& c, int bb, int k, int j) const {
@@ -579,9 +599,11 @@ struct K3_Ep {
         const int iW = bb % NW, p = bb / NW;
         const int m = iW - (g.n3b - 1), nu = i - g.n3f;
         const size_t k2 = (size_t)(m + g.n2b - 1) + (size_t)(2 * g.n2b - 1) * ((size_t)(nu + g.n2f) + (size_t)(2 * g.n2f) * p);
-        double2 k2v = c.K2r[k2];
-        if (c.ch == TCH) k2v = csub(cscale(k2v, 2.0), c.K2a[k2]);
-        v = cadd(v, cmul(k2v, c.csum[(size_t)j + (size_t)NV * bb]));
+        if (!c.full) {
+            double2 k2v = c.K2r[k2];
+            if (c.ch == TCH) k2v = csub(cscale(k2v, 2.0), c.K2a[k2]);
+            v = cadd(v, cmul(k2v, c.csum[(size_t)j + (size_t)NV * bb]));
+        }
         const size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * bb);
         double2 self = c.Lr[o];
         if (c.ch == TCH) self = csub(cscale(self, 2.0), c.La[o]);
