@@ -81,11 +81,19 @@ struct fdga_ctx {
     int* finmap = nullptr;    // k_k3_finish chunk map (XCD-paired Ω blocks)
     int finmap_n = 0, fin_nchunk = 0;
     double2* k3raw = nullptr;                                       // raw t-channel K3 under groups [n3]
+    // hipGraph replay of fdga_iterate(strategy, 1): one graph per strategy and per F/Fbuff pointer
+    // state (set!(S.F, S.Fbuff) swaps the two buffers every iteration)
+    int graph_on = 0;
+    int swap_state = 0;
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     // timing
     int timing = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double stage_ms[11] = {};   // 0..8 stages of fdga_iterate, 9 / 10 the BSE_L_K3! / BSE_K3! GEMM kernels
 };
+
+// drops the captured hipGraphs of the iteration (defined with the graph replay below)
+void graphs_invalidate(fdga_ctx* c);
 
 namespace {
 
@@ -810,6 +818,7 @@ int fdga_destroy(fdga_ctx* c) {
         if (c->R0[r].K1) free_chan(c->R0[r]);
     }
     if (c->scal) (void)hipFree(c->scal);
+    graphs_invalidate(c);
     for (auto& g : c->sg) free_sym(g);
     free_blists(c);
     if (c->k3raw) (void)hipFree(c->k3raw);
@@ -879,8 +888,10 @@ int64_t fdga_size(const fdga_ctx* c, int32_t which) {
 
 static void invalidate(fdga_ctx* c, int which) {
     if (which == FDGA_G0 || which == FDGA_PI0PP || which == FDGA_PI0PH || (which >= FDGA_TAB_FPP && which <= FDGA_TAB_FTX) ||
-        (which >= FDGA_VERTEX(3, 0, 1) && which <= FDGA_VERTEX(3, 2, 3)))
+        (which >= FDGA_VERTEX(3, 0, 1) && which <= FDGA_VERTEX(3, 2, 3))) {
         c->sde0_valid = 0;
+        graphs_invalidate(c);   // a captured fdPA iteration reuses the cached SDE of the reference
+    }
 }
 
 int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
@@ -942,6 +953,7 @@ int fdga_init(fdga_ctx* c) {
     if (bubbles_into(c, c->Pipp, c->Piph, c->G)) return -3;
     if (refresh_avg(c)) return -3;
     c->sde0_valid = 0;
+    graphs_invalidate(c);
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
@@ -971,6 +983,7 @@ int fdga_init_sym_grp(fdga_ctx* c) {
                        "(the NL solver evaluates every element)");
     using namespace fdga_sym;
     HIPCHK(c, hipStreamSynchronize(c->st));
+    graphs_invalidate(c);
     const Axis aS{false, d.nG}, a1{true, d.nK1}, b2{true, d.nK2b}, f2{false, d.nK2f}, b3{true, d.nK3b}, f3{false, d.nK3f};
     const int64_t NW = c->NW3, NV = c->NV3;
     // reference K3 index iΩ + NW (iν + NV iν') -> device index iν + NV (iν' + NV iΩ)
@@ -1061,6 +1074,7 @@ int fdga_init_sym_grp(fdga_ctx* c) {
 int fdga_reset_sym_grp(fdga_ctx* c) {
     if (!c) return -1;
     HIPCHK(c, hipStreamSynchronize(c->st));
+    graphs_invalidate(c);
     // SGΣ, SGpp, SGph, SGppL, SGphL become identity groups; SG0pp2 / SG0ph2 are left as they are
     for (int id : {SG_S, SG_K1, SG_PP2, SG_PH2, SG_PP3, SG_PH3, SG_PPL3, SG_PHL3}) free_sym(c->sg[id]);
     free_blists(c);
@@ -1120,8 +1134,11 @@ int fdga_bse_k3(fdga_ctx* c, int32_t ch) {
     return launch_k3(c, chans, 1);
 }
 
-int fdga_update_F(fdga_ctx* c) {
-    // Fbuff is overwritten in full by every iteration, so set!(S.F, S.Fbuff) is a buffer swap
+}  // extern "C"
+
+namespace {
+// host side of set!(S.F, S.Fbuff): exchange the buffers (an involution)
+void swap_F_host(fdga_ctx* c) {
     for (int r = 0; r < 3; ++r) {
         std::swap(c->F[r].K1, c->Fb[r].K1);
         std::swap(c->F[r].K2, c->Fb[r].K2);
@@ -1129,6 +1146,25 @@ int fdga_update_F(fdga_ctx* c) {
         if (!c->F[r].own_avg) { c->F[r].A1 = c->F[r].K1; c->F[r].A2 = c->F[r].K2; c->F[r].A3 = c->F[r].K3; }
         c->Fb[r].A1 = c->Fb[r].K1; c->Fb[r].A2 = c->Fb[r].K2; c->Fb[r].A3 = c->Fb[r].K3;
     }
+    c->swap_state ^= 1;
+}
+}  // namespace
+
+// drop the captured iterations (their kernel arguments no longer describe the context)
+void graphs_invalidate(fdga_ctx* c) {
+    for (auto& row : c->gexec)
+        for (auto& g : row)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+}
+
+extern "C" {
+
+int fdga_update_F(fdga_ctx* c) {
+    // Fbuff is overwritten in full by every iteration, so set!(S.F, S.Fbuff) is a buffer swap
+    swap_F_host(c);
     return refresh_avg(c);
 }
 
@@ -1152,9 +1188,48 @@ int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
     return 0;
 }
 
+static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma);
+
+// Captures fdga_iterate(strategy, 1) for both F/Fbuff pointer states.  Capture records the launches
+// without running them, so the host-side buffer swap done while recording is undone after each.
+static int capture_iteration(fdga_ctx* c, int strategy) {
+    for (int k = 0; k < 2; ++k) {
+        const int state = c->swap_state;
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        const int rc = iterate_eager(c, strategy, 1);
+        const hipError_t ec = hipStreamEndCapture(c->st, &g);
+        if (rc || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return fail(c, "hipGraph capture of the iteration failed", -3);
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiateWithFlags(&ex, g, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(c, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei), -3);
+        c->gexec[strategy][state] = ex;
+        swap_F_host(c);   // undo the swap recorded by update_F ...
+        swap_F_host(c);   // ... and move to the other pointer state for the next capture
+    }
+    return 0;
+}
+
 int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     if (!c) return -1;
     if (strategy != FDGA_SCPA && strategy != FDGA_FDPA) return fail(c, "Calculation strategy unknown");
+    // graph replay: the fixed-point iteration with Σ update, stage timing off, and for fdPA once the
+    // reference system's SDE is cached (the first fdPA iteration computes it eagerly)
+    if (c->graph_on && update_sigma && !c->timing && (strategy == FDGA_SCPA || c->sde0_valid)) {
+        if (!c->gexec[strategy][c->swap_state] && capture_iteration(c, strategy)) return -3;
+        HIPCHK(c, hipGraphLaunch(c->gexec[strategy][c->swap_state], c->st));
+        swap_F_host(c);
+        return 0;
+    }
+    return iterate_eager(c, strategy, update_sigma);
+}
+
+static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     stage_begin(c);
     if (update_sigma) {
         if (fdga_dyson(c) || fdga_bubbles(c)) return -3;
@@ -1309,6 +1384,15 @@ int64_t fdga_gemm_blocks(const fdga_ctx* c, int32_t left) {
     size_t nl = 0, nt = (size_t)3 * c->NW3 * c->NP * cdiv(c->NV3, GBM) * cdiv(c->NV3, GBN);
     (void)k3_blocks(c, chans, 3, left != 0, nl, &nt);
     return (int64_t)nt;
+}
+
+// replay fdga_iterate(strategy, 1) from captured hipGraphs (1) or launch eagerly (0)
+int fdga_set_graph(fdga_ctx* c, int32_t on) {
+    if (!c) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (!on) graphs_invalidate(c);
+    c->graph_on = on ? 1 : 0;
+    return 0;
 }
 
 // enable (1) / disable (0) per-stage event timing; resets the accumulated stage times

@@ -56,6 +56,7 @@ SIGNATURES = {
     "fdga_init": (ctypes.c_int, [_vp]),
     "fdga_init_sym_grp": (ctypes.c_int, [_vp]),
     "fdga_gemm_blocks": (_i64, [_vp, _i32]),
+    "fdga_set_graph": (ctypes.c_int, [_vp, _i32]),
     "fdga_reset_sym_grp": (ctypes.c_int, [_vp]),
     "fdga_dyson": (ctypes.c_int, [_vp]),
     "fdga_bubbles": (ctypes.c_int, [_vp]),

@@ -131,6 +131,10 @@ class Solver:
         self._check(self.lib.fdga_time_iterations(self.h, _strategy(strategy), n, ctypes.byref(ms)))
         return ms.value
 
+    def set_graph(self, on):
+        """Replay each fixed-point iteration from a captured hipGraph (True) or launch eagerly."""
+        self._check(self.lib.fdga_set_graph(self.h, int(on)))
+
     def set_stage_timing(self, on):
         self._check(self.lib.fdga_set_stage_timing(self.h, int(on)))
 

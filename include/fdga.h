@@ -102,6 +102,11 @@ int fdga_stage_ms(const fdga_ctx* ctx, int32_t stage, double* ms);
 /* enable (1) / disable (0) per-stage event timing inside fdga_iterate (adds a host sync per
  * stage); resets the accumulated stage times */
 int fdga_set_stage_timing(fdga_ctx* ctx, int32_t on);
+/* Replay fdga_iterate(ctx, strategy, 1) from hipGraphs captured on first use (1), or launch the
+ * kernels one by one (0, the default).  Results are identical; the graphs are recaptured after
+ * fdga_init, fdga_init_sym_grp / fdga_reset_sym_grp and after fdga_set of G0, Π0, the RefVertex
+ * tables or the reference Vertex.  No counterpart in the reference (execution option). */
+int fdga_set_graph(fdga_ctx* ctx, int32_t on);
 /* number of 64 x 64 output tiles (each over the full contraction length) that the batched (p, a, t)
  * BSE_L_K3! (left = 1) or BSE_K3! (left = 0) GEMM launch computes; fewer than the dense count
  * when symmetry groups are installed.  No counterpart in the reference (measurement). */

@@ -169,3 +169,29 @@ def test_fp32_contraction_iteration(case, fdga_mod):
     O.iterate(po.FDPA)
     e = relerr(fdga_mod.flatten(S), O.flatten())
     assert 1e-12 < e < 1e-5, e
+
+
+@pytest.mark.parametrize("case,strategy", [("local_vertexref", "fdPA"), ("nl_L2", "fdPA"), ("local_ref", "scPA")])
+def test_graph_replay_equals_eager(case, strategy, fdga_mod):
+    """hipGraph replay of the iteration: four replays in a row, each followed by eager work
+    (flatten, a device copy plus a sync), agree with the eager iterations; then a fixed_point call
+    (unflatten + replay) and a symmetry-group change (recapture) on the local cases."""
+    E, _ = make_pair(case, seed=9)
+    G, _ = make_pair(case, seed=9)
+    G.set_graph(True)
+    for it in range(4):
+        fdga_mod.iterate_solver(E, strategy)
+        fdga_mod.iterate_solver(G, strategy)
+        x, y = fdga_mod.flatten(E), fdga_mod.flatten(G)
+        assert relerr(y, x) < 1e-14, f"replay {it}: {relerr(y, x):.3e}"
+    R1, R2 = np.empty_like(x), np.empty_like(x)
+    fdga_mod.fixed_point(R1, x * 0.9, E, strategy)
+    fdga_mod.fixed_point(R2, x * 0.9, G, strategy)
+    assert relerr(R2, R1) < 1e-14
+    if case.startswith("local"):
+        E.init_sym_grp()
+        G.init_sym_grp()
+        for it in range(2):
+            fdga_mod.iterate_solver(E, strategy)
+            fdga_mod.iterate_solver(G, strategy)
+            assert relerr(fdga_mod.flatten(G), fdga_mod.flatten(E)) < 1e-14
