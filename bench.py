@@ -174,6 +174,51 @@ def main_c2(args):
                       "fp32_vs_fp64_state_relerr_after_one_iteration": rel}), flush=True)
 
 
+C0 = dict(T=0.1, U=1.0, Delta=3.141592653589793 / 5, nK1=33, nK2=(16, 16), nK3=(8, 8), mPinu=6, nG=96)
+
+
+def main_c0(args):
+    """The small SIAM scPA solve: one iteration launched eagerly vs replayed from a hipGraph, and the
+    fixed point to 1e-8 with the Anderson mixing of solve! vs Newton–Krylov."""
+    import numpy as np
+    import fdga
+    from fdga import newton
+    w = C0
+
+    def mk():
+        S = fdga.parquet_solver_siam_parquet_approximation(w["nG"], w["nK1"], w["nK2"], w["nK3"], e=0.0, Δ=w["Delta"],
+                                                           D=None, T=w["T"], U=w["U"], mΠν_factor=w["mPinu"])
+        S.init_sym_grp()
+        return S
+    out = {"workload": "c0: local SIAM scPA, U=1, beta=10, nK1=33, nK2=(16,16), nK3=(8,8), mPinu=6, nG=96, symmetry groups"}
+    S = mk()
+    for _ in range(args.warmup):
+        fdga.iterate_solver(S, "scPA")
+    n = max(args.steps, 50)
+    ms_eager = S.time_iterations("scPA", n) / n
+    S.set_graph(True)
+    fdga.iterate_solver(S, "scPA")   # capture
+    ms_graph = S.time_iterations("scPA", n) / n
+    out["iteration_ms"] = {"eager": ms_eager, "hipgraph": ms_graph}
+    log(f"bench c0: iteration {ms_eager * 1e3:.1f} us eager, {ms_graph * 1e3:.1f} us hipGraph")
+    A = mk()
+    t0 = time.perf_counter()
+    it_a, res_a = fdga.solve(A, strategy="scPA", tol=1e-8, maxiter=200)
+    t_a = time.perf_counter() - t0
+    N = mk()
+    t0 = time.perf_counter()
+    it_n, res_n, nev = newton.solve_newton(N, strategy="scPA", tol=1e-8)
+    t_n = time.perf_counter() - t0
+    d = float(np.max(np.abs(fdga.flatten(A) - fdga.flatten(N))) / np.max(np.abs(fdga.flatten(A))))
+    out["solve_to_1e-8"] = {"anderson": {"iterations": it_a, "parquet_iterations": it_a, "residual": res_a, "seconds": t_a},
+                            "newton_krylov": {"newton_iterations": it_n, "parquet_iterations": nev, "residual": res_n,
+                                              "seconds": t_n},
+                            "solution_relerr": d}
+    log(f"bench c0: Anderson {it_a} iterations ({t_a:.2f} s); Newton-Krylov {it_n} Newton steps, {nev} parquet "
+        f"iterations ({t_n:.2f} s); solutions differ by {d:.1e}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,9 +232,10 @@ def main():
     ap.add_argument("--no-dense", action="store_true", help="skip the identity-group FP64 timing")
     ap.add_argument("--tol32", type=float, default=1e-7,
                     help="Anderson tolerance of the FP32-MFMA solve (its residual floor is ~5e-8 at this workload)")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2"],
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"],
                     help="c1: the headline SIAM workload (one JSON line of the metric); c2: the NL Hubbard lattice "
-                         "iteration (timing and FP32 error, separate JSON line)")
+                         "iteration (timing and FP32 error, separate JSON line); c0: the small SIAM solve (hipGraph "
+                         "replay and Newton–Krylov vs mixing, separate JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the parquet hot path is run on a single GPU in this build "
@@ -197,6 +243,8 @@ def main():
         sys.exit(2)
     if args.workload == "c2":
         return main_c2(args)
+    if args.workload == "c0":
+        return main_c0(args)
     import fdga
     w = WORKLOAD
     names = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE", "gemm_L_K3", "gemm_K3"]
