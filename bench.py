@@ -78,6 +78,18 @@ def cpu_baseline(sym=True, reps=3):
                       f"median of {reps} = {dt:.2f} s"}
 
 
+def gemm_traffic(path):
+    """HBM bytes per launch of the BSE_K3! GEMM kernel (FETCH + WRITE, calibrated against a plain
+    copy) as measured by rocprofv3 --pmc and recorded in profiles/c1_bse_k3_gemm_traffic.txt; None
+    when the file has no entry for the precision path."""
+    import re
+    f = os.path.join(ROOT, "profiles", "c1_bse_k3_gemm_traffic.txt")
+    if not os.path.exists(f):
+        return None
+    m = re.search(rf"^{path}_hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(f).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def observables(S):
     """Σ, χ_sp = K1a/U², χ_ch = -(2 K1t - K1a)/U² (pSp storage of the connected K1) and the three
     channels' K3, read back from the device."""
@@ -230,6 +242,9 @@ def main():
                     help="identity symmetry groups (every element evaluated) instead of init_sym_grp!")
     ap.add_argument("--paths", default="fp64,fp32", help="precision paths to run (profiling runs select one)")
     ap.add_argument("--no-dense", action="store_true", help="skip the identity-group FP64 timing")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 512 MiB device-to-device copy first (calibration of the rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE counters against a copy of known size)")
     ap.add_argument("--tol32", type=float, default=1e-7,
                     help="Anderson tolerance of the FP32-MFMA solve (its residual floor is ~5e-8 at this workload)")
     ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"],
@@ -241,6 +256,13 @@ def main():
         print("bench.py measures one MI355X: the parquet hot path is run on a single GPU in this build "
               "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        a = torch.ones(64 * 2**20, dtype=torch.float64, device="cuda")   # 512 MiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if args.workload == "c2":
         return main_c2(args)
     if args.workload == "c0":
@@ -295,6 +317,7 @@ def main():
     P = paths[head]
     peak = FP32_MFMA_PEAK if head == "fp32" else FP64_MFMA_PEAK
     achieved = P["f_exec_launch"] / P["gemm_avg_s"]
+    traffic = gemm_traffic(head)
     out = {
         "metric": "parquet iters/s + BSE TFLOP/s, FP32-MFMA vs FP64 GPU @ rel.err<=1e-6 vs FP64",
         "value": 1000.0 / P["ms"],
@@ -320,7 +343,7 @@ def main():
                             "stage_ms": paths["fp64"]["st"], "anderson_iterations": paths["fp64"]["it"]},
                    "speedup_fp32_over_fp64": paths["fp64"]["ms"] / paths["fp32"]["ms"]},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA BSE_K3!/BSE_L_K3! contraction)"},
     }
     if not args.no_cpu_baseline:

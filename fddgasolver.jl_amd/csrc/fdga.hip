@@ -45,6 +45,10 @@ enum { SG_S, SG_K1, SG_PP2, SG_PH2, SG_PP3, SG_PH3, SG_PPL3, SG_PHL3, SG_0PP2, S
 struct fdga_ctx {
     fdga_dims d{};
     hipStream_t st = nullptr;
+    // the stream the stage kernels are launched on: st, or st2 for the second chain of an iteration
+    hipStream_t run = nullptr;
+    hipStream_t st2 = nullptr;
+    hipEvent_t evfork = nullptr, evjoin = nullptr;
     std::string err;
     int NP = 1, NkG = 1, NG = 0;
     int NW1 = 0, NW2 = 0, NV2 = 0, NW3 = 0, NV3 = 0, NWP = 0, NVP = 0, nVP = 0;
@@ -153,7 +157,7 @@ std::vector<int> xcd_order(const std::vector<int>& items, Key key) {
 int dalloc(fdga_ctx* c, double2** p, size_t n) {
     if (n == 0) n = 1;
     HIPCHK(c, hipMalloc((void**)p, n * sizeof(double2)));
-    k_zero<<<cdiv(n, 256), 256, 0, c->st>>>(*p, n);
+    k_zero<<<cdiv(n, 256), 256, 0, c->run>>>(*p, n);
     LAUNCHCHK(c);
     return 0;
 }
@@ -240,9 +244,9 @@ void free_chan(DevChan& dc) {
 int refresh_avg(fdga_ctx* c) {
     if (c->NP == 1) return 0;
     for (int r = 0; r < 3; ++r) {
-        k_pavg<<<cdiv(c->n1, 256), 256, 0, c->st>>>(c->F[r].A1, c->F[r].K1, c->n1, c->NP);
-        k_pavg<<<cdiv(c->n2, 256), 256, 0, c->st>>>(c->F[r].A2, c->F[r].K2, c->n2, c->NP);
-        k_pavg<<<cdiv(c->n3, 256), 256, 0, c->st>>>(c->F[r].A3, c->F[r].K3, c->n3, c->NP);
+        k_pavg<<<cdiv(c->n1, 256), 256, 0, c->run>>>(c->F[r].A1, c->F[r].K1, c->n1, c->NP);
+        k_pavg<<<cdiv(c->n2, 256), 256, 0, c->run>>>(c->F[r].A2, c->F[r].K2, c->n2, c->NP);
+        k_pavg<<<cdiv(c->n3, 256), 256, 0, c->run>>>(c->F[r].A3, c->F[r].K3, c->n3, c->NP);
     }
     LAUNCHCHK(c);
     return 0;
@@ -261,7 +265,7 @@ int ensure_stage(fdga_ctx* c, size_t n) {
 int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale) {
     if (L == 1) {
         if (scale != 1.0) {
-            k_axpby<<<cdiv(nrows, 256), 256, 0, c->st>>>(data, data, scale, 0.0, nrows);
+            k_axpby<<<cdiv(nrows, 256), 256, 0, c->run>>>(data, data, scale, 0.0, nrows);
             LAUNCHCHK(c);
         }
         return 0;
@@ -269,7 +273,7 @@ int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale
     const int LL = L * L;
     int RB = std::max(1, std::min(16, 1024 / LL));
     size_t smem = (size_t)(2 * RB * LL + L) * sizeof(double2);
-    k_dft2_rows<<<cdiv(nrows, RB), 256, smem, c->st>>>(data, nrows, L, RB, sign, scale);
+    k_dft2_rows<<<cdiv(nrows, RB), 256, smem, c->run>>>(data, nrows, L, RB, sign, scale);
     LAUNCHCHK(c);
     return 0;
 }
@@ -277,7 +281,7 @@ int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale
 // tail sums of one array on the bubble mesh (rows of 2 nVP entries)
 int tail_sums(fdga_ctx* c, const double2* X, double2* T) {
     const size_t nrows = (size_t)c->NWP * c->NP;
-    k_tail_sums<<<dim3((unsigned)nrows, 2), 256, 0, c->st>>>(X, T, c->nVP, nrows);
+    k_tail_sums<<<dim3((unsigned)nrows, 2), 256, 0, c->run>>>(X, T, c->nVP, nrows);
     LAUNCHCHK(c);
     return 0;
 }
@@ -293,15 +297,15 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     const fdga_dims& d = c->d;
     if (!d.nonlocal) {
         dim3 g(cdiv(c->NVP, 256), c->NWP);
-        k_bubble_local<<<g, 256, 0, c->st>>>(pp, ph, Gsrc, d.nG, d.nK1, c->nVP, d.T);
+        k_bubble_local<<<g, 256, 0, c->run>>>(pp, ph, Gsrc, d.nG, d.nK1, c->nVP, d.T);
         LAUNCHCHK(c);
         return tail_sums(c, pp, pi_tail(c, pp)) || tail_sums(c, ph, pi_tail(c, ph));
     }
     // G(R) = fft(G)/LG^2 (src/nonlocal/bubble.jl:110)
-    HIPCHK(c, hipMemcpyAsync(c->tmpG1, Gsrc, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->tmpG1, Gsrc, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
     if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG))) return -3;
     dim3 g(cdiv(c->NVP, 256), c->NWP, c->NP);
-    k_bubble_nl_R<<<g, 256, 0, c->st>>>(pp, ph, c->tmpG1, d.nG, d.LG, d.L, d.nK1, c->nVP, d.T);
+    k_bubble_nl_R<<<g, 256, 0, c->run>>>(pp, ph, c->tmpG1, d.nG, d.LG, d.L, d.nK1, c->nVP, d.T);
     LAUNCHCHK(c);
     size_t nrow = (size_t)c->NVP * c->NWP;
     if (dft2(c, pp, nrow, d.L, +1, 1.0)) return -3;
@@ -356,24 +360,24 @@ int sde_sigma(fdga_ctx* c, double2* S, const double2* G, double scale) {
     const fdga_dims& d = c->d;
     if (!d.nonlocal) {
         const SymCls ss = symcls(c, SG_S);
-        k_sde_sigma_local<<<cdiv(nclass(ss, c->NG) * WAVE, 256), 256, 0, c->st>>>(S, G, c->Lpp, c->Lph, d.nG, d.nK2b, d.nK2f,
+        k_sde_sigma_local<<<cdiv(nclass(ss, c->NG) * WAVE, 256), 256, 0, c->run>>>(S, G, c->Lpp, c->Lph, d.nG, d.nK2b, d.nK2f,
                                                                                   d.T, scale, ss);
         LAUNCHCHK(c);
         return 0;
     }
     // real-space evaluation (src/nonlocal/SDE.jl:209-280)
-    HIPCHK(c, hipMemcpyAsync(c->tmpG1, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->tmpG1, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
     if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG))) return -3;
-    HIPCHK(c, hipMemcpyAsync(c->tmpL1, c->Lpp, c->nL * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->tmpL2, c->Lph, c->nL * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->tmpL1, c->Lpp, c->nL * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
+    HIPCHK(c, hipMemcpyAsync(c->tmpL2, c->Lph, c->nL * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
     size_t nrow = (size_t)c->NW2 * c->NV2;
     if (dft2(c, c->tmpL1, nrow, d.L, -1, 1.0 / ((double)d.L * d.L))) return -3;
     if (dft2(c, c->tmpL2, nrow, d.L, -1, 1.0 / ((double)d.L * d.L))) return -3;
     dim3 g(cdiv(c->NG, 128), c->NkG);
-    k_sde_sigma_nl_R<<<g, 128, 0, c->st>>>(c->tmpG2, c->tmpG1, c->tmpL1, c->tmpL2, d.nG, d.nK2b, d.nK2f, d.LG, d.L, d.T);
+    k_sde_sigma_nl_R<<<g, 128, 0, c->run>>>(c->tmpG2, c->tmpG1, c->tmpL1, c->tmpL2, d.nG, d.nK2b, d.nK2f, d.LG, d.L, d.T);
     LAUNCHCHK(c);
     if (dft2(c, c->tmpG2, c->NG, d.LG, +1, 1.0)) return -3;
-    k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(S, c->tmpG2, scale, 1.0, c->nGel);
+    k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(S, c->tmpG2, scale, 1.0, c->nGel);
     LAUNCHCHK(c);
     return 0;
 }
@@ -384,30 +388,30 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
     const double2 fac = cscale(cmul(c->U, c->U), d.T * d.T);
     const int NB = 4 * d.nG - 1;
     if (!d.nonlocal) {
-        k_u2_C<<<cdiv((size_t)NB * WAVE, 256), 256, 0, c->st>>>(c->u2C, G, G, d.nG, 1);
+        k_u2_C<<<cdiv((size_t)NB * WAVE, 256), 256, 0, c->run>>>(c->u2C, G, G, d.nG, 1);
         const SymCls ss = symcls(c, SG_S);
-        k_u2_S<<<cdiv(nclass(ss, c->NG) * WAVE, 256), 256, 0, c->st>>>(S, c->u2C, G, d.nG, 1, fac, 1, ss);
+        k_u2_S<<<cdiv(nclass(ss, c->NG) * WAVE, 256), 256, 0, c->run>>>(S, c->u2C, G, d.nG, 1, fac, 1, ss);
         LAUNCHCHK(c);
         return 0;
     }
     const int LG = d.LG;
-    HIPCHK(c, hipMemcpyAsync(c->tmpG1, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->tmpG2, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->tmpG1, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
+    HIPCHK(c, hipMemcpyAsync(c->tmpG2, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
     if (dft2(c, c->tmpG1, c->NG, LG, -1, 1.0 / ((double)LG * LG))) return -3;   // G(+R)
     if (dft2(c, c->tmpG2, c->NG, LG, +1, 1.0 / ((double)LG * LG))) return -3;   // G(-R)
-    k_u2_C<<<cdiv((size_t)NB * c->NkG * WAVE, 256), 256, 0, c->st>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
-    k_u2_S<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->st>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac, 0,
+    k_u2_C<<<cdiv((size_t)NB * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
+    k_u2_S<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac, 0,
                                                                          SymCls{nullptr, nullptr, 0});
     LAUNCHCHK(c);
     if (dft2(c, c->tmpG3, c->NG, LG, +1, 1.0)) return -3;
-    k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(S, c->tmpG3, 1.0, 1.0, c->nGel);
+    k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(S, c->tmpG3, 1.0, 1.0, c->nGel);
     LAUNCHCHK(c);
     return 0;
 }
 
 int sde_hartree(fdga_ctx* c, double2* S, const double2* G, double sign) {
-    k_occupation<<<1, 256, 0, c->st>>>(c->scal, G, c->nGel, c->d.T, (double)c->NkG);
-    k_hartree<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(S, c->nGel, c->scal, c->U, sign);
+    k_occupation<<<1, 256, 0, c->run>>>(c->scal, G, c->nGel, c->d.T, (double)c->NkG);
+    k_hartree<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(S, c->nGel, c->scal, c->U, sign);
     LAUNCHCHK(c);
     return 0;
 }
@@ -424,15 +428,15 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
     const double2* Tpp = pi_tail(c, Pp);
     const double2* Tph = pi_tail(c, Ph);
     if (!spp.ncls && !sph.ncls) {
-        k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
+        k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
                                                               lv.kind,
                                                               d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, lv.chanNP, TU, NK, spp);
     } else {
-        k_sde_L<1><<<cdiv(nclass(spp, nout) * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
+        k_sde_L<1><<<cdiv(nclass(spp, nout) * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
                                                                            lv.g[2], ref,
                                                                            lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP,
                                                                            lv.chanNP, TU, NK, spp);
-        k_sde_L<2><<<cdiv(nclass(sph, nout) * WAVE, 256), 256, 0, c->st>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
+        k_sde_L<2><<<cdiv(nclass(sph, nout) * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
                                                                            lv.g[2], ref,
                                                                            lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP,
                                                                            lv.chanNP, TU, NK, sph);
@@ -495,7 +499,7 @@ template <int CH>
 static int launch_lk2(fdga_ctx* c) {
     const SymCls sym = sym_k2(c, CH);
     const size_t nw = nclass(sym, c->n2 * c->NP);
-    k_gemv_lk2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->FL[CH].K2, c->FL[ACH].K2, pi_of(c, CH, true), vtx_F(c),
+    k_gemv_lk2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->FL[CH].K2, c->FL[ACH].K2, pi_of(c, CH, true), vtx_F(c),
                                                             c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP, c->d.nonlocal,
                                                             c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
@@ -506,7 +510,7 @@ static int launch_lk2(fdga_ctx* c) {
 template <int CH>
 static int launch_w(fdga_ctx* c) {
     dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
-    k_bse_w<CH><<<gw, 256, 0, c->st>>>(c->Wc[CH], pi_of(c, CH, false), pi_of(c, CH, true), vtx_FL(c), f0_view(c), c->d.nK1,
+    k_bse_w<CH><<<gw, 256, 0, c->run>>>(c->Wc[CH], pi_of(c, CH, false), pi_of(c, CH, true), vtx_FL(c), f0_view(c), c->d.nK1,
                                        c->nVP);
     LAUNCHCHK(c);
     return tail_sums(c, c->Wc[CH], c->Wtail[CH]);
@@ -517,7 +521,7 @@ static int launch_k1(fdga_ctx* c) {
     if (!c->w_fresh && launch_w<CH>(c)) return -3;
     const SymCls sym = symcls(c, SG_K1);
     const size_t nw = nclass(sym, c->n1 * c->NP);
-    k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->st>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], c->Wtail[CH], vtx_F(c), c->d.nK1,
+    k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->run>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], c->Wtail[CH], vtx_F(c), c->d.nK1,
                                                            c->d.nK1, c->nVP, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
@@ -528,7 +532,7 @@ static int launch_k2(fdga_ctx* c) {
     if (!c->w_fresh && launch_w<CH>(c)) return -3;
     const SymCls sym = sym_k2(c, CH);
     const size_t nw = nclass(sym, c->n2 * c->NP);
-    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->Wc[CH], c->Wtail[CH], vtx_F(c),
+    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->Wc[CH], c->Wtail[CH], vtx_F(c),
                                                            chanF(c, c->FL[CH]),
                                                            chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
                                                            c->NP, c->d.T * kSign[CH], box_bound(c), sym);
@@ -541,7 +545,7 @@ static int launch_k2_at(fdga_ctx* c) {
     if (!c->w_fresh && (launch_w<ACH>(c) || launch_w<TCH>(c))) return -3;
     const SymCls sym = symcls(c, SG_PH2);   // a and t share SGph[2]
     const size_t nw = nclass(sym, c->n2 * c->NP);
-    k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->st>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->Wc[ACH], c->Wc[TCH],
+    k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->Wc[ACH], c->Wc[TCH],
                                                           c->Wtail[ACH], c->Wtail[TCH], vtx_F(c),
                                                           chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]), c->d.nK2b, c->d.nK2f,
                                                           c->d.nK1, c->nVP, c->NP, c->d.T, box_bound(c), sym);
@@ -576,7 +580,7 @@ static int k3_finish(fdga_ctx* c, DevChan* out, const int* chans, int n, bool le
     if (!sym) {
         if (!t_in_epilogue)
             for (int s = 0; s < n; ++s)
-                if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(out[TCH].K3, out[ACH].K3, c->n3 * c->NP);
+                if (chans[s] == TCH) k_post_t<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(out[TCH].K3, out[ACH].K3, c->n3 * c->NP);
         LAUNCHCHK(c);
         return 0;
     }
@@ -598,7 +602,7 @@ static int k3_finish(fdga_ctx* c, DevChan* out, const int* chans, int n, bool le
     f.raw0 = r[0]; f.raw1 = r[1]; f.raw2 = r[2];
     f.rep0 = rp[0]; f.rep1 = rp[1]; f.rep2 = rp[2];
     f.afinal = out[ACH].K3;
-    k_k3_finish<<<(unsigned)c->finmap_n, 256, 0, c->st>>>(f, c->finmap, c->fin_nchunk, c->NV3 * c->NV3);
+    k_k3_finish<<<(unsigned)c->finmap_n, 256, 0, c->run>>>(f, c->finmap, c->fin_nchunk, c->NV3 * c->NV3);
     LAUNCHCHK(c);
     return 0;
 }
@@ -636,7 +640,7 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     size_t nl = 0;
     const int* bl = k3_blocks(c, chans, n, true, nl);
     kernel_begin(c);
-    if (zgemm_launch(c->d.precision, n * NW * c->NP, c->st, NV, NV, NV, tm, tn, bl, nl, a, b, e) != hipSuccess)
+    if (zgemm_launch(c->d.precision, n * NW * c->NP, c->run, NV, NV, NV, tm, tn, bl, nl, a, b, e) != hipSuccess)
         return fail(c, "BSE_L_K3 GEMM launch failed", -3);
     kernel_end(c, 9);
     return k3_finish(c, c->FL, chans, n, true, sym, t_epi);
@@ -680,7 +684,7 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
         kc[s].tpost = (ch == TCH && t_epi);
     }
     const size_t ncol = (size_t)NV * NW * c->NP * n;
-    k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->st>>>(g, rc[0], rc[1], rc[2], n);
+    k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->run>>>(g, rc[0], rc[1], rc[2], n);
     LAUNCHCHK(c);
     K3_Bop b{g, c->d.T, kc[0], kc[1], kc[2]};
     K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
@@ -688,9 +692,9 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     const int* bl = k3_blocks(c, chans, n, false, nl);
     kernel_begin(c);
     hipError_t ge;
-    if (c->NP == 1) ge = zgemm_launch(c->d.precision, n * NW, c->st, NV, NV, NV, tm, tn, bl, nl,
+    if (c->NP == 1) ge = zgemm_launch(c->d.precision, n * NW, c->run, NV, NV, NV, tm, tn, bl, nl,
                                       K3_AopT<true>{g, c->d.T, kc[0], kc[1], kc[2]}, b, e);
-    else ge = zgemm_launch(c->d.precision, n * NW * c->NP, c->st, NV, NV, NV, tm, tn, bl, nl,
+    else ge = zgemm_launch(c->d.precision, n * NW * c->NP, c->run, NV, NV, NV, tm, tn, bl, nl,
                            K3_AopT<false>{g, c->d.T, kc[0], kc[1], kc[2]}, b, e);
     if (ge != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
@@ -734,11 +738,14 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     if (d.tab_nb < 1) d.tab_nb = 1;
     if (d.tab_nf < 1) d.tab_nf = 1;
     (void)hipGetLastError();
-    if (hipStreamCreate(&c->st) != hipSuccess) {
+    if (hipStreamCreate(&c->st) != hipSuccess || hipStreamCreate(&c->st2) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evjoin, hipEventDisableTiming) != hipSuccess) {
         fail(c, "hipStreamCreate failed (no HIP device?)", -2);
         *out = c;
         return -2;
     }
+    c->run = c->st;
     c->NP = d.L * d.L;
     c->NkG = d.LG * d.LG;
     c->NG = 2 * d.nG;
@@ -823,6 +830,10 @@ int fdga_destroy(fdga_ctx* c) {
     free_blists(c);
     if (c->k3raw) (void)hipFree(c->k3raw);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
+    if (c->evfork) (void)hipEventDestroy(c->evfork);
+    if (c->evjoin) (void)hipEventDestroy(c->evjoin);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return 0;
@@ -905,8 +916,8 @@ int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
     } else {
         if (ensure_stage(c, a.n)) return -2;
         HIPCHK(c, hipMemcpyAsync(c->stage, host, bytes, hipMemcpyHostToDevice, c->st));
-        if (a.kind == 1) k_k3_to_dev<<<cdiv(a.n, 256), 256, 0, c->st>>>(a.p, c->stage, a.NW, a.NV, a.NP);
-        else k_pi_to_dev<<<cdiv(a.n, 256), 256, 0, c->st>>>(a.p, c->stage, a.NW, a.NV, a.NP);
+        if (a.kind == 1) k_k3_to_dev<<<cdiv(a.n, 256), 256, 0, c->run>>>(a.p, c->stage, a.NW, a.NV, a.NP);
+        else k_pi_to_dev<<<cdiv(a.n, 256), 256, 0, c->run>>>(a.p, c->stage, a.NW, a.NV, a.NP);
         LAUNCHCHK(c);
         if (a.kind == 2 && tail_sums(c, a.p, pi_tail(c, a.p))) return -3;
     }
@@ -929,12 +940,12 @@ int fdga_get(fdga_ctx* c, int32_t which, double* host) {
         HIPCHK(c, hipMemcpyAsync(host, a.p, bytes, hipMemcpyDeviceToHost, c->st));
     } else {
         if (ensure_stage(c, a.n)) return -2;
-        if (a.kind == 1) k_k3_to_ref<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, a.NW, a.NV, a.NP);
-        else if (a.kind == 2) k_pi_to_ref<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, a.NW, a.NV, a.NP);
-        else if (a.kind == 3) k_cache_to_ref<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, a.NW, a.NV, a.NP);
+        if (a.kind == 1) k_k3_to_ref<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, a.NW, a.NV, a.NP);
+        else if (a.kind == 2) k_pi_to_ref<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, a.NW, a.NV, a.NP);
+        else if (a.kind == 3) k_cache_to_ref<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, a.NW, a.NV, a.NP);
         else {
             int ch = a.ch == 0 ? PCH : a.ch == 1 ? ACH : TCH;
-            k_cache_F_julia<<<cdiv(a.n, 256), 256, 0, c->st>>>(c->stage, a.p, chanF(c, c->F[ch]), chanF(c, c->F[ACH]),
+            k_cache_F_julia<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, chanF(c, c->F[ch]), chanF(c, c->F[ACH]),
                                                                ch, c->d.nK3b, c->d.nK3f, c->NP, c->NP == 1);
         }
         LAUNCHCHK(c);
@@ -948,7 +959,7 @@ int fdga_init(fdga_ctx* c) {
     if (!c) return -1;
     if (bubbles_into(c, c->Pi0pp, c->Pi0ph, c->G0)) return -3;
     HIPCHK(c, hipMemcpyAsync(c->Sig, c->Sig0, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->G, c->Sig, c->Gbare, c->nGel);
+    k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->G, c->Sig, c->Gbare, c->nGel);
     LAUNCHCHK(c);
     if (bubbles_into(c, c->Pipp, c->Piph, c->G)) return -3;
     if (refresh_avg(c)) return -3;
@@ -1083,7 +1094,7 @@ int fdga_reset_sym_grp(fdga_ctx* c) {
 }
 
 int fdga_dyson(fdga_ctx* c) {
-    k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->G, c->Sig, c->Gbare, c->nGel);
+    k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->G, c->Sig, c->Gbare, c->nGel);
     LAUNCHCHK(c);
     return 0;
 }
@@ -1094,8 +1105,8 @@ int fdga_build_k3_cache(fdga_ctx* c) {
     CachePtrs cp;
     for (int k = 0; k < 10; ++k) cp.c[k] = c->cache[k];
     dim3 g(cdiv(c->NV3, 64), c->NV3, c->NW3);
-    k_cache1<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
-    k_cache2<<<g, 64, 0, c->st>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f, c->NP == 1);
+    k_cache1<<<g, 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
+    k_cache2<<<g, 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f, c->NP == 1);
     LAUNCHCHK(c);
     return 0;
 }
@@ -1172,16 +1183,16 @@ int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
     // SDE(F0, G0, Π0) of fdPA does not change between iterations: computed once (src/SDE.jl:19-20),
     // before the target levels so that Lpp/Lph end up holding the target's (S.Lpp, S.Lph)
     if (strategy == FDGA_FDPA && (!c->sde0_valid || c->sde0_h != include_hartree)) {
-        k_zero<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->sde0, c->nGel);
+        k_zero<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->sde0, c->nGel);
         if (sde_levels(c, c->sde0, c->G0, c->Pi0pp, c->Pi0ph, 1, include_hartree)) return -3;
         c->sde0_valid = 1;
         c->sde0_h = include_hartree;
     }
-    k_zero<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->nGel);
+    k_zero<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->nGel);
     if (sde_levels(c, c->Sig, c->G, c->Pipp, c->Piph, 0, include_hartree)) return -3;
     if (strategy == FDGA_FDPA) {
-        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->sde0, -1.0, 1.0, c->nGel);
-        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->st>>>(c->Sig, c->Sig0, 1.0, 1.0, c->nGel);
+        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->sde0, -1.0, 1.0, c->nGel);
+        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->Sig0, 1.0, 1.0, c->nGel);
         LAUNCHCHK(c);
         if (include_hartree && c->d.fdpa_h0 && sde_hartree(c, c->Sig, c->G0, -1.0)) return -3;
     }
@@ -1189,6 +1200,39 @@ int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
 }
 
 static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma);
+
+// One iteration with its two independent chains on two streams.  After Dyson and the bubbles,
+// chain A (caches -> BSE_L_K3! -> BSE_K3!) runs on st and chain B (BSE_L_K2! -> weights ->
+// BSE_K1! -> BSE_K2!) on st2; they write disjoint arrays (FL.K3, Fbuff.K3 vs FL.K2, Fbuff.K1/K2)
+// and B reads FL only at ν' = ∞ (K1 and K2 classes).  The join precedes set!(S.F, S.Fbuff) and SDE!.
+static int iterate_two_chains(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
+    if (update_sigma && (fdga_dyson(c) || fdga_bubbles(c))) return -3;
+    HIPCHK(c, hipEventRecord(c->evfork, c->st));
+    HIPCHK(c, hipStreamWaitEvent(c->st2, c->evfork, 0));
+    const int all[3] = {PCH, ACH, TCH};
+    int rc = 0;
+    // chain B on st2
+    c->run = c->st2;
+    if (strategy == FDGA_FDPA)
+        for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_l_k2(c, ch);
+    rc = rc ? rc : (launch_w<PCH>(c) || launch_w<ACH>(c) || launch_w<TCH>(c));
+    c->w_fresh = 1;
+    for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
+    rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));
+    c->w_fresh = 0;
+    c->run = c->st;
+    if (rc) return -3;
+    // chain A on st
+    if (fdga_build_k3_cache(c)) return -3;
+    if (strategy == FDGA_FDPA && launch_lk3(c, all, 3)) return -3;
+    if (launch_k3(c, all, 3)) return -3;
+    // join
+    HIPCHK(c, hipEventRecord(c->evjoin, c->st2));
+    HIPCHK(c, hipStreamWaitEvent(c->st, c->evjoin, 0));
+    if (fdga_update_F(c)) return -3;
+    if (update_sigma && fdga_sde(c, strategy, 1)) return -3;
+    return 0;
+}
 
 // Captures fdga_iterate(strategy, 1) for both F/Fbuff pointer states.  Capture records the launches
 // without running them, so the host-side buffer swap done while recording is undone after each.
@@ -1230,6 +1274,7 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
 }
 
 static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
+    if (!c->timing) return iterate_two_chains(c, strategy, update_sigma);
     stage_begin(c);
     if (update_sigma) {
         if (fdga_dyson(c) || fdga_bubbles(c)) return -3;
@@ -1288,7 +1333,7 @@ int fdga_unflatten(fdga_ctx* c, const double* x) {
         off += c->n1 * c->NP;
         HIPCHK(c, hipMemcpyAsync(c->F[r].K2, c->stage + off, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
         off += c->n2 * c->NP;
-        k_k3_to_dev<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->F[r].K3, c->stage + off, c->NW3, c->NV3, c->NP);
+        k_k3_to_dev<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(c->F[r].K3, c->stage + off, c->NW3, c->NV3, c->NP);
         LAUNCHCHK(c);
         off += c->n3 * c->NP;
     }
@@ -1306,7 +1351,7 @@ int fdga_flatten(fdga_ctx* c, double* x) {
         off += c->n1 * c->NP;
         HIPCHK(c, hipMemcpyAsync(c->stage + off, c->F[r].K2, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
         off += c->n2 * c->NP;
-        k_k3_to_ref<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->st>>>(c->stage + off, c->F[r].K3, c->NW3, c->NV3, c->NP);
+        k_k3_to_ref<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(c->stage + off, c->F[r].K3, c->NW3, c->NV3, c->NP);
         LAUNCHCHK(c);
         off += c->n3 * c->NP;
     }
