@@ -347,6 +347,9 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
     }
     if (bid < 0) return;
     const int batch = bid / tiles;
+    const auto A = al.bind(batch);
+    const auto B = bl.bind(batch);
+    const auto E = ep.bind(batch);
     const int t = bid - batch * tiles;
     const int m0 = (t % tiles_m) * GBM, n0 = (t / tiles_m) * GBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -368,13 +371,14 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
         for (int q = 0; q < QA; ++q) {
             const int e = tid + 256 * q;
             const int gm = m0 + e % GBM, gk = k0 + e / GBM;
-            pa[q] = (gm < M && gk < K) ? al(batch, gm, gk) : czero();
+            pa[q] = (gm < M && gk < K) ? A(gm, gk) : czero();
         }
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
             const int e = tid + 256 * q;
-            const int gn = n0 + e % GBN, gk = k0 + e / GBN;
-            pb[q] = (gn < N && gk < K) ? bl(batch, gk, gn) : czero();
+            // B operands are stored with k (ω) fastest: consecutive threads take consecutive k
+            const int gn = n0 + e / GBK, gk = k0 + e % GBK;
+            pb[q] = (gn < N && gk < K) ? B(gk, gn) : czero();
         }
     };
     constexpr bool PF = Prefetch<AL>::on && Prefetch<BL>::on;
@@ -390,8 +394,8 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
 #pragma unroll
             for (int q = 0; q < QB; ++q) {
                 const int e = tid + 256 * q;
-                Bs[0][e / GBN][e % GBN] = (S)pb[q].x;
-                Bs[1][e / GBN][e % GBN] = (S)pb[q].y;
+                Bs[0][e % GBK][e / GBK] = (S)pb[q].x;
+                Bs[1][e % GBK][e / GBK] = (S)pb[q].y;
             }
         } else {
             // element by element: load and store to LDS without holding the tile in registers
@@ -399,17 +403,17 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
             for (int q = 0; q < QA; ++q) {
                 const int e = tid + 256 * q;
                 const int gm = m0 + e % GBM, gk = k0 + e / GBM;
-                const double2 v = (gm < M && gk < K) ? al(batch, gm, gk) : czero();
+                const double2 v = (gm < M && gk < K) ? A(gm, gk) : czero();
                 As[0][e / GBM][e % GBM] = (S)v.x;
                 As[1][e / GBM][e % GBM] = (S)v.y;
             }
 #pragma unroll
             for (int q = 0; q < QB; ++q) {
                 const int e = tid + 256 * q;
-                const int gn = n0 + e % GBN, gk = k0 + e / GBN;
-                const double2 v = (gn < N && gk < K) ? bl(batch, gk, gn) : czero();
-                Bs[0][e / GBN][e % GBN] = (S)v.x;
-                Bs[1][e / GBN][e % GBN] = (S)v.y;
+                const int gn = n0 + e / GBK, gk = k0 + e % GBK;
+                const double2 v = (gn < N && gk < K) ? B(gk, gn) : czero();
+                Bs[0][e % GBK][e / GBK] = (S)v.x;
+                Bs[1][e % GBK][e / GBK] = (S)v.y;
             }
         }
         __syncthreads();
@@ -457,7 +461,7 @@ __global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int ti
             for (int r = 0; r < 4; ++r) {
                 const int row = m0 + wm + 16 * i + (lane & 15);
                 const int col = n0 + wn + 16 * j + MT::row(lane, r);
-                if (row < M && col < N) ep(batch, row, col, cmk((double)accr[i][j][r], (double)acci[i][j][r]));
+                if (row < M && col < N) E(row, col, cmk((double)accr[i][j][r], (double)acci[i][j][r]));
             }
 }
 
@@ -499,45 +503,61 @@ struct LK3Chan {
     double scale;          // T * s
     int tpost;             // t channel post-processing inside the epilogue
 };
+// Loaders and epilogues resolve their batch (channel slot, Ω, P) once per workgroup with bind(b),
+// which returns base pointers for that batch; the per-element calls do only the in-matrix offset.
+// The slot is selected by branches, never by indexing the kernel-argument descriptors.
 struct LK3_Aop {
     K3Geom g; LK3Chan c0, c1, c2;
-    __device__ __forceinline__ double2 op(const LK3Chan& c, int bb, int i, int k) const {
-        const int NV = g.NV(), NW = g.NW();
-        const int iW = bb % NW;
-        return c.G[(size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
+    struct Bound {
+        const double2* G; int NV;
+        __device__ __forceinline__ double2 operator()(int i, int k) const { return G[(size_t)i + (size_t)NV * k]; }
+    };
+    __device__ __forceinline__ Bound one(const LK3Chan& c, int bb) const {
+        const int NV = g.NV();
+        return Bound{c.G + (size_t)NV * NV * (bb % g.NW()), NV};
     }
-    __device__ __forceinline__ double2 operator()(int b, int i, int k) const {
+    __device__ __forceinline__ Bound bind(int b) const {
         const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
-        return s == 0 ? op(c0, bb, i, k) : s == 1 ? op(c1, bb, i, k) : op(c2, bb, i, k);
+        return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
 struct LK3_Bop {
     K3Geom g; LK3Chan c0, c1, c2;
-    __device__ __forceinline__ double2 op(const LK3Chan& c, int bb, int k, int j) const {
+    struct Bound {
+        const double2* pi0; const double2* F0r; int NV;   // pi0[k] = Π0(Ω, ω = k - n3f, P)
+        __device__ __forceinline__ double2 operator()(int k, int j) const {
+            return cmul(pi0[k], F0r[(size_t)k + (size_t)NV * j]);
+        }
+    };
+    __device__ __forceinline__ Bound one(const LK3Chan& c, int bb) const {
         const int NV = g.NV(), NW = g.NW();
-        const int iW = bb % NW, p = bb / NW;
-        const int m = iW - (g.n3b - 1), w = k - g.n3f;
-        return cmul(c.Pi0[g.pidx(m, w, p)], c.F0r[(size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW)]);
+        const int iW = bb % NW, p = bb / NW, m = iW - (g.n3b - 1);
+        return Bound{c.Pi0 + g.pidx(m, -g.n3f, p), c.F0r + (size_t)NV * NV * iW, NV};
     }
-    __device__ __forceinline__ double2 operator()(int b, int k, int j) const {
+    __device__ __forceinline__ Bound bind(int b) const {
         const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
-        return s == 0 ? op(c0, bb, k, j) : s == 1 ? op(c1, bb, k, j) : op(c2, bb, k, j);
+        return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
 struct LK3_Ep {
     K3Geom g; LK3Chan c0, c1, c2;
-    __device__ __forceinline__ void op(const LK3Chan& c, int bb, int i, int j, double2 v) const {
+    struct Bound {
+        double2* out; const double2* outa; double scale; int tpost; int NV;
+        __device__ __forceinline__ void operator()(int i, int j, double2 v) const {
+            const size_t o = (size_t)i + (size_t)NV * j;
+            double2 x = cscale(v, scale);
+            if (tpost) x = cscale(cadd(x, outa[o]), 0.5);
+            out[o] = x;
+        }
+    };
+    __device__ __forceinline__ Bound one(const LK3Chan& c, int bb) const {
         const int NV = g.NV();
-        size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * bb);
-        double2 x = cscale(v, c.scale);
-        if (c.tpost) x = cscale(cadd(x, c.outa[o]), 0.5);
-        c.out[o] = x;
+        const size_t off = (size_t)NV * NV * bb;
+        return Bound{c.out + off, c.tpost ? c.outa + off : nullptr, c.scale, c.tpost, NV};
     }
-    __device__ __forceinline__ void operator()(int b, int i, int j, double2 v) const {
+    __device__ __forceinline__ Bound bind(int b) const {
         const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
-        if (s == 0) op(c0, bb, i, j, v);
-        else if (s == 1) op(c1, bb, i, j, v);
-        else op(c2, bb, i, j, v);
+        return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
 
@@ -565,57 +585,87 @@ struct K3Chan {
 template <bool FULL>
 struct K3_AopT {
     K3Geom g; double T; K3Chan c0, c1, c2;
-    __device__ __forceinline__ double2 op(const K3Chan& c, int bb, int i, int k) const {
-        const int NV = g.NV(), NW = g.NW();
-        const int iW = bb % NW;
-        const size_t cij = (size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * iW);
-        const double2 x = c.Fbase[cij];
-        if (FULL) return x;
-        const size_t vij = (size_t)i + (size_t)NV * ((size_t)k + (size_t)NV * bb);
-        if (c.ch == TCH) return cadd(x, csub(cscale(c.K3r[vij], 2.0), c.K3a[vij]));
-        return cadd(x, c.K3r[vij]);
+    struct Bound {
+        const double2* Fb; const double2* K3r; const double2* K3a; int t; int NV;
+        __device__ __forceinline__ double2 operator()(int i, int k) const {
+            const size_t o = (size_t)i + (size_t)NV * k;
+            const double2 x = Fb[o];
+            if (FULL) return x;
+            if (t) return cadd(x, csub(cscale(K3r[o], 2.0), K3a[o]));
+            return cadd(x, K3r[o]);
+        }
+    };
+    __device__ __forceinline__ Bound one(const K3Chan& c, int bb) const {
+        const int NV = g.NV();
+        const size_t nn = (size_t)NV * NV;
+        const int t = (c.ch == TCH);
+        return Bound{c.Fbase + nn * (bb % g.NW()), FULL ? nullptr : c.K3r + nn * bb,
+                     (FULL || !t) ? nullptr : c.K3a + nn * bb, t, NV};
     }
-    __device__ __forceinline__ double2 operator()(int b, int i, int k) const {
+    __device__ __forceinline__ Bound bind(int b) const {
         const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
-        return s == 0 ? op(c0, bb, i, k) : s == 1 ? op(c1, bb, i, k) : op(c2, bb, i, k);
+        return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
 template <> struct Prefetch<K3_AopT<false>> { static constexpr bool on = false; };
 struct K3_Bop {
     K3Geom g; double T; K3Chan c0, c1, c2;
-    __device__ __forceinline__ double2 op(const K3Chan& c, int bb, int k, int j) const {
+    struct Bound {
+        const double2* R; int NV;
+        __device__ __forceinline__ double2 operator()(int k, int j) const { return R[(size_t)k + (size_t)NV * j]; }
+    };
+    __device__ __forceinline__ Bound one(const K3Chan& c, int bb) const {
         const int NV = g.NV();
-        return c.R[(size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb)];
+        return Bound{c.R + (size_t)NV * NV * bb, NV};
     }
-    __device__ __forceinline__ double2 operator()(int b, int k, int j) const {
+    __device__ __forceinline__ Bound bind(int b) const {
         const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
-        return s == 0 ? op(c0, bb, k, j) : s == 1 ? op(c1, bb, k, j) : op(c2, bb, k, j);
+        return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
 struct K3_Ep {
     K3Geom g; double T; K3Chan c0, c1, c2;
-    __device__ __forceinline__ void op(const K3Chan& c, int bb, int i, int j, double2 v) const {
-        const int NV = g.NV(), NW = g.NW();
-        const int iW = bb % NW, p = bb / NW;
-        const int m = iW - (g.n3b - 1), nu = i - g.n3f;
-        const size_t k2 = (size_t)(m + g.n2b - 1) + (size_t)(2 * g.n2b - 1) * ((size_t)(nu + g.n2f) + (size_t)(2 * g.n2f) * p);
-        if (!c.full) {
-            double2 k2v = c.K2r[k2];
-            if (c.ch == TCH) k2v = csub(cscale(k2v, 2.0), c.K2a[k2]);
-            v = cadd(v, cmul(k2v, c.csum[(size_t)j + (size_t)NV * bb]));
+    struct Bound {
+        const double2* k2r; const double2* k2a;    // k2r[NW2 * i] = K2r(Ω, ν = i - n3f, P)
+        const double2* csum; const double2* Lr; const double2* La;
+        double2* out; const double2* outa;
+        double T; int t, tpost, full, NV, NW2;
+        __device__ __forceinline__ void operator()(int i, int j, double2 v) const {
+            if (!full) {
+                double2 k2v = k2r[(size_t)NW2 * i];
+                if (t) k2v = csub(cscale(k2v, 2.0), k2a[(size_t)NW2 * i]);
+                v = cadd(v, cmul(k2v, csum[j]));
+            }
+            const size_t o = (size_t)i + (size_t)NV * j;
+            double2 self = Lr[o];
+            if (t) self = csub(cscale(self, 2.0), La[o]);
+            double2 x = cadd(cscale(v, T), self);
+            if (tpost) x = cscale(cadd(x, outa[o]), 0.5);
+            out[o] = x;
         }
-        const size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * bb);
-        double2 self = c.Lr[o];
-        if (c.ch == TCH) self = csub(cscale(self, 2.0), c.La[o]);
-        double2 x = cadd(cscale(v, T), self);
-        if (c.tpost) x = cscale(cadd(x, c.outa[o]), 0.5);
-        c.out[o] = x;
+    };
+    __device__ __forceinline__ Bound one(const K3Chan& c, int bb) const {
+        const int NV = g.NV(), NW = g.NW();
+        const int iW = bb % NW, p = bb / NW, m = iW - (g.n3b - 1);
+        const int NW2 = 2 * g.n2b - 1, NV2 = 2 * g.n2f;
+        const size_t nn = (size_t)NV * NV, off = nn * bb;
+        const int t = (c.ch == TCH);
+        // K2r(Ω, ν, P) at ν = i - n3f: index (m + n2b - 1) + NW2 (i - n3f + n2f + NV2 p)
+        const size_t k2o = (size_t)(m + g.n2b - 1) + (size_t)NW2 * ((size_t)(g.n2f - g.n3f) + (size_t)NV2 * p);
+        Bound r;
+        r.k2r = c.full ? nullptr : c.K2r + k2o;
+        r.k2a = (c.full || !t) ? nullptr : c.K2a + k2o;
+        r.csum = c.full ? nullptr : c.csum + (size_t)NV * bb;
+        r.Lr = c.Lr + off;
+        r.La = t ? c.La + off : nullptr;
+        r.out = c.out + off;
+        r.outa = c.tpost ? c.outa + off : nullptr;
+        r.T = T; r.t = t; r.tpost = c.tpost; r.full = c.full; r.NV = NV; r.NW2 = NW2;
+        return r;
     }
-    __device__ __forceinline__ void operator()(int b, int i, int j, double2 v) const {
+    __device__ __forceinline__ Bound bind(int b) const {
         const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
-        if (s == 0) op(c0, bb, i, j, v);
-        else if (s == 1) op(c1, bb, i, j, v);
-        else op(c2, bb, i, j, v);
+        return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
 
@@ -681,9 +731,21 @@ __global__ void k_post_t(double2* __restrict__ t, const double2* __restrict__ a,
 }
 
 // dense column-major operands (library self-test of the GEMM engine)
-struct Dense_A { const double2* A; int lda; __device__ double2 operator()(int b, int i, int k) const { return A[(size_t)i + (size_t)lda * k]; } };
-struct Dense_B { const double2* B; int ldb; __device__ double2 operator()(int b, int k, int j) const { return B[(size_t)k + (size_t)ldb * j]; } };
-struct Dense_C { double2* C; int ldc; __device__ void operator()(int b, int i, int j, double2 v) const { C[(size_t)i + (size_t)ldc * j] = v; } };
+struct Dense_A {
+    const double2* A; int lda;
+    __device__ Dense_A bind(int) const { return *this; }
+    __device__ double2 operator()(int i, int k) const { return A[(size_t)i + (size_t)lda * k]; }
+};
+struct Dense_B {
+    const double2* B; int ldb;
+    __device__ Dense_B bind(int) const { return *this; }
+    __device__ double2 operator()(int k, int j) const { return B[(size_t)k + (size_t)ldb * j]; }
+};
+struct Dense_C {
+    double2* C; int ldc;
+    __device__ Dense_C bind(int) const { return *this; }
+    __device__ void operator()(int i, int j, double2 v) const { C[(size_t)i + (size_t)ldc * j] = v; }
+};
 
 // ---------------------------------------------------------------------------------------
 // BSE weight on the bubble mesh for K1/K2 (src/BSEa/BSEa_K1.jl:38-43, BSEa_K2.jl:77-85):
