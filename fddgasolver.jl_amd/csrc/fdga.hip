@@ -52,6 +52,7 @@ struct fdga_ctx {
     std::string err;
     int NP = 1, NkG = 1, NG = 0;
     int NW1 = 0, NW2 = 0, NV2 = 0, NW3 = 0, NV3 = 0, NWP = 0, NVP = 0, nVP = 0;
+    int wb = 0;   // half-width of the band of the bubble mesh that the gathers read (W, tail sums)
     size_t nGel = 0, nPiel = 0, n1 = 0, n2 = 0, n3 = 0, nL = 0;
     double2 *Gbare = nullptr, *G0 = nullptr, *Sig0 = nullptr, *G = nullptr, *Sig = nullptr;
     double2 *Pi0pp = nullptr, *Pi0ph = nullptr, *Pipp = nullptr, *Piph = nullptr;
@@ -278,10 +279,10 @@ int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale
     return 0;
 }
 
-// tail sums of one array on the bubble mesh (rows of 2 nVP entries)
+// band tail sums of one array on the bubble mesh (rows of 2 nVP entries)
 int tail_sums(fdga_ctx* c, const double2* X, double2* T) {
     const size_t nrows = (size_t)c->NWP * c->NP;
-    k_tail_sums<<<dim3((unsigned)nrows, 2), 256, 0, c->run>>>(X, T, c->nVP, nrows);
+    k_tail_sums<<<(unsigned)nrows, 256, 0, c->run>>>(X, T, c->nVP, c->wb, nrows);
     LAUNCHCHK(c);
     return 0;
 }
@@ -304,12 +305,18 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     // G(R) = fft(G)/LG^2 (src/nonlocal/bubble.jl:110)
     HIPCHK(c, hipMemcpyAsync(c->tmpG1, Gsrc, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
     if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG))) return -3;
-    dim3 g(cdiv(c->NVP, 256), c->NWP, c->NP);
-    k_bubble_nl_R<<<g, 256, 0, c->run>>>(pp, ph, c->tmpG1, d.nG, d.LG, d.L, d.nK1, c->nVP, d.T);
+    // the Π_R products and the backward DFT over R' (src/nonlocal/bubble.jl:110-155) in one pass
+    const int L = d.L;
+    if (L == 16) {
+        constexpr int NB = 16;
+        k_bubble_nl<16, NB><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)NB * 256 * sizeof(double2), c->run>>>(
+            pp, ph, c->tmpG1, d.nG, d.LG, L, d.nK1, c->nVP, d.T, NB);
+    } else {
+        const int NB = std::max(1, std::min(16, (65536 / (int)sizeof(double2) - L) / (L * L)));
+        k_bubble_nl<0, 0><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)(NB * L * L + L) * sizeof(double2), c->run>>>(
+            pp, ph, c->tmpG1, d.nG, d.LG, L, d.nK1, c->nVP, d.T, NB);
+    }
     LAUNCHCHK(c);
-    size_t nrow = (size_t)c->NVP * c->NWP;
-    if (dft2(c, pp, nrow, d.L, +1, 1.0)) return -3;
-    if (dft2(c, ph, nrow, d.L, +1, 1.0)) return -3;
     return tail_sums(c, pp, pi_tail(c, pp)) || tail_sums(c, ph, pi_tail(c, ph));
 }
 
@@ -430,15 +437,15 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
     if (!spp.ncls && !sph.ncls) {
         k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
                                                               lv.kind,
-                                                              d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, lv.chanNP, TU, NK, spp);
+                                                              d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP, lv.chanNP, TU, NK, spp);
     } else {
         k_sde_L<1><<<cdiv(nclass(spp, nout) * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
                                                                            lv.g[2], ref,
-                                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP,
+                                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
                                                                            lv.chanNP, TU, NK, spp);
         k_sde_L<2><<<cdiv(nclass(sph, nout) * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
                                                                            lv.g[2], ref,
-                                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP,
+                                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
                                                                            lv.chanNP, TU, NK, sph);
     }
     LAUNCHCHK(c);
@@ -506,14 +513,16 @@ static int launch_lk2(fdga_ctx* c) {
     return 0;
 }
 
-// K1/K2 weights of channel CH into Wc[CH] and their tail sums into Wtail[CH]
+// K1/K2 weights of channel CH on the band into Wc[CH], their far sums into Wtail[CH]
 template <int CH>
 static int launch_w(fdga_ctx* c) {
-    dim3 gw(cdiv(c->NVP, 256), c->NWP, c->NP);
-    k_bse_w<CH><<<gw, 256, 0, c->run>>>(c->Wc[CH], pi_of(c, CH, false), pi_of(c, CH, true), vtx_FL(c), f0_view(c), c->d.nK1,
-                                       c->nVP);
+    dim3 gw(cdiv(2 * c->wb + 2, 256), c->NWP, c->NP);
+    double2* P = pi_of(c, CH, false);
+    double2* P0 = pi_of(c, CH, true);
+    k_bse_w<CH><<<gw, 256, 0, c->run>>>(c->Wc[CH], c->Wtail[CH], P, P0, pi_tail(c, P), pi_tail(c, P0), vtx_FL(c), f0_view(c),
+                                       c->d.nK1, c->nVP, c->wb, box_bound(c));
     LAUNCHCHK(c);
-    return tail_sums(c, c->Wc[CH], c->Wtail[CH]);
+    return 0;
 }
 
 template <int CH>
@@ -522,7 +531,7 @@ static int launch_k1(fdga_ctx* c) {
     const SymCls sym = symcls(c, SG_K1);
     const size_t nw = nclass(sym, c->n1 * c->NP);
     k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->run>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], c->Wtail[CH], vtx_F(c), c->d.nK1,
-                                                           c->d.nK1, c->nVP, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
+                                                           c->d.nK1, c->nVP, c->wb, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
 }
@@ -535,7 +544,7 @@ static int launch_k2(fdga_ctx* c) {
     k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->Wc[CH], c->Wtail[CH], vtx_F(c),
                                                            chanF(c, c->FL[CH]),
                                                            chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
-                                                           c->NP, c->d.T * kSign[CH], box_bound(c), sym);
+                                                           c->wb, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
 }
@@ -548,7 +557,7 @@ static int launch_k2_at(fdga_ctx* c) {
     k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->Wc[ACH], c->Wc[TCH],
                                                           c->Wtail[ACH], c->Wtail[TCH], vtx_F(c),
                                                           chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]), c->d.nK2b, c->d.nK2f,
-                                                          c->d.nK1, c->nVP, c->NP, c->d.T, box_bound(c), sym);
+                                                          c->d.nK1, c->nVP, c->wb, c->NP, c->d.T, box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
 }
@@ -720,6 +729,11 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
         *out = c;
         return -1;
     }
+    if (d.L > 32) {
+        fail(c, "the lattice bubble transforms L x L planes in LDS: L must be at most 32");
+        *out = c;
+        return -1;
+    }
     if (d.nK1 * d.mPinu < d.nK3f) {
         fail(c, "bubble mesh nK1*mPinu must contain the K3 fermionic box");
         *out = c;
@@ -757,6 +771,9 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     c->NWP = 2 * d.nK1 - 1;
     c->nVP = d.nK1 * d.mPinu;
     c->NVP = 2 * c->nVP;
+    // the gathers read W and the tail sums only for |ω| <= Wc = NK + |Ω| + |ν| + 2 with Ω, ν in the
+    // K1 or K2 box (BSE_K1!, BSE_K2!, SDE L-vertices)
+    c->wb = std::min(c->nVP, box_bound(c) + std::max(d.nK1 - 1, d.nK2b - 1 + d.nK2f) + 3);
     c->nGel = (size_t)c->NG * c->NkG;
     c->nPiel = (size_t)c->NVP * c->NWP * c->NP;
     c->n1 = (size_t)c->NW1;
@@ -767,8 +784,9 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     int rc = 0;
     for (double2** p : {&c->Gbare, &c->G0, &c->Sig0, &c->G, &c->Sig, &c->tmpG1, &c->tmpG2, &c->tmpG3})
         rc |= dalloc(c, p, c->nGel);
-    for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph, &c->Wc[0], &c->Wc[1], &c->Wc[2]}) rc |= dalloc(c, p, c->nPiel);
-    c->ntail = (size_t)c->NWP * c->NP * (2 * (size_t)c->nVP + 2);
+    for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph}) rc |= dalloc(c, p, c->nPiel);
+    for (double2** p : {&c->Wc[0], &c->Wc[1], &c->Wc[2]}) rc |= dalloc(c, p, (size_t)c->NWP * c->NP * (2 * (size_t)c->wb));
+    c->ntail = (size_t)c->NWP * c->NP * (2 * (size_t)c->wb + 2);
     for (double2** p : {&c->Wtail[0], &c->Wtail[1], &c->Wtail[2], &c->Ptail[0], &c->Ptail[1], &c->Ptail[2], &c->Ptail[3]})
         rc |= dalloc(c, p, c->ntail);
     for (double2** p : {&c->Lpp, &c->Lph, &c->tmpL1, &c->tmpL2}) rc |= dalloc(c, p, c->nL);

@@ -3,12 +3,12 @@
 // Kernel map (reference function -> kernel):
 //   Dyson!                 k_dyson
 //   bubbles! (local)       k_bubble_local        elementwise, HBM bound
-//   bubbles_real_space!    k_dft2_rows + k_bubble_nl_R + k_dft2_rows
+//   bubbles_real_space!    k_dft2_rows (G -> G(R)) + k_bubble_nl (products and bfft over R' in LDS)
 //   s-wave averages        k_pavg
 //   build_K3_cache!        k_cache               gather: Γ_irr / F0 / F-base caches
 //   BSE_L_K3!, BSE_K3!     zgemm_tiles<...>      batched complex GEMM on v_mfma_f64_16x16x4f64
 //   BSE_K1!, BSE_K2!,      k_bse_w + k_gemv_*    gather-GEMV over the bubble mesh
-//   BSE_L_K2!
+//   BSE_L_K2!              (k_bse_w and k_tail_sums keep W and the far sums on the band the gathers read)
 //   SDE!                   k_sde_L, k_sde_sigma_local, k_u2_*, k_occupation
 #pragma once
 #include "fdga_device.h"
@@ -161,23 +161,82 @@ __global__ void k_dft2_rows(double2* __restrict__ data, size_t nrows, int L, int
     }
 }
 
-// bubbles_real_space! product step (src/nonlocal/bubble.jl:110-150): Π_R[ω, Ω, R'] accumulated
-// over the R in [-L/2, L/2]^2 that fold onto R' = mod(R, L); GR = G(ν, R) on the LG mesh.
-__global__ void k_bubble_nl_R(double2* __restrict__ pp, double2* __restrict__ ph, const double2* __restrict__ GR,
-                              int nG, int LG, int L, int nW, int nV, double T) {
-    int iw = blockIdx.x * blockDim.x + threadIdx.x;
-    int iW = blockIdx.y;
-    int iR = blockIdx.z;
-    if (iw >= 2 * nV) return;
-    const int NG = 2 * nG;
-    const int m = iW - (nW - 1), n = iw - nV;
-    const int r1p = iR % L, r2p = iR / L;
-    const int h = L / 2;
-    double2 spp = czero(), sph = czero();
-    for (int R2 = -h; R2 <= h; ++R2) {
-        if (((R2 % L) + L) % L != r2p) continue;
-        for (int R1 = -h; R1 <= h; ++R1) {
-            if (((R1 % L) + L) % L != r1p) continue;
+// bubbles_real_space! (src/nonlocal/bubble.jl:87-158) fused for a tile of NB bubble frequencies ω
+// at one Ω: the real-space products Π_R(ω, Ω, R') over the R in [-L/2, L/2]^2 that fold onto
+// R' = mod(R, L) (half weight at |R_i| = LG/2 for even LG; the 1/ν tail of G only at R = 0) are
+// formed in LDS, transformed by the unnormalised backward 2D DFT over R' (FFTW bfft, sign +1) in
+// LDS, and written once as Π(ω, Ω, q), q = q1 + L q2.  Πpp and Πph use the LDS plane in turn.
+// LDS plane: s[R' * NB + ω - ω0] (the NB frequencies of one R' are contiguous).
+template <int L>
+struct BubbleDft {
+    // y = Σ_j x[j] e^{+2πi jk/L} over j < L, twiddles tw[k] = e^{+2πi k/L}
+    __device__ static void run(double2* x, const double2* tw, int Lr) {
+        double2 y[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            double2 acc = czero();
+#pragma unroll
+            for (int j = 0; j < L; ++j) acc = cadd(acc, cmul(x[j], tw[(j * k) % L]));
+            y[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < L; ++k) x[k] = y[k];
+    }
+};
+// length 16: radix-4 in two stages, j = 4 j1 + j2, k = k1 + 4 k2:
+//   X[k1 + 4 k2] = Σ_j2 e^{2πi j2 k2/4} [e^{2πi j2 k1/16} Σ_j1 x[4 j1 + j2] e^{2πi j1 k1/4}]
+__device__ __forceinline__ void dft4p(double2& a, double2& b, double2& c, double2& d) {
+    // sign +1: X0 = a+b+c+d, X1 = a+ib-c-id, X2 = a-b+c-d, X3 = a-ib-c+id
+    const double2 s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = csub(b, d);
+    const double2 is3 = cmk(-s3.y, s3.x);
+    a = cadd(s0, s2);
+    b = cadd(s1, is3);
+    c = csub(s0, s2);
+    d = csub(s1, is3);
+}
+template <>
+struct BubbleDft<16> {
+    __device__ static void run(double2* x, const double2*, int) {
+        constexpr double C1 = 0.92387953251128675613, S1 = 0.38268343236508977173, C2 = 0.70710678118654752440;
+#pragma unroll
+        for (int j2 = 0; j2 < 4; ++j2) dft4p(x[j2], x[4 + j2], x[8 + j2], x[12 + j2]);
+        // x[4 k1 + j2] now holds A[j2][k1]; twiddle e^{2πi j2 k1/16}
+        const double2 w1 = cmk(C1, S1), w2 = cmk(C2, C2), w3 = cmk(S1, C1), w6 = cmk(-C2, C2), w9 = cmk(-C1, -S1);
+        x[4 + 1] = cmul(x[4 + 1], w1);
+        x[4 + 2] = cmul(x[4 + 2], w2);
+        x[4 + 3] = cmul(x[4 + 3], w3);
+        x[8 + 1] = cmul(x[8 + 1], w2);
+        x[8 + 2] = cmk(-x[8 + 2].y, x[8 + 2].x);   // e^{iπ/2}
+        x[8 + 3] = cmul(x[8 + 3], w6);
+        x[12 + 1] = cmul(x[12 + 1], w3);
+        x[12 + 2] = cmul(x[12 + 2], w6);
+        x[12 + 3] = cmul(x[12 + 3], w9);
+        double2 y[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            double2 a = x[4 * k1], b = x[4 * k1 + 1], c = x[4 * k1 + 2], d = x[4 * k1 + 3];
+            dft4p(a, b, c, d);
+            y[k1] = a; y[k1 + 4] = b; y[k1 + 8] = c; y[k1 + 12] = d;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = y[k];
+    }
+};
+
+// Π_R(ω, Ω, R') for one R' (pp: which = 0, ph: 1; see k_bubble_nl)
+__device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, int nG, int LG, int L, double T, int m, int n,
+                                            int r1p, int r2p, int which) {
+    const int NG = 2 * nG, h = L / 2;
+    int R1s[2], R2s[2], n1 = 0, n2 = 0;
+    if (r1p <= h) R1s[n1++] = r1p;
+    if (r1p - L >= -h && r1p - L != r1p) R1s[n1++] = r1p - L;
+    if (r2p <= h) R2s[n2++] = r2p;
+    if (r2p - L >= -h && r2p - L != r2p) R2s[n2++] = r2p - L;
+    double2 acc = czero();
+    const int n2nd = which == 0 ? bmf(m, n) : bpf(m, n);
+    for (int b2 = 0; b2 < n2; ++b2)
+        for (int b1 = 0; b1 < n1; ++b1) {
+            const int R1 = R1s[b1], R2 = R2s[b2];
             double w = 1.0;
             if (LG % 2 == 0) {
                 if (abs(R1) == LG / 2) w *= 0.5;
@@ -190,13 +249,67 @@ __global__ void k_bubble_nl_R(double2* __restrict__ pp, double2* __restrict__ ph
                 if (in_f(k, nG)) return GR[(size_t)(k + nG) + (size_t)NG * kk];
                 return origin ? cmk(1.0 / (M_PI * T * (2.0 * k + 1.0)), 0.0) : czero();
             };
-            spp = cadd(spp, cscale(cmul(gat(bmf(m, n), kp), gat(n, kp)), w));
-            sph = cadd(sph, cscale(cmul(gat(bpf(m, n), kp), gat(n, km)), w));
+            acc = cadd(acc, cscale(cmul(gat(n2nd, kp), gat(n, which == 0 ? kp : km)), w));
+        }
+    return acc;
+}
+
+// LT: L fixed at compile time (16: radix-4 FFT; others: unrolled direct DFT), or 0 for a run-time
+// L <= 32 (direct DFT).  Block: 256 threads, NB * L <= 256 needed only for full use of the block.
+template <int LT, int NBT>
+__global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, double2* __restrict__ ph,
+                                                   const double2* __restrict__ GR, int nG, int LG, int Lr, int nW, int nV,
+                                                   double T, int NBr) {
+    extern __shared__ double2 sm[];   // NB * L * L (+ L twiddles unless L = 16)
+    const int L = LT ? LT : Lr, LL = L * L, NB = NBT ? NBT : NBr;
+    double2* s = sm;
+    double2* tw = sm + NB * LL;
+    const int tid = threadIdx.x;
+    const int iw0 = blockIdx.x * NB, iW = blockIdx.y;
+    const int m = iW - (nW - 1);
+    const int nb = min(NB, 2 * nV - iw0);
+    if (LT != 16)
+        for (int k = tid; k < L; k += blockDim.x) {
+            double sn, cs;
+            sincospi(2.0 * k / L, &sn, &cs);
+            tw[k] = cmk(cs, sn);
+        }
+    for (int which = 0; which < 2; ++which) {
+        // products
+        for (int e = tid; e < NB * LL; e += blockDim.x) {
+            const int wl = e % NB, R = e / NB;
+            s[e] = wl < nb ? bubble_R(GR, nG, LG, L, T, m, iw0 + wl - nV, R % L, R / L, which) : czero();
+        }
+        __syncthreads();
+        // along R1' (thread: ω, R2'), then along R2' (thread: ω, q1) with the write-out
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int e = tid; e < NB * L; e += blockDim.x) {
+                const int wl = e % NB, r = e / NB;
+                const int base = pass == 0 ? L * r : r, step = pass == 0 ? 1 : L;
+                double2 x[LT ? LT : 32];
+                for (int j = 0; j < L; ++j) x[j] = s[(base + step * j) * NB + wl];
+                if (LT) BubbleDft<LT ? LT : 1>::run(x, tw, L);
+                else {
+                    double2 y[32];
+                    for (int k = 0; k < L; ++k) {
+                        double2 acc = czero();
+                        for (int j = 0; j < L; ++j) acc = cadd(acc, cmul(x[j], tw[(j * k) % L]));
+                        y[k] = acc;
+                    }
+                    for (int k = 0; k < L; ++k) x[k] = y[k];
+                }
+                if (pass == 0) {
+                    for (int k = 0; k < L; ++k) s[(base + k) * NB + wl] = x[k];
+                } else if (wl < nb) {
+                    double2* out = which == 0 ? pp : ph;
+                    const size_t o = (size_t)(iw0 + wl) + (size_t)(2 * nV) * (size_t)iW;
+                    const size_t qs = (size_t)(2 * nV) * (size_t)(2 * nW - 1);
+                    for (int k = 0; k < L; ++k) out[o + qs * (size_t)(r + L * k)] = x[k];
+                }
+            }
+            __syncthreads();
         }
     }
-    size_t o = (size_t)iw + (size_t)(2 * nV) * (iW + (size_t)(2 * nW - 1) * iR);
-    pp[o] = spp;
-    ph[o] = sph;
 }
 
 // s-wave average over P of one array of `n` elements per P
@@ -750,26 +863,47 @@ struct Dense_C {
 // ---------------------------------------------------------------------------------------
 // BSE weight on the bubble mesh for K1/K2 (src/BSEa/BSEa_K1.jl:38-43, BSEa_K2.jl:77-85):
 //   W(ω, Ω, P) = (Π - Π0) F0(Ω, ω~, ∞, Ch, Sp) + Π FL(Ω, ω~, ∞, P, Ch, Sp),  ω~ = crossing(Ω, ω)
+// W is stored on the band of 2 wb frequencies around ω = 0 that the gathers read (iw in
+// [nVP - wb, nVP + wb), row stride 2 wb), and its far sums (the tail layout of k_tail_sums) are
+// formed from the tail sums of Π and Π0: for |ω| > NK + |Ω| + 2 the factors F0(Ω, ω~, ∞) and
+// FL(Ω, ω~, ∞) no longer depend on ω (|ω~| lies beyond every box, so only the bare vertex and the
+// channel's K1(Ω) remain), so Σ_far W = F0_far Σ_far (Π - Π0) + FL_far Σ_far Π.  That holds for the
+// tails the gathers use (lo <= nVP - NK - |Ω| - 2); entries nearer ω = 0 are not used.
 template <int CH>
-__global__ __launch_bounds__(256) void k_bse_w(double2* __restrict__ W, const double2* __restrict__ Pi,
-                                               const double2* __restrict__ Pi0, VtxV FL, F0V f0, int nWP, int nVP) {
+__global__ __launch_bounds__(256) void k_bse_w(double2* __restrict__ W, double2* __restrict__ Wt,
+                                               const double2* __restrict__ Pi, const double2* __restrict__ Pi0,
+                                               const double2* __restrict__ Pt, const double2* __restrict__ P0t,
+                                               VtxV FL, F0V f0, int nWP, int nVP, int wb, int NK) {
     constexpr int ch = CH, sp = ChSpin<CH>::SP;
-    int iw = blockIdx.x * blockDim.x + threadIdx.x;
-    int iW = blockIdx.y, p = blockIdx.z;
-    if (iw >= 2 * nVP) return;
-    const int m = iW - (nWP - 1), w = iw - nVP;
-    const size_t o = (size_t)iw + (size_t)(2 * nVP) * ((size_t)iW + (size_t)(2 * nWP - 1) * p);
-    const int wc = crossing(m, w, ch);
-    double2 f0r = f0.eval(m, wc, FDGA_INF, ch, sp);
-    double2 flr = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
-    W[o] = cadd(cmul(csub(Pi[o], Pi0[o]), f0r), cmul(Pi[o], flr));
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iW = blockIdx.y, p = blockIdx.z;
+    const int m = iW - (nWP - 1);
+    const size_t row = (size_t)iW + (size_t)(2 * nWP - 1) * p;
+    if (t < 2 * wb) {
+        const int iw = nVP - wb + t, w = iw - nVP;
+        const size_t o = (size_t)iw + (size_t)(2 * nVP) * row;
+        const int wc = crossing(m, w, ch);
+        const double2 f0r = f0.eval(m, wc, FDGA_INF, ch, sp);
+        const double2 flr = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
+        const double2 pi = Pi[o];
+        W[(size_t)t + (size_t)(2 * wb) * row] = cadd(cmul(csub(pi, Pi0[o]), f0r), cmul(pi, flr));
+    }
+    if (t < 2 * (wb + 1)) {
+        const int wf = t > wb ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
+        const int wc = crossing(m, wf, ch);
+        const double2 f0r = f0.eval(m, wc, FDGA_INF, ch, sp);
+        const double2 flr = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
+        const size_t o = (size_t)t + (size_t)(2 * (wb + 1)) * row;
+        const double2 pt = Pt[o];
+        Wt[o] = cadd(cmul(csub(pt, P0t[o]), f0r), cmul(pt, flr));
+    }
 }
 
-// Tail sums of rows on the bubble ν-mesh (2 nVP entries, ω index iw = ω + nVP): per row r,
-//   left[i]  = Σ_{k < i} X[k]           (i = 0 .. nVP)
-//   right[j] = Σ_{k >= nVP + j} X[k]    (j = 0 .. nVP)
-// stored as T[r * 2 (nVP + 1) + i] and T[r * 2 (nVP + 1) + nVP + 1 + j].  Each side is accumulated
-// from its outer end, so a far tail is summed from its smallest entries.
+// Tail sums of rows on the bubble ν-mesh (2 nVP entries, ω index iw = ω + nVP), kept on the band
+// of half-width wb around ω = 0 that the gathers can ask for: per row r,
+//   left[i]  = Σ_{k < nVP - wb + i} X[k]    (i = 0 .. wb)
+//   right[j] = Σ_{k >= nVP + j} X[k]        (j = 0 .. wb)
+// stored as T[r * 2 (wb + 1) + i] and T[r * 2 (wb + 1) + wb + 1 + j].
 __device__ __forceinline__ void wave_incl_scan(double& r, double& i, int lane) {
 #pragma unroll
     for (int o = 1; o < WAVE; o <<= 1) {
@@ -777,48 +911,60 @@ __device__ __forceinline__ void wave_incl_scan(double& r, double& i, int lane) {
         if (lane >= o) { r += yr; i += yi; }
     }
 }
-// one 256-thread block per (row, side): thread t sums a contiguous segment (from the outer end
-// inwards), the segment totals are scanned across the block, then each thread writes its
-// segment's running sums
-__global__ __launch_bounds__(256) void k_tail_sums(const double2* __restrict__ X, double2* __restrict__ T, int nVP,
+// One 256-thread block per row.  The parts outside the band are summed with a strided
+// (coalesced) loop and a block reduction; the band is scanned in chunks of 256 entries (wave
+// scans, then the preceding waves' totals), the left side from its outer end inwards and the
+// right side from ω = wb - 1 down to 0.
+__global__ __launch_bounds__(256) void k_tail_sums(const double2* __restrict__ X, double2* __restrict__ T, int nVP, int wb,
                                                    size_t nrows) {
-    __shared__ double pr[256 / WAVE], pi[256 / WAVE];
+    constexpr int NWV = 256 / WAVE;
+    __shared__ double sh[4][NWV];
     const size_t row = blockIdx.x;
-    const int side = blockIdx.y, tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     if (row >= nrows) return;
     const double2* x = X + row * (size_t)(2 * nVP);
-    double2* tl = T + row * (size_t)(2 * (nVP + 1));
-    double2* tr = tl + (nVP + 1);
-    // q-th element from the outer end of this side: left k = q, right k = 2 nVP - 1 - q
-    auto at = [&](int q) { return side == 0 ? x[q] : x[2 * nVP - 1 - q]; };
-    const int seg = (nVP + 255) / 256, q0 = tid * seg, q1 = min(nVP, q0 + seg);
-    double sr = 0.0, si = 0.0;
-    for (int q = q0; q < q1; ++q) { const double2 v = at(q); sr += v.x; si += v.y; }
-    // exclusive prefix of the segment totals: wave scan, then the preceding waves' totals
-    double ir = sr, ii = si;
-    wave_incl_scan(ir, ii, lane);
-    if (lane == WAVE - 1) { pr[wv] = ir; pi[wv] = ii; }
+    double2* tl = T + row * (size_t)(2 * (wb + 1));
+    double2* tr = tl + (wb + 1);
+    const int nout = nVP - wb;   // entries of each side outside the band
+    double lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+    for (int q = tid; q < nout; q += 256) {
+        const double2 a = x[q], b = x[2 * nVP - 1 - q];
+        lr += a.x; li += a.y; rr += b.x; ri += b.y;
+    }
+    lr = wave_sum(lr); li = wave_sum(li); rr = wave_sum(rr); ri = wave_sum(ri);
+    if (lane == 0) { sh[0][wv] = lr; sh[1][wv] = li; sh[2][wv] = rr; sh[3][wv] = ri; }
     __syncthreads();
-    double br = ir - sr, bi = ii - si;
-    for (int k = 0; k < wv; ++k) { br += pr[k]; bi += pi[k]; }
-    // running sums: left tl[q + 1] = Σ_{q' <= q}; right tr[k - nVP] = Σ_{k' >= k} with k = 2 nVP - 1 - q
-    if (tid == 0) {
-        if (side == 0) tl[0] = czero();
-        else tr[nVP] = czero();
-    }
-    for (int q = q0; q < q1; ++q) {
-        const double2 v = at(q);
-        br += v.x; bi += v.y;
-        if (side == 0) tl[q + 1] = cmk(br, bi);
-        else tr[nVP - 1 - q] = cmk(br, bi);
+    double cl_r = 0.0, cl_i = 0.0, cr_r = 0.0, cr_i = 0.0;   // running sums carried across chunks
+#pragma unroll
+    for (int k = 0; k < NWV; ++k) { cl_r += sh[0][k]; cl_i += sh[1][k]; cr_r += sh[2][k]; cr_i += sh[3][k]; }
+    if (tid == 0) { tl[0] = cmk(cl_r, cl_i); tr[wb] = cmk(cr_r, cr_i); }
+    for (int c0 = 0; c0 < wb; c0 += 256) {
+        __syncthreads();   // sh is reused
+        const int i = c0 + tid;
+        const double2 a = i < wb ? x[nout + i] : czero();
+        const double2 b = i < wb ? x[nVP + wb - 1 - i] : czero();
+        double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+        wave_incl_scan(ar, ai, lane);
+        wave_incl_scan(br, bi, lane);
+        if (lane == WAVE - 1) { sh[0][wv] = ar; sh[1][wv] = ai; sh[2][wv] = br; sh[3][wv] = bi; }
+        __syncthreads();
+        double pl_r = cl_r, pl_i = cl_i, pr_r = cr_r, pr_i = cr_i;
+        for (int k = 0; k < wv; ++k) { pl_r += sh[0][k]; pl_i += sh[1][k]; pr_r += sh[2][k]; pr_i += sh[3][k]; }
+        if (i < wb) {
+            tl[i + 1] = cmk(pl_r + ar, pl_i + ai);
+            tr[wb - 1 - i] = cmk(pr_r + br, pr_i + bi);
+        }
+#pragma unroll
+        for (int k = 0; k < NWV; ++k) { cl_r += sh[0][k]; cl_i += sh[1][k]; cr_r += sh[2][k]; cr_i += sh[3][k]; }
     }
 }
-// far sums of one row outside the window [lo, hi]: Σ_{k < lo} and Σ_{k > hi} (lo <= nVP < hi + 1)
-__device__ __forceinline__ double2 tail_left(const double2* T, size_t row, int nVP, int lo) {
-    return T[row * (size_t)(2 * (nVP + 1)) + lo];
+// far sums of one row outside the window [lo, hi]: Σ_{k < lo} and Σ_{k > hi}
+// (nVP - wb <= lo <= nVP < hi + 1 <= nVP + wb)
+__device__ __forceinline__ double2 tail_left(const double2* T, size_t row, int nVP, int wb, int lo) {
+    return T[row * (size_t)(2 * (wb + 1)) + (lo - (nVP - wb))];
 }
-__device__ __forceinline__ double2 tail_right(const double2* T, size_t row, int nVP, int hi) {
-    return T[row * (size_t)(2 * (nVP + 1)) + (nVP + 1) + (hi + 1 - nVP)];
+__device__ __forceinline__ double2 tail_right(const double2* T, size_t row, int nVP, int wb, int hi) {
+    return T[row * (size_t)(2 * (wb + 1)) + (wb + 1) + (hi + 1 - nVP)];
 }
 
 // Frequency window of the gather-GEMVs.  Every vertex term that depends on the summed
@@ -840,7 +986,7 @@ __device__ __forceinline__ void far_pair(double2 t, double2& left, double2& righ
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a,
                                                  const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
-                                                 int n1, int nWP, int nVP, int NP, double scale, int NK, SymCls sym) {
+                                                 int n1, int nWP, int nVP, int wb, int NP, double scale, int NK, SymCls sym) {
     constexpr int SP = ChSpin<CH>::SP;
     constexpr int NT = 256;
     __shared__ double red[NT / WAVE][2];
@@ -852,11 +998,11 @@ __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const
     const int m = iW - (n1 - 1);
     const int Wc = window(NK, m, 0);
     const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
-    const double2* Wr = W + (size_t)(2 * nVP) * row;
+    const double2* Wr = W + (size_t)(2 * wb) * row + wb;   // Wr[ω], |ω| within the band
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
     double sr = 0.0, si = 0.0;
     for (int iw = lo + tid; iw <= hi; iw += NT) {
-        double2 x = cmul(F.eval(m, FDGA_INF, iw - nVP, p, CH, SP, FL_ALL), Wr[iw]);
+        double2 x = cmul(F.eval(m, FDGA_INF, iw - nVP, p, CH, SP, FL_ALL), Wr[iw - nVP]);
         sr += x.x; si += x.y;
     }
     sr = wave_sum(sr); si = wave_sum(si);
@@ -869,8 +1015,8 @@ __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const
     far_pair(F.eval(m, FDGA_INF, lane == 0 ? -Wc - 1 : Wc + 1, p, CH, SP, FL_ALL), tl, tr);
     // every lane of wave 0 holds the same sums; lanes write the class members
     double2 acc = cmk(tsr, tsi);
-    if (lo > 0) acc = cadd(acc, cmul(tl, tail_left(Wtail, row, nVP, lo)));
-    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, tail_right(Wtail, row, nVP, hi)));
+    if (lo > 0) acc = cadd(acc, cmul(tl, tail_left(Wtail, row, nVP, wb, lo)));
+    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, tail_right(Wtail, row, nVP, wb, hi)));
     const double2 raw = cscale(acc, scale);
     int b, e;
     sym.range(wid, b, e);
@@ -912,7 +1058,7 @@ __device__ __forceinline__ double2 k2d_rank1(const VtxV& F, int m, int v, int p)
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a,
                                                  const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
-                                                 ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int nVP, int NP,
+                                                 ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int nVP, int wb, int NP,
                                                  double scale, int NK, SymCls sym) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
@@ -922,12 +1068,12 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
     const int m = iW - (n2b - 1), v = iv - n2f;
     const int Wc = window(NK, m, v);
     const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
-    const double2* Wr = W + (size_t)(2 * nVP) * row;
+    const double2* Wr = W + (size_t)(2 * wb) * row + wb;   // Wr[ω], |ω| within the band
     // only the window is iterated for the summand; the far weights come from the tail sums
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
     double sr = 0.0, si = 0.0, mr = 0.0, mi = 0.0;
     for (int iw = lo + lane; iw <= hi; iw += WAVE) {
-        const double2 wv = Wr[iw];
+        const double2 wv = Wr[iw - nVP];
         double2 x = cmul(k2d_term<CH>(F, m, v, iw - nVP, p), wv);
         sr += x.x; si += x.y;
         mr += wv.x; mi += wv.y;
@@ -936,8 +1082,8 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
     far_pair(k2d_term<CH>(F, m, v, lane == 0 ? -Wc - 1 : Wc + 1, p), tl, tr);
     sr = wave_sum(sr); si = wave_sum(si);
     mr = wave_sum(mr); mi = wave_sum(mi);
-    const double2 wl = lo > 0 ? tail_left(Wtail, row, nVP, lo) : czero();
-    const double2 wr = hi < 2 * nVP - 1 ? tail_right(Wtail, row, nVP, hi) : czero();
+    const double2 wl = lo > 0 ? tail_left(Wtail, row, nVP, wb, lo) : czero();
+    const double2 wr = hi < 2 * nVP - 1 ? tail_right(Wtail, row, nVP, wb, hi) : czero();
     double2 acc = cadd(cadd(cmk(sr, si), cmul(tl, wl)), cmul(tr, wr));
     acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), cadd(cmk(mr, mi), cadd(wl, wr))));
     const double2 raw = cscale(acc, scale);
@@ -965,7 +1111,7 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
                                                     const double2* __restrict__ Wa, const double2* __restrict__ Wt,
                                                     const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
                                                     VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP,
-                                                    int NP, double T, int NK, SymCls sym) {
+                                                    int wb, int NP, double T, int NK, SymCls sym) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= sym.count(NW2 * NV2 * NP)) return;
@@ -974,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
     const int m = iW - (n2b - 1), v = iv - n2f;
     const int Wc = window(NK, m, v);
     const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
-    const size_t roff = (size_t)(2 * nVP) * row;
+    const size_t roff = (size_t)(2 * wb) * row + wb;   // band rows, indexed by ω
     const double2* wa = Wa + roff;
     const double2* wt = Wt + roff;
     const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
@@ -984,7 +1130,7 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
         const int w = iw - nVP;
         const double2 fa = k2d_psp<ACH>(F, m, v, w, p);
         const double2 ft = k2d_psp<TCH>(F, m, v, w, p);
-        const double2 wav = wa[iw], wtv = wt[iw];
+        const double2 wav = wa[w], wtv = wt[w];
         const double2 xa = cmul(fa, wav);
         const double2 xt = cmul(csub(cscale(ft, 2.0), fa), wtv);
         ar += xa.x; ai += xa.y; tr += xt.x; ti += xt.y;
@@ -1000,8 +1146,10 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
     ar = wave_sum(ar); ai = wave_sum(ai); tr = wave_sum(tr); ti = wave_sum(ti);
     amr = wave_sum(amr); ami = wave_sum(ami); tmr = wave_sum(tmr); tmi = wave_sum(tmi);
     const bool hl = lo > 0, hr = hi < 2 * nVP - 1;
-    const double2 awl = hl ? tail_left(Wat, row, nVP, lo) : czero(), awr = hr ? tail_right(Wat, row, nVP, hi) : czero();
-    const double2 twl = hl ? tail_left(Wtt, row, nVP, lo) : czero(), twr = hr ? tail_right(Wtt, row, nVP, hi) : czero();
+    const double2 awl = hl ? tail_left(Wat, row, nVP, wb, lo) : czero();
+    const double2 awr = hr ? tail_right(Wat, row, nVP, wb, hi) : czero();
+    const double2 twl = hl ? tail_left(Wtt, row, nVP, wb, lo) : czero();
+    const double2 twr = hr ? tail_right(Wtt, row, nVP, wb, hi) : czero();
     double2 A = cadd(cadd(cmk(ar, ai), cmul(fal, awl)), cmul(far_, awr));
     double2 D = cadd(cadd(cmk(tr, ti), cmul(fdl, twl)), cmul(fdr, twr));
     A = cadd(A, cmul(k2d_rank1<ACH>(F, m, v, p), cadd(cmk(amr, ami), cadd(awl, awr))));
@@ -1088,7 +1236,7 @@ template <int WHICH>
 __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double2* __restrict__ Lph, const double2* __restrict__ Pipp,
                         const double2* __restrict__ Piph, const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
                         ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
-                        int n2b, int n2f, int nWP, int nVP, int NP, int chanNP, double2 TU, int NK, SymCls sym) {
+                        int n2b, int n2f, int nWP, int nVP, int wb, int NP, int chanNP, double2 TU, int NK, SymCls sym) {
     constexpr bool DP = WHICH & 1, DH = WHICH & 2;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
@@ -1115,12 +1263,12 @@ __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double
     ar = wave_sum(ar); ai = wave_sum(ai); br = wave_sum(br); bi = wave_sum(bi);
     double2 A = cmk(ar, ai), B = cmk(br, bi);
     if (lo > 0) {
-        if (DP) A = cadd(A, cmul(tail_left(Tpp, row, nVP, lo), xpl));
-        if (DH) B = cadd(B, cmul(tail_left(Tph, row, nVP, lo), xhl));
+        if (DP) A = cadd(A, cmul(tail_left(Tpp, row, nVP, wb, lo), xpl));
+        if (DH) B = cadd(B, cmul(tail_left(Tph, row, nVP, wb, lo), xhl));
     }
     if (hi < 2 * nVP - 1) {
-        if (DP) A = cadd(A, cmul(tail_right(Tpp, row, nVP, hi), xpr));
-        if (DH) B = cadd(B, cmul(tail_right(Tph, row, nVP, hi), xhr));
+        if (DP) A = cadd(A, cmul(tail_right(Tpp, row, nVP, wb, hi), xpr));
+        if (DH) B = cadd(B, cmul(tail_right(Tph, row, nVP, wb, hi), xhr));
     }
     const double2 rawp = cmul(A, TU), rawh = cmul(B, TU);
     int b, e;

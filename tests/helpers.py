@@ -13,6 +13,14 @@ CASES = {
     "nl_LG4_L2": (True, 4, 2, 1, False),
 }
 
+# lattices used by the bubble and band tests only: L = 16 takes the radix-4 FFT of the fused
+# lattice bubble, L = 6 (even, half weights at |R| = L/2) and L = 5 with LG = 10 the generic DFT
+EXTRA_CASES = {
+    "nl_L16": (True, 16, 16, 0, False),
+    "nl_L6": (True, 6, 6, 1, True),
+    "nl_LG10_L5": (True, 10, 5, 0, False),
+}
+
 BOX = dict(nG=10, nK1=6, nK2=(4, 4), nK3=(3, 3), mPinu=2, T=0.3, U=1.3)
 REF_BOX = (7, (5, 4), (3, 3))
 TAB = (2, 2)
@@ -26,7 +34,7 @@ def make_pair(name, seed=0, fdpa_h0=True, box=None, precision="fp64"):
     """Returns (hip_solver, oracle) with identical random inputs for case `name`."""
     import fdga
     from fdga import _lib as L
-    nonlocal_, LG, Lv, ref_kind, tables = CASES[name]
+    nonlocal_, LG, Lv, ref_kind, tables = CASES[name] if name in CASES else EXTRA_CASES[name]
     b = dict(BOX if box is None else box)
     rng = np.random.default_rng(seed)
     tab = TAB if tables else (1, 1)

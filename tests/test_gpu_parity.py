@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from helpers import CASES, make_pair, relerr
+from helpers import BOX, CASES, make_pair, relerr
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -105,6 +105,32 @@ def test_sde(case, strategy, fdga_mod):
 @pytest.mark.parametrize("strategy", ["scPA", "fdPA"])
 def test_full_iteration(case, strategy, fdga_mod):
     S, O = make_pair(case, seed=3)
+    st = po.SCPA if strategy == "scPA" else po.FDPA
+    for it in range(2):
+        fdga_mod.iterate_solver(S, strategy)
+        O.iterate(st)
+        x = fdga_mod.flatten(S)
+        y = O.flatten()
+        assert relerr(x, y) < TOL, f"{case} {strategy} iteration {it}: {relerr(x, y):.3e}"
+
+
+@pytest.mark.parametrize("case", ["nl_L16", "nl_L6", "nl_LG10_L5"])
+def test_lattice_bubble_fused_dft(case, fdga_mod):
+    """Lattice bubbles (real-space products and the backward DFT over R' in one kernel): the
+    radix-4 FFT at L = 16 and the direct DFT at other L, against the oracle's real-space sums."""
+    from fdga import _lib as L
+    S, O = make_pair(case, seed=2)
+    for hw, ow in ((L.PI0PP, po.PI0PP), (L.PI0PH, po.PI0PH), (L.PIPP, po.PIPP), (L.PIPH, po.PIPH)):
+        _cmp(S, O, hw, ow, what=f"{case} array {hw}")
+
+
+@pytest.mark.parametrize("case", ["local_vertexref", "nl_L2", "nl_L16"])
+@pytest.mark.parametrize("strategy", ["scPA", "fdPA"])
+def test_full_iteration_band_narrower_than_bubble_mesh(case, strategy, fdga_mod):
+    """mΠν = 6: the BSE weights W and the tail sums are stored only on the band of the bubble
+    mesh that the gathers read, and W's far sums come from those of Π and Π0."""
+    box = dict(BOX, mPinu=6, nG=40)
+    S, O = make_pair(case, seed=4, box=box)
     st = po.SCPA if strategy == "scPA" else po.FDPA
     for it in range(2):
         fdga_mod.iterate_solver(S, strategy)
