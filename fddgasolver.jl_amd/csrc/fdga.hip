@@ -62,6 +62,8 @@ struct fdga_ctx {
     double2* Ptail[4] = {nullptr, nullptr, nullptr, nullptr};      // tail sums of Π0pp, Π0ph, Πpp, Πph
     size_t ntail = 0;                                               // elements of one tail-sum array
     int w_fresh = 0;                                                // Wc valid for the current iteration
+    // lattice BSE_K2! / BSE_L_K2! tables and GEMM result (s-wave solver only)
+    double2 *k2x = nullptr, *k2xf = nullptr, *k2y = nullptr, *lk2x = nullptr;
     double2 *sde0 = nullptr;
     int sde0_valid = 0, sde0_h = -1;
     double2 *tab[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -434,6 +436,13 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
     const int NK = box_bound(c);
     const double2* Tpp = pi_tail(c, Pp);
     const double2* Tph = pi_tail(c, Ph);
+    if (d.nonlocal && lv.kind == 0) {   // lattice solver, Vertex level: one wave per (Ω, P)
+        k_sde_L_nl<<<cdiv((size_t)c->NW2 * c->NP * WAVE, 256), 256, 0, c->run>>>(
+            c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
+            lv.chanNP, TU);
+        LAUNCHCHK(c);
+        return 0;
+    }
     if (!spp.ncls && !sph.ncls) {
         k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
                                                               lv.kind,
@@ -504,10 +513,23 @@ static SymCls sym_k2(const fdga_ctx* c, int ch) { return symcls(c, ch == PCH ? S
 
 template <int CH>
 static int launch_lk2(fdga_ctx* c) {
+    if (c->d.nonlocal) {
+        // table Γ F0 over the K2 mesh, then one GEMM per Ω against Π0 (rows of the bubble mesh)
+        const int NV2 = c->NV2, NW2 = c->NW2;
+        const size_t nx = (size_t)NV2 * NV2 * NW2;
+        k_lk2nl_table<CH><<<cdiv(nx, 256), 256, 0, c->run>>>(c->lk2x, vtx_F(c), c->d.nK2b, c->d.nK2f);
+        LAUNCHCHK(c);
+        const K2nl_A a{c->lk2x, NV2, NV2};
+        const K2nl_B b{pi_of(c, CH, true), c->NVP, c->nVP - c->d.nK2f, c->d.nK1 - c->d.nK2b, c->NWP};
+        const LK2nl_E e{c->FL[CH].K2, CH == TCH ? c->FL[ACH].K2 : nullptr, NW2, NV2, c->d.T * kSign[CH], CH == TCH};
+        if (zgemm_launch(0, NW2, c->run, NV2, c->NP, NV2, cdiv(NV2, GBM), cdiv(c->NP, GBN), nullptr, 0, a, b, e) != hipSuccess)
+            return fail(c, "BSE_L_K2 GEMM launch failed", -3);
+        return 0;
+    }
     const SymCls sym = sym_k2(c, CH);
     const size_t nw = nclass(sym, c->n2 * c->NP);
     k_gemv_lk2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->FL[CH].K2, c->FL[ACH].K2, pi_of(c, CH, true), vtx_F(c),
-                                                            c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP, c->d.nonlocal,
+                                                            c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP,
                                                             c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
@@ -536,8 +558,29 @@ static int launch_k1(fdga_ctx* c) {
     return 0;
 }
 
+// lattice BSE_K2!: X table, GEMM X W per Ω over the band, finish per (Ω, P)
+template <int CH>
+static int launch_k2_nl(fdga_ctx* c) {
+    if (!c->w_fresh && launch_w<CH>(c)) return -3;
+    const int NV2 = c->NV2, NW2 = c->NW2, K = 2 * c->wb;
+    const size_t nx = (size_t)NV2 * K * NW2 + (size_t)NV2 * 2 * NW2;
+    k_k2nl_table<CH><<<cdiv(nx, 256), 256, 0, c->run>>>(c->k2x, c->k2xf, vtx_F(c), c->d.nK2b, c->d.nK2f, c->wb);
+    LAUNCHCHK(c);
+    const K2nl_A a{c->k2x, NV2, K};
+    const K2nl_B b{c->Wc[CH], K, 0, c->d.nK1 - c->d.nK2b, c->NWP};
+    const K2nl_E e{c->k2y, NV2, NW2};
+    if (zgemm_launch(0, NW2, c->run, NV2, c->NP, K, cdiv(NV2, GBM), cdiv(c->NP, GBN), nullptr, 0, a, b, e) != hipSuccess)
+        return fail(c, "BSE_K2 GEMM launch failed", -3);
+    k_k2nl_finish<CH><<<cdiv((size_t)NW2 * c->NP * WAVE, 256), 256, 0, c->run>>>(
+        c->Fb[CH].K2, c->Fb[ACH].K2, c->k2y, c->k2xf, c->Wc[CH], c->Wtail[CH], vtx_F(c), chanF(c, c->FL[CH]),
+        chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->wb, c->NP, c->d.T * kSign[CH]);
+    LAUNCHCHK(c);
+    return 0;
+}
+
 template <int CH>
 static int launch_k2(fdga_ctx* c) {
+    if (c->d.nonlocal) return launch_k2_nl<CH>(c);
     if (!c->w_fresh && launch_w<CH>(c)) return -3;
     const SymCls sym = sym_k2(c, CH);
     const size_t nw = nclass(sym, c->n2 * c->NP);
@@ -551,6 +594,7 @@ static int launch_k2(fdga_ctx* c) {
 
 // BSE_K2! of the a and t channels in one kernel (t needs the finished a channel)
 static int launch_k2_at(fdga_ctx* c) {
+    if (c->d.nonlocal) return launch_k2_nl<ACH>(c) || launch_k2_nl<TCH>(c);
     if (!c->w_fresh && (launch_w<ACH>(c) || launch_w<TCH>(c))) return -3;
     const SymCls sym = symcls(c, SG_PH2);   // a and t share SGph[2]
     const size_t nw = nclass(sym, c->n2 * c->NP);
@@ -790,6 +834,12 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     for (double2** p : {&c->Wtail[0], &c->Wtail[1], &c->Wtail[2], &c->Ptail[0], &c->Ptail[1], &c->Ptail[2], &c->Ptail[3]})
         rc |= dalloc(c, p, c->ntail);
     for (double2** p : {&c->Lpp, &c->Lph, &c->tmpL1, &c->tmpL2}) rc |= dalloc(c, p, c->nL);
+    if (d.nonlocal) {
+        rc |= dalloc(c, &c->k2x, (size_t)c->NV2 * (2 * (size_t)c->wb) * c->NW2);
+        rc |= dalloc(c, &c->k2xf, (size_t)c->NV2 * 2 * c->NW2);
+        rc |= dalloc(c, &c->k2y, (size_t)c->NV2 * c->NW2 * c->NP);
+        rc |= dalloc(c, &c->lk2x, (size_t)c->NV2 * c->NV2 * c->NW2);
+    }
     rc |= dalloc(c, &c->sde0, c->nGel);
     rc |= dalloc(c, &c->u2C, (size_t)(4 * d.nG - 1) * c->NkG);
     size_t ntab = (size_t)(2 * d.tab_nb - 1) * (2 * d.tab_nf) * (2 * d.tab_nf);
@@ -830,7 +880,8 @@ int fdga_destroy(fdga_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (double2* p : {c->Gbare, c->G0, c->Sig0, c->G, c->Sig, c->tmpG1, c->tmpG2, c->tmpG3, c->Pi0pp, c->Pi0ph,
                        c->Pipp, c->Piph, c->Wc[0], c->Wc[1], c->Wc[2], c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C,
-                       c->stage, c->Wtail[0], c->Wtail[1], c->Wtail[2], c->Ptail[0], c->Ptail[1], c->Ptail[2], c->Ptail[3]})
+                       c->stage, c->Wtail[0], c->Wtail[1], c->Wtail[2], c->Ptail[0], c->Ptail[1], c->Ptail[2], c->Ptail[3],
+                       c->k2x, c->k2xf, c->k2y, c->lk2x})
         if (p) (void)hipFree(p);
     for (auto* t : c->tab) if (t) (void)hipFree(t);
     for (auto* p : c->cache) if (p) (void)hipFree(p);

@@ -1033,15 +1033,23 @@ __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const
 // exactly those two classes, src/channel.jl:220-339), the channel's K2(Ω, ν) does not depend on
 // ω and factors out as K2(Ω, ν) Σ_ω W(ω), and what depends on ω is the reference vertex part,
 // the channel's own K3 and the two other channels at the converted arguments.
+// the part without the channel's own K3: reference vertex and the two other channels, which the
+// s-wave lattice vertex evaluates P-averaged (src/nonlocal/vertex.jl:216-379), so it does not
+// depend on P
 template <int CH>
-__device__ __forceinline__ double2 k2d_psp(const VtxV& F, int m, int v, int w, int p) {
+__device__ __forceinline__ double2 k2x_psp(const VtxV& F, int m, int v, int w) {
     double2 x = csub(F.f0.eval_psp(m, v, w, CH), F.f0.eval_psp(m, FDGA_INF, w, CH));
-    const ChanV& g = F.g[CH];
-    if (in_b(m, g.n3b) && in_f(v, g.n3f) && in_f(w, g.n3f)) x = cadd(x, g.k3(m, v, w, p));
     int a, b, c;
     if (CH != PCH) { convert(CH, PCH, m, v, w, a, b, c); x = cadd(x, F.g[PCH].eval(a, b, c, -1)); }
     if (CH != TCH) { convert(CH, TCH, m, v, w, a, b, c); x = cadd(x, F.g[TCH].eval(a, b, c, -1)); }
     if (CH != ACH) { convert(CH, ACH, m, v, w, a, b, c); x = cadd(x, F.g[ACH].eval(a, b, c, -1)); }
+    return x;
+}
+template <int CH>
+__device__ __forceinline__ double2 k2d_psp(const VtxV& F, int m, int v, int w, int p) {
+    double2 x = k2x_psp<CH>(F, m, v, w);
+    const ChanV& g = F.g[CH];
+    if (in_b(m, g.n3b) && in_f(v, g.n3f) && in_f(w, g.n3f)) x = cadd(x, g.k3(m, v, w, p));
     return x;
 }
 // the ω-dependent summand in the channel's spin component (t: density, 2 F(t) - F(a))
@@ -1168,14 +1176,13 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
     }
 }
 
-// BSE_L_K2!: local (src/BSEa/BSEa_K2.jl:1-40): Σ over the bubble mesh of
-//   Γ(Ω, ν, ω) Π0(ω) F0(Ω, ω~, ∞);  NL (src/nonlocal/BSEa/BSEa_K2.jl:1-41): Σ over the K2 mesh of
-//   Γ(Ω, ν, ω~) Π0(ω) F0(Ω, ω, ∞).  Γ excludes F0 and the own channel, so it vanishes outside
-//   the frequency window.
+// BSE_L_K2! of the local solver (src/BSEa/BSEa_K2.jl:1-40): Σ over the bubble mesh of
+//   Γ(Ω, ν, ω) Π0(ω) F0(Ω, ω~, ∞).  Γ excludes F0 and the own channel, so it vanishes outside
+//   the frequency window.  (The lattice solver's form is k_lk2nl_table + GEMM.)
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, const double2* __restrict__ outa,
                                                   const double2* __restrict__ Pi0, VtxV F, int n2b, int n2f, int nWP,
-                                                  int nVP, int NP, int nonlocal, double scale, int NK, SymCls sym) {
+                                                  int nVP, int NP, double scale, int NK, SymCls sym) {
     constexpr int SP = ChSpin<CH>::SP;
     constexpr int FLG = FL_ALL & ~FL_F0 & ~(FL_GP << CH);
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
@@ -1186,20 +1193,12 @@ __global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, con
     const int m = iW - (n2b - 1), v = iv - n2f;
     const double2* P0 = Pi0 + (size_t)(2 * nVP) * ((size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p);
     double sr = 0.0, si = 0.0;
-    if (!nonlocal) {
-        const int Wc = window(NK, m, v);
-        const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
-        for (int q = lo + lane; q <= hi; q += WAVE) {
-            const int w = q - nVP;
-            double2 x = cmul(cmul(F.eval(m, v, w, p, CH, SP, FLG), P0[q]), F.f0.eval(m, crossing(m, w, CH), FDGA_INF, CH, SP));
-            sr += x.x; si += x.y;
-        }
-    } else {
-        for (int q = lane; q < NV2; q += WAVE) {
-            const int w = q - n2f;
-            double2 x = cmul(cmul(F.eval(m, v, crossing(m, w, CH), p, CH, SP, FLG), P0[w + nVP]), F.f0.eval(m, w, FDGA_INF, CH, SP));
-            sr += x.x; si += x.y;
-        }
+    const int Wc = window(NK, m, v);
+    const int lo = max(0, nVP - Wc), hi = min(2 * nVP - 1, nVP + Wc);
+    for (int q = lo + lane; q <= hi; q += WAVE) {
+        const int w = q - nVP;
+        double2 x = cmul(cmul(F.eval(m, v, w, p, CH, SP, FLG), P0[q]), F.f0.eval(m, crossing(m, w, CH), FDGA_INF, CH, SP));
+        sr += x.x; si += x.y;
     }
     sr = wave_sum(sr);
     si = wave_sum(si);
@@ -1211,6 +1210,135 @@ __global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, con
         double2 val = sym_apply(code & 3, raw);
         if (CH == TCH) val = cscale(cadd(val, outa[o]), 0.5);
         out[o] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// BSE_K2! and BSE_L_K2! of the s-wave lattice solver as tables and GEMMs.  In the s-wave vertex
+// the cross channels are P-averaged and F0 is local, so the ω-dependent part of the BSE_K2!
+// summand other than the channel's own K3, X(Ω, ν, ω), does not depend on P, and
+//   Σ_ω X(Ω, ν, ω) W(ω, Ω, P)
+// is, for every Ω, a (ν x ω) by (ω x P) complex GEMM over the band of the bubble mesh (beyond the
+// band X is its far value, which multiplies the tail sums of W).  The own-K3 term and the
+// rank-1 term K2(Ω, ν, P) Σ_ω W are added per (Ω, P) by k_k2nl_finish.  BSE_L_K2! in the lattice
+// solver sums Γ(Ω, ν, ω~) F0(Ω, ω, ∞) Π0(ω, Ω, P) over the K2 mesh with Γ free of F0 and of the
+// own channel, which is again a P-independent table times Π0.
+
+// X table of channel CH (t: density, 2 X(t) - X(a)), X[ν + Nν2 (k + 2 wb iΩ)] on the band
+// ω = k - wb, and its far values Xf[ν + Nν2 (side + 2 iΩ)] at ω = -wb - 1 and ω = wb
+template <int CH>
+__device__ __forceinline__ double2 k2x_term(const VtxV& F, int m, int v, int w) {
+    if (CH == TCH) return csub(cscale(k2x_psp<TCH>(F, m, v, w), 2.0), k2x_psp<ACH>(F, m, v, w));
+    return k2x_psp<CH>(F, m, v, w);
+}
+template <int CH>
+__global__ __launch_bounds__(256) void k_k2nl_table(double2* __restrict__ X, double2* __restrict__ Xf, VtxV F, int n2b,
+                                                    int n2f, int wb) {
+    const int NV2 = 2 * n2f, NW2 = 2 * n2b - 1;
+    const size_t nx = (size_t)NV2 * (2 * wb) * NW2;
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e < nx) {
+        const int iv = (int)(e % NV2), k = (int)((e / NV2) % (2 * wb)), iW = (int)(e / ((size_t)NV2 * 2 * wb));
+        X[e] = k2x_term<CH>(F, iW - (n2b - 1), iv - n2f, k - wb);
+    } else if (e < nx + (size_t)NV2 * 2 * NW2) {
+        const size_t f = e - nx;
+        const int iv = (int)(f % NV2), side = (int)((f / NV2) % 2), iW = (int)(f / (2 * (size_t)NV2));
+        Xf[f] = k2x_term<CH>(F, iW - (n2b - 1), iv - n2f, side ? wb : -wb - 1);
+    }
+}
+
+// GEMM operands: batch b = iΩ of the K2 mesh
+struct K2nl_A {   // X(ν, ω) of one Ω
+    const double2* X; int NV, K;
+    __device__ K2nl_A bind(int b) const { K2nl_A r = *this; r.X += (size_t)NV * K * b; return r; }
+    __device__ double2 operator()(int i, int k) const { return X[(size_t)i + (size_t)NV * k]; }
+};
+struct K2nl_B {   // W(ω, P) (or Π0) of one Ω: rows of `ld` entries, element `k0 + k` of row (Ω, P)
+    const double2* W; int ld, k0, rowoff, nrowP;
+    __device__ K2nl_B bind(int b) const { K2nl_B r = *this; r.W += (size_t)ld * (b + rowoff) + k0; return r; }
+    __device__ double2 operator()(int k, int j) const { return W[(size_t)k + (size_t)ld * nrowP * j]; }
+};
+struct K2nl_E {   // Y[ν + Nν2 (iΩ + NΩ2 P)]
+    double2* Y; int NV, NW;
+    __device__ K2nl_E bind(int b) const { K2nl_E r = *this; r.Y += (size_t)NV * b; return r; }
+    __device__ void operator()(int i, int j, double2 v) const { Y[(size_t)i + (size_t)NV * NW * j] = v; }
+};
+// BSE_L_K2! epilogue: out(Ω, ν, P) = scale Y (t: (scale Y + out_a) / 2)
+struct LK2nl_E {
+    double2* out; const double2* outa; int NW, NV; double scale; int t;
+    __device__ LK2nl_E bind(int b) const { LK2nl_E r = *this; r.out += b; if (r.outa) r.outa += b; return r; }
+    __device__ void operator()(int i, int j, double2 v) const {
+        const size_t o = (size_t)NW * ((size_t)i + (size_t)NV * j);
+        double2 val = cscale(v, scale);
+        if (t) val = cscale(cadd(val, outa[o]), 0.5);
+        out[o] = val;
+    }
+};
+
+// BSE_L_K2! table of the lattice solver: XL[ν + Nν2 (q + Nν2 iΩ)] = Γ(Ω, ν, ω~) F0(Ω, ω, ∞) on the
+// K2 mesh ω = q - n2f (src/nonlocal/BSEa/BSEa_K2.jl:1-41)
+template <int CH>
+__global__ __launch_bounds__(256) void k_lk2nl_table(double2* __restrict__ X, VtxV F, int n2b, int n2f) {
+    constexpr int SP = ChSpin<CH>::SP;
+    constexpr int FLG = FL_ALL & ~FL_F0 & ~(FL_GP << CH);
+    const int NV2 = 2 * n2f, NW2 = 2 * n2b - 1;
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= (size_t)NV2 * NV2 * NW2) return;
+    const int iv = (int)(e % NV2), q = (int)((e / NV2) % NV2), iW = (int)(e / ((size_t)NV2 * NV2));
+    const int m = iW - (n2b - 1), v = iv - n2f, w = q - n2f;
+    X[e] = cmul(F.eval(m, v, crossing(m, w, CH), 0, CH, SP, FLG), F.f0.eval(m, w, FDGA_INF, CH, SP));
+}
+
+// BSE_K2! finish of the lattice solver, one wave per (Ω, P):
+//   K2 = T s [Y + Xf_L Wt_L + Xf_R Wt_R + K2r(Ω, ν, P) (Σ_band W + Wt_L + Wt_R) + Σ_ω K3r(Ω, ν, ω, P) W(ω)]
+//        + FL.K2 (t: (... + 2 FL.t.K2 - FL.a.K2 + K2a) / 2)
+// with Wt_L, Wt_R the sums of W beyond the band and K3r the own K3 (t: 2 K3t - K3a).
+template <int CH>
+__global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, const double2* __restrict__ K2a,
+                                                     const double2* __restrict__ Y, const double2* __restrict__ Xf,
+                                                     const double2* __restrict__ W, const double2* __restrict__ Wtail,
+                                                     VtxV F, ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int wb,
+                                                     int NP, double scale) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
+    if (wid >= NW2 * NP) return;
+    const int iW = wid % NW2, p = wid / NW2;
+    const int m = iW - (n2b - 1);
+    const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
+    const double2* Wr = W + (size_t)(2 * wb) * row + wb;   // Wr[ω], ω in [-wb, wb)
+    double sr = 0.0, si = 0.0;
+    for (int k = lane; k < 2 * wb; k += WAVE) { const double2 x = Wr[k - wb]; sr += x.x; si += x.y; }
+    sr = wave_sum(sr); si = wave_sum(si);
+    const double2* Tr = Wtail + (size_t)(2 * (wb + 1)) * row;
+    const double2 wl = Tr[0], wr = Tr[(wb + 1) + wb];
+    const double2 wtot = cadd(cmk(sr, si), cadd(wl, wr));
+    const ChanV& g = F.g[CH];
+    const bool k3m = in_b(m, g.n3b);
+    for (int iv = lane; iv < NV2; iv += WAVE) {
+        const int v = iv - n2f;
+        double2 acc = Y[(size_t)iv + (size_t)NV2 * (iW + (size_t)NW2 * p)];
+        const double2* xf = Xf + (size_t)iv + (size_t)NV2 * 2 * iW;
+        acc = cadd(acc, cadd(cmul(xf[0], wl), cmul(xf[NV2], wr)));
+        acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), wtot));
+        if (k3m && in_f(v, g.n3f)) {
+            double2 k3s = czero();
+            for (int w = -g.n3f; w < g.n3f; ++w) {
+                double2 k3 = g.k3(m, v, w, p);
+                if (CH == TCH) k3 = csub(cscale(k3, 2.0), F.g[ACH].k3(m, v, w, p));
+                k3s = cadd(k3s, cmul(k3, Wr[w]));
+            }
+            acc = cadd(acc, k3s);
+        }
+        const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
+        double2 val = cscale(acc, scale);
+        if (CH == TCH) {
+            val = cadd(val, cscale(Lr.K2[o], 2.0));
+            val = cadd(val, cscale(La.K2[o], -1.0));
+            val = cscale(cadd(val, K2a[o]), 0.5);
+        } else {
+            val = cadd(val, Lr.K2[o]);
+        }
+        K2[o] = val;
     }
 }
 
@@ -1277,6 +1405,86 @@ __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double
         const int code = sym.member(wid, k), o = code >> 2;
         if (DP) Lpp[o] = sym_apply(code & 3, rawp);
         if (DH) Lph[o] = sym_apply(code & 3, rawh);
+    }
+}
+
+// SDE L-vertices of the s-wave lattice solver at a Vertex level (kind 0), one wave per (Ω, P).
+// The channel evaluation (K1 + K2 + K2' + K3 by box, src/nonlocal/channel.jl:90-226) splits the
+// sums over the bubble mesh into a rank-1 term, a ν-independent K2 sum and a K3 product:
+//   Lpp = T U [(K1p + K2p(ν)) ΣΠpp + Σ_ω Πpp(ω) K2p(Ω-ω-1) + Σ_ω Πpp(ω) K3p(Ω, Ω-ω-1, ν)]
+//   Lph = T U Σ_{r=a,t} [(K1r + K2r(ν)) ΣΠph + Σ_ω Πph(ω) K2r(ω) + Σ_ω Πph(ω) K3r(Ω, ν, ω)]
+// with the channels at momentum P (chanNP > 1) or the local vertex (chanNP = 1).  ΣΠ over the
+// whole bubble mesh is the band sum plus the tail sums at the band edges.
+// the classes of a level's channel at its own boxes (src/channel.jl:220-285)
+struct ChanBox {
+    const ChanV& g;
+    int pc;
+    __device__ double2 k1(int m) const { return in_b(m, g.n1) ? g.k1(m, pc) : czero(); }
+    __device__ double2 k2(int m, int n) const { return in_b(m, g.n2b) && in_f(n, g.n2f) ? g.k2(m, n, pc) : czero(); }
+    __device__ double2 k3(int m, int n, int np) const {
+        return in_b(m, g.n3b) && in_f(n, g.n3f) && in_f(np, g.n3f) ? g.k3(m, n, np, pc) : czero();
+    }
+};
+__global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, double2* __restrict__ Lph,
+                                                  const double2* __restrict__ Pipp, const double2* __restrict__ Piph,
+                                                  const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
+                                                  ChanV gp, ChanV gt, ChanV ga, int n2b, int n2f, int nWP, int nVP, int wb,
+                                                  int NP, int chanNP, double2 TU) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
+    if (wid >= NW2 * NP) return;
+    const int iW = wid % NW2, p = wid / NW2;
+    const int m = iW - (n2b - 1);
+    const int pc = chanNP > 1 ? p : 0;
+    const ChanBox P{gp, pc}, A{ga, pc}, T{gt, pc};
+    const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
+    const double2* pp = Pipp + (size_t)(2 * nVP) * row + nVP;   // pp[ω], ω in [-nVP, nVP)
+    const double2* ph = Piph + (size_t)(2 * nVP) * row + nVP;
+    auto inmesh = [&](int w) { return w >= -nVP && w < nVP; };
+    // totals over the bubble mesh (band + tails) and the ν-independent K2 sums
+    double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    for (int w = -wb + lane; w < wb; w += WAVE) {
+        const double2 x = pp[w], y = ph[w];
+        a0 += x.x; a1 += x.y; b0 += y.x; b1 += y.y;
+    }
+    double c0 = 0, c1 = 0, d0 = 0, d1 = 0;
+    for (int n = -gp.n2f + lane; n < gp.n2f; n += WAVE) {
+        const int wp = m - n - 1;   // Lpp: K2p(Ω, n) at n = Ω - ω - 1
+        if (inmesh(wp)) { const double2 x = cmul(P.k2(m, n), pp[wp]); c0 += x.x; c1 += x.y; }
+    }
+    for (int n = -max(ga.n2f, gt.n2f) + lane; n < max(ga.n2f, gt.n2f); n += WAVE) {
+        if (inmesh(n)) { const double2 y = cmul(cadd(A.k2(m, n), T.k2(m, n)), ph[n]); d0 += y.x; d1 += y.y; }
+    }
+    a0 = wave_sum(a0); a1 = wave_sum(a1); b0 = wave_sum(b0); b1 = wave_sum(b1);
+    c0 = wave_sum(c0); c1 = wave_sum(c1); d0 = wave_sum(d0); d1 = wave_sum(d1);
+    const double2* tp = Tpp + (size_t)(2 * (wb + 1)) * row;
+    const double2* th = Tph + (size_t)(2 * (wb + 1)) * row;
+    const double2 spp = cadd(cmk(a0, a1), cadd(tp[0], tp[2 * wb + 1]));
+    const double2 sph = cadd(cmk(b0, b1), cadd(th[0], th[2 * wb + 1]));
+    const double2 k2p = cmk(c0, c1), k2h = cmk(d0, d1);
+    const double2 k1p = P.k1(m), k1h = cadd(A.k1(m), T.k1(m));
+    const int n3p = gp.n3f, n3h = max(ga.n3f, gt.n3f);
+    for (int iv = lane; iv < NV2; iv += WAVE) {
+        const int v = iv - n2f;
+        double2 xp = cadd(cmul(cadd(k1p, P.k2(m, v)), spp), k2p);
+        double2 xh = cadd(cmul(cadd(k1h, cadd(A.k2(m, v), T.k2(m, v))), sph), k2h);
+        if (in_b(m, gp.n3b) && in_f(v, n3p)) {
+            double2 sp = czero();
+            for (int n = -n3p; n < n3p; ++n) {
+                const int wp = m - n - 1;
+                if (inmesh(wp)) sp = cadd(sp, cmul(P.k3(m, n, v), pp[wp]));
+            }
+            xp = cadd(xp, sp);
+        }
+        if ((in_b(m, ga.n3b) || in_b(m, gt.n3b)) && in_f(v, n3h)) {
+            double2 sh = czero();
+            for (int n = -n3h; n < n3h; ++n)
+                if (inmesh(n)) sh = cadd(sh, cmul(cadd(A.k3(m, v, n), T.k3(m, v, n)), ph[n]));
+            xh = cadd(xh, sh);
+        }
+        const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
+        Lpp[o] = cmul(xp, TU);
+        Lph[o] = cmul(xh, TU);
     }
 }
 
