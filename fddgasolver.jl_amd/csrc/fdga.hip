@@ -76,7 +76,8 @@ struct fdga_ctx {
     double2 *tmpG1 = nullptr, *tmpG2 = nullptr, *tmpG3 = nullptr;   // [NG, NkG] scratch
     double2 *tmpL1 = nullptr, *tmpL2 = nullptr;                     // [NW2*NV2, NP] scratch
     double2 *u2C = nullptr;                                         // [4nG-1, NkG]
-    double *scal = nullptr;                                         // device scalars
+    double *scal = nullptr;                                         // device scalars (+ occupation partial sums)
+    BubbleFold* folds = nullptr;                                    // lattice bubble: R -> R' folding table
     double2 *stage = nullptr;                                       // staging for layout changes
     size_t stage_n = 0;
     SymDev sg[SG_N];                                                // installed symmetry groups
@@ -247,9 +248,9 @@ void free_chan(DevChan& dc) {
 int refresh_avg(fdga_ctx* c) {
     if (c->NP == 1) return 0;
     for (int r = 0; r < 3; ++r) {
-        k_pavg<<<cdiv(c->n1, 256), 256, 0, c->run>>>(c->F[r].A1, c->F[r].K1, c->n1, c->NP);
-        k_pavg<<<cdiv(c->n2, 256), 256, 0, c->run>>>(c->F[r].A2, c->F[r].K2, c->n2, c->NP);
-        k_pavg<<<cdiv(c->n3, 256), 256, 0, c->run>>>(c->F[r].A3, c->F[r].K3, c->n3, c->NP);
+        k_pavg<<<cdiv(c->n1, 16), 256, 0, c->run>>>(c->F[r].A1, c->F[r].K1, c->n1, c->NP);
+        k_pavg<<<cdiv(c->n2, 16), 256, 0, c->run>>>(c->F[r].A2, c->F[r].K2, c->n2, c->NP);
+        k_pavg<<<cdiv(c->n3, 16), 256, 0, c->run>>>(c->F[r].A3, c->F[r].K3, c->n3, c->NP);
     }
     LAUNCHCHK(c);
     return 0;
@@ -279,6 +280,35 @@ int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale
     k_dft2_rows<<<cdiv(nrows, RB), 256, smem, c->run>>>(data, nrows, L, RB, sign, scale);
     LAUNCHCHK(c);
     return 0;
+}
+
+// the R in [-L/2, L/2]^2 folding onto each R' = mod(R, L) (src/nonlocal/bubble.jl:118-134)
+std::vector<BubbleFold> bubble_folds(int L, int LG) {
+    std::vector<BubbleFold> f((size_t)L * L);
+    const int h = L / 2;
+    auto md = [](int a, int n) { return ((a % n) + n) % n; };
+    for (int r2p = 0; r2p < L; ++r2p)
+        for (int r1p = 0; r1p < L; ++r1p) {
+            BubbleFold& b = f[(size_t)r1p + (size_t)L * r2p];
+            b = BubbleFold{};
+            for (int R2 = -h; R2 <= h; ++R2) {
+                if (md(R2, L) != r2p) continue;
+                for (int R1 = -h; R1 <= h; ++R1) {
+                    if (md(R1, L) != r1p) continue;
+                    double w = 1.0;
+                    if (LG % 2 == 0) {
+                        if (std::abs(R1) == LG / 2) w *= 0.5;
+                        if (std::abs(R2) == LG / 2) w *= 0.5;
+                    }
+                    const int c = b.n++;
+                    b.kp[c] = md(R1, LG) + LG * md(R2, LG);
+                    b.km[c] = md(-R1, LG) + LG * md(-R2, LG);
+                    b.w[c] = w;
+                    if (R1 == 0 && R2 == 0) b.origin |= 1 << c;
+                }
+            }
+        }
+    return f;
 }
 
 // band tail sums of one array on the bubble mesh (rows of 2 nVP entries)
@@ -312,11 +342,11 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     if (L == 16) {
         constexpr int NB = 16;
         k_bubble_nl<16, NB><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)NB * 256 * sizeof(double2), c->run>>>(
-            pp, ph, c->tmpG1, d.nG, d.LG, L, d.nK1, c->nVP, d.T, NB);
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB);
     } else {
         const int NB = std::max(1, std::min(16, (65536 / (int)sizeof(double2) - L) / (L * L)));
         k_bubble_nl<0, 0><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)(NB * L * L + L) * sizeof(double2), c->run>>>(
-            pp, ph, c->tmpG1, d.nG, d.LG, L, d.nK1, c->nVP, d.T, NB);
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB);
     }
     LAUNCHCHK(c);
     return tail_sums(c, pp, pi_tail(c, pp)) || tail_sums(c, ph, pi_tail(c, ph));
@@ -419,7 +449,8 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
 }
 
 int sde_hartree(fdga_ctx* c, double2* S, const double2* G, double sign) {
-    k_occupation<<<1, 256, 0, c->run>>>(c->scal, G, c->nGel, c->d.T, (double)c->NkG);
+    k_occupation_part<<<OCC_BLOCKS, 256, 0, c->run>>>(c->scal + 8, G, c->nGel);
+    k_occupation<<<1, WAVE, 0, c->run>>>(c->scal, c->scal + 8, c->d.T, (double)c->NkG);
     k_hartree<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(S, c->nGel, c->scal, c->U, sign);
     LAUNCHCHK(c);
     return 0;
@@ -538,11 +569,12 @@ static int launch_lk2(fdga_ctx* c) {
 // K1/K2 weights of channel CH on the band into Wc[CH], their far sums into Wtail[CH]
 template <int CH>
 static int launch_w(fdga_ctx* c) {
-    dim3 gw(cdiv(2 * c->wb + 2, 256), c->NWP, c->NP);
+    const size_t nrows = (size_t)c->NWP * c->NP;
     double2* P = pi_of(c, CH, false);
     double2* P0 = pi_of(c, CH, true);
-    k_bse_w<CH><<<gw, 256, 0, c->run>>>(c->Wc[CH], c->Wtail[CH], P, P0, pi_tail(c, P), pi_tail(c, P0), vtx_FL(c), f0_view(c),
-                                       c->d.nK1, c->nVP, c->wb, box_bound(c));
+    k_bse_w<CH><<<cdiv(nrows * (2 * c->wb + 2), 256), 256, 0, c->run>>>(c->Wc[CH], c->Wtail[CH], P, P0, pi_tail(c, P),
+                                                                       pi_tail(c, P0), vtx_FL(c), f0_view(c), c->d.nK1,
+                                                                       c->nVP, c->wb, box_bound(c), nrows);
     LAUNCHCHK(c);
     return 0;
 }
@@ -859,7 +891,13 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
         rc |= dalloc(c, &c->R[r], c->n3 * c->NP);
         rc |= dalloc(c, &c->csum[r], (size_t)c->NV3 * c->NW3 * c->NP);
     }
-    if (hipMalloc((void**)&c->scal, 8 * sizeof(double)) != hipSuccess) rc = -2;
+    if (hipMalloc((void**)&c->scal, (8 + OCC_BLOCKS) * sizeof(double)) != hipSuccess) rc = -2;
+    if (d.nonlocal && !rc) {
+        const std::vector<BubbleFold> f = bubble_folds(d.L, d.LG);
+        if (hipMalloc((void**)&c->folds, f.size() * sizeof(BubbleFold)) != hipSuccess ||
+            hipMemcpy(c->folds, f.data(), f.size() * sizeof(BubbleFold), hipMemcpyHostToDevice) != hipSuccess)
+            rc = -2;
+    }
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) rc = -2;
     if (rc) {
@@ -894,6 +932,7 @@ int fdga_destroy(fdga_ctx* c) {
         if (c->R0[r].K1) free_chan(c->R0[r]);
     }
     if (c->scal) (void)hipFree(c->scal);
+    if (c->folds) (void)hipFree(c->folds);
     graphs_invalidate(c);
     for (auto& g : c->sg) free_sym(g);
     free_blists(c);

@@ -223,34 +223,29 @@ struct BubbleDft<16> {
     }
 };
 
+// The R in [-L/2, L/2]^2 that fold onto R' = mod(R, L) (up to two per axis), with their
+// k-mesh indices on the LG mesh (+R and -R), weights (1/2 per axis at |R_i| = LG/2 for even LG)
+// and whether R = 0; built once per solver on the host (bubble_folds in the library).
+struct BubbleFold {
+    int n;          // number of R
+    int kp[4], km[4];
+    int origin;     // bit c: R of slot c is the origin
+    double w[4];
+};
 // Π_R(ω, Ω, R') for one R' (pp: which = 0, ph: 1; see k_bubble_nl)
-__device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, int nG, int LG, int L, double T, int m, int n,
-                                            int r1p, int r2p, int which) {
-    const int NG = 2 * nG, h = L / 2;
-    int R1s[2], R2s[2], n1 = 0, n2 = 0;
-    if (r1p <= h) R1s[n1++] = r1p;
-    if (r1p - L >= -h && r1p - L != r1p) R1s[n1++] = r1p - L;
-    if (r2p <= h) R2s[n2++] = r2p;
-    if (r2p - L >= -h && r2p - L != r2p) R2s[n2++] = r2p - L;
+__device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, const BubbleFold& f, int nG, double T, int m,
+                                            int n, int which) {
+    const int NG = 2 * nG;
     double2 acc = czero();
     const int n2nd = which == 0 ? bmf(m, n) : bpf(m, n);
-    for (int b2 = 0; b2 < n2; ++b2)
-        for (int b1 = 0; b1 < n1; ++b1) {
-            const int R1 = R1s[b1], R2 = R2s[b2];
-            double w = 1.0;
-            if (LG % 2 == 0) {
-                if (abs(R1) == LG / 2) w *= 0.5;
-                if (abs(R2) == LG / 2) w *= 0.5;
-            }
-            const int kp = (((R1 % LG) + LG) % LG) + LG * (((R2 % LG) + LG) % LG);
-            const int km = (((-R1 % LG) + LG) % LG) + LG * (((-R2 % LG) + LG) % LG);
-            const bool origin = (R1 == 0 && R2 == 0);
-            auto gat = [&](int k, int kk) -> double2 {
-                if (in_f(k, nG)) return GR[(size_t)(k + nG) + (size_t)NG * kk];
-                return origin ? cmk(1.0 / (M_PI * T * (2.0 * k + 1.0)), 0.0) : czero();
-            };
-            acc = cadd(acc, cscale(cmul(gat(n2nd, kp), gat(n, which == 0 ? kp : km)), w));
-        }
+    for (int c = 0; c < f.n; ++c) {
+        const bool origin = (f.origin >> c) & 1;
+        auto gat = [&](int k, int kk) -> double2 {
+            if (in_f(k, nG)) return GR[(size_t)(k + nG) + (size_t)NG * kk];
+            return origin ? cmk(1.0 / (M_PI * T * (2.0 * k + 1.0)), 0.0) : czero();
+        };
+        acc = cadd(acc, cscale(cmul(gat(n2nd, f.kp[c]), gat(n, which == 0 ? f.kp[c] : f.km[c])), f.w[c]));
+    }
     return acc;
 }
 
@@ -258,8 +253,8 @@ __device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, int 
 // L <= 32 (direct DFT).  Block: 256 threads, NB * L <= 256 needed only for full use of the block.
 template <int LT, int NBT>
 __global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, double2* __restrict__ ph,
-                                                   const double2* __restrict__ GR, int nG, int LG, int Lr, int nW, int nV,
-                                                   double T, int NBr) {
+                                                   const double2* __restrict__ GR, const BubbleFold* __restrict__ folds,
+                                                   int nG, int Lr, int nW, int nV, double T, int NBr) {
     extern __shared__ double2 sm[];   // NB * L * L (+ L twiddles unless L = 16)
     const int L = LT ? LT : Lr, LL = L * L, NB = NBT ? NBT : NBr;
     double2* s = sm;
@@ -278,7 +273,7 @@ __global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, dou
         // products
         for (int e = tid; e < NB * LL; e += blockDim.x) {
             const int wl = e % NB, R = e / NB;
-            s[e] = wl < nb ? bubble_R(GR, nG, LG, L, T, m, iw0 + wl - nV, R % L, R / L, which) : czero();
+            s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : czero();
         }
         __syncthreads();
         // along R1' (thread: ω, R2'), then along R2' (thread: ω, q1) with the write-out
@@ -312,13 +307,24 @@ __global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, dou
     }
 }
 
-// s-wave average over P of one array of `n` elements per P
-__global__ void k_pavg(double2* __restrict__ A, const double2* __restrict__ K, size_t n, int NP) {
-    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// s-wave average over P of one array of `n` elements per P: a block takes 16 consecutive
+// elements, its 16 thread groups sum every 16th momentum, and the groups' sums are added in a
+// fixed order
+__global__ __launch_bounds__(256) void k_pavg(double2* __restrict__ A, const double2* __restrict__ K, size_t n, int NP) {
+    __shared__ double2 red[16][17];
+    const int el = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const size_t i = blockIdx.x * (size_t)16 + el;
     double2 s = czero();
-    for (int p = 0; p < NP; ++p) s = cadd(s, K[i + n * p]);
-    A[i] = cscale(s, 1.0 / NP);
+    if (i < n)
+        for (int p = grp; p < NP; p += 16) s = cadd(s, K[i + n * p]);
+    red[grp][el] = s;
+    __syncthreads();
+    if (grp == 0 && i < n) {
+        double2 t = czero();
+#pragma unroll
+        for (int g = 0; g < 16; ++g) t = cadd(t, red[g][el]);
+        A[i] = cscale(t, 1.0 / NP);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -869,16 +875,20 @@ struct Dense_C {
 // FL(Ω, ω~, ∞) no longer depend on ω (|ω~| lies beyond every box, so only the bare vertex and the
 // channel's K1(Ω) remain), so Σ_far W = F0_far Σ_far (Π - Π0) + FL_far Σ_far Π.  That holds for the
 // tails the gathers use (lo <= nVP - NK - |Ω| - 2); entries nearer ω = 0 are not used.
+// One thread per band entry and per tail entry of a row (2 wb + 2 per row, rows (Ω, P) in order).
 template <int CH>
 __global__ __launch_bounds__(256) void k_bse_w(double2* __restrict__ W, double2* __restrict__ Wt,
                                                const double2* __restrict__ Pi, const double2* __restrict__ Pi0,
                                                const double2* __restrict__ Pt, const double2* __restrict__ P0t,
-                                               VtxV FL, F0V f0, int nWP, int nVP, int wb, int NK) {
+                                               VtxV FL, F0V f0, int nWP, int nVP, int wb, int NK, size_t nrows) {
     constexpr int ch = CH, sp = ChSpin<CH>::SP;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iW = blockIdx.y, p = blockIdx.z;
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const int per = 2 * wb + 2;
+    const size_t row = e / per;
+    if (row >= nrows) return;
+    const int t = (int)(e - row * per);
+    const int iW = (int)(row % (size_t)(2 * nWP - 1)), p = (int)(row / (size_t)(2 * nWP - 1));
     const int m = iW - (nWP - 1);
-    const size_t row = (size_t)iW + (size_t)(2 * nWP - 1) * p;
     if (t < 2 * wb) {
         const int iw = nVP - wb + t, w = iw - nVP;
         const size_t o = (size_t)iw + (size_t)(2 * nVP) * row;
@@ -888,7 +898,7 @@ __global__ __launch_bounds__(256) void k_bse_w(double2* __restrict__ W, double2*
         const double2 pi = Pi[o];
         W[(size_t)t + (size_t)(2 * wb) * row] = cadd(cmul(csub(pi, Pi0[o]), f0r), cmul(pi, flr));
     }
-    if (t < 2 * (wb + 1)) {
+    {
         const int wf = t > wb ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
         const int wc = crossing(m, wf, ch);
         const double2 f0r = f0.eval(m, wc, FDGA_INF, ch, sp);
@@ -1614,17 +1624,23 @@ __global__ __launch_bounds__(256) void k_u2_S(double2* __restrict__ S, const dou
 }
 
 // occupation n = 1/2 + T Im Σ G / Nk (src/dyson.jl:33-41): single-workgroup reduction
-__global__ void k_occupation(double* __restrict__ out, const double2* __restrict__ G, size_t n, double T, double nk) {
-    __shared__ double red[256];
+// partial sums of Im G over OCC_BLOCKS blocks, then one block adds them in a fixed order
+constexpr int OCC_BLOCKS = 128;
+__global__ __launch_bounds__(256) void k_occupation_part(double* __restrict__ part, const double2* __restrict__ G, size_t n) {
+    __shared__ double red[256 / WAVE];
     double s = 0.0;
-    for (size_t i = threadIdx.x; i < n; i += blockDim.x) s += G[i].y;
-    red[threadIdx.x] = s;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)256 * OCC_BLOCKS) s += G[i].y;
+    s = wave_sum(s);
+    if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = 0.5 + red[0] * T / nk;
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ __launch_bounds__(WAVE) void k_occupation(double* __restrict__ out, const double* __restrict__ part, double T,
+                                                     double nk) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < OCC_BLOCKS; b += WAVE) s += part[b];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) out[0] = 0.5 + s * T / nk;
 }
 // Σ += sign (n - 1/2) U i
 __global__ void k_hartree(double2* __restrict__ S, size_t n, const double* __restrict__ occ, double2 U, double sign) {
