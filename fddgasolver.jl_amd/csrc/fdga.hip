@@ -702,11 +702,13 @@ static bool k3_sym(const fdga_ctx* c, const int* chans, int n, bool left) {
 // BSE_L_K3! for `n` channels in one launch.  A single t channel without a symmetry group is
 // post-processed in the epilogue (its a channel is complete); otherwise t is post-processed
 // after the launch (and after the symmetry fill).
-static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
-    const int NV = c->NV3, NW = c->NW3;
-    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
-    const bool sym = k3_sym(c, chans, n, true);
-    const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
+struct LK3Set {
+    LK3_Aop a;
+    LK3_Bop b;
+    LK3_Ep e;
+};
+// operand loaders and epilogue of a BSE_L_K3! launch (batch = (slot NW3 + iΩ) NP... see K3Geom)
+static LK3Set lk3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) {
     LK3Chan cc[3] = {};
     for (int s = 0; s < n; ++s) {
         const int ch = chans[s];
@@ -719,27 +721,39 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
         cc[s].tpost = (ch == TCH && t_epi);
     }
     const K3Geom g = k3geom(c);
-    LK3_Aop a{g, cc[0], cc[1], cc[2]};
-    LK3_Bop b{g, cc[0], cc[1], cc[2]};
-    LK3_Ep e{g, cc[0], cc[1], cc[2]};
+    return LK3Set{LK3_Aop{g, cc[0], cc[1], cc[2]}, LK3_Bop{g, cc[0], cc[1], cc[2]}, LK3_Ep{g, cc[0], cc[1], cc[2]}};
+}
+static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
+    const int NV = c->NV3, NW = c->NW3;
+    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
+    const bool sym = k3_sym(c, chans, n, true);
+    const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
+    const LK3Set ls = lk3_set(c, chans, n, sym, t_epi);
     size_t nl = 0;
     const int* bl = k3_blocks(c, chans, n, true, nl);
     kernel_begin(c);
-    if (zgemm_launch(c->d.precision, n * NW * c->NP, c->run, NV, NV, NV, tm, tn, bl, nl, a, b, e) != hipSuccess)
+    if (zgemm_launch(c->d.precision, n * NW * c->NP, c->run, NV, NV, NV, tm, tn, bl, nl, ls.a, ls.b, ls.e) != hipSuccess)
         return fail(c, "BSE_L_K3 GEMM launch failed", -3);
     kernel_end(c, 9);
     return k3_finish(c, c->FL, chans, n, true, sym, t_epi);
 }
 
-// BSE_K3! for `n` channels: right operands and column sums, then one GEMM launch
-static int launch_k3(fdga_ctx* c, const int* chans, int n) {
-    const int NV = c->NV3, NW = c->NW3;
-    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
-    const bool sym = k3_sym(c, chans, n, false);
-    const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
-    const K3Geom g = k3geom(c);
-    K3RChan rc[3] = {};
-    K3Chan kc[3] = {};
+// BSE_K3! operands: right-operand builders (k_k3_R) and the GEMM's loaders and epilogue
+struct K3Set {
+    K3Geom g;
+    K3RChan rc[3];
+    K3Chan kc[3];
+    double T;
+    K3_Bop b() const { return K3_Bop{g, T, kc[0], kc[1], kc[2]}; }
+    K3_Ep e() const { return K3_Ep{g, T, kc[0], kc[1], kc[2]}; }
+    template <bool FULL> K3_AopT<FULL> a() const { return K3_AopT<FULL>{g, T, kc[0], kc[1], kc[2]}; }
+};
+static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) {
+    K3Set ks{};
+    ks.g = k3geom(c);
+    ks.T = c->d.T;
+    K3RChan* rc = ks.rc;
+    K3Chan* kc = ks.kc;
     for (int s = 0; s < n; ++s) {
         const int ch = chans[s];
         rc[s].ch = ch;
@@ -768,19 +782,31 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
         kc[s].outa = c->Fb[ACH].K3;
         kc[s].tpost = (ch == TCH && t_epi);
     }
-    const size_t ncol = (size_t)NV * NW * c->NP * n;
-    k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->run>>>(g, rc[0], rc[1], rc[2], n);
+    return ks;
+}
+// the right operands R and their column sums of a BSE_K3! launch over `n` channels
+static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n) {
+    const size_t ncol = (size_t)c->NV3 * c->NW3 * c->NP * n;
+    k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
     LAUNCHCHK(c);
-    K3_Bop b{g, c->d.T, kc[0], kc[1], kc[2]};
-    K3_Ep e{g, c->d.T, kc[0], kc[1], kc[2]};
+    return 0;
+}
+// BSE_K3! for `n` channels: right operands and column sums, then one GEMM launch
+static int launch_k3(fdga_ctx* c, const int* chans, int n) {
+    const int NV = c->NV3, NW = c->NW3;
+    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
+    const bool sym = k3_sym(c, chans, n, false);
+    const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
+    const K3Set ks = k3_set(c, chans, n, sym, t_epi);
+    if (k3_rhs(c, ks, n)) return -3;
+    const K3_Bop b = ks.b();
+    const K3_Ep e = ks.e();
     size_t nl = 0;
     const int* bl = k3_blocks(c, chans, n, false, nl);
     kernel_begin(c);
     hipError_t ge;
-    if (c->NP == 1) ge = zgemm_launch(c->d.precision, n * NW, c->run, NV, NV, NV, tm, tn, bl, nl,
-                                      K3_AopT<true>{g, c->d.T, kc[0], kc[1], kc[2]}, b, e);
-    else ge = zgemm_launch(c->d.precision, n * NW * c->NP, c->run, NV, NV, NV, tm, tn, bl, nl,
-                           K3_AopT<false>{g, c->d.T, kc[0], kc[1], kc[2]}, b, e);
+    if (c->NP == 1) ge = zgemm_launch(c->d.precision, n * NW, c->run, NV, NV, NV, tm, tn, bl, nl, ks.a<true>(), b, e);
+    else ge = zgemm_launch(c->d.precision, n * NW * c->NP, c->run, NV, NV, NV, tm, tn, bl, nl, ks.a<false>(), b, e);
     if (ge != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
     return k3_finish(c, c->Fb, chans, n, false, sym, t_epi);
@@ -1509,8 +1535,9 @@ int fdga_stage_ms(const fdga_ctx* c, int32_t stage, double* ms) {
 }
 
 // C = A B for dense column-major complex matrices through the BSE GEMM engine
-int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, const double* A, const double* B, double* C) {
-    if (M < 1 || N < 1 || K < 1 || !A || !B || !C) return -1;
+static int selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, int32_t tile, int64_t grid, const double* A,
+                          const double* B, double* C) {
+    if (M < 1 || N < 1 || K < 1 || !A || !B || !C || tile >= TILE_N || grid < 0 || grid % 8) return -1;
     double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
     if (hipMalloc((void**)&dA, na * sizeof(double2)) != hipSuccess) return -2;
@@ -1520,13 +1547,30 @@ int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, cons
     if (hipMemcpy(dA, A, na * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dB, B, nb * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
     if (!rc) {
-        const int tm = cdiv(M, GBM), tn = cdiv(N, GBN);
-        if (zgemm_launch(precision, 1, 0, M, N, K, tm, tn, nullptr, 0, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M}) != hipSuccess ||
-            hipDeviceSynchronize() != hipSuccess) rc = -3;
+        hipError_t e;
+        if (tile < 0) {
+            const int tm = cdiv(M, GBM), tn = cdiv(N, GBN);
+            e = zgemm_launch(precision, 1, 0, M, N, K, tm, tn, nullptr, 0, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M});
+        } else {
+            const int tm = cdiv(M, tile_bm(tile)), tn = cdiv(N, tile_bn(tile));
+            const GemmSchedOne sc{M, N, K, tm, tn, 1, 8 * tm * tn, nullptr};
+            e = zgemm_run(precision, tile, grid ? (size_t)grid : (size_t)sc.nitems, 0, sc, Dense_A{dA, M}, Dense_B{dB, K},
+                          Dense_C{dC, M});
+        }
+        if (e != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = -3;
     }
     if (!rc && hipMemcpy(C, dC, nc * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = -2;
     (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);
     return rc;
+}
+
+int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, const double* A, const double* B, double* C) {
+    return selftest_zgemm(M, N, K, precision, -1, 0, A, B, C);
+}
+int fdga_selftest_zgemm_tiles(int32_t M, int32_t N, int32_t K, int32_t precision, int32_t tile, int64_t grid, const double* A,
+                              const double* B, double* C) {
+    if (tile < 0) return -1;
+    return selftest_zgemm(M, N, K, precision, tile, grid, A, B, C);
 }
 
 // GEMM blocks (GBM x GBN output tiles over the full contraction length) of the batched (p, a, t)
@@ -1553,6 +1597,473 @@ int fdga_set_stage_timing(fdga_ctx* c, int32_t on) {
     if (!c) return -1;
     c->timing = on;
     for (auto& s : c->stage_ms) s = 0.0;
+    return 0;
+}
+
+}  // extern "C"
+
+// =========================================================================================
+// Groups of solver contexts (a parameter sweep on one device).  Each member runs its own stages
+// on its own stream, concurrently with the other members; the BSE_L_K3! and BSE_K3! contractions
+// of all members run as one grouped GEMM launch each on the group's stream (the (Ω, P, channel)
+// blocks of every member as work items, with the member's own box sizes), persistent over a
+// chosen grid and with a chosen tile shape.  Each member's results are those of fdga_iterate on
+// the member alone.  No counterpart in the reference, which solves one parameter point per solver.
+struct GroupItems {   // work items of one tile shape
+    int* items = nullptr;
+    GemmGroupGeom* geo = nullptr;
+    int nitems = 0;
+    int64_t computed = 0;   // non-idle items
+};
+
+struct fdga_group {
+    std::vector<fdga_ctx*> m;
+    int n = 0, precision = 0, nonlocal = 0;
+    hipStream_t st = nullptr;
+    std::vector<hipEvent_t> ev_m;    // member -> group joins
+    hipEvent_t ev_g = nullptr;       // group -> members forks
+    std::string err;
+    int* boff = nullptr;             // first batch of each member in the launch's batch space
+    std::vector<int> boff_h;
+    GroupItems gi[TILE_N];
+    // per-member functors on the device: BSE_L_K3! (independent of the F/Fbuff state) and BSE_K3!
+    // for each F/Fbuff pointer state
+    LK3_Aop* lk3a = nullptr;
+    LK3_Bop* lk3b = nullptr;
+    LK3_Ep* lk3e = nullptr;
+    void* k3a[2] = {nullptr, nullptr};
+    K3_Bop* k3b[2] = {nullptr, nullptr};
+    K3_Ep* k3e[2] = {nullptr, nullptr};
+    int tile[2] = {TILE_64x64, TILE_64x64};   // [0] BSE_L_K3!, [1] BSE_K3!
+    size_t grid[2] = {0, 0};                  // workgroups; 0: one per work item
+    int graph_on = 0;
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int timing = 0;
+    hipEvent_t tev[2] = {nullptr, nullptr};
+    double gemm_ms[2] = {0.0, 0.0};
+};
+
+namespace {
+
+int gfail(fdga_group* g, const std::string& msg, int code = -1) {
+    g->err = msg;
+    return code;
+}
+#define GHIPCHK(g, call)                                                                   \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) return gfail((g), std::string(#call) + ": " + hipGetErrorString(e_), -2); \
+    } while (0)
+
+bool has_k3_groups(const fdga_ctx* c) {
+    for (int k : {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3})
+        if (c->sg[k].ncls) return true;
+    return false;
+}
+
+// work items of one tile shape: members in order of decreasing box (the larger items first), each
+// member's batches in chunks of 8 with the tiles of a batch on one XCD (item index mod 8)
+int group_build_items(fdga_group* g, int tile) {
+    GroupItems& gi = g->gi[tile];
+    if (gi.items) return 0;
+    const int BM = tile_bm(tile), BN = tile_bn(tile);
+    std::vector<GemmGroupGeom> geo((size_t)g->n);
+    std::vector<int> order((size_t)g->n);
+    for (int j = 0; j < g->n; ++j) order[(size_t)j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g->m[(size_t)a]->NV3 > g->m[(size_t)b]->NV3; });
+    std::vector<int> items;
+    int64_t computed = 0;
+    for (int j = 0; j < g->n; ++j) {
+        const fdga_ctx* c = g->m[(size_t)j];
+        const int NV = c->NV3;
+        geo[(size_t)j] = GemmGroupGeom{NV, NV, NV, (int)cdiv(NV, BM), (int)cdiv(NV, BN), g->boff_h[(size_t)j]};
+    }
+    for (int j : order) {
+        const GemmGroupGeom& q = geo[(size_t)j];
+        const int tiles = q.tiles_m * q.tiles_n;
+        const int nb = g->boff_h[(size_t)j + 1] - g->boff_h[(size_t)j];
+        if ((int64_t)nb * tiles >= (1 << 24)) return gfail(g, "grouped GEMM: too many blocks in one member", -1);
+        for (int c0 = 0; c0 < nb; c0 += 8)
+            for (int t = 0; t < tiles; ++t)
+                for (int x = 0; x < 8; ++x) {
+                    const int b = c0 + x;
+                    items.push_back(b < nb ? (j << 24) | (b * tiles + t) : -1);
+                    computed += b < nb;
+                }
+    }
+    GHIPCHK(g, hipMalloc((void**)&gi.items, items.size() * sizeof(int)));
+    GHIPCHK(g, hipMalloc((void**)&gi.geo, geo.size() * sizeof(GemmGroupGeom)));
+    GHIPCHK(g, hipMemcpy(gi.items, items.data(), items.size() * sizeof(int), hipMemcpyHostToDevice));
+    GHIPCHK(g, hipMemcpy(gi.geo, geo.data(), geo.size() * sizeof(GemmGroupGeom), hipMemcpyHostToDevice));
+    gi.nitems = (int)items.size();
+    gi.computed = computed;
+    return 0;
+}
+
+template <class T>
+int group_upload(fdga_group* g, T** dst, const std::vector<T>& src) {
+    GHIPCHK(g, hipMalloc((void**)dst, src.size() * sizeof(T)));
+    GHIPCHK(g, hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// per-member functors of the two contractions (the three channels p, a, t in one launch, no
+// symmetry groups); BSE_K3! for both F/Fbuff pointer states
+int group_build_functors(fdga_group* g) {
+    const int all[3] = {PCH, ACH, TCH};
+    std::vector<LK3_Aop> la;
+    std::vector<LK3_Bop> lb;
+    std::vector<LK3_Ep> le;
+    std::vector<K3_AopT<true>> af[2];
+    std::vector<K3_AopT<false>> an[2];
+    std::vector<K3_Bop> kb[2];
+    std::vector<K3_Ep> ke[2];
+    for (fdga_ctx* c : g->m) {
+        const LK3Set ls = lk3_set(c, all, 3, false, false);
+        la.push_back(ls.a);
+        lb.push_back(ls.b);
+        le.push_back(ls.e);
+        for (int k = 0; k < 2; ++k) {
+            const int s = c->swap_state;
+            const K3Set ks = k3_set(c, all, 3, false, false);
+            af[s].push_back(ks.a<true>());
+            an[s].push_back(ks.a<false>());
+            kb[s].push_back(ks.b());
+            ke[s].push_back(ks.e());
+            swap_F_host(c);   // the other state; two swaps restore the context
+        }
+    }
+    if (group_upload(g, &g->lk3a, la) || group_upload(g, &g->lk3b, lb) || group_upload(g, &g->lk3e, le)) return -2;
+    for (int s = 0; s < 2; ++s) {
+        int rc = g->nonlocal ? group_upload(g, (K3_AopT<false>**)&g->k3a[s], an[s])
+                             : group_upload(g, (K3_AopT<true>**)&g->k3a[s], af[s]);
+        if (rc || group_upload(g, &g->k3b[s], kb[s]) || group_upload(g, &g->k3e[s], ke[s])) return -2;
+    }
+    return 0;
+}
+
+void group_graphs_invalidate(fdga_group* g) {
+    for (auto& row : g->gexec)
+        for (auto& x : row)
+            if (x) {
+                (void)hipGraphExecDestroy(x);
+                x = nullptr;
+            }
+}
+
+size_t group_grid(const fdga_group* g, int which) {
+    const GroupItems& gi = g->gi[g->tile[which]];
+    return g->grid[which] ? std::min(g->grid[which], (size_t)gi.nitems) : (size_t)gi.nitems;
+}
+
+// the grouped contraction `which` (0 BSE_L_K3!, 1 BSE_K3!) on the group's stream
+int group_gemm(fdga_group* g, int which) {
+    const int tile = g->tile[which];
+    const GroupItems& gi = g->gi[tile];
+    const GemmSchedGroup sc{gi.items, gi.geo, gi.nitems};
+    const size_t grid = group_grid(g, which);
+    if (g->timing) GHIPCHK(g, hipEventRecord(g->tev[0], g->st));
+    hipError_t e;
+    if (which == 0) {
+        e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<LK3_Aop>{g->lk3a, g->boff, g->n},
+                      GroupOp<LK3_Bop>{g->lk3b, g->boff, g->n}, GroupOp<LK3_Ep>{g->lk3e, g->boff, g->n});
+    } else {
+        const int s = g->m[0]->swap_state;
+        for (const fdga_ctx* c : g->m)
+            if (c->swap_state != s) return gfail(g, "group members are not at the same iteration (F/Fbuff state)");
+        const GroupOp<K3_Bop> b{g->k3b[s], g->boff, g->n};
+        const GroupOp<K3_Ep> ep{g->k3e[s], g->boff, g->n};
+        if (g->nonlocal)
+            e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<false>>{(K3_AopT<false>*)g->k3a[s], g->boff, g->n},
+                          b, ep);
+        else
+            e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<true>>{(K3_AopT<true>*)g->k3a[s], g->boff, g->n},
+                          b, ep);
+    }
+    if (e != hipSuccess) return gfail(g, std::string("grouped GEMM launch: ") + hipGetErrorString(e), -3);
+    if (g->timing) {
+        GHIPCHK(g, hipEventRecord(g->tev[1], g->st));
+        GHIPCHK(g, hipEventSynchronize(g->tev[1]));
+        float ms = 0.f;
+        GHIPCHK(g, hipEventElapsedTime(&ms, g->tev[0], g->tev[1]));
+        g->gemm_ms[which] += ms;
+    }
+    return 0;
+}
+
+// group stream -> every member stream
+int group_fork(fdga_group* g) {
+    GHIPCHK(g, hipEventRecord(g->ev_g, g->st));
+    for (fdga_ctx* c : g->m) GHIPCHK(g, hipStreamWaitEvent(c->st, g->ev_g, 0));
+    return 0;
+}
+// every member stream -> group stream
+int group_join(fdga_group* g) {
+    for (int j = 0; j < g->n; ++j) {
+        GHIPCHK(g, hipEventRecord(g->ev_m[(size_t)j], g->m[(size_t)j]->st));
+        GHIPCHK(g, hipStreamWaitEvent(g->st, g->ev_m[(size_t)j], 0));
+    }
+    return 0;
+}
+int member_err(fdga_group* g, const fdga_ctx* c) { return gfail(g, std::string("member: ") + c->err, -3); }
+
+// one iterate_solver! of every member (src/solve.jl:4-116), the contractions grouped
+int group_iterate_eager(fdga_group* g, int strategy, int update_sigma) {
+    const int all[3] = {PCH, ACH, TCH};
+    for (fdga_ctx* c : g->m) {
+        if (has_k3_groups(c)) return gfail(g, "group members must not have K3 symmetry groups installed");
+        c->run = c->st;
+    }
+    if (group_fork(g)) return -2;
+    for (fdga_ctx* c : g->m) {
+        if (update_sigma && (fdga_dyson(c) || fdga_bubbles(c))) return member_err(g, c);
+        if (fdga_build_k3_cache(c)) return member_err(g, c);
+        if (strategy == FDGA_FDPA)
+            for (int ch : {PCH, ACH, TCH})
+                if (fdga_bse_l_k2(c, ch)) return member_err(g, c);
+    }
+    if (strategy == FDGA_FDPA) {
+        if (group_join(g) || group_gemm(g, 0) || group_fork(g)) return -3;
+        for (fdga_ctx* c : g->m)
+            if (k3_finish(c, c->FL, all, 3, true, false, false)) return member_err(g, c);
+    }
+    for (fdga_ctx* c : g->m) {
+        if (launch_w<PCH>(c) || launch_w<ACH>(c) || launch_w<TCH>(c)) return member_err(g, c);
+        c->w_fresh = 1;
+        int rc = 0;
+        for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
+        rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));
+        c->w_fresh = 0;
+        if (rc) return member_err(g, c);
+        if (k3_rhs(c, k3_set(c, all, 3, false, false), 3)) return member_err(g, c);
+    }
+    if (group_join(g) || group_gemm(g, 1) || group_fork(g)) return -3;
+    for (fdga_ctx* c : g->m) {
+        if (k3_finish(c, c->Fb, all, 3, false, false, false)) return member_err(g, c);
+        if (fdga_update_F(c)) return member_err(g, c);
+        if (update_sigma && fdga_sde(c, strategy, 1)) return member_err(g, c);
+    }
+    return group_join(g);
+}
+
+int group_capture(fdga_group* g, int strategy) {
+    for (int k = 0; k < 2; ++k) {
+        const int state = g->m[0]->swap_state;
+        hipGraph_t gr = nullptr;
+        GHIPCHK(g, hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+        const int rc = group_iterate_eager(g, strategy, 1);
+        const hipError_t ec = hipStreamEndCapture(g->st, &gr);
+        if (rc || ec != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            (void)hipGetLastError();
+            return gfail(g, "hipGraph capture of the group iteration failed" + (g->err.empty() ? "" : ": " + g->err), -3);
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiateWithFlags(&ex, gr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) return gfail(g, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei), -3);
+        g->gexec[strategy][state] = ex;
+        for (fdga_ctx* c : g->m) {   // undo the recorded swap, then move to the other state
+            swap_F_host(c);
+            swap_F_host(c);
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdga_group_create(fdga_group** out, fdga_ctx* const* members, int32_t n) {
+    if (!out) return -1;
+    *out = nullptr;
+    auto* g = new fdga_group();
+    *out = g;
+    if (!members || n < 1 || n > 127) return gfail(g, "a group needs 1 to 127 members");
+    for (int j = 0; j < n; ++j) {
+        fdga_ctx* c = members[j];
+        if (!c) return gfail(g, "null member");
+        for (fdga_ctx* o : g->m)
+            if (o == c) return gfail(g, "a context appears twice in the group");
+        g->m.push_back(c);
+    }
+    g->n = n;
+    g->precision = g->m[0]->d.precision;
+    g->nonlocal = g->m[0]->d.nonlocal;
+    for (fdga_ctx* c : g->m) {
+        if (c->d.precision != g->precision) return gfail(g, "group members must share the contraction precision");
+        if (c->d.nonlocal != g->nonlocal) return gfail(g, "group members must be all local or all lattice solvers");
+        if (has_k3_groups(c)) return gfail(g, "group members must not have K3 symmetry groups installed");
+    }
+    GHIPCHK(g, hipStreamCreate(&g->st));
+    GHIPCHK(g, hipEventCreateWithFlags(&g->ev_g, hipEventDisableTiming));
+    g->ev_m.assign((size_t)n, nullptr);
+    for (auto& e : g->ev_m) GHIPCHK(g, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : g->tev) GHIPCHK(g, hipEventCreate(&e));
+    g->boff_h.assign((size_t)n + 1, 0);
+    for (int j = 0; j < n; ++j) {
+        const fdga_ctx* c = g->m[(size_t)j];
+        const int64_t nb = (int64_t)3 * c->NW3 * c->NP;
+        if ((int64_t)g->boff_h[(size_t)j] + nb > (int64_t)1 << 30) return gfail(g, "grouped GEMM: too many batches");
+        g->boff_h[(size_t)j + 1] = g->boff_h[(size_t)j] + (int)nb;
+    }
+    if (group_upload(g, &g->boff, g->boff_h)) return -2;
+    for (fdga_ctx* c : g->m) GHIPCHK(g, hipStreamSynchronize(c->st));
+    if (group_build_items(g, TILE_64x64) || group_build_functors(g)) return -2;
+    return 0;
+}
+
+int fdga_group_destroy(fdga_group* g) {
+    if (!g) return 0;
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    group_graphs_invalidate(g);
+    for (auto& gi : g->gi) {
+        if (gi.items) (void)hipFree(gi.items);
+        if (gi.geo) (void)hipFree(gi.geo);
+    }
+    for (void* p : {(void*)g->boff, (void*)g->lk3a, (void*)g->lk3b, (void*)g->lk3e, g->k3a[0], g->k3a[1], (void*)g->k3b[0],
+                    (void*)g->k3b[1], (void*)g->k3e[0], (void*)g->k3e[1]})
+        if (p) (void)hipFree(p);
+    for (auto e : g->ev_m)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : g->tev)
+        if (e) (void)hipEventDestroy(e);
+    if (g->ev_g) (void)hipEventDestroy(g->ev_g);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    delete g;
+    return 0;
+}
+
+const char* fdga_group_last_error(const fdga_group* g) { return g ? g->err.c_str() : "null group"; }
+
+int fdga_group_iterate(fdga_group* g, int32_t strategy, int32_t update_sigma) {
+    if (!g) return -1;
+    if (strategy != FDGA_SCPA && strategy != FDGA_FDPA) return gfail(g, "Calculation strategy unknown");
+    bool sde0 = true;
+    for (const fdga_ctx* c : g->m) sde0 = sde0 && c->sde0_valid;
+    if (g->graph_on && update_sigma && !g->timing && (strategy == FDGA_SCPA || sde0)) {
+        const int s = g->m[0]->swap_state;
+        for (const fdga_ctx* c : g->m)
+            if (c->swap_state != s || has_k3_groups(c)) return gfail(g, "group members changed since the group was made");
+        if (!g->gexec[strategy][s] && group_capture(g, strategy)) return -3;
+        GHIPCHK(g, hipGraphLaunch(g->gexec[strategy][s], g->st));
+        for (fdga_ctx* c : g->m) swap_F_host(c);
+        return 0;
+    }
+    return group_iterate_eager(g, strategy, update_sigma);
+}
+
+int fdga_group_sync(fdga_group* g) {
+    if (!g) return -1;
+    GHIPCHK(g, hipStreamSynchronize(g->st));
+    for (fdga_ctx* c : g->m) GHIPCHK(g, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int fdga_group_time_iterations(fdga_group* g, int32_t strategy, int32_t n, double* ms) {
+    if (!g || !ms || n < 1) return -1;
+    if (fdga_group_sync(g)) return -2;
+    GHIPCHK(g, hipEventRecord(g->tev[0], g->st));
+    for (int i = 0; i < n; ++i)
+        if (fdga_group_iterate(g, strategy, 1)) return -3;
+    GHIPCHK(g, hipEventRecord(g->tev[1], g->st));
+    GHIPCHK(g, hipEventSynchronize(g->tev[1]));
+    float t = 0.f;
+    GHIPCHK(g, hipEventElapsedTime(&t, g->tev[0], g->tev[1]));
+    *ms = t;
+    return 0;
+}
+
+int fdga_group_set_graph(fdga_group* g, int32_t on) {
+    if (!g) return -1;
+    if (fdga_group_sync(g)) return -2;
+    if (!on) group_graphs_invalidate(g);
+    g->graph_on = on ? 1 : 0;
+    return 0;
+}
+
+// event timing of the grouped contractions inside fdga_group_iterate (adds a host sync per
+// contraction); resets the accumulated times
+int fdga_group_set_timing(fdga_group* g, int32_t on) {
+    if (!g) return -1;
+    g->timing = on ? 1 : 0;
+    g->gemm_ms[0] = g->gemm_ms[1] = 0.0;
+    return 0;
+}
+int fdga_group_gemm_ms(const fdga_group* g, int32_t which, double* ms) {
+    if (!g || !ms || which < 0 || which > 1) return -1;
+    *ms = g->gemm_ms[which];
+    return 0;
+}
+
+// tile shape and grid of contraction `which` (0 BSE_L_K3!, 1 BSE_K3!); grid 0 = one workgroup per
+// work item; the number of computed output tiles of that shape in `computed`
+int fdga_group_tiles(fdga_group* g, int32_t which, int32_t* tile, int64_t* grid, int64_t* computed) {
+    if (!g || which < 0 || which > 1) return -1;
+    if (tile) *tile = g->tile[which];
+    if (grid) *grid = (int64_t)g->grid[which];
+    if (computed) *computed = g->gi[g->tile[which]].computed;
+    return 0;
+}
+int fdga_group_set_tiles(fdga_group* g, int32_t which, int32_t tile, int64_t grid) {
+    if (!g || which < 0 || which > 1) return -1;
+    if (tile < 0 || tile >= TILE_N) return gfail(g, "unknown tile shape");
+    if (grid != 0 && (grid < 8 || grid % 8)) return gfail(g, "grid must be 0 or a positive multiple of 8");
+    if (fdga_group_sync(g) || group_build_items(g, tile)) return -2;
+    g->tile[which] = tile;
+    g->grid[which] = (size_t)grid;
+    group_graphs_invalidate(g);
+    return 0;
+}
+
+// Times every tile shape and grid (one workgroup per item; 4, 8 or 16 workgroups per CU) of both
+// contractions on the members' current operands and keeps the fastest.  The contractions'
+// outputs are scratch until the next iteration recomputes them; F, Fbuff's inputs and Σ are left
+// as they are.
+int fdga_group_autotune(fdga_group* g, int32_t reps) {
+    if (!g || reps < 1) return -1;
+    const int all[3] = {PCH, ACH, TCH};
+    for (int t = 0; t < TILE_N; ++t)
+        if (group_build_items(g, t)) return -2;
+    if (fdga_group_sync(g) || group_fork(g)) return -2;
+    for (fdga_ctx* c : g->m) {
+        c->run = c->st;
+        if (fdga_build_k3_cache(c) || k3_rhs(c, k3_set(c, all, 3, false, false), 3)) return member_err(g, c);
+    }
+    if (group_join(g) || fdga_group_sync(g)) return -2;
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount > 0) ncu = pr.multiProcessorCount;
+    }
+    const int keep_timing = g->timing;
+    g->timing = 0;
+    for (int which = 0; which < 2; ++which) {
+        float best = 1e30f;
+        int bt = g->tile[which];
+        size_t bg = g->grid[which];
+        for (int t = 0; t < TILE_N; ++t)
+            for (int per : {0, 4, 8, 16}) {
+                const size_t grid = per ? (size_t)((ncu * per + 7) / 8 * 8) : 0;
+                if (grid && grid >= (size_t)g->gi[t].nitems) continue;
+                g->tile[which] = t;
+                g->grid[which] = grid;
+                if (group_gemm(g, which)) return -3;   // warm-up
+                GHIPCHK(g, hipEventRecord(g->tev[0], g->st));
+                for (int r = 0; r < reps; ++r)
+                    if (group_gemm(g, which)) return -3;
+                GHIPCHK(g, hipEventRecord(g->tev[1], g->st));
+                GHIPCHK(g, hipEventSynchronize(g->tev[1]));
+                float ms = 0.f;
+                GHIPCHK(g, hipEventElapsedTime(&ms, g->tev[0], g->tev[1]));
+                if (ms < best) { best = ms; bt = t; bg = grid; }
+            }
+        g->tile[which] = bt;
+        g->grid[which] = bg;
+    }
+    g->timing = keep_timing;
+    group_graphs_invalidate(g);
     return 0;
 }
 

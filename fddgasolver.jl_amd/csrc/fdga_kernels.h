@@ -441,159 +441,246 @@ template <> struct MfmaT<true> {
 // raises the register count to one wave per SIMD): a loader type with `static constexpr bool
 // prefetch = false` is staged synchronously.
 template <class L> struct Prefetch { static constexpr bool on = true; };
-template <bool F32, class AL, class BL, class EP>
-__global__ __launch_bounds__(256) void zgemm_tiles_t(int M, int N, int K, int tiles_m, int tiles_n, int nbatch,
-                                                    const int* __restrict__ blist, AL al, BL bl, EP ep) {
+
+// One work item of a GEMM launch: an output tile of one batch.
+struct GemmWork { int batch, m0, n0, M, N, K; };
+
+// Work items of a launch with one problem shape.  Without a block list, item it runs batch
+// 8 (it / 8 / tiles) + it % 8, tile (it / 8) % tiles: items go round-robin to the 8 XCDs, and the
+// tiles of one batch (which share their A rows and B columns) stay on one XCD; the item count
+// covers a multiple of 8 batches, items past the last batch are idle.  A block list (built in the
+// same order, -1 = idle) selects the computed tiles instead.
+struct GemmSchedOne {
+    int M, N, K, tiles_m, tiles_n, nbatch, nitems;
+    const int* blist;
+    __device__ __forceinline__ bool item(int it, int BM, int BN, GemmWork& w) const {
+        const int tiles = tiles_m * tiles_n;
+        int bid;
+        if (blist) {
+            bid = blist[it];
+        } else {
+            const int r = it >> 3;
+            const int bt = (r / tiles) * 8 + (it & 7);
+            bid = bt < nbatch ? bt * tiles + r % tiles : -1;
+        }
+        if (bid < 0) return false;
+        w.batch = bid / tiles;
+        const int t = bid - w.batch * tiles;
+        w.m0 = (t % tiles_m) * BM;
+        w.n0 = (t / tiles_m) * BN;
+        w.M = M; w.N = N; w.K = K;
+        return true;
+    }
+};
+
+// Work items of a grouped launch over several problems (solver contexts) with their own shapes:
+// items[it] = member << 24 | local block (batch * tiles + tile of that member), -1 = idle; the
+// member's batches are numbered from geo[member].boff in the launch's batch space.
+struct GemmGroupGeom { int M, N, K, tiles_m, tiles_n, boff; };
+struct GemmSchedGroup {
+    const int* items;
+    const GemmGroupGeom* geo;
+    int nitems;
+    __device__ __forceinline__ bool item(int it, int BM, int BN, GemmWork& w) const {
+        const int e = items[it];
+        if (e < 0) return false;
+        const GemmGroupGeom g = geo[e >> 24];
+        const int lb = e & 0xFFFFFF, tiles = g.tiles_m * g.tiles_n;
+        const int b = lb / tiles, t = lb - b * tiles;
+        w.batch = g.boff + b;
+        w.m0 = (t % g.tiles_m) * BM;
+        w.n0 = (t / g.tiles_m) * BN;
+        w.M = g.M; w.N = g.N; w.K = g.K;
+        return true;
+    }
+};
+
+// A functor over the members of a grouped launch: bind(batch) binds the member's own functor to
+// its local batch (members in batch order, boff[j] = first batch of member j).
+template <class F>
+struct GroupOp {
+    const F* f;
+    const int* boff;
+    int n;
+    __device__ __forceinline__ auto bind(int batch) const -> decltype(f->bind(0)) {
+        int j = 0;
+        while (j + 1 < n && batch >= boff[j + 1]) ++j;
+        return f[j].bind(batch - boff[j]);
+    }
+};
+template <class F> struct Prefetch<GroupOp<F>> { static constexpr bool on = Prefetch<F>::on; };
+
+// Tile shapes: a workgroup of WGM x WGN waves, each wave a 32 x 32 sub-tile (2 x 2 MFMA tiles), so
+// the workgroup tile is 32 WGM x 32 WGN.  The workgroups loop over the work items (it = blockIdx,
+// blockIdx + gridDim, ...): a grid smaller than the item count makes the launch persistent; every
+// workgroup leaves the loop when the items run out.  Each output element accumulates over k in
+// the same order (k-steps of GBK, then 4 at a time) whatever the tile shape or grid, so the results
+// do not depend on them.
+template <bool F32, int WGM, int WGN, class SC, class AL, class BL, class EP>
+__global__ __launch_bounds__(64 * WGM * WGN) void zgemm_tiles_t(SC sc, AL al, BL bl, EP ep) {
     typedef MfmaT<F32> MT;
     typedef typename MT::S S;
     typedef typename MT::Acc Acc;
-    __shared__ S As[2][GBK][GBM + GPAD];
-    __shared__ S Bs[2][GBK][GBN + GPAD];
-    const int tiles = tiles_m * tiles_n;
-    // block = batch * tiles + tile.  Workgroups go round-robin to the 8 XCDs; the tiles of one
-    // batch share their A rows and B columns, so they are placed on one XCD: workgroup w runs
-    // batch 8 (w / 8 / tiles) + w % 8, tile (w / 8) % tiles (the grid covers a multiple of 8
-    // batches; blocks past the last batch are idle).  A block list (built in the same order,
-    // -1 = idle) selects the computed tiles instead.
-    int bid;
-    if (blist) {
-        bid = blist[blockIdx.x];
-    } else {
-        const int w = (int)blockIdx.x, r = w >> 3;
-        const int bt = (r / tiles) * 8 + (w & 7);
-        bid = bt * tiles + r % tiles;
-        if (bt >= nbatch) bid = -1;
-    }
-    if (bid < 0) return;
-    const int batch = bid / tiles;
-    const auto A = al.bind(batch);
-    const auto B = bl.bind(batch);
-    const auto E = ep.bind(batch);
-    const int t = bid - batch * tiles;
-    const int m0 = (t % tiles_m) * GBM, n0 = (t / tiles_m) * GBN;
+    constexpr int NT = 64 * WGM * WGN, BM = 32 * WGM, BN = 32 * WGN;
+    __shared__ S As[2][GBK][BM + GPAD];
+    __shared__ S Bs[2][GBK][BN + GPAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    Acc accr[2][2], acci[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            accr[i][j] = Acc{0, 0, 0, 0};
-            acci[i][j] = Acc{0, 0, 0, 0};
-        }
-    // software pipeline: the operands of the next K-tile are loaded into registers while the
-    // current one is multiplied from LDS
-    constexpr int QA = (GBM * GBK) / 256, QB = (GBN * GBK) / 256;
-    double2 pa[QA], pb[QB];
-    auto fetch = [&](int k0) {
-#pragma unroll
-        for (int q = 0; q < QA; ++q) {
-            const int e = tid + 256 * q;
-            const int gm = m0 + e % GBM, gk = k0 + e / GBM;
-            pa[q] = (gm < M && gk < K) ? A(gm, gk) : czero();
-        }
-#pragma unroll
-        for (int q = 0; q < QB; ++q) {
-            const int e = tid + 256 * q;
-            // B operands are stored with k (ω) fastest: consecutive threads take consecutive k
-            const int gn = n0 + e / GBK, gk = k0 + e % GBK;
-            pb[q] = (gn < N && gk < K) ? B(gk, gn) : czero();
-        }
-    };
+    const int wm = (wave % WGM) * 32, wn = (wave / WGM) * 32;
+    constexpr int QA = (BM * GBK) / NT, QB = (BN * GBK) / NT;
     constexpr bool PF = Prefetch<AL>::on && Prefetch<BL>::on;
-    if (PF) fetch(0);
-    for (int k0 = 0; k0 < K; k0 += GBK) {
-        if (PF) {
+    for (int it = blockIdx.x; it < sc.nitems; it += gridDim.x) {
+        GemmWork w;
+        if (!sc.item(it, BM, BN, w)) continue;
+        const int M = w.M, N = w.N, K = w.K, m0 = w.m0, n0 = w.n0;
+        const auto A = al.bind(w.batch);
+        const auto B = bl.bind(w.batch);
+        const auto E = ep.bind(w.batch);
+        Acc accr[2][2], acci[2][2];
 #pragma unroll
-            for (int q = 0; q < QA; ++q) {
-                const int e = tid + 256 * q;
-                As[0][e / GBM][e % GBM] = (S)pa[q].x;
-                As[1][e / GBM][e % GBM] = (S)pa[q].y;
-            }
-#pragma unroll
-            for (int q = 0; q < QB; ++q) {
-                const int e = tid + 256 * q;
-                Bs[0][e % GBK][e / GBK] = (S)pb[q].x;
-                Bs[1][e % GBK][e / GBK] = (S)pb[q].y;
-            }
-        } else {
-            // element by element: load and store to LDS without holding the tile in registers
-#pragma unroll
-            for (int q = 0; q < QA; ++q) {
-                const int e = tid + 256 * q;
-                const int gm = m0 + e % GBM, gk = k0 + e / GBM;
-                const double2 v = (gm < M && gk < K) ? A(gm, gk) : czero();
-                As[0][e / GBM][e % GBM] = (S)v.x;
-                As[1][e / GBM][e % GBM] = (S)v.y;
-            }
-#pragma unroll
-            for (int q = 0; q < QB; ++q) {
-                const int e = tid + 256 * q;
-                const int gn = n0 + e / GBK, gk = k0 + e % GBK;
-                const double2 v = (gn < N && gk < K) ? B(gk, gn) : czero();
-                Bs[0][e % GBK][e / GBK] = (S)v.x;
-                Bs[1][e % GBK][e / GBK] = (S)v.y;
-            }
-        }
-        __syncthreads();
-        if (PF && k0 + GBK < K) fetch(k0 + GBK);
-#pragma unroll
-        for (int kk = 0; kk < GBK; kk += 4) {
-            const int kr = kk + (lane >> 4), c = lane & 15;
-            S ar[2], ai[2], nai[2], br[2], bi[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                ar[i] = As[0][kr][wm + 16 * i + c];
-                ai[i] = As[1][kr][wm + 16 * i + c];
-                nai[i] = -ai[i];
-            }
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                br[j] = Bs[0][kr][wn + 16 * j + c];
-                bi[j] = Bs[1][kr][wn + 16 * j + c];
+                accr[i][j] = Acc{0, 0, 0, 0};
+                acci[i][j] = Acc{0, 0, 0, 0};
             }
-            // B fragment as the MFMA's left operand: the accumulator is the transposed tile, so
-            // lane % 16 runs along the output rows (contiguous in memory).  The eight first terms
-            // are independent; each accumulator's second term follows eight MFMAs later.
+        // software pipeline: the operands of the next K-tile are loaded into registers while the
+        // current one is multiplied from LDS
+        double2 pa[QA], pb[QB];
+        auto fetch = [&](int k0) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int q = 0; q < QA; ++q) {
+                const int e = tid + NT * q;
+                const int gm = m0 + e % BM, gk = k0 + e / BM;
+                pa[q] = (gm < M && gk < K) ? A(gm, gk) : czero();
+            }
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    accr[i][j] = MT::mfma(br[j], ar[i], accr[i][j]);
-                    acci[i][j] = MT::mfma(bi[j], ar[i], acci[i][j]);
+            for (int q = 0; q < QB; ++q) {
+                const int e = tid + NT * q;
+                // B operands are stored with k (ω) fastest: consecutive threads take consecutive k
+                const int gn = n0 + e / GBK, gk = k0 + e % GBK;
+                pb[q] = (gn < N && gk < K) ? B(gk, gn) : czero();
+            }
+        };
+        if (PF) fetch(0);
+        for (int k0 = 0; k0 < K; k0 += GBK) {
+            if (PF) {
+#pragma unroll
+                for (int q = 0; q < QA; ++q) {
+                    const int e = tid + NT * q;
+                    As[0][e / BM][e % BM] = (S)pa[q].x;
+                    As[1][e / BM][e % BM] = (S)pa[q].y;
                 }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+                for (int q = 0; q < QB; ++q) {
+                    const int e = tid + NT * q;
+                    Bs[0][e % GBK][e / GBK] = (S)pb[q].x;
+                    Bs[1][e % GBK][e / GBK] = (S)pb[q].y;
+                }
+            } else {
+                // element by element: load and store to LDS without holding the tile in registers
+#pragma unroll
+                for (int q = 0; q < QA; ++q) {
+                    const int e = tid + NT * q;
+                    const int gm = m0 + e % BM, gk = k0 + e / BM;
+                    const double2 v = (gm < M && gk < K) ? A(gm, gk) : czero();
+                    As[0][e / BM][e % BM] = (S)v.x;
+                    As[1][e / BM][e % BM] = (S)v.y;
+                }
+#pragma unroll
+                for (int q = 0; q < QB; ++q) {
+                    const int e = tid + NT * q;
+                    const int gn = n0 + e / GBK, gk = k0 + e % GBK;
+                    const double2 v = (gn < N && gk < K) ? B(gk, gn) : czero();
+                    Bs[0][e % GBK][e / GBK] = (S)v.x;
+                    Bs[1][e % GBK][e / GBK] = (S)v.y;
+                }
+            }
+            __syncthreads();
+            if (PF && k0 + GBK < K) fetch(k0 + GBK);
+#pragma unroll
+            for (int kk = 0; kk < GBK; kk += 4) {
+                const int kr = kk + (lane >> 4), c = lane & 15;
+                S ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    ar[i] = As[0][kr][wm + 16 * i + c];
+                    ai[i] = As[1][kr][wm + 16 * i + c];
+                    nai[i] = -ai[i];
+                }
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    accr[i][j] = MT::mfma(bi[j], nai[i], accr[i][j]);
-                    acci[i][j] = MT::mfma(br[j], ai[i], acci[i][j]);
+                    br[j] = Bs[0][kr][wn + 16 * j + c];
+                    bi[j] = Bs[1][kr][wn + 16 * j + c];
                 }
+                // B fragment as the MFMA's left operand: the accumulator is the transposed tile, so
+                // lane % 16 runs along the output rows (contiguous in memory).  The eight first terms
+                // are independent; each accumulator's second term follows eight MFMAs later.
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        accr[i][j] = MT::mfma(br[j], ar[i], accr[i][j]);
+                        acci[i][j] = MT::mfma(bi[j], ar[i], acci[i][j]);
+                    }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        accr[i][j] = MT::mfma(bi[j], nai[i], accr[i][j]);
+                        acci[i][j] = MT::mfma(br[j], ai[i], acci[i][j]);
+                    }
+            }
+            __syncthreads();
         }
-        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm + 16 * i + (lane & 15);
+                    const int col = n0 + wn + 16 * j + MT::row(lane, r);
+                    if (row < M && col < N) E(row, col, cmk((double)accr[i][j][r], (double)acci[i][j][r]));
+                }
     }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + 16 * i + (lane & 15);
-                const int col = n0 + wn + 16 * j + MT::row(lane, r);
-                if (row < M && col < N) E(row, col, cmk((double)accr[i][j][r], (double)acci[i][j][r]));
-            }
 }
 
-// launch helper: precision 0 = FP64 MFMA, 1 = FP32 MFMA.  Without a block list the grid covers
-// nbatch batches of tm x tn tiles (rounded up to a multiple of 8 batches, see zgemm_tiles_t);
-// with one, nlist workgroups run the listed blocks.
+// tile shapes of the GEMM launches: workgroups of 2 x 2, 1 x 1, 2 x 1 and 1 x 2 waves
+enum { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_32x64 = 3, TILE_N = 4 };
+inline int tile_bm(int tile) { return tile == TILE_64x64 || tile == TILE_64x32 ? 64 : 32; }
+inline int tile_bn(int tile) { return tile == TILE_64x64 || tile == TILE_32x64 ? 64 : 32; }
+
+// launch of one scheduler over `grid` workgroups (persistent when grid < sc.nitems) in the given
+// precision (0 FP64, 1 FP32 MFMA) and tile shape
+template <int WGM, int WGN, class SC, class AL, class BL, class EP>
+inline void zgemm_go(int precision, unsigned grid, hipStream_t st, const SC& sc, const AL& a, const BL& b, const EP& e) {
+    if (precision == 1) zgemm_tiles_t<true, WGM, WGN><<<grid, 64 * WGM * WGN, 0, st>>>(sc, a, b, e);
+    else zgemm_tiles_t<false, WGM, WGN><<<grid, 64 * WGM * WGN, 0, st>>>(sc, a, b, e);
+}
+template <class SC, class AL, class BL, class EP>
+inline hipError_t zgemm_run(int precision, int tile, size_t grid, hipStream_t st, const SC& sc, const AL& a, const BL& b,
+                            const EP& e) {
+    if (grid == 0 || sc.nitems == 0) return hipSuccess;
+    switch (tile) {
+    case TILE_32x32: zgemm_go<1, 1>(precision, (unsigned)grid, st, sc, a, b, e); break;
+    case TILE_64x32: zgemm_go<2, 1>(precision, (unsigned)grid, st, sc, a, b, e); break;
+    case TILE_32x64: zgemm_go<1, 2>(precision, (unsigned)grid, st, sc, a, b, e); break;
+    default: zgemm_go<2, 2>(precision, (unsigned)grid, st, sc, a, b, e); break;
+    }
+    return hipGetLastError();
+}
+
+// launch helper of the single-problem GEMMs (64 x 64 tiles, one workgroup per item): without a
+// block list the items cover nbatch batches of tm x tn tiles (rounded up to a multiple of 8
+// batches); with one, nlist workgroups run the listed blocks.
 template <class AL, class BL, class EP>
 inline hipError_t zgemm_launch(int precision, int nbatch, hipStream_t st, int M, int N, int K, int tm, int tn,
                                const int* blist, size_t nlist, const AL& a, const BL& b, const EP& e) {
-    const size_t nblocks = blist ? nlist : (size_t)((nbatch + 7) / 8) * 8 * tm * tn;
-    if (nblocks == 0) return hipSuccess;
-    if (precision == 1) zgemm_tiles_t<true><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, nbatch, blist, a, b, e);
-    else zgemm_tiles_t<false><<<(unsigned)nblocks, 256, 0, st>>>(M, N, K, tm, tn, nbatch, blist, a, b, e);
+    const size_t nitems = blist ? nlist : (size_t)((nbatch + 7) / 8) * 8 * tm * tn;
+    if (nitems == 0) return hipSuccess;
+    const GemmSchedOne sc{M, N, K, tm, tn, nbatch, (int)nitems, blist};
+    if (precision == 1) zgemm_tiles_t<true, 2, 2><<<(unsigned)nitems, 256, 0, st>>>(sc, a, b, e);
+    else zgemm_tiles_t<false, 2, 2><<<(unsigned)nitems, 256, 0, st>>>(sc, a, b, e);
     return hipGetLastError();
 }
 

@@ -1,7 +1,7 @@
 """MI355X (gfx950) Bethe-Salpeter hot path of fdDGAsolver.jl: host API over libfdga.so."""
 from ._lib import build, load, LIB_PATH, HEADER  # noqa: F401
 from .solver import (  # noqa: F401
-    FdgaError, Solver, pCh, tCh, aCh,
+    FdgaError, Solver, SolverGroup, pCh, tCh, aCh,
     siam_bare_Green, hubbard_bare_Green, hubbard_band, compute_occupation,
     ParquetSolver, NL_ParquetSolver,
     parquet_solver_siam_parquet_approximation, parquet_solver_hubbard_parquet_approximation,

@@ -22,6 +22,7 @@ CACHE0 = 32
 PCH, TCH, ACH = 0, 1, 2
 SCPA, FDPA = 0, 1
 FP64, FP32 = 0, 1
+TILE_64x64, TILE_32x32, TILE_64x32, TILE_32x64 = 0, 1, 2, 3
 
 
 def vertex_id(vtx, ch, cls):
@@ -78,6 +79,19 @@ SIGNATURES = {
     "fdga_stage_ms": (ctypes.c_int, [_vp, _i32, _dp]),
     "fdga_set_stage_timing": (ctypes.c_int, [_vp, _i32]),
     "fdga_selftest_zgemm": (ctypes.c_int, [_i32, _i32, _i32, _i32, _dp, _dp, _dp]),
+    "fdga_selftest_zgemm_tiles": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32, _i64, _dp, _dp, _dp]),
+    "fdga_group_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), _i32]),
+    "fdga_group_destroy": (ctypes.c_int, [_vp]),
+    "fdga_group_last_error": (ctypes.c_char_p, [_vp]),
+    "fdga_group_iterate": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_group_sync": (ctypes.c_int, [_vp]),
+    "fdga_group_time_iterations": (ctypes.c_int, [_vp, _i32, _i32, _dp]),
+    "fdga_group_set_graph": (ctypes.c_int, [_vp, _i32]),
+    "fdga_group_set_timing": (ctypes.c_int, [_vp, _i32]),
+    "fdga_group_gemm_ms": (ctypes.c_int, [_vp, _i32, _dp]),
+    "fdga_group_tiles": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "fdga_group_set_tiles": (ctypes.c_int, [_vp, _i32, _i32, _i64]),
+    "fdga_group_autotune": (ctypes.c_int, [_vp, _i32]),
 }
 
 _lib = None

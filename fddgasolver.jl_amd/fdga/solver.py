@@ -151,6 +151,85 @@ class Solver:
         self._check(self.lib.fdga_sync(self.h))
 
 
+class SolverGroup:
+    """Several solvers on one device iterated together (a parameter sweep): each member's stages
+    run on its own stream, and the BSE_L_K3!/BSE_K3! contractions of all members run as one
+    grouped GEMM launch each.  Members keep their own boxes and lattices; they must share the
+    contraction precision and be all local or all lattice solvers, without K3 symmetry groups.
+    No counterpart in the reference (one solver per parameter point); each member ends where
+    `iterate_solver` on the member alone would.  Close the group before its members."""
+
+    TILES = {"64x64": L.TILE_64x64, "32x32": L.TILE_32x32, "64x32": L.TILE_64x32, "32x64": L.TILE_32x64}
+
+    def __init__(self, members):
+        self.members = list(members)
+        self.lib = L.load()
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        rc = self.lib.fdga_group_create(ctypes.byref(h), arr, len(self.members))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.fdga_group_last_error(h).decode() if h.value else "fdga_group_create failed"
+            if h.value:
+                self.lib.fdga_group_destroy(h)
+            self.h = None
+            raise FdgaError(msg)
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.lib.fdga_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FdgaError(self.lib.fdga_group_last_error(self.h).decode())
+
+    def iterate(self, strategy="fdPA", update_Σ=True):
+        """iterate_solver! (src/solve.jl:4-116) of every member."""
+        self._check(self.lib.fdga_group_iterate(self.h, _strategy(strategy), int(update_Σ)))
+
+    def sync(self):
+        self._check(self.lib.fdga_group_sync(self.h))
+
+    def time_iterations(self, strategy, n):
+        ms = ctypes.c_double(0.0)
+        self._check(self.lib.fdga_group_time_iterations(self.h, _strategy(strategy), n, ctypes.byref(ms)))
+        return ms.value
+
+    def set_graph(self, on):
+        self._check(self.lib.fdga_group_set_graph(self.h, int(on)))
+
+    def set_timing(self, on):
+        self._check(self.lib.fdga_group_set_timing(self.h, int(on)))
+
+    def gemm_ms(self):
+        """Accumulated times of the grouped BSE_L_K3! and BSE_K3! launches while timing is on."""
+        out = []
+        for w in (0, 1):
+            ms = ctypes.c_double(0.0)
+            self._check(self.lib.fdga_group_gemm_ms(self.h, w, ctypes.byref(ms)))
+            out.append(ms.value)
+        return out
+
+    def tiles(self, which):
+        """(tile shape name, grid, computed output tiles) of contraction `which` (0 BSE_L_K3!,
+        1 BSE_K3!); grid 0 = one workgroup per work item."""
+        t, g, n = ctypes.c_int32(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        self._check(self.lib.fdga_group_tiles(self.h, which, ctypes.byref(t), ctypes.byref(g), ctypes.byref(n)))
+        name = {v: k for k, v in self.TILES.items()}[t.value]
+        return name, g.value, n.value
+
+    def set_tiles(self, which, tile, grid=0):
+        self._check(self.lib.fdga_group_set_tiles(self.h, which, self.TILES[tile], grid))
+
+    def autotune(self, reps=3):
+        """Time every tile shape and grid of both contractions and keep the fastest."""
+        self._check(self.lib.fdga_group_autotune(self.h, reps))
+
+
 def _strategy(strategy):
     if isinstance(strategy, int):
         return strategy

@@ -116,6 +116,40 @@ int64_t fdga_gemm_blocks(const fdga_ctx* ctx, int32_t left);
  * column-major M x K and K x N ComplexF64 matrices on the FP64 (precision 0) or FP32 (1) MFMA path.  No counterpart in the reference; it exists
  * so that the library's tests can check the MFMA tiling directly. */
 int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, const double* A, const double* B, double* C);
+/* The same through tile shape `tile` (FDGA_TILE_*) over `grid` workgroups (0: one per output tile;
+ * otherwise a multiple of 8, the workgroups looping over the tiles).  No counterpart in the reference. */
+int fdga_selftest_zgemm_tiles(int32_t M, int32_t N, int32_t K, int32_t precision, int32_t tile, int64_t grid,
+                              const double* A, const double* B, double* C);
+
+/* ---- groups of solver contexts: a parameter sweep on one device ----------------------------
+ * The reference solves one parameter point per solver; a sweep calls solve! per point.  A group
+ * iterates several contexts together: each member's stages run on the member's stream, and the
+ * BSE_L_K3!/BSE_K3! contractions of all members run as one grouped GEMM launch each (the (Ω, P,
+ * channel) blocks of every member as work items, each with the member's own box sizes).  Each
+ * member's results equal those of fdga_iterate on the member alone.  Members must share the
+ * precision, be all local or all lattice solvers and have no K3 symmetry groups; they must not be
+ * iterated alone while in a group, and the group must be destroyed before its members.  No
+ * counterpart in the reference. */
+typedef struct fdga_group fdga_group;
+enum { FDGA_TILE_64x64 = 0, FDGA_TILE_32x32 = 1, FDGA_TILE_64x32 = 2, FDGA_TILE_32x64 = 3 };
+int         fdga_group_create(fdga_group** g, fdga_ctx* const* members, int32_t n);
+int         fdga_group_destroy(fdga_group* g);
+const char* fdga_group_last_error(const fdga_group* g);
+/* iterate_solver! (src/solve.jl:4-116) of every member */
+int fdga_group_iterate(fdga_group* g, int32_t strategy, int32_t update_sigma);
+int fdga_group_sync(fdga_group* g);
+int fdga_group_time_iterations(fdga_group* g, int32_t strategy, int32_t n, double* ms);
+/* replay fdga_group_iterate(strategy, 1) from captured hipGraphs (as fdga_set_graph) */
+int fdga_group_set_graph(fdga_group* g, int32_t on);
+/* event timing of the two grouped contractions (which: 0 BSE_L_K3!, 1 BSE_K3!), ms summed */
+int fdga_group_set_timing(fdga_group* g, int32_t on);
+int fdga_group_gemm_ms(const fdga_group* g, int32_t which, double* ms);
+/* tile shape and grid of contraction `which`; `computed`: non-idle output tiles of one launch */
+int fdga_group_tiles(fdga_group* g, int32_t which, int32_t* tile, int64_t* grid, int64_t* computed);
+int fdga_group_set_tiles(fdga_group* g, int32_t which, int32_t tile, int64_t grid);
+/* autotuner: times every tile shape and grid of both contractions (`reps` launches each) on the
+ * members' current operands and keeps the fastest; the solver state is left as it was */
+int fdga_group_autotune(fdga_group* g, int32_t reps);
 
 #ifdef __cplusplus
 }

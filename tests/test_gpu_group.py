@@ -1,0 +1,119 @@
+"""Groups of solvers iterated together (parameter sweeps): each member's stages on its own stream,
+the BSE_L_K3!/BSE_K3! contractions of all members in one grouped GEMM launch each.  Every member
+must end exactly where iterate_solver on the member alone ends (same kernels, and the grouped
+GEMM accumulates every element in the same order whatever the tile shape or grid), under every
+tile shape and persistent grid, with the hipGraph replay and after the autotuner."""
+import numpy as np
+import pytest
+
+from helpers import BOX, make_pair, relerr
+
+pytestmark = pytest.mark.gpu
+
+# lattice members with uneven boxes and lattices (K3 boxes (3,3), (3,2), (2,4); L = 2, 3, 16)
+NL_MEMBERS = [("nl_L2", BOX), ("nl_L3", dict(BOX, nK3=(3, 2))), ("nl_L16", dict(BOX, nK3=(2, 4)))]
+LOCAL_MEMBERS = [("local_ref", BOX), ("local_vertexref", dict(BOX, nK3=(2, 4)))]
+
+
+def _pairs(members, seed, precision="fp64"):
+    solo = [make_pair(c, seed=seed + k, box=b, precision=precision)[0] for k, (c, b) in enumerate(members)]
+    grp = [make_pair(c, seed=seed + k, box=b, precision=precision)[0] for k, (c, b) in enumerate(members)]
+    return solo, grp
+
+
+def _assert_same(solo, grp, what):
+    import fdga
+    for k, (a, b) in enumerate(zip(solo, grp)):
+        x, y = fdga.flatten(a), fdga.flatten(b)
+        assert relerr(y, x) == 0.0, f"{what} member {k}: {relerr(y, x):.3e}"
+
+
+@pytest.mark.parametrize("strategy", ["fdPA", "scPA"])
+@pytest.mark.parametrize("members", ["nl", "local"])
+def test_group_iteration_equals_members_alone(members, strategy):
+    import fdga
+    solo, grp = _pairs(NL_MEMBERS if members == "nl" else LOCAL_MEMBERS, seed=11)
+    G = fdga.SolverGroup(grp)
+    try:
+        for it in range(2):
+            for S in solo:
+                fdga.iterate_solver(S, strategy)
+            G.iterate(strategy)
+            G.sync()
+            _assert_same(solo, grp, f"{members} {strategy} iteration {it}")
+    finally:
+        G.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_group_every_tile_shape_and_grid(precision):
+    import fdga
+    solo, grp = _pairs(NL_MEMBERS, seed=21, precision=precision)
+    G = fdga.SolverGroup(grp)
+    try:
+        it = 0
+        for tile in ("64x64", "32x32", "64x32", "32x64"):
+            for grid in (0, 8):
+                G.set_tiles(0, tile, grid)
+                G.set_tiles(1, tile, grid)
+                assert G.tiles(1)[:2] == (tile, grid)
+                for S in solo:
+                    fdga.iterate_solver(S, "fdPA")
+                G.iterate("fdPA")
+                G.sync()
+                _assert_same(solo, grp, f"{precision} tile {tile} grid {grid} iteration {it}")
+                it += 1
+    finally:
+        G.close()
+
+
+def test_group_graph_replay_and_autotune():
+    """Eager first iteration (caches the reference SDE), then hipGraph replays with eager work in
+    between; then the autotuner (which must leave the state alone) and more replays."""
+    import fdga
+    solo, grp = _pairs(NL_MEMBERS, seed=31)
+    G = fdga.SolverGroup(grp)
+    try:
+        G.set_graph(True)
+        for it in range(3):
+            for S in solo:
+                fdga.iterate_solver(S, "fdPA")
+            G.iterate("fdPA")
+            G.sync()
+            _assert_same(solo, grp, f"graph replay {it}")
+        G.autotune(reps=1)
+        for which in (0, 1):
+            tile, grid, computed = G.tiles(which)
+            assert tile in G.TILES and computed > 0
+        _assert_same(solo, grp, "after autotune")
+        for it in range(2):
+            for S in solo:
+                fdga.iterate_solver(S, "fdPA")
+            G.iterate("fdPA")
+            G.sync()
+            _assert_same(solo, grp, f"replay after autotune {it}")
+    finally:
+        G.close()
+
+
+def test_group_errors():
+    import fdga
+    a = make_pair("nl_L2", seed=1)[0]
+    b = make_pair("local_ref", seed=1)[0]
+    with pytest.raises(fdga.FdgaError, match="all local or all lattice"):
+        fdga.SolverGroup([a, b])
+    c = make_pair("nl_L2", seed=1, precision="fp32")[0]
+    with pytest.raises(fdga.FdgaError, match="precision"):
+        fdga.SolverGroup([a, c])
+    with pytest.raises(fdga.FdgaError, match="twice"):
+        fdga.SolverGroup([a, a])
+    d = make_pair("local_vertexref", seed=2)[0]
+    d.init_sym_grp()
+    with pytest.raises(fdga.FdgaError, match="symmetry"):
+        fdga.SolverGroup([b, d])
+    G = fdga.SolverGroup([a])
+    with pytest.raises(fdga.FdgaError, match="tile"):
+        G._check(G.lib.fdga_group_set_tiles(G.h, 1, 9, 0))
+    with pytest.raises(fdga.FdgaError, match="multiple of 8"):
+        G.set_tiles(1, "32x32", 12)
+    G.close()
