@@ -126,16 +126,17 @@ def time_path(S, args, names):
 C2 = dict(L=16, T=0.2, U=2.0, mu=0.0, t1=1.0, nK1=64, nK2=(32, 32), nK3=(16, 32), mPinu=32, nG=256)
 
 
-def build_c2(precision, w=C2):
-    """Nonlocal 2D Hubbard fdPA solver (NL_ParquetSolver, src/nonlocal/ParquetSolver.jl:77-152) on a
-    16 x 16 lattice with a synthetic local reference Vertex: K1 = K2 = 0 and K3 a seed-0 complex
-    Gaussian times 0.1 U / (1 + |ν| + |ν'|) (ν in units of πT), G0 the bare lattice G, Σ0 = 0."""
+def build_c2(precision, w=C2, seed=0):
+    """Nonlocal 2D Hubbard fdPA solver (NL_ParquetSolver, src/nonlocal/ParquetSolver.jl:77-152) on an
+    L x L lattice with a synthetic local reference Vertex: K1 = K2 = 0 and K3 a complex Gaussian
+    (numpy PCG64, `seed`) times 0.1 U / (1 + |ν| + |ν'|) (ν in units of πT), G0 the bare lattice G,
+    Σ0 = 0."""
     import numpy as np
     import fdga
     L = w["L"]
     Gb = fdga.hubbard_bare_Green(w["nG"], w["T"], L, μ=w["mu"], t1=w["t1"])
     nb, nf = w["nK3"]
-    rng = np.random.default_rng(0)
+    rng = np.random.default_rng(seed)
     NW, NV = 2 * nb - 1, 2 * nf
     n = np.arange(-nf, nf)
     amp = 0.1 * w["U"] / (1 + np.abs(2 * n[None, :] + 1) + np.abs(2 * n[:, None] + 1))   # [ν', ν] -> (ν, ν') symmetric
@@ -184,6 +185,72 @@ def main_c2(args):
                                   "nG=256, synthetic local reference vertex", "steps": args.steps, "warmup": args.warmup,
                       "fp64": res["fp64"], "fp32_mfma": res["fp32"],
                       "fp32_vs_fp64_state_relerr_after_one_iteration": rel}), flush=True)
+
+
+# batched sweep: 8 Hubbard points (U, beta, nK3f) on a 24 x 24 lattice, mΠν = 1; per point
+# K3 = (nK3f/2, nK3f), K2 = (nK3f, nK3f), K1 = 2 nK3f, nG = 4 nK3f (beta 5 for U <= 2, 10 above)
+C4_POINTS = [(0.5, 5.0, 24), (1.0, 5.0, 28), (1.5, 5.0, 32), (2.0, 5.0, 36),
+             (2.05, 10.0, 40), (2.09, 10.0, 44), (2.5, 10.0, 48), (2.9, 10.0, 48)]
+C4_L = 24
+
+
+def build_c4_point(k, precision):
+    U, beta, f = C4_POINTS[k]
+    w = dict(L=C4_L, T=1.0 / beta, U=U, mu=0.0, t1=1.0, nK1=2 * f, nK2=(f, f), nK3=(f // 2, f), mPinu=1, nG=4 * f)
+    return build_c2(precision, w, seed=k), w
+
+
+def main_c4(args):
+    """The batched sweep: one parquet iteration of all 8 points per step, every member's stages on
+    its own stream and the BSE contractions of all points in one grouped persistent launch each,
+    tile shapes and grids from the autotuner, the group iteration replayed from a hipGraph; FP32
+    MFMA contraction.  Compared with the same points iterated one after the other."""
+    import fdga
+    t0 = time.perf_counter()
+    S, ws = [], []
+    for k in range(len(C4_POINTS)):
+        Sk, w = build_c4_point(k, "fp32")
+        S.append(Sk)
+        ws.append(w)
+        log(f"bench c4: point {k} U={w['U']} beta={1 / w['T']:g} K3={w['nK3']} built ({time.perf_counter() - t0:.1f} s)")
+    G = fdga.SolverGroup(S)
+    G.iterate("fdPA")   # first iteration eagerly (caches each point's reference SDE)
+    G.sync()
+    t1 = time.perf_counter()
+    G.autotune(reps=2)
+    tuned = {("L_K3", "K3")[wh]: dict(zip(("tile", "grid", "computed_tiles"), G.tiles(wh))) for wh in (0, 1)}
+    log(f"bench c4: autotuned in {time.perf_counter() - t1:.1f} s: {tuned}")
+    G.set_graph(True)
+    for _ in range(max(1, args.warmup)):
+        G.iterate("fdPA")
+    G.sync()
+    ms = G.time_iterations("fdPA", args.steps) / args.steps
+    G.set_timing(True)
+    G.iterate("fdPA")
+    G.sync()
+    gl, gk = G.gemm_ms()
+    G.set_timing(False)
+    # algorithmic flops of one grouped launch: 3 channels x 8 flop per complex MAC, dense per point
+    f_launch = sum(3 * 8.0 * (2 * w["nK3"][0] - 1) * w["L"] ** 2 * (2 * w["nK3"][1]) ** 3 for w in ws)
+    tf_l, tf_k = f_launch / (gl * 1e-3) / 1e12, f_launch / (gk * 1e-3) / 1e12
+    log(f"bench c4: {ms:.1f} ms per sweep iteration; grouped GEMMs {gl:.2f} ms ({tf_l:.1f} TFLOP/s) and "
+        f"{gk:.2f} ms ({tf_k:.1f} TFLOP/s)")
+    G.close()
+    solo = []
+    for Sk in S:   # the same points one after the other (eager, each alone)
+        solo.append(Sk.time_iterations("fdPA", 1))
+    log(f"bench c4: points one after the other {sum(solo):.1f} ms ({', '.join(f'{x:.1f}' for x in solo)})")
+    peak = 157.3
+    print(json.dumps({
+        "workload": "c4: 8 NL Hubbard points, 24x24 lattice, U in {0.5,1,1.5,2 (beta 5), 2.05,2.09,2.5,2.9 (beta 10)}, "
+                    "K3 (12,24)..(24,48), mPinu=1, fdPA, FP32-MFMA contraction, synthetic local reference vertices",
+        "steps": args.steps, "warmup": args.warmup,
+        "sweep_ms_per_iteration": ms, "sweep_iters_per_s": 1000.0 / ms,
+        "points_one_after_another_ms": sum(solo),
+        "grouped_gemm": {"L_K3_ms": gl, "K3_ms": gk, "flops_per_launch": f_launch,
+                         "L_K3_tflops": tf_l, "K3_tflops": tf_k, "peak_tflops_fp32": peak,
+                         "roofline_frac": max(tf_l, tf_k) / peak},
+        "autotuned": tuned}), flush=True)
 
 
 C0 = dict(T=0.1, U=1.0, Delta=3.141592653589793 / 5, nK1=33, nK2=(16, 16), nK3=(8, 8), mPinu=6, nG=96)
@@ -247,10 +314,11 @@ def main():
                          "WRITE_SIZE counters against a copy of known size)")
     ap.add_argument("--tol32", type=float, default=1e-7,
                     help="Anderson tolerance of the FP32-MFMA solve (its residual floor is ~5e-8 at this workload)")
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"],
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"],
                     help="c1: the headline SIAM workload (one JSON line of the metric); c2: the NL Hubbard lattice "
                          "iteration (timing and FP32 error, separate JSON line); c0: the small SIAM solve (hipGraph "
-                         "replay and Newton–Krylov vs mixing, separate JSON line)")
+                         "replay and Newton–Krylov vs mixing, separate JSON line); c4: the 8-point lattice sweep in one "
+                         "solver group (grouped GEMM launches, autotuned tiles, hipGraph; separate JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the parquet hot path is run on a single GPU in this build "
@@ -267,6 +335,8 @@ def main():
         return main_c2(args)
     if args.workload == "c0":
         return main_c0(args)
+    if args.workload == "c4":
+        return main_c4(args)
     import fdga
     w = WORKLOAD
     names = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE", "gemm_L_K3", "gemm_K3"]

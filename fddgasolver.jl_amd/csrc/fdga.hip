@@ -343,6 +343,10 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
         constexpr int NB = 16;
         k_bubble_nl<16, NB><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)NB * 256 * sizeof(double2), c->run>>>(
             pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB);
+    } else if (L == 24) {
+        constexpr int NB = 6;   // 6 x 576 complex + 24 twiddles in LDS (55.7 KB)
+        k_bubble_nl<24, NB><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)(NB * 576 + 24) * sizeof(double2), c->run>>>(
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB);
     } else {
         const int NB = std::max(1, std::min(16, (65536 / (int)sizeof(double2) - L) / (L * L)));
         k_bubble_nl<0, 0><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)(NB * L * L + L) * sizeof(double2), c->run>>>(

@@ -232,6 +232,59 @@ struct BubbleFold {
     int origin;     // bit c: R of slot c is the origin
     double w[4];
 };
+// length 24 = 3 x 8, j = 8 j1 + j2, k = k1 + 3 k2:
+//   X[k1 + 3 k2] = Σ_j2 e^{2πi j2 k2/8} [e^{2πi j2 k1/24} Σ_j1 x[8 j1 + j2] e^{2πi j1 k1/3}]
+// (3-point DFTs, twiddles from tw[k] = e^{2πi k/24}, 8-point FFTs as 4 x 2)
+__device__ __forceinline__ void dft3p(double2& a, double2& b, double2& c) {
+    constexpr double S3 = 0.86602540378443864676;   // sin(2π/3)
+    const double2 s = cadd(b, c), d = csub(b, c);
+    const double2 m = csub(a, cscale(s, 0.5));
+    const double2 id = cmk(-S3 * d.y, S3 * d.x);   // i sin(2π/3) (b - c)
+    a = cadd(a, s);
+    b = cadd(m, id);
+    c = csub(m, id);
+}
+__device__ __forceinline__ void dft8p(double2* y) {
+    constexpr double C2 = 0.70710678118654752440;
+    dft4p(y[0], y[2], y[4], y[6]);
+    dft4p(y[1], y[3], y[5], y[7]);
+    // odd half times e^{2πi k/8}, k = 0..3 (now at y[1], y[3], y[5], y[7])
+    y[3] = cmul(y[3], cmk(C2, C2));
+    y[5] = cmk(-y[5].y, y[5].x);
+    y[7] = cmul(y[7], cmk(-C2, C2));
+    double2 o[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        o[k] = cadd(y[2 * k], y[2 * k + 1]);
+        o[k + 4] = csub(y[2 * k], y[2 * k + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) y[k] = o[k];
+}
+template <>
+struct BubbleDft<24> {
+    __device__ static void run(double2* x, const double2* tw, int) {
+        double2 A[24];   // A[3 j2 + k1]
+#pragma unroll
+        for (int j2 = 0; j2 < 8; ++j2) {
+            double2 a = x[j2], b = x[8 + j2], c = x[16 + j2];
+            dft3p(a, b, c);
+            A[3 * j2] = a;
+            A[3 * j2 + 1] = j2 ? cmul(b, tw[j2]) : b;
+            A[3 * j2 + 2] = j2 ? cmul(c, tw[2 * j2]) : c;
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) {
+            double2 y[8];
+#pragma unroll
+            for (int j2 = 0; j2 < 8; ++j2) y[j2] = A[3 * j2 + k1];
+            dft8p(y);
+#pragma unroll
+            for (int k2 = 0; k2 < 8; ++k2) x[k1 + 3 * k2] = y[k2];
+        }
+    }
+};
+
 // Π_R(ω, Ω, R') for one R' (pp: which = 0, ph: 1; see k_bubble_nl)
 __device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, const BubbleFold& f, int nG, double T, int m,
                                             int n, int which) {
@@ -249,8 +302,8 @@ __device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, cons
     return acc;
 }
 
-// LT: L fixed at compile time (16: radix-4 FFT; others: unrolled direct DFT), or 0 for a run-time
-// L <= 32 (direct DFT).  Block: 256 threads, NB * L <= 256 needed only for full use of the block.
+// LT: L fixed at compile time (16: radix-4 FFT; 24: unrolled direct DFT), or 0 for a run-time
+// L <= 32 (direct DFT through private arrays, for small test lattices).  Block: 256 threads.
 template <int LT, int NBT>
 __global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, double2* __restrict__ ph,
                                                    const double2* __restrict__ GR, const BubbleFold* __restrict__ folds,
@@ -813,7 +866,7 @@ struct K3_AopT {
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
-template <> struct Prefetch<K3_AopT<false>> { static constexpr bool on = false; };
+template <> struct Prefetch<K3_AopT<false>> { static constexpr bool on = true; };
 struct K3_Bop {
     K3Geom g; double T; K3Chan c0, c1, c2;
     struct Bound {

@@ -13,10 +13,12 @@ CASES = {
     "nl_LG4_L2": (True, 4, 2, 1, False),
 }
 
-# lattices used by the bubble and band tests only: L = 16 takes the radix-4 FFT of the fused
-# lattice bubble, L = 6 (even, half weights at |R| = L/2) and L = 5 with LG = 10 the generic DFT
+# lattices used by the bubble, band and group tests only: L = 16 takes the radix-4 FFT of the
+# fused lattice bubble, L = 24 the 3 x 8 FFT, L = 6 (even, half weights at |R| = L/2) and L = 5
+# with LG = 10 the generic DFT
 EXTRA_CASES = {
     "nl_L16": (True, 16, 16, 0, False),
+    "nl_L24": (True, 24, 24, 0, False),
     "nl_L6": (True, 6, 6, 1, True),
     "nl_LG10_L5": (True, 10, 5, 0, False),
 }

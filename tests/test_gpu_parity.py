@@ -114,10 +114,11 @@ def test_full_iteration(case, strategy, fdga_mod):
         assert relerr(x, y) < TOL, f"{case} {strategy} iteration {it}: {relerr(x, y):.3e}"
 
 
-@pytest.mark.parametrize("case", ["nl_L16", "nl_L6", "nl_LG10_L5"])
+@pytest.mark.parametrize("case", ["nl_L16", "nl_L24", "nl_L6", "nl_LG10_L5"])
 def test_lattice_bubble_fused_dft(case, fdga_mod):
     """Lattice bubbles (real-space products and the backward DFT over R' in one kernel): the
-    radix-4 FFT at L = 16 and the direct DFT at other L, against the oracle's real-space sums."""
+    radix-4 FFT at L = 16, the 3 x 8 FFT at L = 24 and the direct DFT at other L, against the
+    oracle's real-space sums."""
     from fdga import _lib as L
     S, O = make_pair(case, seed=2)
     for hw, ow in ((L.PI0PP, po.PI0PP), (L.PI0PH, po.PI0PH), (L.PIPP, po.PIPP), (L.PIPH, po.PIPH)):
