@@ -1538,43 +1538,49 @@ int fdga_stage_ms(const fdga_ctx* c, int32_t stage, double* ms) {
     return 0;
 }
 
-// C = A B for dense column-major complex matrices through the BSE GEMM engine
-static int selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, int32_t tile, int64_t grid, const double* A,
-                          const double* B, double* C) {
-    if (M < 1 || N < 1 || K < 1 || !A || !B || !C || tile >= TILE_N || grid < 0 || grid % 8) return -1;
+// C[b] = A[b] B[b] for nbatch dense column-major complex matrices through the BSE GEMM engine
+// (tile < 0: the single-problem launch with 64 x 64 tiles)
+static int selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t nbatch, int32_t precision, int32_t tile, int64_t grid,
+                          const double* A, const double* B, double* C) {
+    if (M < 1 || N < 1 || K < 1 || nbatch < 1 || !A || !B || !C || tile >= TILE_N || grid < 0 || grid % 8) return -1;
     double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
-    if (hipMalloc((void**)&dA, na * sizeof(double2)) != hipSuccess) return -2;
-    if (hipMalloc((void**)&dB, nb * sizeof(double2)) != hipSuccess) { (void)hipFree(dA); return -2; }
-    if (hipMalloc((void**)&dC, nc * sizeof(double2)) != hipSuccess) { (void)hipFree(dA); (void)hipFree(dB); return -2; }
+    if (hipMalloc((void**)&dA, na * nbatch * sizeof(double2)) != hipSuccess) return -2;
+    if (hipMalloc((void**)&dB, nb * nbatch * sizeof(double2)) != hipSuccess) { (void)hipFree(dA); return -2; }
+    if (hipMalloc((void**)&dC, nc * nbatch * sizeof(double2)) != hipSuccess) { (void)hipFree(dA); (void)hipFree(dB); return -2; }
     int rc = 0;
-    if (hipMemcpy(dA, A, na * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dB, B, nb * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+    if (hipMemcpy(dA, A, na * nbatch * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dB, B, nb * nbatch * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
     if (!rc) {
-        hipError_t e;
+        const Dense_A a{dA, M, na};
+        const Dense_B b{dB, K, nb};
+        const Dense_C e{dC, M, nc};
+        hipError_t err;
         if (tile < 0) {
-            const int tm = cdiv(M, GBM), tn = cdiv(N, GBN);
-            e = zgemm_launch(precision, 1, 0, M, N, K, tm, tn, nullptr, 0, Dense_A{dA, M}, Dense_B{dB, K}, Dense_C{dC, M});
+            err = zgemm_launch(precision, nbatch, 0, M, N, K, cdiv(M, GBM), cdiv(N, GBN), nullptr, 0, a, b, e);
         } else {
-            const int tm = cdiv(M, tile_bm(tile)), tn = cdiv(N, tile_bn(tile));
-            const GemmSchedOne sc{M, N, K, tm, tn, 1, 8 * tm * tn, nullptr};
-            e = zgemm_run(precision, tile, grid ? (size_t)grid : (size_t)sc.nitems, 0, sc, Dense_A{dA, M}, Dense_B{dB, K},
-                          Dense_C{dC, M});
+            const GemmSchedOne sc = sched_one(M, N, K, cdiv(M, tile_bm(tile)), cdiv(N, tile_bn(tile)), nbatch, nullptr, 0);
+            err = zgemm_run(precision, tile, grid ? (size_t)grid : (size_t)sc.nitems, 0, sc, a, b, e);
         }
-        if (e != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = -3;
+        if (err != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = -3;
     }
-    if (!rc && hipMemcpy(C, dC, nc * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = -2;
+    if (!rc && hipMemcpy(C, dC, nc * nbatch * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = -2;
     (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);
     return rc;
 }
 
 int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, const double* A, const double* B, double* C) {
-    return selftest_zgemm(M, N, K, precision, -1, 0, A, B, C);
+    return selftest_zgemm(M, N, K, 1, precision, -1, 0, A, B, C);
 }
 int fdga_selftest_zgemm_tiles(int32_t M, int32_t N, int32_t K, int32_t precision, int32_t tile, int64_t grid, const double* A,
                               const double* B, double* C) {
     if (tile < 0) return -1;
-    return selftest_zgemm(M, N, K, precision, tile, grid, A, B, C);
+    return selftest_zgemm(M, N, K, 1, precision, tile, grid, A, B, C);
+}
+int fdga_selftest_zgemm_batched(int32_t M, int32_t N, int32_t K, int32_t nbatch, int32_t precision, int32_t tile, int64_t grid,
+                                const double* A, const double* B, double* C) {
+    if (tile < 0) return -1;
+    return selftest_zgemm(M, N, K, nbatch, precision, tile, grid, A, B, C);
 }
 
 // GEMM blocks (GBM x GBN output tiles over the full contraction length) of the batched (p, a, t)

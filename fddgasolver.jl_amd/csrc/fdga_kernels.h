@@ -498,11 +498,12 @@ template <class L> struct Prefetch { static constexpr bool on = true; };
 // One work item of a GEMM launch: an output tile of one batch.
 struct GemmWork { int batch, m0, n0, M, N, K; };
 
-// Work items of a launch with one problem shape.  Without a block list, item it runs batch
-// 8 (it / 8 / tiles) + it % 8, tile (it / 8) % tiles: items go round-robin to the 8 XCDs, and the
-// tiles of one batch (which share their A rows and B columns) stay on one XCD; the item count
-// covers a multiple of 8 batches, items past the last batch are idle.  A block list (built in the
-// same order, -1 = idle) selects the computed tiles instead.
+// Work items of a launch with one problem shape.  Without a block list and with at least 8
+// batches, item it runs batch 8 (it / 8 / tiles) + it % 8, tile (it / 8) % tiles: items go
+// round-robin to the 8 XCDs, and the tiles of one batch (which share their A rows and B columns)
+// stay on one XCD; the item count covers a multiple of 8 batches, items past the last batch are
+// idle.  With fewer batches the items are the tiles in order (spread over all XCDs).  A block list
+// (built in the XCD order, -1 = idle) selects the computed tiles instead.
 struct GemmSchedOne {
     int M, N, K, tiles_m, tiles_n, nbatch, nitems;
     const int* blist;
@@ -511,6 +512,8 @@ struct GemmSchedOne {
         int bid;
         if (blist) {
             bid = blist[it];
+        } else if (nbatch < 8) {
+            bid = it;
         } else {
             const int r = it >> 3;
             const int bt = (r / tiles) * 8 + (it & 7);
@@ -726,12 +729,16 @@ inline hipError_t zgemm_run(int precision, int tile, size_t grid, hipStream_t st
 // launch helper of the single-problem GEMMs (64 x 64 tiles, one workgroup per item): without a
 // block list the items cover nbatch batches of tm x tn tiles (rounded up to a multiple of 8
 // batches); with one, nlist workgroups run the listed blocks.
+inline GemmSchedOne sched_one(int M, int N, int K, int tm, int tn, int nbatch, const int* blist, size_t nlist) {
+    const size_t nitems = blist ? nlist : nbatch < 8 ? (size_t)nbatch * tm * tn : (size_t)((nbatch + 7) / 8) * 8 * tm * tn;
+    return GemmSchedOne{M, N, K, tm, tn, nbatch, (int)nitems, blist};
+}
 template <class AL, class BL, class EP>
 inline hipError_t zgemm_launch(int precision, int nbatch, hipStream_t st, int M, int N, int K, int tm, int tn,
                                const int* blist, size_t nlist, const AL& a, const BL& b, const EP& e) {
-    const size_t nitems = blist ? nlist : (size_t)((nbatch + 7) / 8) * 8 * tm * tn;
+    const GemmSchedOne sc = sched_one(M, N, K, tm, tn, nbatch, blist, nlist);
+    const size_t nitems = (size_t)sc.nitems;
     if (nitems == 0) return hipSuccess;
-    const GemmSchedOne sc{M, N, K, tm, tn, nbatch, (int)nitems, blist};
     if (precision == 1) zgemm_tiles_t<true, 2, 2><<<(unsigned)nitems, 256, 0, st>>>(sc, a, b, e);
     else zgemm_tiles_t<false, 2, 2><<<(unsigned)nitems, 256, 0, st>>>(sc, a, b, e);
     return hipGetLastError();
@@ -989,20 +996,20 @@ __global__ void k_post_t(double2* __restrict__ t, const double2* __restrict__ a,
     if (i < n) t[i] = cscale(cadd(t[i], a[i]), 0.5);
 }
 
-// dense column-major operands (library self-test of the GEMM engine)
+// dense column-major operands, batch b at offset b * stride (library self-test of the GEMM engine)
 struct Dense_A {
-    const double2* A; int lda;
-    __device__ Dense_A bind(int) const { return *this; }
+    const double2* A; int lda; size_t stride;
+    __device__ Dense_A bind(int b) const { Dense_A r = *this; r.A += stride * b; return r; }
     __device__ double2 operator()(int i, int k) const { return A[(size_t)i + (size_t)lda * k]; }
 };
 struct Dense_B {
-    const double2* B; int ldb;
-    __device__ Dense_B bind(int) const { return *this; }
+    const double2* B; int ldb; size_t stride;
+    __device__ Dense_B bind(int b) const { Dense_B r = *this; r.B += stride * b; return r; }
     __device__ double2 operator()(int k, int j) const { return B[(size_t)k + (size_t)ldb * j]; }
 };
 struct Dense_C {
-    double2* C; int ldc;
-    __device__ Dense_C bind(int) const { return *this; }
+    double2* C; int ldc; size_t stride;
+    __device__ Dense_C bind(int b) const { Dense_C r = *this; r.C += stride * b; return r; }
     __device__ void operator()(int i, int j, double2 v) const { C[(size_t)i + (size_t)ldc * j] = v; }
 };
 

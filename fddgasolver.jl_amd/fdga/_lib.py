@@ -80,6 +80,7 @@ SIGNATURES = {
     "fdga_set_stage_timing": (ctypes.c_int, [_vp, _i32]),
     "fdga_selftest_zgemm": (ctypes.c_int, [_i32, _i32, _i32, _i32, _dp, _dp, _dp]),
     "fdga_selftest_zgemm_tiles": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32, _i64, _dp, _dp, _dp]),
+    "fdga_selftest_zgemm_batched": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32, _i32, _i64, _dp, _dp, _dp]),
     "fdga_group_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), _i32]),
     "fdga_group_destroy": (ctypes.c_int, [_vp]),
     "fdga_group_last_error": (ctypes.c_char_p, [_vp]),

@@ -120,6 +120,10 @@ int fdga_selftest_zgemm(int32_t M, int32_t N, int32_t K, int32_t precision, cons
  * otherwise a multiple of 8, the workgroups looping over the tiles).  No counterpart in the reference. */
 int fdga_selftest_zgemm_tiles(int32_t M, int32_t N, int32_t K, int32_t precision, int32_t tile, int64_t grid,
                               const double* A, const double* B, double* C);
+/* The same for `nbatch` independent products, A, B and C holding the batches one after the other
+ * (the BSE contractions are batches of small products).  No counterpart in the reference. */
+int fdga_selftest_zgemm_batched(int32_t M, int32_t N, int32_t K, int32_t nbatch, int32_t precision, int32_t tile,
+                                int64_t grid, const double* A, const double* B, double* C);
 
 /* ---- groups of solver contexts: a parameter sweep on one device ----------------------------
  * The reference solves one parameter point per solver; a sweep calls solve! per point.  A group

@@ -63,3 +63,26 @@ def test_zgemm_tile_shapes_and_persistent_grids(M, N, K, precision, tol):
             np.testing.assert_array_equal(c, base, err_msg=f"tile {tile} grid {grid}")
             err = np.max(np.abs(c.reshape(M, N, order="F") - ref)) / np.max(np.abs(ref))
             assert err < tol, (tile, grid, err)
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-13), (1, 5e-6)])
+def test_zgemm_batched(precision, tol):
+    """A batch of independent products (the shape of the BSE contractions: many small products),
+    fewer and more than 8 batches, every tile shape."""
+    import fdga
+    lib = fdga.load()
+    M = N = K = 40
+    for nbatch in (3, 19):
+        rng = np.random.default_rng(nbatch)
+        A = rng.standard_normal((nbatch, K, M)) + 1j * rng.standard_normal((nbatch, K, M))   # [b, k, i]: column-major A[b]
+        B = rng.standard_normal((nbatch, N, K)) + 1j * rng.standard_normal((nbatch, N, K))
+        ref = np.einsum("bki,bjk->bji", A, B)   # C[b] column-major: [b, j, i]
+        for tile in range(4):
+            c = np.zeros(nbatch * M * N, dtype=np.complex128)
+            a = np.ascontiguousarray(A.ravel())
+            b = np.ascontiguousarray(B.ravel())
+            rc = lib.fdga_selftest_zgemm_batched(M, N, K, nbatch, precision, tile, 0, _dp(a.view(np.float64)),
+                                                 _dp(b.view(np.float64)), _dp(c.view(np.float64)))
+            assert rc == 0
+            err = np.max(np.abs(c.reshape(nbatch, N, M) - ref)) / np.max(np.abs(ref))
+            assert err < tol, (nbatch, tile, err)
