@@ -70,7 +70,7 @@ struct fdga_ctx {
     int tabs_nonzero = 0;
     double2 U{};
     DevChan F[3], Fb[3], FL[3], R0[3];
-    double2 *cache[10] = {};
+    double2 *cache[12] = {};   // build_K3_cache! caches; [10], [11] the a and t Γ caches transposed
     double2 *R[3] = {nullptr, nullptr, nullptr};                    // BSE_K3! right operands [ω, νp, Ω, P]
     double2 *csum[3] = {nullptr, nullptr, nullptr};                 // their column sums [νp, Ω, P]
     double2 *tmpG1 = nullptr, *tmpG2 = nullptr, *tmpG3 = nullptr;   // [NG, NkG] scratch
@@ -766,7 +766,7 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) 
         rc[s].Pi = pi_of(c, ch, false);
         rc[s].Pi0 = pi_of(c, ch, true);
         rc[s].F0r = c->cache[idx_f0(ch)];
-        rc[s].Gam = c->cache[(ch == PCH) ? 0 : idx_gamma(ch)];
+        rc[s].Gam = c->cache[(ch == PCH) ? 0 : (ch == ACH) ? 10 : 11];
         rc[s].Lr = c->FL[ch].K3;
         rc[s].La = c->FL[ACH].K3;
         rc[s].R = c->R[ch];
@@ -1241,10 +1241,12 @@ int fdga_bubbles(fdga_ctx* c) { return bubbles_into(c, c->Pipp, c->Piph, c->G); 
 
 int fdga_build_k3_cache(fdga_ctx* c) {
     CachePtrs cp;
-    for (int k = 0; k < 10; ++k) cp.c[k] = c->cache[k];
+    for (int k = 0; k < 12; ++k) cp.c[k] = c->cache[k];
     dim3 g(cdiv(c->NV3, 64), c->NV3, c->NW3);
     k_cache1<<<g, 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
     k_cache2<<<g, 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f, c->NP == 1);
+    LAUNCHCHK(c);
+    k_cache_transpose<<<dim3(cdiv(c->NV3, 32), cdiv(c->NV3, 32), 2 * c->NW3), 256, 0, c->run>>>(cp, c->NV3);
     LAUNCHCHK(c);
     return 0;
 }

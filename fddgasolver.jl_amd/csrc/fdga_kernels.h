@@ -387,7 +387,9 @@ __global__ __launch_bounds__(256) void k_pavg(double2* __restrict__ A, const dou
 //   c[0] Γpx [ω, νp]   c[1] F0p [ω, νp]   c[2] F0a [ω, νp]   c[3] F0t(d) [ω, νp]
 //   c[4] Γpp [ν, ω]    c[5] Γa  [ν, ω]    c[6] Γt(d) [ν, ω]
 //   c[7] Fbase_p [ν, ω]  c[8] Fbase_a [ν, ω]  c[9] Fbase_t(d) [ν, ω]
-struct CachePtrs { double2* c[10]; };
+// c[0..9] the caches of build_K3_cache! (see k_cache2); c[10], c[11] Γa and Γt (t: 2 Γt - Γa) again,
+// transposed to [ω, ν, Ω] for the right operands of BSE_K3!
+struct CachePtrs { double2* c[12]; };
 
 // loop 1 of build_K3_cache! at (Ω, ω, νp): Γpx and the F0 caches
 __global__ __launch_bounds__(64) void k_cache1(CachePtrs out, VtxV F, int n3b, int n3f) {
@@ -439,6 +441,26 @@ __global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, i
     out.c[7][o] = fbp;
     out.c[8][o] = fba;
     out.c[9][o] = csub(cscale(fbt, 2.0), fba);
+}
+
+// per-Ω transpose of the Γa and Γt caches ([ν, ω, Ω] -> [ω, ν, Ω]) through 32 x 32 LDS tiles
+// (coalesced reads and writes); blockIdx.z = 2 iΩ + which
+__global__ __launch_bounds__(256) void k_cache_transpose(CachePtrs cp, int NV) {
+    __shared__ double2 t[32][33];
+    const int which = blockIdx.z & 1;
+    const size_t base = (size_t)NV * NV * (blockIdx.z >> 1);
+    const double2* in = (which ? cp.c[6] : cp.c[5]) + base;
+    double2* out = (which ? cp.c[11] : cp.c[10]) + base;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + tx, j = j0 + r;
+        if (i < NV && j < NV) t[r][tx] = in[(size_t)i + (size_t)NV * j];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int j = j0 + tx, i = i0 + r;
+        if (i < NV && j < NV) out[(size_t)j + (size_t)NV * i] = t[tx][r];
+    }
 }
 
 // materialise cache_F*[Ω, ν, ω, P] in the reference layout (ABI read-back only)
@@ -942,7 +964,7 @@ struct K3RChan {
     const double2* Pi;
     const double2* Pi0;
     const double2* F0r;   // [ω, νp, Ω]
-    const double2* Gam;   // p: Γpx [ω, νp, Ω]; a, t: Γr [ν, ω, Ω] read transposed
+    const double2* Gam;   // p: Γpx [ω, νp, Ω]; a, t: Γr transposed, [ω, ν, Ω]
     const double2* Lr;    // FL.γr.K3 [ν, νp, Ω, P]
     const double2* La;    // FL.γa.K3
     double2* R;           // [ω, νp, Ω, P]
@@ -960,7 +982,7 @@ __device__ __forceinline__ void k3_R_column(const K3Geom& g, const K3RChan& c, i
         const double2 P1 = c.Pi[pi], P0 = c.Pi0[pi];
         const size_t kj = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW);
         const double2 f0 = c.F0r[kj];
-        const double2 gm = (CH == PCH) ? c.Gam[kj] : c.Gam[(size_t)j + (size_t)NV * ((size_t)k + (size_t)NV * iW)];
+        const double2 gm = c.Gam[kj];
         double2 x = cscale(cadd(cmul(csub(P1, P0), f0), cmul(P1, gm)), c.s1);
         const int w_ = crossing(m, w, CH);
         if (in_f(w_, g.n3f)) {
