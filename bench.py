@@ -14,9 +14,9 @@ fixed point; the max relative error of Σ, χ_sp, χ_ch and K3 between the two i
 (`config.value_path` says which).
 
 Prints one JSON line.  `roofline` is for the BSE contraction kernel of the headline path
-(BSE_K3! + BSE_L_K3!, batched complex GEMM on MFMA): flops per launch = 8 * 64 * 64 * Nν3 per
-computed 64 x 64 output tile (all tiles, 3 NΩ3 (Nν3/64)^2, with identity groups; the tiles that
-hold a class representative under init_sym_grp!) over the measured average launch time.
+(BSE_K3! + BSE_L_K3!, batched complex GEMM on MFMA, tile shapes chosen by the autotuner): flops per
+launch = 8 * BM * BN * Nν3 per computed BM x BN output tile (all tiles with identity groups; the
+tiles that hold a class representative under init_sym_grp!) over the measured average launch time.
 `bse_tflops_dense_equiv` divides the dense count 8 NΩ3 Nν3^3 per channel by the same time.  `cpu_baseline` times the CPU restatement
 (oracle/, one thread) on one iteration of the same workload.
 """
@@ -358,14 +358,16 @@ def main():
         obs = observables(S)
         if x64 is None:
             x64 = fdga.flatten(S)
+        S.autotune(reps=5)   # tile shapes of the two contraction launches (results do not depend on them)
+        tiles = S.gemm_tiles()
+        log(f"bench[{prec}]: autotuned tile shapes (BSE_L_K3!, BSE_K3!): {tiles}")
         ms_step, st = time_path(S, args, names)
         log(f"bench[{prec}]: {ms_step:.3f} ms/iteration; stages " + ", ".join(f"{n}={v:.3f}" for n, v in st.items()))
         # the two batched (p, a, t) GEMM kernels of one fdPA iteration, timed alone with HIP events
         t_bse = (st["gemm_L_K3"] + st["gemm_K3"]) * 1e-3
-        # flops the GEMM launches execute: computed 64 x 64 tiles over the full contraction length
-        nblk = S.lib.fdga_gemm_blocks(S.h, 1) + S.lib.fdga_gemm_blocks(S.h, 0)
-        f_exec = nblk * 8.0 * 64 * 64 * (2 * w["nK3"][1])
-        paths[prec] = dict(ms=ms_step, st=st, obs=obs, it=it, res=res, gemm_avg_s=t_bse / 2.0,
+        # flops the GEMM launches execute: computed output tiles over the full contraction length
+        f_exec = S.gemm_flops(True) + S.gemm_flops(False)
+        paths[prec] = dict(ms=ms_step, st=st, obs=obs, it=it, res=res, gemm_avg_s=t_bse / 2.0, tiles=tiles,
                            f_exec_launch=f_exec / 2.0, bse_tflops=f_exec / t_bse / 1e12,
                            bse_tflops_dense_equiv=6 * f_gemm / t_bse / 1e12)
         if prec == "fp64" and not args.no_sym and not args.no_dense:
@@ -407,10 +409,11 @@ def main():
                    "fp32_mfma": {"iters_per_s": 1000.0 / paths["fp32"]["ms"], "bse_tflops": paths["fp32"]["bse_tflops"],
                                  "bse_tflops_dense_equiv": paths["fp32"]["bse_tflops_dense_equiv"],
                                  "stage_ms": paths["fp32"]["st"], "anderson_iterations": paths["fp32"]["it"],
-                                 "max_relerr_vs_fp64": errs},
+                                 "gemm_tiles_L_K3_K3": paths["fp32"]["tiles"], "max_relerr_vs_fp64": errs},
                    "fp64": {"iters_per_s": 1000.0 / paths["fp64"]["ms"], "bse_tflops": paths["fp64"]["bse_tflops"],
                             "bse_tflops_dense_equiv": paths["fp64"]["bse_tflops_dense_equiv"],
-                            "stage_ms": paths["fp64"]["st"], "anderson_iterations": paths["fp64"]["it"]},
+                            "stage_ms": paths["fp64"]["st"], "anderson_iterations": paths["fp64"]["it"],
+                            "gemm_tiles_L_K3_K3": paths["fp64"]["tiles"]},
                    "speedup_fp32_over_fp64": paths["fp64"]["ms"] / paths["fp32"]["ms"]},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,

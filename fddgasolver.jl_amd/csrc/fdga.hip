@@ -33,7 +33,7 @@ struct SymDev {
     int* off = nullptr;
     int* mem = nullptr;
     int ncls = 0;
-    std::vector<int> tiles;   // K3 groups: GEMM blocks (iΩ * tiles + tile) holding a representative
+    std::vector<int> tiles[TILE_N];   // K3 groups, per tile shape: GEMM blocks (iΩ * tiles + tile) holding a representative
     int* repop = nullptr;     // K3 groups: per element (representative << 2) | op
 };
 // groups of init_sym_grp! (src/ParquetSolver.jl:203-231): SGΣ; SGpp[1] = SGph[1] (same generator);
@@ -83,9 +83,10 @@ struct fdga_ctx {
     SymDev sg[SG_N];                                                // installed symmetry groups
     // GEMM block lists under the K3 groups: [0..3] one channel of SG_PP3, SG_PH3, SG_PPL3, SG_PHL3;
     // [4] BSE_L_K3! and [5] BSE_K3! over the channels (p, a, t) in one launch
-    int* blist[6] = {};
-    int blist_n[6] = {};      // workgroups of the list (computed tiles and idle padding)
-    int blist_tiles[6] = {};  // computed tiles
+    int* blist[6][TILE_N] = {};
+    int blist_n[6][TILE_N] = {};      // workgroups of the list (computed tiles and idle padding)
+    int blist_tiles[6][TILE_N] = {};  // computed tiles
+    int ktile[2] = {TILE_64x64, TILE_64x64};   // tile shape of the BSE_L_K3! / BSE_K3! launches
     int* finmap = nullptr;    // k_k3_finish chunk map (XCD-paired Ω blocks)
     int finmap_n = 0, fin_nchunk = 0;
     double2* k3raw = nullptr;                                       // raw t-channel K3 under groups [n3]
@@ -133,12 +134,13 @@ void free_sym(SymDev& g) {
 }
 
 void free_blists(fdga_ctx* c) {
-    for (int k = 0; k < 6; ++k) {
-        if (c->blist[k]) (void)hipFree(c->blist[k]);
-        c->blist[k] = nullptr;
-        c->blist_n[k] = 0;
-        c->blist_tiles[k] = 0;
-    }
+    for (int t = 0; t < TILE_N; ++t)
+        for (int k = 0; k < 6; ++k) {
+            if (c->blist[k][t]) (void)hipFree(c->blist[k][t]);
+            c->blist[k][t] = nullptr;
+            c->blist_n[k][t] = 0;
+            c->blist_tiles[k][t] = 0;
+        }
     if (c->finmap) (void)hipFree(c->finmap);
     c->finmap = nullptr;
     c->finmap_n = 0;
@@ -648,18 +650,19 @@ static SymCls sym_k3(const fdga_ctx* c, int ch, bool left) {
     return symcls(c, ch == PCH ? (left ? SG_PPL3 : SG_PP3) : (left ? SG_PHL3 : SG_PH3));
 }
 
-// GEMM block list of a K3 launch over `n` channels: the precomputed list when every slot has a
-// group and the channels are one channel or (p, a, t); null (all blocks) otherwise
-static const int* k3_blocks(const fdga_ctx* c, const int* chans, int n, bool left, size_t& nblocks, size_t* ntiles = nullptr) {
+// GEMM block list of a K3 launch over `n` channels with tile shape `tile`: the precomputed list when
+// every slot has a group and the channels are one channel or (p, a, t); null (all blocks) otherwise
+static const int* k3_blocks(const fdga_ctx* c, const int* chans, int n, bool left, int tile, size_t& nblocks,
+                            size_t* ntiles = nullptr) {
     for (int s = 0; s < n; ++s)
         if (!sym_k3(c, chans[s], left).ncls) return nullptr;
     int k = -1;
     if (n == 1) k = (chans[0] == PCH ? 0 : 1) + (left ? 2 : 0);
     else if (n == 3 && chans[0] == PCH && chans[1] == ACH && chans[2] == TCH) k = left ? 4 : 5;
-    if (k < 0 || !c->blist[k]) return nullptr;
-    nblocks = (size_t)c->blist_n[k];
-    if (ntiles) *ntiles = (size_t)c->blist_tiles[k];
-    return c->blist[k];
+    if (k < 0 || !c->blist[k][tile]) return nullptr;
+    nblocks = (size_t)c->blist_n[k][tile];
+    if (ntiles) *ntiles = (size_t)c->blist_tiles[k][tile];
+    return c->blist[k][tile];
 }
 
 // Finish of a K3 launch: with symmetry groups the class fill and the t post-processing in one
@@ -727,16 +730,22 @@ static LK3Set lk3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi
     const K3Geom g = k3geom(c);
     return LK3Set{LK3_Aop{g, cc[0], cc[1], cc[2]}, LK3_Bop{g, cc[0], cc[1], cc[2]}, LK3_Ep{g, cc[0], cc[1], cc[2]}};
 }
+// one K3-class GEMM launch of `n` channels with the context's tile shape for it (which: 0 BSE_L_K3!,
+// 1 BSE_K3!), over the symmetry block list when there is one
+template <class AL, class BL, class EP>
+static hipError_t k3_gemm(fdga_ctx* c, const int* chans, int n, int which, const AL& a, const BL& b, const EP& e) {
+    const int NV = c->NV3, tile = c->ktile[which];
+    size_t nl = 0;
+    const int* bl = k3_blocks(c, chans, n, which == 0, tile, nl);
+    const GemmSchedOne sc = sched_one(NV, NV, NV, cdiv(NV, tile_bm(tile)), cdiv(NV, tile_bn(tile)), n * c->NW3 * c->NP, bl, nl);
+    return zgemm_run(c->d.precision, tile, (size_t)sc.nitems, c->run, sc, a, b, e);
+}
 static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
-    const int NV = c->NV3, NW = c->NW3;
-    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
     const bool sym = k3_sym(c, chans, n, true);
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const LK3Set ls = lk3_set(c, chans, n, sym, t_epi);
-    size_t nl = 0;
-    const int* bl = k3_blocks(c, chans, n, true, nl);
     kernel_begin(c);
-    if (zgemm_launch(c->d.precision, n * NW * c->NP, c->run, NV, NV, NV, tm, tn, bl, nl, ls.a, ls.b, ls.e) != hipSuccess)
+    if (k3_gemm(c, chans, n, 0, ls.a, ls.b, ls.e) != hipSuccess)
         return fail(c, "BSE_L_K3 GEMM launch failed", -3);
     kernel_end(c, 9);
     return k3_finish(c, c->FL, chans, n, true, sym, t_epi);
@@ -797,20 +806,14 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n) {
 }
 // BSE_K3! for `n` channels: right operands and column sums, then one GEMM launch
 static int launch_k3(fdga_ctx* c, const int* chans, int n) {
-    const int NV = c->NV3, NW = c->NW3;
-    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN);
     const bool sym = k3_sym(c, chans, n, false);
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const K3Set ks = k3_set(c, chans, n, sym, t_epi);
     if (k3_rhs(c, ks, n)) return -3;
     const K3_Bop b = ks.b();
     const K3_Ep e = ks.e();
-    size_t nl = 0;
-    const int* bl = k3_blocks(c, chans, n, false, nl);
     kernel_begin(c);
-    hipError_t ge;
-    if (c->NP == 1) ge = zgemm_launch(c->d.precision, n * NW, c->run, NV, NV, NV, tm, tn, bl, nl, ks.a<true>(), b, e);
-    else ge = zgemm_launch(c->d.precision, n * NW * c->NP, c->run, NV, NV, NV, tm, tn, bl, nl, ks.a<false>(), b, e);
+    const hipError_t ge = c->NP == 1 ? k3_gemm(c, chans, n, 1, ks.a<true>(), b, e) : k3_gemm(c, chans, n, 1, ks.a<false>(), b, e);
     if (ge != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
     return k3_finish(c, c->Fb, chans, n, false, sym, t_epi);
@@ -1154,10 +1157,9 @@ int fdga_init_sym_grp(fdga_ctx* c) {
     rc |= upload_sym(c, SG_0PP2, pp2);
     rc |= upload_sym(c, SG_0PH2, ph2);
     if (rc) return -2;
-    // GEMM blocks holding a representative (block = iΩ * tiles + ti + tm * tj, rows ν, columns ν')
-    const int tm = cdiv(NV, GBM), tn = cdiv(NV, GBN), tiles = tm * tn;
+    // GEMM blocks holding a representative (block = iΩ * tiles + ti + tm * tj, rows ν, columns ν'),
+    // for every tile shape
     for (int id : {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3}) {
-        std::vector<char> hit((size_t)NW * tiles, 0);
         SymDev& g = c->sg[id];
         std::vector<int> off(g.ncls + 1), mem;
         HIPCHK(c, hipMemcpy(off.data(), g.off, off.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -1166,36 +1168,46 @@ int fdga_init_sym_grp(fdga_ctx* c) {
         std::vector<int> repop(c->n3);
         for (int k = 0; k < g.ncls; ++k) {
             const int64_t dv = mem[off[k]] >> 2;
-            const int iv = (int)(dv % NV), ivp = (int)((dv / NV) % NV), iW = (int)(dv / (NV * NV));
-            hit[(size_t)iW * tiles + iv / GBM + tm * (ivp / GBN)] = 1;
             for (int q = off[k]; q < off[k + 1]; ++q) repop[mem[q] >> 2] = (int)((dv << 2) | (mem[q] & 3));
         }
         HIPCHK(c, hipMalloc((void**)&g.repop, repop.size() * sizeof(int)));
         HIPCHK(c, hipMemcpy(g.repop, repop.data(), repop.size() * sizeof(int), hipMemcpyHostToDevice));
-        g.tiles.clear();
-        for (size_t b = 0; b < hit.size(); ++b)
-            if (hit[b]) g.tiles.push_back((int)b);
+        for (int t = 0; t < TILE_N; ++t) {
+            const int tm = cdiv(NV, tile_bm(t)), tn = cdiv(NV, tile_bn(t)), tiles = tm * tn;
+            std::vector<char> hit((size_t)NW * tiles, 0);
+            for (int k = 0; k < g.ncls; ++k) {
+                const int64_t dv = mem[off[k]] >> 2;
+                const int iv = (int)(dv % NV), ivp = (int)((dv / NV) % NV), iW = (int)(dv / (NV * NV));
+                hit[(size_t)iW * tiles + iv / tile_bm(t) + tm * (ivp / tile_bn(t))] = 1;
+            }
+            g.tiles[t].clear();
+            for (size_t b = 0; b < hit.size(); ++b)
+                if (hit[b]) g.tiles[t].push_back((int)b);
+        }
     }
     free_blists(c);
     if (!c->k3raw && dalloc(c, &c->k3raw, c->n3)) return -2;
     // block lists in XCD order: all tiles of one batch (channel slot, Ω) on one XCD, batch b on XCD b % 8
-    auto upload_list = [&](int k, const std::vector<int>& v) -> int {
-        const std::vector<int> o = xcd_order(v, [tiles](int b) { return b / tiles; });
-        HIPCHK(c, hipMalloc((void**)&c->blist[k], std::max<size_t>(1, o.size()) * sizeof(int)));
-        HIPCHK(c, hipMemcpy(c->blist[k], o.data(), o.size() * sizeof(int), hipMemcpyHostToDevice));
-        c->blist_n[k] = (int)o.size();
-        c->blist_tiles[k] = (int)v.size();
-        return 0;
-    };
-    const int ids1[4] = {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3};
-    for (int k = 0; k < 4; ++k) rc |= upload_list(k, c->sg[ids1[k]].tiles);
-    for (int left = 1; left >= 0; --left) {
-        // slots (p, a, t) of the batched launches
-        const int ids3[3] = {left ? SG_PPL3 : SG_PP3, left ? SG_PHL3 : SG_PH3, left ? SG_PHL3 : SG_PH3};
-        std::vector<int> all;
-        for (int sl = 0; sl < 3; ++sl)
-            for (int b : c->sg[ids3[sl]].tiles) all.push_back(b + sl * (int)NW * tiles);
-        rc |= upload_list(left ? 4 : 5, all);
+    for (int t = 0; t < TILE_N; ++t) {
+        const int tiles = cdiv(NV, tile_bm(t)) * cdiv(NV, tile_bn(t));
+        auto upload_list = [&](int k, const std::vector<int>& v) -> int {
+            const std::vector<int> o = xcd_order(v, [tiles](int b) { return b / tiles; });
+            HIPCHK(c, hipMalloc((void**)&c->blist[k][t], std::max<size_t>(1, o.size()) * sizeof(int)));
+            HIPCHK(c, hipMemcpy(c->blist[k][t], o.data(), o.size() * sizeof(int), hipMemcpyHostToDevice));
+            c->blist_n[k][t] = (int)o.size();
+            c->blist_tiles[k][t] = (int)v.size();
+            return 0;
+        };
+        const int ids1[4] = {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3};
+        for (int k = 0; k < 4; ++k) rc |= upload_list(k, c->sg[ids1[k]].tiles[t]);
+        for (int left = 1; left >= 0; --left) {
+            // slots (p, a, t) of the batched launches
+            const int ids3[3] = {left ? SG_PPL3 : SG_PP3, left ? SG_PHL3 : SG_PH3, left ? SG_PHL3 : SG_PH3};
+            std::vector<int> all;
+            for (int sl = 0; sl < 3; ++sl)
+                for (int b : c->sg[ids3[sl]].tiles[t]) all.push_back(b + sl * (int)NW * tiles);
+            rc |= upload_list(left ? 4 : 5, all);
+        }
     }
     // k_k3_finish chunks: 256 elements of one Ω block; the blocks Ω and -Ω on one XCD
     {
@@ -1590,9 +1602,81 @@ int fdga_selftest_zgemm_batched(int32_t M, int32_t N, int32_t K, int32_t nbatch,
 int64_t fdga_gemm_blocks(const fdga_ctx* c, int32_t left) {
     if (!c) return -1;
     const int chans[3] = {PCH, ACH, TCH};
-    size_t nl = 0, nt = (size_t)3 * c->NW3 * c->NP * cdiv(c->NV3, GBM) * cdiv(c->NV3, GBN);
-    (void)k3_blocks(c, chans, 3, left != 0, nl, &nt);
+    const int tile = c->ktile[left ? 0 : 1];
+    size_t nl = 0, nt = (size_t)3 * c->NW3 * c->NP * cdiv(c->NV3, tile_bm(tile)) * cdiv(c->NV3, tile_bn(tile));
+    (void)k3_blocks(c, chans, 3, left != 0, tile, nl, &nt);
     return (int64_t)nt;
+}
+// executed flops of that launch: 8 BM BN Nν3 per computed tile of the current tile shape
+double fdga_gemm_flops(const fdga_ctx* c, int32_t left) {
+    if (!c) return -1.0;
+    const int tile = c->ktile[left ? 0 : 1];
+    return (double)fdga_gemm_blocks(c, left) * 8.0 * tile_bm(tile) * tile_bn(tile) * c->NV3;
+}
+// tile shape of the BSE_L_K3! (left = 1) or BSE_K3! (left = 0) launches
+int fdga_gemm_tile(const fdga_ctx* c, int32_t left) { return c ? c->ktile[left ? 0 : 1] : -1; }
+int fdga_set_gemm_tile(fdga_ctx* c, int32_t left, int32_t tile) {
+    if (!c) return -1;
+    if (tile < 0 || tile >= TILE_N) return fail(c, "unknown tile shape");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->ktile[left ? 0 : 1] = tile;
+    graphs_invalidate(c);
+    return 0;
+}
+
+// Times every tile shape of the two contraction launches (p, a, t batched) on the current operands,
+// `reps` launches each, and keeps the fastest.  The caches and right operands are rebuilt first; the
+// BSE_L_K3! timing writes into the right-operand scratch instead of FL, the BSE_K3! timing into
+// Fbuff (overwritten by every iteration), so the solver state is left as it was.
+int fdga_autotune(fdga_ctx* c, int32_t reps) {
+    if (!c || reps < 1) return -1;
+    const int all[3] = {PCH, ACH, TCH};
+    c->run = c->st;
+    if (fdga_build_k3_cache(c)) return -3;
+    const bool symL = k3_sym(c, all, 3, true), symK = k3_sym(c, all, 3, false);
+    LK3Set ls = lk3_set(c, all, 3, symL, false);
+    ls.e.c0.out = c->R[0];   // scratch outputs
+    ls.e.c1.out = c->R[1];
+    ls.e.c2.out = c->R[2];
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    auto timeit = [&](auto&& launch) -> float {
+        (void)launch();
+        (void)hipEventRecord(a, c->st);
+        for (int r = 0; r < reps; ++r) (void)launch();
+        (void)hipEventRecord(b, c->st);
+        (void)hipEventSynchronize(b);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        return ms;
+    };
+    int rc = 0;
+    float best = 1e30f;
+    int bt = c->ktile[0];
+    for (int t = 0; t < TILE_N; ++t) {
+        c->ktile[0] = t;
+        const float ms = timeit([&] { return k3_gemm(c, all, 3, 0, ls.a, ls.b, ls.e); });
+        if (ms < best) { best = ms; bt = t; }
+    }
+    c->ktile[0] = bt;
+    const K3Set ks = k3_set(c, all, 3, symK, false);
+    rc |= k3_rhs(c, ks, 3);
+    best = 1e30f;
+    bt = c->ktile[1];
+    for (int t = 0; t < TILE_N; ++t) {
+        c->ktile[1] = t;
+        const float ms = c->NP == 1 ? timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<true>(), ks.b(), ks.e()); })
+                                    : timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<false>(), ks.b(), ks.e()); });
+        if (ms < best) { best = ms; bt = t; }
+    }
+    c->ktile[1] = bt;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (hipGetLastError() != hipSuccess || rc) return fail(c, "autotune launch failed", -3);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    graphs_invalidate(c);
+    return 0;
 }
 
 // replay fdga_iterate(strategy, 1) from captured hipGraphs (1) or launch eagerly (0)

@@ -57,6 +57,10 @@ SIGNATURES = {
     "fdga_init": (ctypes.c_int, [_vp]),
     "fdga_init_sym_grp": (ctypes.c_int, [_vp]),
     "fdga_gemm_blocks": (_i64, [_vp, _i32]),
+    "fdga_gemm_flops": (ctypes.c_double, [_vp, _i32]),
+    "fdga_gemm_tile": (ctypes.c_int, [_vp, _i32]),
+    "fdga_set_gemm_tile": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_autotune": (ctypes.c_int, [_vp, _i32]),
     "fdga_set_graph": (ctypes.c_int, [_vp, _i32]),
     "fdga_reset_sym_grp": (ctypes.c_int, [_vp]),
     "fdga_dyson": (ctypes.c_int, [_vp]),

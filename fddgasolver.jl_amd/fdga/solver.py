@@ -135,6 +135,24 @@ class Solver:
         """Replay each fixed-point iteration from a captured hipGraph (True) or launch eagerly."""
         self._check(self.lib.fdga_set_graph(self.h, int(on)))
 
+    _TILE_NAMES = ("64x64", "32x32", "64x32", "32x64")
+
+    def autotune(self, reps=3):
+        """Time every tile shape of the two contraction launches and keep the fastest (the state is
+        left as it was; results do not depend on the tile shape)."""
+        self._check(self.lib.fdga_autotune(self.h, reps))
+
+    def gemm_tiles(self):
+        """Tile shapes of the BSE_L_K3! and BSE_K3! launches."""
+        return tuple(self._TILE_NAMES[self.lib.fdga_gemm_tile(self.h, left)] for left in (1, 0))
+
+    def set_gemm_tile(self, left, tile):
+        self._check(self.lib.fdga_set_gemm_tile(self.h, int(left), self._TILE_NAMES.index(tile)))
+
+    def gemm_flops(self, left):
+        """Executed flops of one batched BSE_L_K3! (left) or BSE_K3! launch."""
+        return self.lib.fdga_gemm_flops(self.h, int(left))
+
     def set_stage_timing(self, on):
         self._check(self.lib.fdga_set_stage_timing(self.h, int(on)))
 

@@ -39,6 +39,8 @@ typedef struct fdga_dims {
 } fdga_dims;
 
 enum { FDGA_FP64 = 0, FDGA_FP32 = 1 };
+/* tile shapes of the contraction GEMMs (workgroups of 2x2, 1x1, 2x1, 1x2 waves of 32x32 each) */
+enum { FDGA_TILE_64x64 = 0, FDGA_TILE_32x32 = 1, FDGA_TILE_64x32 = 2, FDGA_TILE_32x64 = 3 };
 enum { FDGA_PCH = 0, FDGA_TCH = 1, FDGA_ACH = 2 };
 enum { FDGA_SCPA = 0, FDGA_FDPA = 1 };
 
@@ -111,6 +113,16 @@ int fdga_set_graph(fdga_ctx* ctx, int32_t on);
  * BSE_L_K3! (left = 1) or BSE_K3! (left = 0) GEMM launch computes; fewer than the dense count
  * when symmetry groups are installed.  No counterpart in the reference (measurement). */
 int64_t fdga_gemm_blocks(const fdga_ctx* ctx, int32_t left);
+/* executed flops of that launch (8 BM BN Nν3 per computed BM x BN tile), its tile shape (FDGA_TILE_*),
+ * and a manual choice of the shape.  No counterpart in the reference (measurement / execution option). */
+double  fdga_gemm_flops(const fdga_ctx* ctx, int32_t left);
+int     fdga_gemm_tile(const fdga_ctx* ctx, int32_t left);
+int     fdga_set_gemm_tile(fdga_ctx* ctx, int32_t left, int32_t tile);
+/* autotuner: times every tile shape of the batched (p, a, t) BSE_L_K3! and BSE_K3! launches on the
+ * current operands (`reps` launches each) and keeps the fastest; the solver state is left as it was
+ * (the launches write into scratch).  Results do not depend on the tile shape.  No counterpart in
+ * the reference. */
+int     fdga_autotune(fdga_ctx* ctx, int32_t reps);
 
 /* Self-test of the batched complex GEMM engine used by BSE_L_K3!/BSE_K3!: C = A B for dense
  * column-major M x K and K x N ComplexF64 matrices on the FP64 (precision 0) or FP32 (1) MFMA path.  No counterpart in the reference; it exists
@@ -135,7 +147,6 @@ int fdga_selftest_zgemm_batched(int32_t M, int32_t N, int32_t K, int32_t nbatch,
  * iterated alone while in a group, and the group must be destroyed before its members.  No
  * counterpart in the reference. */
 typedef struct fdga_group fdga_group;
-enum { FDGA_TILE_64x64 = 0, FDGA_TILE_32x32 = 1, FDGA_TILE_64x32 = 2, FDGA_TILE_32x64 = 3 };
 int         fdga_group_create(fdga_group** g, fdga_ctx* const* members, int32_t n);
 int         fdga_group_destroy(fdga_group* g);
 const char* fdga_group_last_error(const fdga_group* g);
