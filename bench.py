@@ -107,12 +107,15 @@ def max_relerr(a, b):
 
 
 def time_path(S, args, names):
-    """warmup + timed iterations (HIP events on the solver's stream), then a per-stage pass."""
+    """warmup + timed iterations (HIP events on the solver's stream; replayed from the captured
+    hipGraph of one iteration unless --eager), then a per-stage pass (eager, one stage at a time)."""
     import fdga
+    S.set_graph(not args.eager)
     for _ in range(args.warmup):
         fdga.iterate_solver(S, "fdPA")
     S.sync()
     ms = S.time_iterations("fdPA", args.steps)
+    S.set_graph(False)
     nst = max(1, min(args.steps, 5))
     S.set_stage_timing(True)
     for _ in range(nst):
@@ -309,6 +312,8 @@ def main():
                     help="identity symmetry groups (every element evaluated) instead of init_sym_grp!")
     ap.add_argument("--paths", default="fp64,fp32", help="precision paths to run (profiling runs select one)")
     ap.add_argument("--no-dense", action="store_true", help="skip the identity-group FP64 timing")
+    ap.add_argument("--eager", action="store_true", help="launch the timed iterations kernel by kernel instead of "
+                                                         "replaying the captured hipGraph of one iteration")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 512 MiB device-to-device copy first (calibration of the rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE counters against a copy of known size)")
@@ -406,6 +411,7 @@ def main():
         "config": {"workload": "local SIAM fdPA, U=2, beta=20, nK1=128, nK2=(64,64), nK3=(32,64), mPinu=6, nG=256",
                    "symmetry_groups": "identity" if args.no_sym else "init_sym_grp! (frequency symmetries)",
                    "value_path": head,
+                   "iteration_launch": "eager" if args.eager else "hipGraph replay of one captured iteration",
                    "fp32_mfma": {"iters_per_s": 1000.0 / paths["fp32"]["ms"], "bse_tflops": paths["fp32"]["bse_tflops"],
                                  "bse_tflops_dense_equiv": paths["fp32"]["bse_tflops_dense_equiv"],
                                  "stage_ms": paths["fp32"]["st"], "anderson_iterations": paths["fp32"]["it"],
