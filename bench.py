@@ -78,15 +78,15 @@ def cpu_baseline(sym=True, reps=3):
                       f"median of {reps} = {dt:.2f} s"}
 
 
-def gemm_traffic(path):
-    """HBM bytes per launch of the BSE_K3! GEMM kernel (FETCH + WRITE, calibrated against a plain
-    copy) as measured by rocprofv3 --pmc and recorded in profiles/c1_bse_k3_gemm_traffic.txt; None
-    when the file has no entry for the precision path."""
+def gemm_traffic(path, tile):
+    """HBM bytes per launch of the BSE_K3! GEMM kernel with tile shape `tile` (FETCH + WRITE,
+    calibrated against a plain copy) as measured by rocprofv3 --pmc and recorded in
+    profiles/c1_bse_k3_gemm_traffic.txt; None when the file has no entry for the path and shape."""
     import re
     f = os.path.join(ROOT, "profiles", "c1_bse_k3_gemm_traffic.txt")
     if not os.path.exists(f):
         return None
-    m = re.search(rf"^{path}_hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(f).read(), re.M)
+    m = re.search(rf"^{path}_hbm_bytes_per_launch_{tile}\s*=\s*([0-9.eE+]+)", open(f).read(), re.M)
     return float(m.group(1)) if m else None
 
 
@@ -394,7 +394,7 @@ def main():
     P = paths[head]
     peak = FP32_MFMA_PEAK if head == "fp32" else FP64_MFMA_PEAK
     achieved = P["f_exec_launch"] / P["gemm_avg_s"]
-    traffic = gemm_traffic(head)
+    traffic = gemm_traffic(head, P["tiles"][1])
     out = {
         "metric": "parquet iters/s + BSE TFLOP/s, FP32-MFMA vs FP64 GPU @ rel.err<=1e-6 vs FP64",
         "value": 1000.0 / P["ms"],
