@@ -323,10 +323,23 @@ __global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, dou
             tw[k] = cmk(cs, sn);
         }
     for (int which = 0; which < 2; ++which) {
-        // products
-        for (int e = tid; e < NB * LL; e += blockDim.x) {
-            const int wl = e % NB, R = e / NB;
-            s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : czero();
+        // products (a compile-time trip count when L and NB are fixed, so that the loads of
+        // several products are in flight together)
+        if (LT && NBT) {
+            constexpr int NQ = (NBT * (LT ? LT : 1) * (LT ? LT : 1) + 255) / 256;
+#pragma unroll 8
+            for (int q = 0; q < NQ; ++q) {
+                const int e = tid + 256 * q;
+                if (e < NB * LL) {
+                    const int wl = e % NB, R = e / NB;
+                    s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : czero();
+                }
+            }
+        } else {
+            for (int e = tid; e < NB * LL; e += blockDim.x) {
+                const int wl = e % NB, R = e / NB;
+                s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : czero();
+            }
         }
         __syncthreads();
         // along R1' (thread: ω, R2'), then along R2' (thread: ω, q1) with the write-out
