@@ -11,6 +11,8 @@
 //   BSE_L_K2!              (k_bse_w and k_tail_sums keep W and the far sums on the band the gathers read)
 //   SDE!                   k_sde_L, k_sde_sigma_local, k_u2_*, k_occupation
 #pragma once
+#include <type_traits>
+
 #include "fdga_device.h"
 
 namespace fdga {
@@ -530,6 +532,26 @@ template <> struct MfmaT<true> {
 // prefetch = false` is staged synchronously.
 template <class L> struct Prefetch { static constexpr bool on = true; };
 
+// A bound loader whose element combines several loads may expose them separately: a `Raw` type,
+// `raw(i, k)` issuing the loads and `combine(Raw)` the arithmetic.  The register prefetch then holds
+// the raw loads and combines them only when the tile is written to LDS, after the MFMAs of the
+// current tile, so that the loads stay in flight during the MFMAs.  Other loaders are their own raw
+// value.  A zero-filled Raw (the padding outside the matrix) must combine to zero.
+template <class L, class = void>
+struct LoaderRaw {
+    typedef double2 Raw;
+    __device__ __forceinline__ static Raw load(const L& l, int a, int b) { return l(a, b); }
+    __device__ __forceinline__ static Raw zero() { return czero(); }
+    __device__ __forceinline__ static double2 combine(const L&, const Raw& r) { return r; }
+};
+template <class L>
+struct LoaderRaw<L, std::void_t<typename L::Raw>> {
+    typedef typename L::Raw Raw;
+    __device__ __forceinline__ static Raw load(const L& l, int a, int b) { return l.raw(a, b); }
+    __device__ __forceinline__ static Raw zero() { return Raw{}; }
+    __device__ __forceinline__ static double2 combine(const L& l, const Raw& r) { return l.combine(r); }
+};
+
 // One work item of a GEMM launch: an output tile of one batch.
 struct GemmWork { int batch, m0, n0, M, N, K; };
 
@@ -636,20 +658,23 @@ __global__ __launch_bounds__(64 * WGM * WGN) void zgemm_tiles_t(SC sc, AL al, BL
             }
         // software pipeline: the operands of the next K-tile are loaded into registers while the
         // current one is multiplied from LDS
-        double2 pa[QA], pb[QB];
+        typedef LoaderRaw<std::decay_t<decltype(A)>> RA;
+        typedef LoaderRaw<std::decay_t<decltype(B)>> RB;
+        typename RA::Raw pa[QA];
+        typename RB::Raw pb[QB];
         auto fetch = [&](int k0) {
 #pragma unroll
             for (int q = 0; q < QA; ++q) {
                 const int e = tid + NT * q;
                 const int gm = m0 + e % BM, gk = k0 + e / BM;
-                pa[q] = (gm < M && gk < K) ? A(gm, gk) : czero();
+                pa[q] = (gm < M && gk < K) ? RA::load(A, gm, gk) : RA::zero();
             }
 #pragma unroll
             for (int q = 0; q < QB; ++q) {
                 const int e = tid + NT * q;
                 // B operands are stored with k (ω) fastest: consecutive threads take consecutive k
                 const int gn = n0 + e / GBK, gk = k0 + e % GBK;
-                pb[q] = (gn < N && gk < K) ? B(gk, gn) : czero();
+                pb[q] = (gn < N && gk < K) ? RB::load(B, gk, gn) : RB::zero();
             }
         };
         if (PF) fetch(0);
@@ -658,14 +683,16 @@ __global__ __launch_bounds__(64 * WGM * WGN) void zgemm_tiles_t(SC sc, AL al, BL
 #pragma unroll
                 for (int q = 0; q < QA; ++q) {
                     const int e = tid + NT * q;
-                    As[0][e / BM][e % BM] = (S)pa[q].x;
-                    As[1][e / BM][e % BM] = (S)pa[q].y;
+                    const double2 v = RA::combine(A, pa[q]);
+                    As[0][e / BM][e % BM] = (S)v.x;
+                    As[1][e / BM][e % BM] = (S)v.y;
                 }
 #pragma unroll
                 for (int q = 0; q < QB; ++q) {
                     const int e = tid + NT * q;
-                    Bs[0][e % GBK][e / GBK] = (S)pb[q].x;
-                    Bs[1][e % GBK][e / GBK] = (S)pb[q].y;
+                    const double2 v = RB::combine(B, pb[q]);
+                    Bs[0][e % GBK][e / GBK] = (S)v.x;
+                    Bs[1][e % GBK][e / GBK] = (S)v.y;
                 }
             } else {
                 // element by element: load and store to LDS without holding the tile in registers
@@ -829,6 +856,10 @@ struct LK3_Bop {
         __device__ __forceinline__ double2 operator()(int k, int j) const {
             return cmul(pi0[k], F0r[(size_t)k + (size_t)NV * j]);
         }
+        // the two loads held raw by the prefetch (see LoaderRaw)
+        struct Raw { double2 p, f; };
+        __device__ __forceinline__ Raw raw(int k, int j) const { return Raw{pi0[k], F0r[(size_t)k + (size_t)NV * j]}; }
+        __device__ __forceinline__ double2 combine(const Raw& x) const { return cmul(x.p, x.f); }
     };
     __device__ __forceinline__ Bound one(const LK3Chan& c, int bb) const {
         const int NV = g.NV(), NW = g.NW();
@@ -896,6 +927,7 @@ struct K3_AopT {
             return cadd(x, K3r[o]);
         }
     };
+
     __device__ __forceinline__ Bound one(const K3Chan& c, int bb) const {
         const int NV = g.NV();
         const size_t nn = (size_t)NV * NV;
@@ -903,12 +935,13 @@ struct K3_AopT {
         return Bound{c.Fbase + nn * (bb % g.NW()), FULL ? nullptr : c.K3r + nn * bb,
                      (FULL || !t) ? nullptr : c.K3a + nn * bb, t, NV};
     }
+    // (the lattice form's three loads are combined in the prefetch: holding them raw costs more
+    // occupancy than it gains)
     __device__ __forceinline__ Bound bind(int b) const {
         const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
-template <> struct Prefetch<K3_AopT<false>> { static constexpr bool on = true; };
 struct K3_Bop {
     K3Geom g; double T; K3Chan c0, c1, c2;
     struct Bound {
