@@ -760,6 +760,9 @@ struct K3Set {
     K3_Bop b() const { return K3_Bop{g, T, kc[0], kc[1], kc[2]}; }
     K3_Ep e() const { return K3_Ep{g, T, kc[0], kc[1], kc[2]}; }
     template <bool FULL> K3_AopT<FULL> a() const { return K3_AopT<FULL>{g, T, kc[0], kc[1], kc[2]}; }
+    bool op32() const { return kc[0].A32 != nullptr; }   // FP32 operands (lattice solver, FP32 contraction)
+    K3_Aop32 a32() const { return K3_Aop32{g, kc[0], kc[1], kc[2]}; }
+    K3_Bop32 b32() const { return K3_Bop32{g, kc[0], kc[1], kc[2]}; }
 };
 static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) {
     K3Set ks{};
@@ -794,6 +797,13 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) 
         kc[s].out = (sym && ch == TCH) ? c->k3raw : c->Fb[ch].K3;
         kc[s].outa = c->Fb[ACH].K3;
         kc[s].tpost = (ch == TCH && t_epi);
+        if (c->NP > 1 && c->d.precision == FDGA_FP32) {
+            // FP32 operands in the right-operand buffer: R32 in its first half, A32 in its second
+            float2* buf = reinterpret_cast<float2*>(c->R[ch]);
+            rc[s].R32 = buf;
+            kc[s].R32 = buf;
+            kc[s].A32 = buf + c->n3 * c->NP;
+        }
     }
     return ks;
 }
@@ -802,6 +812,10 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n) {
     const size_t ncol = (size_t)c->NV3 * c->NW3 * c->NP * n;
     k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
     LAUNCHCHK(c);
+    if (ks.op32()) {
+        k_k3_A32<<<cdiv(c->n3 * c->NP * n, 256), 256, 0, c->run>>>(ks.g, ks.kc[0], ks.kc[1], ks.kc[2], n);
+        LAUNCHCHK(c);
+    }
     return 0;
 }
 // BSE_K3! for `n` channels: right operands and column sums, then one GEMM launch
@@ -813,7 +827,9 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     const K3_Bop b = ks.b();
     const K3_Ep e = ks.e();
     kernel_begin(c);
-    const hipError_t ge = c->NP == 1 ? k3_gemm(c, chans, n, 1, ks.a<true>(), b, e) : k3_gemm(c, chans, n, 1, ks.a<false>(), b, e);
+    const hipError_t ge = c->NP == 1 ? k3_gemm(c, chans, n, 1, ks.a<true>(), b, e)
+                          : ks.op32() ? k3_gemm(c, chans, n, 1, ks.a32(), ks.b32(), e)
+                                      : k3_gemm(c, chans, n, 1, ks.a<false>(), b, e);
     if (ge != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
     return k3_finish(c, c->Fb, chans, n, false, sym, t_epi);
@@ -1667,7 +1683,8 @@ int fdga_autotune(fdga_ctx* c, int32_t reps) {
     for (int t = 0; t < TILE_N; ++t) {
         c->ktile[1] = t;
         const float ms = c->NP == 1 ? timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<true>(), ks.b(), ks.e()); })
-                                    : timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<false>(), ks.b(), ks.e()); });
+                         : ks.op32() ? timeit([&] { return k3_gemm(c, all, 3, 1, ks.a32(), ks.b32(), ks.e()); })
+                                     : timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<false>(), ks.b(), ks.e()); });
         if (ms < best) { best = ms; bt = t; }
     }
     c->ktile[1] = bt;
@@ -1727,8 +1744,8 @@ struct fdga_group {
     LK3_Aop* lk3a = nullptr;
     LK3_Bop* lk3b = nullptr;
     LK3_Ep* lk3e = nullptr;
-    void* k3a[2] = {nullptr, nullptr};
-    K3_Bop* k3b[2] = {nullptr, nullptr};
+    void* k3a[2] = {nullptr, nullptr};   // K3_AopT<true> (local), K3_AopT<false> or K3_Aop32 (lattice FP64 / FP32)
+    void* k3b[2] = {nullptr, nullptr};   // K3_Bop or K3_Bop32
     K3_Ep* k3e[2] = {nullptr, nullptr};
     int tile[2] = {TILE_64x64, TILE_64x64};   // [0] BSE_L_K3!, [1] BSE_K3!
     size_t grid[2] = {0, 0};                  // workgroups; 0: one per work item
@@ -1812,7 +1829,9 @@ int group_build_functors(fdga_group* g) {
     std::vector<LK3_Ep> le;
     std::vector<K3_AopT<true>> af[2];
     std::vector<K3_AopT<false>> an[2];
+    std::vector<K3_Aop32> a32[2];
     std::vector<K3_Bop> kb[2];
+    std::vector<K3_Bop32> b32[2];
     std::vector<K3_Ep> ke[2];
     for (fdga_ctx* c : g->m) {
         const LK3Set ls = lk3_set(c, all, 3, false, false);
@@ -1824,16 +1843,21 @@ int group_build_functors(fdga_group* g) {
             const K3Set ks = k3_set(c, all, 3, false, false);
             af[s].push_back(ks.a<true>());
             an[s].push_back(ks.a<false>());
+            a32[s].push_back(ks.a32());
             kb[s].push_back(ks.b());
+            b32[s].push_back(ks.b32());
             ke[s].push_back(ks.e());
             swap_F_host(c);   // the other state; two swaps restore the context
         }
     }
     if (group_upload(g, &g->lk3a, la) || group_upload(g, &g->lk3b, lb) || group_upload(g, &g->lk3e, le)) return -2;
+    const bool op32 = g->nonlocal && g->precision == FDGA_FP32;
     for (int s = 0; s < 2; ++s) {
-        int rc = g->nonlocal ? group_upload(g, (K3_AopT<false>**)&g->k3a[s], an[s])
-                             : group_upload(g, (K3_AopT<true>**)&g->k3a[s], af[s]);
-        if (rc || group_upload(g, &g->k3b[s], kb[s]) || group_upload(g, &g->k3e[s], ke[s])) return -2;
+        int rc = op32 ? group_upload(g, (K3_Aop32**)&g->k3a[s], a32[s])
+                 : g->nonlocal ? group_upload(g, (K3_AopT<false>**)&g->k3a[s], an[s])
+                               : group_upload(g, (K3_AopT<true>**)&g->k3a[s], af[s]);
+        rc = rc ? rc : op32 ? group_upload(g, (K3_Bop32**)&g->k3b[s], b32[s]) : group_upload(g, (K3_Bop**)&g->k3b[s], kb[s]);
+        if (rc || group_upload(g, &g->k3e[s], ke[s])) return -2;
     }
     return 0;
 }
@@ -1867,9 +1891,12 @@ int group_gemm(fdga_group* g, int which) {
         const int s = g->m[0]->swap_state;
         for (const fdga_ctx* c : g->m)
             if (c->swap_state != s) return gfail(g, "group members are not at the same iteration (F/Fbuff state)");
-        const GroupOp<K3_Bop> b{g->k3b[s], g->boff, g->n};
+        const GroupOp<K3_Bop> b{(const K3_Bop*)g->k3b[s], g->boff, g->n};
         const GroupOp<K3_Ep> ep{g->k3e[s], g->boff, g->n};
-        if (g->nonlocal)
+        if (g->nonlocal && g->precision == FDGA_FP32)
+            e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_Aop32>{(const K3_Aop32*)g->k3a[s], g->boff, g->n},
+                          GroupOp<K3_Bop32>{(const K3_Bop32*)g->k3b[s], g->boff, g->n}, ep);
+        else if (g->nonlocal)
             e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<false>>{(K3_AopT<false>*)g->k3a[s], g->boff, g->n},
                           b, ep);
         else
@@ -2018,8 +2045,8 @@ int fdga_group_destroy(fdga_group* g) {
         if (gi.items) (void)hipFree(gi.items);
         if (gi.geo) (void)hipFree(gi.geo);
     }
-    for (void* p : {(void*)g->boff, (void*)g->lk3a, (void*)g->lk3b, (void*)g->lk3e, g->k3a[0], g->k3a[1], (void*)g->k3b[0],
-                    (void*)g->k3b[1], (void*)g->k3e[0], (void*)g->k3e[1]})
+    for (void* p : {(void*)g->boff, (void*)g->lk3a, (void*)g->lk3b, (void*)g->lk3e, g->k3a[0], g->k3a[1], g->k3b[0],
+                    g->k3b[1], (void*)g->k3e[0], (void*)g->k3e[1]})
         if (p) (void)hipFree(p);
     for (auto e : g->ev_m)
         if (e) (void)hipEventDestroy(e);

@@ -912,6 +912,10 @@ struct K3Chan {
     double2* out;          // Fbuff.γr.K3
     const double2* outa;   // Fbuff.γa.K3 (in-launch t post-processing)
     int tpost;
+    // lattice solver, FP32 contraction: both operands materialised in FP32 (A32 = the A-loader's
+    // element, R32 = the right operand), each rounded as the FP32 GEMM would round it
+    float2* A32;
+    const float2* R32;
 };
 // FULL: the cache holds the whole cache_Fr (one momentum, a single load per element)
 template <bool FULL>
@@ -957,6 +961,58 @@ struct K3_Bop {
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
+// FP32 operands of the lattice BSE_K3! (FP32 contraction): single 8-byte loads, held raw by the
+// prefetch (LoaderRaw) and widened only to be rounded back at the LDS store (exact)
+struct K3_Aop32 {
+    K3Geom g; K3Chan c0, c1, c2;
+    struct Bound {
+        const float2* A; int NV;
+        typedef float2 Raw;
+        __device__ __forceinline__ Raw raw(int i, int k) const { return A[(size_t)i + (size_t)NV * k]; }
+        __device__ __forceinline__ double2 combine(const Raw& x) const { return cmk(x.x, x.y); }
+        __device__ __forceinline__ double2 operator()(int i, int k) const { return combine(raw(i, k)); }
+    };
+    __device__ __forceinline__ Bound one(const K3Chan& c, int bb) const {
+        const int NV = g.NV();
+        return Bound{c.A32 + (size_t)NV * NV * bb, NV};
+    }
+    __device__ __forceinline__ Bound bind(int b) const {
+        const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
+        return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
+    }
+};
+struct K3_Bop32 {
+    K3Geom g; K3Chan c0, c1, c2;
+    struct Bound {
+        const float2* R; int NV;
+        typedef float2 Raw;
+        __device__ __forceinline__ Raw raw(int k, int j) const { return R[(size_t)k + (size_t)NV * j]; }
+        __device__ __forceinline__ double2 combine(const Raw& x) const { return cmk(x.x, x.y); }
+        __device__ __forceinline__ double2 operator()(int k, int j) const { return combine(raw(k, j)); }
+    };
+    __device__ __forceinline__ Bound one(const K3Chan& c, int bb) const {
+        const int NV = g.NV();
+        return Bound{c.R32 + (size_t)NV * NV * bb, NV};
+    }
+    __device__ __forceinline__ Bound bind(int b) const {
+        const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
+        return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
+    }
+};
+// A32 of the lattice BSE_K3! over `nchan` channel slots: the K3_AopT<false> element rounded to FP32
+__global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, K3Chan c2, int nchan) {
+    const int NV = g.NV(), NW = g.NW();
+    const size_t nn = (size_t)NV * NV, per = nn * NW * g.NP;
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= per * nchan) return;
+    const int s = (int)(e / per);
+    const size_t r = e - per * s, bb = r / nn, o = r - bb * nn;
+    const K3Chan& c = s == 0 ? c0 : s == 1 ? c1 : c2;
+    const double2 x = c.Fbase[o + nn * (bb % NW)];
+    const double2 k3 = c.K3r[r];
+    const double2 v = c.ch == TCH ? cadd(x, csub(cscale(k3, 2.0), c.K3a[r])) : cadd(x, k3);
+    c.A32[r] = make_float2((float)v.x, (float)v.y);
+}
 struct K3_Ep {
     K3Geom g; double T; K3Chan c0, c1, c2;
     struct Bound {
@@ -1015,6 +1071,7 @@ struct K3RChan {
     const double2* La;    // FL.γa.K3
     double2* R;           // [ω, νp, Ω, P]
     double2* csum;        // [νp, Ω, P]
+    float2* R32;          // when set: R rounded to FP32 goes here instead of R
 };
 template <int CH>
 __device__ __forceinline__ void k3_R_column(const K3Geom& g, const K3RChan& c, int j, int bb, int lane) {
@@ -1037,7 +1094,9 @@ __device__ __forceinline__ void k3_R_column(const K3Geom& g, const K3RChan& c, i
             if (CH == TCH) L = csub(cscale(L, 2.0), c.La[lo]);
             x = cadd(x, cscale(cmul(P1, L), c.s2));
         }
-        c.R[(size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb)] = x;
+        const size_t ro = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb);
+        if (c.R32) c.R32[ro] = make_float2((float)x.x, (float)x.y);
+        else c.R[ro] = x;
         cr += x.x; ci += x.y;
     }
     cr = wave_sum(cr); ci = wave_sum(ci);
