@@ -760,7 +760,8 @@ struct K3Set {
     K3_Bop b() const { return K3_Bop{g, T, kc[0], kc[1], kc[2]}; }
     K3_Ep e() const { return K3_Ep{g, T, kc[0], kc[1], kc[2]}; }
     template <bool FULL> K3_AopT<FULL> a() const { return K3_AopT<FULL>{g, T, kc[0], kc[1], kc[2]}; }
-    bool op32() const { return kc[0].A32 != nullptr; }   // FP32 operands (lattice solver, FP32 contraction)
+    bool op32() const { return kc[0].A32 != nullptr; }   // FP32 A and R (lattice solver, FP32 contraction)
+    bool r32() const { return kc[0].R32 != nullptr; }    // FP32 R (any solver, FP32 contraction)
     K3_Aop32 a32() const { return K3_Aop32{g, kc[0], kc[1], kc[2]}; }
     K3_Bop32 b32() const { return K3_Bop32{g, kc[0], kc[1], kc[2]}; }
 };
@@ -797,12 +798,13 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) 
         kc[s].out = (sym && ch == TCH) ? c->k3raw : c->Fb[ch].K3;
         kc[s].outa = c->Fb[ACH].K3;
         kc[s].tpost = (ch == TCH && t_epi);
-        if (c->NP > 1 && c->d.precision == FDGA_FP32) {
-            // FP32 operands in the right-operand buffer: R32 in its first half, A32 in its second
+        if (c->d.precision == FDGA_FP32) {
+            // FP32 operands in the right-operand buffer: R32 in its first half, and for the lattice
+            // solver A32 in its second (the local solver's A is the full FP64 cache)
             float2* buf = reinterpret_cast<float2*>(c->R[ch]);
             rc[s].R32 = buf;
             kc[s].R32 = buf;
-            kc[s].A32 = buf + c->n3 * c->NP;
+            if (c->NP > 1) kc[s].A32 = buf + c->n3 * c->NP;
         }
     }
     return ks;
@@ -827,7 +829,8 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     const K3_Bop b = ks.b();
     const K3_Ep e = ks.e();
     kernel_begin(c);
-    const hipError_t ge = c->NP == 1 ? k3_gemm(c, chans, n, 1, ks.a<true>(), b, e)
+    const hipError_t ge = c->NP == 1 ? (ks.r32() ? k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b32(), e)
+                                                 : k3_gemm(c, chans, n, 1, ks.a<true>(), b, e))
                           : ks.op32() ? k3_gemm(c, chans, n, 1, ks.a32(), ks.b32(), e)
                                       : k3_gemm(c, chans, n, 1, ks.a<false>(), b, e);
     if (ge != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
@@ -1682,7 +1685,9 @@ int fdga_autotune(fdga_ctx* c, int32_t reps) {
     bt = c->ktile[1];
     for (int t = 0; t < TILE_N; ++t) {
         c->ktile[1] = t;
-        const float ms = c->NP == 1 ? timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<true>(), ks.b(), ks.e()); })
+        const float ms = c->NP == 1
+                             ? (ks.r32() ? timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<true>(), ks.b32(), ks.e()); })
+                                         : timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<true>(), ks.b(), ks.e()); }))
                          : ks.op32() ? timeit([&] { return k3_gemm(c, all, 3, 1, ks.a32(), ks.b32(), ks.e()); })
                                      : timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<false>(), ks.b(), ks.e()); });
         if (ms < best) { best = ms; bt = t; }
@@ -1856,7 +1861,9 @@ int group_build_functors(fdga_group* g) {
         int rc = op32 ? group_upload(g, (K3_Aop32**)&g->k3a[s], a32[s])
                  : g->nonlocal ? group_upload(g, (K3_AopT<false>**)&g->k3a[s], an[s])
                                : group_upload(g, (K3_AopT<true>**)&g->k3a[s], af[s]);
-        rc = rc ? rc : op32 ? group_upload(g, (K3_Bop32**)&g->k3b[s], b32[s]) : group_upload(g, (K3_Bop**)&g->k3b[s], kb[s]);
+        rc = rc ? rc
+                : g->precision == FDGA_FP32 ? group_upload(g, (K3_Bop32**)&g->k3b[s], b32[s])
+                                            : group_upload(g, (K3_Bop**)&g->k3b[s], kb[s]);
         if (rc || group_upload(g, &g->k3e[s], ke[s])) return -2;
     }
     return 0;
@@ -1899,6 +1906,9 @@ int group_gemm(fdga_group* g, int which) {
         else if (g->nonlocal)
             e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<false>>{(K3_AopT<false>*)g->k3a[s], g->boff, g->n},
                           b, ep);
+        else if (g->precision == FDGA_FP32)
+            e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<true>>{(K3_AopT<true>*)g->k3a[s], g->boff, g->n},
+                          GroupOp<K3_Bop32>{(const K3_Bop32*)g->k3b[s], g->boff, g->n}, ep);
         else
             e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<true>>{(K3_AopT<true>*)g->k3a[s], g->boff, g->n},
                           b, ep);
