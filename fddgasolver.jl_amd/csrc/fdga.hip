@@ -390,6 +390,15 @@ double2* pi_of(fdga_ctx* c, int ch, bool zero) {
 }
 
 // bound of every box of F, F0 and the RefVertex tables (frequency window of the gather-GEMVs)
+// Window bound of the SDE L-vertex sums: their channel evaluations keep the bosonic argument Ω
+// fixed (src/SDE.jl:79-217), so K1 is a constant there and the summand depends on ω only through the
+// K2 and K3 boxes (target and reference Vertex) and the RefVertex tables.
+int sde_box_bound(const fdga_ctx* c) {
+    const fdga_dims& d = c->d;
+    int nk = std::max({d.nK2f, d.nK3f, d.tab_nf});
+    if (d.ref_kind == 1) nk = std::max({nk, d.r_nK2f, d.r_nK3f});
+    return nk + 1;
+}
 int box_bound(const fdga_ctx* c) {
     const fdga_dims& d = c->d;
     int nk = std::max({d.nK1, d.nK2b, d.nK2f, d.nK3b, d.nK3f, d.tab_nb, d.tab_nf});
@@ -470,7 +479,7 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
     RefV ref = ref_view(c, false);
     const SymCls spp = symcls(c, zero ? SG_0PP2 : SG_PP2), sph = symcls(c, zero ? SG_0PH2 : SG_PH2);
     const double2 TU = cscale(c->U, d.T);
-    const int NK = box_bound(c);
+    const int NK = sde_box_bound(c);
     const double2* Tpp = pi_tail(c, Pp);
     const double2* Tph = pi_tail(c, Ph);
     if (d.nonlocal && lv.kind == 0) {   // lattice solver, Vertex level: one wave per (Ω, P)
