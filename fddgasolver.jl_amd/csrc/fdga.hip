@@ -824,7 +824,8 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n) {
     k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
     LAUNCHCHK(c);
     if (ks.op32()) {
-        k_k3_A32<<<cdiv(c->n3 * c->NP * n, 256), 256, 0, c->run>>>(ks.g, ks.kc[0], ks.kc[1], ks.kc[2], n);
+        k_k3_A32<<<dim3((unsigned)(c->NW3 * c->NP * n), cdiv((size_t)c->NV3 * c->NV3, 256)), 256, 0, c->run>>>(
+            ks.g, ks.kc[0], ks.kc[1], ks.kc[2]);
         LAUNCHCHK(c);
     }
     return 0;

@@ -999,16 +999,17 @@ struct K3_Bop32 {
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
-// A32 of the lattice BSE_K3! over `nchan` channel slots: the K3_AopT<false> element rounded to FP32
-__global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, K3Chan c2, int nchan) {
+// A32 of the lattice BSE_K3!: the K3_AopT<false> element rounded to FP32.  Grid: x over the
+// batches (Ω, P) of all channel slots, y over chunks of 256 of the Nν3 x Nν3 elements of a batch.
+__global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, K3Chan c2) {
     const int NV = g.NV(), NW = g.NW();
-    const size_t nn = (size_t)NV * NV, per = nn * NW * g.NP;
-    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (e >= per * nchan) return;
-    const int s = (int)(e / per);
-    const size_t r = e - per * s, bb = r / nn, o = r - bb * nn;
+    const int nn = NV * NV, nb = NW * g.NP;
+    const int o = blockIdx.y * blockDim.x + threadIdx.x;
+    if (o >= nn) return;
+    const int s = (int)blockIdx.x / nb, bb = (int)blockIdx.x - s * nb;
     const K3Chan& c = s == 0 ? c0 : s == 1 ? c1 : c2;
-    const double2 x = c.Fbase[o + nn * (bb % NW)];
+    const size_t r = (size_t)o + (size_t)nn * bb;
+    const double2 x = c.Fbase[(size_t)o + (size_t)nn * (bb % NW)];
     const double2 k3 = c.K3r[r];
     const double2 v = c.ch == TCH ? cadd(x, csub(cscale(k3, 2.0), c.K3a[r])) : cadd(x, k3);
     c.A32[r] = make_float2((float)v.x, (float)v.y);
