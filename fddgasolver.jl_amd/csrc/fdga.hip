@@ -89,7 +89,8 @@ struct fdga_ctx {
     int ktile[2] = {TILE_64x64, TILE_64x64};   // tile shape of the BSE_L_K3! / BSE_K3! launches
     int* finmap = nullptr;    // k_k3_finish chunk map (XCD-paired Ω blocks)
     int finmap_n = 0, fin_nchunk = 0;
-    double2* k3raw = nullptr;                                       // raw t-channel K3 under groups [n3]
+    double2* k3raw = nullptr;                                       // raw t-channel K3 under groups [n3, NP]
+    double2* k2raw = nullptr;                                       // lattice BSE_K2! raw values under groups [2 n2 NP]
     // hipGraph replay of fdga_iterate(strategy, 1): one graph per strategy and per F/Fbuff pointer
     // state (set!(S.F, S.Fbuff) swaps the two buffers every iteration)
     int graph_on = 0;
@@ -126,6 +127,14 @@ int fail(fdga_ctx* c, const std::string& msg, int code = -1) {
 SymCls symcls(const fdga_ctx* c, int id) { return SymCls{c->sg[id].off, c->sg[id].mem, c->sg[id].ncls}; }
 // number of waves of a one-wave-per-output-class launch
 size_t nclass(const SymCls& s, size_t nout) { return s.ncls ? (size_t)s.ncls : nout; }
+// class members of the n-element array x from their representatives (per-element table rep); no-op
+// without a table
+int sym_fill(fdga_ctx* c, double2* x, const int* rep, size_t n) {
+    if (!rep) return 0;
+    k_sym_fill<<<cdiv(n, 256), 256, 0, c->run>>>(x, rep, n);
+    LAUNCHCHK(c);
+    return 0;
+}
 void free_sym(SymDev& g) {
     if (g.off) (void)hipFree(g.off);
     if (g.mem) (void)hipFree(g.mem);
@@ -431,6 +440,7 @@ int sde_sigma(fdga_ctx* c, double2* S, const double2* G, double scale) {
     k_sde_sigma_nl_R<<<g, 128, 0, c->run>>>(c->tmpG2, c->tmpG1, c->tmpL1, c->tmpL2, d.nG, d.nK2b, d.nK2f, d.LG, d.L, d.T);
     LAUNCHCHK(c);
     if (dft2(c, c->tmpG2, c->NG, d.LG, +1, 1.0)) return -3;
+    if (sym_fill(c, c->tmpG2, c->sg[SG_S].repop, c->nGel)) return -3;   // SGΣ(Σ) (src/nonlocal/SDE.jl:280)
     k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(S, c->tmpG2, scale, 1.0, c->nGel);
     LAUNCHCHK(c);
     return 0;
@@ -458,6 +468,7 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
                                                                          SymCls{nullptr, nullptr, 0});
     LAUNCHCHK(c);
     if (dft2(c, c->tmpG3, c->NG, LG, +1, 1.0)) return -3;
+    if (sym_fill(c, c->tmpG3, c->sg[SG_S].repop, c->nGel)) return -3;   // SGΣ(Σ_U²) (src/nonlocal/SDE.jl:444)
     k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(S, c->tmpG3, 1.0, 1.0, c->nGel);
     LAUNCHCHK(c);
     return 0;
@@ -487,7 +498,10 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
             c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
             lv.chanNP, TU);
         LAUNCHCHK(c);
-        return 0;
+        // SGpp2 / SGph2 (SG0pp2 / SG0ph2 for the reference system): members from representatives
+        const int idp = zero ? SG_0PP2 : SG_PP2, idh = zero ? SG_0PH2 : SG_PH2;
+        if (sym_fill(c, c->Lpp, spp.ncls ? c->sg[idp].repop : nullptr, nout)) return -3;
+        return sym_fill(c, c->Lph, sph.ncls ? c->sg[idh].repop : nullptr, nout);
     }
     if (!spp.ncls && !sph.ncls) {
         k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
@@ -570,7 +584,10 @@ static int launch_lk2(fdga_ctx* c) {
         const LK2nl_E e{c->FL[CH].K2, CH == TCH ? c->FL[ACH].K2 : nullptr, NW2, NV2, c->d.T * kSign[CH], CH == TCH};
         if (zgemm_launch(0, NW2, c->run, NV2, c->NP, NV2, cdiv(NV2, GBM), cdiv(c->NP, GBN), nullptr, 0, a, b, e) != hipSuccess)
             return fail(c, "BSE_L_K2 GEMM launch failed", -3);
-        return 0;
+        // SGppL[2] / SGphL[2]: class members from their representative (the t channel's
+        // post-processing commutes with the fill: its a channel is already symmetric)
+        return sym_fill(c, c->FL[CH].K2, sym_k2(c, CH).ncls ? c->sg[CH == PCH ? SG_PP2 : SG_PH2].repop : nullptr,
+                        c->n2 * c->NP);
     }
     const SymCls sym = sym_k2(c, CH);
     const size_t nw = nclass(sym, c->n2 * c->NP);
@@ -618,10 +635,20 @@ static int launch_k2_nl(fdga_ctx* c) {
     const K2nl_E e{c->k2y, NV2, NW2};
     if (zgemm_launch(0, NW2, c->run, NV2, c->NP, K, cdiv(NV2, GBM), cdiv(c->NP, GBN), nullptr, 0, a, b, e) != hipSuccess)
         return fail(c, "BSE_K2 GEMM launch failed", -3);
+    // under SGpp[2] / SGph[2] the diagram values go to k2raw first: members take op(representative)
+    // and then their own FL.K2 term (src/BSEa/BSEa_K2.jl:60-100)
+    const int* rp = sym_k2(c, CH).ncls ? c->sg[CH == PCH ? SG_PP2 : SG_PH2].repop : nullptr;
+    const size_t n2 = c->n2 * c->NP;
+    double2* raw = rp ? c->k2raw + (CH == PCH ? 0 : n2) : nullptr;
     k_k2nl_finish<CH><<<cdiv((size_t)NW2 * c->NP * WAVE, 256), 256, 0, c->run>>>(
         c->Fb[CH].K2, c->Fb[ACH].K2, c->k2y, c->k2xf, c->Wc[CH], c->Wtail[CH], vtx_F(c), chanF(c, c->FL[CH]),
-        chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->wb, c->NP, c->d.T * kSign[CH]);
+        chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->wb, c->NP, c->d.T * kSign[CH], raw);
     LAUNCHCHK(c);
+    if (rp) {
+        k_k2nl_symfinish<CH><<<cdiv(n2, 256), 256, 0, c->run>>>(c->Fb[CH].K2, c->Fb[ACH].K2, raw, rp, c->FL[CH].K2,
+                                                              c->FL[ACH].K2, n2);
+        LAUNCHCHK(c);
+    }
     return 0;
 }
 
@@ -999,6 +1026,7 @@ int fdga_destroy(fdga_ctx* c) {
     for (auto& g : c->sg) free_sym(g);
     free_blists(c);
     if (c->k3raw) (void)hipFree(c->k3raw);
+    if (c->k2raw) (void)hipFree(c->k2raw);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->st2) (void)hipStreamSynchronize(c->st2);
     if (c->evfork) (void)hipEventDestroy(c->evfork);
@@ -1142,7 +1170,9 @@ int fdga_init(fdga_ctx* c) {
 }  // extern "C"
 
 namespace {
-int upload_sym(fdga_ctx* c, int id, const fdga_sym::Classes& cl) {
+// uploads a group's classes; with `n` > 0 also the per-element table (representative << 2) | op
+// over the n elements of the output
+int upload_sym(fdga_ctx* c, int id, const fdga_sym::Classes& cl, size_t n = 0) {
     free_sym(c->sg[id]);
     SymDev& g = c->sg[id];
     HIPCHK(c, hipMalloc((void**)&g.off, cl.off.size() * sizeof(int)));
@@ -1150,6 +1180,15 @@ int upload_sym(fdga_ctx* c, int id, const fdga_sym::Classes& cl) {
     HIPCHK(c, hipMemcpy(g.off, cl.off.data(), cl.off.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(g.mem, cl.mem.data(), cl.mem.size() * sizeof(int), hipMemcpyHostToDevice));
     g.ncls = cl.ncls();
+    if (n) {
+        std::vector<int> repop(n);
+        for (int k = 0; k < g.ncls; ++k) {
+            const int64_t dv = cl.mem[cl.off[k]] >> 2;
+            for (int q = cl.off[k]; q < cl.off[k + 1]; ++q) repop[cl.mem[q] >> 2] = (int)((dv << 2) | (cl.mem[q] & 3));
+        }
+        HIPCHK(c, hipMalloc((void**)&g.repop, repop.size() * sizeof(int)));
+        HIPCHK(c, hipMemcpy(g.repop, repop.data(), repop.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     return 0;
 }
 }  // namespace
@@ -1159,55 +1198,76 @@ extern "C" {
 int fdga_init_sym_grp(fdga_ctx* c) {
     if (!c) return -1;
     const fdga_dims& d = c->d;
-    if (d.nonlocal)
-        return fail(c, "init_sym_grp!: the lattice symmetry groups of NL_ParquetSolver are not provided "
-                       "(the NL solver evaluates every element)");
+    const bool nl = d.nonlocal;
     using namespace fdga_sym;
     HIPCHK(c, hipStreamSynchronize(c->st));
     graphs_invalidate(c);
-    const Axis aS{false, d.nG}, a1{true, d.nK1}, b2{true, d.nK2b}, f2{false, d.nK2f}, b3{true, d.nK3b}, f3{false, d.nK3f};
-    const int64_t NW = c->NW3, NV = c->NV3;
-    // reference K3 index iΩ + NW (iν + NV iν') -> device index iν + NV (iν' + NV iΩ)
-    auto k3dev = [NW, NV](int64_t r) {
-        const int64_t iW = r % NW, iv = (r / NW) % NV, ivp = r / (NW * NV);
-        return iv + NV * (ivp + NV * iW);
+    const int64_t NW = c->NW3, NV = c->NV3, NP = c->NP;
+    // K3 element counts beyond 2^29 do not fit the (index << 2) | op class encoding
+    if ((int64_t)c->n3 * NP >= ((int64_t)1 << 29))
+        return fail(c, "init_sym_grp!: the K3 mesh is too large for the symmetry class tables");
+    // meshes: the frequency axes, for the lattice solver followed by the momentum mesh (the L x L
+    // transfer momenta; Σ: the LG x LG momenta of G)
+    auto mesh = [nl](std::vector<Axis> ax, int L) {
+        if (nl) ax.push_back(Axis::momentum(L));
+        return ax;
     };
-    const Classes pp2 = build({b2, f2}, {g_k2_1, g_k2pp_2});
-    const Classes ph2 = build({b2, f2}, {g_k2_1, g_k2ph_2});
+    // generators in the order of init_sym_grp!; each frequency map carries the momentum map of its
+    // lattice form (src/nonlocal/symmetries.jl), and the lattice groups end with the reflection and
+    // the rotation (src/nonlocal/ParquetSolver.jl:198-300).  Momentum maps are unused without a
+    // momentum axis.
+    auto gens = [nl](std::vector<Generator> g) {
+        if (nl) {
+            g.push_back(Generator(nullptr, K_REF));
+            g.push_back(Generator(nullptr, K_ROT));
+        }
+        return g;
+    };
+    const Generator Gs(g_sigma, K_NEG), G1(g_k1, K_NEG), G21(g_k2_1, K_NEG), G2pp(g_k2pp_2, K_ID),
+        G2ph(g_k2ph_2, K_NEG), G31(g_k3_1, K_NEG), G32(g_k3_2, K_ID), G3pp(g_k3pp_3, K_ID), G3ph(g_k3ph_3, K_NEG);
+    const std::vector<Axis> mS = mesh({Axis(false, d.nG)}, d.LG), m1 = mesh({Axis(true, d.nK1)}, d.L),
+                            m2 = mesh({Axis(true, d.nK2b), Axis(false, d.nK2f)}, d.L),
+                            m3 = mesh({Axis(true, d.nK3b), Axis(false, d.nK3f), Axis(false, d.nK3f)}, d.L);
+    // reference K3 index iΩ + NW (iν + NV (iν' + NV P)) -> device index iν + NV (iν' + NV (iΩ + NW P))
+    auto k3dev = [NW, NV](int64_t r) {
+        const int64_t iW = r % NW, iv = (r / NW) % NV, rest = r / (NW * NV), ivp = rest % NV, p = rest / NV;
+        return iv + NV * (ivp + NV * (iW + NW * p));
+    };
+    const Classes pp2 = build(m2, gens({G21, G2pp}));
+    const Classes ph2 = build(m2, gens({G21, G2ph}));
+    // the lattice solver evaluates Σ, K2, L_K2 and the SDE L-vertices at every element and fills the
+    // class members afterwards (per-element tables); K1 and the local gathers evaluate by class
+    const size_t nS = nl ? c->nGel : 0, n2 = nl ? c->n2 * c->NP : 0;
     int rc = 0;
-    rc |= upload_sym(c, SG_S, build({aS}, {g_sigma}));
-    rc |= upload_sym(c, SG_K1, build({a1}, {g_k1}));
-    rc |= upload_sym(c, SG_PP2, pp2);
-    rc |= upload_sym(c, SG_PH2, ph2);
-    rc |= upload_sym(c, SG_PP3, build({b3, f3, f3}, {g_k3_1, g_k3_2, g_k3pp_3}, k3dev));
-    rc |= upload_sym(c, SG_PH3, build({b3, f3, f3}, {g_k3_1, g_k3_2, g_k3ph_3}, k3dev));
-    rc |= upload_sym(c, SG_PPL3, build({b3, f3, f3}, {g_k3_1, g_k3pp_3}, k3dev));
-    rc |= upload_sym(c, SG_PHL3, build({b3, f3, f3}, {g_k3_1, g_k3ph_3}, k3dev));
-    rc |= upload_sym(c, SG_0PP2, pp2);
-    rc |= upload_sym(c, SG_0PH2, ph2);
+    rc |= upload_sym(c, SG_S, build(mS, gens({Gs})), nS);
+    rc |= upload_sym(c, SG_K1, build(m1, gens({G1})));
+    rc |= upload_sym(c, SG_PP2, pp2, n2);
+    rc |= upload_sym(c, SG_PH2, ph2, n2);
+    const size_t nk3 = c->n3 * c->NP;
+    rc |= upload_sym(c, SG_PP3, build(m3, gens({G31, G32, G3pp}), k3dev), nk3);
+    rc |= upload_sym(c, SG_PH3, build(m3, gens({G31, G32, G3ph}), k3dev), nk3);
+    rc |= upload_sym(c, SG_PPL3, build(m3, gens({G31, G3pp}), k3dev), nk3);
+    rc |= upload_sym(c, SG_PHL3, build(m3, gens({G31, G3ph}), k3dev), nk3);
+    rc |= upload_sym(c, SG_0PP2, pp2, n2);
+    rc |= upload_sym(c, SG_0PH2, ph2, n2);
     if (rc) return -2;
-    // GEMM blocks holding a representative (block = iΩ * tiles + ti + tm * tj, rows ν, columns ν'),
-    // for every tile shape
+    // GEMM blocks holding a representative (block = (iΩ + NW P) * tiles + ti + tm * tj, rows ν,
+    // columns ν'), for every tile shape
+    const int64_t NB = NW * NP;   // GEMM batches of one channel
     for (int id : {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3}) {
         SymDev& g = c->sg[id];
         std::vector<int> off(g.ncls + 1), mem;
         HIPCHK(c, hipMemcpy(off.data(), g.off, off.size() * sizeof(int), hipMemcpyDeviceToHost));
         mem.resize(off.back());
         HIPCHK(c, hipMemcpy(mem.data(), g.mem, mem.size() * sizeof(int), hipMemcpyDeviceToHost));
-        std::vector<int> repop(c->n3);
-        for (int k = 0; k < g.ncls; ++k) {
-            const int64_t dv = mem[off[k]] >> 2;
-            for (int q = off[k]; q < off[k + 1]; ++q) repop[mem[q] >> 2] = (int)((dv << 2) | (mem[q] & 3));
-        }
-        HIPCHK(c, hipMalloc((void**)&g.repop, repop.size() * sizeof(int)));
-        HIPCHK(c, hipMemcpy(g.repop, repop.data(), repop.size() * sizeof(int), hipMemcpyHostToDevice));
         for (int t = 0; t < TILE_N; ++t) {
             const int tm = cdiv(NV, tile_bm(t)), tn = cdiv(NV, tile_bn(t)), tiles = tm * tn;
-            std::vector<char> hit((size_t)NW * tiles, 0);
+            std::vector<char> hit((size_t)NB * tiles, 0);
             for (int k = 0; k < g.ncls; ++k) {
                 const int64_t dv = mem[off[k]] >> 2;
-                const int iv = (int)(dv % NV), ivp = (int)((dv / NV) % NV), iW = (int)(dv / (NV * NV));
-                hit[(size_t)iW * tiles + iv / tile_bm(t) + tm * (ivp / tile_bn(t))] = 1;
+                const int iv = (int)(dv % NV), ivp = (int)((dv / NV) % NV);
+                const int64_t bt = dv / (NV * NV);
+                hit[(size_t)bt * tiles + iv / tile_bm(t) + tm * (ivp / tile_bn(t))] = 1;
             }
             g.tiles[t].clear();
             for (size_t b = 0; b < hit.size(); ++b)
@@ -1215,7 +1275,9 @@ int fdga_init_sym_grp(fdga_ctx* c) {
         }
     }
     free_blists(c);
-    if (!c->k3raw && dalloc(c, &c->k3raw, c->n3)) return -2;
+    if (!c->k3raw && dalloc(c, &c->k3raw, c->n3 * c->NP)) return -2;
+    // lattice BSE_K2!: raw class values before the FL.K2 term, p in the first half, a and t in the second
+    if (nl && !c->k2raw && dalloc(c, &c->k2raw, 2 * c->n2 * c->NP)) return -2;
     // block lists in XCD order: all tiles of one batch (channel slot, Ω) on one XCD, batch b on XCD b % 8
     for (int t = 0; t < TILE_N; ++t) {
         const int tiles = cdiv(NV, tile_bm(t)) * cdiv(NV, tile_bn(t));
@@ -1234,12 +1296,22 @@ int fdga_init_sym_grp(fdga_ctx* c) {
             const int ids3[3] = {left ? SG_PPL3 : SG_PP3, left ? SG_PHL3 : SG_PH3, left ? SG_PHL3 : SG_PH3};
             std::vector<int> all;
             for (int sl = 0; sl < 3; ++sl)
-                for (int b : c->sg[ids3[sl]].tiles[t]) all.push_back(b + sl * (int)NW * tiles);
+                for (int b : c->sg[ids3[sl]].tiles[t]) all.push_back(b + sl * (int)NB * tiles);
             rc |= upload_list(left ? 4 : 5, all);
         }
     }
-    // k_k3_finish chunks: 256 elements of one Ω block; the blocks Ω and -Ω on one XCD
-    {
+    // k_k3_finish chunks: 256 elements of one (Ω, P) block.  Local solver: the blocks Ω and -Ω on
+    // one XCD; lattice solver (representatives in other momentum blocks too): blocks in order
+    if (nl) {
+        const int nchunk = cdiv((size_t)NV * NV, 256);
+        std::vector<int> items((size_t)NB * nchunk);
+        for (size_t i = 0; i < items.size(); ++i) items[i] = (int)i;
+        const std::vector<int> o = xcd_order(items, [nchunk](int it) { return it / nchunk; });
+        HIPCHK(c, hipMalloc((void**)&c->finmap, o.size() * sizeof(int)));
+        HIPCHK(c, hipMemcpy(c->finmap, o.data(), o.size() * sizeof(int), hipMemcpyHostToDevice));
+        c->finmap_n = (int)o.size();
+        c->fin_nchunk = nchunk;
+    } else {
         const int nchunk = cdiv((size_t)NV * NV, 256);
         // per pair (Ω, -Ω) the chunks of both blocks alternate, so that an XCD works on both
         // blocks of a pair at the same time

@@ -74,6 +74,15 @@ __device__ __forceinline__ double2 fin_load(const double2* raw, const int* rep, 
     const int code = rep[j];
     return sym_apply(code & 3, raw[code >> 2]);
 }
+// class members of x from their representatives, in place: x[j] = op(x[rep(j)]).  Representatives
+// map to themselves and are not written, so every read sees the representative's own value.
+__global__ void k_sym_fill(double2* __restrict__ x, const int* __restrict__ rep, size_t n) {
+    const size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int code = rep[j];
+    if ((size_t)(code >> 2) != j) x[j] = sym_apply(code & 3, x[code >> 2]);
+}
+
 __global__ __launch_bounds__(256) void k_k3_finish(K3Fin f, const int* __restrict__ wgmap, int nchunk, int blk) {
     const int code = wgmap[blockIdx.x];
     if (code < 0) return;
@@ -1574,6 +1583,18 @@ __global__ __launch_bounds__(256) void k_lk2nl_table(double2* __restrict__ X, Vt
     X[e] = cmul(F.eval(m, v, crossing(m, w, CH), 0, CH, SP, FLG), F.f0.eval(m, w, FDGA_INF, CH, SP));
 }
 
+// K2 = diagram + FL.K2 (t: (diagram + 2 FL.t.K2 - FL.a.K2 + K2a) / 2)  (src/BSEa/BSEa_K2.jl:95-100,
+// src/BSE_templates.jl:29-33)
+template <int CH>
+__device__ __forceinline__ double2 k2_add_fl(double2 val, double2 lr, double2 la, double2 k2a) {
+    if (CH == TCH) {
+        val = cadd(val, cscale(lr, 2.0));
+        val = cadd(val, cscale(la, -1.0));
+        return cscale(cadd(val, k2a), 0.5);
+    }
+    return cadd(val, lr);
+}
+
 // BSE_K2! finish of the lattice solver, one wave per (Ω, P):
 //   K2 = T s [Y + Xf_L Wt_L + Xf_R Wt_R + K2r(Ω, ν, P) (Σ_band W + Wt_L + Wt_R) + Σ_ω K3r(Ω, ν, ω, P) W(ω)]
 //        + FL.K2 (t: (... + 2 FL.t.K2 - FL.a.K2 + K2a) / 2)
@@ -1583,7 +1604,7 @@ __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, c
                                                      const double2* __restrict__ Y, const double2* __restrict__ Xf,
                                                      const double2* __restrict__ W, const double2* __restrict__ Wtail,
                                                      VtxV F, ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int wb,
-                                                     int NP, double scale) {
+                                                     int NP, double scale, double2* __restrict__ raw) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= NW2 * NP) return;
@@ -1615,16 +1636,21 @@ __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, c
             acc = cadd(acc, k3s);
         }
         const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
-        double2 val = cscale(acc, scale);
-        if (CH == TCH) {
-            val = cadd(val, cscale(Lr.K2[o], 2.0));
-            val = cadd(val, cscale(La.K2[o], -1.0));
-            val = cscale(cadd(val, K2a[o]), 0.5);
-        } else {
-            val = cadd(val, Lr.K2[o]);
-        }
-        K2[o] = val;
+        if (raw) { raw[o] = cscale(acc, scale); continue; }
+        K2[o] = k2_add_fl<CH>(cscale(acc, scale), Lr.K2[o], La.K2[o], K2a[o]);
     }
+}
+
+// lattice BSE_K2! under a symmetry group: K2[j] = op(raw[rep(j)]) + the FL.K2 term of j (t: then
+// the post-processing with the finished a channel)
+template <int CH>
+__global__ void k_k2nl_symfinish(double2* __restrict__ K2, const double2* __restrict__ K2a, const double2* __restrict__ raw,
+                                 const int* __restrict__ rep, const double2* __restrict__ Lr, const double2* __restrict__ La,
+                                 size_t n) {
+    const size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int code = rep[j];
+    K2[j] = k2_add_fl<CH>(sym_apply(code & 3, raw[code >> 2]), Lr[j], La[j], K2a[j]);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -119,7 +119,9 @@ class Solver:
 
     # ------------------------------------------------------------------ timing
     def init_sym_grp(self):
-        """init_sym_grp!(S) (src/ParquetSolver.jl:203-231): frequency symmetry groups (local solver)."""
+        """init_sym_grp!(S): frequency symmetry groups of the local solver (src/ParquetSolver.jl:203-231);
+        for the s-wave lattice solver with their momentum forms plus the square lattice's reflection and
+        rotation (src/nonlocal/ParquetSolver.jl:198-300)."""
         self._check(self.lib.fdga_init_sym_grp(self.h))
 
     def reset_sym_grp(self):

@@ -65,12 +65,14 @@ int         fdga_get(fdga_ctx* ctx, int32_t which, double* host);
 /* Π0 = bubbles(G0); Σ = Σ0; G = Dyson(Σ, Gbare); Π = bubbles(G)   (src/ParquetSolver.jl:97-114) */
 int         fdga_init(fdga_ctx* ctx);
 
-/* symmetry groups of the local solver: init_sym_grp!(S) (src/ParquetSolver.jl:203-231) installs the
- * frequency symmetry groups of src/symmetries.jl:1-94; afterwards the BSE, L and SDE stages evaluate
- * one representative per class and write op(value) to the other members, as SG(f, InitFunction)
- * does.  reset_sym_grp!(S) (src/ParquetSolver.jl:233-243) restores the identity groups (SG0pp2 and
- * SG0ph2 are left installed, as there).  The NL solver's lattice groups are not provided: the call
- * fails for NL_ParquetSolver. */
+/* symmetry groups: init_sym_grp!(S) (src/ParquetSolver.jl:203-231) installs the frequency symmetry
+ * groups of src/symmetries.jl:1-94; for NL_ParquetSolver (src/nonlocal/ParquetSolver.jl:198-300)
+ * their momentum forms plus the square lattice's reflection and rotation
+ * (src/nonlocal/symmetries.jl:1-133).  Afterwards every stage's result holds op(value of the class
+ * representative) at the other members, as SG(f, InitFunction) and SG(f) give.  The K3 contractions
+ * compute only the tiles that hold representatives.  reset_sym_grp!(S) (src/ParquetSolver.jl:233-243)
+ * restores the identity groups (SG0pp2 and SG0ph2 are left installed, as there).  Fails when a K3
+ * array has 2^29 elements or more. */
 int fdga_init_sym_grp(fdga_ctx* ctx);
 int fdga_reset_sym_grp(fdga_ctx* ctx);
 

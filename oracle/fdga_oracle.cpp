@@ -581,21 +581,31 @@ void orc_init(orc_solver* s) {
 }
 
 // ---------------------------------------------------------------------------------------
-// init_sym_grp! (src/ParquetSolver.jl:203-231), symmetries of src/symmetries.jl:1-94
+// init_sym_grp! (src/ParquetSolver.jl:203-231), symmetries of src/symmetries.jl:1-94; lattice
+// solver: init_sym_grp! of src/nonlocal/ParquetSolver.jl:198-300 with src/nonlocal/symmetries.jl:1-133
 int orc_init_sym_grp(orc_solver* s) {
-    if (s->prm.nonlocal) return -1;
     const orc_params& p = s->prm;
-    // generic mesh description: dims are B (boson) or F (fermion) with their N
-    struct Dim { bool boson; int N; };
-    auto build = [&](SymGroup& sg, std::vector<Dim> dims, std::vector<SymFn> syms) {
-        int sizes[4];
-        for (size_t d = 0; d < dims.size(); ++d) sizes[d] = dims[d].boson ? 2 * dims[d].N - 1 : 2 * dims[d].N;
-        auto valof = [dims](int d, int i) { return dims[d].boson ? i - dims[d].N + 1 : i - dims[d].N; };
+    const bool nl = p.nonlocal != 0;
+    // mesh axes: fermionic / bosonic Matsubara meshes, or an L x L momentum mesh whose points
+    // BrillouinPoint(j1, j2) are numbered j1 + L j2 (the layout of G[ν, k] and of the P axis)
+    enum { FERM = 0, BOS = 1, MOM = 2 };
+    struct Dim { int kind; int N; };
+    auto build = [&](SymGroup& sg, const std::vector<Dim>& dims, const std::vector<SymFn>& syms) {
+        int sizes[5];
+        for (size_t d = 0; d < dims.size(); ++d)
+            sizes[d] = dims[d].kind == BOS ? 2 * dims[d].N - 1 : dims[d].kind == FERM ? 2 * dims[d].N : dims[d].N * dims[d].N;
+        auto valof = [dims](int d, int i) {
+            return dims[d].kind == BOS ? i - dims[d].N + 1 : dims[d].kind == FERM ? i - dims[d].N : i;
+        };
         auto inb = [dims](int d, int x) {
             if (x == INF) return false;
-            return dims[d].boson ? (x >= -(dims[d].N - 1) && x <= dims[d].N - 1) : (x >= -dims[d].N && x <= dims[d].N - 1);
+            if (dims[d].kind == BOS) return x >= -(dims[d].N - 1) && x <= dims[d].N - 1;
+            if (dims[d].kind == FERM) return x >= -dims[d].N && x <= dims[d].N - 1;
+            return x >= 0 && x < dims[d].N * dims[d].N;
         };
-        auto idxof = [dims](int d, int x) { return dims[d].boson ? x + dims[d].N - 1 : x + dims[d].N; };
+        auto idxof = [dims](int d, int x) {
+            return dims[d].kind == BOS ? x + dims[d].N - 1 : dims[d].kind == FERM ? x + dims[d].N : x;
+        };
         build_group(sg, (int)dims.size(), sizes, valof, inb, idxof, syms);
     };
     auto nf = [](int n) { return -n - 1; };  // -ν
@@ -610,25 +620,102 @@ int orc_init_sym_grp(orc_solver* s) {
     SymFn sK3pp3 = [&](const int* w, int* o, uint8_t& op) { o[0] = w[0]; o[1] = bmf(w[0], w[1]); o[2] = bmf(w[0], w[2]); op = 0; return true; };
     SymFn sK3ph3 = [&](const int* w, int* o, uint8_t& op) { o[0] = -w[0]; o[1] = bpf(w[0], w[1]); o[2] = bpf(w[0], w[2]); op = 0; return true; };
 
-    std::vector<Dim> dS = {{false, p.nG}};
-    std::vector<Dim> d1 = {{true, p.nK1}};
-    std::vector<Dim> d2 = {{true, p.nK2b}, {false, p.nK2f}};
-    std::vector<Dim> d3 = {{true, p.nK3b}, {false, p.nK3f}, {false, p.nK3f}};
-    build(s->SGS, dS, {sSig});
-    build(s->SGpp[1], d1, {sK1});
-    build(s->SGpp[2], d2, {sK2pp1, sK2pp2});
-    build(s->SGpp[3], d3, {sK3pp1, sK3pp2, sK3pp3});
+    // momentum parts (src/nonlocal/symmetries.jl): -k folded back, _ref (x, y) -> (y, x),
+    // _rot (x, y) -> fold_back((y, -x))
+    enum { KID = 0, KNEG = 1, KREF = 2, KROT = 3 };
+    auto kmove = [](int code, int k, int L) {
+        const int x = k % L, y = k / L;
+        const int mx = (L - x) % L, my = (L - y) % L;
+        if (code == KNEG) return mx + L * my;
+        if (code == KREF) return y + L * x;
+        if (code == KROT) return y + L * mx;
+        return k;
+    };
+    // one group: the frequency symmetries (each with the momentum part of its lattice form), and for
+    // the lattice solver the momentum axis plus sX_ref and sX_rot at the end of the list
+    auto group = [&](SymGroup& sg, std::vector<Dim> dims, std::vector<std::pair<SymFn, int>> syms, int L) {
+        const int nfreq = (int)dims.size();
+        std::vector<SymFn> fns;
+        if (!nl) {
+            for (auto& f : syms) fns.push_back(f.first);
+            build(sg, dims, fns);
+            return;
+        }
+        dims.push_back({MOM, L});
+        SymFn keep = [nfreq](const int* w, int* o, uint8_t& op) {
+            for (int d = 0; d < nfreq; ++d) o[d] = w[d];
+            op = 0;
+            return true;
+        };
+        syms.push_back({keep, KREF});
+        syms.push_back({keep, KROT});
+        for (auto& f : syms) {
+            SymFn fr = f.first;
+            const int code = f.second;
+            fns.push_back([=](const int* w, int* o, uint8_t& op) {
+                if (!fr(w, o, op)) return false;
+                o[nfreq] = kmove(code, w[nfreq], L);
+                return true;
+            });
+        }
+        build(sg, dims, fns);
+    };
+
+    const int L = p.L, LG = p.LG;
+    std::vector<Dim> dS = {{FERM, p.nG}};
+    std::vector<Dim> d1 = {{BOS, p.nK1}};
+    std::vector<Dim> d2 = {{BOS, p.nK2b}, {FERM, p.nK2f}};
+    std::vector<Dim> d3 = {{BOS, p.nK3b}, {FERM, p.nK3f}, {FERM, p.nK3f}};
+    group(s->SGS, dS, {{sSig, KNEG}}, LG);
+    group(s->SGpp[1], d1, {{sK1, KNEG}}, L);
+    group(s->SGpp[2], d2, {{sK2pp1, KNEG}, {sK2pp2, KID}}, L);
+    group(s->SGpp[3], d3, {{sK3pp1, KNEG}, {sK3pp2, KID}, {sK3pp3, KID}}, L);
     s->SGppL[1] = s->SGpp[1];
     s->SGppL[2] = s->SGpp[2];
-    build(s->SGppL[3], d3, {sK3pp1, sK3pp3});
-    build(s->SGph[1], d1, {sK1});
-    build(s->SGph[2], d2, {sK2pp1, sK2ph2});
-    build(s->SGph[3], d3, {sK3pp1, sK3pp2, sK3ph3});
+    group(s->SGppL[3], d3, {{sK3pp1, KNEG}, {sK3pp3, KID}}, L);
+    s->SGph[1] = s->SGpp[1];
+    group(s->SGph[2], d2, {{sK2pp1, KNEG}, {sK2ph2, KNEG}}, L);
+    group(s->SGph[3], d3, {{sK3pp1, KNEG}, {sK3pp2, KID}, {sK3ph3, KNEG}}, L);
     s->SGphL[1] = s->SGph[1];
     s->SGphL[2] = s->SGph[2];
-    build(s->SGphL[3], d3, {sK3pp1, sK3ph3});
-    build(s->SG0pp2, d2, {sK2pp1, sK2pp2});
-    build(s->SG0ph2, d2, {sK2pp1, sK2ph2});
+    group(s->SGphL[3], d3, {{sK3pp1, KNEG}, {sK3ph3, KNEG}}, L);
+    group(s->SG0pp2, d2, {{sK2pp1, KNEG}, {sK2pp2, KID}}, L);
+    group(s->SG0ph2, d2, {{sK2pp1, KNEG}, {sK2ph2, KNEG}}, L);
+    return 0;
+}
+
+static const SymGroup* sym_by_id(const orc_solver* s, int32_t id) {
+    switch (id) {
+        case 0: return &s->SGS;
+        case 1: return &s->SGpp[1];
+        case 2: return &s->SGpp[2];
+        case 3: return &s->SGpp[3];
+        case 4: return &s->SGppL[3];
+        case 5: return &s->SGph[2];
+        case 6: return &s->SGph[3];
+        case 7: return &s->SGphL[3];
+        case 8: return &s->SG0pp2;
+        case 9: return &s->SG0ph2;
+        default: return nullptr;
+    }
+}
+
+int64_t orc_sym_size(const orc_solver* s, int32_t id) {
+    const SymGroup* g = sym_by_id(s, id);
+    if (!g || g->identity) return -1;
+    int64_t n = 0;
+    for (auto& cls : g->classes) n += (int64_t)cls.size();
+    return n;
+}
+
+int orc_sym_classes(const orc_solver* s, int32_t id, int64_t* rep, int32_t* op) {
+    const SymGroup* g = sym_by_id(s, id);
+    if (!g || g->identity) return -1;
+    for (auto& cls : g->classes)
+        for (auto& [j, o] : cls) {
+            rep[j] = cls[0].first;
+            op[j] = o;
+        }
     return 0;
 }
 
@@ -698,9 +785,12 @@ void orc_bse_l_k2(orc_solver* s, int32_t ch) {
     const int sp = kSp[ch], sg = kSign[ch];
     const int fl = FL_ALL & ~FL_F0 & ~(ch == PCH ? FL_GP : ch == TCH ? FL_GT : FL_GA);
     const size_t n2 = out.n2();
-    for (int P = 0; P < s->NP; ++P) {
-        std::vector<cplx> tmp(n2);
-        sg_eval(sg_for(s, ch, 2, true), tmp, n2, [&](size_t i) {
+    // one evaluation over every (Ω, ν, P): the lattice groups relate different momenta
+    {
+        std::vector<cplx> tmp(n2 * s->NP);
+        sg_eval(sg_for(s, ch, 2, true), tmp, tmp.size(), [&](size_t ii) {
+            const size_t i = ii % n2;
+            const int P = (int)(ii / n2);
             int m = out.b2.val((int)(i % out.b2.size())), v = out.f2.val((int)(i / out.b2.size()));
             cplx val = 0.0;
             if (!s->prm.nonlocal) {
@@ -720,7 +810,7 @@ void orc_bse_l_k2(orc_solver* s, int32_t ch) {
             }
             return s->T * val * (double)sg;
         });
-        std::copy(tmp.begin(), tmp.end(), out.K2.begin() + n2 * P);
+        std::copy(tmp.begin(), tmp.end(), out.K2.begin());
     }
     if (ch == TCH) t_post(s->FL.g[TCH].K2, s->FL.g[ACH].K2);
     s->FL.g[ch].refresh();
@@ -735,9 +825,11 @@ void orc_bse_l_k3(orc_solver* s, int32_t ch) {
     const int sg = (ch == ACH) ? +1 : -1;  // src/BSE_templates.jl:122,130,138
     const int nb = out.b3.size(), nf = out.f3.size();
     const size_t n3 = out.n3();
-    for (int P = 0; P < s->NP; ++P) {
-        std::vector<cplx> tmp(n3);
-        sg_eval(sg_for(s, ch, 3, true), tmp, n3, [&](size_t i) {
+    {
+        std::vector<cplx> tmp(n3 * s->NP);
+        sg_eval(sg_for(s, ch, 3, true), tmp, tmp.size(), [&](size_t ii) {
+            const size_t i = ii % n3;
+            const int P = (int)(ii / n3);
             int i1 = (int)(i % nb), i2 = (int)((i / nb) % nf), i3 = (int)(i / ((size_t)nb * nf));
             int m = out.b3.val(i1);
             cplx val = 0.0;
@@ -749,7 +841,7 @@ void orc_bse_l_k3(orc_solver* s, int32_t ch) {
             }
             return s->T * val * (double)sg;
         });
-        std::copy(tmp.begin(), tmp.end(), out.K3.begin() + n3 * P);
+        std::copy(tmp.begin(), tmp.end(), out.K3.begin());
     }
     if (ch == TCH) t_post(s->FL.g[TCH].K3, s->FL.g[ACH].K3);
     s->FL.g[ch].refresh();
@@ -764,9 +856,11 @@ void orc_bse_k1(orc_solver* s, int32_t ch) {
     const std::vector<cplx>& Pi = (ch == PCH) ? s->Pipp : s->Piph;
     const int sp = kSp[ch], sg = kSign[ch];
     const size_t n1 = out.n1();
-    for (int P = 0; P < s->NP; ++P) {
-        std::vector<cplx> tmp(n1);
-        sg_eval(sg_for(s, ch, 1, false), tmp, n1, [&](size_t i) {
+    {
+        std::vector<cplx> tmp(n1 * s->NP);
+        sg_eval(sg_for(s, ch, 1, false), tmp, tmp.size(), [&](size_t ii) {
+            const size_t i = ii % n1;
+            const int P = (int)(ii / n1);
             int m = out.b1.val((int)i);
             cplx val = 0.0;
             for (int iw = 0; iw < s->mPiF.size(); ++iw) {
@@ -779,7 +873,7 @@ void orc_bse_k1(orc_solver* s, int32_t ch) {
             }
             return s->T * val * (double)sg;
         });
-        std::copy(tmp.begin(), tmp.end(), out.K1.begin() + n1 * P);
+        std::copy(tmp.begin(), tmp.end(), out.K1.begin());
     }
     if (ch == TCH) t_post(s->Fbuff.g[TCH].K1, s->Fbuff.g[ACH].K1);
 }
@@ -793,9 +887,11 @@ void orc_bse_k2(orc_solver* s, int32_t ch) {
     const std::vector<cplx>& Pi = (ch == PCH) ? s->Pipp : s->Piph;
     const int sp = kSp[ch], sg = kSign[ch];
     const size_t n2 = out.n2();
-    for (int P = 0; P < s->NP; ++P) {
-        std::vector<cplx> tmp(n2);
-        sg_eval(sg_for(s, ch, 2, false), tmp, n2, [&](size_t i) {
+    {
+        std::vector<cplx> tmp(n2 * s->NP);
+        sg_eval(sg_for(s, ch, 2, false), tmp, tmp.size(), [&](size_t ii) {
+            const size_t i = ii % n2;
+            const int P = (int)(ii / n2);
             int m = out.b2.val((int)(i % out.b2.size())), v = out.f2.val((int)(i / out.b2.size()));
             cplx val = 0.0;
             for (int iw = 0; iw < s->mPiF.size(); ++iw) {
@@ -808,7 +904,7 @@ void orc_bse_k2(orc_solver* s, int32_t ch) {
             }
             return s->T * val * (double)sg;
         });
-        std::copy(tmp.begin(), tmp.end(), out.K2.begin() + n2 * P);
+        std::copy(tmp.begin(), tmp.end(), out.K2.begin());
     }
     // add FL.K2 (t: 2 FLt - FLa) (src/BSEa/BSEa_K2.jl:95-100)
     if (ch == TCH) {
@@ -838,9 +934,11 @@ void orc_bse_k3(orc_solver* s, int32_t ch) {
     auto c3 = [&](const std::vector<cplx>& C, int i1, int i2, int i3, int P) {
         return C[i1 + (size_t)nb * (i2 + (size_t)nf * (i3 + (size_t)nf * P))];
     };
-    for (int P = 0; P < s->NP; ++P) {
-        std::vector<cplx> tmp(n3);
-        sg_eval(sg_for(s, ch, 3, false), tmp, n3, [&](size_t i) {
+    {
+        std::vector<cplx> tmp(n3 * s->NP);
+        sg_eval(sg_for(s, ch, 3, false), tmp, tmp.size(), [&](size_t ii) {
+            const size_t i = ii % n3;
+            const int P = (int)(ii / n3);
             int i1 = (int)(i % nb), i2 = (int)((i / nb) % nf), i3 = (int)(i / ((size_t)nb * nf));
             int m = out.b3.val(i1), v = out.f3.val(i2), vp = out.f3.val(i3);
             cplx val = 0.0;
@@ -864,7 +962,7 @@ void orc_bse_k3(orc_solver* s, int32_t ch) {
             else self = Lr.K3[Lr.i3(m, v, vp) + Lr.n3() * P];
             return s->T * val + self;
         });
-        std::copy(tmp.begin(), tmp.end(), out.K3.begin() + n3 * P);
+        std::copy(tmp.begin(), tmp.end(), out.K3.begin());
     }
     if (ch == TCH) t_post(s->Fbuff.g[TCH].K3, s->Fbuff.g[ACH].K3);
 }
@@ -902,9 +1000,11 @@ void sde_channel_L(orc_solver* s, const SdeIn& in, const Vert& Fv) {
     const size_t n2 = (size_t)nb * nf;
     const cplx U = Fv.bare();
     const bool isref = Fv.ref != nullptr;
-    for (int P = 0; P < s->NP; ++P) {
-        std::vector<cplx> tpp(n2), tph(n2);
-        sg_eval(*in.SGpp2, tpp, n2, [&](size_t i) {
+    {
+        std::vector<cplx> tpp(n2 * s->NP), tph(n2 * s->NP);
+        sg_eval(*in.SGpp2, tpp, tpp.size(), [&](size_t ii) {
+            const size_t i = ii % n2;
+            const int P = (int)(ii / n2);
             int m = shape.b2.val((int)(i % nb)), v = shape.f2.val((int)(i / nb));
             cplx val = 0.0;
             for (int iw = 0; iw < s->mPiF.size(); ++iw) {
@@ -915,7 +1015,9 @@ void sde_channel_L(orc_solver* s, const SdeIn& in, const Vert& Fv) {
             }
             return s->T * val;
         });
-        sg_eval(*in.SGph2, tph, n2, [&](size_t i) {
+        sg_eval(*in.SGph2, tph, tph.size(), [&](size_t ii) {
+            const size_t i = ii % n2;
+            const int P = (int)(ii / n2);
             int m = shape.b2.val((int)(i % nb)), v = shape.f2.val((int)(i / nb));
             cplx val = 0.0;
             for (int iw = 0; iw < s->mPiF.size(); ++iw) {
@@ -934,8 +1036,8 @@ void sde_channel_L(orc_solver* s, const SdeIn& in, const Vert& Fv) {
             }
             return s->T * val;
         });
-        std::copy(tpp.begin(), tpp.end(), in.Lpp->begin() + n2 * P);
-        std::copy(tph.begin(), tph.end(), in.Lph->begin() + n2 * P);
+        std::copy(tpp.begin(), tpp.end(), in.Lpp->begin());
+        std::copy(tph.begin(), tph.end(), in.Lph->begin());
     }
 }
 

@@ -65,8 +65,14 @@ void orc_siam_bare_green(orc_solver* s, double e, double Delta, double D);   /* 
 void orc_hubbard_bare_green(orc_solver* s, double mu, double t1, double t2, double t3);
 /* constructor tail: Π0 = bubbles(G0); G = Dyson(Σ0, Gbare); Π = bubbles(G)  (src/ParquetSolver.jl:97-114) */
 void orc_init(orc_solver* s);
-/* symmetry groups of init_sym_grp! (src/ParquetSolver.jl:203-231); local solver only */
+/* symmetry groups of init_sym_grp! (src/ParquetSolver.jl:203-231; NL src/nonlocal/ParquetSolver.jl:198-300) */
 int  orc_init_sym_grp(orc_solver* s);
+/* class table of one installed group, for tests: id 0 SGΣ, 1 SGpp[1] (= SGph[1]), 2 SGpp[2], 3 SGpp[3],
+ * 4 SGppL[3], 5 SGph[2], 6 SGph[3], 7 SGphL[3], 8 SG0pp2, 9 SG0ph2.  orc_sym_size returns the number
+ * of elements (-1: identity group or unknown id); orc_sym_classes writes per element the
+ * representative's linear index and the operation (bit 0 sgn, bit 1 con) */
+int64_t orc_sym_size(const orc_solver* s, int32_t id);
+int  orc_sym_classes(const orc_solver* s, int32_t id, int64_t* rep, int32_t* op);
 
 /* stages of iterate_solver! (src/solve.jl:4-116) */
 void orc_dyson(orc_solver* s);

@@ -70,6 +70,8 @@ def lib():
             "orc_hubbard_bare_green": (None, [vp] + [ctypes.c_double] * 4),
             "orc_init": (None, [vp]),
             "orc_init_sym_grp": (ctypes.c_int, [vp]),
+            "orc_sym_size": (ctypes.c_int64, [vp, i32]),
+            "orc_sym_classes": (ctypes.c_int, [vp, i32, ctypes.c_void_p, ctypes.c_void_p]),
             "orc_dyson": (None, [vp]),
             "orc_bubbles": (None, [vp]),
             "orc_build_k3_cache": (None, [vp]),
@@ -163,6 +165,16 @@ class Oracle:
 
     def init_sym_grp(self):
         assert self.lib.orc_init_sym_grp(self.h) == 0
+
+    def sym_classes(self, gid):
+        """(representative, operation) per element of an installed group (ids: fdga_oracle.h)."""
+        n = self.lib.orc_sym_size(self.h, gid)
+        if n < 0:
+            raise KeyError(gid)
+        rep = np.empty(n, dtype=np.int64)
+        op = np.empty(n, dtype=np.int32)
+        assert self.lib.orc_sym_classes(self.h, gid, rep.ctypes.data, op.ctypes.data) == 0
+        return rep, op
 
     # stages
     def dyson(self): self.lib.orc_dyson(self.h)

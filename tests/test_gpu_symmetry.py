@@ -1,7 +1,9 @@
-"""Symmetry-class evaluation on the GPU (init_sym_grp!, src/ParquetSolver.jl:203-231).
+"""Symmetry-class evaluation on the GPU (init_sym_grp!, src/ParquetSolver.jl:203-231; lattice
+solver src/nonlocal/ParquetSolver.jl:198-300).
 
-With the frequency symmetry groups installed, every SG-evaluated stage computes one
-representative per class and writes op(value) to the other members.  The inputs of
+With the symmetry groups installed, every SG-evaluated stage gives op(value of the class
+representative) at the other members (the lattice groups add the square lattice's reflection and
+rotation and the momentum forms of the frequency symmetries).  The inputs of
 `make_pair` are random and NOT symmetric, so the comparison with the oracle checks the class
 partition, the choice of representative and every member's operation exactly: a different
 representative or a missing conjugation changes the result at O(1), not at rounding level.
@@ -14,12 +16,13 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from helpers import CASES, make_pair, relerr
+from helpers import CASES, EXTRA_CASES, make_pair, relerr
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 CH_ORDER = (po.PCH, po.ACH, po.TCH)
-LOCAL = ["local_ref", "local_vertexref"]
+SYM_CASES = ["local_ref", "local_vertexref", "nl_L2", "nl_L3", "nl_LG4_L2", "nl_L6"]
+ALL_CASES = {**CASES, **EXTRA_CASES}
 
 
 @pytest.fixture(scope="module")
@@ -43,7 +46,7 @@ def _cmp_vertex(S, O, vtx_h, vtx_o, classes, what):
             assert e < TOL, f"{what} ch={ch} cls={cls}: rel err {e:.3e}"
 
 
-@pytest.mark.parametrize("case", LOCAL)
+@pytest.mark.parametrize("case", SYM_CASES)
 def test_sym_bse_stages(case, fdga_mod):
     S, O = _pair_sym(case)
     fdga_mod.build_K3_cache(S)
@@ -70,7 +73,7 @@ def test_sym_bse_stages(case, fdga_mod):
     _cmp_vertex(S, O, 1, po.VTX_FBUFF, (3,), f"{case} K3")
 
 
-@pytest.mark.parametrize("case", LOCAL)
+@pytest.mark.parametrize("case", SYM_CASES)
 @pytest.mark.parametrize("strategy", ["scPA", "fdPA"])
 def test_sym_sde(case, strategy, fdga_mod):
     from fdga import _lib as L
@@ -80,14 +83,14 @@ def test_sym_sde(case, strategy, fdga_mod):
     arrays = [(L.SIGMA, po.SIGMA, "Σ")]
     # Lpp/Lph hold the last SDE level (the RefVertex); with null tables that level is exactly
     # zero, the GPU path skips it and its Lpp/Lph keep the previous level
-    if CASES[case][4]:
+    if ALL_CASES[case][4]:
         arrays += [(L.LPP, po.LPP, "Lpp"), (L.LPH, po.LPH, "Lph")]
     for hw, ow, name in arrays:
         e = relerr(S.get(hw), O.get(ow))
         assert e < TOL, f"{case} {strategy} {name}: rel err {e:.3e}"
 
 
-@pytest.mark.parametrize("case", LOCAL)
+@pytest.mark.parametrize("case", SYM_CASES)
 @pytest.mark.parametrize("strategy", ["scPA", "fdPA"])
 def test_sym_full_iteration(case, strategy, fdga_mod):
     S, O = _pair_sym(case, seed=3)
@@ -116,10 +119,26 @@ def test_sym_changes_result_and_reset_restores_dense(fdga_mod):
     assert relerr(x_sym, x_dense) > 1e-3
 
 
-def test_sym_nonlocal_is_refused(fdga_mod):
-    S, _ = make_pair("nl_L2")
-    with pytest.raises(fdga_mod.FdgaError, match="NL_ParquetSolver"):
-        S.init_sym_grp()
+def test_sym_lattice_skips_tiles_and_changes_result(fdga_mod):
+    """Lattice groups: the K3 contractions compute only the tiles holding representatives, the
+    class evaluation differs from the dense one at O(1) on asymmetric inputs, and reset_sym_grp!
+    gives back the dense result."""
+    S, O = make_pair("nl_L6", seed=11)
+    x0 = fdga_mod.flatten(S)
+    dense_flops = [S.gemm_flops(left) for left in (1, 0)]
+    S.init_sym_grp()
+    sym_flops = [S.gemm_flops(left) for left in (1, 0)]
+    assert all(s < 0.5 * d for s, d in zip(sym_flops, dense_flops)), (sym_flops, dense_flops)
+    # scPA: reset_sym_grp! leaves SG0pp2 / SG0ph2 (fdPA's reference system) installed
+    fdga_mod.iterate_solver(S, "scPA")
+    x_sym = fdga_mod.flatten(S)
+    fdga_mod.unflatten(S, x0)
+    S.reset_sym_grp()
+    fdga_mod.iterate_solver(S, "scPA")
+    O.iterate(po.SCPA)
+    x_dense = fdga_mod.flatten(S)
+    assert relerr(x_dense, O.flatten()) < TOL
+    assert relerr(x_sym, x_dense) > 1e-3
 
 
 # ----------------------------------------------------------------------- reference goldens

@@ -243,3 +243,122 @@ def test_symmetrized_solution_obeys_symmetries():
         np.testing.assert_allclose(K2[::-1, ::-1], np.conj(K2), atol=1e-14)   # sK2pp1/ph1
     S = o.get(po.SIGMA)
     np.testing.assert_allclose(S[::-1], -np.conj(S), atol=1e-14)   # sΣ: iΣ(-ν) = -(iΣ(ν))*
+
+
+
+# ---------------------------------------------------------------- symmetry classes
+# init_sym_grp! (src/ParquetSolver.jl:203-231; NL src/nonlocal/ParquetSolver.jl:198-300) with the
+# generators of src/symmetries.jl and src/nonlocal/symmetries.jl, restated here on value tuples
+# (momenta as BrillouinPoint pairs (j1, j2); linear index j1 + L j2, first mesh index fastest).
+# The class partition is checked against the group's orbits computed by union-find (independent
+# of the exploration order), the representative against the lowest linear index, and each
+# member's operation wherever all paths through the class give the same one.
+SYM_BOX = dict(nG=8, nK1=5, nK2=(3, 3), nK3=(2, 2), mPinu=2, T=0.25, U=1.2)
+SGN, CON = 1, 2
+
+
+def _sym_groups(L, nonlocal_):
+    b = SYM_BOX
+    F = lambda N: ("f", N)
+    B = lambda N: ("b", N)
+    fm = lambda n: -n - 1                  # -ν
+    neg = lambda k: ((-k[0]) % L, (-k[1]) % L)
+    ref = lambda k: (k[1], k[0])           # _ref: (x, y) -> (y, x)
+    rot = lambda k: (k[1], (-k[0]) % L)    # _rot: (x, y) -> fold_back((y, -x))
+
+    def g(freq, kmap):
+        if not nonlocal_:
+            return lambda w: freq(w)
+        return lambda w: (lambda r: (r[0] + (kmap(w[-1]),), r[1]))(freq(w[:-1]))
+
+    spatial = [g(lambda w: (w, 0), ref), g(lambda w: (w, 0), rot)] if nonlocal_ else []
+    sS = g(lambda w: ((fm(w[0]),), SGN | CON), neg)
+    sK1 = g(lambda w: ((-w[0],), CON), neg)
+    sK2pp1 = g(lambda w: ((-w[0], fm(w[1])), CON), neg)
+    sK2pp2 = g(lambda w: ((w[0], w[0] - w[1] - 1), 0), lambda k: k)
+    sK2ph2 = g(lambda w: ((-w[0], w[0] + w[1]), 0), neg)
+    sK3pp1 = g(lambda w: ((-w[0], fm(w[1]), fm(w[2])), CON), neg)
+    sK3pp2 = g(lambda w: ((w[0], w[2], w[1]), 0), lambda k: k)
+    sK3pp3 = g(lambda w: ((w[0], w[0] - w[1] - 1, w[0] - w[2] - 1), 0), lambda k: k)
+    sK3ph3 = g(lambda w: ((-w[0], w[0] + w[1], w[0] + w[2]), 0), neg)
+    dS, d1 = [F(b["nG"])], [B(b["nK1"])]
+    d2 = [B(b["nK2"][0]), F(b["nK2"][1])]
+    d3 = [B(b["nK3"][0]), F(b["nK3"][1]), F(b["nK3"][1])]
+    return {0: (dS, [sS]), 1: (d1, [sK1]), 2: (d2, [sK2pp1, sK2pp2]), 3: (d3, [sK3pp1, sK3pp2, sK3pp3]),
+            4: (d3, [sK3pp1, sK3pp3]), 5: (d2, [sK2pp1, sK2ph2]), 6: (d3, [sK3pp1, sK3pp2, sK3ph3]),
+            7: (d3, [sK3pp1, sK3ph3]), 8: (d2, [sK2pp1, sK2pp2]), 9: (d2, [sK2pp1, sK2ph2])}, spatial
+
+
+def _mesh_points(dims, L, nonlocal_):
+    axes = []
+    for kind, N in dims:
+        axes.append(list(range(-(N - 1), N)) if kind == "b" else list(range(-N, N)))
+    if nonlocal_:
+        axes.append([(j % L, j // L) for j in range(L * L)])
+    pts = [()]
+    for ax in axes:                        # first index fastest
+        pts = [p + (v,) for v in ax for p in pts]
+    index = {p: i for i, p in enumerate(pts)}
+    return pts, index
+
+
+@pytest.mark.parametrize("nonlocal_", [False, True])
+def test_symmetry_classes_are_the_group_orbits(nonlocal_):
+    L = 4 if nonlocal_ else 1
+    o = po.Oracle(nonlocal_=nonlocal_, LG=L, L=L, **SYM_BOX)
+    if nonlocal_:
+        o.hubbard(0.3, 1.0, -0.2)
+    else:
+        o.siam(0.3, 0.7, None)
+    o.init()
+    o.init_sym_grp()
+    groups, spatial = _sym_groups(L, nonlocal_)
+    for gid, (dims, gens) in groups.items():
+        gens = gens + spatial
+        pts, index = _mesh_points(dims, L, nonlocal_)   # Σ's momentum mesh is G's: LG = L here
+        n = len(pts)
+        edges = []
+        for i, w in enumerate(pts):
+            for gen in gens:
+                w2, op = gen(w)
+                j = index.get(w2)
+                if j is not None:
+                    edges.append((i, j, op))
+        parent = list(range(n))
+
+        def find(x):
+            while parent[x] != x:
+                parent[x] = parent[parent[x]]
+                x = parent[x]
+            return x
+        for i, j, _ in edges:
+            ri, rj = find(i), find(j)
+            if ri != rj:
+                parent[max(ri, rj)] = min(ri, rj)
+        root = np.array([find(i) for i in range(n)])
+        rep, op = o.sym_classes(gid)
+        assert rep.size == n, gid
+        # representatives: the lowest linear index of the orbit (find() returns the orbit minimum)
+        np.testing.assert_array_equal(rep, root, err_msg=f"group {gid}")
+        # operations: forced wherever every path in the class gives the same one
+        adj = [[] for _ in range(n)]
+        for i, j, o_ in edges:
+            adj[i].append((j, o_))
+            adj[j].append((i, o_))
+        forced = np.full(n, -1)
+        conflict = set()
+        for r in np.unique(root):
+            forced[r] = 0
+            stack = [r]
+            while stack:
+                x = stack.pop()
+                for y, o_ in adj[x]:
+                    want = forced[x] ^ o_
+                    if forced[y] < 0:
+                        forced[y] = want
+                        stack.append(y)
+                    elif forced[y] != want:
+                        conflict.add(r)
+        ok = ~np.isin(root, list(conflict))
+        assert ok.sum() > n // 4, f"group {gid}: too few classes with a unique operation"
+        np.testing.assert_array_equal(op[ok], forced[ok], err_msg=f"group {gid}")
