@@ -181,12 +181,25 @@ def main_c2(args):
         t_bse = (st["gemm_L_K3"] + st["gemm_K3"]) * 1e-3
         res[prec] = {"ms_per_step": ms, "iters_per_s": 1000.0 / ms, "stage_ms": st,
                      "bse_tflops": f_dense / t_bse / 1e12}
+        if prec == "fp32" and not args.no_sym:
+            # the same iteration with init_sym_grp!'s lattice groups installed (src/nonlocal/ParquetSolver.jl:
+            # 198-300): a different result on this workload, whose synthetic reference K3 is not symmetric
+            t0 = time.perf_counter()
+            S.init_sym_grp()
+            log(f"bench c2[{prec}, lattice groups]: groups built ({time.perf_counter() - t0:.1f} s)")
+            fdga.unflatten(S, x0)
+            ms_g, st_g = time_path(S, args, names)
+            log(f"bench c2[{prec}, lattice groups]: {ms_g:.2f} ms/iteration; stages "
+                + ", ".join(f"{k}={v:.2f}" for k, v in st_g.items()))
+            res["fp32_groups"] = {"ms_per_step": ms_g, "iters_per_s": 1000.0 / ms_g, "stage_ms": st_g,
+                                  "bse_gemm_flops_executed": S.gemm_flops(1) + S.gemm_flops(0)}
         S.close()
     a, b = first["fp32"], first["fp64"]
     rel = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     print(json.dumps({"workload": "c2: NL Hubbard fdPA, L=16, beta=5, U=2, nK1=64, nK2=(32,32), nK3=(16,32), mPinu=32, "
                                   "nG=256, synthetic local reference vertex", "steps": args.steps, "warmup": args.warmup,
                       "fp64": res["fp64"], "fp32_mfma": res["fp32"],
+                      "fp32_mfma_with_lattice_groups": res.get("fp32_groups"),
                       "fp32_vs_fp64_state_relerr_after_one_iteration": rel}), flush=True)
 
 

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <future>
 #include <vector>
 
 #include "../../include/fdga.h"
@@ -61,6 +62,7 @@ struct fdga_ctx {
     double2* Wtail[3] = {nullptr, nullptr, nullptr};               // their tail sums (k_tail_sums)
     double2* Ptail[4] = {nullptr, nullptr, nullptr, nullptr};      // tail sums of Π0pp, Π0ph, Πpp, Πph
     size_t ntail = 0;                                               // elements of one tail-sum array
+    double2* pchunk[2] = {nullptr, nullptr};   // lattice bubble: chunk sums of the last Πpp / Πph [P, chunk, Ω]
     int w_fresh = 0;                                                // Wc valid for the current iteration
     // lattice BSE_K2! / BSE_L_K2! tables and GEMM result (s-wave solver only)
     double2 *k2x = nullptr, *k2xf = nullptr, *k2y = nullptr, *lk2x = nullptr;
@@ -323,6 +325,14 @@ std::vector<BubbleFold> bubble_folds(int L, int LG) {
 }
 
 // band tail sums of one array on the bubble mesh (rows of 2 nVP entries)
+// ω entries per block of the lattice bubble (k_bubble_nl): 16 at L = 16 (16 x 256 complex in LDS);
+// 6 at L = 24 (6 x 576 complex + 24 twiddles, 55.7 KB); otherwise as many L x L tiles as fit 64 KB
+int bubble_nb(int L) {
+    if (L == 16) return 16;
+    if (L == 24) return 6;
+    return std::max(1, std::min(16, (65536 / (int)sizeof(double2) - L) / (L * L)));
+}
+
 int tail_sums(fdga_ctx* c, const double2* X, double2* T) {
     const size_t nrows = (size_t)c->NWP * c->NP;
     k_tail_sums<<<(unsigned)nrows, 256, 0, c->run>>>(X, T, c->nVP, c->wb, nrows);
@@ -349,22 +359,27 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     HIPCHK(c, hipMemcpyAsync(c->tmpG1, Gsrc, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
     if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG))) return -3;
     // the Π_R products and the backward DFT over R' (src/nonlocal/bubble.jl:110-155) in one pass
-    const int L = d.L;
+    const int L = d.L, NB = bubble_nb(L);
+    const dim3 grid(cdiv(c->NVP, NB), c->NWP);
+    double2 *cpp = c->pchunk[0], *cph = c->pchunk[1];
     if (L == 16) {
-        constexpr int NB = 16;
-        k_bubble_nl<16, NB><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)NB * 256 * sizeof(double2), c->run>>>(
-            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB);
+        k_bubble_nl<16, 16, 512><<<grid, 512, (size_t)NB * 256 * sizeof(double2), c->run>>>(
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph);
     } else if (L == 24) {
-        constexpr int NB = 6;   // 6 x 576 complex + 24 twiddles in LDS (55.7 KB)
-        k_bubble_nl<24, NB><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)(NB * 576 + 24) * sizeof(double2), c->run>>>(
-            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB);
+        k_bubble_nl<24, 6><<<grid, 256, (size_t)(NB * 576 + 24) * sizeof(double2), c->run>>>(
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph);
     } else {
-        const int NB = std::max(1, std::min(16, (65536 / (int)sizeof(double2) - L) / (L * L)));
-        k_bubble_nl<0, 0><<<dim3(cdiv(c->NVP, NB), c->NWP), 256, (size_t)(NB * L * L + L) * sizeof(double2), c->run>>>(
-            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB);
+        k_bubble_nl<0, 0><<<grid, 256, (size_t)(NB * L * L + L) * sizeof(double2), c->run>>>(
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph);
     }
     LAUNCHCHK(c);
-    return tail_sums(c, pp, pi_tail(c, pp)) || tail_sums(c, ph, pi_tail(c, ph));
+    // tail sums from the bubble's chunk sums
+    const size_t nrows = (size_t)c->NWP * c->NP;
+    for (int k = 0; k < 2; ++k)
+        k_tail_sums_chunked<<<(unsigned)nrows, 256, 0, c->run>>>(k ? ph : pp, k ? cph : cpp, pi_tail(c, k ? ph : pp),
+                                                                 c->nVP, c->wb, NB, (int)grid.x, c->NWP, c->NP, nrows);
+    LAUNCHCHK(c);
+    return 0;
 }
 
 void stage_begin(fdga_ctx* c) {
@@ -956,6 +971,9 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
         rc |= dalloc(c, p, c->ntail);
     for (double2** p : {&c->Lpp, &c->Lph, &c->tmpL1, &c->tmpL2}) rc |= dalloc(c, p, c->nL);
     if (d.nonlocal) {
+        const size_t nch = (size_t)c->NP * cdiv(c->NVP, bubble_nb(d.L)) * c->NWP;
+        rc |= dalloc(c, &c->pchunk[0], nch);
+        rc |= dalloc(c, &c->pchunk[1], nch);
         rc |= dalloc(c, &c->k2x, (size_t)c->NV2 * (2 * (size_t)c->wb) * c->NW2);
         rc |= dalloc(c, &c->k2xf, (size_t)c->NV2 * 2 * c->NW2);
         rc |= dalloc(c, &c->k2y, (size_t)c->NV2 * c->NW2 * c->NP);
@@ -1008,7 +1026,7 @@ int fdga_destroy(fdga_ctx* c) {
     for (double2* p : {c->Gbare, c->G0, c->Sig0, c->G, c->Sig, c->tmpG1, c->tmpG2, c->tmpG3, c->Pi0pp, c->Pi0ph,
                        c->Pipp, c->Piph, c->Wc[0], c->Wc[1], c->Wc[2], c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C,
                        c->stage, c->Wtail[0], c->Wtail[1], c->Wtail[2], c->Ptail[0], c->Ptail[1], c->Ptail[2], c->Ptail[3],
-                       c->k2x, c->k2xf, c->k2y, c->lk2x})
+                       c->k2x, c->k2xf, c->k2y, c->lk2x, c->pchunk[0], c->pchunk[1]})
         if (p) (void)hipFree(p);
     for (auto* t : c->tab) if (t) (void)hipFree(t);
     for (auto* p : c->cache) if (p) (void)hipFree(p);
@@ -1244,10 +1262,14 @@ int fdga_init_sym_grp(fdga_ctx* c) {
     rc |= upload_sym(c, SG_PP2, pp2, n2);
     rc |= upload_sym(c, SG_PH2, ph2, n2);
     const size_t nk3 = c->n3 * c->NP;
-    rc |= upload_sym(c, SG_PP3, build(m3, gens({G31, G32, G3pp}), k3dev), nk3);
-    rc |= upload_sym(c, SG_PH3, build(m3, gens({G31, G32, G3ph}), k3dev), nk3);
-    rc |= upload_sym(c, SG_PPL3, build(m3, gens({G31, G3pp}), k3dev), nk3);
-    rc |= upload_sym(c, SG_PHL3, build(m3, gens({G31, G3ph}), k3dev), nk3);
+    // the four K3 tables (tens of millions of elements on a 16 x 16 lattice) are built concurrently
+    const std::vector<Generator> k3gens[4] = {gens({G31, G32, G3pp}), gens({G31, G32, G3ph}), gens({G31, G3pp}),
+                                              gens({G31, G3ph})};
+    std::future<Classes> k3cls[4];
+    for (int k = 0; k < 4; ++k)
+        k3cls[k] = std::async(std::launch::async, [&m3, &k3gens, k, k3dev] { return build(m3, k3gens[k], k3dev); });
+    const int k3ids[4] = {SG_PP3, SG_PH3, SG_PPL3, SG_PHL3};
+    for (int k = 0; k < 4; ++k) rc |= upload_sym(c, k3ids[k], k3cls[k].get(), nk3);
     rc |= upload_sym(c, SG_0PP2, pp2, n2);
     rc |= upload_sym(c, SG_0PH2, ph2, n2);
     if (rc) return -2;

@@ -314,11 +314,14 @@ __device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, cons
 }
 
 // LT: L fixed at compile time (16: radix-4 FFT; 24: unrolled direct DFT), or 0 for a run-time
-// L <= 32 (direct DFT through private arrays, for small test lattices).  Block: 256 threads.
-template <int LT, int NBT>
-__global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, double2* __restrict__ ph,
+// L <= 32 (direct DFT through private arrays, for small test lattices).  Block: NT threads (512 at
+// L = 16, held to 128 VGPRs: two blocks per CU, twice the waves on the same LDS tiles, to hide the
+// latency of the G(R) gathers).
+template <int LT, int NBT, int NT = 256>
+__global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __restrict__ pp, double2* __restrict__ ph,
                                                    const double2* __restrict__ GR, const BubbleFold* __restrict__ folds,
-                                                   int nG, int Lr, int nW, int nV, double T, int NBr) {
+                                                   int nG, int Lr, int nW, int nV, double T, int NBr,
+                                                   double2* __restrict__ cpp, double2* __restrict__ cph) {
     extern __shared__ double2 sm[];   // NB * L * L (+ L twiddles unless L = 16)
     const int L = LT ? LT : Lr, LL = L * L, NB = NBT ? NBT : NBr;
     double2* s = sm;
@@ -337,10 +340,10 @@ __global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, dou
         // products (a compile-time trip count when L and NB are fixed, so that the loads of
         // several products are in flight together)
         if (LT && NBT) {
-            constexpr int NQ = (NBT * (LT ? LT : 1) * (LT ? LT : 1) + 255) / 256;
+            constexpr int NQ = (NBT * (LT ? LT : 1) * (LT ? LT : 1) + NT - 1) / NT;
 #pragma unroll 8
             for (int q = 0; q < NQ; ++q) {
-                const int e = tid + 256 * q;
+                const int e = tid + NT * q;
                 if (e < NB * LL) {
                     const int wl = e % NB, R = e / NB;
                     s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : czero();
@@ -372,14 +375,33 @@ __global__ __launch_bounds__(256) void k_bubble_nl(double2* __restrict__ pp, dou
                 }
                 if (pass == 0) {
                     for (int k = 0; k < L; ++k) s[(base + k) * NB + wl] = x[k];
-                } else if (wl < nb) {
-                    double2* out = which == 0 ? pp : ph;
-                    const size_t o = (size_t)(iw0 + wl) + (size_t)(2 * nV) * (size_t)iW;
-                    const size_t qs = (size_t)(2 * nV) * (size_t)(2 * nW - 1);
-                    for (int k = 0; k < L; ++k) out[o + qs * (size_t)(r + L * k)] = x[k];
+                } else {
+                    // back into the thread's own column, for the chunk sums below
+                    for (int k = 0; k < L; ++k) s[(r + L * k) * NB + wl] = x[k];
+                    if (wl < nb) {
+                        double2* out = which == 0 ? pp : ph;
+                        const size_t o = (size_t)(iw0 + wl) + (size_t)(2 * nV) * (size_t)iW;
+                        const size_t qs = (size_t)(2 * nV) * (size_t)(2 * nW - 1);
+                        for (int k = 0; k < L; ++k) out[o + qs * (size_t)(r + L * k)] = x[k];
+                    }
                 }
             }
             __syncthreads();
+        }
+        // chunk sums C[P + LL (chunk + chunks iΩ)] = Σ_{ω in this block's chunk} Π(ω, Ω, P) for the
+        // tail sums (k_tail_sums_chunked); each thread a momentum, the ω order rotated by P so that
+        // the lanes' LDS reads spread over the banks
+        double2* cs = which == 0 ? cpp : cph;
+        if (cs) {
+            for (int P = tid; P < LL; P += blockDim.x) {
+                double sr = 0.0, si = 0.0;
+                for (int j = 0; j < NB; ++j) {
+                    const int wl = (j + P) % NB;
+                    if (wl < nb) { const double2 v = s[P * NB + wl]; sr += v.x; si += v.y; }
+                }
+                cs[(size_t)P + (size_t)LL * (blockIdx.x + (size_t)gridDim.x * iW)] = cmk(sr, si);
+            }
+            __syncthreads();   // s is reused by the next products
         }
     }
 }
@@ -1221,10 +1243,77 @@ __global__ __launch_bounds__(256) void k_tail_sums(const double2* __restrict__ X
     double2* tr = tl + (wb + 1);
     const int nout = nVP - wb;   // entries of each side outside the band
     double lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
-    for (int q = tid; q < nout; q += 256) {
+    int q = tid;
+    for (; q + 3 * 256 < nout; q += 4 * 256) {   // four loads of each side in flight
+        double2 a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = x[q + 256 * u];
+            b[u] = x[2 * nVP - 1 - q - 256 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { lr += a[u].x; li += a[u].y; rr += b[u].x; ri += b[u].y; }
+    }
+    for (; q < nout; q += 256) {
         const double2 a = x[q], b = x[2 * nVP - 1 - q];
         lr += a.x; li += a.y; rr += b.x; ri += b.y;
     }
+    lr = wave_sum(lr); li = wave_sum(li); rr = wave_sum(rr); ri = wave_sum(ri);
+    if (lane == 0) { sh[0][wv] = lr; sh[1][wv] = li; sh[2][wv] = rr; sh[3][wv] = ri; }
+    __syncthreads();
+    double cl_r = 0.0, cl_i = 0.0, cr_r = 0.0, cr_i = 0.0;   // running sums carried across chunks
+#pragma unroll
+    for (int k = 0; k < NWV; ++k) { cl_r += sh[0][k]; cl_i += sh[1][k]; cr_r += sh[2][k]; cr_i += sh[3][k]; }
+    if (tid == 0) { tl[0] = cmk(cl_r, cl_i); tr[wb] = cmk(cr_r, cr_i); }
+    for (int c0 = 0; c0 < wb; c0 += 256) {
+        __syncthreads();   // sh is reused
+        const int i = c0 + tid;
+        const double2 a = i < wb ? x[nout + i] : czero();
+        const double2 b = i < wb ? x[nVP + wb - 1 - i] : czero();
+        double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+        wave_incl_scan(ar, ai, lane);
+        wave_incl_scan(br, bi, lane);
+        if (lane == WAVE - 1) { sh[0][wv] = ar; sh[1][wv] = ai; sh[2][wv] = br; sh[3][wv] = bi; }
+        __syncthreads();
+        double pl_r = cl_r, pl_i = cl_i, pr_r = cr_r, pr_i = cr_i;
+        for (int k = 0; k < wv; ++k) { pl_r += sh[0][k]; pl_i += sh[1][k]; pr_r += sh[2][k]; pr_i += sh[3][k]; }
+        if (i < wb) {
+            tl[i + 1] = cmk(pl_r + ar, pl_i + ai);
+            tr[wb - 1 - i] = cmk(pr_r + br, pr_i + bi);
+        }
+#pragma unroll
+        for (int k = 0; k < NWV; ++k) { cl_r += sh[0][k]; cl_i += sh[1][k]; cr_r += sh[2][k]; cr_i += sh[3][k]; }
+    }
+}
+
+// k_tail_sums for an array whose chunk sums C[P + NP (chunk + NC iΩ)] over NB consecutive ω
+// were written by the lattice bubble: the parts outside the band are the sums of the chunks that
+// lie wholly outside it plus the few entries of the chunks it cuts; the band as in k_tail_sums.
+// Reads NC + 2 wb + O(NB) entries per row instead of 2 nVP.
+__global__ __launch_bounds__(256) void k_tail_sums_chunked(const double2* __restrict__ X, const double2* __restrict__ C,
+                                                           double2* __restrict__ T, int nVP, int wb, int NB, int NC,
+                                                           int NWP, int NP, size_t nrows) {
+    constexpr int NWV = 256 / WAVE;
+    __shared__ double sh[4][NWV];
+    const size_t row = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    if (row >= nrows) return;
+    const int iW = (int)(row % NWP), p = (int)(row / NWP);
+    const double2* x = X + row * (size_t)(2 * nVP);
+    double2* tl = T + row * (size_t)(2 * (wb + 1));
+    double2* tr = tl + (wb + 1);
+    const int nout = nVP - wb;                 // far left: ω index < nout; far right: >= nVP + wb
+    const int cl = nout / NB;                  // chunks [0, cl) lie wholly left of the band
+    const int cr = (nVP + wb + NB - 1) / NB;   // chunks [cr, NC) lie wholly right of it
+    double lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+    for (int c = tid; c < NC; c += 256) {
+        if (c < cl || c >= cr) {
+            const double2 v = C[(size_t)p + (size_t)NP * (c + (size_t)NC * iW)];
+            if (c < cl) { lr += v.x; li += v.y; } else { rr += v.x; ri += v.y; }
+        }
+    }
+    for (int k = cl * NB + tid; k < nout; k += 256) { const double2 v = x[k]; lr += v.x; li += v.y; }
+    for (int k = nVP + wb + tid; k < min(cr * NB, 2 * nVP); k += 256) { const double2 v = x[k]; rr += v.x; ri += v.y; }
     lr = wave_sum(lr); li = wave_sum(li); rr = wave_sum(rr); ri = wave_sum(ri);
     if (lane == 0) { sh[0][wv] = lr; sh[1][wv] = li; sh[2][wv] = rr; sh[3][wv] = ri; }
     __syncthreads();
