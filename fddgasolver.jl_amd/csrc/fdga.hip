@@ -791,8 +791,27 @@ static hipError_t k3_gemm(fdga_ctx* c, const int* chans, int n, int which, const
     const GemmSchedOne sc = sched_one(NV, NV, NV, cdiv(NV, tile_bm(tile)), cdiv(NV, tile_bn(tile)), n * c->NW3 * c->NP, bl, nl);
     return zgemm_run(c->d.precision, tile, (size_t)sc.nitems, c->run, sc, a, b, e);
 }
+// Without symmetry groups a (p, a, t) contraction over many (Ω, P) batches runs as two launches, (p, a)
+// and then t with its post-processing (src/BSE_templates.jl:176-177) in the epilogue, where its a
+// channel is complete: one K3-sized pass (k_post_t) less.  Few batches (the local solver) keep one
+// launch, where a second, small launch would not fill the GPU.
+static bool split_t(const fdga_ctx* c, const int* chans, int n, bool sym) {
+    return !sym && n == 3 && chans[0] == PCH && chans[1] == ACH && chans[2] == TCH && (size_t)c->NW3 * c->NP >= 512;
+}
+static const int kChansPA[2] = {PCH, ACH};
+static const int kChansT[1] = {TCH};
+
 static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     const bool sym = k3_sym(c, chans, n, true);
+    if (split_t(c, chans, n, sym)) {
+        const LK3Set l1 = lk3_set(c, kChansPA, 2, false, false), l2 = lk3_set(c, kChansT, 1, false, true);
+        kernel_begin(c);
+        if (k3_gemm(c, kChansPA, 2, 0, l1.a, l1.b, l1.e) != hipSuccess ||
+            k3_gemm(c, kChansT, 1, 0, l2.a, l2.b, l2.e) != hipSuccess)
+            return fail(c, "BSE_L_K3 GEMM launch failed", -3);
+        kernel_end(c, 9);
+        return 0;
+    }
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const LK3Set ls = lk3_set(c, chans, n, sym, t_epi);
     kernel_begin(c);
@@ -872,20 +891,30 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n) {
     }
     return 0;
 }
-// BSE_K3! for `n` channels: right operands and column sums, then one GEMM launch
+// the BSE_K3! GEMM launch of `n` channels with the loaders of the solver kind and precision
+static hipError_t k3_gemm_set(fdga_ctx* c, const int* chans, int n, const K3Set& ks) {
+    const K3_Ep e = ks.e();
+    if (c->NP == 1)
+        return ks.r32() ? k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b32(), e) : k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b(), e);
+    return ks.op32() ? k3_gemm(c, chans, n, 1, ks.a32(), ks.b32(), e) : k3_gemm(c, chans, n, 1, ks.a<false>(), ks.b(), e);
+}
+// BSE_K3! for `n` channels: right operands and column sums, then the GEMM launch(es)
 static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     const bool sym = k3_sym(c, chans, n, false);
+    if (split_t(c, chans, n, sym)) {
+        if (k3_rhs(c, k3_set(c, chans, n, false, false), n)) return -3;
+        kernel_begin(c);
+        if (k3_gemm_set(c, kChansPA, 2, k3_set(c, kChansPA, 2, false, false)) != hipSuccess ||
+            k3_gemm_set(c, kChansT, 1, k3_set(c, kChansT, 1, false, true)) != hipSuccess)
+            return fail(c, "BSE_K3 GEMM launch failed", -3);
+        kernel_end(c, 10);
+        return 0;
+    }
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const K3Set ks = k3_set(c, chans, n, sym, t_epi);
     if (k3_rhs(c, ks, n)) return -3;
-    const K3_Bop b = ks.b();
-    const K3_Ep e = ks.e();
     kernel_begin(c);
-    const hipError_t ge = c->NP == 1 ? (ks.r32() ? k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b32(), e)
-                                                 : k3_gemm(c, chans, n, 1, ks.a<true>(), b, e))
-                          : ks.op32() ? k3_gemm(c, chans, n, 1, ks.a32(), ks.b32(), e)
-                                      : k3_gemm(c, chans, n, 1, ks.a<false>(), b, e);
-    if (ge != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
+    if (k3_gemm_set(c, chans, n, ks) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
     return k3_finish(c, c->Fb, chans, n, false, sym, t_epi);
 }
@@ -1832,10 +1861,11 @@ int fdga_set_stage_timing(fdga_ctx* c, int32_t on) {
 // chosen grid and with a chosen tile shape.  Each member's results are those of fdga_iterate on
 // the member alone.  No counterpart in the reference, which solves one parameter point per solver.
 struct GroupItems {   // work items of one tile shape
-    int* items = nullptr;
+    int* items = nullptr;     // lattice groups: the p and a channels' items; else all three channels'
+    int* items_t = nullptr;   // lattice groups: the t channel's items (a second launch)
     GemmGroupGeom* geo = nullptr;
-    int nitems = 0;
-    int64_t computed = 0;   // non-idle items
+    int nitems = 0, nitems_t = 0;
+    int64_t computed = 0;   // non-idle items (both launches)
 };
 
 struct fdga_group {
@@ -1893,31 +1923,45 @@ int group_build_items(fdga_group* g, int tile) {
     std::vector<int> order((size_t)g->n);
     for (int j = 0; j < g->n; ++j) order[(size_t)j] = j;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g->m[(size_t)a]->NV3 > g->m[(size_t)b]->NV3; });
-    std::vector<int> items;
+    std::vector<int> items, items_t;
     int64_t computed = 0;
     for (int j = 0; j < g->n; ++j) {
         const fdga_ctx* c = g->m[(size_t)j];
         const int NV = c->NV3;
         geo[(size_t)j] = GemmGroupGeom{NV, NV, NV, (int)cdiv(NV, BM), (int)cdiv(NV, BN), g->boff_h[(size_t)j]};
     }
+    // batches [b0, b1) of member j in chunks of 8, the tiles of a batch on one XCD
+    auto add = [&](std::vector<int>& v, int j, int tiles, int b0, int b1) {
+        for (int c0 = b0; c0 < b1; c0 += 8)
+            for (int t = 0; t < tiles; ++t)
+                for (int x = 0; x < 8; ++x) {
+                    const int b = c0 + x;
+                    v.push_back(b < b1 ? (j << 24) | (b * tiles + t) : -1);
+                    computed += b < b1;
+                }
+    };
     for (int j : order) {
         const GemmGroupGeom& q = geo[(size_t)j];
         const int tiles = q.tiles_m * q.tiles_n;
         const int nb = g->boff_h[(size_t)j + 1] - g->boff_h[(size_t)j];
         if ((int64_t)nb * tiles >= (1 << 24)) return gfail(g, "grouped GEMM: too many blocks in one member", -1);
-        for (int c0 = 0; c0 < nb; c0 += 8)
-            for (int t = 0; t < tiles; ++t)
-                for (int x = 0; x < 8; ++x) {
-                    const int b = c0 + x;
-                    items.push_back(b < nb ? (j << 24) | (b * tiles + t) : -1);
-                    computed += b < nb;
-                }
+        if (g->nonlocal) {   // slots p, a in the first launch, t (batches from 2 nb / 3 on) in the second
+            add(items, j, tiles, 0, 2 * nb / 3);
+            add(items_t, j, tiles, 2 * nb / 3, nb);
+        } else {
+            add(items, j, tiles, 0, nb);
+        }
     }
     GHIPCHK(g, hipMalloc((void**)&gi.items, items.size() * sizeof(int)));
     GHIPCHK(g, hipMalloc((void**)&gi.geo, geo.size() * sizeof(GemmGroupGeom)));
     GHIPCHK(g, hipMemcpy(gi.items, items.data(), items.size() * sizeof(int), hipMemcpyHostToDevice));
     GHIPCHK(g, hipMemcpy(gi.geo, geo.data(), geo.size() * sizeof(GemmGroupGeom), hipMemcpyHostToDevice));
+    if (!items_t.empty()) {
+        GHIPCHK(g, hipMalloc((void**)&gi.items_t, items_t.size() * sizeof(int)));
+        GHIPCHK(g, hipMemcpy(gi.items_t, items_t.data(), items_t.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     gi.nitems = (int)items.size();
+    gi.nitems_t = (int)items_t.size();
     gi.computed = computed;
     return 0;
 }
@@ -1929,8 +1973,9 @@ int group_upload(fdga_group* g, T** dst, const std::vector<T>& src) {
     return 0;
 }
 
-// per-member functors of the two contractions (the three channels p, a, t in one launch, no
-// symmetry groups); BSE_K3! for both F/Fbuff pointer states
+// per-member functors of the two contractions (the three channels p, a, t, no symmetry groups;
+// lattice groups: the t channel post-processed in its epilogue, in a launch after p and a);
+// BSE_K3! for both F/Fbuff pointer states
 int group_build_functors(fdga_group* g) {
     const int all[3] = {PCH, ACH, TCH};
     std::vector<LK3_Aop> la;
@@ -1943,13 +1988,13 @@ int group_build_functors(fdga_group* g) {
     std::vector<K3_Bop32> b32[2];
     std::vector<K3_Ep> ke[2];
     for (fdga_ctx* c : g->m) {
-        const LK3Set ls = lk3_set(c, all, 3, false, false);
+        const LK3Set ls = lk3_set(c, all, 3, false, g->nonlocal);
         la.push_back(ls.a);
         lb.push_back(ls.b);
         le.push_back(ls.e);
         for (int k = 0; k < 2; ++k) {
             const int s = c->swap_state;
-            const K3Set ks = k3_set(c, all, 3, false, false);
+            const K3Set ks = k3_set(c, all, 3, false, g->nonlocal);
             af[s].push_back(ks.a<true>());
             an[s].push_back(ks.a<false>());
             a32[s].push_back(ks.a32());
@@ -1982,41 +2027,47 @@ void group_graphs_invalidate(fdga_group* g) {
             }
 }
 
-size_t group_grid(const fdga_group* g, int which) {
+size_t group_grid(const fdga_group* g, int which, bool t_launch = false) {
     const GroupItems& gi = g->gi[g->tile[which]];
-    return g->grid[which] ? std::min(g->grid[which], (size_t)gi.nitems) : (size_t)gi.nitems;
+    const size_t n = (size_t)(t_launch ? gi.nitems_t : gi.nitems);
+    return g->grid[which] ? std::min(g->grid[which], n) : n;
 }
 
-// the grouped contraction `which` (0 BSE_L_K3!, 1 BSE_K3!) on the group's stream
-int group_gemm(fdga_group* g, int which) {
+// one launch of the grouped contraction `which` (0 BSE_L_K3!, 1 BSE_K3!) over the item list `sc`
+hipError_t group_gemm_launch(fdga_group* g, int which, const GemmSchedGroup& sc, size_t grid) {
     const int tile = g->tile[which];
-    const GroupItems& gi = g->gi[tile];
-    const GemmSchedGroup sc{gi.items, gi.geo, gi.nitems};
-    const size_t grid = group_grid(g, which);
-    if (g->timing) GHIPCHK(g, hipEventRecord(g->tev[0], g->st));
-    hipError_t e;
-    if (which == 0) {
-        e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<LK3_Aop>{g->lk3a, g->boff, g->n},
-                      GroupOp<LK3_Bop>{g->lk3b, g->boff, g->n}, GroupOp<LK3_Ep>{g->lk3e, g->boff, g->n});
-    } else {
+    if (which == 0)
+        return zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<LK3_Aop>{g->lk3a, g->boff, g->n},
+                         GroupOp<LK3_Bop>{g->lk3b, g->boff, g->n}, GroupOp<LK3_Ep>{g->lk3e, g->boff, g->n});
+    const int s = g->m[0]->swap_state;
+    const GroupOp<K3_Bop> b{(const K3_Bop*)g->k3b[s], g->boff, g->n};
+    const GroupOp<K3_Ep> ep{g->k3e[s], g->boff, g->n};
+    if (g->nonlocal && g->precision == FDGA_FP32)
+        return zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_Aop32>{(const K3_Aop32*)g->k3a[s], g->boff, g->n},
+                         GroupOp<K3_Bop32>{(const K3_Bop32*)g->k3b[s], g->boff, g->n}, ep);
+    if (g->nonlocal)
+        return zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<false>>{(K3_AopT<false>*)g->k3a[s], g->boff, g->n},
+                         b, ep);
+    if (g->precision == FDGA_FP32)
+        return zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<true>>{(K3_AopT<true>*)g->k3a[s], g->boff, g->n},
+                         GroupOp<K3_Bop32>{(const K3_Bop32*)g->k3b[s], g->boff, g->n}, ep);
+    return zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<true>>{(K3_AopT<true>*)g->k3a[s], g->boff, g->n},
+                     b, ep);
+}
+
+// the grouped contraction `which` (0 BSE_L_K3!, 1 BSE_K3!) on the group's stream: one launch, or for
+// lattice groups the p and a channels and then the t channel
+int group_gemm(fdga_group* g, int which) {
+    const GroupItems& gi = g->gi[g->tile[which]];
+    if (which == 1) {
         const int s = g->m[0]->swap_state;
         for (const fdga_ctx* c : g->m)
             if (c->swap_state != s) return gfail(g, "group members are not at the same iteration (F/Fbuff state)");
-        const GroupOp<K3_Bop> b{(const K3_Bop*)g->k3b[s], g->boff, g->n};
-        const GroupOp<K3_Ep> ep{g->k3e[s], g->boff, g->n};
-        if (g->nonlocal && g->precision == FDGA_FP32)
-            e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_Aop32>{(const K3_Aop32*)g->k3a[s], g->boff, g->n},
-                          GroupOp<K3_Bop32>{(const K3_Bop32*)g->k3b[s], g->boff, g->n}, ep);
-        else if (g->nonlocal)
-            e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<false>>{(K3_AopT<false>*)g->k3a[s], g->boff, g->n},
-                          b, ep);
-        else if (g->precision == FDGA_FP32)
-            e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<true>>{(K3_AopT<true>*)g->k3a[s], g->boff, g->n},
-                          GroupOp<K3_Bop32>{(const K3_Bop32*)g->k3b[s], g->boff, g->n}, ep);
-        else
-            e = zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<K3_AopT<true>>{(K3_AopT<true>*)g->k3a[s], g->boff, g->n},
-                          b, ep);
     }
+    if (g->timing) GHIPCHK(g, hipEventRecord(g->tev[0], g->st));
+    hipError_t e = group_gemm_launch(g, which, GemmSchedGroup{gi.items, gi.geo, gi.nitems}, group_grid(g, which));
+    if (e == hipSuccess && gi.nitems_t)
+        e = group_gemm_launch(g, which, GemmSchedGroup{gi.items_t, gi.geo, gi.nitems_t}, group_grid(g, which, true));
     if (e != hipSuccess) return gfail(g, std::string("grouped GEMM launch: ") + hipGetErrorString(e), -3);
     if (g->timing) {
         GHIPCHK(g, hipEventRecord(g->tev[1], g->st));
@@ -2062,7 +2113,7 @@ int group_iterate_eager(fdga_group* g, int strategy, int update_sigma) {
     if (strategy == FDGA_FDPA) {
         if (group_join(g) || group_gemm(g, 0) || group_fork(g)) return -3;
         for (fdga_ctx* c : g->m)
-            if (k3_finish(c, c->FL, all, 3, true, false, false)) return member_err(g, c);
+            if (k3_finish(c, c->FL, all, 3, true, false, g->nonlocal)) return member_err(g, c);
     }
     for (fdga_ctx* c : g->m) {
         if (launch_w<PCH>(c) || launch_w<ACH>(c) || launch_w<TCH>(c)) return member_err(g, c);
@@ -2076,7 +2127,7 @@ int group_iterate_eager(fdga_group* g, int strategy, int update_sigma) {
     }
     if (group_join(g) || group_gemm(g, 1) || group_fork(g)) return -3;
     for (fdga_ctx* c : g->m) {
-        if (k3_finish(c, c->Fb, all, 3, false, false, false)) return member_err(g, c);
+        if (k3_finish(c, c->Fb, all, 3, false, false, g->nonlocal)) return member_err(g, c);
         if (fdga_update_F(c)) return member_err(g, c);
         if (update_sigma && fdga_sde(c, strategy, 1)) return member_err(g, c);
     }
@@ -2157,6 +2208,7 @@ int fdga_group_destroy(fdga_group* g) {
     group_graphs_invalidate(g);
     for (auto& gi : g->gi) {
         if (gi.items) (void)hipFree(gi.items);
+        if (gi.items_t) (void)hipFree(gi.items_t);
         if (gi.geo) (void)hipFree(gi.geo);
     }
     for (void* p : {(void*)g->boff, (void*)g->lk3a, (void*)g->lk3b, (void*)g->lk3e, g->k3a[0], g->k3a[1], g->k3b[0],
