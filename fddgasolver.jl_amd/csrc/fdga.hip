@@ -302,24 +302,18 @@ std::vector<BubbleFold> bubble_folds(int L, int LG) {
     auto md = [](int a, int n) { return ((a % n) + n) % n; };
     for (int r2p = 0; r2p < L; ++r2p)
         for (int r1p = 0; r1p < L; ++r1p) {
-            BubbleFold& b = f[(size_t)r1p + (size_t)L * r2p];
-            b = BubbleFold{};
+            int t[4] = {0, 0, 0, 0}, n = 0;
             for (int R2 = -h; R2 <= h; ++R2) {
                 if (md(R2, L) != r2p) continue;
                 for (int R1 = -h; R1 <= h; ++R1) {
                     if (md(R1, L) != r1p) continue;
-                    double w = 1.0;
-                    if (LG % 2 == 0) {
-                        if (std::abs(R1) == LG / 2) w *= 0.5;
-                        if (std::abs(R2) == LG / 2) w *= 0.5;
-                    }
-                    const int c = b.n++;
-                    b.kp[c] = md(R1, LG) + LG * md(R2, LG);
-                    b.km[c] = md(-R1, LG) + LG * md(-R2, LG);
-                    b.w[c] = w;
-                    if (R1 == 0 && R2 == 0) b.origin |= 1 << c;
+                    int halves = 0;   // weight 2^-halves
+                    if (LG % 2 == 0) halves = (std::abs(R1) == LG / 2) + (std::abs(R2) == LG / 2);
+                    const int kp = md(R1, LG) + LG * md(R2, LG), km = md(-R1, LG) + LG * md(-R2, LG);
+                    t[n++] = kp | (km << 12) | ((R1 == 0 && R2 == 0) << 24) | ((1 + halves) << 25);
                 }
             }
+            f[(size_t)r1p + (size_t)L * r2p] = BubbleFold{t[0], t[1], t[2], t[3]};
         }
     return f;
 }
@@ -938,8 +932,9 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
         *out = c;
         return -1;
     }
-    if (d.L > 32) {
-        fail(c, "the lattice bubble transforms L x L planes in LDS: L must be at most 32");
+    if (d.L > 32 || d.LG > 64) {
+        fail(c, "the lattice bubble transforms L x L planes in LDS and indexes the LG x LG mesh of G(R) in 12 bits: "
+                "L must be at most 32 and LG at most 64");
         *out = c;
         return -1;
     }

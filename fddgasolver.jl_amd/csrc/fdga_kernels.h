@@ -237,12 +237,15 @@ struct BubbleDft<16> {
 // The R in [-L/2, L/2]^2 that fold onto R' = mod(R, L) (up to two per axis), with their
 // k-mesh indices on the LG mesh (+R and -R), weights (1/2 per axis at |R_i| = LG/2 for even LG)
 // and whether R = 0; built once per solver on the host (bubble_folds in the library).
-struct BubbleFold {
-    int n;          // number of R
-    int kp[4], km[4];
-    int origin;     // bit c: R of slot c is the origin
-    double w[4];
-};
+// The R in [-L/2, L/2]^2 that fold onto one R' = mod(R, L), at most 4, packed one per int of an
+// int4 so that a product loads its whole fold with one 16-byte load:
+//   bits 0-11 index of R in G(R) (LG x LG), bits 12-23 index of -R, bit 24 R is the origin,
+//   bits 25-26 weight code (0: no term, 1: weight 1, 2: 1/2, 3: 1/4; half weights at |R| = LG/2)
+typedef int4 BubbleFold;
+__device__ __forceinline__ int fold_kp(int t) { return t & 0xFFF; }
+__device__ __forceinline__ int fold_km(int t) { return (t >> 12) & 0xFFF; }
+__device__ __forceinline__ bool fold_origin(int t) { return (t >> 24) & 1; }
+__device__ __forceinline__ int fold_wcode(int t) { return (t >> 25) & 3; }
 // length 24 = 3 x 8, j = 8 j1 + j2, k = k1 + 3 k2:
 //   X[k1 + 3 k2] = Σ_j2 e^{2πi j2 k2/8} [e^{2πi j2 k1/24} Σ_j1 x[8 j1 + j2] e^{2πi j1 k1/3}]
 // (3-point DFTs, twiddles from tw[k] = e^{2πi k/24}, 8-point FFTs as 4 x 2)
@@ -302,13 +305,18 @@ __device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, cons
     const int NG = 2 * nG;
     double2 acc = czero();
     const int n2nd = which == 0 ? bmf(m, n) : bpf(m, n);
-    for (int c = 0; c < f.n; ++c) {
-        const bool origin = (f.origin >> c) & 1;
+    const int terms[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int t = terms[c], wc = fold_wcode(t);
+        if (wc == 0) break;
+        const bool origin = fold_origin(t);
         auto gat = [&](int k, int kk) -> double2 {
             if (in_f(k, nG)) return GR[(size_t)(k + nG) + (size_t)NG * kk];
             return origin ? cmk(1.0 / (M_PI * T * (2.0 * k + 1.0)), 0.0) : czero();
         };
-        acc = cadd(acc, cscale(cmul(gat(n2nd, f.kp[c]), gat(n, which == 0 ? f.kp[c] : f.km[c])), f.w[c]));
+        const double2 x = cmul(gat(n2nd, fold_kp(t)), gat(n, which == 0 ? fold_kp(t) : fold_km(t)));
+        acc = cadd(acc, wc == 1 ? x : cscale(x, wc == 2 ? 0.5 : 0.25));
     }
     return acc;
 }
