@@ -875,11 +875,16 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) 
 }
 // the right operands R and their column sums of a BSE_K3! launch over `n` channels
 static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n) {
-    const size_t ncol = (size_t)c->NV3 * c->NW3 * c->NP * n;
-    k_k3_R<<<cdiv(ncol * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
+    if (c->NV3 <= WAVE && c->NP > 1) {   // lattice: P-independent operands loaded once per momentum chunk
+        const size_t nwave = (size_t)cdiv(c->NV3, K3R_COLS) * c->NW3 * cdiv(c->NP, K3_PCH) * n;
+        k_k3_R_pchunk<<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
+    } else {
+        const size_t nwave = (size_t)cdiv(c->NV3, K3R_COLS) * c->NW3 * c->NP * n;
+        k_k3_R<<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
+    }
     LAUNCHCHK(c);
     if (ks.op32()) {
-        k_k3_A32<<<dim3((unsigned)(c->NW3 * c->NP * n), cdiv((size_t)c->NV3 * c->NV3, 256)), 256, 0, c->run>>>(
+        k_k3_A32<<<dim3((unsigned)(c->NW3 * cdiv(c->NP, K3_PCH) * n), cdiv((size_t)c->NV3 * c->NV3, 256)), 256, 0, c->run>>>(
             ks.g, ks.kc[0], ks.kc[1], ks.kc[2]);
         LAUNCHCHK(c);
     }

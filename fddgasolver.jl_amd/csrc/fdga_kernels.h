@@ -1038,20 +1038,42 @@ struct K3_Bop32 {
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
-// A32 of the lattice BSE_K3!: the K3_AopT<false> element rounded to FP32.  Grid: x over the
-// batches (Ω, P) of all channel slots, y over chunks of 256 of the Nν3 x Nν3 elements of a batch.
+// A32 of the lattice BSE_K3!: the K3_AopT<false> element rounded to FP32.  Grid: x over (channel
+// slot, Ω, chunk of K3_PCH momenta), y over chunks of 256 of the Nν3 x Nν3 elements; a thread
+// loads its P-independent Fbase element once and takes it through the chunk's momenta, four
+// momenta's loads in flight together.
+constexpr int K3_PCH = 8;
 __global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, K3Chan c2) {
     const int NV = g.NV(), NW = g.NW();
-    const int nn = NV * NV, nb = NW * g.NP;
-    const int o = blockIdx.y * blockDim.x + threadIdx.x;
+    const int nn = NV * NV, npc = (g.NP + K3_PCH - 1) / K3_PCH;
+    const int o = blockIdx.y * 256 + threadIdx.x;
     if (o >= nn) return;
-    const int s = (int)blockIdx.x / nb, bb = (int)blockIdx.x - s * nb;
+    const int s = (int)blockIdx.x / (NW * npc), rest = (int)blockIdx.x - s * NW * npc;
+    const int iW = rest % NW, p0 = (rest / NW) * K3_PCH;
     const K3Chan& c = s == 0 ? c0 : s == 1 ? c1 : c2;
-    const size_t r = (size_t)o + (size_t)nn * bb;
-    const double2 x = c.Fbase[(size_t)o + (size_t)nn * (bb % NW)];
-    const double2 k3 = c.K3r[r];
-    const double2 v = c.ch == TCH ? cadd(x, csub(cscale(k3, 2.0), c.K3a[r])) : cadd(x, k3);
-    c.A32[r] = make_float2((float)v.x, (float)v.y);
+    const bool t = c.ch == TCH;
+    const double2 x = c.Fbase[(size_t)o + (size_t)nn * iW];
+    const int pend = min(p0 + K3_PCH, g.NP);
+    for (int pq = p0; pq < pend; pq += 4) {
+        double2 k3[4], ka[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = pq + u;
+            if (p < pend) {
+                const size_t r = (size_t)o + (size_t)nn * (iW + (size_t)NW * p);
+                k3[u] = c.K3r[r];
+                if (t) ka[u] = c.K3a[r];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = pq + u;
+            if (p < pend) {
+                const double2 v = t ? cadd(x, csub(cscale(k3[u], 2.0), ka[u])) : cadd(x, k3[u]);
+                c.A32[(size_t)o + (size_t)nn * (iW + (size_t)NW * p)] = make_float2((float)v.x, (float)v.y);
+            }
+        }
+    }
 }
 struct K3_Ep {
     K3Geom g; double T; K3Chan c0, c1, c2;
@@ -1113,48 +1135,152 @@ struct K3RChan {
     double2* csum;        // [νp, Ω, P]
     float2* R32;          // when set: R rounded to FP32 goes here instead of R
 };
+// K3R_COLS columns νp = j0 .. j0 + K3R_COLS - 1 per wave: the Π loads are shared and the columns'
+// loads issued together
+constexpr int K3R_COLS = 4;
 template <int CH>
-__device__ __forceinline__ void k3_R_column(const K3Geom& g, const K3RChan& c, int j, int bb, int lane) {
+__device__ __forceinline__ void k3_R_columns(const K3Geom& g, const K3RChan& c, int j0, int bb, int lane) {
+    constexpr int NC = K3R_COLS;
     const int NV = g.NV(), NW = g.NW();
     const int iW = bb % NW, p = bb / NW;
     const int m = iW - (g.n3b - 1);
-    double cr = 0.0, ci = 0.0;
+    double cr[NC], ci[NC];
+#pragma unroll
+    for (int u = 0; u < NC; ++u) cr[u] = ci[u] = 0.0;
     for (int k = lane; k < NV; k += WAVE) {
         const int w = k - g.n3f;
         const size_t pi = g.pidx(m, w, p);
         const double2 P1 = c.Pi[pi], P0 = c.Pi0[pi];
-        const size_t kj = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW);
-        const double2 f0 = c.F0r[kj];
-        const double2 gm = c.Gam[kj];
-        double2 x = cscale(cadd(cmul(csub(P1, P0), f0), cmul(P1, gm)), c.s1);
         const int w_ = crossing(m, w, CH);
-        if (in_f(w_, g.n3f)) {
-            const size_t lo = (size_t)(w_ + g.n3f) + (size_t)NV * ((size_t)j + (size_t)NV * bb);
-            double2 L = c.Lr[lo];
-            if (CH == TCH) L = csub(cscale(L, 2.0), c.La[lo]);
-            x = cadd(x, cscale(cmul(P1, L), c.s2));
+        const bool inb = in_f(w_, g.n3f);
+        double2 f0[NC], gm[NC], L[NC], La[NC];
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+            const int j = j0 + u;
+            if (j < NV) {
+                const size_t kj = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW);
+                f0[u] = c.F0r[kj];
+                gm[u] = c.Gam[kj];
+                if (inb) {
+                    const size_t lo = (size_t)(w_ + g.n3f) + (size_t)NV * ((size_t)j + (size_t)NV * bb);
+                    L[u] = c.Lr[lo];
+                    if (CH == TCH) La[u] = c.La[lo];
+                }
+            }
         }
-        const size_t ro = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb);
-        if (c.R32) c.R32[ro] = make_float2((float)x.x, (float)x.y);
-        else c.R[ro] = x;
-        cr += x.x; ci += x.y;
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+            const int j = j0 + u;
+            if (j >= NV) continue;
+            double2 x = cscale(cadd(cmul(csub(P1, P0), f0[u]), cmul(P1, gm[u])), c.s1);
+            if (inb) {
+                double2 Lx = L[u];
+                if (CH == TCH) Lx = csub(cscale(Lx, 2.0), La[u]);
+                x = cadd(x, cscale(cmul(P1, Lx), c.s2));
+            }
+            const size_t ro = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb);
+            if (c.R32) c.R32[ro] = make_float2((float)x.x, (float)x.y);
+            else c.R[ro] = x;
+            cr[u] += x.x; ci[u] += x.y;
+        }
     }
-    cr = wave_sum(cr); ci = wave_sum(ci);
-    if (lane == 0) c.csum[(size_t)j + (size_t)NV * bb] = cmk(cr, ci);
+#pragma unroll
+    for (int u = 0; u < NC; ++u) {
+        const double sr = wave_sum(cr[u]), si = wave_sum(ci[u]);
+        if (lane == 0 && j0 + u < NV) c.csum[(size_t)(j0 + u) + (size_t)NV * bb] = cmk(sr, si);
+    }
 }
-__device__ __forceinline__ void k3_R_dispatch(const K3Geom& g, const K3RChan& c, int j, int bb, int lane) {
-    if (c.ch == PCH) k3_R_column<PCH>(g, c, j, bb, lane);
-    else if (c.ch == TCH) k3_R_column<TCH>(g, c, j, bb, lane);
-    else k3_R_column<ACH>(g, c, j, bb, lane);
+__device__ __forceinline__ void k3_R_dispatch(const K3Geom& g, const K3RChan& c, int j0, int bb, int lane) {
+    if (c.ch == PCH) k3_R_columns<PCH>(g, c, j0, bb, lane);
+    else if (c.ch == TCH) k3_R_columns<TCH>(g, c, j0, bb, lane);
+    else k3_R_columns<ACH>(g, c, j0, bb, lane);
 }
+// Nν3 <= 64 (one ω per lane): a wave takes K3R_COLS columns νp of one Ω through a chunk of K3_PCH
+// momenta, loading the P-independent F0r and Γ columns once
+template <int CH>
+__device__ __forceinline__ void k3_R_columns_pchunk(const K3Geom& g, const K3RChan& c, int j0, int iW, int p0, int lane) {
+    constexpr int NC = K3R_COLS;
+    const int NV = g.NV(), NW = g.NW();
+    const int m = iW - (g.n3b - 1);
+    const int k = lane;
+    const bool kin = k < NV;
+    const int w = k - g.n3f;
+    const int w_ = crossing(m, w, CH);
+    const bool inb = kin && in_f(w_, g.n3f);
+    double2 f0[NC], gm[NC];
+#pragma unroll
+    for (int u = 0; u < NC; ++u) {
+        const int j = j0 + u;
+        if (kin && j < NV) {
+            const size_t kj = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW);
+            f0[u] = c.F0r[kj];
+            gm[u] = c.Gam[kj];
+        } else {
+            f0[u] = gm[u] = czero();
+        }
+    }
+    const int pend = min(p0 + K3_PCH, g.NP);
+    for (int p = p0; p < pend; ++p) {
+        const size_t bb = (size_t)iW + (size_t)NW * p;
+        double2 P1 = czero(), P0 = czero(), L[NC], La[NC];
+        if (kin) {
+            const size_t pi = g.pidx(m, w, p);
+            P1 = c.Pi[pi];
+            P0 = c.Pi0[pi];
+        }
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+            L[u] = La[u] = czero();
+            if (inb && j0 + u < NV) {
+                const size_t lo = (size_t)(w_ + g.n3f) + (size_t)NV * ((size_t)(j0 + u) + (size_t)NV * bb);
+                L[u] = c.Lr[lo];
+                if (CH == TCH) La[u] = c.La[lo];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+            const int j = j0 + u;
+            if (j >= NV) continue;   // uniform over the wave
+            double2 x = czero();
+            if (kin) {
+                x = cscale(cadd(cmul(csub(P1, P0), f0[u]), cmul(P1, gm[u])), c.s1);
+                if (inb) {
+                    double2 Lx = L[u];
+                    if (CH == TCH) Lx = csub(cscale(Lx, 2.0), La[u]);
+                    x = cadd(x, cscale(cmul(P1, Lx), c.s2));
+                }
+                const size_t ro = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb);
+                if (c.R32) c.R32[ro] = make_float2((float)x.x, (float)x.y);
+                else c.R[ro] = x;
+            }
+            const double sr = wave_sum(x.x), si = wave_sum(x.y);
+            if (lane == 0) c.csum[(size_t)j + (size_t)NV * bb] = cmk(sr, si);
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_k3_R_pchunk(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NV = 2 * g.n3f, NW = 2 * g.n3b - 1, nq = (NV + K3R_COLS - 1) / K3R_COLS;
+    const int npc = (g.NP + K3_PCH - 1) / K3_PCH;
+    if (wid >= nq * NW * npc * nchan) return;
+    int r = wid;
+    const int q = r % nq; r /= nq;
+    const int iW = r % NW; r /= NW;
+    const int pc = r % npc, s = r / npc;
+    const K3RChan& c = s == 0 ? c0 : s == 1 ? c1 : c2;
+    if (c.ch == PCH) k3_R_columns_pchunk<PCH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane);
+    else if (c.ch == TCH) k3_R_columns_pchunk<TCH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane);
+    else k3_R_columns_pchunk<ACH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane);
+}
+// one wave per K3R_COLS columns (νp, Ω, P) of a channel slot
 __global__ __launch_bounds__(256) void k_k3_R(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
-    const int NV = 2 * g.n3f, nb = (2 * g.n3b - 1) * g.NP;
-    if (wid >= NV * nb * nchan) return;
-    const int j = wid % NV, r = wid / NV, bb = r % nb, s = r / nb;
-    if (s == 0) k3_R_dispatch(g, c0, j, bb, lane);
-    else if (s == 1) k3_R_dispatch(g, c1, j, bb, lane);
-    else k3_R_dispatch(g, c2, j, bb, lane);
+    const int NV = 2 * g.n3f, nb = (2 * g.n3b - 1) * g.NP, nq = (NV + K3R_COLS - 1) / K3R_COLS;
+    if (wid >= nq * nb * nchan) return;
+    const int q = wid % nq, r = wid / nq, bb = r % nb, s = r / nb;
+    if (s == 0) k3_R_dispatch(g, c0, q * K3R_COLS, bb, lane);
+    else if (s == 1) k3_R_dispatch(g, c1, q * K3R_COLS, bb, lane);
+    else k3_R_dispatch(g, c2, q * K3R_COLS, bb, lane);
 }
 
 // t channel post-processing after a multi-channel launch: γt = (γt_d + γa) / 2
