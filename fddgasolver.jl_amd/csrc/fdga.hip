@@ -93,6 +93,8 @@ struct fdga_ctx {
     int finmap_n = 0, fin_nchunk = 0;
     double2* k3raw = nullptr;                                       // raw t-channel K3 under groups [n3, NP]
     double2* k2raw = nullptr;                                       // lattice BSE_K2! raw values under groups [2 n2 NP]
+    int* k3_pl = nullptr;     // lattice BSE_K3! groups: momenta whose batches hold representatives
+    int k3_npl = 0;
     // hipGraph replay of fdga_iterate(strategy, 1): one graph per strategy and per F/Fbuff pointer
     // state (set!(S.F, S.Fbuff) swaps the two buffers every iteration)
     int graph_on = 0;
@@ -874,18 +876,22 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) 
     return ks;
 }
 // the right operands R and their column sums of a BSE_K3! launch over `n` channels
-static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n) {
+// With `sym` (BSE_K3! symmetry groups installed) the lattice solver builds the operands only at the
+// momenta whose batches hold class representatives: the contraction reads no other.
+static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n, bool sym = false) {
+    const int* pl = (sym && c->k3_pl) ? c->k3_pl : nullptr;
+    const int npl = pl ? c->k3_npl : c->NP;
     if (c->NV3 <= WAVE && c->NP > 1) {   // lattice: P-independent operands loaded once per momentum chunk
-        const size_t nwave = (size_t)cdiv(c->NV3, K3R_COLS) * c->NW3 * cdiv(c->NP, K3_PCH) * n;
-        k_k3_R_pchunk<<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
+        const size_t nwave = (size_t)cdiv(c->NV3, K3R_COLS) * c->NW3 * cdiv(npl, K3_PCH) * n;
+        k_k3_R_pchunk<<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n, pl, npl);
     } else {
         const size_t nwave = (size_t)cdiv(c->NV3, K3R_COLS) * c->NW3 * c->NP * n;
         k_k3_R<<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
     }
     LAUNCHCHK(c);
     if (ks.op32()) {
-        k_k3_A32<<<dim3((unsigned)(c->NW3 * cdiv(c->NP, K3_PCH) * n), cdiv((size_t)c->NV3 * c->NV3, 256)), 256, 0, c->run>>>(
-            ks.g, ks.kc[0], ks.kc[1], ks.kc[2]);
+        k_k3_A32<<<dim3((unsigned)(c->NW3 * cdiv(npl, K3_PCH) * n), cdiv((size_t)c->NV3 * c->NV3, 256)), 256, 0, c->run>>>(
+            ks.g, ks.kc[0], ks.kc[1], ks.kc[2], pl, npl);
         LAUNCHCHK(c);
     }
     return 0;
@@ -911,7 +917,7 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n) {
     }
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const K3Set ks = k3_set(c, chans, n, sym, t_epi);
-    if (k3_rhs(c, ks, n)) return -3;
+    if (k3_rhs(c, ks, n, sym)) return -3;
     kernel_begin(c);
     if (k3_gemm_set(c, chans, n, ks) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
@@ -1074,6 +1080,7 @@ int fdga_destroy(fdga_ctx* c) {
     free_blists(c);
     if (c->k3raw) (void)hipFree(c->k3raw);
     if (c->k2raw) (void)hipFree(c->k2raw);
+    if (c->k3_pl) (void)hipFree(c->k3_pl);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->st2) (void)hipStreamSynchronize(c->st2);
     if (c->evfork) (void)hipEventDestroy(c->evfork);
@@ -1325,6 +1332,21 @@ int fdga_init_sym_grp(fdga_ctx* c) {
                 if (hit[b]) g.tiles[t].push_back((int)b);
         }
     }
+    // lattice: the momenta of the BSE_K3! representatives (the operand builders skip the others)
+    if (c->k3_pl) (void)hipFree(c->k3_pl);
+    c->k3_pl = nullptr;
+    c->k3_npl = 0;
+    if (nl) {
+        std::vector<char> used((size_t)NP, 0);
+        for (int id : {SG_PP3, SG_PH3})
+            for (int b : c->sg[id].tiles[0]) used[(size_t)(b / (cdiv(NV, tile_bm(0)) * cdiv(NV, tile_bn(0))) / NW)] = 1;
+        std::vector<int> pl;
+        for (int64_t p = 0; p < NP; ++p)
+            if (used[(size_t)p]) pl.push_back((int)p);
+        HIPCHK(c, hipMalloc((void**)&c->k3_pl, std::max<size_t>(1, pl.size()) * sizeof(int)));
+        HIPCHK(c, hipMemcpy(c->k3_pl, pl.data(), pl.size() * sizeof(int), hipMemcpyHostToDevice));
+        c->k3_npl = (int)pl.size();
+    }
     free_blists(c);
     if (!c->k3raw && dalloc(c, &c->k3raw, c->n3 * c->NP)) return -2;
     // lattice BSE_K2!: raw class values before the FL.K2 term, p in the first half, a and t in the second
@@ -1391,6 +1413,9 @@ int fdga_reset_sym_grp(fdga_ctx* c) {
     // SGΣ, SGpp, SGph, SGppL, SGphL become identity groups; SG0pp2 / SG0ph2 are left as they are
     for (int id : {SG_S, SG_K1, SG_PP2, SG_PH2, SG_PP3, SG_PH3, SG_PPL3, SG_PHL3}) free_sym(c->sg[id]);
     free_blists(c);
+    if (c->k3_pl) (void)hipFree(c->k3_pl);
+    c->k3_pl = nullptr;
+    c->k3_npl = 0;
     c->sde0_valid = 0;
     return 0;
 }

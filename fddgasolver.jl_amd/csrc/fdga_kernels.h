@@ -1042,10 +1042,13 @@ struct K3_Bop32 {
 // slot, Ω, chunk of K3_PCH momenta), y over chunks of 256 of the Nν3 x Nν3 elements; a thread
 // loads its P-independent Fbase element once and takes it through the chunk's momenta, four
 // momenta's loads in flight together.
+// With a momentum list `pl` (npl entries) only the listed momenta are built (the batches that hold
+// symmetry-class representatives); otherwise all NP.
 constexpr int K3_PCH = 8;
-__global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, K3Chan c2) {
+__global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, K3Chan c2, const int* __restrict__ pl,
+                                                int npl) {
     const int NV = g.NV(), NW = g.NW();
-    const int nn = NV * NV, npc = (g.NP + K3_PCH - 1) / K3_PCH;
+    const int nn = NV * NV, npc = (npl + K3_PCH - 1) / K3_PCH;
     const int o = blockIdx.y * 256 + threadIdx.x;
     if (o >= nn) return;
     const int s = (int)blockIdx.x / (NW * npc), rest = (int)blockIdx.x - s * NW * npc;
@@ -1053,13 +1056,14 @@ __global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, 
     const K3Chan& c = s == 0 ? c0 : s == 1 ? c1 : c2;
     const bool t = c.ch == TCH;
     const double2 x = c.Fbase[(size_t)o + (size_t)nn * iW];
-    const int pend = min(p0 + K3_PCH, g.NP);
+    const int pend = min(p0 + K3_PCH, npl);
     for (int pq = p0; pq < pend; pq += 4) {
         double2 k3[4], ka[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int p = pq + u;
-            if (p < pend) {
+            const int i = pq + u;
+            if (i < pend) {
+                const int p = pl ? pl[i] : i;
                 const size_t r = (size_t)o + (size_t)nn * (iW + (size_t)NW * p);
                 k3[u] = c.K3r[r];
                 if (t) ka[u] = c.K3a[r];
@@ -1067,8 +1071,9 @@ __global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, 
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int p = pq + u;
-            if (p < pend) {
+            const int i = pq + u;
+            if (i < pend) {
+                const int p = pl ? pl[i] : i;
                 const double2 v = t ? cadd(x, csub(cscale(k3[u], 2.0), ka[u])) : cadd(x, k3[u]);
                 c.A32[(size_t)o + (size_t)nn * (iW + (size_t)NW * p)] = make_float2((float)v.x, (float)v.y);
             }
@@ -1198,7 +1203,8 @@ __device__ __forceinline__ void k3_R_dispatch(const K3Geom& g, const K3RChan& c,
 // Nν3 <= 64 (one ω per lane): a wave takes K3R_COLS columns νp of one Ω through a chunk of K3_PCH
 // momenta, loading the P-independent F0r and Γ columns once
 template <int CH>
-__device__ __forceinline__ void k3_R_columns_pchunk(const K3Geom& g, const K3RChan& c, int j0, int iW, int p0, int lane) {
+__device__ __forceinline__ void k3_R_columns_pchunk(const K3Geom& g, const K3RChan& c, int j0, int iW, int p0, int lane,
+                                                    const int* pl, int npl) {
     constexpr int NC = K3R_COLS;
     const int NV = g.NV(), NW = g.NW();
     const int m = iW - (g.n3b - 1);
@@ -1219,8 +1225,9 @@ __device__ __forceinline__ void k3_R_columns_pchunk(const K3Geom& g, const K3RCh
             f0[u] = gm[u] = czero();
         }
     }
-    const int pend = min(p0 + K3_PCH, g.NP);
-    for (int p = p0; p < pend; ++p) {
+    const int pend = min(p0 + K3_PCH, npl);
+    for (int ip = p0; ip < pend; ++ip) {
+        const int p = pl ? pl[ip] : ip;
         const size_t bb = (size_t)iW + (size_t)NW * p;
         double2 P1 = czero(), P0 = czero(), L[NC], La[NC];
         if (kin) {
@@ -1258,19 +1265,21 @@ __device__ __forceinline__ void k3_R_columns_pchunk(const K3Geom& g, const K3RCh
         }
     }
 }
-__global__ __launch_bounds__(256) void k_k3_R_pchunk(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan) {
+// (momenta as in k_k3_A32: the list pl, or all NP when pl is null)
+__global__ __launch_bounds__(256) void k_k3_R_pchunk(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan,
+                                                     const int* __restrict__ pl, int npl) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NV = 2 * g.n3f, NW = 2 * g.n3b - 1, nq = (NV + K3R_COLS - 1) / K3R_COLS;
-    const int npc = (g.NP + K3_PCH - 1) / K3_PCH;
+    const int npc = (npl + K3_PCH - 1) / K3_PCH;
     if (wid >= nq * NW * npc * nchan) return;
     int r = wid;
     const int q = r % nq; r /= nq;
     const int iW = r % NW; r /= NW;
     const int pc = r % npc, s = r / npc;
     const K3RChan& c = s == 0 ? c0 : s == 1 ? c1 : c2;
-    if (c.ch == PCH) k3_R_columns_pchunk<PCH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane);
-    else if (c.ch == TCH) k3_R_columns_pchunk<TCH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane);
-    else k3_R_columns_pchunk<ACH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane);
+    if (c.ch == PCH) k3_R_columns_pchunk<PCH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane, pl, npl);
+    else if (c.ch == TCH) k3_R_columns_pchunk<TCH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane, pl, npl);
+    else k3_R_columns_pchunk<ACH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane, pl, npl);
 }
 // one wave per K3R_COLS columns (νp, Ω, P) of a channel slot
 __global__ __launch_bounds__(256) void k_k3_R(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan) {
