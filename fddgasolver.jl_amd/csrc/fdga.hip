@@ -486,6 +486,11 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
 }
 
 int sde_hartree(fdga_ctx* c, double2* S, const double2* G, double sign) {
+    if (c->nGel <= 16384) {   // one block: the sum and the update without two more launches
+        k_hartree_small<<<1, 256, 0, c->run>>>(S, G, c->nGel, c->scal, c->d.T, (double)c->NkG, c->U, sign);
+        LAUNCHCHK(c);
+        return 0;
+    }
     k_occupation_part<<<OCC_BLOCKS, 256, 0, c->run>>>(c->scal + 8, G, c->nGel);
     k_occupation<<<1, WAVE, 0, c->run>>>(c->scal, c->scal + 8, c->d.T, (double)c->NkG);
     k_hartree<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(S, c->nGel, c->scal, c->U, sign);
@@ -1150,7 +1155,8 @@ int64_t fdga_size(const fdga_ctx* c, int32_t which) {
 }
 
 static void invalidate(fdga_ctx* c, int which) {
-    if (which == FDGA_G0 || which == FDGA_PI0PP || which == FDGA_PI0PH || (which >= FDGA_TAB_FPP && which <= FDGA_TAB_FTX) ||
+    if (which == FDGA_G0 || which == FDGA_SIGMA0 || which == FDGA_PI0PP || which == FDGA_PI0PH ||
+        (which >= FDGA_TAB_FPP && which <= FDGA_TAB_FTX) ||
         (which >= FDGA_VERTEX(3, 0, 1) && which <= FDGA_VERTEX(3, 2, 3))) {
         c->sde0_valid = 0;
         graphs_invalidate(c);   // a captured fdPA iteration reuses the cached SDE of the reference
@@ -1509,21 +1515,23 @@ int fdga_update_F(fdga_ctx* c) {
 }
 
 int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
-    // SDE(F0, G0, Π0) of fdPA does not change between iterations: computed once (src/SDE.jl:19-20),
-    // before the target levels so that Lpp/Lph end up holding the target's (S.Lpp, S.Lph)
+    // fdPA (src/SDE.jl:7-31): Σ = SDE(F, G) - SDE(F0, G0, Π0) + Σ0 [- (n0 - 1/2) U i].  Everything
+    // but SDE(F, G) is the same every iteration: formed once into sde0, before the target levels so
+    // that Lpp/Lph end up holding the target's (S.Lpp, S.Lph)
     if (strategy == FDGA_FDPA && (!c->sde0_valid || c->sde0_h != include_hartree)) {
         k_zero<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->sde0, c->nGel);
         if (sde_levels(c, c->sde0, c->G0, c->Pi0pp, c->Pi0ph, 1, include_hartree)) return -3;
+        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->sde0, c->Sig0, 1.0, -1.0, c->nGel);   // Σ0 - SDE(F0, ..)
+        LAUNCHCHK(c);
+        if (include_hartree && c->d.fdpa_h0 && sde_hartree(c, c->sde0, c->G0, -1.0)) return -3;
         c->sde0_valid = 1;
         c->sde0_h = include_hartree;
     }
     k_zero<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->nGel);
     if (sde_levels(c, c->Sig, c->G, c->Pipp, c->Piph, 0, include_hartree)) return -3;
     if (strategy == FDGA_FDPA) {
-        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->sde0, -1.0, 1.0, c->nGel);
-        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->Sig0, 1.0, 1.0, c->nGel);
+        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->sde0, 1.0, 1.0, c->nGel);
         LAUNCHCHK(c);
-        if (include_hartree && c->d.fdpa_h0 && sde_hartree(c, c->Sig, c->G0, -1.0)) return -3;
     }
     return 0;
 }

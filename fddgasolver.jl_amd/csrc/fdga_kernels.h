@@ -2183,6 +2183,23 @@ __global__ void k_hartree(double2* __restrict__ S, size_t n, const double* __res
     S[i] = cadd(S[i], cmul(cmk(0.0, f), U));
 }
 
+// The three steps above in one block, for small G arrays: occupation n (written to occ[0]), then
+// Σ += sign (n - 1/2) U i
+__global__ __launch_bounds__(256) void k_hartree_small(double2* __restrict__ S, const double2* __restrict__ G, size_t n,
+                                                      double* __restrict__ occ, double T, double nk, double2 U,
+                                                      double sign) {
+    __shared__ double red[256 / WAVE];
+    double s = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += 256) s += G[i].y;
+    s = wave_sum(s);
+    if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    const double nocc = 0.5 + (red[0] + red[1] + red[2] + red[3]) * T / nk;
+    if (threadIdx.x == 0) occ[0] = nocc;
+    const double2 h = cmul(cmk(0.0, (nocc - 0.5) * sign), U);
+    for (size_t i = threadIdx.x; i < n; i += 256) S[i] = cadd(S[i], h);
+}
+
 // zero fill (used instead of hipMemsetAsync so that a captured iteration holds kernel nodes only)
 __global__ void k_zero(double2* __restrict__ y, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
