@@ -520,18 +520,14 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
         return sym_fill(c, c->Lph, sph.ncls ? c->sg[idh].repop : nullptr, nout);
     }
     if (!spp.ncls && !sph.ncls) {
-        k_sde_L<3><<<cdiv(nout * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
-                                                              lv.kind,
-                                                              d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP, lv.chanNP, TU, NK, spp);
+        k_sde_L<<<cdiv(nout * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
+                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
+                                                           lv.chanNP, TU, NK, spp);
     } else {
-        k_sde_L<1><<<cdiv(nclass(spp, nout) * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
-                                                                           lv.g[2], ref,
-                                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
-                                                                           lv.chanNP, TU, NK, spp);
-        k_sde_L<2><<<cdiv(nclass(sph, nout) * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1],
-                                                                           lv.g[2], ref,
-                                                                           lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
-                                                                           lv.chanNP, TU, NK, sph);
+        const size_t nw = nclass(spp, nout) + nclass(sph, nout);
+        k_sde_L_pair<<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], ref,
+                                                               lv.kind, d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
+                                                               lv.chanNP, TU, NK, spp, sph);
     }
     LAUNCHCHK(c);
     return 0;
@@ -1527,13 +1523,11 @@ int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
         c->sde0_valid = 1;
         c->sde0_h = include_hartree;
     }
-    k_zero<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->nGel);
-    if (sde_levels(c, c->Sig, c->G, c->Pipp, c->Piph, 0, include_hartree)) return -3;
-    if (strategy == FDGA_FDPA) {
-        k_axpby<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->sde0, 1.0, 1.0, c->nGel);
-        LAUNCHCHK(c);
-    }
-    return 0;
+    // the target levels accumulate onto the constant part (fdPA) or onto zero (scPA)
+    if (strategy == FDGA_FDPA) k_copy<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->sde0, c->nGel);
+    else k_zero<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->Sig, c->nGel);
+    LAUNCHCHK(c);
+    return sde_levels(c, c->Sig, c->G, c->Pipp, c->Piph, 0, include_hartree) ? -3 : 0;
 }
 
 static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma);

@@ -1902,14 +1902,15 @@ __device__ __forceinline__ void sde_terms(const ChanV& gp, const ChanV& gt, cons
     }
 }
 // WHICH: bit 0 computes Lpp, bit 1 Lph.  Under a symmetry group the two outputs have their own
-// classes (SGpp2, SGph2), so each is launched separately over its classes.
+// classes (SGpp2, SGph2): one wave per class of either (k_sde_L_pair).
 template <int WHICH>
-__global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double2* __restrict__ Lph, const double2* __restrict__ Pipp,
-                        const double2* __restrict__ Piph, const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
-                        ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
-                        int n2b, int n2f, int nWP, int nVP, int wb, int NP, int chanNP, double2 TU, int NK, SymCls sym) {
+__device__ __forceinline__ void sde_L_wave(int wid, int lane, double2* __restrict__ Lpp, double2* __restrict__ Lph,
+                                           const double2* __restrict__ Pipp, const double2* __restrict__ Piph,
+                                           const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
+                                           const ChanV& gp, const ChanV& gt, const ChanV& ga, const RefV& ref, int kind,
+                                           int n2b, int n2f, int nWP, int nVP, int wb, int NP, int chanNP, double2 TU,
+                                           int NK, const SymCls& sym) {
     constexpr bool DP = WHICH & 1, DH = WHICH & 2;
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= sym.count(NW2 * NV2 * NP)) return;
     const int o0 = sym.rep(wid);
@@ -1949,6 +1950,30 @@ __global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double
         if (DP) Lpp[o] = sym_apply(code & 3, rawp);
         if (DH) Lph[o] = sym_apply(code & 3, rawh);
     }
+}
+// both outputs per element (no symmetry groups)
+__global__ __launch_bounds__(256) void k_sde_L(double2* __restrict__ Lpp, double2* __restrict__ Lph, const double2* __restrict__ Pipp,
+                        const double2* __restrict__ Piph, const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
+                        ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
+                        int n2b, int n2f, int nWP, int nVP, int wb, int NP, int chanNP, double2 TU, int NK, SymCls sym) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    sde_L_wave<3>(wid, lane, Lpp, Lph, Pipp, Piph, Tpp, Tph, gp, gt, ga, ref, kind, n2b, n2f, nWP, nVP, wb, NP, chanNP, TU,
+                  NK, sym);
+}
+// the classes of SGpp2 (waves [0, ncls_pp)) and of SGph2 (the waves after them) in one launch
+__global__ __launch_bounds__(256) void k_sde_L_pair(double2* __restrict__ Lpp, double2* __restrict__ Lph,
+                        const double2* __restrict__ Pipp, const double2* __restrict__ Piph, const double2* __restrict__ Tpp,
+                        const double2* __restrict__ Tph, ChanV gp, ChanV gt, ChanV ga, RefV ref, int kind,
+                        int n2b, int n2f, int nWP, int nVP, int wb, int NP, int chanNP, double2 TU, int NK, SymCls spp,
+                        SymCls sph) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int npp = spp.count((2 * n2b - 1) * (2 * n2f) * NP);
+    if (wid < npp)
+        sde_L_wave<1>(wid, lane, Lpp, Lph, Pipp, Piph, Tpp, Tph, gp, gt, ga, ref, kind, n2b, n2f, nWP, nVP, wb, NP, chanNP,
+                      TU, NK, spp);
+    else
+        sde_L_wave<2>(wid - npp, lane, Lpp, Lph, Pipp, Piph, Tpp, Tph, gp, gt, ga, ref, kind, n2b, n2f, nWP, nVP, wb, NP,
+                      chanNP, TU, NK, sph);
 }
 
 // SDE L-vertices of the s-wave lattice solver at a Vertex level (kind 0), one wave per (Ω, P).
@@ -2204,6 +2229,12 @@ __global__ __launch_bounds__(256) void k_hartree_small(double2* __restrict__ S, 
 __global__ void k_zero(double2* __restrict__ y, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < n) y[i] = czero();
+}
+
+// y = x
+__global__ void k_copy(double2* __restrict__ y, const double2* __restrict__ x, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i];
 }
 
 // y = a x + b y (real scalars a, b)
