@@ -349,7 +349,10 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
         dim3 g(cdiv(c->NVP, 256), c->NWP);
         k_bubble_local<<<g, 256, 0, c->run>>>(pp, ph, Gsrc, d.nG, d.nK1, c->nVP, d.T);
         LAUNCHCHK(c);
-        return tail_sums(c, pp, pi_tail(c, pp)) || tail_sums(c, ph, pi_tail(c, ph));
+        const size_t nrows = (size_t)c->NWP * c->NP;
+        k_tail_sums2<<<(unsigned)(2 * nrows), 256, 0, c->run>>>(pp, pi_tail(c, pp), ph, pi_tail(c, ph), c->nVP, c->wb, nrows);
+        LAUNCHCHK(c);
+        return 0;
     }
     // G(R) = fft(G)/LG^2 (src/nonlocal/bubble.jl:110)
     HIPCHK(c, hipMemcpyAsync(c->tmpG1, Gsrc, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
@@ -610,15 +613,27 @@ static int launch_lk2(fdga_ctx* c) {
     return 0;
 }
 
+// the arrays of channel ch's weight: W on the band and its far sums from Π and Π0 and their tails
+static WArgs w_args(fdga_ctx* c, int ch) {
+    double2* P = pi_of(c, ch, false);
+    double2* P0 = pi_of(c, ch, true);
+    return WArgs{c->Wc[ch], c->Wtail[ch], P, P0, pi_tail(c, P), pi_tail(c, P0)};
+}
 // K1/K2 weights of channel CH on the band into Wc[CH], their far sums into Wtail[CH]
 template <int CH>
 static int launch_w(fdga_ctx* c) {
     const size_t nrows = (size_t)c->NWP * c->NP;
-    double2* P = pi_of(c, CH, false);
-    double2* P0 = pi_of(c, CH, true);
-    k_bse_w<CH><<<cdiv(nrows * (2 * c->wb + 2), 256), 256, 0, c->run>>>(c->Wc[CH], c->Wtail[CH], P, P0, pi_tail(c, P),
-                                                                       pi_tail(c, P0), vtx_FL(c), f0_view(c), c->d.nK1,
+    k_bse_w<CH><<<cdiv(nrows * (2 * c->wb + 2), 256), 256, 0, c->run>>>(w_args(c, CH), vtx_FL(c), f0_view(c), c->d.nK1,
                                                                        c->nVP, c->wb, box_bound(c), nrows);
+    LAUNCHCHK(c);
+    return 0;
+}
+// the weights of the three channels in one launch
+static int launch_w_all(fdga_ctx* c) {
+    const size_t nrows = (size_t)c->NWP * c->NP;
+    const unsigned nb = cdiv(nrows * (2 * c->wb + 2), 256);
+    k_bse_w3<<<3 * nb, 256, 0, c->run>>>(w_args(c, PCH), w_args(c, TCH), w_args(c, ACH), vtx_FL(c), f0_view(c), c->d.nK1,
+                                         c->nVP, c->wb, box_bound(c), nrows, nb);
     LAUNCHCHK(c);
     return 0;
 }
@@ -1546,7 +1561,7 @@ static int iterate_two_chains(fdga_ctx* c, int32_t strategy, int32_t update_sigm
     c->run = c->st2;
     if (strategy == FDGA_FDPA)
         for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_l_k2(c, ch);
-    rc = rc ? rc : (launch_w<PCH>(c) || launch_w<ACH>(c) || launch_w<TCH>(c));
+    rc = rc ? rc : launch_w_all(c);
     c->w_fresh = 1;
     for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
     rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));
@@ -1625,7 +1640,7 @@ static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     }
     stage_begin(c);
     // the K1/K2 weights depend on Π, Π0, FL and F0 only: computed once for both stages
-    if (launch_w<PCH>(c) || launch_w<ACH>(c) || launch_w<TCH>(c)) return -3;
+    if (launch_w_all(c)) return -3;
     c->w_fresh = 1;
     int rc = 0;
     for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
@@ -2143,7 +2158,7 @@ int group_iterate_eager(fdga_group* g, int strategy, int update_sigma) {
             if (k3_finish(c, c->FL, all, 3, true, false, g->nonlocal)) return member_err(g, c);
     }
     for (fdga_ctx* c : g->m) {
-        if (launch_w<PCH>(c) || launch_w<ACH>(c) || launch_w<TCH>(c)) return member_err(g, c);
+        if (launch_w_all(c)) return member_err(g, c);
         c->w_fresh = 1;
         int rc = 0;
         for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);

@@ -1325,13 +1325,20 @@ struct Dense_C {
 // channel's K1(Ω) remain), so Σ_far W = F0_far Σ_far (Π - Π0) + FL_far Σ_far Π.  That holds for the
 // tails the gathers use (lo <= nVP - NK - |Ω| - 2); entries nearer ω = 0 are not used.
 // One thread per band entry and per tail entry of a row (2 wb + 2 per row, rows (Ω, P) in order).
+struct WArgs {   // the arrays of one channel's weight
+    double2* W; double2* Wt;
+    const double2* Pi; const double2* Pi0; const double2* Pt; const double2* P0t;
+};
 template <int CH>
-__global__ __launch_bounds__(256) void k_bse_w(double2* __restrict__ W, double2* __restrict__ Wt,
-                                               const double2* __restrict__ Pi, const double2* __restrict__ Pi0,
-                                               const double2* __restrict__ Pt, const double2* __restrict__ P0t,
-                                               VtxV FL, F0V f0, int nWP, int nVP, int wb, int NK, size_t nrows) {
+__device__ __forceinline__ void bse_w_elem(size_t e, const WArgs& a, const VtxV& FL, const F0V& f0, int nWP, int nVP,
+                                           int wb, int NK, size_t nrows) {
     constexpr int ch = CH, sp = ChSpin<CH>::SP;
-    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    double2* __restrict__ W = a.W;
+    double2* __restrict__ Wt = a.Wt;
+    const double2* __restrict__ Pi = a.Pi;
+    const double2* __restrict__ Pi0 = a.Pi0;
+    const double2* __restrict__ Pt = a.Pt;
+    const double2* __restrict__ P0t = a.P0t;
     const int per = 2 * wb + 2;
     const size_t row = e / per;
     if (row >= nrows) return;
@@ -1357,6 +1364,19 @@ __global__ __launch_bounds__(256) void k_bse_w(double2* __restrict__ W, double2*
         Wt[o] = cadd(cmul(csub(pt, P0t[o]), f0r), cmul(pt, flr));
     }
 }
+template <int CH>
+__global__ __launch_bounds__(256) void k_bse_w(WArgs a, VtxV FL, F0V f0, int nWP, int nVP, int wb, int NK, size_t nrows) {
+    bse_w_elem<CH>(blockIdx.x * (size_t)blockDim.x + threadIdx.x, a, FL, f0, nWP, nVP, wb, NK, nrows);
+}
+// the three channels' weights in one launch (nb blocks each, in the order p, t, a)
+__global__ __launch_bounds__(256) void k_bse_w3(WArgs ap, WArgs at, WArgs aa, VtxV FL, F0V f0, int nWP, int nVP, int wb,
+                                                int NK, size_t nrows, unsigned nb) {
+    const unsigned ch = blockIdx.x / nb;
+    const size_t e = (blockIdx.x - ch * nb) * (size_t)blockDim.x + threadIdx.x;
+    if (ch == 0) bse_w_elem<PCH>(e, ap, FL, f0, nWP, nVP, wb, NK, nrows);
+    else if (ch == 1) bse_w_elem<TCH>(e, at, FL, f0, nWP, nVP, wb, NK, nrows);
+    else bse_w_elem<ACH>(e, aa, FL, f0, nWP, nVP, wb, NK, nrows);
+}
 
 // Tail sums of rows on the bubble ν-mesh (2 nVP entries, ω index iw = ω + nVP), kept on the band
 // of half-width wb around ω = 0 that the gathers can ask for: per row r,
@@ -1374,11 +1394,10 @@ __device__ __forceinline__ void wave_incl_scan(double& r, double& i, int lane) {
 // (coalesced) loop and a block reduction; the band is scanned in chunks of 256 entries (wave
 // scans, then the preceding waves' totals), the left side from its outer end inwards and the
 // right side from ω = wb - 1 down to 0.
-__global__ __launch_bounds__(256) void k_tail_sums(const double2* __restrict__ X, double2* __restrict__ T, int nVP, int wb,
-                                                   size_t nrows) {
+__device__ __forceinline__ void tail_sums_row(const double2* __restrict__ X, double2* __restrict__ T, int nVP, int wb,
+                                              size_t nrows, size_t row) {
     constexpr int NWV = 256 / WAVE;
     __shared__ double sh[4][NWV];
-    const size_t row = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     if (row >= nrows) return;
     const double2* x = X + row * (size_t)(2 * nVP);
@@ -1427,6 +1446,17 @@ __global__ __launch_bounds__(256) void k_tail_sums(const double2* __restrict__ X
 #pragma unroll
         for (int k = 0; k < NWV; ++k) { cl_r += sh[0][k]; cl_i += sh[1][k]; cr_r += sh[2][k]; cr_i += sh[3][k]; }
     }
+}
+__global__ __launch_bounds__(256) void k_tail_sums(const double2* __restrict__ X, double2* __restrict__ T, int nVP, int wb,
+                                                   size_t nrows) {
+    tail_sums_row(X, T, nVP, wb, nrows, blockIdx.x);
+}
+// two arrays in one launch (blocks [0, nrows) the first)
+__global__ __launch_bounds__(256) void k_tail_sums2(const double2* __restrict__ X0, double2* __restrict__ T0,
+                                                    const double2* __restrict__ X1, double2* __restrict__ T1, int nVP, int wb,
+                                                    size_t nrows) {
+    if (blockIdx.x < nrows) tail_sums_row(X0, T0, nVP, wb, nrows, blockIdx.x);
+    else tail_sums_row(X1, T1, nVP, wb, nrows, blockIdx.x - nrows);
 }
 
 // k_tail_sums for an array whose chunk sums C[P + NP (chunk + NC iΩ)] over NB consecutive ω
