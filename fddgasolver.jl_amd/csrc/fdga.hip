@@ -613,6 +613,18 @@ static int launch_lk2(fdga_ctx* c) {
     return 0;
 }
 
+// BSE_L_K2! of the p and a channels: one launch for the local solver
+static int launch_lk2_pa(fdga_ctx* c) {
+    if (c->d.nonlocal) return launch_lk2<PCH>(c) || launch_lk2<ACH>(c);
+    const SymCls sp = sym_k2(c, PCH), sa = sym_k2(c, ACH);
+    const size_t np = nclass(sp, c->n2 * c->NP), na = nclass(sa, c->n2 * c->NP);
+    k_gemv_lk2_pa<<<cdiv((np + na) * WAVE, 256), 256, 0, c->run>>>(
+        c->FL[PCH].K2, pi_of(c, PCH, true), sp, c->d.T * kSign[PCH], c->FL[ACH].K2, pi_of(c, ACH, true), sa,
+        c->d.T * kSign[ACH], vtx_F(c), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->NP, box_bound(c), (int)np);
+    LAUNCHCHK(c);
+    return 0;
+}
+
 // the arrays of channel ch's weight: W on the band and its far sums from Π and Π0 and their tails
 static WArgs w_args(fdga_ctx* c, int ch) {
     double2* P = pi_of(c, ch, false);
@@ -645,6 +657,20 @@ static int launch_k1(fdga_ctx* c) {
     const size_t nw = nclass(sym, c->n1 * c->NP);
     k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->run>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], c->Wtail[CH], vtx_F(c), c->d.nK1,
                                                            c->d.nK1, c->nVP, c->wb, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
+    LAUNCHCHK(c);
+    return 0;
+}
+
+// BSE_K1! of the p and a channels in one launch (t separately: its post-processing reads the
+// finished a channel)
+static int launch_k1_pa(fdga_ctx* c) {
+    if (!c->w_fresh && (launch_w<PCH>(c) || launch_w<ACH>(c))) return -3;
+    const SymCls sym = symcls(c, SG_K1);
+    const int nw = (int)nclass(sym, c->n1 * c->NP);
+    k_gemv_k1_pa<<<(unsigned)(2 * nw), 256, 0, c->run>>>(c->Fb[PCH].K1, c->Wc[PCH], c->Wtail[PCH], c->d.T * kSign[PCH],
+                                                          c->Fb[ACH].K1, c->Wc[ACH], c->Wtail[ACH], c->d.T * kSign[ACH],
+                                                          vtx_F(c), c->d.nK1, c->d.nK1, c->nVP, c->wb, c->NP,
+                                                          box_bound(c), sym, nw);
     LAUNCHCHK(c);
     return 0;
 }
@@ -1559,11 +1585,10 @@ static int iterate_two_chains(fdga_ctx* c, int32_t strategy, int32_t update_sigm
     int rc = 0;
     // chain B on st2
     c->run = c->st2;
-    if (strategy == FDGA_FDPA)
-        for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_l_k2(c, ch);
+    if (strategy == FDGA_FDPA) rc = launch_lk2_pa(c) || fdga_bse_l_k2(c, TCH);
     rc = rc ? rc : launch_w_all(c);
     c->w_fresh = 1;
-    for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
+    rc = rc ? rc : (launch_k1_pa(c) || fdga_bse_k1(c, TCH));
     rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));
     c->w_fresh = 0;
     c->run = c->st;
@@ -1631,7 +1656,7 @@ static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     stage_end(c, 1);
     if (strategy == FDGA_FDPA) {
         stage_begin(c);
-        for (int ch : {PCH, ACH, TCH}) if (fdga_bse_l_k2(c, ch)) return -3;
+        if (launch_lk2_pa(c) || fdga_bse_l_k2(c, TCH)) return -3;
         stage_end(c, 2);
         stage_begin(c);
         const int all[3] = {PCH, ACH, TCH};
@@ -1642,8 +1667,7 @@ static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     // the K1/K2 weights depend on Π, Π0, FL and F0 only: computed once for both stages
     if (launch_w_all(c)) return -3;
     c->w_fresh = 1;
-    int rc = 0;
-    for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
+    int rc = launch_k1_pa(c) || fdga_bse_k1(c, TCH);
     stage_end(c, 4);
     stage_begin(c);
     rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));

@@ -446,11 +446,8 @@ __global__ __launch_bounds__(256) void k_pavg(double2* __restrict__ A, const dou
 struct CachePtrs { double2* c[12]; };
 
 // loop 1 of build_K3_cache! at (Ω, ω, νp): Γpx and the F0 caches
-__global__ __launch_bounds__(64) void k_cache1(CachePtrs out, VtxV F, int n3b, int n3f) {
+__device__ __forceinline__ void cache1_elem(const CachePtrs& out, const VtxV& F, int n3b, int n3f, int i, int j, int iW) {
     const int NV = 2 * n3f;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y;
-    int iW = blockIdx.z;
     if (i >= NV) return;
     const int m = iW - (n3b - 1);
     const int a = i - n3f, b = j - n3f;
@@ -467,11 +464,9 @@ __global__ __launch_bounds__(64) void k_cache1(CachePtrs out, VtxV F, int n3b, i
 // (one momentum) the F caches are the reference's cache_Fr = F0 part + Γr + K2r(Ω,ν) + K3r(Ω,ν,ω)
 // (t: 2 (t) - (a)); otherwise (NL) they hold only the P-independent part Fbase_r and the same-channel
 // K2r + K3r(P) are added by the BSE_K3! loader and epilogue.
-__global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, int n3f, int full) {
+__device__ __forceinline__ void cache2_elem(const CachePtrs& out, const VtxV& F, int n3b, int n3f, int full, int i, int j,
+                                            int iW) {
     const int NV = 2 * n3f;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y;
-    int iW = blockIdx.z;
     if (i >= NV) return;
     const int m = iW - (n3b - 1);
     const int a = i - n3f, b = j - n3f;
@@ -495,6 +490,12 @@ __global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, i
     out.c[7][o] = fbp;
     out.c[8][o] = fba;
     out.c[9][o] = csub(cscale(fbt, 2.0), fba);
+}
+__global__ __launch_bounds__(64) void k_cache1(CachePtrs out, VtxV F, int n3b, int n3f) {
+    cache1_elem(out, F, n3b, n3f, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
+}
+__global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, int n3f, int full) {
+    cache2_elem(out, F, n3b, n3f, full, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // per-Ω transpose of the Γa and Γt caches ([ν, ω, Ω] -> [ω, ν, Ω]) through 32 x 32 LDS tiles
@@ -1540,13 +1541,14 @@ __device__ __forceinline__ void far_pair(double2 t, double2& left, double2& righ
 // BSE_K1!: K1(Ω, P) = T s Σ_ω F(Ω, ∞, ω, P, Ch, Sp) W(ω).  One 256-thread block per output
 // class (the K1 mesh has few outputs; four waves per output shorten each one's chain of gathers).
 template <int CH>
-__global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a,
-                                                 const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
-                                                 int n1, int nWP, int nVP, int wb, int NP, double scale, int NK, SymCls sym) {
+__device__ __forceinline__ void gemv_k1_block(int wid, double2* __restrict__ K1, const double2* __restrict__ K1a,
+                                              const double2* __restrict__ W, const double2* __restrict__ Wtail,
+                                              const VtxV& F, int n1, int nWP, int nVP, int wb, int NP, double scale, int NK,
+                                              const SymCls& sym) {
     constexpr int SP = ChSpin<CH>::SP;
     constexpr int NT = 256;
     __shared__ double red[NT / WAVE][2];
-    const int wid = blockIdx.x, tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const int NW1 = 2 * n1 - 1;
     if (wid >= sym.count(NW1 * NP)) return;
     const int o0 = sym.rep(wid);
@@ -1582,6 +1584,21 @@ __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const
         if (CH == TCH) v = cscale(cadd(v, K1a[o]), 0.5);
         K1[o] = v;
     }
+}
+template <int CH>
+__global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a,
+                                                 const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
+                                                 int n1, int nWP, int nVP, int wb, int NP, double scale, int NK, SymCls sym) {
+    gemv_k1_block<CH>(blockIdx.x, K1, K1a, W, Wtail, F, n1, nWP, nVP, wb, NP, scale, NK, sym);
+}
+// the p and a channels in one launch (blocks [0, nb) p, the rest a); t needs the finished a channel
+__global__ __launch_bounds__(256) void k_gemv_k1_pa(double2* __restrict__ K1p, const double2* __restrict__ Wp,
+                                                    const double2* __restrict__ Wtp, double scalep, double2* __restrict__ K1a_,
+                                                    const double2* __restrict__ Wa, const double2* __restrict__ Wta,
+                                                    double scalea, VtxV F, int n1, int nWP, int nVP, int wb, int NP, int NK,
+                                                    SymCls sym, int nb) {
+    if ((int)blockIdx.x < nb) gemv_k1_block<PCH>(blockIdx.x, K1p, nullptr, Wp, Wtp, F, n1, nWP, nVP, wb, NP, scalep, NK, sym);
+    else gemv_k1_block<ACH>(blockIdx.x - nb, K1a_, nullptr, Wa, Wta, F, n1, nWP, nVP, wb, NP, scalea, NK, sym);
 }
 
 // BSE_K2!: K2(Ω, ν, P) = T s Σ_ω [F(Ω,ν,ω) - F(Ω,∞,ω)] W(ω) + FL.K2 (t: 2 FLt - FLa).
@@ -1736,12 +1753,11 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
 //   Γ(Ω, ν, ω) Π0(ω) F0(Ω, ω~, ∞).  Γ excludes F0 and the own channel, so it vanishes outside
 //   the frequency window.  (The lattice solver's form is k_lk2nl_table + GEMM.)
 template <int CH>
-__global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, const double2* __restrict__ outa,
-                                                  const double2* __restrict__ Pi0, VtxV F, int n2b, int n2f, int nWP,
-                                                  int nVP, int NP, double scale, int NK, SymCls sym) {
+__device__ __forceinline__ void gemv_lk2_wave(int wid, int lane, double2* __restrict__ out, const double2* __restrict__ outa,
+                                              const double2* __restrict__ Pi0, const VtxV& F, int n2b, int n2f, int nWP,
+                                              int nVP, int NP, double scale, int NK, const SymCls& sym) {
     constexpr int SP = ChSpin<CH>::SP;
     constexpr int FLG = FL_ALL & ~FL_F0 & ~(FL_GP << CH);
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= sym.count(NW2 * NV2 * NP)) return;
     const int o0 = sym.rep(wid);
@@ -1767,6 +1783,22 @@ __global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, con
         if (CH == TCH) val = cscale(cadd(val, outa[o]), 0.5);
         out[o] = val;
     }
+}
+template <int CH>
+__global__ __launch_bounds__(256) void k_gemv_lk2(double2* __restrict__ out, const double2* __restrict__ outa,
+                                                  const double2* __restrict__ Pi0, VtxV F, int n2b, int n2f, int nWP,
+                                                  int nVP, int NP, double scale, int NK, SymCls sym) {
+    gemv_lk2_wave<CH>((blockIdx.x * blockDim.x + threadIdx.x) / WAVE, threadIdx.x & (WAVE - 1), out, outa, Pi0, F, n2b, n2f,
+                      nWP, nVP, NP, scale, NK, sym);
+}
+// the p and a channels in one launch (waves [0, wp) p, the rest a); t needs the finished a channel
+__global__ __launch_bounds__(256) void k_gemv_lk2_pa(double2* __restrict__ outp, const double2* __restrict__ Pi0p,
+                                                     SymCls symp, double scalep, double2* __restrict__ outa_,
+                                                     const double2* __restrict__ Pi0a, SymCls syma, double scalea, VtxV F,
+                                                     int n2b, int n2f, int nWP, int nVP, int NP, int NK, int wp) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid < wp) gemv_lk2_wave<PCH>(wid, lane, outp, nullptr, Pi0p, F, n2b, n2f, nWP, nVP, NP, scalep, NK, symp);
+    else gemv_lk2_wave<ACH>(wid - wp, lane, outa_, nullptr, Pi0a, F, n2b, n2f, nWP, nVP, NP, scalea, NK, syma);
 }
 
 // ---------------------------------------------------------------------------------------
