@@ -733,6 +733,20 @@ static int launch_k2_at(fdga_ctx* c) {
     return 0;
 }
 
+// BSE_K2! of the three channels: one launch for the local solver
+static int launch_k2_all(fdga_ctx* c) {
+    if (c->d.nonlocal) return launch_k2<PCH>(c) || launch_k2_at(c);
+    if (!c->w_fresh && (launch_w<PCH>(c) || launch_w<ACH>(c) || launch_w<TCH>(c))) return -3;
+    const SymCls sp = sym_k2(c, PCH), sat = symcls(c, SG_PH2);
+    const size_t np = nclass(sp, c->n2 * c->NP), nat = nclass(sat, c->n2 * c->NP);
+    k_gemv_k2_all<<<cdiv((np + nat) * WAVE, 256), 256, 0, c->run>>>(
+        c->Fb[PCH].K2, c->Wc[PCH], c->Wtail[PCH], chanF(c, c->FL[PCH]), sp, (int)np, c->Fb[ACH].K2, c->Fb[TCH].K2,
+        c->Wc[ACH], c->Wc[TCH], c->Wtail[ACH], c->Wtail[TCH], vtx_F(c), chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]),
+        c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->wb, c->NP, c->d.T, box_bound(c), sat);
+    LAUNCHCHK(c);
+    return 0;
+}
+
 // classes of the channel's K3 (BSE_K3!: SGpp[3] / SGph[3]; BSE_L_K3!: SGppL[3] / SGphL[3]),
 // stored with device (K3 [ν, νp, Ω]) indices
 static SymCls sym_k3(const fdga_ctx* c, int ch, bool left) {
@@ -1589,7 +1603,7 @@ static int iterate_two_chains(fdga_ctx* c, int32_t strategy, int32_t update_sigm
     rc = rc ? rc : launch_w_all(c);
     c->w_fresh = 1;
     rc = rc ? rc : (launch_k1_pa(c) || fdga_bse_k1(c, TCH));
-    rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));
+    rc = rc ? rc : launch_k2_all(c);
     c->w_fresh = 0;
     c->run = c->st;
     if (rc) return -3;
@@ -1670,7 +1684,7 @@ static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     int rc = launch_k1_pa(c) || fdga_bse_k1(c, TCH);
     stage_end(c, 4);
     stage_begin(c);
-    rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));
+    rc = rc ? rc : launch_k2_all(c);
     c->w_fresh = 0;
     if (rc) return -3;
     stage_end(c, 5);
@@ -2186,7 +2200,7 @@ int group_iterate_eager(fdga_group* g, int strategy, int update_sigma) {
         c->w_fresh = 1;
         int rc = 0;
         for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
-        rc = rc ? rc : (fdga_bse_k2(c, PCH) || launch_k2_at(c));
+        rc = rc ? rc : launch_k2_all(c);
         c->w_fresh = 0;
         if (rc) return member_err(g, c);
         if (k3_rhs(c, k3_set(c, all, 3, false, false), 3)) return member_err(g, c);

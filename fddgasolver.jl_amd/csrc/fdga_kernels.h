@@ -1637,11 +1637,10 @@ __device__ __forceinline__ double2 k2d_rank1(const VtxV& F, int m, int v, int p)
     return F.g[CH].k2(m, v, p);
 }
 template <int CH>
-__global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a,
-                                                 const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
-                                                 ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int nVP, int wb, int NP,
-                                                 double scale, int NK, SymCls sym) {
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+__device__ __forceinline__ void gemv_k2_wave(int wid, int lane, double2* __restrict__ K2, const double2* __restrict__ K2a,
+                                             const double2* __restrict__ W, const double2* __restrict__ Wtail,
+                                             const VtxV& F, const ChanV& Lr, const ChanV& La, int n2b, int n2f, int nWP,
+                                             int nVP, int wb, int NP, double scale, int NK, const SymCls& sym) {
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= sym.count(NW2 * NV2 * NP)) return;
     const int o0 = sym.rep(wid);
@@ -1683,17 +1682,24 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
         K2[o] = val;
     }
 }
+template <int CH>
+__global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a,
+                                                 const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
+                                                 ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int nVP, int wb, int NP,
+                                                 double scale, int NK, SymCls sym) {
+    gemv_k2_wave<CH>((blockIdx.x * blockDim.x + threadIdx.x) / WAVE, threadIdx.x & (WAVE - 1), K2, K2a, W, Wtail, F, Lr, La,
+                     n2b, n2f, nWP, nVP, wb, NP, scale, NK, sym);
+}
 
 // BSE_K2! of the a and t channels in one pass.  The t channel runs in the density component,
 // F(t, dSp) = 2 F(t, pSp) + F(t, xSp) = 2 F(t, pSp) - F(a, pSp) (src/vertex.jl:288-336), so the
 // a-channel evaluation is shared:  K2a = T Σ fa Wa + FL.a.K2,
 // K2t = (T (-1) Σ (2 ft - fa) Wt + 2 FL.t.K2 - FL.a.K2 + K2a) / 2   (src/BSE_templates.jl:95-110)
-__global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, double2* __restrict__ K2t,
-                                                    const double2* __restrict__ Wa, const double2* __restrict__ Wt,
-                                                    const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
-                                                    VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP,
-                                                    int wb, int NP, double T, int NK, SymCls sym) {
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+__device__ __forceinline__ void gemv_k2_at_wave(int wid, int lane, double2* __restrict__ K2a, double2* __restrict__ K2t,
+                                                const double2* __restrict__ Wa, const double2* __restrict__ Wt,
+                                                const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
+                                                const VtxV& F, const ChanV& La, const ChanV& Lt, int n2b, int n2f, int nWP,
+                                                int nVP, int wb, int NP, double T, int NK, const SymCls& sym) {
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= sym.count(NW2 * NV2 * NP)) return;
     const int o0 = sym.rep(wid);
@@ -1747,6 +1753,29 @@ __global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, d
         K2a[o] = ka;
         K2t[o] = cscale(cadd(kt, ka), 0.5);
     }
+}
+// BSE_K2! of all three channels in one launch: waves [0, wp) the p classes (SGpp[2]), then the a and
+// t classes (SGph[2], t finished from a in the same wave)
+__global__ __launch_bounds__(256) void k_gemv_k2_all(double2* __restrict__ K2p, const double2* __restrict__ Wp,
+                                                     const double2* __restrict__ Wtp, ChanV Lp, SymCls symp, int wp,
+                                                     double2* __restrict__ K2a, double2* __restrict__ K2t,
+                                                     const double2* __restrict__ Wa, const double2* __restrict__ Wt,
+                                                     const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
+                                                     VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP, int wb,
+                                                     int NP, double T, int NK, SymCls symat) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid < wp)
+        gemv_k2_wave<PCH>(wid, lane, K2p, nullptr, Wp, Wtp, F, Lp, La, n2b, n2f, nWP, nVP, wb, NP, T, NK, symp);
+    else
+        gemv_k2_at_wave(wid - wp, lane, K2a, K2t, Wa, Wt, Wat, Wtt, F, La, Lt, n2b, n2f, nWP, nVP, wb, NP, T, NK, symat);
+}
+__global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, double2* __restrict__ K2t,
+                                                    const double2* __restrict__ Wa, const double2* __restrict__ Wt,
+                                                    const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
+                                                    VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP,
+                                                    int wb, int NP, double T, int NK, SymCls sym) {
+    gemv_k2_at_wave((blockIdx.x * blockDim.x + threadIdx.x) / WAVE, threadIdx.x & (WAVE - 1), K2a, K2t, Wa, Wt, Wat, Wtt, F,
+                    La, Lt, n2b, n2f, nWP, nVP, wb, NP, T, NK, sym);
 }
 
 // BSE_L_K2! of the local solver (src/BSEa/BSEa_K2.jl:1-40): Σ over the bubble mesh of
