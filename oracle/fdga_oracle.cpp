@@ -11,7 +11,10 @@
 //  * caches that the reference stores per P but that do not depend on P are still
 //    stored per P here (literal), the HIP product stores them once.
 // Parity status of every function is listed in the header comment of tests/test_oracle_*.py
-// and in the repository design notes.
+// and in the repository design notes.  Parity unpinned: in the symmetry classes (build_group), the
+// operation of a member that the exploration reaches along paths with different operations is the
+// first path's (MatsubaraFunctions' SymmetryGroup is not vendored); partitions, representatives and
+// all path-independent operations are pinned.
 #include "fdga_oracle.h"
 
 #include <algorithm>

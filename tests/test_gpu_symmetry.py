@@ -21,7 +21,8 @@ from helpers import CASES, EXTRA_CASES, make_pair, relerr
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 CH_ORDER = (po.PCH, po.ACH, po.TCH)
-SYM_CASES = ["local_ref", "local_vertexref", "nl_L2", "nl_L3", "nl_LG4_L2", "nl_L6"]
+# nl_L16: 45 of 256 momenta hold K3 representatives (the operand builders skip the rest)
+SYM_CASES = ["local_ref", "local_vertexref", "nl_L2", "nl_L3", "nl_LG4_L2", "nl_L6", "nl_L16"]
 ALL_CASES = {**CASES, **EXTRA_CASES}
 
 

@@ -10,6 +10,13 @@ Each test states what fixes the expected value independently of the oracle's own
   * K1-only channel conversions (bosonic parts)   test/test_nonlocal_vertex.jl:148-154, s-wave form
   * first iteration from F = 0                    K1_r = s T Σ_ω U (Π - Π0) U (src/BSEa/BSEa_K1.jl:38-43)
   * NL solver with a 1x1 k-mesh and NL solver with k-independent G reduce to the local solver
+  * symmetry classes of init_sym_grp! (local and lattice)   the group orbits by union-find from the
+                                                  generators restated on value tuples; lowest-index
+                                                  representatives; every operation that all paths
+                                                  through a class agree on
+Parity unpinned: the operation of a class member reached along paths with different operations
+(the depth-first exploration order of MatsubaraFunctions' SymmetryGroup, not vendored here) is taken
+from the first path found, the same hypothesis as the representative order.
 """
 import numpy as np
 import pytest
