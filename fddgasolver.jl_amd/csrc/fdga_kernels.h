@@ -1756,7 +1756,7 @@ __device__ __forceinline__ void gemv_k2_at_wave(int wid, int lane, double2* __re
 }
 // BSE_K2! of all three channels in one launch: waves [0, wp) the p classes (SGpp[2]), then the a and
 // t classes (SGph[2], t finished from a in the same wave)
-__global__ __launch_bounds__(256) void k_gemv_k2_all(double2* __restrict__ K2p, const double2* __restrict__ Wp,
+__global__ __launch_bounds__(256, 4) void k_gemv_k2_all(double2* __restrict__ K2p, const double2* __restrict__ Wp,
                                                      const double2* __restrict__ Wtp, ChanV Lp, SymCls symp, int wp,
                                                      double2* __restrict__ K2a, double2* __restrict__ K2t,
                                                      const double2* __restrict__ Wa, const double2* __restrict__ Wt,
@@ -1769,7 +1769,7 @@ __global__ __launch_bounds__(256) void k_gemv_k2_all(double2* __restrict__ K2p, 
     else
         gemv_k2_at_wave(wid - wp, lane, K2a, K2t, Wa, Wt, Wat, Wtt, F, La, Lt, n2b, n2f, nWP, nVP, wb, NP, T, NK, symat);
 }
-__global__ __launch_bounds__(256) void k_gemv_k2_at(double2* __restrict__ K2a, double2* __restrict__ K2t,
+__global__ __launch_bounds__(256, 4) void k_gemv_k2_at(double2* __restrict__ K2a, double2* __restrict__ K2t,
                                                     const double2* __restrict__ Wa, const double2* __restrict__ Wt,
                                                     const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
                                                     VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP,
