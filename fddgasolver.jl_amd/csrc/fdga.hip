@@ -59,7 +59,7 @@ struct fdga_ctx {
     double2 *Pi0pp = nullptr, *Pi0ph = nullptr, *Pipp = nullptr, *Piph = nullptr;
     double2 *Lpp = nullptr, *Lph = nullptr;
     double2* Wc[3] = {nullptr, nullptr, nullptr};                  // BSE_K1!/BSE_K2! weights per channel
-    double2* Wtail[3] = {nullptr, nullptr, nullptr};               // their tail sums (k_tail_sums)
+    double2* Wfar[3] = {nullptr, nullptr, nullptr};                // their far factors per row (WTail)
     double2* Ptail[4] = {nullptr, nullptr, nullptr, nullptr};      // tail sums of Π0pp, Π0ph, Πpp, Πph
     size_t ntail = 0;                                               // elements of one tail-sum array
     double2* pchunk[2] = {nullptr, nullptr};   // lattice bubble: chunk sums of the last Πpp / Πph [P, chunk, Ω]
@@ -629,13 +629,17 @@ static int launch_lk2_pa(fdga_ctx* c) {
 static WArgs w_args(fdga_ctx* c, int ch) {
     double2* P = pi_of(c, ch, false);
     double2* P0 = pi_of(c, ch, true);
-    return WArgs{c->Wc[ch], c->Wtail[ch], P, P0, pi_tail(c, P), pi_tail(c, P0)};
+    return WArgs{c->Wc[ch], c->Wfar[ch], P, P0};
 }
-// K1/K2 weights of channel CH on the band into Wc[CH], their far sums into Wtail[CH]
+// the tail sums of channel ch's weight, formed from those of Π and Π0 and the far factors
+static WTail wtail(fdga_ctx* c, int ch) {
+    return WTail{pi_tail(c, pi_of(c, ch, false)), pi_tail(c, pi_of(c, ch, true)), c->Wfar[ch]};
+}
+// K1/K2 weights of channel CH on the band into Wc[CH], their far factors into Wfar[CH]
 template <int CH>
 static int launch_w(fdga_ctx* c) {
     const size_t nrows = (size_t)c->NWP * c->NP;
-    k_bse_w<CH><<<cdiv(nrows * (2 * c->wb + 2), 256), 256, 0, c->run>>>(w_args(c, CH), vtx_FL(c), f0_view(c), c->d.nK1,
+    k_bse_w<CH><<<cdiv(nrows * (2 * c->wb), 256), 256, 0, c->run>>>(w_args(c, CH), vtx_FL(c), f0_view(c), c->d.nK1,
                                                                        c->nVP, c->wb, box_bound(c), nrows);
     LAUNCHCHK(c);
     return 0;
@@ -643,7 +647,7 @@ static int launch_w(fdga_ctx* c) {
 // the weights of the three channels in one launch
 static int launch_w_all(fdga_ctx* c) {
     const size_t nrows = (size_t)c->NWP * c->NP;
-    const unsigned nb = cdiv(nrows * (2 * c->wb + 2), 256);
+    const unsigned nb = cdiv(nrows * (2 * c->wb), 256);
     k_bse_w3<<<3 * nb, 256, 0, c->run>>>(w_args(c, PCH), w_args(c, TCH), w_args(c, ACH), vtx_FL(c), f0_view(c), c->d.nK1,
                                          c->nVP, c->wb, box_bound(c), nrows, nb);
     LAUNCHCHK(c);
@@ -655,7 +659,7 @@ static int launch_k1(fdga_ctx* c) {
     if (!c->w_fresh && launch_w<CH>(c)) return -3;
     const SymCls sym = symcls(c, SG_K1);
     const size_t nw = nclass(sym, c->n1 * c->NP);
-    k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->run>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], c->Wtail[CH], vtx_F(c), c->d.nK1,
+    k_gemv_k1<CH><<<(unsigned)nw, 256, 0, c->run>>>(c->Fb[CH].K1, c->Fb[ACH].K1, c->Wc[CH], wtail(c, CH), vtx_F(c), c->d.nK1,
                                                            c->d.nK1, c->nVP, c->wb, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
     LAUNCHCHK(c);
     return 0;
@@ -667,8 +671,8 @@ static int launch_k1_pa(fdga_ctx* c) {
     if (!c->w_fresh && (launch_w<PCH>(c) || launch_w<ACH>(c))) return -3;
     const SymCls sym = symcls(c, SG_K1);
     const int nw = (int)nclass(sym, c->n1 * c->NP);
-    k_gemv_k1_pa<<<(unsigned)(2 * nw), 256, 0, c->run>>>(c->Fb[PCH].K1, c->Wc[PCH], c->Wtail[PCH], c->d.T * kSign[PCH],
-                                                          c->Fb[ACH].K1, c->Wc[ACH], c->Wtail[ACH], c->d.T * kSign[ACH],
+    k_gemv_k1_pa<<<(unsigned)(2 * nw), 256, 0, c->run>>>(c->Fb[PCH].K1, c->Wc[PCH], wtail(c, PCH), c->d.T * kSign[PCH],
+                                                          c->Fb[ACH].K1, c->Wc[ACH], wtail(c, ACH), c->d.T * kSign[ACH],
                                                           vtx_F(c), c->d.nK1, c->d.nK1, c->nVP, c->wb, c->NP,
                                                           box_bound(c), sym, nw);
     LAUNCHCHK(c);
@@ -694,7 +698,7 @@ static int launch_k2_nl(fdga_ctx* c) {
     const size_t n2 = c->n2 * c->NP;
     double2* raw = rp ? c->k2raw + (CH == PCH ? 0 : n2) : nullptr;
     k_k2nl_finish<CH><<<cdiv((size_t)NW2 * c->NP * WAVE, 256), 256, 0, c->run>>>(
-        c->Fb[CH].K2, c->Fb[ACH].K2, c->k2y, c->k2xf, c->Wc[CH], c->Wtail[CH], vtx_F(c), chanF(c, c->FL[CH]),
+        c->Fb[CH].K2, c->Fb[ACH].K2, c->k2y, c->k2xf, c->Wc[CH], wtail(c, CH), vtx_F(c), chanF(c, c->FL[CH]),
         chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->wb, c->NP, c->d.T * kSign[CH], raw);
     LAUNCHCHK(c);
     if (rp) {
@@ -711,7 +715,7 @@ static int launch_k2(fdga_ctx* c) {
     if (!c->w_fresh && launch_w<CH>(c)) return -3;
     const SymCls sym = sym_k2(c, CH);
     const size_t nw = nclass(sym, c->n2 * c->NP);
-    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->Wc[CH], c->Wtail[CH], vtx_F(c),
+    k_gemv_k2<CH><<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->Fb[CH].K2, c->Fb[ACH].K2, c->Wc[CH], wtail(c, CH), vtx_F(c),
                                                            chanF(c, c->FL[CH]),
                                                            chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP,
                                                            c->wb, c->NP, c->d.T * kSign[CH], box_bound(c), sym);
@@ -726,7 +730,7 @@ static int launch_k2_at(fdga_ctx* c) {
     const SymCls sym = symcls(c, SG_PH2);   // a and t share SGph[2]
     const size_t nw = nclass(sym, c->n2 * c->NP);
     k_gemv_k2_at<<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(c->Fb[ACH].K2, c->Fb[TCH].K2, c->Wc[ACH], c->Wc[TCH],
-                                                          c->Wtail[ACH], c->Wtail[TCH], vtx_F(c),
+                                                          wtail(c, ACH), wtail(c, TCH), vtx_F(c),
                                                           chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]), c->d.nK2b, c->d.nK2f,
                                                           c->d.nK1, c->nVP, c->wb, c->NP, c->d.T, box_bound(c), sym);
     LAUNCHCHK(c);
@@ -740,8 +744,8 @@ static int launch_k2_all(fdga_ctx* c) {
     const SymCls sp = sym_k2(c, PCH), sat = symcls(c, SG_PH2);
     const size_t np = nclass(sp, c->n2 * c->NP), nat = nclass(sat, c->n2 * c->NP);
     k_gemv_k2_all<<<cdiv((np + nat) * WAVE, 256), 256, 0, c->run>>>(
-        c->Fb[PCH].K2, c->Wc[PCH], c->Wtail[PCH], chanF(c, c->FL[PCH]), sp, (int)np, c->Fb[ACH].K2, c->Fb[TCH].K2,
-        c->Wc[ACH], c->Wc[TCH], c->Wtail[ACH], c->Wtail[TCH], vtx_F(c), chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]),
+        c->Fb[PCH].K2, c->Wc[PCH], wtail(c, PCH), chanF(c, c->FL[PCH]), sp, (int)np, c->Fb[ACH].K2, c->Fb[TCH].K2,
+        c->Wc[ACH], c->Wc[TCH], wtail(c, ACH), wtail(c, TCH), vtx_F(c), chanF(c, c->FL[ACH]), chanF(c, c->FL[TCH]),
         c->d.nK2b, c->d.nK2f, c->d.nK1, c->nVP, c->wb, c->NP, c->d.T, box_bound(c), sat);
     LAUNCHCHK(c);
     return 0;
@@ -1058,8 +1062,8 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
     for (double2** p : {&c->Pi0pp, &c->Pi0ph, &c->Pipp, &c->Piph}) rc |= dalloc(c, p, c->nPiel);
     for (double2** p : {&c->Wc[0], &c->Wc[1], &c->Wc[2]}) rc |= dalloc(c, p, (size_t)c->NWP * c->NP * (2 * (size_t)c->wb));
     c->ntail = (size_t)c->NWP * c->NP * (2 * (size_t)c->wb + 2);
-    for (double2** p : {&c->Wtail[0], &c->Wtail[1], &c->Wtail[2], &c->Ptail[0], &c->Ptail[1], &c->Ptail[2], &c->Ptail[3]})
-        rc |= dalloc(c, p, c->ntail);
+    for (double2** p : {&c->Ptail[0], &c->Ptail[1], &c->Ptail[2], &c->Ptail[3]}) rc |= dalloc(c, p, c->ntail);
+    for (double2** p : {&c->Wfar[0], &c->Wfar[1], &c->Wfar[2]}) rc |= dalloc(c, p, (size_t)4 * c->NWP * c->NP);
     for (double2** p : {&c->Lpp, &c->Lph, &c->tmpL1, &c->tmpL2}) rc |= dalloc(c, p, c->nL);
     if (d.nonlocal) {
         const size_t nch = (size_t)c->NP * cdiv(c->NVP, bubble_nb(d.L)) * c->NWP;
@@ -1116,7 +1120,7 @@ int fdga_destroy(fdga_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (double2* p : {c->Gbare, c->G0, c->Sig0, c->G, c->Sig, c->tmpG1, c->tmpG2, c->tmpG3, c->Pi0pp, c->Pi0ph,
                        c->Pipp, c->Piph, c->Wc[0], c->Wc[1], c->Wc[2], c->Lpp, c->Lph, c->tmpL1, c->tmpL2, c->sde0, c->u2C,
-                       c->stage, c->Wtail[0], c->Wtail[1], c->Wtail[2], c->Ptail[0], c->Ptail[1], c->Ptail[2], c->Ptail[3],
+                       c->stage, c->Wfar[0], c->Wfar[1], c->Wfar[2], c->Ptail[0], c->Ptail[1], c->Ptail[2], c->Ptail[3],
                        c->k2x, c->k2xf, c->k2y, c->lk2x, c->pchunk[0], c->pchunk[1]})
         if (p) (void)hipFree(p);
     for (auto* t : c->tab) if (t) (void)hipFree(t);

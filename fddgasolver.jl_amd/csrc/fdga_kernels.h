@@ -1327,26 +1327,27 @@ struct Dense_C {
 // tails the gathers use (lo <= nVP - NK - |Ω| - 2); entries nearer ω = 0 are not used.
 // One thread per band entry and per tail entry of a row (2 wb + 2 per row, rows (Ω, P) in order).
 struct WArgs {   // the arrays of one channel's weight
-    double2* W; double2* Wt;
-    const double2* Pi; const double2* Pi0; const double2* Pt; const double2* P0t;
+    double2* W;      // W on the band [2 wb, rows]
+    double2* far;    // per row: F0 and FL at the left far point, then at the right one [4, rows]
+    const double2* Pi; const double2* Pi0;
 };
+// The tail sums of W are not stored: at row r and tail entry k (the layout of k_tail_sums) they are
+// (Πt - Π0t)·F0_far + Πt·FL_far with the tail sums Πt, Π0t of Π and Π0 and the row's far factors of
+// that side (WTail::at).  One thread per band entry; the row's first also evaluates the far factors.
 template <int CH>
 __device__ __forceinline__ void bse_w_elem(size_t e, const WArgs& a, const VtxV& FL, const F0V& f0, int nWP, int nVP,
                                            int wb, int NK, size_t nrows) {
     constexpr int ch = CH, sp = ChSpin<CH>::SP;
     double2* __restrict__ W = a.W;
-    double2* __restrict__ Wt = a.Wt;
     const double2* __restrict__ Pi = a.Pi;
     const double2* __restrict__ Pi0 = a.Pi0;
-    const double2* __restrict__ Pt = a.Pt;
-    const double2* __restrict__ P0t = a.P0t;
-    const int per = 2 * wb + 2;
+    const int per = 2 * wb;
     const size_t row = e / per;
     if (row >= nrows) return;
     const int t = (int)(e - row * per);
     const int iW = (int)(row % (size_t)(2 * nWP - 1)), p = (int)(row / (size_t)(2 * nWP - 1));
     const int m = iW - (nWP - 1);
-    if (t < 2 * wb) {
+    {
         const int iw = nVP - wb + t, w = iw - nVP;
         const size_t o = (size_t)iw + (size_t)(2 * nVP) * row;
         const int wc = crossing(m, w, ch);
@@ -1355,15 +1356,13 @@ __device__ __forceinline__ void bse_w_elem(size_t e, const WArgs& a, const VtxV&
         const double2 pi = Pi[o];
         W[(size_t)t + (size_t)(2 * wb) * row] = cadd(cmul(csub(pi, Pi0[o]), f0r), cmul(pi, flr));
     }
-    {
-        const int wf = t > wb ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
-        const int wc = crossing(m, wf, ch);
-        const double2 f0r = f0.eval(m, wc, FDGA_INF, ch, sp);
-        const double2 flr = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
-        const size_t o = (size_t)t + (size_t)(2 * (wb + 1)) * row;
-        const double2 pt = Pt[o];
-        Wt[o] = cadd(cmul(csub(pt, P0t[o]), f0r), cmul(pt, flr));
-    }
+    if (t == 0)
+        for (int side = 0; side < 2; ++side) {
+            const int wf = side ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
+            const int wc = crossing(m, wf, ch);
+            a.far[4 * row + 2 * side] = f0.eval(m, wc, FDGA_INF, ch, sp);
+            a.far[4 * row + 2 * side + 1] = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
+        }
 }
 template <int CH>
 __global__ __launch_bounds__(256) void k_bse_w(WArgs a, VtxV FL, F0V f0, int nWP, int nVP, int wb, int NK, size_t nrows) {
@@ -1523,6 +1522,21 @@ __device__ __forceinline__ double2 tail_left(const double2* T, size_t row, int n
 __device__ __forceinline__ double2 tail_right(const double2* T, size_t row, int nVP, int wb, int hi) {
     return T[row * (size_t)(2 * (wb + 1)) + (wb + 1) + (hi + 1 - nVP)];
 }
+// tail sums of a BSE weight W = (Π - Π0) F0 + Π FL, formed from the tail sums of Π and Π0 and the
+// row's far factors (k_bse_w): Σ_far W = F0_far Σ_far (Π - Π0) + FL_far Σ_far Π
+struct WTail {
+    const double2* Pt; const double2* P0t; const double2* far;
+    __device__ __forceinline__ double2 at(size_t row, int k, int wb) const {   // entry k of the row's tail layout
+        const size_t o = row * (size_t)(2 * (wb + 1)) + k;
+        const int side = k > wb ? 1 : 0;
+        const double2 pt = Pt[o];
+        return cadd(cmul(csub(pt, P0t[o]), far[4 * row + 2 * side]), cmul(pt, far[4 * row + 2 * side + 1]));
+    }
+    __device__ __forceinline__ double2 left(size_t row, int nVP, int wb, int lo) const { return at(row, lo - (nVP - wb), wb); }
+    __device__ __forceinline__ double2 right(size_t row, int nVP, int wb, int hi) const {
+        return at(row, (wb + 1) + (hi + 1 - nVP), wb);
+    }
+};
 
 // Frequency window of the gather-GEMVs.  Every vertex term that depends on the summed
 // frequency ω is supported on |ω| <= Wc = NK + |m| + |n| + 2, where NK bounds every box of the
@@ -1542,7 +1556,7 @@ __device__ __forceinline__ void far_pair(double2 t, double2& left, double2& righ
 // class (the K1 mesh has few outputs; four waves per output shorten each one's chain of gathers).
 template <int CH>
 __device__ __forceinline__ void gemv_k1_block(int wid, double2* __restrict__ K1, const double2* __restrict__ K1a,
-                                              const double2* __restrict__ W, const double2* __restrict__ Wtail,
+                                              const double2* __restrict__ W, const WTail& Wtail,
                                               const VtxV& F, int n1, int nWP, int nVP, int wb, int NP, double scale, int NK,
                                               const SymCls& sym) {
     constexpr int SP = ChSpin<CH>::SP;
@@ -1573,8 +1587,8 @@ __device__ __forceinline__ void gemv_k1_block(int wid, double2* __restrict__ K1,
     far_pair(F.eval(m, FDGA_INF, lane == 0 ? -Wc - 1 : Wc + 1, p, CH, SP, FL_ALL), tl, tr);
     // every lane of wave 0 holds the same sums; lanes write the class members
     double2 acc = cmk(tsr, tsi);
-    if (lo > 0) acc = cadd(acc, cmul(tl, tail_left(Wtail, row, nVP, wb, lo)));
-    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, tail_right(Wtail, row, nVP, wb, hi)));
+    if (lo > 0) acc = cadd(acc, cmul(tl, Wtail.left(row, nVP, wb, lo)));
+    if (hi < 2 * nVP - 1) acc = cadd(acc, cmul(tr, Wtail.right(row, nVP, wb, hi)));
     const double2 raw = cscale(acc, scale);
     int b, e;
     sym.range(wid, b, e);
@@ -1587,15 +1601,14 @@ __device__ __forceinline__ void gemv_k1_block(int wid, double2* __restrict__ K1,
 }
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k1(double2* __restrict__ K1, const double2* __restrict__ K1a,
-                                                 const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
+                                                 const double2* __restrict__ W, WTail Wtail, VtxV F,
                                                  int n1, int nWP, int nVP, int wb, int NP, double scale, int NK, SymCls sym) {
     gemv_k1_block<CH>(blockIdx.x, K1, K1a, W, Wtail, F, n1, nWP, nVP, wb, NP, scale, NK, sym);
 }
 // the p and a channels in one launch (blocks [0, nb) p, the rest a); t needs the finished a channel
 __global__ __launch_bounds__(256) void k_gemv_k1_pa(double2* __restrict__ K1p, const double2* __restrict__ Wp,
-                                                    const double2* __restrict__ Wtp, double scalep, double2* __restrict__ K1a_,
-                                                    const double2* __restrict__ Wa, const double2* __restrict__ Wta,
-                                                    double scalea, VtxV F, int n1, int nWP, int nVP, int wb, int NP, int NK,
+                                                    WTail Wtp, double scalep, double2* __restrict__ K1a_,
+                                                    const double2* __restrict__ Wa, WTail Wta, double scalea, VtxV F, int n1, int nWP, int nVP, int wb, int NP, int NK,
                                                     SymCls sym, int nb) {
     if ((int)blockIdx.x < nb) gemv_k1_block<PCH>(blockIdx.x, K1p, nullptr, Wp, Wtp, F, n1, nWP, nVP, wb, NP, scalep, NK, sym);
     else gemv_k1_block<ACH>(blockIdx.x - nb, K1a_, nullptr, Wa, Wta, F, n1, nWP, nVP, wb, NP, scalea, NK, sym);
@@ -1638,7 +1651,7 @@ __device__ __forceinline__ double2 k2d_rank1(const VtxV& F, int m, int v, int p)
 }
 template <int CH>
 __device__ __forceinline__ void gemv_k2_wave(int wid, int lane, double2* __restrict__ K2, const double2* __restrict__ K2a,
-                                             const double2* __restrict__ W, const double2* __restrict__ Wtail,
+                                             const double2* __restrict__ W, const WTail& Wtail,
                                              const VtxV& F, const ChanV& Lr, const ChanV& La, int n2b, int n2f, int nWP,
                                              int nVP, int wb, int NP, double scale, int NK, const SymCls& sym) {
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
@@ -1662,8 +1675,8 @@ __device__ __forceinline__ void gemv_k2_wave(int wid, int lane, double2* __restr
     far_pair(k2d_term<CH>(F, m, v, lane == 0 ? -Wc - 1 : Wc + 1, p), tl, tr);
     sr = wave_sum(sr); si = wave_sum(si);
     mr = wave_sum(mr); mi = wave_sum(mi);
-    const double2 wl = lo > 0 ? tail_left(Wtail, row, nVP, wb, lo) : czero();
-    const double2 wr = hi < 2 * nVP - 1 ? tail_right(Wtail, row, nVP, wb, hi) : czero();
+    const double2 wl = lo > 0 ? Wtail.left(row, nVP, wb, lo) : czero();
+    const double2 wr = hi < 2 * nVP - 1 ? Wtail.right(row, nVP, wb, hi) : czero();
     double2 acc = cadd(cadd(cmk(sr, si), cmul(tl, wl)), cmul(tr, wr));
     acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), cadd(cmk(mr, mi), cadd(wl, wr))));
     const double2 raw = cscale(acc, scale);
@@ -1684,7 +1697,7 @@ __device__ __forceinline__ void gemv_k2_wave(int wid, int lane, double2* __restr
 }
 template <int CH>
 __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const double2* __restrict__ K2a,
-                                                 const double2* __restrict__ W, const double2* __restrict__ Wtail, VtxV F,
+                                                 const double2* __restrict__ W, WTail Wtail, VtxV F,
                                                  ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int nVP, int wb, int NP,
                                                  double scale, int NK, SymCls sym) {
     gemv_k2_wave<CH>((blockIdx.x * blockDim.x + threadIdx.x) / WAVE, threadIdx.x & (WAVE - 1), K2, K2a, W, Wtail, F, Lr, La,
@@ -1697,7 +1710,7 @@ __global__ __launch_bounds__(256) void k_gemv_k2(double2* __restrict__ K2, const
 // K2t = (T (-1) Σ (2 ft - fa) Wt + 2 FL.t.K2 - FL.a.K2 + K2a) / 2   (src/BSE_templates.jl:95-110)
 __device__ __forceinline__ void gemv_k2_at_wave(int wid, int lane, double2* __restrict__ K2a, double2* __restrict__ K2t,
                                                 const double2* __restrict__ Wa, const double2* __restrict__ Wt,
-                                                const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
+                                                const WTail& Wat, const WTail& Wtt,
                                                 const VtxV& F, const ChanV& La, const ChanV& Lt, int n2b, int n2f, int nWP,
                                                 int nVP, int wb, int NP, double T, int NK, const SymCls& sym) {
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
@@ -1733,10 +1746,10 @@ __device__ __forceinline__ void gemv_k2_at_wave(int wid, int lane, double2* __re
     ar = wave_sum(ar); ai = wave_sum(ai); tr = wave_sum(tr); ti = wave_sum(ti);
     amr = wave_sum(amr); ami = wave_sum(ami); tmr = wave_sum(tmr); tmi = wave_sum(tmi);
     const bool hl = lo > 0, hr = hi < 2 * nVP - 1;
-    const double2 awl = hl ? tail_left(Wat, row, nVP, wb, lo) : czero();
-    const double2 awr = hr ? tail_right(Wat, row, nVP, wb, hi) : czero();
-    const double2 twl = hl ? tail_left(Wtt, row, nVP, wb, lo) : czero();
-    const double2 twr = hr ? tail_right(Wtt, row, nVP, wb, hi) : czero();
+    const double2 awl = hl ? Wat.left(row, nVP, wb, lo) : czero();
+    const double2 awr = hr ? Wat.right(row, nVP, wb, hi) : czero();
+    const double2 twl = hl ? Wtt.left(row, nVP, wb, lo) : czero();
+    const double2 twr = hr ? Wtt.right(row, nVP, wb, hi) : czero();
     double2 A = cadd(cadd(cmk(ar, ai), cmul(fal, awl)), cmul(far_, awr));
     double2 D = cadd(cadd(cmk(tr, ti), cmul(fdl, twl)), cmul(fdr, twr));
     A = cadd(A, cmul(k2d_rank1<ACH>(F, m, v, p), cadd(cmk(amr, ami), cadd(awl, awr))));
@@ -1757,10 +1770,10 @@ __device__ __forceinline__ void gemv_k2_at_wave(int wid, int lane, double2* __re
 // BSE_K2! of all three channels in one launch: waves [0, wp) the p classes (SGpp[2]), then the a and
 // t classes (SGph[2], t finished from a in the same wave)
 __global__ __launch_bounds__(256, 4) void k_gemv_k2_all(double2* __restrict__ K2p, const double2* __restrict__ Wp,
-                                                     const double2* __restrict__ Wtp, ChanV Lp, SymCls symp, int wp,
+                                                     WTail Wtp, ChanV Lp, SymCls symp, int wp,
                                                      double2* __restrict__ K2a, double2* __restrict__ K2t,
                                                      const double2* __restrict__ Wa, const double2* __restrict__ Wt,
-                                                     const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
+                                                     WTail Wat, WTail Wtt,
                                                      VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP, int wb,
                                                      int NP, double T, int NK, SymCls symat) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
@@ -1771,7 +1784,7 @@ __global__ __launch_bounds__(256, 4) void k_gemv_k2_all(double2* __restrict__ K2
 }
 __global__ __launch_bounds__(256, 4) void k_gemv_k2_at(double2* __restrict__ K2a, double2* __restrict__ K2t,
                                                     const double2* __restrict__ Wa, const double2* __restrict__ Wt,
-                                                    const double2* __restrict__ Wat, const double2* __restrict__ Wtt,
+                                                    WTail Wat, WTail Wtt,
                                                     VtxV F, ChanV La, ChanV Lt, int n2b, int n2f, int nWP, int nVP,
                                                     int wb, int NP, double T, int NK, SymCls sym) {
     gemv_k2_at_wave((blockIdx.x * blockDim.x + threadIdx.x) / WAVE, threadIdx.x & (WAVE - 1), K2a, K2t, Wa, Wt, Wat, Wtt, F,
@@ -1925,7 +1938,7 @@ __device__ __forceinline__ double2 k2_add_fl(double2 val, double2 lr, double2 la
 template <int CH>
 __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, const double2* __restrict__ K2a,
                                                      const double2* __restrict__ Y, const double2* __restrict__ Xf,
-                                                     const double2* __restrict__ W, const double2* __restrict__ Wtail,
+                                                     const double2* __restrict__ W, WTail Wtail,
                                                      VtxV F, ChanV Lr, ChanV La, int n2b, int n2f, int nWP, int wb,
                                                      int NP, double scale, double2* __restrict__ raw) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
@@ -1938,8 +1951,7 @@ __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, c
     double sr = 0.0, si = 0.0;
     for (int k = lane; k < 2 * wb; k += WAVE) { const double2 x = Wr[k - wb]; sr += x.x; si += x.y; }
     sr = wave_sum(sr); si = wave_sum(si);
-    const double2* Tr = Wtail + (size_t)(2 * (wb + 1)) * row;
-    const double2 wl = Tr[0], wr = Tr[(wb + 1) + wb];
+    const double2 wl = Wtail.at(row, 0, wb), wr = Wtail.at(row, (wb + 1) + wb, wb);
     const double2 wtot = cadd(cmk(sr, si), cadd(wl, wr));
     const ChanV& g = F.g[CH];
     const bool k3m = in_b(m, g.n3b);
