@@ -1963,6 +1963,7 @@ __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, c
         acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), wtot));
         if (k3m && in_f(v, g.n3f)) {
             double2 k3s = czero();
+#pragma unroll 8
             for (int w = -g.n3f; w < g.n3f; ++w) {
                 double2 k3 = g.k3(m, v, w, p);
                 if (CH == TCH) k3 = csub(cscale(k3, 2.0), F.g[ACH].k3(m, v, w, p));
@@ -2141,6 +2142,7 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
         double2 xh = cadd(cmul(cadd(k1h, cadd(A.k2(m, v), T.k2(m, v))), sph), k2h);
         if (in_b(m, gp.n3b) && in_f(v, n3p)) {
             double2 sp = czero();
+#pragma unroll 8
             for (int n = -n3p; n < n3p; ++n) {
                 const int wp = m - n - 1;
                 if (inmesh(wp)) sp = cadd(sp, cmul(P.k3(m, n, v), pp[wp]));
@@ -2149,6 +2151,7 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
         }
         if ((in_b(m, ga.n3b) || in_b(m, gt.n3b)) && in_f(v, n3h)) {
             double2 sh = czero();
+#pragma unroll 8
             for (int n = -n3h; n < n3h; ++n)
                 if (inmesh(n)) sh = cadd(sh, cmul(cadd(A.k3(m, v, n), T.k3(m, v, n)), ph[n]));
             xh = cadd(xh, sh);
