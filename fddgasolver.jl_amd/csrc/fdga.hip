@@ -262,11 +262,25 @@ void free_chan(DevChan& dc) {
 // P averages of the target vertex F (s-wave cross-channel lookups)
 int refresh_avg(fdga_ctx* c) {
     if (c->NP == 1) return 0;
+    // the K1, K2 and K3 averages of the three channels in one launch
+    PavgJobs jobs{};
+    unsigned blk = 0;
+    int j = 0;
     for (int r = 0; r < 3; ++r) {
-        k_pavg<<<cdiv(c->n1, 16), 256, 0, c->run>>>(c->F[r].A1, c->F[r].K1, c->n1, c->NP);
-        k_pavg<<<cdiv(c->n2, 16), 256, 0, c->run>>>(c->F[r].A2, c->F[r].K2, c->n2, c->NP);
-        k_pavg<<<cdiv(c->n3, 16), 256, 0, c->run>>>(c->F[r].A3, c->F[r].K3, c->n3, c->NP);
+        const std::pair<double2*, const double2*> ak[3] = {{c->F[r].A1, c->F[r].K1}, {c->F[r].A2, c->F[r].K2},
+                                                           {c->F[r].A3, c->F[r].K3}};
+        const size_t nn[3] = {c->n1, c->n2, c->n3};
+        for (int k = 0; k < 3; ++k, ++j) {
+            jobs.A[j] = ak[k].first;
+            jobs.K[j] = ak[k].second;
+            jobs.n[j] = nn[k];
+            jobs.blk0[j] = blk;
+            blk += cdiv(nn[k], 16);
+        }
     }
+    jobs.blk0[j] = blk;
+    jobs.njob = j;
+    k_pavg_multi<<<blk, 256, 0, c->run>>>(jobs, c->NP);
     LAUNCHCHK(c);
     return 0;
 }

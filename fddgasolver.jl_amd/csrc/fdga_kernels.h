@@ -4,7 +4,7 @@
 //   Dyson!                 k_dyson
 //   bubbles! (local)       k_bubble_local        elementwise, HBM bound
 //   bubbles_real_space!    k_dft2_rows (G -> G(R)) + k_bubble_nl (products and bfft over R' in LDS)
-//   s-wave averages        k_pavg
+//   s-wave averages        k_pavg_multi
 //   build_K3_cache!        k_cache               gather: Γ_irr / F0 / F-base caches
 //   BSE_L_K3!, BSE_K3!     zgemm_tiles<...>      batched complex GEMM on v_mfma_f64_16x16x4f64
 //   BSE_K1!, BSE_K2!,      k_bse_w + k_gemv_*    gather-GEMV over the bubble mesh
@@ -417,10 +417,11 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __
 // s-wave average over P of one array of `n` elements per P: a block takes 16 consecutive
 // elements, its 16 thread groups sum every 16th momentum, and the groups' sums are added in a
 // fixed order
-__global__ __launch_bounds__(256) void k_pavg(double2* __restrict__ A, const double2* __restrict__ K, size_t n, int NP) {
+__device__ __forceinline__ void pavg_block(double2* __restrict__ A, const double2* __restrict__ K, size_t n, int NP,
+                                           size_t blk) {
     __shared__ double2 red[16][17];
     const int el = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    const size_t i = blockIdx.x * (size_t)16 + el;
+    const size_t i = blk * 16 + el;
     double2 s = czero();
     if (i < n)
         for (int p = grp; p < NP; p += 16) s = cadd(s, K[i + n * p]);
@@ -432,6 +433,20 @@ __global__ __launch_bounds__(256) void k_pavg(double2* __restrict__ A, const dou
         for (int g = 0; g < 16; ++g) t = cadd(t, red[g][el]);
         A[i] = cscale(t, 1.0 / NP);
     }
+}
+// several averages in one launch: job j takes blocks [blk0[j], blk0[j + 1])
+constexpr int PAVG_JOBS = 9;
+struct PavgJobs {
+    double2* A[PAVG_JOBS];
+    const double2* K[PAVG_JOBS];
+    size_t n[PAVG_JOBS];
+    unsigned blk0[PAVG_JOBS + 1];
+    int njob;
+};
+__global__ __launch_bounds__(256) void k_pavg_multi(PavgJobs jobs, int NP) {
+    int j = 0;
+    while (j + 1 < jobs.njob && blockIdx.x >= jobs.blk0[j + 1]) ++j;
+    pavg_block(jobs.A[j], jobs.K[j], jobs.n[j], NP, blockIdx.x - jobs.blk0[j]);
 }
 
 // ---------------------------------------------------------------------------------------
