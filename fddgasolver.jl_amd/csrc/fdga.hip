@@ -957,7 +957,11 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n, bool sym = false) {
     const int npl = pl ? c->k3_npl : c->NP;
     if (c->NV3 <= WAVE && c->NP > 1) {   // lattice: P-independent operands loaded once per momentum chunk
         const size_t nwave = (size_t)cdiv(c->NV3, K3R_COLS) * c->NW3 * cdiv(npl, K3_PCH) * n;
-        k_k3_R_pchunk<<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n, pl, npl);
+        k_k3_R_pchunk<1, K3R_COLS><<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n, pl,
+                                                                                npl);
+    } else if (c->NV3 <= 2 * WAVE && c->NP > 1) {   // two ω per lane, two columns per wave
+        const size_t nwave = (size_t)cdiv(c->NV3, 2) * c->NW3 * cdiv(npl, K3_PCH) * n;
+        k_k3_R_pchunk<2, 2><<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n, pl, npl);
     } else {
         const size_t nwave = (size_t)cdiv(c->NV3, K3R_COLS) * c->NW3 * c->NP * n;
         k_k3_R<<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);

@@ -1216,76 +1216,87 @@ __device__ __forceinline__ void k3_R_dispatch(const K3Geom& g, const K3RChan& c,
     else if (c.ch == TCH) k3_R_columns<TCH>(g, c, j0, bb, lane);
     else k3_R_columns<ACH>(g, c, j0, bb, lane);
 }
-// Nν3 <= 64 (one ω per lane): a wave takes K3R_COLS columns νp of one Ω through a chunk of K3_PCH
+// Nν3 <= 64 KPL (KPL ω per lane): a wave takes NC columns νp of one Ω through a chunk of K3_PCH
 // momenta, loading the P-independent F0r and Γ columns once
-template <int CH>
+template <int CH, int KPL, int NC>
 __device__ __forceinline__ void k3_R_columns_pchunk(const K3Geom& g, const K3RChan& c, int j0, int iW, int p0, int lane,
                                                     const int* pl, int npl) {
-    constexpr int NC = K3R_COLS;
     const int NV = g.NV(), NW = g.NW();
     const int m = iW - (g.n3b - 1);
-    const int k = lane;
-    const bool kin = k < NV;
-    const int w = k - g.n3f;
-    const int w_ = crossing(m, w, CH);
-    const bool inb = kin && in_f(w_, g.n3f);
-    double2 f0[NC], gm[NC];
+    int w_[KPL];
+    bool kin[KPL], inb[KPL];
+    double2 f0[KPL][NC], gm[KPL][NC];
 #pragma unroll
-    for (int u = 0; u < NC; ++u) {
-        const int j = j0 + u;
-        if (kin && j < NV) {
-            const size_t kj = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW);
-            f0[u] = c.F0r[kj];
-            gm[u] = c.Gam[kj];
-        } else {
-            f0[u] = gm[u] = czero();
+    for (int q = 0; q < KPL; ++q) {
+        const int k = lane + WAVE * q;
+        kin[q] = k < NV;
+        w_[q] = crossing(m, k - g.n3f, CH);
+        inb[q] = kin[q] && in_f(w_[q], g.n3f);
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+            const int j = j0 + u;
+            if (kin[q] && j < NV) {
+                const size_t kj = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * iW);
+                f0[q][u] = c.F0r[kj];
+                gm[q][u] = c.Gam[kj];
+            } else {
+                f0[q][u] = gm[q][u] = czero();
+            }
         }
     }
     const int pend = min(p0 + K3_PCH, npl);
     for (int ip = p0; ip < pend; ++ip) {
         const int p = pl ? pl[ip] : ip;
         const size_t bb = (size_t)iW + (size_t)NW * p;
-        double2 P1 = czero(), P0 = czero(), L[NC], La[NC];
-        if (kin) {
-            const size_t pi = g.pidx(m, w, p);
-            P1 = c.Pi[pi];
-            P0 = c.Pi0[pi];
-        }
+        double2 P1[KPL], P0[KPL], L[KPL][NC], La[KPL][NC];
 #pragma unroll
-        for (int u = 0; u < NC; ++u) {
-            L[u] = La[u] = czero();
-            if (inb && j0 + u < NV) {
-                const size_t lo = (size_t)(w_ + g.n3f) + (size_t)NV * ((size_t)(j0 + u) + (size_t)NV * bb);
-                L[u] = c.Lr[lo];
-                if (CH == TCH) La[u] = c.La[lo];
+        for (int q = 0; q < KPL; ++q) {
+            P1[q] = P0[q] = czero();
+            if (kin[q]) {
+                const size_t pi = g.pidx(m, lane + WAVE * q - g.n3f, p);
+                P1[q] = c.Pi[pi];
+                P0[q] = c.Pi0[pi];
+            }
+#pragma unroll
+            for (int u = 0; u < NC; ++u) {
+                L[q][u] = La[q][u] = czero();
+                if (inb[q] && j0 + u < NV) {
+                    const size_t lo = (size_t)(w_[q] + g.n3f) + (size_t)NV * ((size_t)(j0 + u) + (size_t)NV * bb);
+                    L[q][u] = c.Lr[lo];
+                    if (CH == TCH) La[q][u] = c.La[lo];
+                }
             }
         }
 #pragma unroll
         for (int u = 0; u < NC; ++u) {
             const int j = j0 + u;
             if (j >= NV) continue;   // uniform over the wave
-            double2 x = czero();
-            if (kin) {
-                x = cscale(cadd(cmul(csub(P1, P0), f0[u]), cmul(P1, gm[u])), c.s1);
-                if (inb) {
-                    double2 Lx = L[u];
-                    if (CH == TCH) Lx = csub(cscale(Lx, 2.0), La[u]);
-                    x = cadd(x, cscale(cmul(P1, Lx), c.s2));
+            double cr = 0.0, ci = 0.0;
+#pragma unroll
+            for (int q = 0; q < KPL; ++q) {
+                if (!kin[q]) continue;
+                double2 x = cscale(cadd(cmul(csub(P1[q], P0[q]), f0[q][u]), cmul(P1[q], gm[q][u])), c.s1);
+                if (inb[q]) {
+                    double2 Lx = L[q][u];
+                    if (CH == TCH) Lx = csub(cscale(Lx, 2.0), La[q][u]);
+                    x = cadd(x, cscale(cmul(P1[q], Lx), c.s2));
                 }
-                const size_t ro = (size_t)k + (size_t)NV * ((size_t)j + (size_t)NV * bb);
+                const size_t ro = (size_t)(lane + WAVE * q) + (size_t)NV * ((size_t)j + (size_t)NV * bb);
                 if (c.R32) c.R32[ro] = make_float2((float)x.x, (float)x.y);
                 else c.R[ro] = x;
+                cr += x.x; ci += x.y;
             }
-            const double sr = wave_sum(x.x), si = wave_sum(x.y);
+            const double sr = wave_sum(cr), si = wave_sum(ci);
             if (lane == 0) c.csum[(size_t)j + (size_t)NV * bb] = cmk(sr, si);
         }
     }
 }
 // (momenta as in k_k3_A32: the list pl, or all NP when pl is null)
-__global__ __launch_bounds__(256) void k_k3_R_pchunk(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan,
+template <int KPL, int NC>
+__global__ __launch_bounds__(256, 4) void k_k3_R_pchunk(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan,
                                                      const int* __restrict__ pl, int npl) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
-    const int NV = 2 * g.n3f, NW = 2 * g.n3b - 1, nq = (NV + K3R_COLS - 1) / K3R_COLS;
+    const int NV = 2 * g.n3f, NW = 2 * g.n3b - 1, nq = (NV + NC - 1) / NC;
     const int npc = (npl + K3_PCH - 1) / K3_PCH;
     if (wid >= nq * NW * npc * nchan) return;
     int r = wid;
@@ -1293,9 +1304,9 @@ __global__ __launch_bounds__(256) void k_k3_R_pchunk(K3Geom g, K3RChan c0, K3RCh
     const int iW = r % NW; r /= NW;
     const int pc = r % npc, s = r / npc;
     const K3RChan& c = s == 0 ? c0 : s == 1 ? c1 : c2;
-    if (c.ch == PCH) k3_R_columns_pchunk<PCH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane, pl, npl);
-    else if (c.ch == TCH) k3_R_columns_pchunk<TCH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane, pl, npl);
-    else k3_R_columns_pchunk<ACH>(g, c, q * K3R_COLS, iW, pc * K3_PCH, lane, pl, npl);
+    if (c.ch == PCH) k3_R_columns_pchunk<PCH, KPL, NC>(g, c, q * NC, iW, pc * K3_PCH, lane, pl, npl);
+    else if (c.ch == TCH) k3_R_columns_pchunk<TCH, KPL, NC>(g, c, q * NC, iW, pc * K3_PCH, lane, pl, npl);
+    else k3_R_columns_pchunk<ACH, KPL, NC>(g, c, q * NC, iW, pc * K3_PCH, lane, pl, npl);
 }
 // one wave per K3R_COLS columns (νp, Ω, P) of a channel slot
 __global__ __launch_bounds__(256) void k_k3_R(K3Geom g, K3RChan c0, K3RChan c1, K3RChan c2, int nchan) {

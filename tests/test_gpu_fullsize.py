@@ -20,11 +20,13 @@ from helpers import relerr
 pytestmark = pytest.mark.gpu
 
 W = dict(L=16, T=0.2, U=2.0, nK1=64, nK2=(32, 32), nK3=(16, 32), mPinu=32, nG=256)
+# the largest point of the batched sweep: 24 x 24 lattice (the 3 x 8 FFT of the fused bubble) and
+# Nν3 = 96 > 64 (the BSE_K3! right-operand builder with two ω per lane)
+W24 = dict(L=24, T=0.1, U=2.9, nK1=96, nK2=(48, 48), nK3=(24, 48), mPinu=1, nG=192)
 
 
-def _pair(precision, seed=3):
+def _pair(precision, w=W, seed=3):
     import fdga
-    w = W
     Gloc = fdga.siam_bare_Green(w["nG"], w["T"], e=0.1, Δ=0.8, D=None)
     NP = w["L"] ** 2
     Gk = np.repeat(Gloc[:, None], NP, axis=1)
@@ -58,9 +60,10 @@ def _compare(S_nl, S_loc, NP, tol, what):
 
 @pytest.mark.parametrize("strategy", ["fdPA", "scPA"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
-def test_lattice_solver_with_k_independent_data_is_the_local_solver(precision, tol, strategy):
+@pytest.mark.parametrize("workload", ["L16", "L24"])
+def test_lattice_solver_with_k_independent_data_is_the_local_solver(precision, tol, strategy, workload):
     import fdga
-    S_nl, S_loc, NP = _pair(precision)
+    S_nl, S_loc, NP = _pair(precision, W if workload == "L16" else W24)
     for it in range(2):
         fdga.iterate_solver(S_nl, strategy)
         fdga.iterate_solver(S_loc, strategy)
