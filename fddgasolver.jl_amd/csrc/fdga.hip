@@ -475,12 +475,12 @@ int sde_sigma(fdga_ctx* c, double2* S, const double2* G, double scale) {
 }
 
 // Σ += U² T² Σ G G G  (src/SDE.jl:328-364; NL src/nonlocal/SDE.jl:400-447)
-int sde_u2(fdga_ctx* c, double2* S, const double2* G) {
+int sde_u2(fdga_ctx* c, double2* S, const double2* G, bool c_ready = false) {
     const fdga_dims& d = c->d;
     const double2 fac = cscale(cmul(c->U, c->U), d.T * d.T);
     const int NB = 4 * d.nG - 1;
     if (!d.nonlocal) {
-        k_u2_C<<<cdiv((size_t)NB * WAVE, 256), 256, 0, c->run>>>(c->u2C, G, G, d.nG, 1);
+        if (!c_ready) k_u2_C<<<cdiv((size_t)NB * WAVE, 256), 256, 0, c->run>>>(c->u2C, G, G, d.nG, 1);
         const SymCls ss = symcls(c, SG_S);
         k_u2_S<<<cdiv(nclass(ss, c->NG) * WAVE, 256), 256, 0, c->run>>>(S, c->u2C, G, d.nG, 1, fac, 1, ss);
         LAUNCHCHK(c);
@@ -570,12 +570,22 @@ int sde_levels(fdga_ctx* c, double2* S, const double2* G, const double2* Pp, con
     for (size_t i = 0; i < lv.size(); ++i) {
         const bool refl = lv[i].kind == 1;
         // a RefVertex with null tables contributes (U + 0) - U = 0 exactly
+        bool u2c_ready = false;
         if (!(refl && !c->tabs_nonzero)) {
             if (sde_level_L(c, lv[i], Pp, Ph, start == 1)) return -3;
-            if (sde_sigma(c, S, G, lv[i].scale)) return -3;
+            if (i == 0 && !c->d.nonlocal) {   // this level's Σ sum and the U² term's C(m) in one launch
+                const SymCls ss = symcls(c, SG_S);
+                const size_t nsig = nclass(ss, c->NG), nu = (size_t)(4 * c->d.nG - 1);
+                k_sigma_u2c<<<cdiv((nsig + nu) * WAVE, 256), 256, 0, c->run>>>(
+                    S, G, c->Lpp, c->Lph, c->d.nG, c->d.nK2b, c->d.nK2f, c->d.T, lv[i].scale, ss, c->u2C, (int)nsig);
+                LAUNCHCHK(c);
+                u2c_ready = true;
+            } else if (sde_sigma(c, S, G, lv[i].scale)) {
+                return -3;
+            }
         }
         if (i == 0) {
-            if (sde_u2(c, S, G)) return -3;
+            if (sde_u2(c, S, G, u2c_ready)) return -3;
             if (incH && sde_hartree(c, S, G, +1.0)) return -3;
         }
     }

@@ -2189,10 +2189,9 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
 }
 
 // Σ(ν) += scale T Σ_Ω [G(Ω-ν) Lpp(Ω,ν) + G(Ω+ν) Lph(Ω,ν)]  (src/SDE.jl:242-263); one wave per ν
-__global__ __launch_bounds__(256) void k_sde_sigma_local(double2* __restrict__ S, const double2* __restrict__ G,
-                                                         const double2* __restrict__ Lpp, const double2* __restrict__ Lph,
-                                                         int nG, int n2b, int n2f, double T, double scale, SymCls sym) {
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+__device__ __forceinline__ void sde_sigma_local_wave(int wid, int lane, double2* __restrict__ S, const double2* __restrict__ G,
+                                                     const double2* __restrict__ Lpp, const double2* __restrict__ Lph,
+                                                     int nG, int n2b, int n2f, double T, double scale, const SymCls& sym) {
     if (wid >= sym.count(2 * nG)) return;
     const int iv = sym.rep(wid);
     const int v = iv - nG;
@@ -2217,6 +2216,12 @@ __global__ __launch_bounds__(256) void k_sde_sigma_local(double2* __restrict__ S
         const int code = sym.member(wid, k), o = code >> 2;
         S[o] = cadd(S[o], sym_apply(code & 3, raw));
     }
+}
+__global__ __launch_bounds__(256) void k_sde_sigma_local(double2* __restrict__ S, const double2* __restrict__ G,
+                                                         const double2* __restrict__ Lpp, const double2* __restrict__ Lph,
+                                                         int nG, int n2b, int n2f, double T, double scale, SymCls sym) {
+    sde_sigma_local_wave((blockIdx.x * blockDim.x + threadIdx.x) / WAVE, threadIdx.x & (WAVE - 1), S, G, Lpp, Lph, nG, n2b,
+                         n2f, T, scale, sym);
 }
 
 // NL SDE_compute_inner! real-space step (src/nonlocal/SDE.jl:209-276): for output column q of
@@ -2267,10 +2272,9 @@ __global__ void k_sde_sigma_nl_R(double2* __restrict__ SR, const double2* __rest
 // SDE_U2_using_G (src/SDE.jl:328-364) via C(m) = Σ_n G(n) G(n+m):
 //   Σ_U²(ν) = U² T² Σ_m C(m) G(m + ν); for the NL solver one column per R of the real-space G.
 // One wave per output, lanes over the summed index.
-__global__ __launch_bounds__(256) void k_u2_C(double2* __restrict__ C, const double2* __restrict__ Gm,
-                                              const double2* __restrict__ Gp, int nG, int nR) {
+__device__ __forceinline__ void u2_C_wave(int wid, int lane, double2* __restrict__ C, const double2* __restrict__ Gm,
+                                          const double2* __restrict__ Gp, int nG, int nR) {
     const int NG = 2 * nG, NB = 4 * nG - 1;
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     if (wid >= NB * nR) return;
     const int im = wid % NB, r = wid / NB;
     const int m = im - (2 * nG - 1);
@@ -2284,6 +2288,20 @@ __global__ __launch_bounds__(256) void k_u2_C(double2* __restrict__ C, const dou
     }
     sr = wave_sum(sr); si = wave_sum(si);
     if (lane == 0) C[(size_t)im + (size_t)NB * r] = cmk(sr, si);
+}
+__global__ __launch_bounds__(256) void k_u2_C(double2* __restrict__ C, const double2* __restrict__ Gm,
+                                              const double2* __restrict__ Gp, int nG, int nR) {
+    u2_C_wave((blockIdx.x * blockDim.x + threadIdx.x) / WAVE, threadIdx.x & (WAVE - 1), C, Gm, Gp, nG, nR);
+}
+// local solver, first SDE level: the Σ sum over the L-vertices (waves [0, nsig)) and the U² term's
+// C(m) (the waves after them) in one launch; they read different arrays and write S and C
+__global__ __launch_bounds__(256) void k_sigma_u2c(double2* __restrict__ S, const double2* __restrict__ G,
+                                                   const double2* __restrict__ Lpp, const double2* __restrict__ Lph,
+                                                   int nG, int n2b, int n2f, double T, double scale, SymCls sym,
+                                                   double2* __restrict__ C, int nsig) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid < nsig) sde_sigma_local_wave(wid, lane, S, G, Lpp, Lph, nG, n2b, n2f, T, scale, sym);
+    else u2_C_wave(wid - nsig, lane, C, G, G, nG, 1);
 }
 __global__ __launch_bounds__(256) void k_u2_S(double2* __restrict__ S, const double2* __restrict__ C,
                                               const double2* __restrict__ Gp, int nG, int nR, double2 fac, int accumulate,
