@@ -1522,9 +1522,8 @@ int fdga_build_k3_cache(fdga_ctx* c) {
     for (int k = 0; k < 12; ++k) cp.c[k] = c->cache[k];
     dim3 g(cdiv(c->NV3, 64), c->NV3, c->NW3);
     k_cache1<<<g, 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
-    k_cache2<<<g, 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f, c->NP == 1);
-    LAUNCHCHK(c);
-    k_cache_transpose<<<dim3(cdiv(c->NV3, 32), cdiv(c->NV3, 32), 2 * c->NW3), 256, 0, c->run>>>(cp, c->NV3);
+    k_cache2<<<dim3(cdiv(c->NV3, 16), cdiv(c->NV3, 4), c->NW3), 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f,
+                                                                                 c->NP == 1);
     LAUNCHCHK(c);
     return 0;
 }

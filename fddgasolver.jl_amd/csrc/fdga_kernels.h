@@ -457,7 +457,7 @@ __global__ __launch_bounds__(256) void k_pavg_multi(PavgJobs jobs, int NP) {
 //   c[4] Γpp [ν, ω]    c[5] Γa  [ν, ω]    c[6] Γt(d) [ν, ω]
 //   c[7] Fbase_p [ν, ω]  c[8] Fbase_a [ν, ω]  c[9] Fbase_t(d) [ν, ω]
 // c[0..9] the caches of build_K3_cache! (see k_cache2); c[10], c[11] Γa and Γt (t: 2 Γt - Γa) again,
-// transposed to [ω, ν, Ω] for the right operands of BSE_K3!
+// transposed to [ω, ν, Ω] for the right operands of BSE_K3! (written by k_cache2 through LDS)
 struct CachePtrs { double2* c[12]; };
 
 // loop 1 of build_K3_cache! at (Ω, ω, νp): Γpx and the F0 caches
@@ -479,10 +479,10 @@ __device__ __forceinline__ void cache1_elem(const CachePtrs& out, const VtxV& F,
 // (one momentum) the F caches are the reference's cache_Fr = F0 part + Γr + K2r(Ω,ν) + K3r(Ω,ν,ω)
 // (t: 2 (t) - (a)); otherwise (NL) they hold only the P-independent part Fbase_r and the same-channel
 // K2r + K3r(P) are added by the BSE_K3! loader and epilogue.
+// (i, j) inside the box; returns the Γa and Γt (2 Γt - Γa) entries for the transposed copies
 __device__ __forceinline__ void cache2_elem(const CachePtrs& out, const VtxV& F, int n3b, int n3f, int full, int i, int j,
-                                            int iW) {
+                                            int iW, double2& ga_out, double2& gt_out) {
     const int NV = 2 * n3f;
-    if (i >= NV) return;
     const int m = iW - (n3b - 1);
     const int a = i - n3f, b = j - n3f;
     const size_t o = (size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * iW);
@@ -492,9 +492,11 @@ __device__ __forceinline__ void cache2_elem(const CachePtrs& out, const VtxV& F,
     double2 dp = csub(F.f0.eval_psp(m, a, b, PCH), F.f0.eval_psp(m, FDGA_INF, b, PCH));
     double2 da = csub(F.f0.eval_psp(m, a, b, ACH), F.f0.eval_psp(m, FDGA_INF, b, ACH));
     double2 dt = csub(F.f0.eval_psp(m, a, b, TCH), F.f0.eval_psp(m, FDGA_INF, b, TCH));
+    ga_out = ga;
+    gt_out = csub(cscale(gt, 2.0), ga);
     out.c[4][o] = gpp;
-    out.c[5][o] = ga;
-    out.c[6][o] = csub(cscale(gt, 2.0), ga);
+    out.c[5][o] = ga_out;
+    out.c[6][o] = gt_out;
     double2 fbp = cadd(dp, gpp), fba = cadd(da, ga), fbt = cadd(dt, gt);
     if (full) {
         // within the K3 box F(ν) - F(νInf) of the channel's own γ is K2(Ω, ν) + K3(Ω, ν, ω)
@@ -509,27 +511,23 @@ __device__ __forceinline__ void cache2_elem(const CachePtrs& out, const VtxV& F,
 __global__ __launch_bounds__(64) void k_cache1(CachePtrs out, VtxV F, int n3b, int n3f) {
     cache1_elem(out, F, n3b, n3f, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
 }
+// loop 2 over tiles of 16 ν x 4 ω (one 64-thread block, blockIdx.z = iΩ); the Γa and Γt entries
+// also go through LDS to their transposed copies, four consecutive ω per store run
 __global__ __launch_bounds__(64) void k_cache2(CachePtrs out, VtxV F, int n3b, int n3f, int full) {
-    cache2_elem(out, F, n3b, n3f, full, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
-}
-
-// per-Ω transpose of the Γa and Γt caches ([ν, ω, Ω] -> [ω, ν, Ω]) through 32 x 32 LDS tiles
-// (coalesced reads and writes); blockIdx.z = 2 iΩ + which
-__global__ __launch_bounds__(256) void k_cache_transpose(CachePtrs cp, int NV) {
-    __shared__ double2 t[32][33];
-    const int which = blockIdx.z & 1;
-    const size_t base = (size_t)NV * NV * (blockIdx.z >> 1);
-    const double2* in = (which ? cp.c[6] : cp.c[5]) + base;
-    double2* out = (which ? cp.c[11] : cp.c[10]) + base;
-    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int r = ty; r < 32; r += 8) {
-        const int i = i0 + tx, j = j0 + r;
-        if (i < NV && j < NV) t[r][tx] = in[(size_t)i + (size_t)NV * j];
+    __shared__ double2 t[2][16][5];
+    const int NV = 2 * n3f;
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 4, iW = blockIdx.z;
+    {
+        const int il = threadIdx.x & 15, jl = threadIdx.x >> 4;
+        if (i0 + il < NV && j0 + jl < NV)
+            cache2_elem(out, F, n3b, n3f, full, i0 + il, j0 + jl, iW, t[0][il][jl], t[1][il][jl]);
     }
     __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int j = j0 + tx, i = i0 + r;
-        if (i < NV && j < NV) out[(size_t)j + (size_t)NV * i] = t[tx][r];
+    const int il = threadIdx.x >> 2, jl = threadIdx.x & 3;
+    if (i0 + il < NV && j0 + jl < NV) {
+        const size_t o = (size_t)(j0 + jl) + (size_t)NV * ((size_t)(i0 + il) + (size_t)NV * iW);
+        out.c[10][o] = t[0][il][jl];
+        out.c[11][o] = t[1][il][jl];
     }
 }
 
