@@ -1520,10 +1520,9 @@ int fdga_bubbles(fdga_ctx* c) { return bubbles_into(c, c->Pipp, c->Piph, c->G); 
 int fdga_build_k3_cache(fdga_ctx* c) {
     CachePtrs cp;
     for (int k = 0; k < 12; ++k) cp.c[k] = c->cache[k];
-    dim3 g(cdiv(c->NV3, 64), c->NV3, c->NW3);
+    const dim3 g(cdiv(c->NV3, 16), cdiv(c->NV3, 4), c->NW3);
     k_cache1<<<g, 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f);
-    k_cache2<<<dim3(cdiv(c->NV3, 16), cdiv(c->NV3, 4), c->NW3), 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f,
-                                                                                 c->NP == 1);
+    k_cache2<<<g, 64, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f, c->NP == 1);
     LAUNCHCHK(c);
     return 0;
 }

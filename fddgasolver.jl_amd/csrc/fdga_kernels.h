@@ -508,8 +508,10 @@ __device__ __forceinline__ void cache2_elem(const CachePtrs& out, const VtxV& F,
     out.c[8][o] = fba;
     out.c[9][o] = csub(cscale(fbt, 2.0), fba);
 }
+// loop 1 over tiles of 16 ω x 4 νp (one 64-thread block, blockIdx.z = iΩ)
 __global__ __launch_bounds__(64) void k_cache1(CachePtrs out, VtxV F, int n3b, int n3f) {
-    cache1_elem(out, F, n3b, n3f, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 4);
+    if (j < 2 * n3f) cache1_elem(out, F, n3b, n3f, blockIdx.x * 16 + (threadIdx.x & 15), j, blockIdx.z);
 }
 // loop 2 over tiles of 16 ν x 4 ω (one 64-thread block, blockIdx.z = iΩ); the Γa and Γt entries
 // also go through LDS to their transposed copies, four consecutive ω per store run
