@@ -94,10 +94,9 @@ def observables(S):
     """Σ, χ_sp = K1a/U², χ_ch = -(2 K1t - K1a)/U² (pSp storage of the connected K1) and the three
     channels' K3, read back from the device."""
     import numpy as np
-    U2 = S.U ** 2
-    k1a = S.vertex(0, 2, 1)
-    k1t = S.vertex(0, 1, 1)
-    return {"Sigma": S.Σ, "chi_sp": k1a / U2, "chi_ch": -(2 * k1t - k1a) / U2,
+    import fdga
+    chi_sp, chi_ch = fdga.susceptibilities(S)
+    return {"Sigma": S.Σ, "chi_sp": chi_sp, "chi_ch": chi_ch,
             "K3": np.concatenate([S.vertex(0, ch, 3) for ch in range(3)])}
 
 

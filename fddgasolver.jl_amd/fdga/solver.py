@@ -419,6 +419,30 @@ def unflatten(S, x):
     S._check(S.lib.fdga_unflatten(S.h, _dp(x.view(np.float64))))
 
 
+def susceptibilities(S):
+    """Spin and charge susceptibilities from the connected K1 of the full vertex in its pSp storage.
+    The density/magnetic relations K1_D = -U² χ_D and K1_M = -U² χ_M with K1a = -K1_M and
+    K1t = (K1_D - K1_M) / 2 (src/utility/load_triqs.jl:73-75,106-115) give
+    χ_sp = K1a / U² and χ_ch = -(2 K1t - K1a) / U².  Returns (χ_sp, χ_ch) over the K1 box's bosonic
+    frequencies Ω = -(nK1 - 1) .. nK1 - 1, with a second axis over the lattice momenta (the vertex's
+    P order) for the lattice solver."""
+    NP = S.NP if S.nonlocal_ else 1
+    return chi_from_k1(S.vertex(0, aCh, 1), S.vertex(0, tCh, 1), S.U, NP)
+
+
+def chi_from_k1(k1a, k1t, U, NP=1):
+    """(χ_sp, χ_ch) from the a- and t-channel K1 arrays of one vertex (Ω fastest, then P)."""
+    k1a = np.asarray(k1a, dtype=np.complex128)
+    k1t = np.asarray(k1t, dtype=np.complex128)
+    if k1a.shape != k1t.shape:
+        raise ValueError(f"K1a and K1t differ in shape: {k1a.shape} vs {k1t.shape}")
+    U2 = float(U) ** 2
+    sp, ch = k1a / U2, -(2.0 * k1t - k1a) / U2
+    if NP > 1:
+        sp, ch = sp.reshape(-1, NP, order="F"), ch.reshape(-1, NP, order="F")
+    return sp, ch
+
+
 def fixed_point(R, x, S, strategy="fdPA"):
     """fixed_point!(R, x, S) (src/solve.jl:119-157): R = G(x) - x."""
     x = np.ascontiguousarray(x, dtype=np.complex128)
