@@ -11,7 +11,7 @@ module FdgaHIP
 using fdDGAsolver
 using MatsubaraFunctions
 import fdDGAsolver: iterate_solver!, bubbles!, build_K3_cache!, BSE_K1!, BSE_K2!, BSE_K3!, BSE_L_K2!, BSE_L_K3!,
-                    Dyson!, SDE!, init_sym_grp!, reset_sym_grp!
+                    Dyson!, SDE!, init_sym_grp!, reset_sym_grp!, fixed_point!
 
 const libfdga = joinpath(@__DIR__, "..", "lib", "libfdga.so")
 
@@ -100,5 +100,54 @@ for (jl, c) in ((:BSE_K1!, :fdga_bse_k1), (:BSE_K2!, :fdga_bse_k2), (:BSE_K3!, :
         check(ctx, ccall(($(QuoteNode(c)), libfdga), Cint, (Ptr{Cvoid}, Int32), ctx, chcode(Ch)))
     end
 end
+for (jl, c) in ((:Dyson!, :fdga_dyson), (:bubbles!, :fdga_bubbles), (:build_K3_cache!, :fdga_build_k3_cache))
+    @eval function $jl(S::ParquetSolver{ComplexF64}, ::Val{:hip})
+        ctx = context(S)
+        check(ctx, ccall(($(QuoteNode(c)), libfdga), Cint, (Ptr{Cvoid},), ctx))
+    end
+end
+# SDE! (src/SDE.jl:3-33): the device path has the default include_U² = true, subtract_asymptotic = false
+function SDE!(S::ParquetSolver{ComplexF64}, ::Val{:hip}; strategy = :scPA, include_U² = true, include_Hartree = true,
+              subtract_asymptotic = false)
+    (include_U² && !subtract_asymptotic) || throw(ArgumentError("the device SDE! has include_U² = true, subtract_asymptotic = false"))
+    ctx = context(S)
+    check(ctx, ccall((:fdga_sde, libfdga), Cint, (Ptr{Cvoid}, Int32, Int32), ctx, stratcode(strategy), Int32(include_Hartree)))
+end
+
+# fixed_point! (src/solve.jl:119-157) with update_Σ = true: R = G(x) - x, one device pass
+function fixed_point!(R::Vector{ComplexF64}, x::Vector{ComplexF64}, S::ParquetSolver{ComplexF64}, ::Val{:hip};
+                      strategy::Symbol = :fdPA)
+    ctx = context(S)
+    n = ccall((:fdga_flat_size, libfdga), Int64, (Ptr{Cvoid},), ctx)
+    (length(x) == n && length(R) == n) || throw(DimensionMismatch("state vectors of length $n expected"))
+    check(ctx, ccall((:fdga_fixed_point, libfdga), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int32),
+                     ctx, R, x, stratcode(strategy)))
+    return nothing
+end
+
+# launch options (no reference counterpart; results do not depend on them)
+set_graph!(S::ParquetSolver{ComplexF64}, on::Bool) =
+    check(context(S), ccall((:fdga_set_graph, libfdga), Cint, (Ptr{Cvoid}, Int32), context(S), Int32(on)))
+autotune!(S::ParquetSolver{ComplexF64}; reps::Integer = 3) =
+    check(context(S), ccall((:fdga_autotune, libfdga), Cint, (Ptr{Cvoid}, Int32), context(S), Int32(reps)))
+
+# solver groups: several solvers (a parameter sweep) iterated together, the contractions of all of
+# them in one launch each; destroy the group before its members
+function group(solvers::Vector{<:ParquetSolver{ComplexF64}})
+    ctxs = [context(S) for S in solvers]
+    g = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:fdga_group_create, libfdga), Cint, (Ref{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Int32), g, ctxs, length(ctxs))
+    rc == 0 || error("fdga_group_create failed ($rc)")
+    return g[]
+end
+group_check(g, rc) = rc == 0 || error(unsafe_string(ccall((:fdga_group_last_error, libfdga), Cstring, (Ptr{Cvoid},), g)))
+group_iterate!(g::Ptr{Cvoid}; strategy::Symbol = :fdPA) =
+    group_check(g, ccall((:fdga_group_iterate, libfdga), Cint, (Ptr{Cvoid}, Int32, Int32), g, stratcode(strategy), 1))
+group_destroy(g::Ptr{Cvoid}) = ccall((:fdga_group_destroy, libfdga), Cint, (Ptr{Cvoid},), g)
+
+# NL_ParquetSolver (src/nonlocal/ParquetSolver.jl:77-152) binds the same way with fdga_dims.nonlocal = 1,
+# LG and L the side lengths of the G and vertex Brillouin-zone meshes, and the momentum-resolved
+# arrays in their MeshFunction data layout (frequency first, then momentum); the Python host
+# (fdga.NL_ParquetSolver) is the tested form of that binding.
 
 end # module
