@@ -144,6 +144,10 @@ group_check(g, rc) = rc == 0 || error(unsafe_string(ccall((:fdga_group_last_erro
 group_iterate!(g::Ptr{Cvoid}; strategy::Symbol = :fdPA) =
     group_check(g, ccall((:fdga_group_iterate, libfdga), Cint, (Ptr{Cvoid}, Int32, Int32), g, stratcode(strategy), 1))
 group_destroy(g::Ptr{Cvoid}) = ccall((:fdga_group_destroy, libfdga), Cint, (Ptr{Cvoid},), g)
+group_set_graph!(g::Ptr{Cvoid}, on::Bool) =
+    group_check(g, ccall((:fdga_group_set_graph, libfdga), Cint, (Ptr{Cvoid}, Int32), g, Int32(on)))
+group_autotune!(g::Ptr{Cvoid}; reps::Integer = 3) =
+    group_check(g, ccall((:fdga_group_autotune, libfdga), Cint, (Ptr{Cvoid}, Int32), g, Int32(reps)))
 
 # NL_ParquetSolver (src/nonlocal/ParquetSolver.jl:77-152) binds the same way with fdga_dims.nonlocal = 1,
 # LG and L the side lengths of the G and vertex Brillouin-zone meshes, and the momentum-resolved
