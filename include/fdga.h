@@ -4,7 +4,7 @@
  * The reference has no FFI; its seam is Julia multiple dispatch on the solver and the
  * `mode` symbol (src/solve.jl:173-175).  Each entry point below replaces one reference
  * function for a solver whose state lives on the device; a Julia method for `S.mode === :hip`
- * binds it with `ccall`.  Host arrays use the reference's MeshFunction
+ * binds it with `ccall` (fddgasolver.jl_amd/julia/FdgaHIP.jl).  Host arrays use the reference's MeshFunction
  * storage (column-major, first index fastest, ComplexF64 = two interleaved doubles) and are
  * borrowed only for the duration of a call.  Every function returns 0 on success and a
  * negative code on error; fdga_last_error() returns the message.  A context is not

@@ -4,7 +4,7 @@ Every stage of iterate_solver! is compared on random inputs in which every term 
 iteration is non-zero (random target vertex, random local reference vertex with RefVertex
 tables, random G0/Σ0), for the local solver and the s-wave lattice solver with even, odd and
 unequal k-meshes.  Tolerance: 1e-12 relative to the largest magnitude of each array (the
-north star asks 1e-10 for the plain FP64 path; only summation order differs).
+plain FP64 path's bar against the reference is 1e-10; only summation order differs).
 """
 import numpy as np
 import pytest
