@@ -42,3 +42,39 @@ def test_no_device_fails_loudly():
     env = dict(os.environ, HIP_VISIBLE_DEVICES="-1")
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=120)
     assert out.stdout.startswith("ERR") or out.stdout.startswith("DEVICE"), out.stdout + out.stderr
+
+
+def _header_dims():
+    """(name, C type) of fdga_dims in declaration order, from the header text."""
+    text = open(L.HEADER).read()
+    body = re.search(r"typedef struct fdga_dims \{(.*?)\} fdga_dims;", text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        ctype, names = decl.split(None, 1)
+        fields += [(n.strip(), ctype) for n in names.split(",")]
+    return fields
+
+
+def test_dims_mirrors_match_the_header():
+    """The Python ctypes and the Julia mirrors of fdga_dims have the header's fields, order and
+    types (a shifted or retyped field would hand the library a corrupted box)."""
+    hdr = _header_dims()
+    assert len(hdr) == 22
+    py = [(n.rstrip("_"), t) for n, t in L.Dims._fields_]
+    ctypes_of = {"int32_t": ctypes.c_int32, "double": ctypes.c_double}
+    assert py == [(n, ctypes_of[t]) for n, t in hdr]
+    off = 0   # natural C layout: each field at a multiple of its size, the struct a multiple of 8
+    for _, t in hdr:
+        sz = 4 if t == "int32_t" else 8
+        off = -(-off // sz) * sz + sz
+    assert ctypes.sizeof(L.Dims) == -(-off // 8) * 8
+    jl_path = os.path.join(os.path.dirname(L.HEADER), "..", "fddgasolver.jl_amd", "julia", "FdgaHIP.jl")
+    jl = open(jl_path).read()
+    body = re.search(r"struct FdgaDims(.*?)\nend", jl, re.S).group(1)
+    jl_fields = re.findall(r"(\w+)::(Int32|Float64)", body)
+    julia_of = {"int32_t": "Int32", "double": "Float64"}
+    assert jl_fields == [(n, julia_of[t]) for n, t in hdr]
