@@ -78,3 +78,55 @@ def test_dims_mirrors_match_the_header():
     jl_fields = re.findall(r"(\w+)::(Int32|Float64)", body)
     julia_of = {"int32_t": "Int32", "double": "Float64"}
     assert jl_fields == [(n, julia_of[t]) for n, t in hdr]
+
+
+def _c_class(decl):
+    decl = decl.strip()
+    if decl.startswith("const char*"):
+        return "str"
+    if "*" in decl:
+        return "ptr"
+    return {"int": "i32", "int32_t": "i32", "int64_t": "i64", "double": "f64"}[decl.split()[0]]
+
+
+def _header_signatures():
+    """name -> (return class, [argument classes]) from the header's declarations."""
+    text = re.sub(r"/\*.*?\*/", "", open(L.HEADER).read(), flags=re.S)
+    sigs = {}
+    for ret, name, params in re.findall(r"^\s*(const char\*|int64_t|int|double)\s+(fdga_\w+)\s*\(([^)]*)\);", text,
+                                        re.M):
+        sigs[name] = (_c_class(ret), [_c_class(p) for p in params.split(",") if p.strip()])
+    return sigs
+
+
+def test_python_signatures_match_the_header():
+    """argtypes/restype of every ctypes binding agree with the header's declaration."""
+    def cls(t):
+        if t in (ctypes.c_int, ctypes.c_int32):
+            return "i32"
+        if t in (ctypes.c_int64, ctypes.c_longlong):
+            return "i64"
+        if t is ctypes.c_double:
+            return "f64"
+        if t is ctypes.c_char_p:
+            return "str"
+        assert t is ctypes.c_void_p or issubclass(t, ctypes._Pointer), t
+        return "ptr"
+    hdr = _header_signatures()
+    assert set(hdr) == set(L.SIGNATURES)
+    for name, (res, args) in L.SIGNATURES.items():
+        assert (cls(res), [cls(a) for a in args]) == hdr[name], name
+
+
+def test_julia_ccalls_match_the_header():
+    """every ccall with a literal symbol in the Julia binding passes the header's argument types."""
+    jl_path = os.path.join(os.path.dirname(L.HEADER), "..", "fddgasolver.jl_amd", "julia", "FdgaHIP.jl")
+    jl = open(jl_path).read()
+    jcls = {"Cint": "i32", "Int32": "i32", "Int64": "i64", "Float64": "f64", "Cdouble": "f64", "Cstring": "str"}
+    hdr = _header_signatures()
+    calls = re.findall(r"ccall\(\(:(\w+), libfdga\), (\w+), \(([^()]*)\)", jl)
+    assert len(calls) >= 20
+    for name, ret, args in calls:
+        a = [t.strip() for t in args.split(",") if t.strip()]
+        got = (jcls[ret], ["ptr" if t.startswith(("Ptr{", "Ref{")) else jcls[t] for t in a])
+        assert got == hdr[name], name
