@@ -15,6 +15,9 @@ from . import _lib as L
 
 pCh, tCh, aCh = L.PCH, L.TCH, L.ACH
 _STRATEGIES = {"scPA": L.SCPA, "fdPA": L.FDPA}
+# strategies iterate_solver! accepts (src/solve.jl:10) whose BSE forms are not on the device path:
+# the alternative K1/K2 forms (src/BSE_templates.jl:188-252) and the one-loop K3 (src/solve.jl:47-58)
+_STRATEGIES_NOT_BUILT = ("scPA_new", "fdPA_new", "fdPA_1loop")
 
 
 def _dp(a):
@@ -34,8 +37,13 @@ class Solver:
 
     def __init__(self, *, nG, nK1, nK2, nK3, T, U, mPinu, nonlocal_=False, LG=1, L_=1,
                  ref_kind=0, ref_boxes=None, tab=(1, 1), fdpa_h0=True, precision="fp64"):
-        if nK1 < max(nK2) or any(a < b for a, b in zip(nK2, nK3)):
-            raise AssertionError("Number of frequencies in K1 >= K2 >= K3 required")
+        # the Channel constructor's box assertions (src/channel.jl:33-34,40)
+        if nK1 < nK2[0]:
+            raise AssertionError("No. bosonic frequencies in K1 must be larger than no. bosonic frequencies in K2")
+        if nK1 < nK2[1]:
+            raise AssertionError("No. bosonic frequencies in K1 must be larger than no. fermionic frequencies in K2")
+        if any(a < b for a, b in zip(nK2, nK3)):
+            raise AssertionError("Number of frequencies in K2 must be larger than in K3")
         d = L.Dims()
         d.nonlocal_ = int(nonlocal_)
         d.nG, d.nK1 = nG, nK1
@@ -253,6 +261,8 @@ class SolverGroup:
 def _strategy(strategy):
     if isinstance(strategy, int):
         return strategy
+    if strategy in _STRATEGIES_NOT_BUILT:
+        raise NotImplementedError(f"strategy {strategy} (a reference strategy) is not on the device path; use scPA or fdPA")
     if strategy not in _STRATEGIES:
         raise AssertionError("Calculation strategy unknown")
     return _STRATEGIES[strategy]
