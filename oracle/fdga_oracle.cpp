@@ -25,6 +25,8 @@
 #include <functional>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 using cplx = std::complex<double>;
@@ -337,10 +339,13 @@ void build_group(SymGroup& sg, int rank, const int* sizes, const std::function<i
 // SG(f, InitFunction(diagram)): evaluate class representatives, fill members
 void sg_eval(const SymGroup& sg, std::vector<cplx>& out, size_t n, const std::function<cplx(size_t)>& f) {
     if (sg.identity) {
-        for (size_t i = 0; i < n; ++i) out[i] = f(i);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < (int64_t)n; ++i) out[i] = f((size_t)i);
         return;
     }
-    for (auto& cls : sg.classes) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t c = 0; c < (int64_t)sg.classes.size(); ++c) {
+        const auto& cls = sg.classes[c];
         cplx r = f((size_t)cls[0].first);
         for (auto& [j, op] : cls) out[j] = apply_op(op, r);
     }
@@ -362,6 +367,7 @@ void dft2(std::vector<cplx>& data, size_t nrows, int L, int sign) {
     for (int k = 0; k < L; ++k) tw[k] = std::polar(1.0, sign * 2.0 * M_PI * k / L);
     std::vector<cplx> buf(L);
     // along j1
+#pragma omp parallel for collapse(2) firstprivate(buf)
     for (int j2 = 0; j2 < L; ++j2)
         for (size_t r = 0; r < nrows; ++r) {
             for (int k = 0; k < L; ++k) {
@@ -372,6 +378,7 @@ void dft2(std::vector<cplx>& data, size_t nrows, int L, int sign) {
             for (int k = 0; k < L; ++k) data[r + nrows * (k + (size_t)L * j2)] = buf[k];
         }
     // along j2
+#pragma omp parallel for collapse(2) firstprivate(buf)
     for (int j1 = 0; j1 < L; ++j1)
         for (size_t r = 0; r < nrows; ++r) {
             for (int k = 0; k < L; ++k) {
@@ -420,6 +427,7 @@ void bubbles_nl(const orc_solver* s, std::vector<cplx>& pp, std::vector<cplx>& p
             int km = imod(-R1, LG) + LG * imod(-R2, LG);
             int iR = imod(R1, L) + L * imod(R2, L);
             bool origin = (R1 == 0 && R2 == 0);
+#pragma omp parallel for
             for (int iv = 0; iv < s->mPiF.size(); ++iv)
                 for (int iW = 0; iW < s->mPiB.size(); ++iW) {
                     int m = s->mPiB.val(iW), n = s->mPiF.val(iv);
@@ -733,6 +741,7 @@ void orc_build_k3_cache(orc_solver* s) {
     const int nb = c.b3.size(), nf = c.f3.size();
     const Vert& F = s->F;
     const Vert& F0 = s->F0v;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
     for (int P = 0; P < s->NP; ++P)
         for (int i3 = 0; i3 < nf; ++i3)
             for (int i2 = 0; i2 < nf; ++i2)
@@ -765,6 +774,66 @@ void orc_build_k3_cache(orc_solver* s) {
                         s->cache[9][i] = Ft * 2.0 - Fa;
                     }
                 }
+}
+
+// build_K3_cache_mfRG!(S, is_first_iteration) (src/nonlocal_2/build_K3_cache.jl:97-164, the method that
+// mfRGLinearMap calls for NL_ParquetSolver): the Γ caches are F(...; γr = false) - F.F0(...; γr = false),
+// the F caches (first call only) the reference vertex's F0(ν) - F0(νInf); the F0 caches are left as they
+// are.  Each cache is evaluated over the K3 symmetry group of its channel (SGpp[3] / SGph[3]).
+void orc_build_k3_cache_mfrg(orc_solver* s, int32_t is_first) {
+    s->F.refresh();
+    const Chan& c = s->F.g[0];
+    const int nb = c.b3.size(), nf = c.f3.size();
+    const size_t n3 = c.n3();
+    const Vert& F = s->F;
+    const Vert& F0 = s->F0v;
+    auto idx = [&](size_t ii, int& m, int& a, int& b, int& P) {
+        const size_t i = ii % n3;
+        P = (int)(ii / n3);
+        m = c.b3.val((int)(i % nb));
+        a = c.f3.val((int)((i / nb) % nf));
+        b = c.f3.val((int)(i / ((size_t)nb * nf)));
+    };
+    const size_t ntot = n3 * s->NP;
+    // γr = false in channel r: F0 and the two other channels
+    auto gam = [&](int m, int a, int b, int P, int ch, int sp) {
+        const int fl = FL_ALL & ~(ch == PCH ? FL_GP : ch == TCH ? FL_GT : FL_GA);
+        return F.eval(m, a, b, P, ch, sp, fl) - F0.eval(m, a, b, 0, ch, sp, fl);
+    };
+    sg_eval(s->SGpp[3], s->cache[0], ntot, [&](size_t ii) {
+        int m, a, b, P; idx(ii, m, a, b, P);
+        return gam(m, a, b, P, PCH, XSP);
+    });
+    sg_eval(s->SGpp[3], s->cache[4], ntot, [&](size_t ii) {
+        int m, a, b, P; idx(ii, m, a, b, P);
+        return gam(m, a, b, P, PCH, PSP);
+    });
+    sg_eval(s->SGph[3], s->cache[5], ntot, [&](size_t ii) {
+        int m, a, b, P; idx(ii, m, a, b, P);
+        return gam(m, a, b, P, ACH, PSP);
+    });
+    sg_eval(s->SGph[3], s->cache[6], ntot, [&](size_t ii) {
+        int m, a, b, P; idx(ii, m, a, b, P);
+        return gam(m, a, b, P, TCH, PSP);
+    });
+    for (size_t i = 0; i < ntot; ++i) s->cache[6][i] = s->cache[6][i] * 2.0 - s->cache[5][i];
+    if (!is_first) return;
+    auto d0 = [&](int m, int a, int b, int ch) {
+        return F0.eval(m, a, b, 0, ch, PSP, FL_ALL) - F0.eval(m, INF, b, 0, ch, PSP, FL_ALL);
+    };
+    sg_eval(s->SGpp[3], s->cache[7], ntot, [&](size_t ii) {
+        int m, a, b, P; idx(ii, m, a, b, P);
+        return d0(m, a, b, PCH);
+    });
+    sg_eval(s->SGph[3], s->cache[8], ntot, [&](size_t ii) {
+        int m, a, b, P; idx(ii, m, a, b, P);
+        return d0(m, a, b, ACH);
+    });
+    sg_eval(s->SGph[3], s->cache[9], ntot, [&](size_t ii) {
+        int m, a, b, P; idx(ii, m, a, b, P);
+        return d0(m, a, b, TCH);
+    });
+    for (size_t i = 0; i < ntot; ++i) s->cache[9][i] = s->cache[9][i] * 2.0 - s->cache[8][i];
 }
 
 static const int kSign[3] = {+1, -1, +1};        // BSE_K1/L_K2/K2 sign: p +1, t -1, a +1
@@ -850,14 +919,18 @@ void orc_bse_l_k3(orc_solver* s, int32_t ch) {
     s->FL.g[ch].refresh();
 }
 
-// BSE_K1! (src/BSEa/BSEa_K1.jl:2-54; NL src/nonlocal/BSEa/BSEa_K1.jl:2-57), is_mfRG = false
-void orc_bse_k1(orc_solver* s, int32_t ch) {
+// BSE_K1! (src/BSEa/BSEa_K1.jl:2-54; NL src/nonlocal/BSEa/BSEa_K1.jl:2-57).  is_mfRG = true: the central
+// part F0(Ω, νInf, ω) Π0 FL(Ω, ω~, νInf); the local form evaluates both vertices in aCh / pSp whatever the
+// channel (src/BSEa/BSEa_K1.jl:29-35), the lattice form in the channel's own (src/nonlocal/BSEa/BSEa_K1.jl:29-37)
+void orc_bse_k1(orc_solver* s, int32_t ch, int32_t is_mfrg) {
     s->F.refresh();
     s->FL.refresh();
     Chan& out = s->Fbuff.g[ch];
     const std::vector<cplx>& Pi0 = (ch == PCH) ? s->Pi0pp : s->Pi0ph;
     const std::vector<cplx>& Pi = (ch == PCH) ? s->Pipp : s->Piph;
     const int sp = kSp[ch], sg = kSign[ch];
+    const bool nl = s->prm.nonlocal != 0;
+    const int mch = nl ? ch : ACH, msp = nl ? sp : PSP;   // vertices' channel / spin of the mfRG form
     const size_t n1 = out.n1();
     {
         std::vector<cplx> tmp(n1 * s->NP);
@@ -868,11 +941,17 @@ void orc_bse_k1(orc_solver* s, int32_t ch) {
             cplx val = 0.0;
             for (int iw = 0; iw < s->mPiF.size(); ++iw) {
                 int w = s->mPiF.val(iw);
-                cplx Fl = s->F.eval(m, INF, w, P, ch, sp, FL_ALL);
-                cplx F0r = s->F0v.eval(m, crossing(m, w, ch), INF, 0, ch, sp, FL_ALL);
-                cplx FLr = s->FL.eval(m, crossing(m, w, ch), INF, P, ch, sp, FL_ALL);
                 size_t k = s->pidx(m, w, P);
-                val += Fl * ((Pi[k] - Pi0[k]) * F0r + Pi[k] * FLr);
+                if (is_mfrg) {
+                    cplx F0l = s->F0v.eval(m, INF, w, 0, mch, msp, FL_ALL);
+                    cplx FLr = s->FL.eval(m, crossing(m, w, ch), INF, P, mch, msp, FL_ALL);
+                    val += F0l * Pi0[k] * FLr;
+                } else {
+                    cplx Fl = s->F.eval(m, INF, w, P, ch, sp, FL_ALL);
+                    cplx F0r = s->F0v.eval(m, crossing(m, w, ch), INF, 0, ch, sp, FL_ALL);
+                    cplx FLr = s->FL.eval(m, crossing(m, w, ch), INF, P, ch, sp, FL_ALL);
+                    val += Fl * ((Pi[k] - Pi0[k]) * F0r + Pi[k] * FLr);
+                }
             }
             return s->T * val * (double)sg;
         });
@@ -881,14 +960,50 @@ void orc_bse_k1(orc_solver* s, int32_t ch) {
     if (ch == TCH) t_post(s->Fbuff.g[TCH].K1, s->Fbuff.g[ACH].K1);
 }
 
-// BSE_K2! (src/BSEa/BSEa_K2.jl:43-103; NL src/nonlocal/BSEa/BSEa_K2.jl:45-104), is_mfRG = false
-void orc_bse_k2(orc_solver* s, int32_t ch) {
+// BSE_K1_new! (src/BSEa/BSEa_K1.jl:57-100; NL src/nonlocal/BSEa/BSEa_K1.jl:62-108):
+// K1 = T s Σ_ω (F(Ω,νInf,ω) Π - F0(Ω,νInf,ω) Π0) U with U = bare_vertex(F, Sp); the lattice is_mfRG form
+// is (F - F0) Π U, the local form has no is_mfRG branch
+void orc_bse_k1_new(orc_solver* s, int32_t ch, int32_t is_mfrg) {
+    s->F.refresh();
+    Chan& out = s->Fbuff.g[ch];
+    const std::vector<cplx>& Pi0 = (ch == PCH) ? s->Pi0pp : s->Pi0ph;
+    const std::vector<cplx>& Pi = (ch == PCH) ? s->Pipp : s->Piph;
+    const int sp = kSp[ch], sg = kSign[ch];
+    const bool mf = is_mfrg && s->prm.nonlocal;
+    const cplx U = sp == XSP ? -s->F.bare() : s->F.bare();
+    const size_t n1 = out.n1();
+    {
+        std::vector<cplx> tmp(n1 * s->NP);
+        sg_eval(sg_for(s, ch, 1, false), tmp, tmp.size(), [&](size_t ii) {
+            const size_t i = ii % n1;
+            const int P = (int)(ii / n1);
+            int m = out.b1.val((int)i);
+            cplx val = 0.0;
+            for (int iw = 0; iw < s->mPiF.size(); ++iw) {
+                int w = s->mPiF.val(iw);
+                size_t k = s->pidx(m, w, P);
+                cplx Fl = s->F.eval(m, INF, w, P, ch, sp, FL_ALL);
+                cplx F0l = s->F0v.eval(m, INF, w, 0, ch, sp, FL_ALL);
+                val += mf ? (Fl - F0l) * Pi[k] * U : (Fl * Pi[k] - F0l * Pi0[k]) * U;
+            }
+            return s->T * val * (double)sg;
+        });
+        std::copy(tmp.begin(), tmp.end(), out.K1.begin());
+    }
+    if (ch == TCH) t_post(s->Fbuff.g[TCH].K1, s->Fbuff.g[ACH].K1);
+}
+
+// BSE_K2! (src/BSEa/BSEa_K2.jl:43-103; NL src/nonlocal/BSEa/BSEa_K2.jl:45-104).  is_mfRG = true: the central
+// part (F0(Ω,ν,.) - F0(Ω,νInf,.)) Π0 FL(Ω, ., νInf) with the crossing on the right vertex (local,
+// src/BSEa/BSEa_K2.jl:70-78) or on the left one (lattice, src/nonlocal/BSEa/BSEa_K2.jl:70-78)
+void orc_bse_k2(orc_solver* s, int32_t ch, int32_t is_mfrg) {
     s->F.refresh();
     s->FL.refresh();
     Chan& out = s->Fbuff.g[ch];
     const std::vector<cplx>& Pi0 = (ch == PCH) ? s->Pi0pp : s->Pi0ph;
     const std::vector<cplx>& Pi = (ch == PCH) ? s->Pipp : s->Piph;
     const int sp = kSp[ch], sg = kSign[ch];
+    const bool nl = s->prm.nonlocal != 0;
     const size_t n2 = out.n2();
     {
         std::vector<cplx> tmp(n2 * s->NP);
@@ -899,11 +1014,18 @@ void orc_bse_k2(orc_solver* s, int32_t ch) {
             cplx val = 0.0;
             for (int iw = 0; iw < s->mPiF.size(); ++iw) {
                 int w = s->mPiF.val(iw);
-                cplx Fl = s->F.eval(m, v, w, P, ch, sp, FL_ALL) - s->F.eval(m, INF, w, P, ch, sp, FL_ALL);
-                cplx F0r = s->F0v.eval(m, crossing(m, w, ch), INF, 0, ch, sp, FL_ALL);
-                cplx FLr = s->FL.eval(m, crossing(m, w, ch), INF, P, ch, sp, FL_ALL);
                 size_t k = s->pidx(m, w, P);
-                val += Fl * ((Pi[k] - Pi0[k]) * F0r + Pi[k] * FLr);
+                if (is_mfrg) {
+                    const int wl = nl ? crossing(m, w, ch) : w, wr = nl ? w : crossing(m, w, ch);
+                    cplx F0l = s->F0v.eval(m, v, wl, 0, ch, sp, FL_ALL) - s->F0v.eval(m, INF, wl, 0, ch, sp, FL_ALL);
+                    cplx FLr = s->FL.eval(m, wr, INF, P, ch, sp, FL_ALL);
+                    val += F0l * Pi0[k] * FLr;
+                } else {
+                    cplx Fl = s->F.eval(m, v, w, P, ch, sp, FL_ALL) - s->F.eval(m, INF, w, P, ch, sp, FL_ALL);
+                    cplx F0r = s->F0v.eval(m, crossing(m, w, ch), INF, 0, ch, sp, FL_ALL);
+                    cplx FLr = s->FL.eval(m, crossing(m, w, ch), INF, P, ch, sp, FL_ALL);
+                    val += Fl * ((Pi[k] - Pi0[k]) * F0r + Pi[k] * FLr);
+                }
             }
             return s->T * val * (double)sg;
         });
@@ -919,8 +1041,41 @@ void orc_bse_k2(orc_solver* s, int32_t ch) {
     }
 }
 
-// BSE_K3! (src/BSEa/BSEa_K3.jl:43-128; NL src/nonlocal/BSEa/BSEa_K3.jl:44-128), is_mfRG = false
-void orc_bse_k3(orc_solver* s, int32_t ch) {
+// BSE_K2_new! (src/BSEa/BSEa_K2.jl:108-151; NL src/nonlocal/BSEa/BSEa_K2.jl:108-155):
+// K2 = T s Σ_ω ((F(ν) - F(νInf)) Π - (F0(ν) - F0(νInf)) Π0) U; lattice is_mfRG: (ΔF - ΔF0) Π U; no FL term
+void orc_bse_k2_new(orc_solver* s, int32_t ch, int32_t is_mfrg) {
+    s->F.refresh();
+    Chan& out = s->Fbuff.g[ch];
+    const std::vector<cplx>& Pi0 = (ch == PCH) ? s->Pi0pp : s->Pi0ph;
+    const std::vector<cplx>& Pi = (ch == PCH) ? s->Pipp : s->Piph;
+    const int sp = kSp[ch], sg = kSign[ch];
+    const bool mf = is_mfrg && s->prm.nonlocal;
+    const cplx U = sp == XSP ? -s->F.bare() : s->F.bare();
+    const size_t n2 = out.n2();
+    {
+        std::vector<cplx> tmp(n2 * s->NP);
+        sg_eval(sg_for(s, ch, 2, false), tmp, tmp.size(), [&](size_t ii) {
+            const size_t i = ii % n2;
+            const int P = (int)(ii / n2);
+            int m = out.b2.val((int)(i % out.b2.size())), v = out.f2.val((int)(i / out.b2.size()));
+            cplx val = 0.0;
+            for (int iw = 0; iw < s->mPiF.size(); ++iw) {
+                int w = s->mPiF.val(iw);
+                size_t k = s->pidx(m, w, P);
+                cplx Fl = s->F.eval(m, v, w, P, ch, sp, FL_ALL) - s->F.eval(m, INF, w, P, ch, sp, FL_ALL);
+                cplx F0l = s->F0v.eval(m, v, w, 0, ch, sp, FL_ALL) - s->F0v.eval(m, INF, w, 0, ch, sp, FL_ALL);
+                val += mf ? (Fl - F0l) * Pi[k] * U : (Fl * Pi[k] - F0l * Pi0[k]) * U;
+            }
+            return s->T * val * (double)sg;
+        });
+        std::copy(tmp.begin(), tmp.end(), out.K2.begin());
+    }
+    if (ch == TCH) t_post(s->Fbuff.g[TCH].K2, s->Fbuff.g[ACH].K2);
+}
+
+// BSE_K3! (src/BSEa/BSEa_K3.jl:43-128; NL src/nonlocal/BSEa/BSEa_K3.jl:44-128).  is_mfRG = true
+// (src/BSEa/BSEa_K3.jl:83-97): A Π0 (s1 Γ~ + s2 L~), no (Π - Π0) F0 term
+void orc_bse_k3(orc_solver* s, int32_t ch, int32_t is_mfrg) {
     Chan& out = s->Fbuff.g[ch];
     const std::vector<cplx>& Pi0 = (ch == PCH) ? s->Pi0pp : s->Pi0ph;
     const std::vector<cplx>& Pi = (ch == PCH) ? s->Pipp : s->Piph;
@@ -951,13 +1106,14 @@ void orc_bse_k3(orc_solver* s, int32_t ch) {
                 cplx Gs = (ch == PCH) ? c3(CG, i1, iw, i3, P) : c3(CG, i1, i3, iw, P);
                 cplx F0s = c3(CF0, i1, iw, i3, P);
                 size_t k = s->pidx(m, w, P);
-                val += A * ((Pi[k] - Pi0[k]) * F0s + Pi[k] * Gs) * (double)s1;
+                if (is_mfrg) val += A * Pi0[k] * Gs * (double)s1;
+                else val += A * ((Pi[k] - Pi0[k]) * F0s + Pi[k] * Gs) * (double)s1;
                 int w_ = crossing(m, w, ch);
                 if (out.f3.in(w_)) {
                     cplx Lx;
                     if (ch == TCH) Lx = 2.0 * Lt.K3[Lt.i3(m, w_, vp) + Lt.n3() * P] - La.K3[La.i3(m, w_, vp) + La.n3() * P];
                     else Lx = Lr.K3[Lr.i3(m, w_, vp) + Lr.n3() * P];
-                    val += A * Pi[k] * Lx * (double)s2;
+                    val += A * (is_mfrg ? Pi0[k] : Pi[k]) * Lx * (double)s2;
                 }
             }
             cplx self;
@@ -1071,6 +1227,7 @@ std::vector<cplx> sde_u2(orc_solver* s, const std::vector<cplx>& G, cplx U) {
     for (auto& x : Gp) x /= (double)L * L;
     for (auto& x : Gm) x /= (double)L * L;
     std::vector<cplx> SR(G.size(), 0.0);
+#pragma omp parallel for
     for (int R = 0; R < L * L; ++R)
         for (int i2 = 0; i2 < NG; ++i2)
             for (int i1 = 0; i1 < NG; ++i1) {
@@ -1135,6 +1292,7 @@ std::vector<cplx> sde_compute(orc_solver* s, const SdeIn& in, const Vert& Fv, bo
                 int imRG = imod(-R1, LG) + LG * imod(-R2, LG);
                 int ipRS = imod(R1, LG) + LG * imod(R2, LG);
                 int imRS = imRG;
+#pragma omp parallel for
                 for (int iv = 0; iv < nf; ++iv) {
                     int v = shape.f2.val(iv);
                     if (!s->mG.in(v)) continue;
@@ -1184,7 +1342,7 @@ void orc_sde(orc_solver* s, int32_t strategy, int32_t include_hartree) {
     std::fill(s->Sig.begin(), s->Sig.end(), 0.0);
     SdeIn in{&s->G, &s->Pipp, &s->Piph, &s->Lpp, &s->Lph, &s->SGpp[2], &s->SGph[2]};
     sde_rec(s, s->Sig, in, s->F, true, include_hartree != 0);
-    if (strategy == ORC_FDPA) {
+    if (strategy == ORC_FDPA || strategy == ORC_FDPA_NEW) {
         std::vector<cplx> S0(s->Sig.size(), 0.0);
         SdeIn in0{&s->G0, &s->Pi0pp, &s->Pi0ph, &s->L0pp, &s->L0ph, &s->SG0pp2, &s->SG0ph2};
         sde_rec(s, S0, in0, s->F0v, true, include_hartree != 0);
@@ -1198,22 +1356,32 @@ void orc_sde(orc_solver* s, int32_t strategy, int32_t include_hartree) {
     }
 }
 
-// iterate_solver! (src/solve.jl:4-116), strategies :scPA and :fdPA
-void orc_iterate(orc_solver* s, int32_t strategy, int32_t update_sigma) {
+// iterate_solver!(S; strategy, update_Σ, compute_Hartree) (src/solve.jl:4-116), strategies :scPA, :fdPA,
+// :scPA_new and :fdPA_new (K3 first, then the alternative K1/K2 forms, src/solve.jl:26-46)
+void orc_iterate(orc_solver* s, int32_t strategy, int32_t update_sigma, int32_t compute_hartree) {
     if (update_sigma) {
         orc_dyson(s);
         orc_bubbles(s);
     }
     orc_build_k3_cache(s);
-    if (strategy == ORC_FDPA) {
-        for (int ch : {PCH, ACH, TCH}) orc_bse_l_k2(s, ch);
-        for (int ch : {PCH, ACH, TCH}) orc_bse_l_k3(s, ch);
+    if (strategy == ORC_SCPA_NEW || strategy == ORC_FDPA_NEW) {
+        if (strategy == ORC_FDPA_NEW)
+            for (int ch : {PCH, ACH, TCH}) orc_bse_l_k3(s, ch);
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k3(s, ch, 0);
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k1_new(s, ch, 0);
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k2_new(s, ch, 0);
+    } else {
+        if (strategy == ORC_FDPA) {
+            for (int ch : {PCH, ACH, TCH}) orc_bse_l_k2(s, ch);
+            for (int ch : {PCH, ACH, TCH}) orc_bse_l_k3(s, ch);
+        }
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k1(s, ch, 0);
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k2(s, ch, 0);
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k3(s, ch, 0);
     }
-    for (int ch : {PCH, ACH, TCH}) orc_bse_k1(s, ch);
-    for (int ch : {PCH, ACH, TCH}) orc_bse_k2(s, ch);
-    for (int ch : {PCH, ACH, TCH}) orc_bse_k3(s, ch);
     orc_update_F(s);
-    if (update_sigma) orc_sde(s, strategy, 1);
+    // SDE!(S; strategy): the _new strategies take the Σ update of their base strategy (src/SDE.jl:4-28)
+    if (update_sigma) orc_sde(s, (strategy == ORC_FDPA || strategy == ORC_FDPA_NEW) ? ORC_FDPA : ORC_SCPA, compute_hartree);
 }
 
 // flatten layout: [γp(K1,K2,K3); γt(...); γa(...); Σ] (src/ParquetSolver.jl:277-306, src/vertex.jl:153-196)
@@ -1240,29 +1408,83 @@ void orc_unflatten(orc_solver* s, const double* x) {
     std::copy(in, in + s->Sig.size(), s->Sig.begin());
 }
 
-// fixed_point! (src/solve.jl:119-157), update_Σ = true
-void orc_fixed_point(orc_solver* s, double* R, const double* x, int32_t strategy) {
-    orc_unflatten(s, x);
-    orc_iterate(s, strategy, 1);
-    orc_flatten(s, R);
-    int64_t n = orc_flat_size(s) * 2;
+// fixed_point! (src/solve.jl:119-157).  update_Σ = false: x and R hold the vertex only, R = flatten(S.F) - x
+// (the vertex-only vector of src/mfRG.jl:111-173: unflatten!(S.F, x) asserts length(x) == length(S.F),
+// src/vertex.jl:195)
+int64_t orc_flat_size_F(const orc_solver* s) { return orc_flat_size(s) - (int64_t)s->Sig.size(); }
+
+void orc_fixed_point(orc_solver* s, double* R, const double* x, int32_t strategy, int32_t update_sigma,
+                     int32_t compute_hartree) {
+    if (update_sigma) {
+        orc_unflatten(s, x);
+    } else {
+        std::vector<cplx> keep = s->Sig;
+        std::vector<double> full((size_t)orc_flat_size(s) * 2);
+        std::memcpy(full.data(), x, (size_t)orc_flat_size_F(s) * 2 * sizeof(double));
+        std::memcpy(full.data() + (size_t)orc_flat_size_F(s) * 2, reinterpret_cast<const double*>(keep.data()),
+                    keep.size() * 2 * sizeof(double));
+        orc_unflatten(s, full.data());
+    }
+    orc_iterate(s, strategy, update_sigma, compute_hartree);
+    int64_t n = (update_sigma ? orc_flat_size(s) : orc_flat_size_F(s)) * 2;
+    if (update_sigma) {
+        orc_flatten(s, R);
+    } else {
+        std::vector<double> full((size_t)orc_flat_size(s) * 2);
+        orc_flatten(s, full.data());
+        std::memcpy(R, full.data(), (size_t)n * sizeof(double));
+    }
     for (int64_t i = 0; i < n; ++i) R[i] -= x[i];
+}
+
+// mfRGLinearMap * x (src/mfRG.jl:34-89): y = x - G_lin(x / 100) * 100, where G_lin is one pass of the
+// linearised BSE around the reference vertex: F = unflatten(x / 100), build_K3_cache_mfRG!, for :fdPA
+// BSE_L_K2! and the is_mfRG forms of BSE_K1!/BSE_K2!, for :fdPA_new the is_mfRG forms of BSE_K1_new!/
+// BSE_K2_new!, then BSE_L_K3! and the is_mfRG BSE_K3!, F = Fbuff.  x, y: the vertex only.
+void orc_mfrg_apply(orc_solver* s, double* y, const double* x, int32_t strategy, int32_t is_first) {
+    const double factor = 1e-2;
+    const int64_t nF = orc_flat_size_F(s);
+    {
+        std::vector<double> full((size_t)orc_flat_size(s) * 2);
+        for (int64_t i = 0; i < 2 * nF; ++i) full[i] = x[i] * factor;
+        std::memcpy(full.data() + 2 * nF, reinterpret_cast<const double*>(s->Sig.data()), s->Sig.size() * 2 * sizeof(double));
+        orc_unflatten(s, full.data());
+    }
+    orc_build_k3_cache_mfrg(s, is_first);
+    if (strategy == ORC_FDPA) {
+        for (int ch : {PCH, ACH, TCH}) orc_bse_l_k2(s, ch);
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k1(s, ch, 1);
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k2(s, ch, 1);
+    } else {
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k1_new(s, ch, 1);
+        for (int ch : {PCH, ACH, TCH}) orc_bse_k2_new(s, ch, 1);
+    }
+    for (int ch : {PCH, ACH, TCH}) orc_bse_l_k3(s, ch);
+    for (int ch : {PCH, ACH, TCH}) orc_bse_k3(s, ch, 1);
+    orc_update_F(s);
+    std::vector<double> full((size_t)orc_flat_size(s) * 2);
+    orc_flatten(s, full.data());
+    for (int64_t i = 0; i < 2 * nF; ++i) y[i] = x[i] - full[i] / factor;
 }
 
 // NLsolve 4.5.1 Anderson (method=:anderson, aa_start=1, droptol=0, xtol=0):
 //   g = x + β f(x); converged when max|f| < ftol; history of Δg and Δf of size m;
 //   x_{k+1} = g_k - Σ_i γ_i Δg_i with γ = argmin ||f_k - ΔF γ|| (QR least squares).
 int orc_solve(orc_solver* s, int32_t strategy, int32_t maxiter, double tol, double beta, int32_t mem,
-              double* final_res) {
-    const int64_t n = orc_flat_size(s);
+              int32_t update_sigma, double* final_res) {
+    const int64_t n = update_sigma ? orc_flat_size(s) : orc_flat_size_F(s);
     std::vector<cplx> x(n), fx(n), g(n), gold(n), fold(n);
-    orc_flatten(s, reinterpret_cast<double*>(x.data()));
+    {
+        std::vector<cplx> full(orc_flat_size(s));
+        orc_flatten(s, reinterpret_cast<double*>(full.data()));
+        std::copy(full.begin(), full.begin() + n, x.begin());
+    }
     const int mmax = (int)std::min<int64_t>(mem, n);
     std::vector<std::vector<cplx>> dG, dF;  // oldest first
     int it = 0;
     double res = INFINITY;
     for (it = 1; it <= maxiter; ++it) {
-        orc_fixed_point(s, reinterpret_cast<double*>(fx.data()), reinterpret_cast<const double*>(x.data()), strategy);
+        orc_fixed_point(s, reinterpret_cast<double*>(fx.data()), reinterpret_cast<const double*>(x.data()), strategy, update_sigma, 1);
         for (int64_t i = 0; i < n; ++i) g[i] = x[i] + beta * fx[i];
         res = 0.0;
         for (auto& v : fx) res = std::max(res, std::abs(v));
@@ -1324,6 +1546,9 @@ void orc_vertex_eval(const orc_solver* s, int32_t vtx, int32_t m, int32_t n, int
     out[0] = r.real();
     out[1] = r.imag();
 }
+
+void orc_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+int32_t orc_max_threads(void) { return omp_get_max_threads(); }
 
 double orc_occupation(const orc_solver* s, int32_t which) {
     return occupation(s, which == ORC_G0 ? s->G0 : which == ORC_GBARE ? s->Gbare : s->G);

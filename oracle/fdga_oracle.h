@@ -48,7 +48,7 @@ enum {
 /* channel / spin / strategy codes */
 enum { ORC_PCH = 0, ORC_TCH = 1, ORC_ACH = 2 };
 enum { ORC_PSP = 0, ORC_XSP = 1, ORC_DSP = 2 };
-enum { ORC_SCPA = 0, ORC_FDPA = 1 };
+enum { ORC_SCPA = 0, ORC_FDPA = 1, ORC_SCPA_NEW = 2, ORC_FDPA_NEW = 3 };
 #define ORC_INF (-2147483647 - 1)   /* νInf */
 
 typedef struct orc_solver orc_solver;
@@ -80,26 +80,39 @@ void orc_bubbles(orc_solver* s);
 void orc_build_k3_cache(orc_solver* s);
 void orc_bse_l_k2(orc_solver* s, int32_t ch);
 void orc_bse_l_k3(orc_solver* s, int32_t ch);
-void orc_bse_k1(orc_solver* s, int32_t ch);
-void orc_bse_k2(orc_solver* s, int32_t ch);
-void orc_bse_k3(orc_solver* s, int32_t ch);
+/* is_mfrg: the is_mfRG = Val(true) forms */
+void orc_bse_k1(orc_solver* s, int32_t ch, int32_t is_mfrg);
+void orc_bse_k2(orc_solver* s, int32_t ch, int32_t is_mfrg);
+void orc_bse_k3(orc_solver* s, int32_t ch, int32_t is_mfrg);
+void orc_bse_k1_new(orc_solver* s, int32_t ch, int32_t is_mfrg);
+void orc_bse_k2_new(orc_solver* s, int32_t ch, int32_t is_mfrg);
 void orc_update_F(orc_solver* s);
 void orc_sde(orc_solver* s, int32_t strategy, int32_t include_hartree);
-void orc_iterate(orc_solver* s, int32_t strategy, int32_t update_sigma);
+void orc_iterate(orc_solver* s, int32_t strategy, int32_t update_sigma, int32_t compute_hartree);
+/* build_K3_cache_mfRG!(S, is_first_iteration) and mfRGLinearMap * x (vertex-only vectors) */
+void orc_build_k3_cache_mfrg(orc_solver* s, int32_t is_first);
+void orc_mfrg_apply(orc_solver* s, double* y, const double* x, int32_t strategy, int32_t is_first);
 
 /* fixed-point layer (src/solve.jl:119-196) */
 int64_t orc_flat_size(const orc_solver* s);
+int64_t orc_flat_size_F(const orc_solver* s);   /* vertex part only */
 void    orc_flatten(const orc_solver* s, double* x);
 void    orc_unflatten(orc_solver* s, const double* x);
-void    orc_fixed_point(orc_solver* s, double* R, const double* x, int32_t strategy);
+/* update_sigma = 0: x and R are vertex-only vectors (orc_flat_size_F) */
+void    orc_fixed_point(orc_solver* s, double* R, const double* x, int32_t strategy, int32_t update_sigma,
+                        int32_t compute_hartree);
 /* Anderson acceleration as NLsolve 4.5.1 method=:anderson; returns iterations, writes final max|f| */
 int     orc_solve(orc_solver* s, int32_t strategy, int32_t maxiter, double tol, double beta,
-                  int32_t mem, double* final_res);
+                  int32_t mem, int32_t update_sigma, double* final_res);
 
 /* point evaluator of a vertex: vtx 0 F, 1 Fbuff, 2 FL, 3 F0; flags bit0 F0, bit1 γp, bit2 γt, bit3 γa */
 void orc_vertex_eval(const orc_solver* s, int32_t vtx, int32_t m, int32_t n, int32_t np, int32_t iP,
                      int32_t ch, int32_t sp, int32_t flags, double* out);
 double orc_occupation(const orc_solver* s, int32_t which);
+/* OpenMP threads of the parallel loops (the CPU baseline's "all cores" leg); default 1 thread is not
+ * forced: the OpenMP runtime's default applies until this is called */
+void    orc_set_threads(int32_t n);
+int32_t orc_max_threads(void);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,7 @@ _LIB_PATH = os.path.join(_HERE, "build", "libfdga_oracle.so")
 
 PCH, TCH, ACH = 0, 1, 2
 PSP, XSP, DSP = 0, 1, 2
-SCPA, FDPA = 0, 1
+SCPA, FDPA, SCPA_NEW, FDPA_NEW = 0, 1, 2, 3
 INF = -(2 ** 31)
 
 GBARE, G0, SIGMA0, G, SIGMA = 0, 1, 2, 3, 4
@@ -77,17 +77,24 @@ def lib():
             "orc_build_k3_cache": (None, [vp]),
             "orc_bse_l_k2": (None, [vp, i32]),
             "orc_bse_l_k3": (None, [vp, i32]),
-            "orc_bse_k1": (None, [vp, i32]),
-            "orc_bse_k2": (None, [vp, i32]),
-            "orc_bse_k3": (None, [vp, i32]),
+            "orc_bse_k1": (None, [vp, i32, i32]),
+            "orc_bse_k2": (None, [vp, i32, i32]),
+            "orc_bse_k3": (None, [vp, i32, i32]),
+            "orc_bse_k1_new": (None, [vp, i32, i32]),
+            "orc_bse_k2_new": (None, [vp, i32, i32]),
             "orc_update_F": (None, [vp]),
             "orc_sde": (None, [vp, i32, i32]),
-            "orc_iterate": (None, [vp, i32, i32]),
+            "orc_iterate": (None, [vp, i32, i32, i32]),
+            "orc_build_k3_cache_mfrg": (None, [vp, i32]),
+            "orc_mfrg_apply": (None, [vp, dp, dp, i32, i32]),
             "orc_flat_size": (i64, [vp]),
+            "orc_flat_size_F": (i64, [vp]),
             "orc_flatten": (None, [vp, dp]),
             "orc_unflatten": (None, [vp, dp]),
-            "orc_fixed_point": (None, [vp, dp, dp, i32]),
-            "orc_solve": (ctypes.c_int, [vp, i32, i32, ctypes.c_double, ctypes.c_double, i32, dp]),
+            "orc_fixed_point": (None, [vp, dp, dp, i32, i32, i32]),
+            "orc_solve": (ctypes.c_int, [vp, i32, i32, ctypes.c_double, ctypes.c_double, i32, i32, dp]),
+            "orc_set_threads": (None, [i32]),
+            "orc_max_threads": (i32, []),
             "orc_vertex_eval": (None, [vp, i32, i32, i32, i32, i32, i32, i32, i32, dp]),
             "orc_occupation": (ctypes.c_double, [vp, i32]),
         }
@@ -182,12 +189,27 @@ class Oracle:
     def build_k3_cache(self): self.lib.orc_build_k3_cache(self.h)
     def bse_l_k2(self, ch): self.lib.orc_bse_l_k2(self.h, ch)
     def bse_l_k3(self, ch): self.lib.orc_bse_l_k3(self.h, ch)
-    def bse_k1(self, ch): self.lib.orc_bse_k1(self.h, ch)
-    def bse_k2(self, ch): self.lib.orc_bse_k2(self.h, ch)
-    def bse_k3(self, ch): self.lib.orc_bse_k3(self.h, ch)
+    def bse_k1(self, ch, is_mfrg=False): self.lib.orc_bse_k1(self.h, ch, int(is_mfrg))
+    def bse_k2(self, ch, is_mfrg=False): self.lib.orc_bse_k2(self.h, ch, int(is_mfrg))
+    def bse_k3(self, ch, is_mfrg=False): self.lib.orc_bse_k3(self.h, ch, int(is_mfrg))
+    def bse_k1_new(self, ch, is_mfrg=False): self.lib.orc_bse_k1_new(self.h, ch, int(is_mfrg))
+    def bse_k2_new(self, ch, is_mfrg=False): self.lib.orc_bse_k2_new(self.h, ch, int(is_mfrg))
+    def build_k3_cache_mfrg(self, is_first=True): self.lib.orc_build_k3_cache_mfrg(self.h, int(is_first))
     def update_F(self): self.lib.orc_update_F(self.h)
     def sde(self, strategy, include_hartree=True): self.lib.orc_sde(self.h, strategy, int(include_hartree))
-    def iterate(self, strategy, update_sigma=True): self.lib.orc_iterate(self.h, strategy, int(update_sigma))
+
+    def iterate(self, strategy, update_sigma=True, compute_hartree=True):
+        self.lib.orc_iterate(self.h, strategy, int(update_sigma), int(compute_hartree))
+
+    def mfrg_apply(self, x, strategy=FDPA, is_first=True):
+        """mfRGLinearMap(S, strategy) * x on the vertex-only vector x."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.empty_like(x)
+        self.lib.orc_mfrg_apply(self.h, _dp(y.view(np.float64)), _dp(x.view(np.float64)), strategy, int(is_first))
+        return y
+
+    def flat_size_F(self):
+        return self.lib.orc_flat_size_F(self.h)
 
     def flatten(self):
         n = self.lib.orc_flat_size(self.h)
@@ -199,15 +221,16 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.complex128)
         self.lib.orc_unflatten(self.h, _dp(x.view(np.float64)))
 
-    def fixed_point(self, x, strategy):
+    def fixed_point(self, x, strategy, update_sigma=True, compute_hartree=True):
         x = np.ascontiguousarray(x, dtype=np.complex128)
         R = np.empty_like(x)
-        self.lib.orc_fixed_point(self.h, _dp(R.view(np.float64)), _dp(x.view(np.float64)), strategy)
+        self.lib.orc_fixed_point(self.h, _dp(R.view(np.float64)), _dp(x.view(np.float64)), strategy,
+                                 int(update_sigma), int(compute_hartree))
         return R
 
-    def solve(self, strategy, maxiter=100, tol=1e-4, beta=0.85, mem=8):
+    def solve(self, strategy, maxiter=100, tol=1e-4, beta=0.85, mem=8, update_sigma=True):
         res = ctypes.c_double(0.0)
-        it = self.lib.orc_solve(self.h, strategy, maxiter, tol, beta, mem, ctypes.byref(res))
+        it = self.lib.orc_solve(self.h, strategy, maxiter, tol, beta, mem, int(update_sigma), ctypes.byref(res))
         return it, res.value
 
     def eval(self, vtx, m, n, np_, iP, ch, sp, flags=15):
@@ -217,3 +240,12 @@ class Oracle:
 
     def occupation(self, which=G):
         return self.lib.orc_occupation(self.h, which)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (results do not depend on the count)."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return lib().orc_max_threads()
