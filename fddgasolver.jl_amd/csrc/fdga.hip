@@ -15,6 +15,7 @@
 
 #include "../../include/fdga.h"
 #include "fdga_kernels.h"
+#include "fdga_forms.h"
 #include "fdga_sym.h"
 
 using namespace fdga;
@@ -73,6 +74,9 @@ struct fdga_ctx {
     double2 U{};
     DevChan F[3], Fb[3], FL[3], R0[3];
     double2 *cache[12] = {};   // build_K3_cache! caches; [10], [11] the a and t Γ caches transposed
+    // the F caches [7..9] hold build_K3_cache_mfRG!'s complete, momentum-independent F0 differences (the plain
+    // lattice build stores only their momentum-independent part, the BSE_K3! loader adds K2r + K3r)
+    int caches_mfrg = 0;
     double2 *R[3] = {nullptr, nullptr, nullptr};                    // BSE_K3! right operands [ω, νp, Ω, P]
     double2 *csum[3] = {nullptr, nullptr, nullptr};                 // their column sums [νp, Ω, P]
     double2 *tmpG1 = nullptr, *tmpG2 = nullptr, *tmpG3 = nullptr;   // [NG, NkG] scratch
@@ -99,7 +103,7 @@ struct fdga_ctx {
     // state (set!(S.F, S.Fbuff) swaps the two buffers every iteration)
     int graph_on = 0;
     int swap_state = 0;
-    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraphExec_t gexec[4][2] = {};   // strategies :scPA, :fdPA, :scPA_new, :fdPA_new
     // timing
     int timing = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -915,7 +919,10 @@ struct K3Set {
     K3_Aop32 a32() const { return K3_Aop32{g, kc[0], kc[1], kc[2]}; }
     K3_Bop32 b32() const { return K3_Bop32{g, kc[0], kc[1], kc[2]}; }
 };
-static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) {
+// mfrg: the is_mfRG = Val(true) form (src/BSEa/BSEa_K3.jl:83-97): R = Π0 (s1 Γ~ + s2 L~) (the right-operand
+// builder with Π0 in place of Π), A = the F caches of build_K3_cache_mfRG! as they are (no K2/K3 of the
+// target vertex added)
+static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, bool mfrg = false) {
     K3Set ks{};
     ks.g = k3geom(c);
     ks.T = c->d.T;
@@ -926,7 +933,7 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) 
         rc[s].ch = ch;
         rc[s].s1 = (ch == ACH) ? 1.0 : -1.0;   // src/BSE_templates.jl:156,164,172
         rc[s].s2 = (ch == TCH) ? -1.0 : 1.0;
-        rc[s].Pi = pi_of(c, ch, false);
+        rc[s].Pi = pi_of(c, ch, mfrg);
         rc[s].Pi0 = pi_of(c, ch, true);
         rc[s].F0r = c->cache[idx_f0(ch)];
         rc[s].Gam = c->cache[(ch == PCH) ? 0 : (ch == ACH) ? 10 : 11];
@@ -936,7 +943,7 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) 
         rc[s].csum = c->csum[ch];
         kc[s].ch = ch;
         kc[s].Fbase = c->cache[idx_fbase(ch)];
-        kc[s].full = (c->NP == 1);
+        kc[s].full = (c->NP == 1) || c->caches_mfrg;
         kc[s].K3r = c->F[ch].K3;
         kc[s].K3a = c->F[ACH].K3;
         kc[s].K2r = c->F[ch].K2;
@@ -954,7 +961,7 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) 
             float2* buf = reinterpret_cast<float2*>(c->R[ch]);
             rc[s].R32 = buf;
             kc[s].R32 = buf;
-            if (c->NP > 1) kc[s].A32 = buf + c->n3 * c->NP;
+            if (!kc[s].full) kc[s].A32 = buf + c->n3 * c->NP;
         }
     }
     return ks;
@@ -987,29 +994,93 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n, bool sym = false) {
 // the BSE_K3! GEMM launch of `n` channels with the loaders of the solver kind and precision
 static hipError_t k3_gemm_set(fdga_ctx* c, const int* chans, int n, const K3Set& ks) {
     const K3_Ep e = ks.e();
-    if (c->NP == 1)
+    if (ks.kc[0].full)
         return ks.r32() ? k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b32(), e) : k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b(), e);
     return ks.op32() ? k3_gemm(c, chans, n, 1, ks.a32(), ks.b32(), e) : k3_gemm(c, chans, n, 1, ks.a<false>(), ks.b(), e);
 }
 // BSE_K3! for `n` channels: right operands and column sums, then the GEMM launch(es)
-static int launch_k3(fdga_ctx* c, const int* chans, int n) {
+static int launch_k3(fdga_ctx* c, const int* chans, int n, bool mfrg = false) {
+    if (mfrg && c->NP > 1 && !c->caches_mfrg)
+        return fail(c, "BSE_K3!(S, Ch, Val(true)) of an NL_ParquetSolver reads the F caches of build_K3_cache_mfRG!(S, true); "
+                       "the device keeps only the momentum-independent part of build_K3_cache!'s");
     const bool sym = k3_sym(c, chans, n, false);
     if (split_t(c, chans, n, sym)) {
-        if (k3_rhs(c, k3_set(c, chans, n, false, false), n)) return -3;
+        if (k3_rhs(c, k3_set(c, chans, n, false, false, mfrg), n)) return -3;
         kernel_begin(c);
-        if (k3_gemm_set(c, kChansPA, 2, k3_set(c, kChansPA, 2, false, false)) != hipSuccess ||
-            k3_gemm_set(c, kChansT, 1, k3_set(c, kChansT, 1, false, true)) != hipSuccess)
+        if (k3_gemm_set(c, kChansPA, 2, k3_set(c, kChansPA, 2, false, false, mfrg)) != hipSuccess ||
+            k3_gemm_set(c, kChansT, 1, k3_set(c, kChansT, 1, false, true, mfrg)) != hipSuccess)
             return fail(c, "BSE_K3 GEMM launch failed", -3);
         kernel_end(c, 10);
         return 0;
     }
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
-    const K3Set ks = k3_set(c, chans, n, sym, t_epi);
+    const K3Set ks = k3_set(c, chans, n, sym, t_epi, mfrg);
     if (k3_rhs(c, ks, n, sym)) return -3;
     kernel_begin(c);
     if (k3_gemm_set(c, chans, n, ks) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
     return k3_finish(c, c->Fb, chans, n, false, sym, t_epi);
+}
+
+// ---------------------------------------------------------------------------------------
+// the is_mfRG forms of BSE_K1!/BSE_K2! and the _new forms (fdga_forms.h): one wave per output class
+template <int FORM, int CH>
+static int launch_form(fdga_ctx* c, int is_mfrg) {
+    constexpr bool K2 = (FORM == FORM_K2_MFRG || FORM == FORM_K2_NEW);
+    FormArgs a{};
+    a.F = vtx_F(c);
+    a.FL = vtx_FL(c);
+    a.f0 = f0_view(c);
+    const bool zero = (FORM == FORM_K1_MFRG || FORM == FORM_K2_MFRG);   // the mfRG forms sum against Π0 only
+    a.Pi = pi_of(c, CH, zero);
+    a.Pi0 = pi_of(c, CH, true);
+    a.Pt = pi_tail(c, a.Pi);
+    a.P0t = pi_tail(c, a.Pi0);
+    a.out = K2 ? c->Fb[CH].K2 : c->Fb[CH].K1;
+    a.outa = K2 ? c->Fb[ACH].K2 : c->Fb[ACH].K1;
+    a.Lr = c->FL[CH].K2;
+    a.La = c->FL[ACH].K2;
+    a.U = c->U;   // bare_vertex(F, Sp) for the pSp (p, a) and dSp (t) components
+    a.scale = c->d.T * kSign[CH];
+    a.nl = c->d.nonlocal;
+    a.mfrg = is_mfrg;
+    a.n1 = c->d.nK1; a.n2b = c->d.nK2b; a.n2f = c->d.nK2f;
+    a.nWP = c->d.nK1; a.nVP = c->nVP; a.wb = c->wb; a.NP = c->NP;
+    a.NK = box_bound(c);
+    a.sym = K2 ? sym_k2(c, CH) : symcls(c, SG_K1);
+    const size_t nw = nclass(a.sym, (K2 ? c->n2 : c->n1) * c->NP);
+    k_form<FORM, CH><<<cdiv(nw * WAVE, 256), 256, 0, c->run>>>(a);
+    LAUNCHCHK(c);
+    return 0;
+}
+template <int FORM>
+static int launch_form_ch(fdga_ctx* c, int ch, int is_mfrg) {
+    if (ch == PCH) return launch_form<FORM, PCH>(c, is_mfrg);
+    if (ch == TCH) return launch_form<FORM, TCH>(c, is_mfrg);
+    if (ch == ACH) return launch_form<FORM, ACH>(c, is_mfrg);
+    return fail(c, "bad channel");
+}
+
+// build_K3_cache_mfRG!(S, is_first_iteration) (src/nonlocal_2/build_K3_cache.jl:97-164): the caches
+// evaluated over the K3 groups SGpp[3] / SGph[3] (a class fill when groups are installed)
+static int build_k3_cache_mfrg(fdga_ctx* c, int first) {
+    if (first) c->caches_mfrg = 1;
+    CachePtrs cp;
+    for (int k = 0; k < 12; ++k) cp.c[k] = c->cache[k];
+    const size_t n = c->n3;
+    k_cache_mfrg<<<cdiv(n, 256), 256, 0, c->run>>>(cp, vtx_F(c), c->d.nK3b, c->d.nK3f, first);
+    LAUNCHCHK(c);
+    const int* rpp = c->sg[SG_PP3].ncls ? c->sg[SG_PP3].repop : nullptr;
+    const int* rph = c->sg[SG_PH3].ncls ? c->sg[SG_PH3].repop : nullptr;
+    const int fills[7][2] = {{0, 0}, {4, 0}, {5, 1}, {6, 1}, {7, 0}, {8, 1}, {9, 1}};
+    for (const auto& f : fills) {
+        if (f[0] >= 7 && !first) continue;
+        if (sym_fill(c, c->cache[f[0]], f[1] ? rph : rpp, n)) return -3;
+    }
+    k_cache_transpose<<<cdiv(n, 256), 256, 0, c->run>>>(c->cache[10], c->cache[11], c->cache[5], c->cache[6], c->NV3,
+                                                         c->NW3);
+    LAUNCHCHK(c);
+    return 0;
 }
 
 // =========================================================================================
@@ -1287,7 +1358,7 @@ int fdga_get(fdga_ctx* c, int32_t which, double* host) {
         else {
             int ch = a.ch == 0 ? PCH : a.ch == 1 ? ACH : TCH;
             k_cache_F_julia<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, chanF(c, c->F[ch]), chanF(c, c->F[ACH]),
-                                                               ch, c->d.nK3b, c->d.nK3f, c->NP, c->NP == 1);
+                                                               ch, c->d.nK3b, c->d.nK3f, c->NP, c->NP == 1 || c->caches_mfrg);
         }
         LAUNCHCHK(c);
         HIPCHK(c, hipMemcpyAsync(host, c->stage, bytes, hipMemcpyDeviceToHost, c->st));
@@ -1518,6 +1589,7 @@ int fdga_dyson(fdga_ctx* c) {
 int fdga_bubbles(fdga_ctx* c) { return bubbles_into(c, c->Pipp, c->Piph, c->G); }
 
 int fdga_build_k3_cache(fdga_ctx* c) {
+    c->caches_mfrg = 0;
     CachePtrs cp;
     for (int k = 0; k < 12; ++k) cp.c[k] = c->cache[k];
     const dim3 g(cdiv(c->NV3, 16), cdiv(c->NV3, 4), c->NW3);
@@ -1541,24 +1613,41 @@ int fdga_bse_l_k3(fdga_ctx* c, int32_t ch) {
     return launch_lk3(c, chans, 1);
 }
 
-int fdga_bse_k1(fdga_ctx* c, int32_t ch) {
+int fdga_bse_k1(fdga_ctx* c, int32_t ch, int32_t is_mfrg) {
+    if (is_mfrg) return launch_form_ch<FORM_K1_MFRG>(c, ch, 1);
     if (ch == PCH) return launch_k1<PCH>(c);
     if (ch == TCH) return launch_k1<TCH>(c);
     if (ch == ACH) return launch_k1<ACH>(c);
     return fail(c, "bad channel");
 }
 
-int fdga_bse_k2(fdga_ctx* c, int32_t ch) {
+int fdga_bse_k2(fdga_ctx* c, int32_t ch, int32_t is_mfrg) {
+    if (is_mfrg) return launch_form_ch<FORM_K2_MFRG>(c, ch, 1);
     if (ch == PCH) return launch_k2<PCH>(c);
     if (ch == TCH) return launch_k2<TCH>(c);
     if (ch == ACH) return launch_k2<ACH>(c);
     return fail(c, "bad channel");
 }
 
-int fdga_bse_k3(fdga_ctx* c, int32_t ch) {
+int fdga_bse_k3(fdga_ctx* c, int32_t ch, int32_t is_mfrg) {
     if (ch < 0 || ch > 2) return fail(c, "bad channel");
     const int chans[1] = {ch};
-    return launch_k3(c, chans, 1);
+    return launch_k3(c, chans, 1, is_mfrg != 0);
+}
+
+int fdga_bse_k1_new(fdga_ctx* c, int32_t ch, int32_t is_mfrg) { return launch_form_ch<FORM_K1_NEW>(c, ch, is_mfrg); }
+
+int fdga_bse_k2_new(fdga_ctx* c, int32_t ch, int32_t is_mfrg) { return launch_form_ch<FORM_K2_NEW>(c, ch, is_mfrg); }
+
+int fdga_build_k3_cache_mfrg(fdga_ctx* c, int32_t is_first) {
+    if (!c) return -1;
+    if (!c->d.nonlocal)
+        return fail(c, "build_K3_cache_mfRG! has no method for ParquetSolver (src/nonlocal_2/build_K3_cache.jl:97-99 "
+                       "defines it for NL_ParquetSolver and NL2_ParquetSolver)");
+    if (c->sg[SG_PP3].ncls || c->sg[SG_PH3].ncls)
+        return fail(c, "build_K3_cache_mfRG! under the lattice symmetry groups is not supported: the class fill would "
+                       "make the caches depend on the momentum");
+    return build_k3_cache_mfrg(c, is_first != 0);
 }
 
 }  // extern "C"
@@ -1596,6 +1685,10 @@ int fdga_update_F(fdga_ctx* c) {
 }
 
 int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
+    // the _new strategies update Σ as their base strategy (src/SDE.jl:4-28)
+    if (strategy == FDGA_FDPA_NEW) strategy = FDGA_FDPA;
+    if (strategy == FDGA_SCPA_NEW) strategy = FDGA_SCPA;
+    if (strategy != FDGA_SCPA && strategy != FDGA_FDPA) return fail(c, "Calculation strategy unknown");
     // fdPA (src/SDE.jl:7-31): Σ = SDE(F, G) - SDE(F0, G0, Π0) + Σ0 [- (n0 - 1/2) U i].  Everything
     // but SDE(F, G) is the same every iteration: formed once into sde0, before the target levels so
     // that Lpp/Lph end up holding the target's (S.Lpp, S.Lph)
@@ -1615,13 +1708,13 @@ int fdga_sde(fdga_ctx* c, int32_t strategy, int32_t include_hartree) {
     return sde_levels(c, c->Sig, c->G, c->Pipp, c->Piph, 0, include_hartree) ? -3 : 0;
 }
 
-static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma);
+static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma, int32_t hartree);
 
 // One iteration with its two independent chains on two streams.  After Dyson and the bubbles,
 // chain A (caches -> BSE_L_K3! -> BSE_K3!) runs on st and chain B (BSE_L_K2! -> weights ->
 // BSE_K1! -> BSE_K2!) on st2; they write disjoint arrays (FL.K3, Fbuff.K3 vs FL.K2, Fbuff.K1/K2)
 // and B reads FL only at ν' = ∞ (K1 and K2 classes).  The join precedes set!(S.F, S.Fbuff) and SDE!.
-static int iterate_two_chains(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
+static int iterate_two_chains(fdga_ctx* c, int32_t strategy, int32_t update_sigma, int32_t hartree) {
     if (update_sigma && (fdga_dyson(c) || fdga_bubbles(c))) return -3;
     HIPCHK(c, hipEventRecord(c->evfork, c->st));
     HIPCHK(c, hipStreamWaitEvent(c->st2, c->evfork, 0));
@@ -1632,7 +1725,7 @@ static int iterate_two_chains(fdga_ctx* c, int32_t strategy, int32_t update_sigm
     if (strategy == FDGA_FDPA) rc = launch_lk2_pa(c) || fdga_bse_l_k2(c, TCH);
     rc = rc ? rc : launch_w_all(c);
     c->w_fresh = 1;
-    rc = rc ? rc : (launch_k1_pa(c) || fdga_bse_k1(c, TCH));
+    rc = rc ? rc : (launch_k1_pa(c) || fdga_bse_k1(c, TCH, 0));
     rc = rc ? rc : launch_k2_all(c);
     c->w_fresh = 0;
     c->run = c->st;
@@ -1645,18 +1738,53 @@ static int iterate_two_chains(fdga_ctx* c, int32_t strategy, int32_t update_sigm
     HIPCHK(c, hipEventRecord(c->evjoin, c->st2));
     HIPCHK(c, hipStreamWaitEvent(c->st, c->evjoin, 0));
     if (fdga_update_F(c)) return -3;
-    if (update_sigma && fdga_sde(c, strategy, 1)) return -3;
+    if (update_sigma && fdga_sde(c, strategy, hartree)) return -3;
     return 0;
 }
 
-// Captures fdga_iterate(strategy, 1) for both F/Fbuff pointer states.  Capture records the launches
+// :scPA_new / :fdPA_new (src/solve.jl:26-46): caches, [BSE_L_K3!], BSE_K3!, then the alternative K1 and
+// K2 forms, which read only F, F0, Π and Π0
+static int iterate_new(fdga_ctx* c, int32_t strategy, int32_t update_sigma, int32_t hartree) {
+    const int all[3] = {PCH, ACH, TCH};
+    stage_begin(c);
+    if (update_sigma && (fdga_dyson(c) || fdga_bubbles(c))) return -3;
+    stage_end(c, 0);
+    stage_begin(c);
+    if (fdga_build_k3_cache(c)) return -3;
+    stage_end(c, 1);
+    if (strategy == FDGA_FDPA_NEW) {
+        stage_begin(c);
+        if (launch_lk3(c, all, 3)) return -3;
+        stage_end(c, 3);
+    }
+    stage_begin(c);
+    if (launch_k3(c, all, 3)) return -3;
+    stage_end(c, 6);
+    stage_begin(c);
+    for (int ch : all)
+        if (launch_form_ch<FORM_K1_NEW>(c, ch, 0)) return -3;
+    stage_end(c, 4);
+    stage_begin(c);
+    for (int ch : all)
+        if (launch_form_ch<FORM_K2_NEW>(c, ch, 0)) return -3;
+    stage_end(c, 5);
+    stage_begin(c);
+    if (fdga_update_F(c)) return -3;
+    stage_end(c, 7);
+    stage_begin(c);
+    if (update_sigma && fdga_sde(c, strategy, hartree)) return -3;
+    stage_end(c, 8);
+    return 0;
+}
+
+// Captures fdga_iterate(strategy, 1, 1) for both F/Fbuff pointer states.  Capture records the launches
 // without running them, so the host-side buffer swap done while recording is undone after each.
 static int capture_iteration(fdga_ctx* c, int strategy) {
     for (int k = 0; k < 2; ++k) {
         const int state = c->swap_state;
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        const int rc = iterate_eager(c, strategy, 1);
+        const int rc = iterate_eager(c, strategy, 1, 1);
         const hipError_t ec = hipStreamEndCapture(c->st, &g);
         if (rc || ec != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
@@ -1674,22 +1802,26 @@ static int capture_iteration(fdga_ctx* c, int strategy) {
     return 0;
 }
 
-int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
+int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma, int32_t compute_hartree) {
     if (!c) return -1;
-    if (strategy != FDGA_SCPA && strategy != FDGA_FDPA) return fail(c, "Calculation strategy unknown");
-    // graph replay: the fixed-point iteration with Σ update, stage timing off, and for fdPA once the
-    // reference system's SDE is cached (the first fdPA iteration computes it eagerly)
-    if (c->graph_on && update_sigma && !c->timing && (strategy == FDGA_SCPA || c->sde0_valid)) {
+    if (strategy < FDGA_SCPA || strategy > FDGA_FDPA_NEW) return fail(c, "Calculation strategy unknown");
+    const bool fd = (strategy == FDGA_FDPA || strategy == FDGA_FDPA_NEW);
+    c->caches_mfrg = 0;   // the iteration's build_K3_cache! (also when replayed from a graph)
+    // graph replay: the fixed-point iteration with Σ update and the Hartree term, stage timing off, and
+    // for the fdPA strategies once the reference system's SDE is cached (the first such iteration
+    // computes it eagerly)
+    if (c->graph_on && update_sigma && compute_hartree && !c->timing && (!fd || (c->sde0_valid && c->sde0_h == 1))) {
         if (!c->gexec[strategy][c->swap_state] && capture_iteration(c, strategy)) return -3;
         HIPCHK(c, hipGraphLaunch(c->gexec[strategy][c->swap_state], c->st));
         swap_F_host(c);
         return 0;
     }
-    return iterate_eager(c, strategy, update_sigma);
+    return iterate_eager(c, strategy, update_sigma, compute_hartree);
 }
 
-static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
-    if (!c->timing) return iterate_two_chains(c, strategy, update_sigma);
+static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma, int32_t hartree) {
+    if (strategy == FDGA_SCPA_NEW || strategy == FDGA_FDPA_NEW) return iterate_new(c, strategy, update_sigma, hartree);
+    if (!c->timing) return iterate_two_chains(c, strategy, update_sigma, hartree);
     stage_begin(c);
     if (update_sigma) {
         if (fdga_dyson(c) || fdga_bubbles(c)) return -3;
@@ -1711,7 +1843,7 @@ static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     // the K1/K2 weights depend on Π, Π0, FL and F0 only: computed once for both stages
     if (launch_w_all(c)) return -3;
     c->w_fresh = 1;
-    int rc = launch_k1_pa(c) || fdga_bse_k1(c, TCH);
+    int rc = launch_k1_pa(c) || fdga_bse_k1(c, TCH, 0);
     stage_end(c, 4);
     stage_begin(c);
     rc = rc ? rc : launch_k2_all(c);
@@ -1728,7 +1860,7 @@ static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
     if (fdga_update_F(c)) return -3;
     stage_end(c, 7);
     stage_begin(c);
-    if (update_sigma && fdga_sde(c, strategy, 1)) return -3;
+    if (update_sigma && fdga_sde(c, strategy, hartree)) return -3;
     stage_end(c, 8);
     return 0;
 }
@@ -1736,11 +1868,22 @@ static int iterate_eager(fdga_ctx* c, int32_t strategy, int32_t update_sigma) {
 int64_t fdga_flat_size(const fdga_ctx* c) {
     return (int64_t)(3 * (c->n1 + c->n2 + c->n3) * c->NP + c->nGel);
 }
+int64_t fdga_flat_size_F(const fdga_ctx* c) { return (int64_t)(3 * (c->n1 + c->n2 + c->n3) * c->NP); }
 
-int fdga_unflatten(fdga_ctx* c, const double* x) {
-    const size_t n = (size_t)fdga_flat_size(c);
+}  // extern "C"
+
+namespace {
+// x (host, reference layout [γp; γt; γa] of K1, K2, K3, then Σ when `sigma`) into the device arrays;
+// the vertex part is multiplied by `scale`
+int unflatten_into(fdga_ctx* c, const double* x, bool sigma, double scale) {
+    const size_t n = (size_t)(sigma ? fdga_flat_size(c) : fdga_flat_size_F(c));
     if (ensure_stage(c, n)) return -2;
     HIPCHK(c, hipMemcpyAsync(c->stage, x, n * sizeof(double2), hipMemcpyHostToDevice, c->st));
+    if (scale != 1.0) {
+        const size_t nf = (size_t)fdga_flat_size_F(c);
+        k_axpby<<<cdiv(nf, 256), 256, 0, c->run>>>(c->stage, c->stage, scale, 0.0, nf);
+        LAUNCHCHK(c);
+    }
     size_t off = 0;
     for (int r = 0; r < 3; ++r) {
         HIPCHK(c, hipMemcpyAsync(c->F[r].K1, c->stage + off, c->n1 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
@@ -1751,13 +1894,13 @@ int fdga_unflatten(fdga_ctx* c, const double* x) {
         LAUNCHCHK(c);
         off += c->n3 * c->NP;
     }
-    HIPCHK(c, hipMemcpyAsync(c->Sig, c->stage + off, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    if (sigma) HIPCHK(c, hipMemcpyAsync(c->Sig, c->stage + off, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
     if (refresh_avg(c)) return -3;
     return 0;
 }
-
-int fdga_flatten(fdga_ctx* c, double* x) {
-    const size_t n = (size_t)fdga_flat_size(c);
+// the device arrays into x (host, the same layout); synchronises the stream
+int flatten_from(fdga_ctx* c, double* x, bool sigma) {
+    const size_t n = (size_t)(sigma ? fdga_flat_size(c) : fdga_flat_size_F(c));
     if (ensure_stage(c, n)) return -2;
     size_t off = 0;
     for (int r = 0; r < 3; ++r) {
@@ -1769,18 +1912,59 @@ int fdga_flatten(fdga_ctx* c, double* x) {
         LAUNCHCHK(c);
         off += c->n3 * c->NP;
     }
-    HIPCHK(c, hipMemcpyAsync(c->stage + off, c->Sig, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    if (sigma) HIPCHK(c, hipMemcpyAsync(c->stage + off, c->Sig, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(x, c->stage, n * sizeof(double2), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
+}  // namespace
 
-int fdga_fixed_point(fdga_ctx* c, double* R, const double* x, int32_t strategy) {
-    if (fdga_unflatten(c, x)) return -3;
-    if (fdga_iterate(c, strategy, 1)) return -3;
-    if (fdga_flatten(c, R)) return -3;
-    const size_t n = 2 * (size_t)fdga_flat_size(c);
+extern "C" {
+
+int fdga_unflatten(fdga_ctx* c, const double* x) { return unflatten_into(c, x, true, 1.0); }
+int fdga_flatten(fdga_ctx* c, double* x) { return flatten_from(c, x, true); }
+
+int fdga_fixed_point(fdga_ctx* c, double* R, const double* x, int32_t strategy, int32_t update_sigma,
+                     int32_t compute_hartree) {
+    if (!c || !R || !x) return -1;
+    if (unflatten_into(c, x, update_sigma != 0, 1.0)) return -3;
+    if (fdga_iterate(c, strategy, update_sigma, compute_hartree)) return -3;
+    if (flatten_from(c, R, update_sigma != 0)) return -3;
+    const size_t n = 2 * (size_t)(update_sigma ? fdga_flat_size(c) : fdga_flat_size_F(c));
     for (size_t i = 0; i < n; ++i) R[i] -= x[i];
+    return 0;
+}
+
+int fdga_mfrg_apply(fdga_ctx* c, double* y, const double* x, int32_t strategy, int32_t is_first) {
+    if (!c || !y || !x) return -1;
+    if (strategy != FDGA_FDPA && strategy != FDGA_FDPA_NEW)
+        return fail(c, "Invalid strategy. Must be fdPA or fdPA_new (fdPA_1loop is not on the device path).");
+    if (!c->d.nonlocal)
+        return fail(c, "build_K3_cache_mfRG! has no method for ParquetSolver (src/nonlocal_2/build_K3_cache.jl:97-99 "
+                       "defines it for NL_ParquetSolver and NL2_ParquetSolver)");
+    const double factor = 1e-2;   // src/mfRG.jl:39
+    const int all[3] = {PCH, ACH, TCH};
+    if (unflatten_into(c, x, false, factor)) return -3;
+    if (fdga_build_k3_cache_mfrg(c, is_first)) return -3;
+    if (strategy == FDGA_FDPA) {
+        for (int ch : all)
+            if (fdga_bse_l_k2(c, ch)) return -3;
+        for (int ch : all)
+            if (launch_form_ch<FORM_K1_MFRG>(c, ch, 1)) return -3;
+        for (int ch : all)
+            if (launch_form_ch<FORM_K2_MFRG>(c, ch, 1)) return -3;
+    } else {
+        for (int ch : all)
+            if (launch_form_ch<FORM_K1_NEW>(c, ch, 1)) return -3;
+        for (int ch : all)
+            if (launch_form_ch<FORM_K2_NEW>(c, ch, 1)) return -3;
+    }
+    if (launch_lk3(c, all, 3)) return -3;
+    if (launch_k3(c, all, 3, true)) return -3;
+    if (fdga_update_F(c)) return -3;
+    if (flatten_from(c, y, false)) return -3;
+    const size_t n = 2 * (size_t)fdga_flat_size_F(c);
+    for (size_t i = 0; i < n; ++i) y[i] = x[i] - y[i] / factor;
     return 0;
 }
 
@@ -1797,7 +1981,7 @@ int fdga_time_iterations(fdga_ctx* c, int32_t strategy, int32_t n, double* ms) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, hipEventRecord(a, c->st));
     for (int i = 0; i < n; ++i)
-        if (fdga_iterate(c, strategy, 1)) return -3;
+        if (fdga_iterate(c, strategy, 1, 1)) return -3;
     HIPCHK(c, hipEventRecord(b, c->st));
     HIPCHK(c, hipEventSynchronize(b));
     float t = 0.f;
@@ -2229,7 +2413,7 @@ int group_iterate_eager(fdga_group* g, int strategy, int update_sigma) {
         if (launch_w_all(c)) return member_err(g, c);
         c->w_fresh = 1;
         int rc = 0;
-        for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch);
+        for (int ch : {PCH, ACH, TCH}) rc = rc ? rc : fdga_bse_k1(c, ch, 0);
         rc = rc ? rc : launch_k2_all(c);
         c->w_fresh = 0;
         if (rc) return member_err(g, c);
@@ -2284,6 +2468,7 @@ int fdga_group_create(fdga_group** out, fdga_ctx* const* members, int32_t n) {
         if (!c) return gfail(g, "null member");
         for (fdga_ctx* o : g->m)
             if (o == c) return gfail(g, "a context appears twice in the group");
+        c->caches_mfrg = 0;   // the group's functors take the plain build_K3_cache! layout
         g->m.push_back(c);
     }
     g->n = n;
@@ -2338,6 +2523,8 @@ const char* fdga_group_last_error(const fdga_group* g) { return g ? g->err.c_str
 
 int fdga_group_iterate(fdga_group* g, int32_t strategy, int32_t update_sigma) {
     if (!g) return -1;
+    if (strategy == FDGA_SCPA_NEW || strategy == FDGA_FDPA_NEW)
+        return gfail(g, "the :scPA_new / :fdPA_new strategies are not on the solver-group path; iterate the members alone");
     if (strategy != FDGA_SCPA && strategy != FDGA_FDPA) return gfail(g, "Calculation strategy unknown");
     bool sde0 = true;
     for (const fdga_ctx* c : g->m) sde0 = sde0 && c->sde0_valid;

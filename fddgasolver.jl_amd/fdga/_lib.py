@@ -20,7 +20,7 @@ LPP, LPH = 9, 10
 TAB_FPP, TAB_FPX, TAB_FTP, TAB_FTX = 11, 12, 13, 14
 CACHE0 = 32
 PCH, TCH, ACH = 0, 1, 2
-SCPA, FDPA = 0, 1
+SCPA, FDPA, SCPA_NEW, FDPA_NEW = 0, 1, 2, 3
 FP64, FP32 = 0, 1
 TILE_64x64, TILE_32x32, TILE_64x32, TILE_32x64 = 0, 1, 2, 3
 
@@ -68,16 +68,21 @@ SIGNATURES = {
     "fdga_build_k3_cache": (ctypes.c_int, [_vp]),
     "fdga_bse_l_k2": (ctypes.c_int, [_vp, _i32]),
     "fdga_bse_l_k3": (ctypes.c_int, [_vp, _i32]),
-    "fdga_bse_k1": (ctypes.c_int, [_vp, _i32]),
-    "fdga_bse_k2": (ctypes.c_int, [_vp, _i32]),
-    "fdga_bse_k3": (ctypes.c_int, [_vp, _i32]),
+    "fdga_bse_k1": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_bse_k2": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_bse_k3": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_bse_k1_new": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_bse_k2_new": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_build_k3_cache_mfrg": (ctypes.c_int, [_vp, _i32]),
     "fdga_update_F": (ctypes.c_int, [_vp]),
     "fdga_sde": (ctypes.c_int, [_vp, _i32, _i32]),
-    "fdga_iterate": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_iterate": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
     "fdga_flat_size": (_i64, [_vp]),
+    "fdga_flat_size_F": (_i64, [_vp]),
     "fdga_flatten": (ctypes.c_int, [_vp, _dp]),
     "fdga_unflatten": (ctypes.c_int, [_vp, _dp]),
-    "fdga_fixed_point": (ctypes.c_int, [_vp, _dp, _dp, _i32]),
+    "fdga_fixed_point": (ctypes.c_int, [_vp, _dp, _dp, _i32, _i32, _i32]),
+    "fdga_mfrg_apply": (ctypes.c_int, [_vp, _dp, _dp, _i32, _i32]),
     "fdga_sync": (ctypes.c_int, [_vp]),
     "fdga_time_iterations": (ctypes.c_int, [_vp, _i32, _i32, _dp]),
     "fdga_stage_ms": (ctypes.c_int, [_vp, _i32, _dp]),

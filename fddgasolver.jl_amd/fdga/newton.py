@@ -69,8 +69,10 @@ def newton_krylov(residual, x0, *, tol=1e-8, maxiter=30, eta=0.1, restart=30, gm
 
     The Newton step solves J dx = -R to the relative tolerance eta (Eisenstat–Walker choice 2,
     bounded by `eta`); J v = (R(x + h v) - R(x)) / h with h = h_rel (1 + ||x||_inf) / ||v||_inf.
-    The step is accepted when it lowers ||R||_2, else halved (up to 8 times).  Converged when
-    max|R| < tol, as in `solve!`.  Returns (x, newton iterations, max|R|, residual evaluations)."""
+    The step is accepted when it lowers ||R||_2, else halved (up to 8 times); when no trial lowers it the
+    iteration stops at the last accepted x (max|R| >= tol tells the caller it did not converge).
+    Converged when max|R| < tol, as in `solve!`.  Returns (x, newton iterations, max|R|, residual
+    evaluations)."""
     x = np.array(x0, dtype=np.complex128)
     R = residual(x)
     nev = 1
@@ -97,13 +99,19 @@ def newton_krylov(residual, x0, *, tol=1e-8, maxiter=30, eta=0.1, restart=30, gm
         dx, lin_rel, nprod = gmres(jv, -R, tol=eta_k, restart=restart, maxiter=gmres_restarts)
         nev += nprod
         lam = 1.0
+        accepted = False
         for _ in range(9):
             xn = x + lam * dx
             Rn = residual(xn)
             nev += 1
             if _norm(Rn) < rn:
+                accepted = True
                 break
             lam *= 0.5
+        if not accepted:
+            if verbose:
+                print(f"newton {it:3d}  no step lowers ||R||: stopping at max|R| = {res:.3e}")
+            break
         x, R, rn_old = xn, Rn, rn
         res = float(np.max(np.abs(R)))
         if verbose:

@@ -3,7 +3,8 @@
 Names follow the reference (src/fdDGAsolver.jl:180-205) with Julia's `!` dropped:
 parquet_solver_siam_parquet_approximation, parquet_solver_hubbard_parquet_approximation,
 ParquetSolver, NL_ParquetSolver, iterate_solver, fixed_point, solve, Dyson, bubbles,
-build_K3_cache, BSE_L_K2, BSE_L_K3, BSE_K1, BSE_K2, BSE_K3, SDE, flatten, unflatten.
+build_K3_cache, build_K3_cache_mfRG, BSE_L_K2, BSE_L_K3, BSE_K1, BSE_K2, BSE_K3, BSE_K1_new, BSE_K2_new,
+SDE, flatten, unflatten, mfRGLinearMap.
 All numerical work runs on the GPU through libfdga.so; this module only allocates the
 context, converts arguments and drives the fixed-point loop.
 """
@@ -14,10 +15,10 @@ import numpy as np
 from . import _lib as L
 
 pCh, tCh, aCh = L.PCH, L.TCH, L.ACH
-_STRATEGIES = {"scPA": L.SCPA, "fdPA": L.FDPA}
-# strategies iterate_solver! accepts (src/solve.jl:10) whose BSE forms are not on the device path:
-# the alternative K1/K2 forms (src/BSE_templates.jl:188-252) and the one-loop K3 (src/solve.jl:47-58)
-_STRATEGIES_NOT_BUILT = ("scPA_new", "fdPA_new", "fdPA_1loop")
+_STRATEGIES = {"scPA": L.SCPA, "fdPA": L.FDPA, "scPA_new": L.SCPA_NEW, "fdPA_new": L.FDPA_NEW}
+# a strategy iterate_solver! accepts (src/solve.jl:10) whose BSE forms are not on the device path: the
+# one-loop K3 of :fdPA_1loop (src/solve.jl:47-58)
+_STRATEGIES_NOT_BUILT = ("fdPA_1loop",)
 
 
 def _dp(a):
@@ -262,7 +263,8 @@ def _strategy(strategy):
     if isinstance(strategy, int):
         return strategy
     if strategy in _STRATEGIES_NOT_BUILT:
-        raise NotImplementedError(f"strategy {strategy} (a reference strategy) is not on the device path; use scPA or fdPA")
+        raise NotImplementedError(f"strategy {strategy} (a reference strategy) is not on the device path; use scPA, "
+                                  "fdPA, scPA_new or fdPA_new")
     if strategy not in _STRATEGIES:
         raise AssertionError("Calculation strategy unknown")
     return _STRATEGIES[strategy]
@@ -384,16 +386,35 @@ def BSE_L_K3(S, Ch):
     S._check(S.lib.fdga_bse_l_k3(S.h, Ch))
 
 
-def BSE_K1(S, Ch):
-    S._check(S.lib.fdga_bse_k1(S.h, Ch))
+def BSE_K1(S, Ch, is_mfRG=False):
+    """BSE_K1!(S, Ch, is_mfRG) (src/BSE_templates.jl:12-35)."""
+    S._check(S.lib.fdga_bse_k1(S.h, Ch, int(is_mfRG)))
 
 
-def BSE_K2(S, Ch):
-    S._check(S.lib.fdga_bse_k2(S.h, Ch))
+def BSE_K2(S, Ch, is_mfRG=False):
+    """BSE_K2!(S, Ch, is_mfRG) (src/BSE_templates.jl:77-110)."""
+    S._check(S.lib.fdga_bse_k2(S.h, Ch, int(is_mfRG)))
 
 
-def BSE_K3(S, Ch):
-    S._check(S.lib.fdga_bse_k3(S.h, Ch))
+def BSE_K3(S, Ch, is_mfRG=False):
+    """BSE_K3!(S, Ch, is_mfRG) (src/BSE_templates.jl:151-180)."""
+    S._check(S.lib.fdga_bse_k3(S.h, Ch, int(is_mfRG)))
+
+
+def BSE_K1_new(S, Ch, is_mfRG=False):
+    """BSE_K1_new!(S, Ch, is_mfRG) (src/BSE_templates.jl:188-211): K1 = (U + K1 + K2') Π U."""
+    S._check(S.lib.fdga_bse_k1_new(S.h, Ch, int(is_mfRG)))
+
+
+def BSE_K2_new(S, Ch, is_mfRG=False):
+    """BSE_K2_new!(S, Ch, is_mfRG) (src/BSE_templates.jl:217-252): K2 = (K2 + K3 + (I - U)) Π U."""
+    S._check(S.lib.fdga_bse_k2_new(S.h, Ch, int(is_mfRG)))
+
+
+def build_K3_cache_mfRG(S, is_first_iteration=True):
+    """build_K3_cache_mfRG!(S, is_first_iteration) (src/nonlocal_2/build_K3_cache.jl:97-164); the
+    reference defines it for NL_ParquetSolver only."""
+    S._check(S.lib.fdga_build_k3_cache_mfrg(S.h, int(is_first_iteration)))
 
 
 def update_F(S):
@@ -412,9 +433,9 @@ def reset_sym_grp(S):
     S.reset_sym_grp()
 
 
-def iterate_solver(S, strategy="fdPA", update_Σ=True):
-    """iterate_solver!(S; strategy, update_Σ) (src/solve.jl:4-116)."""
-    S._check(S.lib.fdga_iterate(S.h, _strategy(strategy), int(update_Σ)))
+def iterate_solver(S, strategy="fdPA", update_Σ=True, compute_Hartree=True):
+    """iterate_solver!(S; strategy, update_Σ, compute_Hartree) (src/solve.jl:4-116)."""
+    S._check(S.lib.fdga_iterate(S.h, _strategy(strategy), int(update_Σ), int(compute_Hartree)))
 
 
 def flatten(S):
@@ -424,8 +445,30 @@ def flatten(S):
     return x
 
 
+def flatten_vertex(S):
+    """flatten(S.F): the vertex part [γp; γt; γa] of the solver vector (src/vertex.jl:169-177)."""
+    return flatten(S)[: S.lib.fdga_flat_size_F(S.h)]
+
+
+def _vector(S, x, n, name):
+    """x as a C-contiguous complex128 array of exactly n elements (the C side reads and writes 2 n doubles)."""
+    a = np.asarray(x)
+    if a.dtype != np.complex128:
+        a = a.astype(np.complex128)
+    a = np.ascontiguousarray(a).reshape(-1)
+    if a.size != n:
+        raise ValueError(f"{name}: expected {n} elements, got {a.size}")
+    return a
+
+
+def _out_vector(R, n, name):
+    if not isinstance(R, np.ndarray) or R.dtype != np.complex128 or not R.flags.c_contiguous or R.size != n:
+        raise ValueError(f"{name}: expected a C-contiguous complex128 array of {n} elements")
+    return R
+
+
 def unflatten(S, x):
-    x = np.ascontiguousarray(x, dtype=np.complex128)
+    x = _vector(S, x, S.lib.fdga_flat_size(S.h), "unflatten")
     S._check(S.lib.fdga_unflatten(S.h, _dp(x.view(np.float64))))
 
 
@@ -453,27 +496,59 @@ def chi_from_k1(k1a, k1t, U, NP=1):
     return sp, ch
 
 
-def fixed_point(R, x, S, strategy="fdPA"):
-    """fixed_point!(R, x, S) (src/solve.jl:119-157): R = G(x) - x."""
-    x = np.ascontiguousarray(x, dtype=np.complex128)
-    S._check(S.lib.fdga_fixed_point(S.h, _dp(R.view(np.float64)), _dp(x.view(np.float64)), _strategy(strategy)))
+def fixed_point(R, x, S, strategy="fdPA", update_Σ=True, compute_Hartree=True):
+    """fixed_point!(R, x, S; update_Σ, compute_Hartree) (src/solve.jl:119-157): R = G(x) - x.  With
+    update_Σ = False x and R hold the vertex only (flatten(S.F)), the form src/mfRG.jl:111-173 uses."""
+    n = S.lib.fdga_flat_size(S.h) if update_Σ else S.lib.fdga_flat_size_F(S.h)
+    x = _vector(S, x, n, "fixed_point: x")
+    _out_vector(R, n, "fixed_point: R")
+    S._check(S.lib.fdga_fixed_point(S.h, _dp(R.view(np.float64)), _dp(x.view(np.float64)), _strategy(strategy),
+                                    int(update_Σ), int(compute_Hartree)))
     return R
 
 
-def solve(S, *, strategy="fdPA", maxiter=100, tol=1e-4, δ=0.85, mem=8, verbose=False):
-    """solve!(S; maxiter, tol, δ, mem) (src/solve.jl:160-196): Anderson acceleration as
-    NLsolve 4.5.1 method=:anderson (beta = δ, m = mem, ftol = tol, xtol = 0, aa_start = 1).
+class mfRGLinearMap:
+    """mfRGLinearMap(S, strategy) (src/mfRG.jl:20-89): the linearised BSE around the reference vertex
+    acting on vertex-only vectors, y = x - G_lin(x / 100) * 100.  Applying it overwrites S.F, S.Fbuff,
+    S.FL and the caches, as in the reference; the first product also builds the F caches
+    (is_first_iteration)."""
+
+    def __init__(self, S, strategy="fdPA"):
+        if strategy not in ("fdPA", "fdPA_new", "fdPA_1loop"):
+            raise ValueError(f"Invalid strategy {strategy}. Must be fdPA or fdPA_new or fdPA_1loop.")
+        self.S, self.strategy, self.is_first_iteration = S, strategy, True
+        n = S.lib.fdga_flat_size_F(S.h)
+        self.shape = (n, n)
+
+    def __matmul__(self, x):
+        return self.matvec(x)
+
+    def matvec(self, x):
+        S = self.S
+        x = _vector(S, x, self.shape[0], "mfRGLinearMap: x")
+        y = np.empty_like(x)
+        S._check(S.lib.fdga_mfrg_apply(S.h, _dp(y.view(np.float64)), _dp(x.view(np.float64)), _strategy(self.strategy),
+                                       int(self.is_first_iteration)))
+        self.is_first_iteration = False
+        return y
+
+
+def solve(S, *, strategy="fdPA", maxiter=100, tol=1e-4, δ=0.85, mem=8, verbose=False, update_Σ=True,
+          compute_Hartree=True):
+    """solve!(S; maxiter, tol, δ, mem, update_Σ, compute_Hartree) (src/solve.jl:160-196): Anderson acceleration
+    as NLsolve 4.5.1 method=:anderson (beta = δ, m = mem, ftol = tol, xtol = 0, aa_start = 1).
     The iterate is x_{k+1} = g_k - Σ_i γ_i Δg_i with g = x + δ f(x) and
-    γ = argmin ||f_k - ΔF γ||; converged when max|f| < tol.  Returns (iterations, residual)."""
+    γ = argmin ||f_k - ΔF γ||; converged when max|f| < tol.  update_Σ = False iterates the vertex only
+    (vertex-only vectors, see fixed_point).  Returns (iterations, residual)."""
     strategy = _strategy(strategy)
-    x = flatten(S)
+    x = flatten(S) if update_Σ else flatten_vertex(S)
     f = np.empty_like(x)
     dG, dF = [], []
     fold = gold = None
     res = np.inf
     it = 0
     for it in range(1, maxiter + 1):
-        fixed_point(f, x, S, strategy)
+        fixed_point(f, x, S, strategy, update_Σ, compute_Hartree)
         g = x + δ * f
         res = float(np.max(np.abs(f)))
         if verbose:

@@ -1,17 +1,35 @@
 # FdgaHIP.jl -- ccall binding of libfdga.so for fdDGAsolver.jl.
 #
-# Adds methods for solvers whose `mode === :hip`: the solver state is mirrored in one device
-# context (include/fdga.h) and the hot-path functions of src/solve.jl:4-116 run on the GPU.
-# Include this file after `using fdDGAsolver` (it extends fdDGAsolver's functions).  Julia is not
-# part of this repository's test image; the Python host in fddgasolver.jl_amd/fdga exercises the
-# same C-ABI.
+# A solver whose `S.mode === :hip` runs the hot path of iterate_solver! (src/solve.jl:4-116) on the
+# GPU.  The methods below extend the reference's own functions for ParquetSolver{ComplexF64} (with a
+# Vertex) and NL_ParquetSolver{ComplexF64}; each branches on S.mode and otherwise `invoke`s the
+# reference's generic method, so the reference's callers reach the device unchanged:
+#     init_sym_grp!(S); solve!(S; parallel_mode = :hip, strategy = :fdPA)
+# (solve! sets S.mode, src/solve.jl:173-175, and NLsolve calls fixed_point!, which runs one device pass).
+#
+# Each solver is mirrored by one device context (include/fdga.h), created on first use from the
+# solver's inputs and current state.  The whole-iteration methods (iterate_solver!, fixed_point!)
+# upload the fixed-point state (S.F, S.Σ) and download everything iterate_solver! leaves in S:
+# F (and Fbuff = F after set!), Σ, G, Πpp, Πph, FL and the SDE's Lpp, Lph.  The K3 caches stay on the
+# device (every iteration rebuilds them; build_K3_cache! called on its own downloads them).  The
+# per-stage methods upload the arrays their stage reads and download what it writes, so each is a
+# drop-in for the reference method alone; the caches they read are the device's, i.e. those of the
+# last device build_K3_cache!.  Gbare, G0, Σ0, Π0 and F0 are uploaded at context creation: after
+# changing them on the host, call release!(S) so that the next use rebuilds the context.
+#
+# Include this file after `using fdDGAsolver`.  Julia is not part of this repository's test image;
+# the Python host in fddgasolver.jl_amd/fdga exercises the same C-ABI, and tests/test_abi.py checks
+# every ccall below against the header.
 
 module FdgaHIP
 
 using fdDGAsolver
 using MatsubaraFunctions
-import fdDGAsolver: iterate_solver!, bubbles!, build_K3_cache!, BSE_K1!, BSE_K2!, BSE_K3!, BSE_L_K2!, BSE_L_K3!,
-                    Dyson!, SDE!, init_sym_grp!, reset_sym_grp!, fixed_point!
+using LinearMaps
+import fdDGAsolver: iterate_solver!, bubbles!, build_K3_cache!, build_K3_cache_mfRG!, BSE_K1!, BSE_K2!, BSE_K3!,
+                    BSE_L_K2!, BSE_L_K3!, BSE_K1_new!, BSE_K2_new!, Dyson!, SDE!, init_sym_grp!, reset_sym_grp!,
+                    fixed_point!, AbstractSolver, ParquetSolver, NL_ParquetSolver, Vertex, RefVertex, mfRGLinearMap,
+                    pCh, tCh, aCh, numK1, numK2, numK3
 
 const libfdga = joinpath(@__DIR__, "..", "lib", "libfdga.so")
 
@@ -23,117 +41,323 @@ struct FdgaDims
     tab_nb::Int32; tab_nf::Int32; fdpa_h0::Int32; precision::Int32   # precision: 0 FP64, 1 FP32-MFMA contraction
 end
 
+const HipLocal = ParquetSolver{ComplexF64, <:Vertex}
+const HipSolver = Union{HipLocal, NL_ParquetSolver{ComplexF64}}
+
 const PCH, TCH, ACH = Int32(0), Int32(1), Int32(2)
 chcode(::Type{pCh}) = PCH
 chcode(::Type{tCh}) = TCH
 chcode(::Type{aCh}) = ACH
-stratcode(s::Symbol) = s === :scPA ? Int32(0) : s === :fdPA ? Int32(1) :
-                       throw(AssertionError("Calculation strategy unknown"))
+function stratcode(s::Symbol)
+    s === :scPA && return Int32(0)
+    s === :fdPA && return Int32(1)
+    s === :scPA_new && return Int32(2)
+    s === :fdPA_new && return Int32(3)
+    s === :fdPA_1loop && throw(ArgumentError("strategy :fdPA_1loop is not on the device path"))
+    throw(AssertionError("Calculation strategy unknown"))
+end
+valcode(::Val{true}) = Int32(1)
+valcode(::Val{false}) = Int32(0)
 
-const CONTEXTS = IdDict{Any, Ptr{Cvoid}}()
+# array identifiers (include/fdga.h)
+const GBARE, G0, SIGMA0, G, SIGMA = Int32(0), Int32(1), Int32(2), Int32(3), Int32(4)
+const PI0PP, PI0PH, PIPP, PIPH, LPP, LPH = Int32(5), Int32(6), Int32(7), Int32(8), Int32(9), Int32(10)
+const CACHE0 = Int32(32)
+vertex_id(v, ch, cls) = Int32(64 + 16v + 3ch + (cls - 1))
+const V_F, V_FBUFF, V_FL, V_F0 = 0, 1, 2, 3
+
+# options read when a context is created: FP32-MFMA contraction, the fdPA SDE's (n0 - 1/2) U i line
+const PRECISION = Ref{Int32}(0)
+const FDPA_H0 = Ref{Int32}(1)
+set_precision!(p::Symbol) = (PRECISION[] = p === :fp32 ? Int32(1) : p === :fp64 ? Int32(0) :
+                             throw(ArgumentError("precision :fp64 or :fp32")); nothing)
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+end
+# weakly keyed: a solver that is no longer referenced is collected and its finalizer frees the device context
+const CONTEXTS = WeakKeyDict{AbstractSolver, Context}()
 
 check(ctx, rc) = rc == 0 || error(unsafe_string(ccall((:fdga_last_error, libfdga), Cstring, (Ptr{Cvoid},), ctx)))
 
 fdga_set(ctx, which, A::Array{ComplexF64}) =
     check(ctx, ccall((:fdga_set, libfdga), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}), ctx, which, A))
-fdga_get!(ctx, which, A::Array{ComplexF64}) =
+fdga_get!(A::Array{ComplexF64}, ctx, which) =
     check(ctx, ccall((:fdga_get, libfdga), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}), ctx, which, A))
-vertex_id(v, ch, cls) = Int32(64 + 16v + 3ch + (cls - 1))
 
-# create the device mirror of a ParquetSolver (src/ParquetSolver.jl:81-156)
-function context(S::ParquetSolver{ComplexF64})
-    get!(CONTEXTS, S) do
-        T = temperature(S); nK1 = numK1(S); nK2 = numK2(S); nK3 = numK3(S)
-        mPinu = div(length(meshes(S.Π0pp, Val(2))), 2nK1)
-        F0 = S.F0
-        refkind = F0 isa RefVertex ? 0 : 1
-        ref = refkind == 0 ? F0 : F0.F0
-        rb = refkind == 0 ? (1, (1, 1), (1, 1)) : (numK1(F0), numK2(F0), numK3(F0))
-        tb = numK3(ref)
-        d = FdgaDims(0, N(meshes(S.G, Val(1))), nK1, nK2..., nK3..., mPinu, 1, 1, T, real(ref.U), refkind,
-                     rb[1], rb[2]..., rb[3]..., tb..., 1, 0)
-        ctx = Ref{Ptr{Cvoid}}(C_NULL)
-        rc = ccall((:fdga_create, libfdga), Cint, (Ref{Ptr{Cvoid}}, Ref{FdgaDims}), ctx, d)
-        check(ctx[], rc)
-        fdga_set(ctx[], Int32(0), S.Gbare.data); fdga_set(ctx[], Int32(1), S.G0.data); fdga_set(ctx[], Int32(2), S.Σ0.data)
-        for (w, t) in zip(11:14, (ref.Fp_p, ref.Fp_x, ref.Ft_p, ref.Ft_x)); fdga_set(ctx[], Int32(w), t.data); end
-        if refkind == 1
-            for (ch, γ) in enumerate((F0.γp, F0.γt, F0.γa)), (cls, K) in enumerate((γ.K1, γ.K2, γ.K3))
-                fdga_set(ctx[], vertex_id(3, ch - 1, cls), K.data)
-            end
-        end
-        check(ctx[], ccall((:fdga_init, libfdga), Cint, (Ptr{Cvoid},), ctx[]))
-        finalizer(_ -> ccall((:fdga_destroy, libfdga), Cint, (Ptr{Cvoid},), ctx[]), S)
-        ctx[]
+channels(F) = ((PCH, F.γp), (TCH, F.γt), (ACH, F.γa))
+γof(F, ch) = ch == PCH ? F.γp : ch == TCH ? F.γt : F.γa
+classes(γ) = ((1, γ.K1), (2, γ.K2), (3, γ.K3))
+
+function push_vertex(ctx, v, F; cls = (1, 2, 3), chs = (PCH, TCH, ACH))
+    for ch in chs, (c, K) in classes(γof(F, ch))
+        c in cls && fdga_set(ctx, vertex_id(v, ch, c), K.data)
+    end
+end
+function pull_vertex!(F, ctx, v; cls = (1, 2, 3), chs = (PCH, TCH, ACH))
+    for ch in chs, (c, K) in classes(γof(F, ch))
+        c in cls && fdga_get!(K.data, ctx, vertex_id(v, ch, c))
+    end
+end
+caches(S) = (S.cache_Γpx, S.cache_F0p, S.cache_F0a, S.cache_F0t, S.cache_Γpp, S.cache_Γa, S.cache_Γt,
+             S.cache_Fp, S.cache_Fa, S.cache_Ft)
+function pull_caches!(S, ctx)
+    for (c, M) in enumerate(caches(S))
+        fdga_get!(M.data, ctx, CACHE0 + Int32(c - 1))
     end
 end
 
-# init_sym_grp! / reset_sym_grp! (src/ParquetSolver.jl:203-243): the host groups for the Julia
-# side and the device class tables for the iteration
-function init_sym_grp!(S::ParquetSolver{ComplexF64}, ::Val{:hip})
-    invoke(init_sym_grp!, Tuple{ParquetSolver}, S)
-    ctx = context(S)
-    check(ctx, ccall((:fdga_init_sym_grp, libfdga), Cint, (Ptr{Cvoid},), ctx))
-end
-function reset_sym_grp!(S::ParquetSolver{ComplexF64}, ::Val{:hip})
-    invoke(reset_sym_grp!, Tuple{fdDGAsolver.AbstractSolver}, S)
-    ctx = context(S)
-    check(ctx, ccall((:fdga_reset_sym_grp, libfdga), Cint, (Ptr{Cvoid},), ctx))
+# fdga_dims of a solver (src/ParquetSolver.jl:81-156, src/nonlocal/ParquetSolver.jl:77-152)
+function dims(S::HipSolver)
+    nl = S isa NL_ParquetSolver
+    F0 = S.F0
+    (F0 isa RefVertex || F0 isa Vertex) ||
+        throw(ArgumentError("the device path takes a RefVertex or a local Vertex as F0, not $(typeof(F0))"))
+    ref = F0 isa RefVertex ? F0 : F0.F0
+    ref isa RefVertex || throw(ArgumentError("the reference Vertex's own F0 must be a RefVertex"))
+    refkind = F0 isa RefVertex ? 0 : 1
+    rb = refkind == 0 ? (1, (1, 1), (1, 1)) : (numK1(F0), numK2(F0), numK3(F0))
+    nK1 = numK1(S)
+    mPinu = div(length(meshes(S.Π0pp, Val(2))), 2nK1)
+    LG = nl ? bz(meshes(S.G, Val(2))).L : 1
+    L = nl ? bz(meshes(S.F.γp.K1, Val(2))).L : 1
+    return FdgaDims(nl, N(meshes(S.G, Val(1))), nK1, numK2(S)..., numK3(S)..., mPinu, LG, L, temperature(S),
+                    real(ref.U), refkind, rb[1], rb[2]..., rb[3]..., numK3(ref)..., FDPA_H0[], PRECISION[])
 end
 
-# iterate_solver! for mode :hip: one device pass between flatten/unflatten of the solver state
-function iterate_solver!(S::ParquetSolver{ComplexF64}, ::Val{:hip}; strategy::Symbol = :fdPA)
-    ctx = context(S)
-    x = flatten(S)
-    check(ctx, ccall((:fdga_unflatten, libfdga), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}), ctx, x))
-    check(ctx, ccall((:fdga_iterate, libfdga), Cint, (Ptr{Cvoid}, Int32, Int32), ctx, stratcode(strategy), 1))
-    check(ctx, ccall((:fdga_flatten, libfdga), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}), ctx, x))
-    unflatten!(S, x)
+# the device context of S, created on first use: inputs, then the current state of S
+function context(S::HipSolver)
+    c = get(CONTEXTS, S, nothing)
+    c !== nothing && c.ptr != C_NULL && return c.ptr
+    d = dims(S)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:fdga_create, libfdga), Cint, (Ref{Ptr{Cvoid}}, Ref{FdgaDims}), r, d)
+    rc == 0 || (r[] == C_NULL ? error("fdga_create failed ($rc)") : check(r[], rc))
+    ctx = Context(r[])
+    CONTEXTS[S] = ctx
+    finalizer(_ -> destroy!(ctx), S)
+    p = ctx.ptr
+    fdga_set(p, GBARE, S.Gbare.data); fdga_set(p, G0, S.G0.data); fdga_set(p, SIGMA0, S.Σ0.data)
+    ref = S.F0 isa RefVertex ? S.F0 : S.F0.F0
+    for (w, t) in zip(11:14, (ref.Fp_p, ref.Fp_x, ref.Ft_p, ref.Ft_x)); fdga_set(p, Int32(w), t.data); end
+    S.F0 isa Vertex && push_vertex(p, V_F0, S.F0)
+    check(p, ccall((:fdga_init, libfdga), Cint, (Ptr{Cvoid},), p))
+    # the solver's current state, which may differ from the constructor's
+    fdga_set(p, SIGMA, S.Σ.data); fdga_set(p, G, S.G.data)
+    fdga_set(p, PIPP, S.Πpp.data); fdga_set(p, PIPH, S.Πph.data)
+    fdga_set(p, PI0PP, S.Π0pp.data); fdga_set(p, PI0PH, S.Π0ph.data)
+    push_vertex(p, V_F, S.F); push_vertex(p, V_FL, S.FL); push_vertex(p, V_FBUFF, S.Fbuff)
+    # symmetry groups installed on the host (init_sym_grp!) are installed on the device too
+    length(S.SGpp[3].classes) < length(S.F.γp.K3.data) &&
+        check(p, ccall((:fdga_init_sym_grp, libfdga), Cint, (Ptr{Cvoid},), p))
+    return p
+end
+
+function destroy!(ctx::Context)
+    ctx.ptr == C_NULL && return nothing
+    ccall((:fdga_destroy, libfdga), Cint, (Ptr{Cvoid},), ctx.ptr)
+    ctx.ptr = C_NULL
+    return nothing
+end
+"""release!(S): free the device context of S now (the next :hip call creates a new one from S)."""
+function release!(S::AbstractSolver)
+    c = get(CONTEXTS, S, nothing)
+    c === nothing || destroy!(c)
+    delete!(CONTEXTS, S)
     return nothing
 end
 
-# per-stage methods: the device state is synchronised by the caller through context()/fdga_set
-for (jl, c) in ((:BSE_K1!, :fdga_bse_k1), (:BSE_K2!, :fdga_bse_k2), (:BSE_K3!, :fdga_bse_k3),
-                (:BSE_L_K2!, :fdga_bse_l_k2), (:BSE_L_K3!, :fdga_bse_l_k3))
-    @eval function $jl(S::ParquetSolver{ComplexF64}, ::Type{Ch}, ::Val{:hip}) where {Ch <: ChannelTag}
-        ctx = context(S)
-        check(ctx, ccall(($(QuoteNode(c)), libfdga), Cint, (Ptr{Cvoid}, Int32), ctx, chcode(Ch)))
+flat_size(ctx) = ccall((:fdga_flat_size, libfdga), Int64, (Ptr{Cvoid},), ctx)
+flat_size_F(ctx) = ccall((:fdga_flat_size_F, libfdga), Int64, (Ptr{Cvoid},), ctx)
+
+# after a device iteration: the state iterate_solver! leaves in S (src/solve.jl:4-116)
+function pull_state!(S, ctx; update_Σ::Bool)
+    pull_vertex!(S.F, ctx, V_F)
+    set!(S.Fbuff, S.F)                       # set!(S.F, S.Fbuff) left both equal
+    pull_vertex!(S.FL, ctx, V_FL; cls = (2, 3))
+    if update_Σ
+        fdga_get!(S.Σ.data, ctx, SIGMA); fdga_get!(S.G.data, ctx, G)
+        fdga_get!(S.Πpp.data, ctx, PIPP); fdga_get!(S.Πph.data, ctx, PIPH)
+        fdga_get!(S.Lpp.data, ctx, LPP); fdga_get!(S.Lph.data, ctx, LPH)
     end
-end
-for (jl, c) in ((:Dyson!, :fdga_dyson), (:bubbles!, :fdga_bubbles), (:build_K3_cache!, :fdga_build_k3_cache))
-    @eval function $jl(S::ParquetSolver{ComplexF64}, ::Val{:hip})
-        ctx = context(S)
-        check(ctx, ccall(($(QuoteNode(c)), libfdga), Cint, (Ptr{Cvoid},), ctx))
-    end
-end
-# SDE! (src/SDE.jl:3-33): the device path has the default include_U² = true, subtract_asymptotic = false
-function SDE!(S::ParquetSolver{ComplexF64}, ::Val{:hip}; strategy = :scPA, include_U² = true, include_Hartree = true,
-              subtract_asymptotic = false)
-    (include_U² && !subtract_asymptotic) || throw(ArgumentError("the device SDE! has include_U² = true, subtract_asymptotic = false"))
-    ctx = context(S)
-    check(ctx, ccall((:fdga_sde, libfdga), Cint, (Ptr{Cvoid}, Int32, Int32), ctx, stratcode(strategy), Int32(include_Hartree)))
+    return nothing
 end
 
-# fixed_point! (src/solve.jl:119-157) with update_Σ = true: R = G(x) - x, one device pass
-function fixed_point!(R::Vector{ComplexF64}, x::Vector{ComplexF64}, S::ParquetSolver{ComplexF64}, ::Val{:hip};
-                      strategy::Symbol = :fdPA)
+# ---------------------------------------------------------------------------------------------
+# iterate_solver!(S; strategy, update_Σ, compute_Hartree) (src/solve.jl:4-116): one device pass
+function iterate_solver!(S::HipSolver; strategy::Symbol = :fdPA, update_Σ::Bool = true, compute_Hartree::Bool = true)
+    S.mode === :hip || return invoke(iterate_solver!, Tuple{AbstractSolver}, S; strategy, update_Σ, compute_Hartree)
     ctx = context(S)
-    n = ccall((:fdga_flat_size, libfdga), Int64, (Ptr{Cvoid},), ctx)
+    push_vertex(ctx, V_F, S.F)
+    fdga_set(ctx, SIGMA, S.Σ.data)
+    check(ctx, ccall((:fdga_iterate, libfdga), Cint, (Ptr{Cvoid}, Int32, Int32, Int32), ctx, stratcode(strategy),
+                     Int32(update_Σ), Int32(compute_Hartree)))
+    pull_state!(S, ctx; update_Σ)
+    return nothing
+end
+
+# fixed_point!(R, x, S; update_Σ, ...) (src/solve.jl:119-157): R = G(x) - x in one device call.  With
+# update_Σ = false x and R are vertex-only (the form of src/mfRG.jl:111-173).  A log file goes through
+# the reference's method (which calls the iterate_solver! above); the chemical-potential callback
+# runs on the host and its Gbare is uploaded.
+function fixed_point!(R::Vector{ComplexF64}, x::Vector{ComplexF64}, S::HipSolver; filename_log = nothing,
+                      update_Σ::Bool = true, callback_chemical_potential_update! = nothing, kwargs_solver...)
+    if S.mode !== :hip || filename_log !== nothing
+        return invoke(fixed_point!, Tuple{Vector{ComplexF64}, Vector{ComplexF64}, AbstractSolver{ComplexF64}}, R, x, S;
+                      filename_log, update_Σ, callback_chemical_potential_update!, kwargs_solver...)
+    end
+    ctx = context(S)
+    n = update_Σ ? flat_size(ctx) : flat_size_F(ctx)
     (length(x) == n && length(R) == n) || throw(DimensionMismatch("state vectors of length $n expected"))
-    check(ctx, ccall((:fdga_fixed_point, libfdga), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int32),
-                     ctx, R, x, stratcode(strategy)))
+    if update_Σ && callback_chemical_potential_update! !== nothing
+        unflatten!(S, x)
+        callback_chemical_potential_update!(S)
+        fdga_set(ctx, GBARE, S.Gbare.data)
+    end
+    strategy = get(kwargs_solver, :strategy, :fdPA)
+    compute_Hartree = get(kwargs_solver, :compute_Hartree, true)
+    check(ctx, ccall((:fdga_fixed_point, libfdga), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int32, Int32, Int32),
+                     ctx, R, x, stratcode(strategy), Int32(update_Σ), Int32(compute_Hartree)))
+    pull_state!(S, ctx; update_Σ)
     return nothing
+end
+
+# ---------------------------------------------------------------------------------------------
+# per-stage methods: each uploads what its stage reads and downloads what it writes
+function Dyson!(S::HipSolver)
+    S.mode === :hip || return invoke(Dyson!, Tuple{AbstractSolver}, S)
+    ctx = context(S)
+    fdga_set(ctx, SIGMA, S.Σ.data)
+    check(ctx, ccall((:fdga_dyson, libfdga), Cint, (Ptr{Cvoid},), ctx))
+    fdga_get!(S.G.data, ctx, G)
+    return nothing
+end
+
+for (ST, generic) in ((HipLocal, AbstractSolver), (NL_ParquetSolver{ComplexF64}, NL_ParquetSolver))
+    @eval function bubbles!(S::$ST)
+        S.mode === :hip || return invoke(bubbles!, Tuple{$generic}, S)
+        ctx = context(S)
+        fdga_set(ctx, G, S.G.data)
+        check(ctx, ccall((:fdga_bubbles, libfdga), Cint, (Ptr{Cvoid},), ctx))
+        fdga_get!(S.Πpp.data, ctx, PIPP); fdga_get!(S.Πph.data, ctx, PIPH)
+        return nothing
+    end
+end
+
+for (ST, generic) in ((HipLocal, ParquetSolver), (NL_ParquetSolver{ComplexF64}, NL_ParquetSolver))
+    @eval function build_K3_cache!(S::$ST)
+        S.mode === :hip || return invoke(build_K3_cache!, Tuple{$generic}, S)
+        ctx = context(S)
+        push_vertex(ctx, V_F, S.F)
+        check(ctx, ccall((:fdga_build_k3_cache, libfdga), Cint, (Ptr{Cvoid},), ctx))
+        pull_caches!(S, ctx)
+        return nothing
+    end
+end
+
+# build_K3_cache_mfRG!(S, is_first_iteration) (src/nonlocal_2/build_K3_cache.jl:97-164)
+function build_K3_cache_mfRG!(S::NL_ParquetSolver{ComplexF64}, is_first_iteration::Bool)
+    S.mode === :hip || return invoke(build_K3_cache_mfRG!, Tuple{NL_ParquetSolver, Bool}, S, is_first_iteration)
+    ctx = context(S)
+    push_vertex(ctx, V_F, S.F)
+    check(ctx, ccall((:fdga_build_k3_cache_mfrg, libfdga), Cint, (Ptr{Cvoid}, Int32), ctx, Int32(is_first_iteration)))
+    pull_caches!(S, ctx)
+    return nothing
+end
+
+const MfrgFlag = Union{Val{true}, Val{false}}
+for Ch in (pCh, tCh, aCh)
+    # BSE_L_K2!(S, Ch) and BSE_L_K3!(S, Ch) (src/BSE_templates.jl:40-145): FL.γ; t reads the finished a
+    @eval function BSE_L_K2!(S::HipSolver, ::Type{$Ch}, is_mfRG::MfrgFlag = Val(false))
+        S.mode === :hip || return invoke(BSE_L_K2!, Tuple{AbstractSolver, Type{$Ch}, MfrgFlag}, S, $Ch, is_mfRG)
+        ctx = context(S)
+        push_vertex(ctx, V_F, S.F)
+        push_vertex(ctx, V_FL, S.FL; cls = (2,), chs = (ACH,))
+        check(ctx, ccall((:fdga_bse_l_k2, libfdga), Cint, (Ptr{Cvoid}, Int32), ctx, chcode($Ch)))
+        pull_vertex!(S.FL, ctx, V_FL; cls = (2,), chs = (chcode($Ch),))
+        return nothing
+    end
+    @eval function BSE_L_K3!(S::HipSolver, ::Type{$Ch}, is_mfRG::MfrgFlag = Val(false))
+        S.mode === :hip || return invoke(BSE_L_K3!, Tuple{AbstractSolver, Type{$Ch}, MfrgFlag}, S, $Ch, is_mfRG)
+        ctx = context(S)
+        push_vertex(ctx, V_FL, S.FL; cls = (3,), chs = (ACH,))
+        check(ctx, ccall((:fdga_bse_l_k3, libfdga), Cint, (Ptr{Cvoid}, Int32), ctx, chcode($Ch)))
+        pull_vertex!(S.FL, ctx, V_FL; cls = (3,), chs = (chcode($Ch),))
+        return nothing
+    end
+    # BSE_K1!/BSE_K2!/BSE_K3!(S, Ch, is_mfRG) (src/BSE_templates.jl:12-180): Fbuff.γ; t reads the finished a
+    for (jl, c, cls) in ((:BSE_K1!, :fdga_bse_k1, 1), (:BSE_K2!, :fdga_bse_k2, 2), (:BSE_K3!, :fdga_bse_k3, 3),
+                         (:BSE_K1_new!, :fdga_bse_k1_new, 1), (:BSE_K2_new!, :fdga_bse_k2_new, 2))
+        @eval function $jl(S::HipSolver, ::Type{$Ch}, is_mfRG::MfrgFlag = Val(false))
+            S.mode === :hip || return invoke($jl, Tuple{AbstractSolver, Type{$Ch}, MfrgFlag}, S, $Ch, is_mfRG)
+            ctx = context(S)
+            push_vertex(ctx, V_F, S.F)
+            push_vertex(ctx, V_FL, S.FL)
+            push_vertex(ctx, V_FBUFF, S.Fbuff; cls = ($cls,), chs = (ACH,))
+            check(ctx, ccall(($(QuoteNode(c)), libfdga), Cint, (Ptr{Cvoid}, Int32, Int32), ctx, chcode($Ch), valcode(is_mfRG)))
+            pull_vertex!(S.Fbuff, ctx, V_FBUFF; cls = ($cls,), chs = (chcode($Ch),))
+            return nothing
+        end
+    end
+end
+
+# SDE!(S; strategy, include_U², include_Hartree, subtract_asymptotic) (src/SDE.jl:3-33)
+function SDE!(S::HipSolver; strategy = :scPA, include_U² = true, include_Hartree = true, subtract_asymptotic = false)
+    S.mode === :hip || return invoke(SDE!, Tuple{AbstractSolver}, S; strategy, include_U², include_Hartree,
+                                     subtract_asymptotic)
+    (include_U² && !subtract_asymptotic) ||
+        throw(ArgumentError("the device SDE! has include_U² = true and subtract_asymptotic = false"))
+    ctx = context(S)
+    push_vertex(ctx, V_F, S.F)
+    fdga_set(ctx, G, S.G.data); fdga_set(ctx, PIPP, S.Πpp.data); fdga_set(ctx, PIPH, S.Πph.data)
+    check(ctx, ccall((:fdga_sde, libfdga), Cint, (Ptr{Cvoid}, Int32, Int32), ctx, stratcode(strategy), Int32(include_Hartree)))
+    fdga_get!(S.Σ.data, ctx, SIGMA); fdga_get!(S.Lpp.data, ctx, LPP); fdga_get!(S.Lph.data, ctx, LPH)
+    return nothing
+end
+
+# init_sym_grp! / reset_sym_grp! (src/ParquetSolver.jl:203-243, src/nonlocal/ParquetSolver.jl:198-300): the
+# host groups, and the device class tables when the solver already has a context
+for (ST, generic) in ((HipLocal, ParquetSolver), (NL_ParquetSolver{ComplexF64}, NL_ParquetSolver))
+    @eval function init_sym_grp!(S::$ST)
+        invoke(init_sym_grp!, Tuple{$generic}, S)
+        c = get(CONTEXTS, S, nothing)
+        (c === nothing || c.ptr == C_NULL) || check(c.ptr, ccall((:fdga_init_sym_grp, libfdga), Cint, (Ptr{Cvoid},), c.ptr))
+        return nothing
+    end
+end
+function reset_sym_grp!(S::HipSolver)
+    invoke(reset_sym_grp!, Tuple{AbstractSolver}, S)
+    c = get(CONTEXTS, S, nothing)
+    (c === nothing || c.ptr == C_NULL) || check(c.ptr, ccall((:fdga_reset_sym_grp, libfdga), Cint, (Ptr{Cvoid},), c.ptr))
+    return nothing
+end
+
+# mfRGLinearMap * x (src/mfRG.jl:34-89) for an NL_ParquetSolver in :hip mode: one device call
+function LinearMaps._unsafe_mul!(y::AbstractVector, A::mfRGLinearMap{<:NL_ParquetSolver{ComplexF64}}, x::AbstractVector)
+    S = A.S
+    S.mode === :hip || return invoke(LinearMaps._unsafe_mul!, Tuple{Any, mfRGLinearMap, AbstractVector}, y, A, x)
+    ctx = context(S)
+    n = flat_size_F(ctx)
+    (length(x) == n && length(y) == n) || throw(DimensionMismatch("vectors of length $n expected"))
+    xv = Vector{ComplexF64}(x)
+    yv = Vector{ComplexF64}(undef, n)
+    check(ctx, ccall((:fdga_mfrg_apply, libfdga), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int32, Int32),
+                     ctx, yv, xv, stratcode(A.strategy), Int32(A.is_first_iteration)))
+    A.is_first_iteration = false
+    pull_state!(S, ctx; update_Σ = false)
+    copyto!(y, yv)
+    return y
 end
 
 # launch options (no reference counterpart; results do not depend on them)
-set_graph!(S::ParquetSolver{ComplexF64}, on::Bool) =
+set_graph!(S::HipSolver, on::Bool) =
     check(context(S), ccall((:fdga_set_graph, libfdga), Cint, (Ptr{Cvoid}, Int32), context(S), Int32(on)))
-autotune!(S::ParquetSolver{ComplexF64}; reps::Integer = 3) =
+autotune!(S::HipSolver; reps::Integer = 3) =
     check(context(S), ccall((:fdga_autotune, libfdga), Cint, (Ptr{Cvoid}, Int32), context(S), Int32(reps)))
 
 # solver groups: several solvers (a parameter sweep) iterated together, the contractions of all of
 # them in one launch each; destroy the group before its members
-function group(solvers::Vector{<:ParquetSolver{ComplexF64}})
+function group(solvers::Vector{<:HipSolver})
     ctxs = [context(S) for S in solvers]
     g = Ref{Ptr{Cvoid}}(C_NULL)
     rc = ccall((:fdga_group_create, libfdga), Cint, (Ref{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Int32), g, ctxs, length(ctxs))
@@ -148,10 +372,5 @@ group_set_graph!(g::Ptr{Cvoid}, on::Bool) =
     group_check(g, ccall((:fdga_group_set_graph, libfdga), Cint, (Ptr{Cvoid}, Int32), g, Int32(on)))
 group_autotune!(g::Ptr{Cvoid}; reps::Integer = 3) =
     group_check(g, ccall((:fdga_group_autotune, libfdga), Cint, (Ptr{Cvoid}, Int32), g, Int32(reps)))
-
-# NL_ParquetSolver (src/nonlocal/ParquetSolver.jl:77-152) binds the same way with fdga_dims.nonlocal = 1,
-# LG and L the side lengths of the G and vertex Brillouin-zone meshes, and the momentum-resolved
-# arrays in their MeshFunction data layout (frequency first, then momentum); the Python host
-# (fdga.NL_ParquetSolver) is the tested form of that binding.
 
 end # module

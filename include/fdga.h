@@ -42,7 +42,8 @@ enum { FDGA_FP64 = 0, FDGA_FP32 = 1 };
 /* tile shapes of the contraction GEMMs (workgroups of 2x2, 1x1, 2x1, 1x2 waves of 32x32 each) */
 enum { FDGA_TILE_64x64 = 0, FDGA_TILE_32x32 = 1, FDGA_TILE_64x32 = 2, FDGA_TILE_32x64 = 3 };
 enum { FDGA_PCH = 0, FDGA_TCH = 1, FDGA_ACH = 2 };
-enum { FDGA_SCPA = 0, FDGA_FDPA = 1 };
+/* strategies of iterate_solver! (src/solve.jl:10); :fdPA_1loop is not on the device path */
+enum { FDGA_SCPA = 0, FDGA_FDPA = 1, FDGA_SCPA_NEW = 2, FDGA_FDPA_NEW = 3 };
 
 /* array identifiers for fdga_set / fdga_get (reference layouts) */
 enum {
@@ -82,22 +83,44 @@ int fdga_bubbles(fdga_ctx* ctx);                        /* bubbles!(S)        sr
 int fdga_build_k3_cache(fdga_ctx* ctx);                 /* build_K3_cache!(S) src/build_K3_cache.jl:20-95 */
 int fdga_bse_l_k2(fdga_ctx* ctx, int32_t ch);           /* BSE_L_K2!(S, Ch)   src/BSE_templates.jl:40-71 */
 int fdga_bse_l_k3(fdga_ctx* ctx, int32_t ch);           /* BSE_L_K3!(S, Ch)   src/BSE_templates.jl:116-145 */
-int fdga_bse_k1(fdga_ctx* ctx, int32_t ch);             /* BSE_K1!(S, Ch)     src/BSE_templates.jl:12-35 */
-int fdga_bse_k2(fdga_ctx* ctx, int32_t ch);             /* BSE_K2!(S, Ch)     src/BSE_templates.jl:77-110 */
-int fdga_bse_k3(fdga_ctx* ctx, int32_t ch);             /* BSE_K3!(S, Ch)     src/BSE_templates.jl:151-180 */
+/* is_mfrg: the is_mfRG = Val(true) argument of the reference wrappers (src/BSEa/BSEa_K1.jl:29-35,
+ * BSEa_K2.jl:70-78, BSEa_K3.jl:83-97; lattice src/nonlocal/BSEa/BSEa_K{1,2,3}.jl) */
+int fdga_bse_k1(fdga_ctx* ctx, int32_t ch, int32_t is_mfrg);   /* BSE_K1!(S, Ch, is_mfRG)  src/BSE_templates.jl:12-35 */
+int fdga_bse_k2(fdga_ctx* ctx, int32_t ch, int32_t is_mfrg);   /* BSE_K2!(S, Ch, is_mfRG)  src/BSE_templates.jl:77-110 */
+int fdga_bse_k3(fdga_ctx* ctx, int32_t ch, int32_t is_mfrg);   /* BSE_K3!(S, Ch, is_mfRG)  src/BSE_templates.jl:151-180 */
+/* the alternative forms K1 = (U + K1 + K2') Π U and K2 = (K2 + K3 + (I - U)) Π U (their lattice
+ * is_mfRG form: (F - F0) Π U; the local forms have no is_mfRG branch) */
+int fdga_bse_k1_new(fdga_ctx* ctx, int32_t ch, int32_t is_mfrg);   /* BSE_K1_new!(S, Ch, is_mfRG) src/BSE_templates.jl:188-211 */
+int fdga_bse_k2_new(fdga_ctx* ctx, int32_t ch, int32_t is_mfrg);   /* BSE_K2_new!(S, Ch, is_mfRG) src/BSE_templates.jl:217-252 */
+/* build_K3_cache_mfRG!(S, is_first_iteration) (src/nonlocal_2/build_K3_cache.jl:97-164; NL_ParquetSolver
+ * only, as in the reference; not under lattice symmetry groups) */
+int fdga_build_k3_cache_mfrg(fdga_ctx* ctx, int32_t is_first);
 int fdga_update_F(fdga_ctx* ctx);                       /* set!(S.F, S.Fbuff) src/solve.jl:87 */
 int fdga_sde(fdga_ctx* ctx, int32_t strategy, int32_t include_hartree);   /* SDE!(S) src/SDE.jl:3-33 */
-int fdga_iterate(fdga_ctx* ctx, int32_t strategy, int32_t update_sigma);  /* iterate_solver! src/solve.jl:4-116 */
+/* iterate_solver!(S; strategy, update_Σ, compute_Hartree) src/solve.jl:4-116 */
+int fdga_iterate(fdga_ctx* ctx, int32_t strategy, int32_t update_sigma, int32_t compute_hartree);
 
 /* fixed-point layer: x = [γp(K1,K2,K3); γt; γa; Σ] (src/ParquetSolver.jl:277-306) */
 int64_t fdga_flat_size(const fdga_ctx* ctx);
+int64_t fdga_flat_size_F(const fdga_ctx* ctx);   /* the vertex part [γp; γt; γa] (src/vertex.jl:153-196) */
 int     fdga_flatten(fdga_ctx* ctx, double* x);
 int     fdga_unflatten(fdga_ctx* ctx, const double* x);
-int     fdga_fixed_point(fdga_ctx* ctx, double* R, const double* x, int32_t strategy);  /* src/solve.jl:119-157 */
+/* fixed_point!(R, x, S; update_Σ, compute_Hartree) (src/solve.jl:119-157): R = G(x) - x.  update_sigma = 0:
+ * x and R are vertex-only vectors of fdga_flat_size_F elements (the form src/mfRG.jl:111-173 uses;
+ * src/solve.jl:138 passes the whole solver vector to unflatten!(S.F, x), whose length assertion,
+ * src/vertex.jl:195, it fails) */
+int     fdga_fixed_point(fdga_ctx* ctx, double* R, const double* x, int32_t strategy, int32_t update_sigma,
+                         int32_t compute_hartree);
+/* mfRGLinearMap(S, strategy) * x (src/mfRG.jl:20-89): y = x - G_lin(x / 100) * 100 with G_lin the
+ * linearised BSE (build_K3_cache_mfRG!, for :fdPA BSE_L_K2! and the is_mfRG BSE_K1!/BSE_K2!, for :fdPA_new
+ * the is_mfRG BSE_K1_new!/BSE_K2_new!, then BSE_L_K3! and the is_mfRG BSE_K3!); x, y vertex-only
+ * (fdga_flat_size_F).  is_first: the map's is_first_iteration.  NL_ParquetSolver only. */
+int     fdga_mfrg_apply(fdga_ctx* ctx, double* y, const double* x, int32_t strategy, int32_t is_first);
 
 /* synchronisation and timing on the context's stream */
 int fdga_sync(fdga_ctx* ctx);
-/* runs `n` iterate_solver! passes and returns the elapsed milliseconds measured with HIP events */
+/* runs `n` iterate_solver! passes (update_Σ, compute_Hartree) and returns the elapsed milliseconds
+ * measured with HIP events */
 int fdga_time_iterations(fdga_ctx* ctx, int32_t strategy, int32_t n, double* ms);
 /* per-stage device time accumulated by fdga_iterate while stage timing is on (ms, summed over
  * passes); stage: 0 dyson+bubbles, 1 cache, 2 L_K2, 3 L_K3, 4 K1, 5 K2, 6 K3, 7 update, 8 SDE;
@@ -106,7 +129,7 @@ int fdga_stage_ms(const fdga_ctx* ctx, int32_t stage, double* ms);
 /* enable (1) / disable (0) per-stage event timing inside fdga_iterate (adds a host sync per
  * stage); resets the accumulated stage times */
 int fdga_set_stage_timing(fdga_ctx* ctx, int32_t on);
-/* Replay fdga_iterate(ctx, strategy, 1) from hipGraphs captured on first use (1), or launch the
+/* Replay fdga_iterate(ctx, strategy, 1, 1) from hipGraphs captured on first use (1), or launch the
  * kernels one by one (0, the default).  Results are identical; the graphs are recaptured after
  * fdga_init, fdga_init_sym_grp / fdga_reset_sym_grp and after fdga_set of G0, Π0, the RefVertex
  * tables or the reference Vertex.  No counterpart in the reference (execution option). */

@@ -23,7 +23,7 @@ def _run(monkeypatch, mem, maxiter=400):
     A, b, xs = _problem()
     last = {}
 
-    def fixed_point(R, x, S, strategy):
+    def fixed_point(R, x, S, strategy, update_Σ=True, compute_Hartree=True):
         last["x"] = x.copy()
         R[:] = A @ x + b - x
         return R

@@ -79,3 +79,22 @@ def test_fixed_point_sensitivity_nonlinear_against_finite_differences():
     dx, _, _ = newton.fixed_point_sensitivity(lambda x: G(x, p0) - x, xs, dR_dp, tol=1e-12)
     fd = (solve(p0 + h) - solve(p0 - h)) / (2 * h)
     np.testing.assert_allclose(dx, fd, rtol=1e-5, atol=1e-7)
+
+
+def test_newton_stops_when_no_step_lowers_the_residual():
+    """R(x) = x with the linear solves made to return the step of the wrong sign: no trial of the
+    backtracking lowers ||R||, so the iteration keeps the last accepted x and reports a residual above
+    tol instead of taking a worsening step."""
+    x0 = np.full(4, 1.0 + 0.5j)
+    orig = newton.gmres
+
+    def gmres_wrong_sign(matvec, b, **kw):
+        x, rel, n = orig(matvec, b, **kw)
+        return -x, rel, n
+    newton.gmres = gmres_wrong_sign
+    try:
+        x, it, res, nev = newton.newton_krylov(lambda x: x.copy(), x0, tol=1e-10, maxiter=5)
+    finally:
+        newton.gmres = orig
+    assert res >= 1e-10
+    np.testing.assert_array_equal(x, x0)
