@@ -1,24 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark: parquet iterations/s and BSE TFLOP/s of the FP32-MFMA and FP64 GPU paths on one MI355X.
 
-Workload (one "step" = one iterate_solver! pass, src/solve.jl:4-116, strategy :fdPA):
-SIAM with G0(iν) = 1/(iν + iΔ sign ν), Δ = π/5, U = 2, β = 20 (T = 0.05), nK1 = 128,
-nK2 = (64, 64), nK3 = (32, 64), mΠν_factor = 6, nG = 256.  The solver is first brought to
-its fixed point with the Anderson loop (untimed), then `warmup` passes run and `steps` passes
-are timed with HIP events on the solver's stream, all inputs resident in HBM.
+Headline workload (the configuration the metric is quoted on): the nonlocal 2D Hubbard fdPA solver
+(NL_ParquetSolver) on a 16 x 16 lattice, t = 1, t' = 0, U = 2, β = 5, half filling, nK1 = 64,
+nK2 = (32, 32), nK3 = (16, 32), mΠν_factor = 32, nG = 256, around a local reference system: a
+synthetic reference vertex with K1 = K2 = 0 and K3 a complex Gaussian (numpy PCG64, seed 0) times
+0.1 U / (1 + |ν| + |ν'|) (ν, ν' in frequency units), G0 the momentum-averaged bare lattice G, Σ0 = 0.  One "step" is one iterate_solver! pass (src/solve.jl:4-116, strategy :fdPA) with all
+inputs resident in HBM, replayed from the captured hipGraph of one iteration; `warmup` passes run
+first, then `steps` passes are timed with HIP events on the solver's stream.
 
-Both precision paths are run: the FP64 solver to its fixed point, then the FP32-MFMA solver
-(the BSE_L_K3!/BSE_K3! GEMMs in FP32 on MFMA, everything else FP64) from that point to its own
-fixed point; the max relative error of Σ, χ_sp, χ_ch and K3 between the two is reported.
-`value` is the FP32-MFMA path's iterations/s when that error is <= 1e-6, else the FP64 path's
-(`config.value_path` says which).
+Both precision paths run from the same initial state: the FP64 path and the FP32-MFMA form (the BSE
+contractions on FP32 MFMA with FP32 accumulation, everything else FP64).  The max relative error of
+Σ, χ_sp(Ω, q), χ_ch(Ω, q) and the three K3 classes of the FP32 path against FP64 is taken after
+`--acc-iters` iterations from that state (the lattice fixed point is not solved for: the host
+Anderson loop over 1.6 GB state vectors is outside the timed path).  `value` is the FP32-MFMA path's
+iterations/s when every error is <= 1e-6, else the FP64 path's (`config.value_path`).
 
-Prints one JSON line.  `roofline` is for the BSE contraction kernel of the headline path
-(BSE_K3! + BSE_L_K3!, batched complex GEMM on MFMA, tile shapes chosen by the autotuner): flops per
-launch = 8 * BM * BN * Nν3 per computed BM x BN output tile (all tiles with identity groups; the
-tiles that hold a class representative under init_sym_grp!) over the measured average launch time.
-`bse_tflops_dense_equiv` divides the dense count 8 NΩ3 Nν3^3 per channel by the same time.  `cpu_baseline` times the CPU restatement
-(oracle/, one thread) on one iteration of the same workload.
+`roofline` is the lattice bubble kernel (k_bubble_nl, bubbles_real_space!, the largest single kernel of
+the iteration): algorithmic bytes per launch B_b = 16 (2 NΩΠ NνΠ NP + NνG NkG) (both bubbles written
+once, G read once) over its launch time measured with HIP events in the per-stage pass; `traffic` is
+its FETCH_SIZE + WRITE_SIZE per launch from rocprofv3 (profiles/c2_bubble_nl_traffic.txt).
+`roofline_bse_gemm` is the BSE contraction (BSE_L_K3! + BSE_K3!, batched complex GEMM on MFMA):
+executed flops 8 BM BN Nν3 per computed tile over the two launches' times.  `cpu_baseline` times the CPU
+restatement of the reference loops (oracle/) on a stated bounded sample of the same workload, at one
+thread and at all cores.
+
+Other workloads print their own JSON line: --workload c1 (local SIAM fdPA, nK3 = (32, 64)), c0 (the
+small SIAM scPA solve: hipGraph replay, Newton–Krylov vs mixing), c4 (the 8-point lattice sweep in one
+solver group).
 """
 import argparse
 import json
@@ -53,29 +62,90 @@ def build_solver(w, precision="fp64"):
     return S
 
 
-def cpu_baseline(sym=True, reps=3):
-    """The CPU restatement of the reference loops (oracle/, one thread) on the same workload:
-    fdPA iterate_solver! passes of the full box from the solver's initial state, with the same
-    symmetry groups as the GPU run; one warm-up pass, then the median of `reps` passes."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
-    w = WORKLOAD
-    o = po.Oracle(nG=w["nG"], nK1=w["nK1"], nK2=w["nK2"], nK3=w["nK3"], mPinu=w["mPinu"], T=w["T"], U=w["U"])
-    o.siam(0.0, w["Delta"], None)
-    o.init()
-    if sym:
-        o.init_sym_grp()
-    o.iterate(po.FDPA)
+def _threads_all():
+    """the CPU threads this process may use (the box's share, or OMP_NUM_THREADS when set)"""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = len(os.sched_getaffinity(0))
+    return min(n, int(env)) if env and env.isdigit() and int(env) > 0 else n
+
+
+def _time_oracle(o, strategy, reps):
+    """the median of `reps` timed iterate_solver! passes of the oracle solver o (the pass's cost does not
+    depend on the state it starts from)"""
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        o.iterate(po.FDPA)
+        o.iterate(strategy)
         ts.append(time.perf_counter() - t0)
-    dt = sorted(ts)[len(ts) // 2]
-    return {"value": 1.0 / dt, "unit": "iters/s", "cores": 1, "kind": "port",
-            "sample": f"fdPA iterate_solver! passes of the full workload box on the CPU restatement "
-                      f"(oracle/fdga_oracle.cpp, 1 thread{', init_sym_grp!' if sym else ''}): 1 warm-up, "
-                      f"median of {reps} = {dt:.2f} s"}
+    return sorted(ts)[len(ts) // 2]
+
+
+def _time_both(po, make, strategy, reps_all, reps_one):
+    """one warm-up pass at all cores, then `reps_all` passes at all cores and `reps_one` at one thread"""
+    nall = _threads_all()
+    po.set_threads(nall)
+    o = make()
+    o.iterate(strategy)
+    tn = _time_oracle(o, strategy, reps_all)
+    po.set_threads(1)
+    t1 = _time_oracle(o, strategy, reps_one)
+    po.set_threads(nall)
+    return nall, tn, t1
+
+
+def cpu_baseline_c1(sym=True, reps=5):
+    """The CPU restatement of the reference loops (oracle/) on the local SIAM fdPA workload: full-box
+    iterate_solver! passes from the solver's initial state with the same symmetry groups as the GPU run,
+    at one thread and at all cores (1 warm-up, median of `reps`)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    w = WORKLOAD
+
+    def make():
+        o = po.Oracle(nG=w["nG"], nK1=w["nK1"], nK2=w["nK2"], nK3=w["nK3"], mPinu=w["mPinu"], T=w["T"], U=w["U"])
+        o.siam(0.0, w["Delta"], None)
+        o.init()
+        if sym:
+            o.init_sym_grp()
+        return o
+    nall, tn, t1 = _time_both(po, make, po.FDPA, reps, reps)
+    return {"value": 1.0 / tn, "unit": "iters/s", "cores": nall, "kind": "port",
+            "single_thread": {"value": 1.0 / t1, "cores": 1},
+            "sample": f"full fdPA iterate_solver! passes of the workload box on the CPU restatement "
+                      f"(oracle/fdga_oracle.cpp{', init_sym_grp!' if sym else ''}): 1 warm-up, median of {reps} "
+                      f"each: {tn:.2f} s at {nall} threads, {t1:.2f} s at 1 thread"}
+
+
+def cpu_baseline_c2(reps=5, reps_one=3):
+    """The CPU restatement of the reference loops (oracle/) on a bounded sample of the lattice workload:
+    its frequency boxes, bubble mesh, G mesh size per momentum and synthetic reference vertex on a 1 x 1
+    lattice (NP = 1), fdPA iterate_solver! passes timed at all cores (median of `reps`) and at one thread
+    (median of `reps_one`) after one warm-up, scaled by 1/256 to the 256 momenta of the 16 x 16 lattice (the restated loops
+    run every stage once per momentum; the momentum FFTs, O(L² log L), are a small part)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle as po
+    w = C2
+    NP = w["L"] ** 2
+
+    def make():
+        o = po.Oracle(nonlocal_=True, nG=w["nG"], nK1=w["nK1"], nK2=w["nK2"], nK3=w["nK3"], mPinu=w["mPinu"], T=w["T"],
+                      U=w["U"], LG=1, L=1, ref_kind=1, ref_boxes=(w["nK1"], w["nK2"], w["nK3"]))
+        o.hubbard(w["mu"], w["t1"])
+        Gb = o.get(po.GBARE)
+        o.set(po.G0, Gb)
+        for (ch, cls), a in c2_reference_vertex(w, seed=0).items():
+            o.set_vertex(po.VTX_F0, ch, cls, a)
+        o.init()
+        return o
+    nall, tn, t1 = _time_both(po, make, po.FDPA, reps, reps_one)
+    return {"value": 1.0 / (tn * NP), "unit": "iters/s", "cores": nall, "kind": "port",
+            "single_thread": {"value": 1.0 / (t1 * NP), "cores": 1},
+            "sample": f"the workload's frequency boxes, bubble mesh and synthetic reference vertex on a 1x1 lattice "
+                      f"(NP = 1, the Hubbard G at k = 0), fdPA iterate_solver! passes of the CPU restatement "
+                      f"(oracle/fdga_oracle.cpp), 1 warm-up: median of {reps} at {nall} threads {tn:.2f} s, median of "
+                      f"{reps_one} at 1 thread {t1:.2f} s; scaled x{NP} to the 16x16 lattice's {NP} momenta "
+                      f"(value = 1 / ({NP} t))"}
 
 
 def gemm_traffic(path, tile):
@@ -128,42 +198,67 @@ def time_path(S, args, names):
 C2 = dict(L=16, T=0.2, U=2.0, mu=0.0, t1=1.0, nK1=64, nK2=(32, 32), nK3=(16, 32), mPinu=32, nG=256)
 
 
-def build_c2(precision, w=C2, seed=0):
-    """Nonlocal 2D Hubbard fdPA solver (NL_ParquetSolver, src/nonlocal/ParquetSolver.jl:77-152) on an
-    L x L lattice with a synthetic local reference Vertex: K1 = K2 = 0 and K3 a complex Gaussian
-    (numpy PCG64, `seed`) times 0.1 U / (1 + |ν| + |ν'|) (ν in units of πT), G0 the bare lattice G,
-    Σ0 = 0."""
+STAGES = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE", "gemm_L_K3", "gemm_K3",
+          "bubble_kernel"]
+
+
+def c2_reference_vertex(w, seed=0):
+    """The synthetic local reference vertex of the lattice workloads: K1 = K2 = 0, K3 of each channel a
+    complex Gaussian (numpy PCG64, `seed`) times 0.1 U / (1 + |ν| + |ν'|) with ν = πT(2n+1) in frequency
+    units.  Returns {(ch, cls): array in the reference layout}."""
     import numpy as np
-    import fdga
-    L = w["L"]
-    Gb = fdga.hubbard_bare_Green(w["nG"], w["T"], L, μ=w["mu"], t1=w["t1"])
     nb, nf = w["nK3"]
     rng = np.random.default_rng(seed)
     NW, NV = 2 * nb - 1, 2 * nf
-    n = np.arange(-nf, nf)
-    amp = 0.1 * w["U"] / (1 + np.abs(2 * n[None, :] + 1) + np.abs(2 * n[:, None] + 1))   # [ν', ν] -> (ν, ν') symmetric
+    nu = np.pi * w["T"] * (2 * np.arange(-nf, nf) + 1)
+    amp = 0.1 * w["U"] / (1 + np.abs(nu)[:, None] + np.abs(nu)[None, :])   # [ν, ν']
     F0 = {}
     for ch in range(3):
         k3 = (rng.standard_normal((NW, NV, NV)) + 1j * rng.standard_normal((NW, NV, NV))) * amp[None, :, :]
         F0[(ch, 1)] = np.zeros(2 * w["nK1"] - 1, complex)
         F0[(ch, 2)] = np.zeros((2 * w["nK2"][0] - 1) * 2 * w["nK2"][1], complex)
         F0[(ch, 3)] = k3.reshape(-1, order="F")
-    S = fdga.NL_ParquetSolver(w["nK1"], w["nK2"], w["nK3"], L, Gb, Gb, np.zeros_like(Gb), F0, T=w["T"], U=w["U"],
+    return F0
+
+
+def build_c2(precision, w=C2, seed=0):
+    """Nonlocal 2D Hubbard fdPA solver (NL_ParquetSolver, src/nonlocal/ParquetSolver.jl:77-152) on an
+    L x L lattice around a local reference system: the synthetic local reference vertex of
+    c2_reference_vertex, G0(ν, k) = the momentum average of the bare lattice G (the same at every k, as
+    the local reference systems of fdDΓA, cf. test/test_nonlocal_symmetry.jl:144-155) and Σ0 = 0.  (With
+    G0 equal to the lattice G the bubbles of both systems coincide and the fdPA correction vanishes.)"""
+    import numpy as np
+    import fdga
+    L = w["L"]
+    Gb = fdga.hubbard_bare_Green(w["nG"], w["T"], L, μ=w["mu"], t1=w["t1"])
+    G0 = np.repeat(Gb.mean(axis=1, keepdims=True), L * L, axis=1)
+    F0 = c2_reference_vertex(w, seed)
+    S = fdga.NL_ParquetSolver(w["nK1"], w["nK2"], w["nK3"], L, Gb, G0, np.zeros_like(Gb), F0, T=w["T"], U=w["U"],
                               mΠν_factor=w["mPinu"], F0_boxes=(w["nK1"], w["nK2"], w["nK3"]), precision=precision)
     return S
 
 
+def bubble_traffic():
+    """HBM bytes per launch of k_bubble_nl on the 16 x 16 workload (FETCH_SIZE x 2 + WRITE_SIZE, the factors
+    calibrated against a 512 MiB copy) as recorded in profiles/c2_bubble_nl_traffic.txt; None if absent."""
+    import re
+    f = os.path.join(ROOT, "profiles", "c2_bubble_nl_traffic.txt")
+    if not os.path.exists(f):
+        return None
+    m = re.search(r"^hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(f).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def main_c2(args):
-    """One-GPU timing of the C2 lattice iteration in both precisions, from the same state; the
-    FP32-MFMA path's relative error against FP64 after one iteration."""
-    import numpy as np
+    """The headline line: the lattice fdPA iteration in both precision paths."""
     import fdga
-    names = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE", "gemm_L_K3", "gemm_K3"]
     w = C2
     NW3, NV3, NP = 2 * w["nK3"][0] - 1, 2 * w["nK3"][1], w["L"] ** 2
-    f_dense = 2 * 3 * 8.0 * NW3 * NP * NV3 ** 3
-    res, x0, first = {}, None, {}
-    for prec in ("fp64", "fp32"):
+    NWP, NVP = 2 * w["nK1"] - 1, 2 * w["nK1"] * w["mPinu"]
+    f_dense = 2 * 3 * 8.0 * NW3 * NP * NV3 ** 3          # BSE_L_K3! + BSE_K3!, three channels, dense
+    paths, x0 = {}, None
+    run_paths = [q for q in ("fp64", "fp32") if q in args.paths.split(",")]
+    for prec in run_paths:
         t0 = time.perf_counter()
         S = build_c2(prec)
         log(f"bench c2[{prec}]: solver built ({time.perf_counter() - t0:.1f} s)")
@@ -171,35 +266,69 @@ def main_c2(args):
             x0 = fdga.flatten(S)
         else:
             fdga.unflatten(S, x0)
-        fdga.iterate_solver(S, "fdPA")
-        S.sync()
-        first[prec] = fdga.flatten(S)
+        for _ in range(args.acc_iters):
+            fdga.iterate_solver(S, "fdPA")
+        obs = observables(S)
         fdga.unflatten(S, x0)
-        ms, st = time_path(S, args, names)
-        log(f"bench c2[{prec}]: {ms:.2f} ms/iteration; stages " + ", ".join(f"{k}={v:.2f}" for k, v in st.items()))
-        t_bse = (st["gemm_L_K3"] + st["gemm_K3"]) * 1e-3
-        res[prec] = {"ms_per_step": ms, "iters_per_s": 1000.0 / ms, "stage_ms": st,
-                     "bse_tflops": f_dense / t_bse / 1e12}
-        if prec == "fp32" and not args.no_sym:
-            # the same iteration with init_sym_grp!'s lattice groups installed (src/nonlocal/ParquetSolver.jl:
-            # 198-300): a different result on this workload, whose synthetic reference K3 is not symmetric
-            t0 = time.perf_counter()
-            S.init_sym_grp()
-            log(f"bench c2[{prec}, lattice groups]: groups built ({time.perf_counter() - t0:.1f} s)")
-            fdga.unflatten(S, x0)
-            ms_g, st_g = time_path(S, args, names)
-            log(f"bench c2[{prec}, lattice groups]: {ms_g:.2f} ms/iteration; stages "
-                + ", ".join(f"{k}={v:.2f}" for k, v in st_g.items()))
-            res["fp32_groups"] = {"ms_per_step": ms_g, "iters_per_s": 1000.0 / ms_g, "stage_ms": st_g,
-                                  "bse_gemm_flops_executed": S.gemm_flops(1) + S.gemm_flops(0)}
+        ms, st = time_path(S, args, STAGES)
+        f_exec = S.gemm_flops(True) + S.gemm_flops(False)
+        t_gemm = (st["gemm_L_K3"] + st["gemm_K3"]) * 1e-3
+        log(f"bench c2[{prec}]: {ms:.3f} ms/iteration; stages " + ", ".join(f"{k}={v:.3f}" for k, v in st.items()))
+        paths[prec] = dict(ms=ms, st=st, obs=obs, f_exec=f_exec, t_gemm=t_gemm, bse_tflops=f_exec / t_gemm / 1e12,
+                           bse_tflops_dense_equiv=f_dense / t_gemm / 1e12)
         S.close()
-    a, b = first["fp32"], first["fp64"]
-    rel = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-    print(json.dumps({"workload": "c2: NL Hubbard fdPA, L=16, beta=5, U=2, nK1=64, nK2=(32,32), nK3=(16,32), mPinu=32, "
-                                  "nG=256, synthetic local reference vertex", "steps": args.steps, "warmup": args.warmup,
-                      "fp64": res["fp64"], "fp32_mfma": res["fp32"],
-                      "fp32_mfma_with_lattice_groups": res.get("fp32_groups"),
-                      "fp32_vs_fp64_state_relerr_after_one_iteration": rel}), flush=True)
+    if len(run_paths) < 2:   # profiling run of one path: per-stage line only
+        q = run_paths[0]
+        print(json.dumps({"path": q, "ms_per_step": paths[q]["ms"], "stage_ms": paths[q]["st"]}), flush=True)
+        return
+    errs = {k: max_relerr(paths["fp32"]["obs"][k], paths["fp64"]["obs"][k]) for k in paths["fp64"]["obs"]}
+    log("bench c2: FP32-MFMA vs FP64 max rel.err after %d iterations: " % args.acc_iters
+        + ", ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    head = "fp32" if max(errs.values()) <= 1e-6 else "fp64"
+    P = paths[head]
+    b_bubble = 16.0 * (2 * NWP * NVP * NP + 2 * w["nG"] * NP)
+    t_bub = P["st"]["bubble_kernel"] * 1e-3
+    gemm_peak = FP32_MFMA_PEAK if head == "fp32" else FP64_MFMA_PEAK
+    out = {
+        "metric": "parquet iters/s + BSE TFLOP/s, FP32-MFMA vs FP64 GPU @ rel.err<=1e-6 vs FP64",
+        "value": 1000.0 / P["ms"],
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": P["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mfma contraction, f64 elsewhere" if head == "fp32" else "f64",
+        "data": "synthetic (Hubbard model inputs, local reference system: Gaussian reference K3 seed 0, k-averaged G0)",
+        "config": {"workload": "nonlocal 2D Hubbard fdPA (NL_ParquetSolver), 16x16 lattice, t=1, t'=0, U=2, beta=5, "
+                               "half filling, nK1=64, nK2=(32,32), nK3=(16,32), mPinu=32, nG=256, local reference system "
+                               "(Gaussian K3 of amplitude 0.1 U/(1+|nu|+|nu'|), k-averaged bare G0, Sigma0=0)",
+                   "value_path": head,
+                   "iteration_launch": "eager" if args.eager else "hipGraph replay of one captured iteration",
+                   "accuracy": {"iterations_from_one_state": args.acc_iters, "max_relerr_fp32_vs_fp64": errs},
+                   "fp32_mfma": {"iters_per_s": 1000.0 / paths["fp32"]["ms"], "bse_tflops": paths["fp32"]["bse_tflops"],
+                                 "bse_tflops_dense_equiv": paths["fp32"]["bse_tflops_dense_equiv"],
+                                 "stage_ms": paths["fp32"]["st"]},
+                   "fp64": {"iters_per_s": 1000.0 / paths["fp64"]["ms"], "bse_tflops": paths["fp64"]["bse_tflops"],
+                            "bse_tflops_dense_equiv": paths["fp64"]["bse_tflops_dense_equiv"],
+                            "stage_ms": paths["fp64"]["st"]},
+                   "speedup_fp32_over_fp64": paths["fp64"]["ms"] / paths["fp32"]["ms"]},
+        "roofline": {"bound": "hbm", "achieved": b_bubble / t_bub / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": b_bubble / t_bub / HBM_PEAK, "traffic": bubble_traffic(),
+                     "kernel": "k_bubble_nl (bubbles_real_space!: Πpp, Πph of one iteration)",
+                     "algorithmic_bytes_per_launch": b_bubble, "launch_ms": t_bub * 1e3},
+        "roofline_bse_gemm": {"bound": "mfma", "achieved": P["f_exec"] / P["t_gemm"] / 1e12, "peak": gemm_peak / 1e12,
+                              "unit": "TFLOP/s", "frac": P["f_exec"] / P["t_gemm"] / gemm_peak,
+                              "kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA, BSE_L_K3! + "
+                                        "BSE_K3! launches of one iteration)",
+                              "executed_flops": P["f_exec"], "launch_ms": P["t_gemm"] * 1e3},
+    }
+    if not args.no_cpu_baseline:
+        log("bench c2: timing the CPU baseline (oracle, bounded sample, 1 thread and all cores)")
+        out["cpu_baseline"] = cpu_baseline_c2()
+    print(json.dumps(out), flush=True)
 
 
 # batched sweep: 8 Hubbard points (U, beta, nK3f) on a 24 x 24 lattice, mΠν = 1; per point
@@ -331,11 +460,13 @@ def main():
                          "WRITE_SIZE counters against a copy of known size)")
     ap.add_argument("--tol32", type=float, default=1e-7,
                     help="Anderson tolerance of the FP32-MFMA solve (its residual floor is ~5e-8 at this workload)")
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"],
-                    help="c1: the headline SIAM workload (one JSON line of the metric); c2: the NL Hubbard lattice "
-                         "iteration (timing and FP32 error, separate JSON line); c0: the small SIAM solve (hipGraph "
-                         "replay and Newton–Krylov vs mixing, separate JSON line); c4: the 8-point lattice sweep in one "
-                         "solver group (grouped GEMM launches, autotuned tiles, hipGraph; separate JSON line)")
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c4"],
+                    help="c2: the headline NL Hubbard lattice workload (the metric's JSON line); c1: the local SIAM fdPA "
+                         "workload at its fixed point (separate JSON line); c0: the small SIAM solve (hipGraph replay and "
+                         "Newton–Krylov vs mixing, separate JSON line); c4: the 8-point lattice sweep in one solver group "
+                         "(grouped GEMM launches, autotuned tiles, hipGraph; separate JSON line)")
+    ap.add_argument("--acc-iters", type=int, default=5,
+                    help="c2: iterations from the initial state after which the FP32 path is compared with FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the parquet hot path is run on a single GPU in this build "
@@ -354,9 +485,14 @@ def main():
         return main_c0(args)
     if args.workload == "c4":
         return main_c4(args)
+    return main_c1(args)
+
+
+def main_c1(args):
+    """The local SIAM fdPA workload at its fixed point (secondary line)."""
     import fdga
     w = WORKLOAD
-    names = ["dyson+bubbles", "k3_cache", "L_K2", "L_K3", "K1", "K2", "K3", "update", "SDE", "gemm_L_K3", "gemm_K3"]
+    names = STAGES
     f_gemm = flops_bse_k3(w)   # algorithmic flops of one dense channel GEMM
     paths = {}
     x64 = None
@@ -408,6 +544,7 @@ def main():
     achieved = P["f_exec_launch"] / P["gemm_avg_s"]
     traffic = gemm_traffic(head, P["tiles"][1])
     out = {
+        "workload": "c1 (secondary line)",
         "metric": "parquet iters/s + BSE TFLOP/s, FP32-MFMA vs FP64 GPU @ rel.err<=1e-6 vs FP64",
         "value": 1000.0 / P["ms"],
         "unit": "iters/s",
@@ -438,12 +575,15 @@ def main():
                      "kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA BSE_K3!/BSE_L_K3! contraction)"},
     }
     if not args.no_cpu_baseline:
-        log("bench: timing the CPU baseline (oracle, one thread)")
-        out["cpu_baseline"] = cpu_baseline(sym=not args.no_sym)
+        log("bench: timing the CPU baseline (oracle, 1 thread and all cores)")
+        out["cpu_baseline"] = cpu_baseline_c1(sym=not args.no_sym)
     if "fp64_dense" in paths:
         out["config"]["fp64_dense"] = {"iters_per_s": 1000.0 / paths["fp64_dense"]["ms"],
                                        "stage_ms": paths["fp64_dense"]["st"]}
     print(json.dumps(out), flush=True)
+
+
+
 
 
 if __name__ == "__main__":

@@ -107,7 +107,7 @@ struct fdga_ctx {
     // timing
     int timing = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    double stage_ms[11] = {};   // 0..8 stages of fdga_iterate, 9 / 10 the BSE_L_K3! / BSE_K3! GEMM kernels
+    double stage_ms[12] = {};   // 0..8 stages of fdga_iterate, 9 / 10 the BSE_L_K3! / BSE_K3! GEMM kernels, 11 the bubble kernel
 };
 
 // drops the captured hipGraphs of the iteration (defined with the graph replay below)
@@ -360,12 +360,17 @@ double2* pi_tail(fdga_ctx* c, const double2* Pi) {
     return c->Ptail[3];
 }
 
+void kernel_begin(fdga_ctx* c);
+void kernel_end(fdga_ctx* c, int s);
+
 // bubbles for G (target, into Πpp/Πph) or G0 (reference, into Π0pp/Π0ph), with their tail sums
 int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     const fdga_dims& d = c->d;
     if (!d.nonlocal) {
         dim3 g(cdiv(c->NVP, 256), c->NWP);
+        kernel_begin(c);
         k_bubble_local<<<g, 256, 0, c->run>>>(pp, ph, Gsrc, d.nG, d.nK1, c->nVP, d.T);
+        kernel_end(c, 11);
         LAUNCHCHK(c);
         const size_t nrows = (size_t)c->NWP * c->NP;
         k_tail_sums2<<<(unsigned)(2 * nrows), 256, 0, c->run>>>(pp, pi_tail(c, pp), ph, pi_tail(c, ph), c->nVP, c->wb, nrows);
@@ -379,6 +384,7 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     const int L = d.L, NB = bubble_nb(L);
     const dim3 grid(cdiv(c->NVP, NB), c->NWP);
     double2 *cpp = c->pchunk[0], *cph = c->pchunk[1];
+    kernel_begin(c);
     if (L == 16) {
         k_bubble_nl<16, 16, 512><<<grid, 512, (size_t)NB * 256 * sizeof(double2), c->run>>>(
             pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph);
@@ -389,6 +395,7 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
         k_bubble_nl<0, 0><<<grid, 256, (size_t)(NB * L * L + L) * sizeof(double2), c->run>>>(
             pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph);
     }
+    kernel_end(c, 11);
     LAUNCHCHK(c);
     // tail sums from the bubble's chunk sums
     const size_t nrows = (size_t)c->NWP * c->NP;
@@ -1993,7 +2000,7 @@ int fdga_time_iterations(fdga_ctx* c, int32_t strategy, int32_t n, double* ms) {
 }
 
 int fdga_stage_ms(const fdga_ctx* c, int32_t stage, double* ms) {
-    if (!c || !ms || stage < 0 || stage > 10) return -1;
+    if (!c || !ms || stage < 0 || stage > 11) return -1;
     *ms = c->stage_ms[stage];
     return 0;
 }

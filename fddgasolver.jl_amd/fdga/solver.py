@@ -168,9 +168,10 @@ class Solver:
         self._check(self.lib.fdga_set_stage_timing(self.h, int(on)))
 
     def stage_ms(self):
-        """Stage times 0..8 and the two BSE GEMM kernels (9: BSE_L_K3!, 10: BSE_K3!), ms."""
+        """Stage times 0..8, the two BSE GEMM kernels (9: BSE_L_K3!, 10: BSE_K3!) and the bubble
+        kernel (11), ms."""
         out = []
-        for s in range(11):
+        for s in range(12):
             ms = ctypes.c_double(0.0)
             self._check(self.lib.fdga_stage_ms(self.h, s, ctypes.byref(ms)))
             out.append(ms.value)

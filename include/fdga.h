@@ -124,7 +124,8 @@ int fdga_sync(fdga_ctx* ctx);
 int fdga_time_iterations(fdga_ctx* ctx, int32_t strategy, int32_t n, double* ms);
 /* per-stage device time accumulated by fdga_iterate while stage timing is on (ms, summed over
  * passes); stage: 0 dyson+bubbles, 1 cache, 2 L_K2, 3 L_K3, 4 K1, 5 K2, 6 K3, 7 update, 8 SDE;
- * 9 and 10: the BSE_L_K3! and BSE_K3! GEMM kernels alone (inside stages 3 and 6) */
+ * 9 and 10: the BSE_L_K3! and BSE_K3! GEMM kernels alone (inside stages 3 and 6); 11: the bubble
+ * kernel alone (inside stage 0) */
 int fdga_stage_ms(const fdga_ctx* ctx, int32_t stage, double* ms);
 /* enable (1) / disable (0) per-stage event timing inside fdga_iterate (adds a host sync per
  * stage); resets the accumulated stage times */

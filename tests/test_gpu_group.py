@@ -2,7 +2,9 @@
 the BSE_L_K3!/BSE_K3! contractions of all members in one grouped GEMM launch each.  Every member
 must end exactly where iterate_solver on the member alone ends (same kernels, and the grouped
 GEMM accumulates every element in the same order whatever the tile shape or grid), under every
-tile shape and persistent grid, with the hipGraph replay and after the autotuner."""
+tile shape and persistent grid, with the hipGraph replay and after the autotuner; and every grouped
+member equals the CPU oracle run on the same inputs (1e-12).  The full-size sweep is in
+tests/test_gpu_sweep.py."""
 import numpy as np
 import pytest
 
@@ -32,15 +34,23 @@ def _assert_same(solo, grp, what):
 @pytest.mark.parametrize("members", ["nl", "local"])
 def test_group_iteration_equals_members_alone(members, strategy):
     import fdga
-    solo, grp = _pairs(NL_MEMBERS if members == "nl" else LOCAL_MEMBERS, seed=11)
+    import pyoracle as po
+    mem = NL_MEMBERS if members == "nl" else LOCAL_MEMBERS
+    solo, grp = _pairs(mem, seed=11)
+    orcs = [make_pair(c, seed=11 + k, box=b)[1] for k, (c, b) in enumerate(mem)]
     G = fdga.SolverGroup(grp)
     try:
         for it in range(2):
             for S in solo:
                 fdga.iterate_solver(S, strategy)
+            for O in orcs:
+                O.iterate(po.SCPA if strategy == "scPA" else po.FDPA)
             G.iterate(strategy)
             G.sync()
             _assert_same(solo, grp, f"{members} {strategy} iteration {it}")
+            for k, (S, O) in enumerate(zip(grp, orcs)):
+                e = relerr(fdga.flatten(S), O.flatten())
+                assert e < 1e-12, f"{members} {strategy} iteration {it} member {k} vs oracle: {e:.3e}"
     finally:
         G.close()
 
