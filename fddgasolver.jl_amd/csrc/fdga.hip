@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <future>
+#include <mutex>
 #include <vector>
 
 #include "../../include/fdga.h"
@@ -44,9 +45,25 @@ enum { SG_S, SG_K1, SG_PP2, SG_PH2, SG_PP3, SG_PH3, SG_PPL3, SG_PHL3, SG_0PP2, S
 
 }  // namespace
 
+// Scratch of fdga_fixed_point_noalloc: the staging of the host state vector for its layout changes, in a
+// file-scope device array so that the entry point allocates nothing.  Each device the library runs on
+// holds one copy until the process ends.  In host code `scratch` is only a symbol: its device address is
+// taken once per device with hipGetSymbolAddress at context creation (g_scratch_ptr), and only that
+// address is handed to copies and kernels.
+constexpr size_t kScratchBytes = (size_t)FDGA_SCRATCH_BYTES;
+static __device__ __align__(256) double2 scratch[kScratchBytes / sizeof(double2)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
 struct fdga_ctx {
     fdga_dims d{};
     hipStream_t st = nullptr;
+    int device = 0;
+    double2* scratch = nullptr;   // this device's copy of the file-scope scratch array
     // the stream the stage kernels are launched on: st, or st2 for the second chain of an iteration
     hipStream_t run = nullptr;
     hipStream_t st2 = nullptr;
@@ -1141,6 +1158,26 @@ int fdga_create(fdga_ctx** out, const fdga_dims* dims) {
         return -2;
     }
     c->run = c->st;
+    // the scratch array's address on this device, taken once per device with the device current
+    if (hipGetDevice(&c->device) != hipSuccess || c->device < 0 || c->device >= kMaxDevices) {
+        fail(c, "hipGetDevice failed", -2);
+        *out = c;
+        return -2;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[c->device]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                fail(c, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e), -2);
+                *out = c;
+                return -2;
+            }
+            g_scratch_ptr[c->device] = static_cast<double2*>(p);
+        }
+        c->scratch = g_scratch_ptr[c->device];
+    }
     c->NP = d.L * d.L;
     c->NkG = d.LG * d.LG;
     c->NG = 2 * d.nG;
@@ -1882,45 +1919,52 @@ int64_t fdga_flat_size_F(const fdga_ctx* c) { return (int64_t)(3 * (c->n1 + c->n
 namespace {
 // x (host, reference layout [γp; γt; γa] of K1, K2, K3, then Σ when `sigma`) into the device arrays;
 // the vertex part is multiplied by `scale`
-int unflatten_into(fdga_ctx* c, const double* x, bool sigma, double scale) {
+// `buf`: the staging buffer (nullptr: the context's, allocated on first use)
+int unflatten_into(fdga_ctx* c, const double* x, bool sigma, double scale, double2* buf = nullptr) {
     const size_t n = (size_t)(sigma ? fdga_flat_size(c) : fdga_flat_size_F(c));
-    if (ensure_stage(c, n)) return -2;
-    HIPCHK(c, hipMemcpyAsync(c->stage, x, n * sizeof(double2), hipMemcpyHostToDevice, c->st));
+    if (!buf) {
+        if (ensure_stage(c, n)) return -2;
+        buf = c->stage;
+    }
+    HIPCHK(c, hipMemcpyAsync(buf, x, n * sizeof(double2), hipMemcpyHostToDevice, c->st));
     if (scale != 1.0) {
         const size_t nf = (size_t)fdga_flat_size_F(c);
-        k_axpby<<<cdiv(nf, 256), 256, 0, c->run>>>(c->stage, c->stage, scale, 0.0, nf);
+        k_axpby<<<cdiv(nf, 256), 256, 0, c->run>>>(buf, buf, scale, 0.0, nf);
         LAUNCHCHK(c);
     }
     size_t off = 0;
     for (int r = 0; r < 3; ++r) {
-        HIPCHK(c, hipMemcpyAsync(c->F[r].K1, c->stage + off, c->n1 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->F[r].K1, buf + off, c->n1 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
         off += c->n1 * c->NP;
-        HIPCHK(c, hipMemcpyAsync(c->F[r].K2, c->stage + off, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->F[r].K2, buf + off, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
         off += c->n2 * c->NP;
-        k_k3_to_dev<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(c->F[r].K3, c->stage + off, c->NW3, c->NV3, c->NP);
+        k_k3_to_dev<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(c->F[r].K3, buf + off, c->NW3, c->NV3, c->NP);
         LAUNCHCHK(c);
         off += c->n3 * c->NP;
     }
-    if (sigma) HIPCHK(c, hipMemcpyAsync(c->Sig, c->stage + off, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    if (sigma) HIPCHK(c, hipMemcpyAsync(c->Sig, buf + off, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
     if (refresh_avg(c)) return -3;
     return 0;
 }
 // the device arrays into x (host, the same layout); synchronises the stream
-int flatten_from(fdga_ctx* c, double* x, bool sigma) {
+int flatten_from(fdga_ctx* c, double* x, bool sigma, double2* buf = nullptr) {
     const size_t n = (size_t)(sigma ? fdga_flat_size(c) : fdga_flat_size_F(c));
-    if (ensure_stage(c, n)) return -2;
+    if (!buf) {
+        if (ensure_stage(c, n)) return -2;
+        buf = c->stage;
+    }
     size_t off = 0;
     for (int r = 0; r < 3; ++r) {
-        HIPCHK(c, hipMemcpyAsync(c->stage + off, c->F[r].K1, c->n1 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(buf + off, c->F[r].K1, c->n1 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
         off += c->n1 * c->NP;
-        HIPCHK(c, hipMemcpyAsync(c->stage + off, c->F[r].K2, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(buf + off, c->F[r].K2, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
         off += c->n2 * c->NP;
-        k_k3_to_ref<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(c->stage + off, c->F[r].K3, c->NW3, c->NV3, c->NP);
+        k_k3_to_ref<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(buf + off, c->F[r].K3, c->NW3, c->NV3, c->NP);
         LAUNCHCHK(c);
         off += c->n3 * c->NP;
     }
-    if (sigma) HIPCHK(c, hipMemcpyAsync(c->stage + off, c->Sig, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(x, c->stage, n * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+    if (sigma) HIPCHK(c, hipMemcpyAsync(buf + off, c->Sig, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(x, buf, n * sizeof(double2), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
@@ -1939,6 +1983,25 @@ int fdga_fixed_point(fdga_ctx* c, double* R, const double* x, int32_t strategy, 
     if (flatten_from(c, R, update_sigma != 0)) return -3;
     const size_t n = 2 * (size_t)(update_sigma ? fdga_flat_size(c) : fdga_flat_size_F(c));
     for (size_t i = 0; i < n; ++i) R[i] -= x[i];
+    return 0;
+}
+
+int fdga_fixed_point_noalloc(fdga_ctx* c, double* R, const double* x, int32_t strategy, int32_t update_sigma,
+                             int32_t compute_hartree) {
+    if (!c || !R || !x) return -1;
+    if (!c->scratch) return fail(c, "no scratch address for this device");
+    const size_t n = (size_t)(update_sigma ? fdga_flat_size(c) : fdga_flat_size_F(c));
+    if (n * sizeof(double2) > kScratchBytes)
+        return fail(c, "fdga_fixed_point_noalloc: the state vector (" + std::to_string(n * sizeof(double2)) +
+                           " bytes) exceeds the scratch of " + std::to_string(kScratchBytes) +
+                           " bytes; use fdga_fixed_point");
+    if (strategy < FDGA_SCPA || strategy > FDGA_FDPA_NEW) return fail(c, "Calculation strategy unknown");
+    if (unflatten_into(c, x, update_sigma != 0, 1.0, c->scratch)) return -3;
+    // always the eager launches: capturing and instantiating a hipGraph allocates
+    c->caches_mfrg = 0;
+    if (iterate_eager(c, strategy, update_sigma, compute_hartree)) return -3;
+    if (flatten_from(c, R, update_sigma != 0, c->scratch)) return -3;
+    for (size_t i = 0; i < 2 * n; ++i) R[i] -= x[i];
     return 0;
 }
 

@@ -82,6 +82,7 @@ SIGNATURES = {
     "fdga_flatten": (ctypes.c_int, [_vp, _dp]),
     "fdga_unflatten": (ctypes.c_int, [_vp, _dp]),
     "fdga_fixed_point": (ctypes.c_int, [_vp, _dp, _dp, _i32, _i32, _i32]),
+    "fdga_fixed_point_noalloc": (ctypes.c_int, [_vp, _dp, _dp, _i32, _i32, _i32]),
     "fdga_mfrg_apply": (ctypes.c_int, [_vp, _dp, _dp, _i32, _i32]),
     "fdga_sync": (ctypes.c_int, [_vp]),
     "fdga_time_iterations": (ctypes.c_int, [_vp, _i32, _i32, _dp]),

@@ -111,6 +111,16 @@ int     fdga_unflatten(fdga_ctx* ctx, const double* x);
  * src/vertex.jl:195, it fails) */
 int     fdga_fixed_point(fdga_ctx* ctx, double* R, const double* x, int32_t strategy, int32_t update_sigma,
                          int32_t compute_hartree);
+/* fdga_fixed_point without any device allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The state vector is staged through a file-scope device array of FDGA_SCRATCH_BYTES
+ * (64 MiB, held by each device the library runs on until the process ends); a state vector larger than
+ * that (fdga_flat_size or fdga_flat_size_F elements of 16 bytes) is refused with an error.  The iteration
+ * runs its launches eagerly (no hipGraph capture).  Every call on one device shares that array: calls
+ * that overlap on one device, from any context, race. */
+#define FDGA_SCRATCH_BYTES ((int64_t)64 << 20)
+int     fdga_fixed_point_noalloc(fdga_ctx* ctx, double* R, const double* x, int32_t strategy,
+                                 int32_t update_sigma, int32_t compute_hartree);
 /* mfRGLinearMap(S, strategy) * x (src/mfRG.jl:20-89): y = x - G_lin(x / 100) * 100 with G_lin the
  * linearised BSE (build_K3_cache_mfRG!, for :fdPA BSE_L_K2! and the is_mfRG BSE_K1!/BSE_K2!, for :fdPA_new
  * the is_mfRG BSE_K1_new!/BSE_K2_new!, then BSE_L_K3! and the is_mfRG BSE_K3!); x, y vertex-only
