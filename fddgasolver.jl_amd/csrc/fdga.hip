@@ -64,6 +64,9 @@ struct fdga_ctx {
     hipStream_t st = nullptr;
     int device = 0;
     double2* scratch = nullptr;   // this device's copy of the file-scope scratch array
+    // Πpp (bit 0) / Πph (bit 1) stored only on the band by the last lattice bubble; pi_band_G: G is still
+    // the one that bubble read (fdga_get can recompute the whole array)
+    int pi_band = 0, pi_band_G = 0;
     // the stream the stage kernels are launched on: st, or st2 for the second chain of an iteration
     hipStream_t run = nullptr;
     hipStream_t st2 = nullptr;
@@ -380,8 +383,11 @@ double2* pi_tail(fdga_ctx* c, const double2* Pi) {
 void kernel_begin(fdga_ctx* c);
 void kernel_end(fdga_ctx* c, int s);
 
-// bubbles for G (target, into Πpp/Πph) or G0 (reference, into Π0pp/Π0ph), with their tail sums
-int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
+// bubbles for G (target, into Πpp/Πph) or G0 (reference, into Π0pp/Π0ph), with their tail sums.
+// band: the lattice bubble stores Π only on the 16-ω chunks that hold the band ω ∈ [-wb, wb) (the
+// gathers, contractions, weights and SDE sums read nothing else; the far regions enter through the
+// chunk sums); fdga_get of Πpp/Πph then recomputes the whole array from the same G.
+int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc, bool band = false) {
     const fdga_dims& d = c->d;
     if (!d.nonlocal) {
         dim3 g(cdiv(c->NVP, 256), c->NWP);
@@ -401,24 +407,31 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc) {
     const int L = d.L, NB = bubble_nb(L);
     const dim3 grid(cdiv(c->NVP, NB), c->NWP);
     double2 *cpp = c->pchunk[0], *cph = c->pchunk[1];
+    // the chunks [wlo, whi) that k_tail_sums_chunked reads entry by entry: those that meet the band
+    const int wlo = band ? (c->nVP - c->wb) / NB : 0;
+    const int whi = band ? (c->nVP + c->wb + NB - 1) / NB : (int)grid.x;
     kernel_begin(c);
     if (L == 16) {
         k_bubble_nl<16, 16, 512><<<grid, 512, (size_t)NB * 256 * sizeof(double2), c->run>>>(
-            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph);
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph, wlo, whi);
     } else if (L == 24) {
         k_bubble_nl<24, 6><<<grid, 256, (size_t)(NB * 576 + 24) * sizeof(double2), c->run>>>(
-            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph);
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph, wlo, whi);
     } else {
         k_bubble_nl<0, 0><<<grid, 256, (size_t)(NB * L * L + L) * sizeof(double2), c->run>>>(
-            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph);
+            pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph, wlo, whi);
+    }
+    if (pp == c->Pipp) {
+        c->pi_band = band ? 3 : 0;
+        c->pi_band_G = 1;
     }
     kernel_end(c, 11);
     LAUNCHCHK(c);
     // tail sums from the bubble's chunk sums
-    const size_t nrows = (size_t)c->NWP * c->NP;
+    const unsigned nblk = (unsigned)c->NWP * (unsigned)cdiv(c->NP, TAIL_NPB);
     for (int k = 0; k < 2; ++k)
-        k_tail_sums_chunked<<<(unsigned)nrows, 256, 0, c->run>>>(k ? ph : pp, k ? cph : cpp, pi_tail(c, k ? ph : pp),
-                                                                 c->nVP, c->wb, NB, (int)grid.x, c->NWP, c->NP, nrows);
+        k_tail_sums_chunked<<<nblk, 256, 0, c->run>>>(k ? ph : pp, k ? cph : cpp, pi_tail(c, k ? ph : pp), c->nVP, c->wb,
+                                                      NB, (int)grid.x, c->NWP, c->NP);
     LAUNCHCHK(c);
     return 0;
 }
@@ -1381,6 +1394,9 @@ int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
         for (size_t i = 0; i < 2 * a.n; ++i)
             if (host[i] != 0.0) { c->tabs_nonzero = 1; break; }
     }
+    if (which == FDGA_G) c->pi_band_G = 0;
+    if (which == FDGA_PIPP) c->pi_band &= ~1;
+    if (which == FDGA_PIPH) c->pi_band &= ~2;
     invalidate(c, which);
     if (which >= FDGA_VERTEX(0, 0, 1) && which <= FDGA_VERTEX(0, 2, 3) && refresh_avg(c)) return -3;
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1391,6 +1407,12 @@ int fdga_get(fdga_ctx* c, int32_t which, double* host) {
     if (!c || !host) return -1;
     ArrInfo a;
     if (arr_info(c, which, a)) return fail(c, "unknown array id " + std::to_string(which));
+    if ((which == FDGA_PIPP && (c->pi_band & 1)) || (which == FDGA_PIPH && (c->pi_band & 2))) {
+        if (!c->pi_band_G)
+            return fail(c, "Πpp/Πph are stored on the band only and G has changed since the bubbles were computed: "
+                           "call fdga_bubbles first");
+        if (bubbles_into(c, c->Pipp, c->Piph, c->G, false)) return -3;
+    }
     const size_t bytes = a.n * sizeof(double2);
     if (a.kind == 0) {
         HIPCHK(c, hipMemcpyAsync(host, a.p, bytes, hipMemcpyDeviceToHost, c->st));
@@ -1625,12 +1647,13 @@ int fdga_reset_sym_grp(fdga_ctx* c) {
 }
 
 int fdga_dyson(fdga_ctx* c) {
+    c->pi_band_G = 0;
     k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->G, c->Sig, c->Gbare, c->nGel);
     LAUNCHCHK(c);
     return 0;
 }
 
-int fdga_bubbles(fdga_ctx* c) { return bubbles_into(c, c->Pipp, c->Piph, c->G); }
+int fdga_bubbles(fdga_ctx* c) { return bubbles_into(c, c->Pipp, c->Piph, c->G, true); }
 
 int fdga_build_k3_cache(fdga_ctx* c) {
     c->caches_mfrg = 0;
@@ -1858,6 +1881,8 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma, int32_t co
         if (!c->gexec[strategy][c->swap_state] && capture_iteration(c, strategy)) return -3;
         HIPCHK(c, hipGraphLaunch(c->gexec[strategy][c->swap_state], c->st));
         swap_F_host(c);
+        if (c->d.nonlocal) c->pi_band = 3;   // the replayed Dyson and band-stored bubbles
+        c->pi_band_G = 1;
         return 0;
     }
     return iterate_eager(c, strategy, update_sigma, compute_hartree);
@@ -2604,7 +2629,11 @@ int fdga_group_iterate(fdga_group* g, int32_t strategy, int32_t update_sigma) {
             if (c->swap_state != s || has_k3_groups(c)) return gfail(g, "group members changed since the group was made");
         if (!g->gexec[strategy][s] && group_capture(g, strategy)) return -3;
         GHIPCHK(g, hipGraphLaunch(g->gexec[strategy][s], g->st));
-        for (fdga_ctx* c : g->m) swap_F_host(c);
+        for (fdga_ctx* c : g->m) {
+            swap_F_host(c);
+            if (c->d.nonlocal) c->pi_band = 3;   // the replayed Dyson and band-stored bubbles
+            c->pi_band_G = 1;
+        }
         return 0;
     }
     return group_iterate_eager(g, strategy, update_sigma);

@@ -321,6 +321,9 @@ __device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, cons
     return acc;
 }
 
+// Blocks (16 ω chunks) with blockIdx.x in [wlo, whi) store their Π values; every block stores its chunk
+// sums.  The iteration passes the chunks that hold the band ω ∈ [-wb, wb) (every Π entry a later
+// stage reads outside the band enters only through the chunk sums); [0, gridDim.x) stores all of Π.
 // LT: L fixed at compile time (16: radix-4 FFT; 24: unrolled direct DFT), or 0 for a run-time
 // L <= 32 (direct DFT through private arrays, for small test lattices).  Block: NT threads (512 at
 // L = 16, held to 128 VGPRs: two blocks per CU, twice the waves on the same LDS tiles, to hide the
@@ -329,7 +332,8 @@ template <int LT, int NBT, int NT = 256>
 __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __restrict__ pp, double2* __restrict__ ph,
                                                    const double2* __restrict__ GR, const BubbleFold* __restrict__ folds,
                                                    int nG, int Lr, int nW, int nV, double T, int NBr,
-                                                   double2* __restrict__ cpp, double2* __restrict__ cph) {
+                                                   double2* __restrict__ cpp, double2* __restrict__ cph, int wlo,
+                                                   int whi) {
     extern __shared__ double2 sm[];   // NB * L * L (+ L twiddles unless L = 16)
     const int L = LT ? LT : Lr, LL = L * L, NB = NBT ? NBT : NBr;
     double2* s = sm;
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __
                 } else {
                     // back into the thread's own column, for the chunk sums below
                     for (int k = 0; k < L; ++k) s[(r + L * k) * NB + wl] = x[k];
-                    if (wl < nb) {
+                    if (wl < nb && (int)blockIdx.x >= wlo && (int)blockIdx.x < whi) {
                         double2* out = which == 0 ? pp : ph;
                         const size_t o = (size_t)(iw0 + wl) + (size_t)(2 * nV) * (size_t)iW;
                         const size_t qs = (size_t)(2 * nV) * (size_t)(2 * nW - 1);
@@ -1488,56 +1492,69 @@ __global__ __launch_bounds__(256) void k_tail_sums2(const double2* __restrict__ 
 // k_tail_sums for an array whose chunk sums C[P + NP (chunk + NC iΩ)] over NB consecutive ω
 // were written by the lattice bubble: the parts outside the band are the sums of the chunks that
 // lie wholly outside it plus the few entries of the chunks it cuts; the band as in k_tail_sums.
-// Reads NC + 2 wb + O(NB) entries per row instead of 2 nVP.
+// Reads NC + 2 wb + O(NB) entries per row instead of 2 nVP.  A block takes the rows of 8
+// consecutive momenta at one Ω: its chunk-sum reads cover whole 128-byte lines (8 momenta per
+// line), the 32 thread groups' partial sums are added in a fixed order, and each wave then scans
+// the band of two rows.
+constexpr int TAIL_NPB = 8;
 __global__ __launch_bounds__(256) void k_tail_sums_chunked(const double2* __restrict__ X, const double2* __restrict__ C,
                                                            double2* __restrict__ T, int nVP, int wb, int NB, int NC,
-                                                           int NWP, int NP, size_t nrows) {
-    constexpr int NWV = 256 / WAVE;
-    __shared__ double sh[4][NWV];
-    const size_t row = blockIdx.x;
+                                                           int NWP, int NP) {
+    __shared__ double2 red[2][32][TAIL_NPB + 1];
+    __shared__ double2 far[2][TAIL_NPB];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-    if (row >= nrows) return;
-    const int iW = (int)(row % NWP), p = (int)(row / NWP);
-    const double2* x = X + row * (size_t)(2 * nVP);
-    double2* tl = T + row * (size_t)(2 * (wb + 1));
-    double2* tr = tl + (wb + 1);
+    const int iW = (int)(blockIdx.x % NWP), p0 = (int)(blockIdx.x / NWP) * TAIL_NPB;
     const int nout = nVP - wb;                 // far left: ω index < nout; far right: >= nVP + wb
     const int cl = nout / NB;                  // chunks [0, cl) lie wholly left of the band
     const int cr = (nVP + wb + NB - 1) / NB;   // chunks [cr, NC) lie wholly right of it
-    double lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
-    for (int c = tid; c < NC; c += 256) {
-        if (c < cl || c >= cr) {
-            const double2 v = C[(size_t)p + (size_t)NP * (c + (size_t)NC * iW)];
-            if (c < cl) { lr += v.x; li += v.y; } else { rr += v.x; ri += v.y; }
+    {
+        const int pl = tid % TAIL_NPB, cg = tid / TAIL_NPB, p = p0 + pl;
+        double lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+        if (p < NP) {
+            const double2* cp = C + (size_t)p + (size_t)NP * (size_t)NC * iW;
+            for (int c = cg; c < cl; c += 32) { const double2 v = cp[(size_t)NP * c]; lr += v.x; li += v.y; }
+            for (int c = cr + cg; c < NC; c += 32) { const double2 v = cp[(size_t)NP * c]; rr += v.x; ri += v.y; }
         }
+        red[0][cg][pl] = cmk(lr, li);
+        red[1][cg][pl] = cmk(rr, ri);
     }
-    for (int k = cl * NB + tid; k < nout; k += 256) { const double2 v = x[k]; lr += v.x; li += v.y; }
-    for (int k = nVP + wb + tid; k < min(cr * NB, 2 * nVP); k += 256) { const double2 v = x[k]; rr += v.x; ri += v.y; }
-    lr = wave_sum(lr); li = wave_sum(li); rr = wave_sum(rr); ri = wave_sum(ri);
-    if (lane == 0) { sh[0][wv] = lr; sh[1][wv] = li; sh[2][wv] = rr; sh[3][wv] = ri; }
     __syncthreads();
-    double cl_r = 0.0, cl_i = 0.0, cr_r = 0.0, cr_i = 0.0;   // running sums carried across chunks
-#pragma unroll
-    for (int k = 0; k < NWV; ++k) { cl_r += sh[0][k]; cl_i += sh[1][k]; cr_r += sh[2][k]; cr_i += sh[3][k]; }
-    if (tid == 0) { tl[0] = cmk(cl_r, cl_i); tr[wb] = cmk(cr_r, cr_i); }
-    for (int c0 = 0; c0 < wb; c0 += 256) {
-        __syncthreads();   // sh is reused
-        const int i = c0 + tid;
-        const double2 a = i < wb ? x[nout + i] : czero();
-        const double2 b = i < wb ? x[nVP + wb - 1 - i] : czero();
-        double ar = a.x, ai = a.y, br = b.x, bi = b.y;
-        wave_incl_scan(ar, ai, lane);
-        wave_incl_scan(br, bi, lane);
-        if (lane == WAVE - 1) { sh[0][wv] = ar; sh[1][wv] = ai; sh[2][wv] = br; sh[3][wv] = bi; }
-        __syncthreads();
-        double pl_r = cl_r, pl_i = cl_i, pr_r = cr_r, pr_i = cr_i;
-        for (int k = 0; k < wv; ++k) { pl_r += sh[0][k]; pl_i += sh[1][k]; pr_r += sh[2][k]; pr_i += sh[3][k]; }
-        if (i < wb) {
-            tl[i + 1] = cmk(pl_r + ar, pl_i + ai);
-            tr[wb - 1 - i] = cmk(pr_r + br, pr_i + bi);
+    if (tid < 2 * TAIL_NPB) {
+        const int side = tid / TAIL_NPB, pl = tid % TAIL_NPB;
+        double sr = 0.0, si = 0.0;
+        for (int g = 0; g < 32; ++g) { sr += red[side][g][pl].x; si += red[side][g][pl].y; }
+        far[side][pl] = cmk(sr, si);
+    }
+    __syncthreads();
+    for (int pl = wv; pl < TAIL_NPB; pl += 256 / WAVE) {
+        const int p = p0 + pl;
+        if (p >= NP) break;
+        const size_t row = (size_t)iW + (size_t)NWP * p;
+        const double2* x = X + row * (size_t)(2 * nVP);
+        double2* tl = T + row * (size_t)(2 * (wb + 1));
+        double2* tr = tl + (wb + 1);
+        // the entries of the two cut chunks that lie outside the band
+        double lr = 0.0, li = 0.0, rr = 0.0, ri = 0.0;
+        for (int k = cl * NB + lane; k < nout; k += WAVE) { const double2 v = x[k]; lr += v.x; li += v.y; }
+        for (int k = nVP + wb + lane; k < min(cr * NB, 2 * nVP); k += WAVE) { const double2 v = x[k]; rr += v.x; ri += v.y; }
+        double cl_r = far[0][pl].x + wave_sum(lr), cl_i = far[0][pl].y + wave_sum(li);
+        double cr_r = far[1][pl].x + wave_sum(rr), cr_i = far[1][pl].y + wave_sum(ri);
+        if (lane == 0) { tl[0] = cmk(cl_r, cl_i); tr[wb] = cmk(cr_r, cr_i); }
+        // the band: left from its outer end inwards, right from ω = wb - 1 down to 0
+        for (int c0 = 0; c0 < wb; c0 += WAVE) {
+            const int i = c0 + lane;
+            const double2 a = i < wb ? x[nout + i] : czero();
+            const double2 b = i < wb ? x[nVP + wb - 1 - i] : czero();
+            double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+            wave_incl_scan(ar, ai, lane);
+            wave_incl_scan(br, bi, lane);
+            if (i < wb) {
+                tl[i + 1] = cmk(cl_r + ar, cl_i + ai);
+                tr[wb - 1 - i] = cmk(cr_r + br, cr_i + bi);
+            }
+            cl_r += __shfl(ar, WAVE - 1, WAVE); cl_i += __shfl(ai, WAVE - 1, WAVE);
+            cr_r += __shfl(br, WAVE - 1, WAVE); cr_i += __shfl(bi, WAVE - 1, WAVE);
         }
-#pragma unroll
-        for (int k = 0; k < NWV; ++k) { cl_r += sh[0][k]; cl_i += sh[1][k]; cr_r += sh[2][k]; cr_i += sh[3][k]; }
     }
 }
 // far sums of one row outside the window [lo, hi]: Σ_{k < lo} and Σ_{k > hi}

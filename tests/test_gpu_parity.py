@@ -141,6 +141,35 @@ def test_full_iteration_band_narrower_than_bubble_mesh(case, strategy, fdga_mod)
         assert relerr(x, y) < TOL, f"{case} {strategy} iteration {it}: {relerr(x, y):.3e}"
 
 
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_band_stored_bubble_read_back(graph, fdga_mod):
+    """The iteration's lattice bubble stores Πpp/Πph only on the 16-ω chunks that hold the band
+    (mΠν = 6: the band is narrower than the bubble mesh); reading them back recomputes the whole
+    arrays from the same G, and is refused once G has changed (Dyson after the SDE's Σ update)."""
+    from fdga import _lib as L
+    box = dict(BOX, mPinu=6, nG=40)
+    S, O = make_pair("nl_L16", seed=12, box=box)
+    S.set_graph(graph)
+    for it in range(2):
+        fdga_mod.iterate_solver(S, "fdPA")
+        O.iterate(po.FDPA)
+    S.sync()
+    for hw, ow in ((L.PIPP, po.PIPP), (L.PIPH, po.PIPH)):
+        _cmp(S, O, hw, ow, what=f"array {hw} after two iterations")
+    assert relerr(fdga_mod.flatten(S), O.flatten()) < TOL
+    fdga_mod.iterate_solver(S, "fdPA")
+    O.iterate(po.FDPA)
+    fdga_mod.Dyson(S)
+    O.dyson()
+    with pytest.raises(fdga_mod.FdgaError, match="band only"):
+        S.get(L.PIPP)
+    fdga_mod.bubbles(S)
+    O.bubbles()
+    for hw, ow in ((L.PIPH, po.PIPH), (L.PIPP, po.PIPP)):
+        _cmp(S, O, hw, ow, what=f"array {hw} after Dyson and bubbles")
+    S.close()
+
+
 def test_flatten_roundtrip_and_fixed_point(fdga_mod):
     S, O = make_pair("nl_L2", seed=7)
     x = fdga_mod.flatten(S)
