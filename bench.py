@@ -238,15 +238,25 @@ def build_c2(precision, w=C2, seed=0):
     return S
 
 
-def bubble_traffic():
-    """HBM bytes per launch of k_bubble_nl on the 16 x 16 workload (FETCH_SIZE x 2 + WRITE_SIZE, the factors
-    calibrated against a 512 MiB copy) as recorded in profiles/c2_bubble_nl_traffic.txt; None if absent."""
+def recorded_traffic(name, key="hbm_bytes_per_launch"):
+    """HBM bytes per launch recorded in profiles/<name> (rocprofv3 --pmc FETCH_SIZE x 2 + WRITE_SIZE x 1, the
+    factors calibrated against a 512 MiB copy); None if absent."""
     import re
-    f = os.path.join(ROOT, "profiles", "c2_bubble_nl_traffic.txt")
+    f = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(f):
         return None
-    m = re.search(r"^hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(f).read(), re.M)
+    m = re.search(rf"^{key}\s*=\s*([0-9.eE+]+)", open(f).read(), re.M)
     return float(m.group(1)) if m else None
+
+
+def c2_band(w):
+    """(wb, number of 16-ω chunks the lattice bubble stores) of the lattice workload: the band half-width
+    wb = min(nVP, NK + max(nK1 - 1, nK2b - 1 + nK2f) + 3) with NK = 1 + the largest box of F and its
+    reference vertex, and the chunks of 16 bubble frequencies that meet [nVP - wb, nVP + wb)."""
+    nVP = w["nK1"] * w["mPinu"]
+    NK = 1 + max(w["nK1"], w["nK2"][0], w["nK2"][1], w["nK3"][0], w["nK3"][1])
+    wb = min(nVP, NK + max(w["nK1"] - 1, w["nK2"][0] - 1 + w["nK2"][1]) + 3)
+    return wb, (nVP + wb + 15) // 16 - (nVP - wb) // 16
 
 
 def main_c2(args):
@@ -286,9 +296,13 @@ def main_c2(args):
         + ", ".join(f"{k}={v:.2e}" for k, v in errs.items()))
     head = "fp32" if max(errs.values()) <= 1e-6 else "fp64"
     P = paths[head]
-    b_bubble = 16.0 * (2 * NWP * NVP * NP + 2 * w["nG"] * NP)
+    # k_bubble_nl as built: G(R) read once, Πpp and Πph stored on the band chunks, the chunk sums of
+    # all 16-ω chunks written (the far regions enter the later stages only through those sums)
+    wb, nchunk = c2_band(w)
+    b_bubble = 16.0 * (2 * w["nG"] * NP + 2 * nchunk * 16 * NWP * NP + 2 * (NVP // 16) * NWP * NP)
     t_bub = P["st"]["bubble_kernel"] * 1e-3
     gemm_peak = FP32_MFMA_PEAK if head == "fp32" else FP64_MFMA_PEAK
+    gemm_traffic = recorded_traffic("c2_bse_gemm_traffic.txt", f"{head}_hbm_bytes_per_launch")
     out = {
         "metric": "parquet iters/s + BSE TFLOP/s, FP32-MFMA vs FP64 GPU @ rel.err<=1e-6 vs FP64",
         "value": 1000.0 / P["ms"],
@@ -315,15 +329,20 @@ def main_c2(args):
                             "bse_tflops_dense_equiv": paths["fp64"]["bse_tflops_dense_equiv"],
                             "stage_ms": paths["fp64"]["st"]},
                    "speedup_fp32_over_fp64": paths["fp64"]["ms"] / paths["fp32"]["ms"]},
-        "roofline": {"bound": "hbm", "achieved": b_bubble / t_bub / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": b_bubble / t_bub / HBM_PEAK, "traffic": bubble_traffic(),
-                     "kernel": "k_bubble_nl (bubbles_real_space!: Πpp, Πph of one iteration)",
-                     "algorithmic_bytes_per_launch": b_bubble, "launch_ms": t_bub * 1e3},
-        "roofline_bse_gemm": {"bound": "mfma", "achieved": P["f_exec"] / P["t_gemm"] / 1e12, "peak": gemm_peak / 1e12,
-                              "unit": "TFLOP/s", "frac": P["f_exec"] / P["t_gemm"] / gemm_peak,
-                              "kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA, BSE_L_K3! + "
-                                        "BSE_K3! launches of one iteration)",
-                              "executed_flops": P["f_exec"], "launch_ms": P["t_gemm"] * 1e3},
+        # the dominant kernel: the BSE contraction GEMMs (four launches per iteration: BSE_L_K3! p+a and t,
+        # BSE_K3! p+a and t), executed flops over their summed durations (HIP events, one at a time)
+        "roofline": {"bound": "mfma", "achieved": P["f_exec"] / P["t_gemm"] / 1e12, "peak": gemm_peak / 1e12,
+                     "unit": "TFLOP/s", "frac": P["f_exec"] / P["t_gemm"] / gemm_peak, "traffic": gemm_traffic,
+                     "kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA, BSE_L_K3! + BSE_K3!, "
+                               "4 launches per iteration)",
+                     "executed_flops_per_iteration": P["f_exec"], "launches_per_iteration": 4,
+                     "ms_per_iteration": P["t_gemm"] * 1e3},
+        "roofline_bubble": {"bound": "hbm", "achieved": b_bubble / t_bub / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                            "frac": b_bubble / t_bub / HBM_PEAK,
+                            "traffic": recorded_traffic("c2_bubble_nl_traffic.txt"),
+                            "kernel": "k_bubble_nl (bubbles_real_space!: Πpp, Πph stored on the band chunks, chunk sums)",
+                            "algorithmic_bytes_per_launch": b_bubble, "band_halfwidth": wb, "stored_chunks": nchunk,
+                            "launch_ms": t_bub * 1e3},
     }
     if not args.no_cpu_baseline:
         log("bench c2: timing the CPU baseline (oracle, bounded sample, 1 thread and all cores)")

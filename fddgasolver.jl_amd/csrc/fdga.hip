@@ -67,6 +67,9 @@ struct fdga_ctx {
     // Πpp (bit 0) / Πph (bit 1) stored only on the band by the last lattice bubble; pi_band_G: G is still
     // the one that bubble read (fdga_get can recompute the whole array)
     int pi_band = 0, pi_band_G = 0;
+    // channels (bit ch) whose FP32 BSE_K3! right operand was written by the last BSE_L_K3! epilogue
+    // from the current Π, caches and FL (cleared by anything that changes them)
+    int r32_fresh = 0;
     // the stream the stage kernels are launched on: st, or st2 for the second chain of an iteration
     hipStream_t run = nullptr;
     hipStream_t st2 = nullptr;
@@ -389,6 +392,7 @@ void kernel_end(fdga_ctx* c, int s);
 // chunk sums); fdga_get of Πpp/Πph then recomputes the whole array from the same G.
 int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc, bool band = false) {
     const fdga_dims& d = c->d;
+    c->r32_fresh = 0;
     if (!d.nonlocal) {
         dim3 g(cdiv(c->NVP, 256), c->NWP);
         kernel_begin(c);
@@ -556,8 +560,10 @@ int sde_hartree(fdga_ctx* c, double2* S, const double2* G, double sign) {
     return 0;
 }
 
-// Lpp/Lph of one level; `zero` selects the reference system's groups SG0pp2/SG0ph2
-int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* Ph, bool zero) {
+// Lpp/Lph of one level (lattice solver: plus a second Vertex level `lv2` of the same scale, summed
+// in the same pass); `zero` selects the reference system's groups SG0pp2/SG0ph2
+int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* Ph, bool zero,
+                const Level* lv2 = nullptr) {
     // the bubbles' tail sums are kept current by bubbles_into and fdga_set
     const fdga_dims& d = c->d;
     const size_t nout = (size_t)c->NW2 * c->NV2 * c->NP;
@@ -568,9 +574,18 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
     const double2* Tpp = pi_tail(c, Pp);
     const double2* Tph = pi_tail(c, Ph);
     if (d.nonlocal && lv.kind == 0) {   // lattice solver, Vertex level: one wave per (Ω, P)
-        k_sde_L_nl<<<cdiv((size_t)c->NW2 * c->NP * WAVE, 256), 256, 0, c->run>>>(
-            c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, lv.g[0], lv.g[1], lv.g[2], d.nK2b, d.nK2f, d.nK1, c->nVP, c->wb, c->NP,
-            lv.chanNP, TU);
+        if (c->NV2 > WAVE * SDE_NL_ROUNDS)
+            return fail(c, "lattice SDE: at most " + std::to_string(WAVE * SDE_NL_ROUNDS / 2) + " fermionic K2 frequencies");
+        const unsigned nb = (unsigned)cdiv((size_t)c->NW2 * c->NP * WAVE, 256);
+        const SdeLevelNL l0{lv.g[0], lv.g[1], lv.g[2], lv.chanNP};
+        if (lv2) {
+            const SdeLevelNL l1{lv2->g[0], lv2->g[1], lv2->g[2], lv2->chanNP};
+            k_sde_L_nl<2><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f, d.nK1, c->nVP,
+                                                  c->wb, c->NP, TU);
+        } else {
+            k_sde_L_nl<1><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l0, d.nK2b, d.nK2f, d.nK1, c->nVP,
+                                                  c->wb, c->NP, TU);
+        }
         LAUNCHCHK(c);
         // SGpp2 / SGph2 (SG0pp2 / SG0ph2 for the reference system): members from representatives
         const int idp = zero ? SG_0PP2 : SG_PP2, idh = zero ? SG_0PH2 : SG_PH2;
@@ -608,12 +623,17 @@ int sde_levels(fdga_ctx* c, double2* S, const double2* G, const double2* Pp, con
     }
     Level lr{1, {}, 1, 1.0 / 3.0};
     lv.push_back(lr);
+    // lattice solver: the target and the local reference vertex (two Vertex levels of scale 1) in one
+    // L pass and one Σ sum (Σ is linear in the L-vertices)
+    const bool merge01 = c->d.nonlocal && lv.size() >= 2 && lv[0].kind == 0 && lv[1].kind == 0 &&
+                         lv[0].scale == lv[1].scale;
     for (size_t i = 0; i < lv.size(); ++i) {
+        if (merge01 && i == 1) continue;
         const bool refl = lv[i].kind == 1;
         // a RefVertex with null tables contributes (U + 0) - U = 0 exactly
         bool u2c_ready = false;
         if (!(refl && !c->tabs_nonzero)) {
-            if (sde_level_L(c, lv[i], Pp, Ph, start == 1)) return -3;
+            if (sde_level_L(c, lv[i], Pp, Ph, start == 1, merge01 && i == 0 ? &lv[1] : nullptr)) return -3;
             if (i == 0 && !c->d.nonlocal) {   // this level's Σ sum and the U² term's C(m) in one launch
                 const SymCls ss = symcls(c, SG_S);
                 const size_t nsig = nclass(ss, c->NG), nu = (size_t)(4 * c->d.nG - 1);
@@ -885,7 +905,14 @@ struct LK3Set {
     LK3_Aop a;
     LK3_Bop b;
     LK3_Ep e;
+    LK3R_Ep er;   // the epilogue that also writes BSE_K3!'s FP32 right operand (when ro)
+    bool ro;
 };
+// the lattice solver's FP32 contraction without K3 symmetry groups: BSE_L_K3!'s epilogue, which
+// holds every element of FL.K3, also writes BSE_K3!'s right operand (no separate pass over FL.K3)
+static bool lk3_writes_r32(const fdga_ctx* c, bool sym) {
+    return c->d.precision == FDGA_FP32 && c->NP > 1 && !sym;
+}
 // operand loaders and epilogue of a BSE_L_K3! launch (batch = (slot NW3 + iΩ) NP... see K3Geom)
 static LK3Set lk3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) {
     LK3Chan cc[3] = {};
@@ -898,18 +925,34 @@ static LK3Set lk3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi
         cc[s].outa = c->FL[ACH].K3;
         cc[s].scale = c->d.T * ((ch == ACH) ? 1.0 : -1.0);   // src/BSE_templates.jl:122,130,138
         cc[s].tpost = (ch == TCH && t_epi);
+        if (lk3_writes_r32(c, sym)) {   // BSE_K3!'s FP32 right operand from the same epilogue
+            cc[s].R32 = reinterpret_cast<float2*>(c->R[ch]);
+            cc[s].Pi = pi_of(c, ch, false);
+            cc[s].Gam = c->cache[(ch == PCH) ? 0 : (ch == ACH) ? 10 : 11];
+            cc[s].s1 = (ch == ACH) ? 1.0 : -1.0;   // src/BSE_templates.jl:156,164,172
+            cc[s].s2 = (ch == TCH) ? -1.0 : 1.0;
+            cc[s].ch = ch;
+        }
     }
     const K3Geom g = k3geom(c);
-    return LK3Set{LK3_Aop{g, cc[0], cc[1], cc[2]}, LK3_Bop{g, cc[0], cc[1], cc[2]}, LK3_Ep{g, cc[0], cc[1], cc[2]}};
+    return LK3Set{LK3_Aop{g, cc[0], cc[1], cc[2]}, LK3_Bop{g, cc[0], cc[1], cc[2]}, LK3_Ep{g, cc[0], cc[1], cc[2]},
+                  LK3R_Ep{g, cc[0], cc[1], cc[2]}, lk3_writes_r32(c, sym)};
 }
 // one K3-class GEMM launch of `n` channels with the context's tile shape for it (which: 0 BSE_L_K3!,
 // 1 BSE_K3!), over the symmetry block list when there is one
+template <class AL, class BL, class EP>
+static hipError_t k3_gemm(fdga_ctx* c, const int* chans, int n, int which, const AL& a, const BL& b, const EP& e);
+// BSE_L_K3! launch with the set's epilogue
+static hipError_t lk3_gemm(fdga_ctx* c, const int* chans, int n, const LK3Set& ls) {
+    return ls.ro ? k3_gemm(c, chans, n, 0, ls.a, ls.b, ls.er) : k3_gemm(c, chans, n, 0, ls.a, ls.b, ls.e);
+}
 template <class AL, class BL, class EP>
 static hipError_t k3_gemm(fdga_ctx* c, const int* chans, int n, int which, const AL& a, const BL& b, const EP& e) {
     const int NV = c->NV3, tile = c->ktile[which];
     size_t nl = 0;
     const int* bl = k3_blocks(c, chans, n, which == 0, tile, nl);
-    const GemmSchedOne sc = sched_one(NV, NV, NV, cdiv(NV, tile_bm(tile)), cdiv(NV, tile_bn(tile)), n * c->NW3 * c->NP, bl, nl);
+    const GemmSchedOne sc = sched_one(NV, NV, NV, cdiv(NV, tile_bm(tile)), cdiv(NV, tile_bn(tile)), n * c->NW3 * c->NP, bl, nl,
+                                      c->NW3, c->NP);
     return zgemm_run(c->d.precision, tile, (size_t)sc.nitems, c->run, sc, a, b, e);
 }
 // Without symmetry groups a (p, a, t) contraction over many (Ω, P) batches runs as two launches, (p, a)
@@ -924,22 +967,28 @@ static const int kChansT[1] = {TCH};
 
 static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     const bool sym = k3_sym(c, chans, n, true);
+    int bits = 0;
+    for (int s = 0; s < n; ++s) bits |= 1 << chans[s];
+    c->r32_fresh &= ~bits;
     if (split_t(c, chans, n, sym)) {
         const LK3Set l1 = lk3_set(c, kChansPA, 2, false, false), l2 = lk3_set(c, kChansT, 1, false, true);
         kernel_begin(c);
-        if (k3_gemm(c, kChansPA, 2, 0, l1.a, l1.b, l1.e) != hipSuccess ||
-            k3_gemm(c, kChansT, 1, 0, l2.a, l2.b, l2.e) != hipSuccess)
+        if (lk3_gemm(c, kChansPA, 2, l1) != hipSuccess || lk3_gemm(c, kChansT, 1, l2) != hipSuccess)
             return fail(c, "BSE_L_K3 GEMM launch failed", -3);
         kernel_end(c, 9);
+        if (lk3_writes_r32(c, false)) c->r32_fresh |= bits;
         return 0;
     }
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const LK3Set ls = lk3_set(c, chans, n, sym, t_epi);
     kernel_begin(c);
-    if (k3_gemm(c, chans, n, 0, ls.a, ls.b, ls.e) != hipSuccess)
+    if (lk3_gemm(c, chans, n, ls) != hipSuccess)
         return fail(c, "BSE_L_K3 GEMM launch failed", -3);
     kernel_end(c, 9);
-    return k3_finish(c, c->FL, chans, n, true, sym, t_epi);
+    if (k3_finish(c, c->FL, chans, n, true, sym, t_epi)) return -3;
+    // a t channel post-processed after the launch has its R from the unpost-processed values
+    if (lk3_writes_r32(c, sym)) c->r32_fresh |= bits;
+    return 0;
 }
 
 // BSE_K3! operands: right-operand builders (k_k3_R) and the GEMM's loaders and epilogue
@@ -998,7 +1047,13 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, 
             float2* buf = reinterpret_cast<float2*>(c->R[ch]);
             rc[s].R32 = buf;
             kc[s].R32 = buf;
-            if (!kc[s].full) kc[s].A32 = buf + c->n3 * c->NP;
+            if (!kc[s].full) {
+                // A32 carries the rank-1 K2r(ν) term, so R needs no column sums
+                kc[s].A32 = buf + c->n3 * c->NP;
+                kc[s].K2row = c->csum[ch];   // the column-sum buffer is free on this path
+                kc[s].csum = nullptr;
+                rc[s].csum = nullptr;
+            }
         }
     }
     return ks;
@@ -1006,10 +1061,12 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, 
 // the right operands R and their column sums of a BSE_K3! launch over `n` channels
 // With `sym` (BSE_K3! symmetry groups installed) the lattice solver builds the operands only at the
 // momenta whose batches hold class representatives: the contraction reads no other.
-static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n, bool sym = false) {
+// skipR: the FP32 right operands are already there (written by BSE_L_K3!'s epilogue)
+static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n, bool sym = false, bool skipR = false) {
     const int* pl = (sym && c->k3_pl) ? c->k3_pl : nullptr;
     const int npl = pl ? c->k3_npl : c->NP;
-    if (c->NV3 <= WAVE && c->NP > 1) {   // lattice: P-independent operands loaded once per momentum chunk
+    if (skipR) {
+    } else if (c->NV3 <= WAVE && c->NP > 1) {   // lattice: P-independent operands loaded once per momentum chunk
         const size_t nwave = (size_t)cdiv(c->NV3, K3R_COLS) * c->NW3 * cdiv(npl, K3_PCH) * n;
         k_k3_R_pchunk<1, K3R_COLS><<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n, pl,
                                                                                 npl);
@@ -1021,7 +1078,11 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n, bool sym = false) {
         k_k3_R<<<cdiv(nwave * WAVE, 256), 256, 0, c->run>>>(ks.g, ks.rc[0], ks.rc[1], ks.rc[2], n);
     }
     LAUNCHCHK(c);
+    if (!skipR)
+        for (int s = 0; s < n; ++s) c->r32_fresh &= ~(1 << ks.kc[s].ch);
     if (ks.op32()) {
+        k_k2_rows<<<cdiv((size_t)c->NV3 * c->NW3 * c->NP * n, 256), 256, 0, c->run>>>(ks.g, ks.kc[0], ks.kc[1], ks.kc[2], n);
+        LAUNCHCHK(c);
         k_k3_A32<<<dim3((unsigned)(c->NW3 * cdiv(npl, K3_PCH) * n), cdiv((size_t)c->NV3 * c->NV3, 256)), 256, 0, c->run>>>(
             ks.g, ks.kc[0], ks.kc[1], ks.kc[2], pl, npl);
         LAUNCHCHK(c);
@@ -1041,8 +1102,11 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n, bool mfrg = false) {
         return fail(c, "BSE_K3!(S, Ch, Val(true)) of an NL_ParquetSolver reads the F caches of build_K3_cache_mfRG!(S, true); "
                        "the device keeps only the momentum-independent part of build_K3_cache!'s");
     const bool sym = k3_sym(c, chans, n, false);
+    // the right operands from BSE_L_K3!'s epilogue when it wrote them for every channel of the launch
+    bool fresh = !mfrg && !sym && lk3_writes_r32(c, false) && !c->caches_mfrg;
+    for (int s = 0; s < n; ++s) fresh = fresh && (c->r32_fresh >> chans[s] & 1);
     if (split_t(c, chans, n, sym)) {
-        if (k3_rhs(c, k3_set(c, chans, n, false, false, mfrg), n)) return -3;
+        if (k3_rhs(c, k3_set(c, chans, n, false, false, mfrg), n, false, fresh)) return -3;
         kernel_begin(c);
         if (k3_gemm_set(c, kChansPA, 2, k3_set(c, kChansPA, 2, false, false, mfrg)) != hipSuccess ||
             k3_gemm_set(c, kChansT, 1, k3_set(c, kChansT, 1, false, true, mfrg)) != hipSuccess)
@@ -1052,7 +1116,7 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n, bool mfrg = false) {
     }
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const K3Set ks = k3_set(c, chans, n, sym, t_epi, mfrg);
-    if (k3_rhs(c, ks, n, sym)) return -3;
+    if (k3_rhs(c, ks, n, sym, fresh)) return -3;
     kernel_begin(c);
     if (k3_gemm_set(c, chans, n, ks) != hipSuccess) return fail(c, "BSE_K3 GEMM launch failed", -3);
     kernel_end(c, 10);
@@ -1376,6 +1440,7 @@ static void invalidate(fdga_ctx* c, int which) {
 
 int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
     if (!c || !host) return -1;
+    c->r32_fresh = 0;
     ArrInfo a;
     if (arr_info(c, which, a)) return fail(c, "unknown array id " + std::to_string(which));
     if (which == FDGA_LPP || which == FDGA_LPH || a.kind == 3 || a.kind == 4) return fail(c, "array is read-only");
@@ -1435,6 +1500,7 @@ int fdga_get(fdga_ctx* c, int32_t which, double* host) {
 
 int fdga_init(fdga_ctx* c) {
     if (!c) return -1;
+    c->r32_fresh = 0;
     if (bubbles_into(c, c->Pi0pp, c->Pi0ph, c->G0)) return -3;
     HIPCHK(c, hipMemcpyAsync(c->Sig, c->Sig0, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
     k_dyson<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(c->G, c->Sig, c->Gbare, c->nGel);
@@ -1656,6 +1722,7 @@ int fdga_dyson(fdga_ctx* c) {
 int fdga_bubbles(fdga_ctx* c) { return bubbles_into(c, c->Pipp, c->Piph, c->G, true); }
 
 int fdga_build_k3_cache(fdga_ctx* c) {
+    c->r32_fresh = 0;
     c->caches_mfrg = 0;
     CachePtrs cp;
     for (int k = 0; k < 12; ++k) cp.c[k] = c->cache[k];
@@ -1708,6 +1775,7 @@ int fdga_bse_k2_new(fdga_ctx* c, int32_t ch, int32_t is_mfrg) { return launch_fo
 
 int fdga_build_k3_cache_mfrg(fdga_ctx* c, int32_t is_first) {
     if (!c) return -1;
+    c->r32_fresh = 0;
     if (!c->d.nonlocal)
         return fail(c, "build_K3_cache_mfRG! has no method for ParquetSolver (src/nonlocal_2/build_K3_cache.jl:97-99 "
                        "defines it for NL_ParquetSolver and NL2_ParquetSolver)");
@@ -2176,9 +2244,9 @@ int fdga_autotune(fdga_ctx* c, int32_t reps) {
     if (fdga_build_k3_cache(c)) return -3;
     const bool symL = k3_sym(c, all, 3, true), symK = k3_sym(c, all, 3, false);
     LK3Set ls = lk3_set(c, all, 3, symL, false);
-    ls.e.c0.out = c->R[0];   // scratch outputs
-    ls.e.c1.out = c->R[1];
-    ls.e.c2.out = c->R[2];
+    ls.e.c0.out = ls.er.c0.out = c->R[0];   // scratch outputs
+    ls.e.c1.out = ls.er.c1.out = c->R[1];
+    ls.e.c2.out = ls.er.c2.out = c->R[2];
     hipEvent_t a, b;
     HIPCHK(c, hipEventCreate(&a));
     HIPCHK(c, hipEventCreate(&b));
@@ -2197,7 +2265,7 @@ int fdga_autotune(fdga_ctx* c, int32_t reps) {
     int bt = c->ktile[0];
     for (int t = 0; t < TILE_N; ++t) {
         c->ktile[0] = t;
-        const float ms = timeit([&] { return k3_gemm(c, all, 3, 0, ls.a, ls.b, ls.e); });
+        const float ms = timeit([&] { return lk3_gemm(c, all, 3, ls); });
         if (ms < best) { best = ms; bt = t; }
     }
     c->ktile[0] = bt;
@@ -2272,6 +2340,8 @@ struct fdga_group {
     LK3_Aop* lk3a = nullptr;
     LK3_Bop* lk3b = nullptr;
     LK3_Ep* lk3e = nullptr;
+    LK3R_Ep* lk3er = nullptr;   // used instead of lk3e when lk3_ro (every member writes its FP32 R there)
+    int lk3_ro = 0;
     void* k3a[2] = {nullptr, nullptr};   // K3_AopT<true> (local), K3_AopT<false> or K3_Aop32 (lattice FP64 / FP32)
     void* k3b[2] = {nullptr, nullptr};   // K3_Bop or K3_Bop32
     K3_Ep* k3e[2] = {nullptr, nullptr};
@@ -2370,6 +2440,7 @@ int group_build_functors(fdga_group* g) {
     std::vector<LK3_Aop> la;
     std::vector<LK3_Bop> lb;
     std::vector<LK3_Ep> le;
+    std::vector<LK3R_Ep> ler;
     std::vector<K3_AopT<true>> af[2];
     std::vector<K3_AopT<false>> an[2];
     std::vector<K3_Aop32> a32[2];
@@ -2381,6 +2452,8 @@ int group_build_functors(fdga_group* g) {
         la.push_back(ls.a);
         lb.push_back(ls.b);
         le.push_back(ls.e);
+        ler.push_back(ls.er);
+        g->lk3_ro = (c == g->m[0] ? 1 : g->lk3_ro) && ls.ro;
         for (int k = 0; k < 2; ++k) {
             const int s = c->swap_state;
             const K3Set ks = k3_set(c, all, 3, false, g->nonlocal);
@@ -2393,7 +2466,9 @@ int group_build_functors(fdga_group* g) {
             swap_F_host(c);   // the other state; two swaps restore the context
         }
     }
-    if (group_upload(g, &g->lk3a, la) || group_upload(g, &g->lk3b, lb) || group_upload(g, &g->lk3e, le)) return -2;
+    if (group_upload(g, &g->lk3a, la) || group_upload(g, &g->lk3b, lb) ||
+        (g->lk3_ro ? group_upload(g, &g->lk3er, ler) : group_upload(g, &g->lk3e, le)))
+        return -2;
     const bool op32 = g->nonlocal && g->precision == FDGA_FP32;
     for (int s = 0; s < 2; ++s) {
         int rc = op32 ? group_upload(g, (K3_Aop32**)&g->k3a[s], a32[s])
@@ -2425,6 +2500,9 @@ size_t group_grid(const fdga_group* g, int which, bool t_launch = false) {
 // one launch of the grouped contraction `which` (0 BSE_L_K3!, 1 BSE_K3!) over the item list `sc`
 hipError_t group_gemm_launch(fdga_group* g, int which, const GemmSchedGroup& sc, size_t grid) {
     const int tile = g->tile[which];
+    if (which == 0 && g->lk3_ro)
+        return zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<LK3_Aop>{g->lk3a, g->boff, g->n},
+                         GroupOp<LK3_Bop>{g->lk3b, g->boff, g->n}, GroupOp<LK3R_Ep>{g->lk3er, g->boff, g->n});
     if (which == 0)
         return zgemm_run(g->precision, tile, grid, g->st, sc, GroupOp<LK3_Aop>{g->lk3a, g->boff, g->n},
                          GroupOp<LK3_Bop>{g->lk3b, g->boff, g->n}, GroupOp<LK3_Ep>{g->lk3e, g->boff, g->n});
@@ -2512,7 +2590,9 @@ int group_iterate_eager(fdga_group* g, int strategy, int update_sigma) {
         rc = rc ? rc : launch_k2_all(c);
         c->w_fresh = 0;
         if (rc) return member_err(g, c);
-        if (k3_rhs(c, k3_set(c, all, 3, false, false), 3)) return member_err(g, c);
+        // fdPA: the grouped BSE_L_K3! epilogue wrote the FP32 lattice right operands
+        const bool skipR = strategy == FDGA_FDPA && g->lk3_ro;
+        if (k3_rhs(c, k3_set(c, all, 3, false, false), 3, false, skipR)) return member_err(g, c);
     }
     if (group_join(g) || group_gemm(g, 1) || group_fork(g)) return -3;
     for (fdga_ctx* c : g->m) {
@@ -2601,7 +2681,7 @@ int fdga_group_destroy(fdga_group* g) {
         if (gi.items_t) (void)hipFree(gi.items_t);
         if (gi.geo) (void)hipFree(gi.geo);
     }
-    for (void* p : {(void*)g->boff, (void*)g->lk3a, (void*)g->lk3b, (void*)g->lk3e, g->k3a[0], g->k3a[1], g->k3b[0],
+    for (void* p : {(void*)g->boff, (void*)g->lk3a, (void*)g->lk3b, (void*)g->lk3e, (void*)g->lk3er, g->k3a[0], g->k3a[1], g->k3b[0],
                     g->k3b[1], (void*)g->k3e[0], (void*)g->k3e[1]})
         if (p) (void)hipFree(p);
     for (auto e : g->ev_m)

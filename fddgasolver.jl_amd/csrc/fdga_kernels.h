@@ -623,6 +623,10 @@ struct GemmWork { int batch, m0, n0, M, N, K; };
 struct GemmSchedOne {
     int M, N, K, tiles_m, tiles_n, nbatch, nitems;
     const int* blist;
+    // pnp > 1: the batches are b = (slot pnw + iΩ) pnp... in memory order (slot pnp + P) pnw + iΩ, and
+    // they are visited momentum-fastest, so that the workgroups an XCD runs at a time share a few Ω
+    // and their momentum-independent operands stay in that XCD's L2
+    int pnw, pnp;
     __device__ __forceinline__ bool item(int it, int BM, int BN, GemmWork& w) const {
         const int tiles = tiles_m * tiles_n;
         int bid;
@@ -632,7 +636,11 @@ struct GemmSchedOne {
             bid = it;
         } else {
             const int r = it >> 3;
-            const int bt = (r / tiles) * 8 + (it & 7);
+            int bt = (r / tiles) * 8 + (it & 7);
+            if (bt < nbatch && pnp > 1) {
+                const int per = pnw * pnp, sl = bt / per, q = bt - sl * per;
+                bt = sl * per + (q % pnp) * pnw + q / pnp;
+            }
             bid = bt < nbatch ? bt * tiles + r % tiles : -1;
         }
         if (bid < 0) return false;
@@ -850,9 +858,10 @@ inline hipError_t zgemm_run(int precision, int tile, size_t grid, hipStream_t st
 // launch helper of the single-problem GEMMs (64 x 64 tiles, one workgroup per item): without a
 // block list the items cover nbatch batches of tm x tn tiles (rounded up to a multiple of 8
 // batches); with one, nlist workgroups run the listed blocks.
-inline GemmSchedOne sched_one(int M, int N, int K, int tm, int tn, int nbatch, const int* blist, size_t nlist) {
+inline GemmSchedOne sched_one(int M, int N, int K, int tm, int tn, int nbatch, const int* blist, size_t nlist,
+                              int pnw = 1, int pnp = 1) {
     const size_t nitems = blist ? nlist : nbatch < 8 ? (size_t)nbatch * tm * tn : (size_t)((nbatch + 7) / 8) * 8 * tm * tn;
-    return GemmSchedOne{M, N, K, tm, tn, nbatch, (int)nitems, blist};
+    return GemmSchedOne{M, N, K, tm, tn, nbatch, (int)nitems, blist, pnw, pnp};
 }
 template <class AL, class BL, class EP>
 inline hipError_t zgemm_launch(int precision, int nbatch, hipStream_t st, int M, int N, int K, int tm, int tn,
@@ -889,6 +898,15 @@ struct LK3Chan {
     const double2* outa;   // FL.γa.K3 (t channel, in-launch post-processing)
     double scale;          // T * s
     int tpost;             // t channel post-processing inside the epilogue
+    // FP32 contraction of the lattice solver: the epilogue also writes BSE_K3!'s right operand
+    //   R(ω, νp) = s1 [(Π-Π0)(ω) F0r[ω, νp] + Π(ω) Γ~[ω, νp]] + s2 Π(ω) L~(ω, νp) 1(ω~ in box)
+    // rounded to FP32 (L~ = the unpost-processed result at row ω~, t: 2 γt - γa of the finished
+    // values); R32 null: FL.γr.K3 only
+    float2* R32;
+    const double2* Pi;     // Πpp or Πph
+    const double2* Gam;    // p: Γpx [ω, νp, Ω]; a, t: Γr transposed [ω, ν, Ω]
+    double s1, s2;
+    int ch;
 };
 // Loaders and epilogues resolve their batch (channel slot, Ω, P) once per workgroup with bind(b),
 // which returns base pointers for that batch; the per-element calls do only the in-matrix offset.
@@ -930,27 +948,97 @@ struct LK3_Bop {
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
-struct LK3_Ep {
-    K3Geom g; LK3Chan c0, c1, c2;
-    struct Bound {
-        double2* out; const double2* outa; double scale; int tpost; int NV;
-        __device__ __forceinline__ void operator()(int i, int j, double2 v) const {
-            const size_t o = (size_t)i + (size_t)NV * j;
-            double2 x = cscale(v, scale);
-            if (tpost) x = cscale(cadd(x, outa[o]), 0.5);
-            out[o] = x;
+// BSE_L_K3! epilogue.  RO: also BSE_K3!'s FP32 right operand (LK3Chan::R32); the plain form keeps
+// the epilogue's registers low for the other paths.
+// (no FMA contraction in these epilogues: the t post-processing (γt + γa)/2 of the scaled value must
+// round as the separate post-processing pass does)
+struct LK3BoundPlain {
+    double2* out; const double2* outa; double scale; int tpost; int NV;
+    __device__ __forceinline__ void operator()(int i, int j, double2 v) const {
+#pragma clang fp contract(off)
+        const size_t o = (size_t)i + (size_t)NV * j;
+        double xr = v.x * scale, xi = v.y * scale;
+        if (tpost) {
+            const double2 a = outa[o];
+            xr = (xr + a.x) * 0.5;
+            xi = (xi + a.y) * 0.5;
         }
-    };
+        out[o] = cmk(xr, xi);
+    }
+};
+struct LK3BoundR {
+    double2* out; const double2* outa; double scale; int tpost; int NV;
+    float2* R32; const double2* pi; const double2* pi0; const double2* F0r; const double2* Gam;
+    double s1, s2; int m, n3f, ch;
+    // The element at row i is L~ at the R row whose crossing is row i (p: ω = Ω - ν - 1, an
+    // involution; a, t: ω = ν).  For p, the R row ω = ν itself has no L~ term when its crossing
+    // lies outside the box; that row is written here instead (every R row exactly once).
+    __device__ __forceinline__ int r_row(int i, bool& hasL) const {
+        hasL = true;
+        if (ch != PCH) return i;
+        const int wc = m - (i - n3f) - 1;
+        hasL = in_f(wc, n3f);
+        return hasL ? wc + n3f : i;
+    }
+    __device__ __forceinline__ void operator()(int i, int j, double2 v) const {
+#pragma clang fp contract(off)
+        double xr = v.x * scale, xi = v.y * scale;
+        bool hasL;
+        const int k = r_row(i, hasL);
+        const size_t kj = (size_t)k + (size_t)NV * j;
+        struct { double2 f0, gm, P1, P0, oa; } q;
+        q.f0 = F0r[kj];
+        q.gm = Gam[kj];
+        q.P1 = pi[k];
+        q.P0 = pi0[k];
+        q.oa = tpost ? outa[(size_t)i + (size_t)NV * j] : czero();
+        // r = s1 [(P1 - P0) f0 + P1 gm] + s2 P1 x, written out so that every launch rounds it alike
+        const double dr = q.P1.x - q.P0.x, di = q.P1.y - q.P0.y;
+        double rr = (dr * q.f0.x - di * q.f0.y) + (q.P1.x * q.gm.x - q.P1.y * q.gm.y);
+        double ri = (dr * q.f0.y + di * q.f0.x) + (q.P1.x * q.gm.y + q.P1.y * q.gm.x);
+        rr *= s1;
+        ri *= s1;
+        if (hasL) {
+            rr += s2 * (q.P1.x * xr - q.P1.y * xi);
+            ri += s2 * (q.P1.x * xi + q.P1.y * xr);
+        }
+        R32[(size_t)k + (size_t)NV * j] = make_float2((float)rr, (float)ri);
+        if (tpost) {
+            xr = (xr + q.oa.x) * 0.5;
+            xi = (xi + q.oa.y) * 0.5;
+        }
+        out[(size_t)i + (size_t)NV * j] = cmk(xr, xi);
+    }
+};
+template <bool RO>
+struct LK3_EpT {
+    K3Geom g; LK3Chan c0, c1, c2;
+    typedef typename std::conditional<RO, LK3BoundR, LK3BoundPlain>::type Bound;
     __device__ __forceinline__ Bound one(const LK3Chan& c, int bb) const {
-        const int NV = g.NV();
-        const size_t off = (size_t)NV * NV * bb;
-        return Bound{c.out + off, c.tpost ? c.outa + off : nullptr, c.scale, c.tpost, NV};
+        const int NV = g.NV(), NW = g.NW();
+        const size_t nn = (size_t)NV * NV, off = nn * bb;
+        Bound r;
+        r.out = c.out + off;
+        r.outa = c.tpost ? c.outa + off : nullptr;
+        r.scale = c.scale; r.tpost = c.tpost; r.NV = NV;
+        if constexpr (RO) {
+            const int iW = bb % NW, p = bb / NW, m = iW - (g.n3b - 1);
+            r.R32 = c.R32 + off;
+            r.pi = c.Pi + g.pidx(m, -g.n3f, p);
+            r.pi0 = c.Pi0 + g.pidx(m, -g.n3f, p);
+            r.F0r = c.F0r + nn * iW;
+            r.Gam = c.Gam + nn * iW;
+            r.s1 = c.s1; r.s2 = c.s2; r.m = m; r.n3f = g.n3f; r.ch = c.ch;
+        }
+        return r;
     }
     __device__ __forceinline__ Bound bind(int b) const {
         const int nb = g.NW() * g.NP, s = b / nb, bb = b - s * nb;
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
+typedef LK3_EpT<false> LK3_Ep;
+typedef LK3_EpT<true> LK3R_Ep;
 
 // --- BSE_K3!:
 //   A(ν, ω)  = cache_Fr = Fbase_r[ν, ω, Ω] + K2r(Ω, ν, P) + K3r(Ω, ν, ω, P)   (t: 2·(t) - (a))
@@ -975,6 +1063,9 @@ struct K3Chan {
     // element, R32 = the right operand), each rounded as the FP32 GEMM would round it
     float2* A32;
     const float2* R32;
+    // A32 holds the K2r(ν) term as well (csum null: no rank-1 term in the epilogue), read from
+    // K2row[i + Nν3 (iΩ + NΩ3 P)] = K2r(Ω, ν = i - n3f, P) (t: 2 K2t - K2a), written by k_k2_rows
+    double2* K2row;
 };
 // FULL: the cache holds the whole cache_Fr (one momentum, a single load per element)
 template <bool FULL>
@@ -1077,28 +1168,47 @@ __global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, 
     const bool t = c.ch == TCH;
     const double2 x = c.Fbase[(size_t)o + (size_t)nn * iW];
     const int pend = min(p0 + K3_PCH, npl);
+    const int i = o % NV;   // the element's row ν = i - n3f
     for (int pq = p0; pq < pend; pq += 4) {
-        double2 k3[4], ka[4];
+        double2 k3[4], ka[4], kr[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int i = pq + u;
-            if (i < pend) {
-                const int p = pl ? pl[i] : i;
+            const int ip = pq + u;
+            if (ip < pend) {
+                const int p = pl ? pl[ip] : ip;
                 const size_t r = (size_t)o + (size_t)nn * (iW + (size_t)NW * p);
                 k3[u] = c.K3r[r];
                 if (t) ka[u] = c.K3a[r];
+                kr[u] = c.K2row ? c.K2row[(size_t)i + (size_t)NV * (iW + (size_t)NW * p)] : czero();
             }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int i = pq + u;
-            if (i < pend) {
-                const int p = pl ? pl[i] : i;
-                const double2 v = t ? cadd(x, csub(cscale(k3[u], 2.0), ka[u])) : cadd(x, k3[u]);
+            const int ip = pq + u;
+            if (ip < pend) {
+                const int p = pl ? pl[ip] : ip;
+                double2 v = t ? cadd(x, csub(cscale(k3[u], 2.0), ka[u])) : cadd(x, k3[u]);
+                if (c.K2row) v = cadd(v, kr[u]);
                 c.A32[(size_t)o + (size_t)nn * (iW + (size_t)NW * p)] = make_float2((float)v.x, (float)v.y);
             }
         }
     }
+}
+// K2row of the FP32 lattice A operand (K3Chan::K2row) for every channel slot: one thread per
+// (ν, Ω, P) of the K3 box, coalesced writes along ν
+__global__ __launch_bounds__(256) void k_k2_rows(K3Geom g, K3Chan c0, K3Chan c1, K3Chan c2, int nchan) {
+    const int NV = g.NV(), NW = g.NW(), NW2 = 2 * g.n2b - 1, NV2 = 2 * g.n2f;
+    const size_t per = (size_t)NV * NW * g.NP;
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= per * nchan) return;
+    const int s = (int)(e / per);
+    const size_t r = e - (size_t)s * per;
+    const int i = (int)(r % NV), iW = (int)((r / NV) % NW), p = (int)(r / ((size_t)NV * NW));
+    const K3Chan& c = s == 0 ? c0 : s == 1 ? c1 : c2;
+    const size_t q = (size_t)(iW - (g.n3b - 1) + g.n2b - 1) + (size_t)NW2 * ((size_t)(i - g.n3f + g.n2f) + (size_t)NV2 * p);
+    double2 v = c.K2r[q];
+    if (c.ch == TCH) v = csub(cscale(v, 2.0), c.K2a[q]);
+    c.K2row[r] = v;
 }
 struct K3_Ep {
     K3Geom g; double T; K3Chan c0, c1, c2;
@@ -1108,7 +1218,7 @@ struct K3_Ep {
         double2* out; const double2* outa;
         double T; int t, tpost, full, NV, NW2;
         __device__ __forceinline__ void operator()(int i, int j, double2 v) const {
-            if (!full) {
+            if (csum) {
                 double2 k2v = k2r[(size_t)NW2 * i];
                 if (t) k2v = csub(cscale(k2v, 2.0), k2a[(size_t)NW2 * i]);
                 v = cadd(v, cmul(k2v, csum[j]));
@@ -1130,9 +1240,10 @@ struct K3_Ep {
         // K2r(Ω, ν, P) at ν = i - n3f: index (m + n2b - 1) + NW2 (i - n3f + n2f + NV2 p)
         const size_t k2o = (size_t)(m + g.n2b - 1) + (size_t)NW2 * ((size_t)(g.n2f - g.n3f) + (size_t)NV2 * p);
         Bound r;
-        r.k2r = c.full ? nullptr : c.K2r + k2o;
-        r.k2a = (c.full || !t) ? nullptr : c.K2a + k2o;
-        r.csum = c.full ? nullptr : c.csum + (size_t)NV * bb;
+        const bool cs = !c.full && c.csum;
+        r.k2r = cs ? c.K2r + k2o : nullptr;
+        r.k2a = (cs && t) ? c.K2a + k2o : nullptr;
+        r.csum = cs ? c.csum + (size_t)NV * bb : nullptr;
         r.Lr = c.Lr + off;
         r.La = t ? c.La + off : nullptr;
         r.out = c.out + off;
@@ -1209,10 +1320,12 @@ __device__ __forceinline__ void k3_R_columns(const K3Geom& g, const K3RChan& c, 
             cr[u] += x.x; ci[u] += x.y;
         }
     }
+    if (c.csum) {
 #pragma unroll
-    for (int u = 0; u < NC; ++u) {
-        const double sr = wave_sum(cr[u]), si = wave_sum(ci[u]);
-        if (lane == 0 && j0 + u < NV) c.csum[(size_t)(j0 + u) + (size_t)NV * bb] = cmk(sr, si);
+        for (int u = 0; u < NC; ++u) {
+            const double sr = wave_sum(cr[u]), si = wave_sum(ci[u]);
+            if (lane == 0 && j0 + u < NV) c.csum[(size_t)(j0 + u) + (size_t)NV * bb] = cmk(sr, si);
+        }
     }
 }
 __device__ __forceinline__ void k3_R_dispatch(const K3Geom& g, const K3RChan& c, int j0, int bb, int lane) {
@@ -1290,8 +1403,10 @@ __device__ __forceinline__ void k3_R_columns_pchunk(const K3Geom& g, const K3RCh
                 else c.R[ro] = x;
                 cr += x.x; ci += x.y;
             }
-            const double sr = wave_sum(cr), si = wave_sum(ci);
-            if (lane == 0) c.csum[(size_t)j + (size_t)NV * bb] = cmk(sr, si);
+            if (c.csum) {
+                const double sr = wave_sum(cr), si = wave_sum(ci);
+                if (lane == 0) c.csum[(size_t)j + (size_t)NV * bb] = cmk(sr, si);
+            }
         }
     }
 }
@@ -2140,64 +2255,111 @@ struct ChanBox {
         return in_b(m, g.n3b) && in_f(n, g.n3f) && in_f(np, g.n3f) ? g.k3(m, n, np, pc) : czero();
     }
 };
+// the channels of one Vertex level of the lattice SDE: at momentum P (chanNP > 1) or the local
+// vertex (chanNP = 1)
+struct SdeLevelNL {
+    ChanV gp, gt, ga;
+    int chanNP;
+};
+// NLEV Vertex levels summed into one pair of L-vertices (the target vertex and the local reference
+// vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch).  Nν2 <= WAVE * SDE_NL_ROUNDS.
+constexpr int SDE_NL_ROUNDS = 4;
+template <int NLEV>
 __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, double2* __restrict__ Lph,
                                                   const double2* __restrict__ Pipp, const double2* __restrict__ Piph,
                                                   const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
-                                                  ChanV gp, ChanV gt, ChanV ga, int n2b, int n2f, int nWP, int nVP, int wb,
-                                                  int NP, int chanNP, double2 TU) {
+                                                  SdeLevelNL lv0, SdeLevelNL lv1, int n2b, int n2f, int nWP, int nVP,
+                                                  int wb, int NP, double2 TU) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= NW2 * NP) return;
     const int iW = wid % NW2, p = wid / NW2;
     const int m = iW - (n2b - 1);
-    const int pc = chanNP > 1 ? p : 0;
-    const ChanBox P{gp, pc}, A{ga, pc}, T{gt, pc};
     const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
     const double2* pp = Pipp + (size_t)(2 * nVP) * row + nVP;   // pp[ω], ω in [-nVP, nVP)
     const double2* ph = Piph + (size_t)(2 * nVP) * row + nVP;
     auto inmesh = [&](int w) { return w >= -nVP && w < nVP; };
-    // totals over the bubble mesh (band + tails) and the ν-independent K2 sums
+    // totals over the bubble mesh (band + tails)
     double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
     for (int w = -wb + lane; w < wb; w += WAVE) {
         const double2 x = pp[w], y = ph[w];
         a0 += x.x; a1 += x.y; b0 += y.x; b1 += y.y;
     }
-    double c0 = 0, c1 = 0, d0 = 0, d1 = 0;
-    for (int n = -gp.n2f + lane; n < gp.n2f; n += WAVE) {
-        const int wp = m - n - 1;   // Lpp: K2p(Ω, n) at n = Ω - ω - 1
-        if (inmesh(wp)) { const double2 x = cmul(P.k2(m, n), pp[wp]); c0 += x.x; c1 += x.y; }
-    }
-    for (int n = -max(ga.n2f, gt.n2f) + lane; n < max(ga.n2f, gt.n2f); n += WAVE) {
-        if (inmesh(n)) { const double2 y = cmul(cadd(A.k2(m, n), T.k2(m, n)), ph[n]); d0 += y.x; d1 += y.y; }
-    }
     a0 = wave_sum(a0); a1 = wave_sum(a1); b0 = wave_sum(b0); b1 = wave_sum(b1);
-    c0 = wave_sum(c0); c1 = wave_sum(c1); d0 = wave_sum(d0); d1 = wave_sum(d1);
     const double2* tp = Tpp + (size_t)(2 * (wb + 1)) * row;
     const double2* th = Tph + (size_t)(2 * (wb + 1)) * row;
     const double2 spp = cadd(cmk(a0, a1), cadd(tp[0], tp[2 * wb + 1]));
     const double2 sph = cadd(cmk(b0, b1), cadd(th[0], th[2 * wb + 1]));
-    const double2 k2p = cmk(c0, c1), k2h = cmk(d0, d1);
-    const double2 k1p = P.k1(m), k1h = cadd(A.k1(m), T.k1(m));
-    const int n3p = gp.n3f, n3h = max(ga.n3f, gt.n3f);
+    // per level: the ν-independent K2 sums and K1
+    double2 k2p[NLEV], k2h[NLEV], k1p[NLEV], k1h[NLEV];
+#pragma unroll
+    for (int l = 0; l < NLEV; ++l) {
+        const SdeLevelNL& L = l == 0 ? lv0 : lv1;
+        const int pc = L.chanNP > 1 ? p : 0;
+        const ChanBox P{L.gp, pc}, A{L.ga, pc}, T{L.gt, pc};
+        double c0 = 0, c1 = 0, d0 = 0, d1 = 0;
+        for (int n = -L.gp.n2f + lane; n < L.gp.n2f; n += WAVE) {
+            const int wp = m - n - 1;   // Lpp: K2p(Ω, n) at n = Ω - ω - 1
+            if (inmesh(wp)) { const double2 x = cmul(P.k2(m, n), pp[wp]); c0 += x.x; c1 += x.y; }
+        }
+        for (int n = -max(L.ga.n2f, L.gt.n2f) + lane; n < max(L.ga.n2f, L.gt.n2f); n += WAVE) {
+            if (inmesh(n)) { const double2 y = cmul(cadd(A.k2(m, n), T.k2(m, n)), ph[n]); d0 += y.x; d1 += y.y; }
+        }
+        k2p[l] = cmk(wave_sum(c0), wave_sum(c1));
+        k2h[l] = cmk(wave_sum(d0), wave_sum(d1));
+        k1p[l] = P.k1(m);
+        k1h[l] = cadd(A.k1(m), T.k1(m));
+    }
+    // the p channel's K3 products Σ_n Πpp(Ω - n - 1) K3p(Ω, n, ν): K3p's first index is the summed
+    // one, so the lanes run along it (coalesced column reads) and each column ν is a wave sum,
+    // kept by the lane that owns output ν (iv = ν + n2f, round iv / WAVE)
+    double2 accp[SDE_NL_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < SDE_NL_ROUNDS; ++r) accp[r] = czero();
+#pragma unroll
+    for (int l = 0; l < NLEV; ++l) {
+        const SdeLevelNL& L = l == 0 ? lv0 : lv1;
+        const int pc = L.chanNP > 1 ? p : 0;
+        const ChanBox P{L.gp, pc};
+        const int n3p = L.gp.n3f;
+        if (!in_b(m, L.gp.n3b)) continue;
+        for (int c0 = -n3p; c0 < n3p; c0 += WAVE) {
+            const int n = c0 + lane;
+            const bool ok = n < n3p && inmesh(m - n - 1);
+            const double2 w = ok ? pp[m - n - 1] : czero();
+            for (int v = max(-n3p, -n2f); v < min(n3p, n2f); ++v) {   // outputs inside both boxes
+                const double2 t = ok ? cmul(P.k3(m, n, v), w) : czero();
+                const double sr = wave_sum(t.x), si = wave_sum(t.y);
+                const int iv = v + n2f;
+                if ((iv & (WAVE - 1)) == lane) {
+#pragma unroll
+                    for (int r = 0; r < SDE_NL_ROUNDS; ++r)
+                        if (r == iv / WAVE) accp[r] = cadd(accp[r], cmk(sr, si));
+                }
+            }
+        }
+    }
     for (int iv = lane; iv < NV2; iv += WAVE) {
         const int v = iv - n2f;
-        double2 xp = cadd(cmul(cadd(k1p, P.k2(m, v)), spp), k2p);
-        double2 xh = cadd(cmul(cadd(k1h, cadd(A.k2(m, v), T.k2(m, v))), sph), k2h);
-        if (in_b(m, gp.n3b) && in_f(v, n3p)) {
-            double2 sp = czero();
+        double2 xp = czero(), xh = czero();
+#pragma unroll
+        for (int r = 0; r < SDE_NL_ROUNDS; ++r)
+            if (r == iv / WAVE) xp = accp[r];
+#pragma unroll
+        for (int l = 0; l < NLEV; ++l) {
+            const SdeLevelNL& L = l == 0 ? lv0 : lv1;
+            const int pc = L.chanNP > 1 ? p : 0;
+            const ChanBox P{L.gp, pc}, A{L.ga, pc}, T{L.gt, pc};
+            xp = cadd(xp, cadd(cmul(cadd(k1p[l], P.k2(m, v)), spp), k2p[l]));
+            xh = cadd(xh, cadd(cmul(cadd(k1h[l], cadd(A.k2(m, v), T.k2(m, v))), sph), k2h[l]));
+            const int n3h = max(L.ga.n3f, L.gt.n3f);
+            if ((in_b(m, L.ga.n3b) || in_b(m, L.gt.n3b)) && in_f(v, n3h)) {
+                double2 sh = czero();
 #pragma unroll 8
-            for (int n = -n3p; n < n3p; ++n) {
-                const int wp = m - n - 1;
-                if (inmesh(wp)) sp = cadd(sp, cmul(P.k3(m, n, v), pp[wp]));
+                for (int n = -n3h; n < n3h; ++n)
+                    if (inmesh(n)) sh = cadd(sh, cmul(cadd(A.k3(m, v, n), T.k3(m, v, n)), ph[n]));
+                xh = cadd(xh, sh);
             }
-            xp = cadd(xp, sp);
-        }
-        if ((in_b(m, ga.n3b) || in_b(m, gt.n3b)) && in_f(v, n3h)) {
-            double2 sh = czero();
-#pragma unroll 8
-            for (int n = -n3h; n < n3h; ++n)
-                if (inmesh(n)) sh = cadd(sh, cmul(cadd(A.k3(m, v, n), T.k3(m, v, n)), ph[n]));
-            xh = cadd(xh, sh);
         }
         const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
         Lpp[o] = cmul(xp, TU);
