@@ -259,6 +259,26 @@ def c2_band(w):
     return wb, (nVP + wb + 15) // 16 - (nVP - wb) // 16
 
 
+def gemm_roofline(head, P, gemm_peak, traffic, n3P):
+    """Roofline object of the BSE contraction GEMMs.  FP32 path: HBM-bound (their FP64 epilogues and FP32
+    operands move 264 B per K3 element of the three channels per iteration: FL.K3 and the FP32 right operand
+    written by BSE_L_K3! (48 + 24), FL.a read for its t post-processing (16); BSE_K3! reads A32, R32, FL.K3,
+    FL.a for the t self term and Fbuff.a for the t post-processing (24 + 24 + 48 + 16 + 16) and writes
+    Fbuff.K3 (48)); the MFMA rate is stated beside.  FP64 path: MFMA-bound (executed flops)."""
+    t = P["t_gemm"]
+    mfma = P["f_exec"] / t
+    common = {"kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA, BSE_L_K3! + BSE_K3!, "
+                        "4 launches per iteration)",
+              "launches_per_iteration": 4, "ms_per_iteration": t * 1e3, "traffic": traffic,
+              "executed_flops_per_iteration": P["f_exec"], "mfma_tflops": mfma / 1e12, "mfma_frac": mfma / gemm_peak}
+    if head == "fp32":
+        b = 264.0 * n3P / 4
+        return dict({"bound": "hbm", "achieved": b / (t / 4) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": b / (t / 4) / HBM_PEAK, "algorithmic_bytes_per_launch": b}, **common)
+    return dict({"bound": "mfma", "achieved": mfma / 1e12, "peak": gemm_peak / 1e12, "unit": "TFLOP/s",
+                 "frac": mfma / gemm_peak}, **common)
+
+
 def main_c2(args):
     """The headline line: the lattice fdPA iteration in both precision paths."""
     import fdga
@@ -330,13 +350,8 @@ def main_c2(args):
                             "stage_ms": paths["fp64"]["st"]},
                    "speedup_fp32_over_fp64": paths["fp64"]["ms"] / paths["fp32"]["ms"]},
         # the dominant kernel: the BSE contraction GEMMs (four launches per iteration: BSE_L_K3! p+a and t,
-        # BSE_K3! p+a and t), executed flops over their summed durations (HIP events, one at a time)
-        "roofline": {"bound": "mfma", "achieved": P["f_exec"] / P["t_gemm"] / 1e12, "peak": gemm_peak / 1e12,
-                     "unit": "TFLOP/s", "frac": P["f_exec"] / P["t_gemm"] / gemm_peak, "traffic": gemm_traffic,
-                     "kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA, BSE_L_K3! + BSE_K3!, "
-                               "4 launches per iteration)",
-                     "executed_flops_per_iteration": P["f_exec"], "launches_per_iteration": 4,
-                     "ms_per_iteration": P["t_gemm"] * 1e3},
+        # BSE_K3! p+a and t), over their summed durations (HIP events, one kernel at a time)
+        "roofline": gemm_roofline(head, P, gemm_peak, gemm_traffic, NW3 * NV3 * NV3 * NP),
         "roofline_bubble": {"bound": "hbm", "achieved": b_bubble / t_bub / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                             "frac": b_bubble / t_bub / HBM_PEAK,
                             "traffic": recorded_traffic("c2_bubble_nl_traffic.txt"),
