@@ -300,6 +300,8 @@ def main_c2(args):
             fdga.iterate_solver(S, "fdPA")
         obs = observables(S)
         fdga.unflatten(S, x0)
+        S.autotune(reps=3)   # tile shapes of the contraction launches (results do not depend on them)
+        log(f"bench c2[{prec}]: autotuned tile shapes (BSE_L_K3!, BSE_K3!): {S.gemm_tiles()}")
         ms, st = time_path(S, args, STAGES)
         f_exec = S.gemm_flops(True) + S.gemm_flops(False)
         t_gemm = (st["gemm_L_K3"] + st["gemm_K3"]) * 1e-3

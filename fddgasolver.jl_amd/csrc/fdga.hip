@@ -574,8 +574,6 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
     const double2* Tpp = pi_tail(c, Pp);
     const double2* Tph = pi_tail(c, Ph);
     if (d.nonlocal && lv.kind == 0) {   // lattice solver, Vertex level: one wave per (Ω, P)
-        if (c->NV2 > WAVE * SDE_NL_ROUNDS)
-            return fail(c, "lattice SDE: at most " + std::to_string(WAVE * SDE_NL_ROUNDS / 2) + " fermionic K2 frequencies");
         const unsigned nb = (unsigned)cdiv((size_t)c->NW2 * c->NP * WAVE, 256);
         const SdeLevelNL l0{lv.g[0], lv.g[1], lv.g[2], lv.chanNP};
         if (lv2) {

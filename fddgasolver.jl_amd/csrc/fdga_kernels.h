@@ -2262,8 +2262,7 @@ struct SdeLevelNL {
     int chanNP;
 };
 // NLEV Vertex levels summed into one pair of L-vertices (the target vertex and the local reference
-// vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch).  Nν2 <= WAVE * SDE_NL_ROUNDS.
-constexpr int SDE_NL_ROUNDS = 4;
+// vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch)
 template <int NLEV>
 __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, double2* __restrict__ Lph,
                                                   const double2* __restrict__ Pipp, const double2* __restrict__ Piph,
@@ -2310,41 +2309,9 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
         k1p[l] = P.k1(m);
         k1h[l] = cadd(A.k1(m), T.k1(m));
     }
-    // the p channel's K3 products Σ_n Πpp(Ω - n - 1) K3p(Ω, n, ν): K3p's first index is the summed
-    // one, so the lanes run along it (coalesced column reads) and each column ν is a wave sum,
-    // kept by the lane that owns output ν (iv = ν + n2f, round iv / WAVE)
-    double2 accp[SDE_NL_ROUNDS];
-#pragma unroll
-    for (int r = 0; r < SDE_NL_ROUNDS; ++r) accp[r] = czero();
-#pragma unroll
-    for (int l = 0; l < NLEV; ++l) {
-        const SdeLevelNL& L = l == 0 ? lv0 : lv1;
-        const int pc = L.chanNP > 1 ? p : 0;
-        const ChanBox P{L.gp, pc};
-        const int n3p = L.gp.n3f;
-        if (!in_b(m, L.gp.n3b)) continue;
-        for (int c0 = -n3p; c0 < n3p; c0 += WAVE) {
-            const int n = c0 + lane;
-            const bool ok = n < n3p && inmesh(m - n - 1);
-            const double2 w = ok ? pp[m - n - 1] : czero();
-            for (int v = max(-n3p, -n2f); v < min(n3p, n2f); ++v) {   // outputs inside both boxes
-                const double2 t = ok ? cmul(P.k3(m, n, v), w) : czero();
-                const double sr = wave_sum(t.x), si = wave_sum(t.y);
-                const int iv = v + n2f;
-                if ((iv & (WAVE - 1)) == lane) {
-#pragma unroll
-                    for (int r = 0; r < SDE_NL_ROUNDS; ++r)
-                        if (r == iv / WAVE) accp[r] = cadd(accp[r], cmk(sr, si));
-                }
-            }
-        }
-    }
     for (int iv = lane; iv < NV2; iv += WAVE) {
         const int v = iv - n2f;
         double2 xp = czero(), xh = czero();
-#pragma unroll
-        for (int r = 0; r < SDE_NL_ROUNDS; ++r)
-            if (r == iv / WAVE) xp = accp[r];
 #pragma unroll
         for (int l = 0; l < NLEV; ++l) {
             const SdeLevelNL& L = l == 0 ? lv0 : lv1;
@@ -2352,7 +2319,16 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
             const ChanBox P{L.gp, pc}, A{L.ga, pc}, T{L.gt, pc};
             xp = cadd(xp, cadd(cmul(cadd(k1p[l], P.k2(m, v)), spp), k2p[l]));
             xh = cadd(xh, cadd(cmul(cadd(k1h[l], cadd(A.k2(m, v), T.k2(m, v))), sph), k2h[l]));
-            const int n3h = max(L.ga.n3f, L.gt.n3f);
+            const int n3p = L.gp.n3f, n3h = max(L.ga.n3f, L.gt.n3f);
+            if (in_b(m, L.gp.n3b) && in_f(v, n3p)) {
+                double2 sp = czero();
+#pragma unroll 8
+                for (int n = -n3p; n < n3p; ++n) {
+                    const int wp = m - n - 1;
+                    if (inmesh(wp)) sp = cadd(sp, cmul(P.k3(m, n, v), pp[wp]));
+                }
+                xp = cadd(xp, sp);
+            }
             if ((in_b(m, L.ga.n3b) || in_b(m, L.gt.n3b)) && in_f(v, n3h)) {
                 double2 sh = czero();
 #pragma unroll 8
