@@ -992,7 +992,6 @@ struct LK3BoundR {
         q.P1 = pi[k];
         q.P0 = pi0[k];
         q.oa = tpost ? outa[(size_t)i + (size_t)NV * j] : czero();
-        // r = s1 [(P1 - P0) f0 + P1 gm] + s2 P1 x, written out so that every launch rounds it alike
         const double dr = q.P1.x - q.P0.x, di = q.P1.y - q.P0.y;
         double rr = (dr * q.f0.x - di * q.f0.y) + (q.P1.x * q.gm.x - q.P1.y * q.gm.y);
         double ri = (dr * q.f0.y + di * q.f0.x) + (q.P1.x * q.gm.y + q.P1.y * q.gm.x);
@@ -1217,19 +1216,29 @@ struct K3_Ep {
         const double2* csum; const double2* Lr; const double2* La;
         double2* out; const double2* outa;
         double T; int t, tpost, full, NV, NW2;
-        __device__ __forceinline__ void operator()(int i, int j, double2 v) const {
+        // the element's loads (FL.K3, and for t FL.a and the finished a channel) issued together,
+        // before any arithmetic
+        struct Pre { double2 self, oa; };
+        __device__ __forceinline__ Pre pre(int i, int j) const {
+            const size_t o = (size_t)i + (size_t)NV * j;
+            Pre q;
+            q.self = Lr[o];
+            if (t) q.self = csub(cscale(q.self, 2.0), La[o]);
+            q.oa = tpost ? outa[o] : czero();
+            return q;
+        }
+        __device__ __forceinline__ void post(int i, int j, double2 v, const Pre& q) const {
             if (csum) {
                 double2 k2v = k2r[(size_t)NW2 * i];
                 if (t) k2v = csub(cscale(k2v, 2.0), k2a[(size_t)NW2 * i]);
                 v = cadd(v, cmul(k2v, csum[j]));
             }
             const size_t o = (size_t)i + (size_t)NV * j;
-            double2 self = Lr[o];
-            if (t) self = csub(cscale(self, 2.0), La[o]);
-            double2 x = cadd(cscale(v, T), self);
-            if (tpost) x = cscale(cadd(x, outa[o]), 0.5);
+            double2 x = cadd(cscale(v, T), q.self);
+            if (tpost) x = cscale(cadd(x, q.oa), 0.5);
             out[o] = x;
         }
+        __device__ __forceinline__ void operator()(int i, int j, double2 v) const { post(i, j, v, pre(i, j)); }
     };
     __device__ __forceinline__ Bound one(const K3Chan& c, int bb) const {
         const int NV = g.NV(), NW = g.NW();
