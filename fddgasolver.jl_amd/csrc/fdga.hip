@@ -730,9 +730,12 @@ static int launch_w(fdga_ctx* c) {
 // the weights of the three channels in one launch
 static int launch_w_all(fdga_ctx* c) {
     const size_t nrows = (size_t)c->NWP * c->NP;
-    const unsigned nb = cdiv(nrows * (2 * c->wb), 256);
+    // lattice solver: one thread per chunk of W_PCH momenta (the P-independent reference factor once)
+    const int pchunk = c->NP >= 2 * W_PCH;
+    const size_t nth = pchunk ? (size_t)c->NWP * cdiv(c->NP, W_PCH) * (2 * c->wb) : nrows * (2 * c->wb);
+    const unsigned nb = cdiv(nth, 256);
     k_bse_w3<<<3 * nb, 256, 0, c->run>>>(w_args(c, PCH), w_args(c, TCH), w_args(c, ACH), vtx_FL(c), f0_view(c), c->d.nK1,
-                                         c->nVP, c->wb, box_bound(c), nrows, nb);
+                                         c->nVP, c->wb, box_bound(c), nrows, nb, pchunk, c->NP);
     LAUNCHCHK(c);
     return 0;
 }

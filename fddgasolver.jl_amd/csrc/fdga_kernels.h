@@ -1522,11 +1522,59 @@ template <int CH>
 __global__ __launch_bounds__(256) void k_bse_w(WArgs a, VtxV FL, F0V f0, int nWP, int nVP, int wb, int NK, size_t nrows) {
     bse_w_elem<CH>(blockIdx.x * (size_t)blockDim.x + threadIdx.x, a, FL, f0, nWP, nVP, wb, NK, nrows);
 }
-// the three channels' weights in one launch (nb blocks each, in the order p, t, a)
+// The same weights with one thread per (band entry, Ω) and a chunk of W_PCH momenta: the local
+// reference vertex's factor F0(Ω, ω~, ∞) does not depend on P and is evaluated once for the chunk
+// (the lattice solver; identical values to bse_w_elem).
+constexpr int W_PCH = 8;
+template <int CH>
+__device__ __forceinline__ void bse_w_pchunk(size_t e, const WArgs& a, const VtxV& FL, const F0V& f0, int nWP, int nVP,
+                                             int wb, int NK, int NP) {
+    constexpr int ch = CH, sp = ChSpin<CH>::SP;
+    double2* __restrict__ W = a.W;
+    const double2* __restrict__ Pi = a.Pi;
+    const double2* __restrict__ Pi0 = a.Pi0;
+    const int per = 2 * wb, NWP = 2 * nWP - 1, npc = (NP + W_PCH - 1) / W_PCH;
+    const size_t col = e / per;
+    if (col >= (size_t)NWP * npc) return;
+    const int t = (int)(e - col * per);
+    const int iW = (int)(col % NWP), pc = (int)(col / NWP);
+    const int m = iW - (nWP - 1);
+    const int iw = nVP - wb + t, w = iw - nVP;
+    const int wc = crossing(m, w, ch);
+    const double2 f0r = f0.eval(m, wc, FDGA_INF, ch, sp);
+    const int p0 = pc * W_PCH, p1 = min(p0 + W_PCH, NP);
+#pragma unroll 4
+    for (int p = p0; p < p1; ++p) {
+        const size_t row = (size_t)iW + (size_t)NWP * p;
+        const size_t o = (size_t)iw + (size_t)(2 * nVP) * row;
+        const double2 flr = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
+        const double2 pi = Pi[o];
+        W[(size_t)t + (size_t)(2 * wb) * row] = cadd(cmul(csub(pi, Pi0[o]), f0r), cmul(pi, flr));
+    }
+    if (t == 0)
+        for (int side = 0; side < 2; ++side) {
+            const int wf = side ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
+            const int wcf = crossing(m, wf, ch);
+            const double2 f0f = f0.eval(m, wcf, FDGA_INF, ch, sp);
+            for (int p = p0; p < p1; ++p) {
+                const size_t row = (size_t)iW + (size_t)NWP * p;
+                a.far[4 * row + 2 * side] = f0f;
+                a.far[4 * row + 2 * side + 1] = FL.eval(m, wcf, FDGA_INF, p, ch, sp, FL_ALL);
+            }
+        }
+}
+// the three channels' weights in one launch (nb blocks each, in the order p, t, a); pchunk: one
+// thread per (band entry, Ω, chunk of W_PCH momenta)
 __global__ __launch_bounds__(256) void k_bse_w3(WArgs ap, WArgs at, WArgs aa, VtxV FL, F0V f0, int nWP, int nVP, int wb,
-                                                int NK, size_t nrows, unsigned nb) {
+                                                int NK, size_t nrows, unsigned nb, int pchunk, int NP) {
     const unsigned ch = blockIdx.x / nb;
     const size_t e = (blockIdx.x - ch * nb) * (size_t)blockDim.x + threadIdx.x;
+    if (pchunk) {
+        if (ch == 0) bse_w_pchunk<PCH>(e, ap, FL, f0, nWP, nVP, wb, NK, NP);
+        else if (ch == 1) bse_w_pchunk<TCH>(e, at, FL, f0, nWP, nVP, wb, NK, NP);
+        else bse_w_pchunk<ACH>(e, aa, FL, f0, nWP, nVP, wb, NK, NP);
+        return;
+    }
     if (ch == 0) bse_w_elem<PCH>(e, ap, FL, f0, nWP, nVP, wb, NK, nrows);
     else if (ch == 1) bse_w_elem<TCH>(e, at, FL, f0, nWP, nVP, wb, NK, nrows);
     else bse_w_elem<ACH>(e, aa, FL, f0, nWP, nVP, wb, NK, nrows);
