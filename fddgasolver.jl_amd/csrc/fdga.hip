@@ -116,6 +116,7 @@ struct fdga_ctx {
     int blist_n[6][TILE_N] = {};      // workgroups of the list (computed tiles and idle padding)
     int blist_tiles[6][TILE_N] = {};  // computed tiles
     int ktile[2] = {TILE_64x64, TILE_64x64};   // tile shape of the BSE_L_K3! / BSE_K3! launches
+    int kgrid[2] = {0, 0};   // their workgroup count (0: one per work item; else a persistent grid)
     int* finmap = nullptr;    // k_k3_finish chunk map (XCD-paired Ω blocks)
     int finmap_n = 0, fin_nchunk = 0;
     double2* k3raw = nullptr;                                       // raw t-channel K3 under groups [n3, NP]
@@ -954,7 +955,8 @@ static hipError_t k3_gemm(fdga_ctx* c, const int* chans, int n, int which, const
     const int* bl = k3_blocks(c, chans, n, which == 0, tile, nl);
     const GemmSchedOne sc = sched_one(NV, NV, NV, cdiv(NV, tile_bm(tile)), cdiv(NV, tile_bn(tile)), n * c->NW3 * c->NP, bl, nl,
                                       c->NW3, c->NP);
-    return zgemm_run(c->d.precision, tile, (size_t)sc.nitems, c->run, sc, a, b, e);
+    const size_t grid = c->kgrid[which] ? std::min((size_t)c->kgrid[which], (size_t)sc.nitems) : (size_t)sc.nitems;
+    return zgemm_run(c->d.precision, tile, grid, c->run, sc, a, b, e);
 }
 // Without symmetry groups a (p, a, t) contraction over many (Ω, P) batches runs as two launches, (p, a)
 // and then t with its post-processing (src/BSE_templates.jl:176-177) in the epilogue, where its a
@@ -2225,6 +2227,15 @@ double fdga_gemm_flops(const fdga_ctx* c, int32_t left) {
 }
 // tile shape of the BSE_L_K3! (left = 1) or BSE_K3! (left = 0) launches
 int fdga_gemm_tile(const fdga_ctx* c, int32_t left) { return c ? c->ktile[left ? 0 : 1] : -1; }
+int fdga_gemm_grid(const fdga_ctx* c, int32_t left) { return c ? c->kgrid[left ? 0 : 1] : -1; }
+int fdga_set_gemm_grid(fdga_ctx* c, int32_t left, int32_t grid) {
+    if (!c) return -1;
+    if (grid < 0 || grid % 256) return fail(c, "the workgroup count must be 0 (one per work item) or a multiple of 256");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->kgrid[left ? 0 : 1] = grid;
+    graphs_invalidate(c);
+    return 0;
+}
 int fdga_set_gemm_tile(fdga_ctx* c, int32_t left, int32_t tile) {
     if (!c) return -1;
     if (tile < 0 || tile >= TILE_N) return fail(c, "unknown tile shape");

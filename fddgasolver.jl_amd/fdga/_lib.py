@@ -60,6 +60,8 @@ SIGNATURES = {
     "fdga_gemm_flops": (ctypes.c_double, [_vp, _i32]),
     "fdga_gemm_tile": (ctypes.c_int, [_vp, _i32]),
     "fdga_set_gemm_tile": (ctypes.c_int, [_vp, _i32, _i32]),
+    "fdga_gemm_grid": (ctypes.c_int, [_vp, _i32]),
+    "fdga_set_gemm_grid": (ctypes.c_int, [_vp, _i32, _i32]),
     "fdga_autotune": (ctypes.c_int, [_vp, _i32]),
     "fdga_set_graph": (ctypes.c_int, [_vp, _i32]),
     "fdga_reset_sym_grp": (ctypes.c_int, [_vp]),

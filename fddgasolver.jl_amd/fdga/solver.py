@@ -160,6 +160,15 @@ class Solver:
     def set_gemm_tile(self, left, tile):
         self._check(self.lib.fdga_set_gemm_tile(self.h, int(left), self._TILE_NAMES.index(tile)))
 
+    def gemm_grids(self):
+        """Workgroup counts of the BSE_L_K3! and BSE_K3! launches (0: one per output tile)."""
+        return tuple(self.lib.fdga_gemm_grid(self.h, left) for left in (1, 0))
+
+    def set_gemm_grid(self, left, grid):
+        """A persistent grid of `grid` workgroups (a multiple of 256) for that launch, or 0; results do
+        not depend on it."""
+        self._check(self.lib.fdga_set_gemm_grid(self.h, int(left), int(grid)))
+
     def gemm_flops(self, left):
         """Executed flops of one batched BSE_L_K3! (left) or BSE_K3! launch."""
         return self.lib.fdga_gemm_flops(self.h, int(left))

@@ -154,6 +154,12 @@ int64_t fdga_gemm_blocks(const fdga_ctx* ctx, int32_t left);
 double  fdga_gemm_flops(const fdga_ctx* ctx, int32_t left);
 int     fdga_gemm_tile(const fdga_ctx* ctx, int32_t left);
 int     fdga_set_gemm_tile(fdga_ctx* ctx, int32_t left, int32_t tile);
+/* workgroup count of that launch: 0 = one workgroup per output tile (the default), otherwise a
+ * persistent grid of `grid` workgroups (a multiple of 256, so that every workgroup stays on one XCD)
+ * looping over the tiles; a smaller grid leaves compute units to the other chain of the iteration.
+ * Results do not depend on it.  No counterpart in the reference (execution option). */
+int     fdga_gemm_grid(const fdga_ctx* ctx, int32_t left);
+int     fdga_set_gemm_grid(fdga_ctx* ctx, int32_t left, int32_t grid);
 /* autotuner: times every tile shape of the batched (p, a, t) BSE_L_K3! and BSE_K3! launches on the
  * current operands (`reps` launches each) and keeps the fastest; the solver state is left as it was
  * (the launches write into scratch).  Results do not depend on the tile shape.  No counterpart in

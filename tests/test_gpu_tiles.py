@@ -1,6 +1,7 @@
-"""Tile shapes of the single-solver contraction launches and the autotuner: every shape gives the
-same bits (each element accumulates over k in the same order), with the symmetry block lists of
-that shape on the local solver; the autotuner leaves the solver state as it was."""
+"""Tile shapes and persistent grids of the single-solver contraction launches and the autotuner: every
+shape and grid gives the same bits (each element accumulates over k in the same order), with the
+symmetry block lists of that shape on the local solver; the autotuner leaves the solver state as it
+was."""
 import numpy as np
 import pytest
 
@@ -32,6 +33,36 @@ def test_every_tile_shape_same_iteration(case, sym, precision):
             fdga.iterate_solver(S, "fdPA")
             got = fdga.flatten(S)
             assert relerr(got, want) == 0.0, (tl, tk, relerr(got, want))
+
+
+@pytest.mark.parametrize("case", ["nl_L16", "local_vertexref"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_persistent_gemm_grid_same_iteration(case, precision):
+    """A persistent grid for the contraction launches (workgroups looping over the output tiles, the
+    hipGraph replay included) gives the same bits as one workgroup per tile; invalid counts are refused."""
+    import fdga
+    ref, _ = make_pair(case, seed=7, precision=precision)
+    x0 = fdga.flatten(ref)
+    for _ in range(2):
+        fdga.iterate_solver(ref, "fdPA")
+    want = fdga.flatten(ref)
+    S, _ = make_pair(case, seed=7, precision=precision)
+    with pytest.raises(fdga.FdgaError, match="multiple of 256"):
+        S.set_gemm_grid(True, 100)
+    for graph in (False, True):
+        S.set_graph(graph)
+        for g in (256, 512, 0):
+            fdga.unflatten(S, x0)
+            S.set_gemm_grid(True, g)
+            S.set_gemm_grid(False, g)
+            assert S.gemm_grids() == (g, g)
+            for _ in range(2):
+                fdga.iterate_solver(S, "fdPA")
+            S.sync()
+            got = fdga.flatten(S)
+            assert relerr(got, want) == 0.0, (graph, g, relerr(got, want))
+    ref.close()
+    S.close()
 
 
 @pytest.mark.parametrize("strategy", ["scPA", "fdPA"])
