@@ -100,3 +100,32 @@ def test_headline_workload_iterations_match_oracle(sym, iters, precision, tol):
         O.iterate(po.FDPA)
         e = relerr(fdga.flatten(S), O.flatten())
         assert e < tol, f"{precision} sym={sym} iteration {it}: {e:.3e}"
+
+
+@pytest.mark.timeout(400)
+def test_headline_workload_solved_to_1e_8_matches_oracle():
+    """The SIAM fdPA workload at its full boxes, solved to max|R| < 1e-8 from the initial state with the
+    reference's Anderson mixing (m = 8, β = 0.85) and the symmetry groups, on the GPU and on the oracle
+    (OpenMP): the same number of iterations, the same fixed point (1e-9) and the same χ_sp, χ_ch."""
+    import fdga
+    import pyoracle as po
+    w = C1
+    S = fdga.parquet_solver_siam_parquet_approximation(w["nG"], w["nK1"], w["nK2"], w["nK3"], e=0.0, Δ=w["Delta"],
+                                                       D=None, T=w["T"], U=w["U"], mΠν_factor=w["mPinu"])
+    S.init_sym_grp()
+    print("c1 solve: GPU", flush=True)
+    it, res = fdga.solve(S, strategy="fdPA", tol=1e-8, maxiter=100)
+    print(f"c1 solve: GPU {it} iterations, max|R| {res:.2e}; oracle", flush=True)
+    O = po.Oracle(nG=w["nG"], nK1=w["nK1"], nK2=w["nK2"], nK3=w["nK3"], mPinu=w["mPinu"], T=w["T"], U=w["U"])
+    O.siam(0.0, w["Delta"], None)
+    O.init()
+    O.init_sym_grp()
+    it_o, res_o = O.solve(po.FDPA, maxiter=100, tol=1e-8)
+    print(f"c1 solve: oracle {it_o} iterations, max|R| {res_o:.2e}", flush=True)
+    assert res < 1e-8 and res_o < 1e-8
+    assert it == it_o, (it, it_o)
+    assert relerr(fdga.flatten(S), O.flatten()) < 1e-9
+    sp, ch = fdga.susceptibilities(S)
+    sp_o, ch_o = fdga.chi_from_k1(O.vertex(po.VTX_F, po.ACH, 1), O.vertex(po.VTX_F, po.TCH, 1), w["U"])
+    assert relerr(sp, sp_o) < 1e-9 and relerr(ch, ch_o) < 1e-9
+    S.close()
