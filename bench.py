@@ -285,11 +285,11 @@ def tune_gemm_grid(S, grids=(0, 256, 512, 768, 1024), reps=4):
 
 
 def gemm_roofline(head, P, gemm_peak, traffic, n3P):
-    """Roofline object of the BSE contraction GEMMs.  FP32 path: HBM-bound (their FP64 epilogues and FP32
-    operands move 264 B per K3 element of the three channels per iteration: FL.K3 and the FP32 right operand
-    written by BSE_L_K3! (48 + 24), FL.a read for its t post-processing (16); BSE_K3! reads A32, R32, FL.K3,
-    FL.a for the t self term and Fbuff.a for the t post-processing (24 + 24 + 48 + 16 + 16) and writes
-    Fbuff.K3 (48)); the MFMA rate is stated beside.  FP64 path: MFMA-bound (executed flops)."""
+    """Roofline object of the BSE contraction GEMMs.  FP32 path: HBM-bound (their epilogues and FP32 operands
+    move 200 B per K3 element of the three channels per iteration: FL.K3 in FP32 and the FP32 right operand
+    written by BSE_L_K3! (24 + 24), FL.a (FP32) read for its t post-processing (8); BSE_K3! reads A32, R32,
+    FL.K3 (FP32), FL.a (FP32) for the t self term and Fbuff.a for the t post-processing (24 + 24 + 24 + 8 + 16)
+    and writes Fbuff.K3 (48)); the MFMA rate is stated beside.  FP64 path: MFMA-bound (executed flops)."""
     t = P["t_gemm"]
     mfma = P["f_exec"] / t
     common = {"kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA, BSE_L_K3! + BSE_K3!, "
@@ -297,7 +297,7 @@ def gemm_roofline(head, P, gemm_peak, traffic, n3P):
               "launches_per_iteration": 4, "ms_per_iteration": t * 1e3, "traffic": traffic,
               "executed_flops_per_iteration": P["f_exec"], "mfma_tflops": mfma / 1e12, "mfma_frac": mfma / gemm_peak}
     if head == "fp32":
-        b = 264.0 * n3P / 4
+        b = 200.0 * n3P / 4
         return dict({"bound": "hbm", "achieved": b / (t / 4) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": b / (t / 4) / HBM_PEAK, "algorithmic_bytes_per_launch": b}, **common)
     return dict({"bound": "mfma", "achieved": mfma / 1e12, "peak": gemm_peak / 1e12, "unit": "TFLOP/s",

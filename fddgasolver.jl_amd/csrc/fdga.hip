@@ -70,6 +70,9 @@ struct fdga_ctx {
     // channels (bit ch) whose FP32 BSE_K3! right operand was written by the last BSE_L_K3! epilogue
     // from the current Π, caches and FL (cleared by anything that changes them)
     int r32_fresh = 0;
+    // FL.K3 of the three channels holds FP32 values in the first half of its bytes (the FP32 lattice
+    // iteration's form; fl_widen restores FP64 for other readers)
+    int fl32 = 0;
     // the stream the stage kernels are launched on: st, or st2 for the second chain of an iteration
     hipStream_t run = nullptr;
     hipStream_t st2 = nullptr;
@@ -916,7 +919,7 @@ static bool lk3_writes_r32(const fdga_ctx* c, bool sym) {
     return c->d.precision == FDGA_FP32 && c->NP > 1 && !sym;
 }
 // operand loaders and epilogue of a BSE_L_K3! launch (batch = (slot NW3 + iΩ) NP... see K3Geom)
-static LK3Set lk3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi) {
+static LK3Set lk3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, bool f32 = false) {
     LK3Chan cc[3] = {};
     for (int s = 0; s < n; ++s) {
         const int ch = chans[s];
@@ -934,6 +937,10 @@ static LK3Set lk3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi
             cc[s].s1 = (ch == ACH) ? 1.0 : -1.0;   // src/BSE_templates.jl:156,164,172
             cc[s].s2 = (ch == TCH) ? -1.0 : 1.0;
             cc[s].ch = ch;
+            if (f32) {
+                cc[s].out32 = reinterpret_cast<float2*>(c->FL[ch].K3);
+                cc[s].outa32 = reinterpret_cast<const float2*>(c->FL[ACH].K3);
+            }
         }
     }
     const K3Geom g = k3geom(c);
@@ -967,6 +974,44 @@ static bool split_t(const fdga_ctx* c, const int* chans, int n, bool sym) {
 }
 static const int kChansPA[2] = {PCH, ACH};
 static const int kChansT[1] = {TCH};
+static const int kChansAll[3] = {PCH, ACH, TCH};
+
+// The FP32 lattice iteration (its three-channel BSE_L_K3! in two launches, no K3 symmetry groups)
+// keeps FL.K3 in FP32, in the first half of each FL.K3 array's bytes: BSE_L_K3!'s epilogue writes it
+// rounded and BSE_K3! reads it (1.6 GB less traffic per iteration at L = 16).  fl_widen / fl_narrow
+// convert the three arrays through the right-operand buffers (so the fresh right operands are lost);
+// they run eagerly, never inside a captured graph (fl_prepare before each iteration).
+static bool fl32_mode(const fdga_ctx* c) {
+    return lk3_writes_r32(c, false) && split_t(c, kChansAll, 3, k3_sym(c, kChansAll, 3, true));
+}
+static int fl_widen(fdga_ctx* c) {
+    if (!c->fl32) return 0;
+    const size_t n = (size_t)c->n3 * c->NP;
+    for (int r = 0; r < 3; ++r) {
+        float2* tmp = reinterpret_cast<float2*>(c->R[r]);
+        HIPCHK(c, hipMemcpyAsync(tmp, c->FL[r].K3, n * sizeof(float2), hipMemcpyDeviceToDevice, c->run));
+        k_widen32<<<cdiv(n, 256), 256, 0, c->run>>>(c->FL[r].K3, tmp, n);
+        LAUNCHCHK(c);
+    }
+    c->fl32 = 0;
+    c->r32_fresh = 0;
+    return 0;
+}
+static int fl_narrow(fdga_ctx* c) {
+    if (c->fl32) return 0;
+    const size_t n = (size_t)c->n3 * c->NP;
+    for (int r = 0; r < 3; ++r) {
+        float2* tmp = reinterpret_cast<float2*>(c->R[r]);
+        k_narrow32<<<cdiv(n, 256), 256, 0, c->run>>>(tmp, c->FL[r].K3, n);
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(c->FL[r].K3, tmp, n * sizeof(float2), hipMemcpyDeviceToDevice, c->run));
+    }
+    c->fl32 = 1;
+    c->r32_fresh = 0;
+    return 0;
+}
+// the FL.K3 form an iteration of this context works in
+static int fl_prepare(fdga_ctx* c) { return fl32_mode(c) ? fl_narrow(c) : fl_widen(c); }
 
 static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     const bool sym = k3_sym(c, chans, n, true);
@@ -974,14 +1019,19 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     for (int s = 0; s < n; ++s) bits |= 1 << chans[s];
     c->r32_fresh &= ~bits;
     if (split_t(c, chans, n, sym)) {
-        const LK3Set l1 = lk3_set(c, kChansPA, 2, false, false), l2 = lk3_set(c, kChansT, 1, false, true);
+        // (all three channels are written: FL.K3 takes the FP32 form on the FP32 lattice path)
+        const bool f32 = fl32_mode(c);
+        if (!f32 && fl_widen(c)) return -3;
+        const LK3Set l1 = lk3_set(c, kChansPA, 2, false, false, f32), l2 = lk3_set(c, kChansT, 1, false, true, f32);
         kernel_begin(c);
         if (lk3_gemm(c, kChansPA, 2, l1) != hipSuccess || lk3_gemm(c, kChansT, 1, l2) != hipSuccess)
             return fail(c, "BSE_L_K3 GEMM launch failed", -3);
         kernel_end(c, 9);
+        c->fl32 = f32;
         if (lk3_writes_r32(c, false)) c->r32_fresh |= bits;
         return 0;
     }
+    if (fl_widen(c)) return -3;   // a subset of the channels, or symmetry groups: FP64 FL.K3
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const LK3Set ls = lk3_set(c, chans, n, sym, t_epi);
     kernel_begin(c);
@@ -1011,7 +1061,9 @@ struct K3Set {
 // mfrg: the is_mfRG = Val(true) form (src/BSEa/BSEa_K3.jl:83-97): R = Π0 (s1 Γ~ + s2 L~) (the right-operand
 // builder with Π0 in place of Π), A = the F caches of build_K3_cache_mfRG! as they are (no K2/K3 of the
 // target vertex added)
-static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, bool mfrg = false) {
+// fl32: FL.K3 in its FP32 form (-1: as the context holds it now; solver groups fix it per member)
+static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, bool mfrg = false, int fl32 = -1) {
+    const bool f32 = fl32 < 0 ? c->fl32 != 0 : fl32 != 0;
     K3Set ks{};
     ks.g = k3geom(c);
     ks.T = c->d.T;
@@ -1028,6 +1080,8 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, 
         rc[s].Gam = c->cache[(ch == PCH) ? 0 : (ch == ACH) ? 10 : 11];
         rc[s].Lr = c->FL[ch].K3;
         rc[s].La = c->FL[ACH].K3;
+        rc[s].Lr32 = f32 ? reinterpret_cast<const float2*>(c->FL[ch].K3) : nullptr;
+        rc[s].La32 = f32 ? reinterpret_cast<const float2*>(c->FL[ACH].K3) : nullptr;
         rc[s].R = c->R[ch];
         rc[s].csum = c->csum[ch];
         kc[s].ch = ch;
@@ -1041,6 +1095,8 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, 
         kc[s].csum = c->csum[ch];
         kc[s].Lr = c->FL[ch].K3;
         kc[s].La = c->FL[ACH].K3;
+        kc[s].Lr32 = f32 ? reinterpret_cast<const float2*>(c->FL[ch].K3) : nullptr;
+        kc[s].La32 = f32 ? reinterpret_cast<const float2*>(c->FL[ACH].K3) : nullptr;
         kc[s].out = (sym && ch == TCH) ? c->k3raw : c->Fb[ch].K3;
         kc[s].outa = c->Fb[ACH].K3;
         kc[s].tpost = (ch == TCH && t_epi);
@@ -1446,6 +1502,7 @@ int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
     c->r32_fresh = 0;
     ArrInfo a;
     if (arr_info(c, which, a)) return fail(c, "unknown array id " + std::to_string(which));
+    if (which >= FDGA_VERTEX(2, 0, 1) && which <= FDGA_VERTEX(2, 2, 3) && fl_widen(c)) return -3;
     if (which == FDGA_LPP || which == FDGA_LPH || a.kind == 3 || a.kind == 4) return fail(c, "array is read-only");
     const size_t bytes = a.n * sizeof(double2);
     if (a.kind == 0) {
@@ -1475,6 +1532,7 @@ int fdga_get(fdga_ctx* c, int32_t which, double* host) {
     if (!c || !host) return -1;
     ArrInfo a;
     if (arr_info(c, which, a)) return fail(c, "unknown array id " + std::to_string(which));
+    if (which >= FDGA_VERTEX(2, 0, 1) && which <= FDGA_VERTEX(2, 2, 3) && fl_widen(c)) return -3;   // FL.K3 in FP64
     if ((which == FDGA_PIPP && (c->pi_band & 1)) || (which == FDGA_PIPH && (c->pi_band & 2))) {
         if (!c->pi_band_G)
             return fail(c, "Πpp/Πph are stored on the band only and G has changed since the bubbles were computed: "
@@ -1944,7 +2002,8 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma, int32_t co
     if (!c) return -1;
     if (strategy < FDGA_SCPA || strategy > FDGA_FDPA_NEW) return fail(c, "Calculation strategy unknown");
     const bool fd = (strategy == FDGA_FDPA || strategy == FDGA_FDPA_NEW);
-    c->caches_mfrg = 0;   // the iteration's build_K3_cache! (also when replayed from a graph)
+    c->caches_mfrg = 0;
+    if (fl_prepare(c)) return -3;   // eagerly: a captured iteration finds FL.K3 in its form   // the iteration's build_K3_cache! (also when replayed from a graph)
     // graph replay: the fixed-point iteration with Σ update and the Hartree term, stage timing off, and
     // for the fdPA strategies once the reference system's SDE is cached (the first such iteration
     // computes it eagerly)
@@ -2095,7 +2154,7 @@ int fdga_fixed_point_noalloc(fdga_ctx* c, double* R, const double* x, int32_t st
     if (unflatten_into(c, x, update_sigma != 0, 1.0, c->scratch)) return -3;
     // always the eager launches: capturing and instantiating a hipGraph allocates
     c->caches_mfrg = 0;
-    if (iterate_eager(c, strategy, update_sigma, compute_hartree)) return -3;
+    if (fl_prepare(c) || iterate_eager(c, strategy, update_sigma, compute_hartree)) return -3;
     if (flatten_from(c, R, update_sigma != 0, c->scratch)) return -3;
     for (size_t i = 0; i < 2 * n; ++i) R[i] -= x[i];
     return 0;
@@ -2253,6 +2312,7 @@ int fdga_autotune(fdga_ctx* c, int32_t reps) {
     if (!c || reps < 1) return -1;
     const int all[3] = {PCH, ACH, TCH};
     c->run = c->st;
+    c->r32_fresh = 0;   // the right-operand buffers serve as scratch outputs below
     if (fdga_build_k3_cache(c)) return -3;
     const bool symL = k3_sym(c, all, 3, true), symK = k3_sym(c, all, 3, false);
     LK3Set ls = lk3_set(c, all, 3, symL, false);
@@ -2460,7 +2520,8 @@ int group_build_functors(fdga_group* g) {
     std::vector<K3_Bop32> b32[2];
     std::vector<K3_Ep> ke[2];
     for (fdga_ctx* c : g->m) {
-        const LK3Set ls = lk3_set(c, all, 3, false, g->nonlocal);
+        const bool f32 = fl32_mode(c);   // the FL.K3 form of the member's iterations (fl_prepare)
+        const LK3Set ls = lk3_set(c, all, 3, false, g->nonlocal, f32);
         la.push_back(ls.a);
         lb.push_back(ls.b);
         le.push_back(ls.e);
@@ -2468,7 +2529,7 @@ int group_build_functors(fdga_group* g) {
         g->lk3_ro = (c == g->m[0] ? 1 : g->lk3_ro) && ls.ro;
         for (int k = 0; k < 2; ++k) {
             const int s = c->swap_state;
-            const K3Set ks = k3_set(c, all, 3, false, g->nonlocal);
+            const K3Set ks = k3_set(c, all, 3, false, g->nonlocal, false, f32 ? 1 : 0);
             af[s].push_back(ks.a<true>());
             an[s].push_back(ks.a<false>());
             a32[s].push_back(ks.a32());
@@ -2714,7 +2775,11 @@ int fdga_group_iterate(fdga_group* g, int32_t strategy, int32_t update_sigma) {
         return gfail(g, "the :scPA_new / :fdPA_new strategies are not on the solver-group path; iterate the members alone");
     if (strategy != FDGA_SCPA && strategy != FDGA_FDPA) return gfail(g, "Calculation strategy unknown");
     bool sde0 = true;
-    for (const fdga_ctx* c : g->m) sde0 = sde0 && c->sde0_valid;
+    for (fdga_ctx* c : g->m) {
+        sde0 = sde0 && c->sde0_valid;
+        c->run = c->st;
+        if (fl_prepare(c)) return member_err(g, c);
+    }
     if (g->graph_on && update_sigma && !g->timing && (strategy == FDGA_SCPA || sde0)) {
         const int s = g->m[0]->swap_state;
         for (const fdga_ctx* c : g->m)

@@ -907,6 +907,10 @@ struct LK3Chan {
     const double2* Gam;    // p: Γpx [ω, νp, Ω]; a, t: Γr transposed [ω, ν, Ω]
     double s1, s2;
     int ch;
+    // FL.K3 kept in FP32 (the first half of the FL.K3 array's bytes): out32 replaces out, and the t
+    // post-processing reads the finished a channel from outa32
+    float2* out32;
+    const float2* outa32;
 };
 // Loaders and epilogues resolve their batch (channel slot, Ω, P) once per workgroup with bind(b),
 // which returns base pointers for that batch; the per-element calls do only the in-matrix offset.
@@ -968,6 +972,7 @@ struct LK3BoundPlain {
 };
 struct LK3BoundR {
     double2* out; const double2* outa; double scale; int tpost; int NV;
+    float2* out32; const float2* outa32;
     float2* R32; const double2* pi; const double2* pi0; const double2* F0r; const double2* Gam;
     double s1, s2; int m, n3f, ch;
     // The element at row i is L~ at the R row whose crossing is row i (p: ω = Ω - ν - 1, an
@@ -991,7 +996,13 @@ struct LK3BoundR {
         q.gm = Gam[kj];
         q.P1 = pi[k];
         q.P0 = pi0[k];
-        q.oa = tpost ? outa[(size_t)i + (size_t)NV * j] : czero();
+        if (tpost) {
+            const size_t o = (size_t)i + (size_t)NV * j;
+            if (outa32) { const float2 a = outa32[o]; q.oa = cmk(a.x, a.y); }
+            else q.oa = outa[o];
+        } else {
+            q.oa = czero();
+        }
         const double dr = q.P1.x - q.P0.x, di = q.P1.y - q.P0.y;
         double rr = (dr * q.f0.x - di * q.f0.y) + (q.P1.x * q.gm.x - q.P1.y * q.gm.y);
         double ri = (dr * q.f0.y + di * q.f0.x) + (q.P1.x * q.gm.y + q.P1.y * q.gm.x);
@@ -1006,7 +1017,8 @@ struct LK3BoundR {
             xr = (xr + q.oa.x) * 0.5;
             xi = (xi + q.oa.y) * 0.5;
         }
-        out[(size_t)i + (size_t)NV * j] = cmk(xr, xi);
+        if (out32) out32[(size_t)i + (size_t)NV * j] = make_float2((float)xr, (float)xi);
+        else out[(size_t)i + (size_t)NV * j] = cmk(xr, xi);
     }
 };
 template <bool RO>
@@ -1021,6 +1033,8 @@ struct LK3_EpT {
         r.outa = c.tpost ? c.outa + off : nullptr;
         r.scale = c.scale; r.tpost = c.tpost; r.NV = NV;
         if constexpr (RO) {
+            r.out32 = c.out32 ? c.out32 + off : nullptr;
+            r.outa32 = (c.tpost && c.outa32) ? c.outa32 + off : nullptr;
             const int iW = bb % NW, p = bb / NW, m = iW - (g.n3b - 1);
             r.R32 = c.R32 + off;
             r.pi = c.Pi + g.pidx(m, -g.n3f, p);
@@ -1065,6 +1079,9 @@ struct K3Chan {
     // A32 holds the K2r(ν) term as well (csum null: no rank-1 term in the epilogue), read from
     // K2row[i + Nν3 (iΩ + NΩ3 P)] = K2r(Ω, ν = i - n3f, P) (t: 2 K2t - K2a), written by k_k2_rows
     double2* K2row;
+    // FL.K3 in FP32 (LK3Chan::out32): the self term is read from these when set
+    const float2* Lr32;
+    const float2* La32;
 };
 // FULL: the cache holds the whole cache_Fr (one momentum, a single load per element)
 template <bool FULL>
@@ -1215,6 +1232,7 @@ struct K3_Ep {
         const double2* k2r; const double2* k2a;    // k2r[NW2 * i] = K2r(Ω, ν = i - n3f, P)
         const double2* csum; const double2* Lr; const double2* La;
         double2* out; const double2* outa;
+        const float2* Lr32; const float2* La32;
         double T; int t, tpost, full, NV, NW2;
         // the element's loads (FL.K3, and for t FL.a and the finished a channel) issued together,
         // before any arithmetic
@@ -1222,8 +1240,14 @@ struct K3_Ep {
         __device__ __forceinline__ Pre pre(int i, int j) const {
             const size_t o = (size_t)i + (size_t)NV * j;
             Pre q;
-            q.self = Lr[o];
-            if (t) q.self = csub(cscale(q.self, 2.0), La[o]);
+            if (Lr32) {
+                const float2 l = Lr32[o];
+                q.self = cmk(l.x, l.y);
+                if (t) { const float2 a = La32[o]; q.self = csub(cscale(q.self, 2.0), cmk(a.x, a.y)); }
+            } else {
+                q.self = Lr[o];
+                if (t) q.self = csub(cscale(q.self, 2.0), La[o]);
+            }
             q.oa = tpost ? outa[o] : czero();
             return q;
         }
@@ -1255,6 +1279,8 @@ struct K3_Ep {
         r.csum = cs ? c.csum + (size_t)NV * bb : nullptr;
         r.Lr = c.Lr + off;
         r.La = t ? c.La + off : nullptr;
+        r.Lr32 = c.Lr32 ? c.Lr32 + off : nullptr;
+        r.La32 = (c.Lr32 && t) ? c.La32 + off : nullptr;
         r.out = c.out + off;
         r.outa = c.tpost ? c.outa + off : nullptr;
         r.T = T; r.t = t; r.tpost = c.tpost; r.full = c.full; r.NV = NV; r.NW2 = NW2;
@@ -1279,7 +1305,13 @@ struct K3RChan {
     double2* R;           // [ω, νp, Ω, P]
     double2* csum;        // [νp, Ω, P]
     float2* R32;          // when set: R rounded to FP32 goes here instead of R
+    const float2* Lr32;   // FL.K3 in FP32 (its FP32 form), read instead of Lr / La when set
+    const float2* La32;
 };
+__device__ __forceinline__ double2 k3r_fl(const K3RChan& c, bool a, size_t i) {
+    if (c.Lr32) { const float2 v = (a ? c.La32 : c.Lr32)[i]; return cmk(v.x, v.y); }
+    return (a ? c.La : c.Lr)[i];
+}
 // K3R_COLS columns νp = j0 .. j0 + K3R_COLS - 1 per wave: the Π loads are shared and the columns'
 // loads issued together
 constexpr int K3R_COLS = 4;
@@ -1308,8 +1340,8 @@ __device__ __forceinline__ void k3_R_columns(const K3Geom& g, const K3RChan& c, 
                 gm[u] = c.Gam[kj];
                 if (inb) {
                     const size_t lo = (size_t)(w_ + g.n3f) + (size_t)NV * ((size_t)j + (size_t)NV * bb);
-                    L[u] = c.Lr[lo];
-                    if (CH == TCH) La[u] = c.La[lo];
+                    L[u] = k3r_fl(c, false, lo);
+                    if (CH == TCH) La[u] = k3r_fl(c, true, lo);
                 }
             }
         }
@@ -1388,8 +1420,8 @@ __device__ __forceinline__ void k3_R_columns_pchunk(const K3Geom& g, const K3RCh
                 L[q][u] = La[q][u] = czero();
                 if (inb[q] && j0 + u < NV) {
                     const size_t lo = (size_t)(w_[q] + g.n3f) + (size_t)NV * ((size_t)(j0 + u) + (size_t)NV * bb);
-                    L[q][u] = c.Lr[lo];
-                    if (CH == TCH) La[q][u] = c.La[lo];
+                    L[q][u] = k3r_fl(c, false, lo);
+                    if (CH == TCH) La[q][u] = k3r_fl(c, true, lo);
                 }
             }
         }
@@ -1445,6 +1477,16 @@ __global__ __launch_bounds__(256) void k_k3_R(K3Geom g, K3RChan c0, K3RChan c1, 
     if (s == 0) k3_R_dispatch(g, c0, q * K3R_COLS, bb, lane);
     else if (s == 1) k3_R_dispatch(g, c1, q * K3R_COLS, bb, lane);
     else k3_R_dispatch(g, c2, q * K3R_COLS, bb, lane);
+}
+
+// FL.K3 between its FP32 form (in the first half of the array's bytes) and FP64, through a temporary
+__global__ void k_widen32(double2* __restrict__ dst, const float2* __restrict__ src, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) { const float2 v = src[i]; dst[i] = cmk(v.x, v.y); }
+}
+__global__ void k_narrow32(float2* __restrict__ dst, const double2* __restrict__ src, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) { const double2 v = src[i]; dst[i] = make_float2((float)v.x, (float)v.y); }
 }
 
 // t channel post-processing after a multi-channel launch: γt = (γt_d + γa) / 2

@@ -21,6 +21,9 @@ EXTRA_CASES = {
     "nl_L24": (True, 24, 24, 0, False),
     "nl_L6": (True, 6, 6, 1, True),
     "nl_LG10_L5": (True, 10, 5, 0, False),
+    # 16 x 16 lattice with a reference vertex: non-zero FL, and enough (Ω, P) batches for the
+    # lattice contraction's two-launch form
+    "nl_L16_ref": (True, 16, 16, 1, True),
 }
 
 BOX = dict(nG=10, nK1=6, nK2=(4, 4), nK3=(3, 3), mPinu=2, T=0.3, U=1.3)

@@ -227,6 +227,38 @@ def test_fp32_contraction_iteration(case, fdga_mod):
     assert 1e-12 < e < 1e-5, e
 
 
+def test_fp32_lattice_keeps_fl_k3_in_fp32(fdga_mod):
+    """The FP32 lattice iteration (768 (Ω, P) batches, no K3 symmetry groups) stores FL.K3 in FP32:
+    read back it holds FP32-representable values within FP32 rounding of the oracle's FL.K3; the
+    read (which restores the FP64 form) leaves the following iterations bit-identical to those of an
+    unread twin, a graph replay agrees with them, and three iterations stay within FP32 rounding of
+    the oracle."""
+    from fdga import _lib as L
+    A, O = make_pair("nl_L16_ref", seed=4, precision="fp32")
+    B, _ = make_pair("nl_L16_ref", seed=4, precision="fp32")
+    C, _ = make_pair("nl_L16_ref", seed=4, precision="fp32")
+    C.set_graph(True)
+    for it in range(3):
+        for S in (A, B, C):
+            fdga_mod.iterate_solver(S, "fdPA")
+        O.iterate(po.FDPA)
+        if it == 0:
+            for ch in range(3):
+                fl = B.get(L.vertex_id(2, ch, 3))
+                ref = O.get(po.vertex_id(po.VTX_FL, ch, 3))
+                assert np.abs(ref).max() > 1e-3
+                assert np.array_equal(fl.real.astype(np.float32).astype(np.float64), fl.real)
+                assert np.array_equal(fl.imag.astype(np.float32).astype(np.float64), fl.imag)
+                assert relerr(fl, ref) < 1e-5, f"FL.K3 ch={ch}: {relerr(fl, ref):.3e}"
+        xa, xb, xc = fdga_mod.flatten(A), fdga_mod.flatten(B), fdga_mod.flatten(C)
+        assert np.array_equal(xb, xa), f"iteration {it}: {relerr(xb, xa):.3e}"
+        assert relerr(xc, xa) < 1e-14, f"replay {it}: {relerr(xc, xa):.3e}"
+    e = relerr(xa, O.flatten())
+    assert 1e-12 < e < 1e-5, e
+    for S in (A, B, C):
+        S.close()
+
+
 @pytest.mark.parametrize("case,strategy", [("local_vertexref", "fdPA"), ("nl_L2", "fdPA"), ("local_ref", "scPA")])
 def test_graph_replay_equals_eager(case, strategy, fdga_mod):
     """hipGraph replay of the iteration: four replays in a row, each followed by eager work
