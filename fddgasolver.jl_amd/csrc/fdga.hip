@@ -16,6 +16,7 @@
 
 #include "../../include/fdga.h"
 #include "fdga_kernels.h"
+#include "fdga_adjoint.h"
 #include "fdga_forms.h"
 #include "fdga_sym.h"
 
@@ -135,6 +136,14 @@ struct fdga_ctx {
     int timing = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double stage_ms[12] = {};   // 0..8 stages of fdga_iterate, 9 / 10 the BSE_L_K3! / BSE_K3! GEMM kernels, 11 the bubble kernel
+    // a tangent context's scratch for the adjoint of BSE_K3! (first fdga_bse_k3_vjp on it): the cache
+    // cotangents [7 n3], the averaged vertex cotangent [3 (n1 + n2 + n3)], one channel's Ā [n3 NP] and
+    // the gather's transposed map (destination offsets, sources, coefficients)
+    double2 *adj_cbar = nullptr, *adj_avg = nullptr, *adj_Ab = nullptr;
+    int64_t* adj_off = nullptr;
+    int* adj_src = nullptr;
+    float* adj_cf = nullptr;
+    size_t adj_nd = 0;
 };
 
 // drops the captured hipGraphs of the iteration (defined with the graph replay below)
@@ -1420,6 +1429,9 @@ int fdga_destroy(fdga_ctx* c) {
     if (c->k3raw) (void)hipFree(c->k3raw);
     if (c->k2raw) (void)hipFree(c->k2raw);
     if (c->k3_pl) (void)hipFree(c->k3_pl);
+    for (void* q : {(void*)c->adj_cbar, (void*)c->adj_avg, (void*)c->adj_Ab, (void*)c->adj_off, (void*)c->adj_src,
+                    (void*)c->adj_cf})
+        if (q) (void)hipFree(q);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->st2) (void)hipStreamSynchronize(c->st2);
     if (c->evfork) (void)hipEventDestroy(c->evfork);
@@ -2907,6 +2919,179 @@ int fdga_group_autotune(fdga_group* g, int32_t reps) {
     g->timing = keep_timing;
     group_graphs_invalidate(g);
     return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Tangent and adjoint of build_K3_cache! followed by BSE_K3! of the three channels (fdga_adjoint.h).
+// The tangent context `t` holds the tangent inputs (JVP: F, FL.K3, Πpp/Πph; output Fbuff.K3) or the
+// cotangent (VJP: Fbuff.K3; outputs F, FL.K3, Πpp/Πph).  Everything runs on the solver's stream.
+namespace {
+
+adj::Boxes adj_boxes(const fdga_dims& d) { return adj::Boxes{d.nK1, d.nK2b, d.nK2f, d.nK3b, d.nK3f}; }
+
+int adj_check(fdga_ctx* c, fdga_ctx* t) {
+    const fdga_dims &a = c->d, &b = t->d;
+    if (a.nonlocal != b.nonlocal || a.nG != b.nG || a.nK1 != b.nK1 || a.nK2b != b.nK2b || a.nK2f != b.nK2f ||
+        a.nK3b != b.nK3b || a.nK3f != b.nK3f || a.mPinu != b.mPinu || a.L != b.L || a.LG != b.LG)
+        return fail(c, "tangent context: its dimensions differ from the solver's");
+    if (a.precision != FDGA_FP64 || b.precision != FDGA_FP64)
+        return fail(c, "the tangent and adjoint of BSE_K3! run on FP64 contexts only");
+    if (b.ref_kind != 0 || b.U != 0.0 || t->tabs_nonzero)
+        return fail(c, "tangent context: needs U = 0 and no reference vertex (ref_kind 0, zero RefVertex tables)");
+    if (has_k3_groups(c)) return fail(c, "the tangent and adjoint of BSE_K3! need the identity K3 symmetry groups");
+    c->run = c->st;
+    t->run = c->st;   // the tangent context's kernels on the solver's stream (restored by adj_done)
+    HIPCHK(c, hipStreamSynchronize(t->st));
+    return 0;
+}
+int adj_done(fdga_ctx* c, fdga_ctx* t) {
+    t->run = t->st;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+AdjAChan adj_chan_a(const fdga_ctx* c, int ch) {
+    return AdjAChan{ch, c->NP == 1, c->cache[idx_fbase(ch)], c->F[ch].K2, c->F[ACH].K2, c->F[ch].K3, c->F[ACH].K3};
+}
+const double2* adj_gam(const fdga_ctx* c, int ch) { return c->cache[ch == PCH ? 0 : ch == ACH ? 10 : 11]; }
+// the linearisation point: the solver's caches and FP64 right operands from its current F, FL and Π
+int adj_point(fdga_ctx* c) {
+    const int all[3] = {PCH, ACH, TCH};
+    if (fdga_build_k3_cache(c)) return -3;
+    return k3_rhs(c, k3_set(c, all, 3, false, false), 3) ? -3 : 0;
+}
+template <class AL, class BL, class EP>
+int adj_gemm(fdga_ctx* c, int K, const AL& a, const BL& b, const EP& e) {
+    const int NV = c->NV3;
+    const GemmSchedOne sc = sched_one(NV, NV, K, cdiv(NV, 64), cdiv(NV, 64), c->NW3 * c->NP, nullptr, 0);
+    if (zgemm_run(FDGA_FP64, TILE_64x64, (size_t)sc.nitems, c->st, sc, a, b, e) != hipSuccess)
+        return fail(c, "tangent / adjoint GEMM launch failed", -3);
+    return 0;
+}
+AdjRT adj_rt(fdga_ctx* c, int ch, const double2* Rb) {
+    return AdjRT{ch, (ch == ACH) ? 1.0 : -1.0, (ch == TCH) ? -1.0 : 1.0, pi_of(c, ch, false), c->cache[idx_f0(ch)],
+                      adj_gam(c, ch), c->FL[ch].K3, c->FL[ACH].K3, Rb};
+}
+// the transposed gather map of a tangent context, built and uploaded once
+int adj_ensure(fdga_ctx* c, fdga_ctx* t) {
+    if (t->adj_off) return 0;
+    adj::GatherMap gm;
+    if (!adj::gather_map(adj_boxes(t->d), t->NP == 1, gm)) return fail(c, "the gather map of these boxes does not fit");
+    const adj::Boxes B = adj_boxes(t->d);
+    t->adj_nd = B.ndst();
+    if (dalloc(t, &t->adj_cbar, (size_t)adj::kSlots * t->n3) || dalloc(t, &t->adj_avg, t->adj_nd) ||
+        dalloc(t, &t->adj_Ab, t->n3 * t->NP))
+        return fail(c, "tangent context: " + t->err, -2);
+    HIPCHK(c, hipMalloc((void**)&t->adj_off, gm.off.size() * sizeof(int64_t)));
+    HIPCHK(c, hipMalloc((void**)&t->adj_src, std::max<size_t>(gm.src.size(), 1) * sizeof(int)));
+    HIPCHK(c, hipMalloc((void**)&t->adj_cf, std::max<size_t>(gm.cf.size(), 1) * sizeof(float)));
+    HIPCHK(c, hipMemcpy(t->adj_off, gm.off.data(), gm.off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (!gm.src.empty()) {
+        HIPCHK(c, hipMemcpy(t->adj_src, gm.src.data(), gm.src.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(t->adj_cf, gm.cf.data(), gm.cf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t fdga_k3_gather_map(const fdga_dims* d, int64_t* off, int32_t* src, float* coef) {
+    if (!d || d->nK1 < 1 || d->nK2b < 1 || d->nK2f < 1 || d->nK3b < 1 || d->nK3f < 1) return -1;
+    adj::GatherMap gm;
+    if (!adj::gather_map(adj_boxes(*d), d->nonlocal == 0 || d->L == 1, gm)) return -1;
+    if (off) std::memcpy(off, gm.off.data(), gm.off.size() * sizeof(int64_t));
+    if (src) std::memcpy(src, gm.src.data(), gm.src.size() * sizeof(int32_t));
+    if (coef) std::memcpy(coef, gm.cf.data(), gm.cf.size() * sizeof(float));
+    return (int64_t)gm.src.size();
+}
+
+int fdga_bse_k3_jvp(fdga_ctx* c, fdga_ctx* t) {
+    if (!c || !t) return -1;
+    if (int rc = adj_check(c, t)) return rc;
+    if (adj_point(c) || refresh_avg(t) || fdga_build_k3_cache(t)) return -3;   // tangent caches: F0 = 0 in t
+    const K3Geom g = k3geom(c);
+    const size_t n = c->n3 * c->NP;
+    for (int ch : {PCH, ACH, TCH}) {
+        const AdjR r{ch, (ch == ACH) ? 1.0 : -1.0, (ch == TCH) ? -1.0 : 1.0, pi_of(c, ch, false), pi_of(t, ch, false),
+                          c->cache[idx_f0(ch)], adj_gam(c, ch), adj_gam(t, ch), c->FL[ch].K3, c->FL[ACH].K3,
+                          t->FL[ch].K3, t->FL[ACH].K3, t->R[ch]};
+        k_adj_dR<<<cdiv(n, 256), 256, 0, c->st>>>(g, r);
+        LAUNCHCHK(c);
+        const AdjJvpA A{g, adj_chan_a(t, ch), adj_chan_a(c, ch)};
+        const AdjJvpB B{g, c->R[ch], t->R[ch]};
+        const AdjJvpE E{g, c->d.T, ch == TCH, t->FL[ch].K3, t->FL[ACH].K3, t->Fb[ch].K3};
+        if (adj_gemm(c, 2 * c->NV3, A, B, E)) return -3;
+    }
+    k_adj_tpost<<<cdiv(n, 256), 256, 0, c->st>>>(t->Fb[TCH].K3, t->Fb[ACH].K3, n);
+    LAUNCHCHK(c);
+    return adj_done(c, t);
+}
+
+int fdga_bse_k3_vjp(fdga_ctx* c, fdga_ctx* t) {
+    if (!c || !t) return -1;
+    if (int rc = adj_check(c, t)) return rc;
+    if (adj_ensure(c, t) || adj_point(c)) return -3;
+    const K3Geom g = k3geom(c);
+    const size_t n = c->n3 * c->NP, n3 = c->n3;
+    const double T = c->d.T;
+    // raw cotangents: the t post-processing transposed, in place
+    k_adj_tpost_t<<<cdiv(n, 256), 256, 0, c->st>>>(t->Fb[TCH].K3, t->Fb[ACH].K3, n);
+    LAUNCHCHK(c);
+    // outputs start at zero: the vertex cotangent accumulates, Π̄ is written on the K3 box only
+    for (int r = 0; r < 3; ++r)
+        for (double2* q : {t->F[r].K1, t->F[r].K2, t->F[r].K3}) {
+            const size_t m = q == t->F[r].K1 ? c->n1 * c->NP : q == t->F[r].K2 ? c->n2 * c->NP : n;
+            k_zero<<<cdiv(m, 256), 256, 0, c->st>>>(q, m);
+        }
+    k_zero<<<cdiv(c->nPiel, 256), 256, 0, c->st>>>(t->Pipp, c->nPiel);
+    k_zero<<<cdiv(c->nPiel, 256), 256, 0, c->st>>>(t->Piph, c->nPiel);
+    k_zero<<<cdiv((size_t)adj::kSlots * n3, 256), 256, 0, c->st>>>(t->adj_cbar, (size_t)adj::kSlots * n3);
+    LAUNCHCHK(c);
+    // R̄_r = T A_r^T Ḡ_r (into t's right-operand buffers)
+    for (int ch : {PCH, ACH, TCH})
+        if (adj_gemm(c, c->NV3, AdjAT{g, adj_chan_a(c, ch)}, AdjDense<false>{g, t->Fb[ch].K3},
+                     AdjOut{g, t->R[ch], T}))
+            return -3;
+    const AdjRT rp = adj_rt(c, PCH, t->R[PCH]), ra = adj_rt(c, ACH, t->R[ACH]), rt = adj_rt(c, TCH, t->R[TCH]);
+    k_adj_pibar<<<cdiv((size_t)c->NV3 * c->NW3 * c->NP, 256), 256, 0, c->st>>>(g, rp, ra, rt, t->Pipp, t->Piph);
+    k_adj_gambar<<<cdiv(n3, 256), 256, 0, c->st>>>(g, rp, ra, rt, t->adj_cbar);
+    k_adj_flbar<<<cdiv(n, 256), 256, 0, c->st>>>(g, rp, ra, rt, t->Fb[PCH].K3, t->Fb[ACH].K3, t->Fb[TCH].K3,
+                                                      t->FL[PCH].K3, t->FL[ACH].K3, t->FL[TCH].K3);
+    LAUNCHCHK(c);
+    // Ā_r = T Ḡ_r R_r^T: its momentum sum is the Fbase cotangent; on the lattice the same-channel
+    // K2r + K3r terms of A take it per momentum
+    for (int ch : {PCH, ACH, TCH}) {
+        if (adj_gemm(c, c->NV3, AdjDense<false>{g, t->Fb[ch].K3}, AdjDense<true>{g, c->R[ch]},
+                     AdjOut{g, t->adj_Ab, T}))
+            return -3;
+        const int slot = ch == PCH ? 4 : ch == ACH ? 5 : 6;
+        k_adj_fbase<<<cdiv(n3, 256), 256, 0, c->st>>>(g, t->adj_Ab, t->adj_cbar + (size_t)slot * n3);
+        LAUNCHCHK(c);
+        if (c->NP > 1) {
+            const unsigned nb = cdiv((size_t)c->NV3 * c->NW3 * c->NP, 256);
+            k_adj_own<<<nb, 256, 0, c->st>>>(g, t->adj_Ab, ch == TCH ? 2.0 : 1.0, t->F[ch].K2, t->F[ch].K3);
+            if (ch == TCH) k_adj_own<<<nb, 256, 0, c->st>>>(g, t->adj_Ab, -1.0, t->F[ACH].K2, t->F[ACH].K3);
+            LAUNCHCHK(c);
+        }
+    }
+    // the gather's transpose onto the averaged vertex, then the average's transpose (1 / NP per momentum)
+    k_adj_gather_t<<<cdiv(t->adj_nd * WAVE, 256), 256, 0, c->st>>>(t->adj_avg, t->adj_off, t->adj_src, t->adj_cf,
+                                                                         t->adj_cbar, t->adj_nd);
+    LAUNCHCHK(c);
+    const size_t per = c->n1 + c->n2 + c->n3;
+    for (int r = 0; r < 3; ++r) {
+        const double2* a = t->adj_avg + (size_t)r * per;
+        k_adj_bcast<<<cdiv(c->n1 * c->NP, 256), 256, 0, c->st>>>(t->F[r].K1, a, c->n1, c->NP, 1.0 / c->NP);
+        k_adj_bcast<<<cdiv(c->n2 * c->NP, 256), 256, 0, c->st>>>(t->F[r].K2, a + c->n1, c->n2, c->NP, 1.0 / c->NP);
+        k_adj_bcast<<<cdiv(n, 256), 256, 0, c->st>>>(t->F[r].K3, a + c->n1 + c->n2, c->n3, c->NP, 1.0 / c->NP);
+    }
+    LAUNCHCHK(c);
+    t->pi_band = 0;
+    t->r32_fresh = 0;
+    return adj_done(c, t);
 }
 
 }  // extern "C"

@@ -86,6 +86,10 @@ SIGNATURES = {
     "fdga_fixed_point": (ctypes.c_int, [_vp, _dp, _dp, _i32, _i32, _i32]),
     "fdga_fixed_point_noalloc": (ctypes.c_int, [_vp, _dp, _dp, _i32, _i32, _i32]),
     "fdga_mfrg_apply": (ctypes.c_int, [_vp, _dp, _dp, _i32, _i32]),
+    "fdga_bse_k3_jvp": (ctypes.c_int, [_vp, _vp]),
+    "fdga_bse_k3_vjp": (ctypes.c_int, [_vp, _vp]),
+    "fdga_k3_gather_map": (_i64, [ctypes.POINTER(Dims), ctypes.POINTER(_i64), ctypes.POINTER(_i32),
+                                  ctypes.POINTER(ctypes.c_float)]),
     "fdga_sync": (ctypes.c_int, [_vp]),
     "fdga_time_iterations": (ctypes.c_int, [_vp, _i32, _i32, _dp]),
     "fdga_stage_ms": (ctypes.c_int, [_vp, _i32, _dp]),

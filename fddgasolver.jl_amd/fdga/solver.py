@@ -411,6 +411,51 @@ def BSE_K3(S, Ch, is_mfRG=False):
     S._check(S.lib.fdga_bse_k3(S.h, Ch, int(is_mfRG)))
 
 
+def tangent_solver(S):
+    """A context with S's dimensions and precision, U = 0 and no reference vertex: the carrier of the
+    tangent and cotangent arrays of BSE_K3_jvp / BSE_K3_vjp (F, FL.K3, Πpp/Πph and Fbuff.K3)."""
+    return Solver(nG=S.nG, nK1=S.nK1, nK2=S.nK2, nK3=S.nK3, T=S.T, U=0.0, mPinu=S.mPinu, nonlocal_=S.nonlocal_,
+                  LG=S.LG, L_=S.L, precision=S.precision)
+
+
+def BSE_K3_jvp(S, Tn):
+    """Tangent of build_K3_cache!(S) followed by BSE_K3!(S, Ch) for Ch = p, a, t (src/build_K3_cache.jl:20-95,
+    src/BSEa/BSEa_K3.jl:43-128, src/BSE_templates.jl:151-180) at S's current F, FL and Π: reads the tangent
+    F, FL.K3, Πpp, Πph from Tn (a tangent_solver(S)) and writes the tangent Fbuff.K3 into Tn."""
+    S._check(S.lib.fdga_bse_k3_jvp(S.h, Tn.h))
+
+
+def BSE_K3_vjp(S, Tn):
+    """Adjoint of the same map (pairing sum(u * v), no conjugation): reads the cotangent of Fbuff.K3 from Tn
+    (overwritten) and writes the cotangents of F (K1, K2, K3), FL.K3, Πpp and Πph into Tn."""
+    S._check(S.lib.fdga_bse_k3_vjp(S.h, Tn.h))
+
+
+def k3_gather_map(nK1, nK2, nK3, lattice=False):
+    """The gather of build_K3_cache! as a transposed, destination-sorted linear map (host only, no
+    device): (off, src, coef) with the cache entries src[off[d]:off[d + 1]] summed into vertex element d
+    of the momentum-averaged vertex (lattice) or of the vertex (local); see fdga_k3_gather_map."""
+    d = L.Dims()
+    d.nonlocal_ = int(lattice)
+    d.nK1 = nK1
+    d.nK2b, d.nK2f = nK2
+    d.nK3b, d.nK3f = nK3
+    d.L = d.LG = 2 if lattice else 1
+    lib = L.load()
+    n = lib.fdga_k3_gather_map(ctypes.byref(d), None, None, None)
+    if n < 0:
+        raise FdgaError("fdga_k3_gather_map failed")
+    NV3, NW3 = 2 * nK3[1], 2 * nK3[0] - 1
+    nd = 3 * ((2 * nK1 - 1) + (2 * nK2[0] - 1) * 2 * nK2[1] + NW3 * NV3 * NV3)
+    off = np.empty(nd + 1, dtype=np.int64)
+    src = np.empty(max(n, 1), dtype=np.int32)
+    cf = np.empty(max(n, 1), dtype=np.float32)
+    lib.fdga_k3_gather_map(ctypes.byref(d), off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                           src.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                           cf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return off, src[:n], cf[:n]
+
+
 def BSE_K1_new(S, Ch, is_mfRG=False):
     """BSE_K1_new!(S, Ch, is_mfRG) (src/BSE_templates.jl:188-211): K1 = (U + K1 + K2') Π U."""
     S._check(S.lib.fdga_bse_k1_new(S.h, Ch, int(is_mfRG)))

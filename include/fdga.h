@@ -127,6 +127,28 @@ int     fdga_fixed_point_noalloc(fdga_ctx* ctx, double* R, const double* x, int3
  * (fdga_flat_size_F).  is_first: the map's is_first_iteration.  NL_ParquetSolver only. */
 int     fdga_mfrg_apply(fdga_ctx* ctx, double* y, const double* x, int32_t strategy, int32_t is_first);
 
+/* Tangent (JVP) and adjoint (VJP) of build_K3_cache! followed by BSE_K3! of the three channels p, a, t
+ * (src/build_K3_cache.jl:20-95, src/BSEa/BSEa_K3.jl:43-128, src/BSE_templates.jl:151-180; NL
+ * src/nonlocal/build_K3_cache.jl:18-94, src/nonlocal/BSEa/BSEa_K3.jl:1-128), as a map of S.F, S.FL.K3 and
+ * Πpp/Πph to S.Fbuff.K3, linearised at ctx's current state (its caches and right operands are rebuilt).
+ * No counterpart in the reference, which differentiates nothing: for Newton-Krylov, sensitivity and
+ * gradient callers; the reference's own linearisation is mfRGLinearMap (src/mfRG.jl:20-89).
+ * `tan` is a second context of the same dimensions with U = 0 and no reference vertex (ref_kind 0, zero
+ * tables); both FP64, no K3 symmetry groups on ctx.
+ *   JVP: reads tan's F (K1, K2, K3), FL.K3 and Πpp/Πph as the tangent; writes tan's Fbuff.K3.
+ *   VJP: reads tan's Fbuff.K3 as the cotangent (overwritten); writes tan's F (K1, K2, K3), FL.K3 and
+ *        Πpp/Πph (zero outside the K3 box).  Pairing <u, v> = sum of u v (no conjugation).
+ * The VJP's gather transpose is a segmented sum over the map fdga_k3_gather_map describes. */
+int     fdga_bse_k3_jvp(fdga_ctx* ctx, fdga_ctx* tan);
+int     fdga_bse_k3_vjp(fdga_ctx* ctx, fdga_ctx* tan);
+/* The gather of build_K3_cache! as a linear map of the momentum-averaged vertex (the vertex itself when
+ * local), transposed and sorted by destination; host only, no device needed.  Destinations: per channel
+ * p, t, a the arrays K1 [NΩ1], K2 [NΩ2, Nν2], K3 [Nν3, Nν3, NΩ3] in a row; sources: s n3 + o over the
+ * cache slots s = Γpx, Γpp, Γa, Γt(d), Fp, Fa, Ft(d) (the F caches' momentum-independent part on the
+ * lattice), entry o = ν + Nν3 (ν' + Nν3 iΩ).  Returns the number of entries (< 0 on error); fills
+ * off [3 (NΩ1 + NΩ2 Nν2 + NΩ3 Nν3²) + 1], src and coef [entries] when they are not NULL. */
+int64_t fdga_k3_gather_map(const fdga_dims* dims, int64_t* off, int32_t* src, float* coef);
+
 /* synchronisation and timing on the context's stream */
 int fdga_sync(fdga_ctx* ctx);
 /* runs `n` iterate_solver! passes (update_Σ, compute_Hartree) and returns the elapsed milliseconds
