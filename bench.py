@@ -27,7 +27,7 @@ thread and at all cores.
 
 Other workloads print their own JSON line: --workload c1 (local SIAM fdPA, nK3 = (32, 64)), c0 (the
 small SIAM scPA solve: hipGraph replay, Newton–Krylov vs mixing), c4 (the 8-point lattice sweep in one
-solver group).
+solver group), c3 (the tangent and adjoint of build_K3_cache! + BSE_K3! on the headline lattice).
 """
 import argparse
 import json
@@ -459,6 +459,127 @@ def main_c4(args):
         "autotuned": tuned}), flush=True)
 
 
+def main_c3(args):
+    """C3 on the headline lattice (FP64): the tangent and adjoint of build_K3_cache! followed by BSE_K3! of
+    the three channels (fdga_bse_k3_jvp / fdga_bse_k3_vjp), linearised at the state after --acc-iters fdPA
+    iterations.  Times of the primal composite, the JVP and the VJP (the first VJP also builds the
+    gather's transposed map), each also in parquet-iteration equivalents; the JVP against central
+    differences with h = 1e-4 along a vertex direction (dF, dFL) and a bubble direction dΠ (on the K3
+    box), each exact for a central difference (the composite is quadratic in (F, FL) and affine in Π);
+    and the identity <ō, J v> = <J^T ō, v> for a random cotangent."""
+    import numpy as np
+    import fdga
+    from fdga import _lib as L
+    S = build_c2("fp64")
+    for _ in range(args.acc_iters):
+        fdga.iterate_solver(S, "fdPA")
+    S.sync()
+    ms_iter = S.time_iterations("fdPA", 3) / 3
+    log(f"bench c3: solver built, {args.acc_iters} iterations; one iteration {ms_iter:.2f} ms")
+    chs = (0, 1, 2)
+    x = ({(ch, cls): S.vertex(0, ch, cls) for ch in chs for cls in (1, 2, 3)},
+         {ch: S.vertex(2, ch, 3) for ch in chs}, [S.get(L.PIPP), S.get(L.PIPH)])
+    rng = np.random.default_rng(7)
+
+    def rnd(n, sc):
+        return sc * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+
+    def load(T_, y):
+        F, FL, Pi = y
+        for (ch, cls), a in F.items():
+            T_.set_vertex(0, ch, cls, a)
+        for ch, a in FL.items():
+            T_.set_vertex(2, ch, 3, a)
+        T_.set(L.PIPP, Pi[0])
+        T_.set(L.PIPH, Pi[1])
+
+    def composite(y):
+        load(S, y)
+        t0 = time.perf_counter()
+        fdga.build_K3_cache(S)
+        for ch in (0, 2, 1):
+            fdga.BSE_K3(S, ch)
+        S.sync()
+        ms = (time.perf_counter() - t0) * 1e3
+        return np.concatenate([S.vertex(1, ch, 3) for ch in chs]), ms
+
+    def axpy(y, v, h):
+        return ({k: y[0][k] + h * v[0][k] for k in y[0]}, {k: y[1][k] + h * v[1][k] for k in y[1]},
+                [a + h * b for a, b in zip(y[2], v[2])])
+
+    zPi = [np.zeros_like(x[2][0]), np.zeros_like(x[2][1])]
+    vF = ({k: rnd(a.size, np.abs(a).max() + 1e-3) for k, a in x[0].items()},
+          {k: rnd(a.size, np.abs(a).max() + 1e-3) for k, a in x[1].items()}, zPi)
+    # dΠ on the K3 box only (the entries the composite reads): reference layout [Ω, ν, P]
+    NWP, NVP, NP = 2 * C2["nK1"] - 1, 2 * C2["nK1"] * C2["mPinu"], C2["L"] ** 2
+    nb, nf = C2["nK3"]
+    dPi = []
+    for a in x[2]:
+        d = np.zeros((NWP, NVP, NP), complex)
+        sub = d[C2["nK1"] - nb:C2["nK1"] + nb - 1, NVP // 2 - nf:NVP // 2 + nf, :]
+        sub[...] = rnd(sub.size, np.abs(a).max()).reshape(sub.shape)
+        dPi.append(d.reshape(-1, order="F"))
+    vP = ({k: np.zeros_like(a) for k, a in x[0].items()}, {k: np.zeros_like(a) for k, a in x[1].items()}, dPi)
+    Tn = fdga.tangent_solver(S)
+    h = 1e-4
+    out = {"workload": "c3: tangent and adjoint of build_K3_cache! + BSE_K3! (p, a, t) on the 16x16 Hubbard lattice "
+                       "(nK3=(16,32), FP64), linearised after %d fdPA iterations" % args.acc_iters,
+           "iteration_ms_fp64": ms_iter}
+    errs, jv = {}, []
+    t_primal = []
+    for name, v in (("vertex", vF), ("bubble", vP)):
+        fp, tp = composite(axpy(x, v, h))
+        fm, tm = composite(axpy(x, v, -h))
+        t_primal += [tp, tm]
+        fd = (fp - fm) / (2 * h)
+        load(S, x)
+        load(Tn, v)
+        S.sync()
+        t0 = time.perf_counter()
+        fdga.BSE_K3_jvp(S, Tn)
+        t_jvp = (time.perf_counter() - t0) * 1e3
+        j = np.concatenate([Tn.vertex(1, ch, 3) for ch in chs])
+        errs[name] = max_relerr(j, fd)
+        jv.append(j)
+        log(f"bench c3: JVP along the {name} direction: {t_jvp:.1f} ms, rel. err. against the central difference "
+            f"{errs[name]:.2e}")
+    # timings at the linearisation point (three of each after the runs above)
+    load(S, x)
+    load(Tn, vF)
+    S.sync()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        fdga.BSE_K3_jvp(S, Tn)
+    t_jvp = (time.perf_counter() - t0) * 1e3 / 3
+    ob = rnd(jv[0].size, 1.0)
+    n3 = ob.size // 3
+
+    def vjp():
+        for k, ch in enumerate(chs):
+            Tn.set_vertex(1, ch, 3, ob[k * n3:(k + 1) * n3])
+        Tn.sync()
+        t0 = time.perf_counter()
+        fdga.BSE_K3_vjp(S, Tn)
+        return (time.perf_counter() - t0) * 1e3
+    t_vjp_first = vjp()
+    t_vjp = min(vjp() for _ in range(3))
+    lhs = np.sum(ob * (jv[0] + jv[1]))
+    rhs = sum(np.sum(Tn.vertex(0, ch, cls) * vF[0][ch, cls]) for ch in chs for cls in (1, 2, 3))
+    rhs += sum(np.sum(Tn.vertex(2, ch, 3) * vF[1][ch]) for ch in chs)
+    rhs += np.sum(Tn.get(L.PIPP) * dPi[0]) + np.sum(Tn.get(L.PIPH) * dPi[1])
+    dot = float(abs(lhs - rhs) / np.sum(np.abs(ob * (jv[0] + jv[1]))))
+    primal = min(t_primal)
+    out.update({"primal_composite_ms": primal, "jvp_ms": t_jvp, "vjp_ms": t_vjp, "vjp_first_call_ms": t_vjp_first,
+                "in_iterations": {"primal": primal / ms_iter, "jvp": t_jvp / ms_iter, "vjp": t_vjp / ms_iter},
+                "jvp_vs_central_difference_relerr": errs, "h": h, "dot_identity_relerr": dot,
+                "timing": "host clock around synchronised calls (array uploads excluded)"})
+    log(f"bench c3: primal composite {primal:.1f} ms, JVP {t_jvp:.1f} ms, VJP {t_vjp:.1f} ms (first call "
+        f"{t_vjp_first:.1f} ms); <ō, J v> vs <J^T ō, v>: {dot:.2e}")
+    Tn.close()
+    S.close()
+    print(json.dumps(out), flush=True)
+
+
 C0 = dict(T=0.1, U=1.0, Delta=3.141592653589793 / 5, nK1=33, nK2=(16, 16), nK3=(8, 8), mPinu=6, nG=96)
 
 
@@ -522,11 +643,12 @@ def main():
                          "WRITE_SIZE counters against a copy of known size)")
     ap.add_argument("--tol32", type=float, default=1e-7,
                     help="Anderson tolerance of the FP32-MFMA solve (its residual floor is ~5e-8 at this workload)")
-    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"],
                     help="c2: the headline NL Hubbard lattice workload (the metric's JSON line); c1: the local SIAM fdPA "
                          "workload at its fixed point (separate JSON line); c0: the small SIAM solve (hipGraph replay and "
                          "Newton–Krylov vs mixing, separate JSON line); c4: the 8-point lattice sweep in one solver group "
-                         "(grouped GEMM launches, autotuned tiles, hipGraph; separate JSON line)")
+                         "(grouped GEMM launches, autotuned tiles, hipGraph; separate JSON line); c3: the tangent and adjoint of "
+                         "build_K3_cache! + BSE_K3! on the c2 lattice (separate JSON line)")
     ap.add_argument("--acc-iters", type=int, default=5,
                     help="c2: iterations from the initial state after which the FP32 path is compared with FP64")
     args = ap.parse_args()
@@ -547,6 +669,8 @@ def main():
         return main_c0(args)
     if args.workload == "c4":
         return main_c4(args)
+    if args.workload == "c3":
+        return main_c3(args)
     return main_c1(args)
 
 

@@ -349,6 +349,51 @@ function LinearMaps._unsafe_mul!(y::AbstractVector, A::mfRGLinearMap{<:NL_Parque
     return y
 end
 
+# Tangent and adjoint of build_K3_cache! followed by BSE_K3! of the three channels at S's device state
+# (fdga_bse_k3_jvp / fdga_bse_k3_vjp; no reference counterpart: for Newton-Krylov and gradient callers).
+# The tangent arrays travel in a second context of S's dimensions with U = 0 and no reference vertex.
+function tangent_context(S::HipSolver)
+    d = dims(S)
+    t = FdgaDims(d.nonlocal, d.nG, d.nK1, d.nK2b, d.nK2f, d.nK3b, d.nK3f, d.mPinu, d.LG, d.L, d.T, 0.0, 0,
+                 1, 1, 1, 1, 1, 1, 1, d.fdpa_h0, 0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:fdga_create, libfdga), Cint, (Ref{Ptr{Cvoid}}, Ref{FdgaDims}), r, t)
+    rc == 0 || (r[] == C_NULL ? error("fdga_create failed ($rc)") : check(r[], rc))
+    return Context(r[])
+end
+# dK3 (a Vertex whose K3 classes receive the tangent of S.Fbuff.K3) from the tangents dF (K1, K2, K3),
+# dFL (K3) and dΠpp, dΠph
+function bse_k3_jvp!(dK3, S::HipSolver, dF, dFL, dΠpp, dΠph)
+    ctx = context(S)
+    T = tangent_context(S)
+    try
+        push_vertex(T.ptr, V_F, dF)
+        push_vertex(T.ptr, V_FL, dFL; cls = (3,))
+        fdga_set(T.ptr, PIPP, dΠpp.data); fdga_set(T.ptr, PIPH, dΠph.data)
+        check(ctx, ccall((:fdga_bse_k3_jvp, libfdga), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx, T.ptr))
+        pull_vertex!(dK3, T.ptr, V_FBUFF; cls = (3,))
+    finally
+        destroy!(T)
+    end
+    return dK3
+end
+# the cotangents F̄ (K1, K2, K3), F̄L (K3), Π̄pp, Π̄ph from the cotangent K̄3 of S.Fbuff.K3 (pairing
+# sum(u .* v), no conjugation)
+function bse_k3_vjp!(F̄, F̄L, Π̄pp, Π̄ph, S::HipSolver, K̄3)
+    ctx = context(S)
+    T = tangent_context(S)
+    try
+        push_vertex(T.ptr, V_FBUFF, K̄3; cls = (3,))
+        check(ctx, ccall((:fdga_bse_k3_vjp, libfdga), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx, T.ptr))
+        pull_vertex!(F̄, T.ptr, V_F)
+        pull_vertex!(F̄L, T.ptr, V_FL; cls = (3,))
+        fdga_get!(Π̄pp.data, T.ptr, PIPP); fdga_get!(Π̄ph.data, T.ptr, PIPH)
+    finally
+        destroy!(T)
+    end
+    return F̄, F̄L, Π̄pp, Π̄ph
+end
+
 # launch options (no reference counterpart; results do not depend on them)
 set_graph!(S::HipSolver, on::Bool) =
     check(context(S), ccall((:fdga_set_graph, libfdga), Cint, (Ptr{Cvoid}, Int32), context(S), Int32(on)))
