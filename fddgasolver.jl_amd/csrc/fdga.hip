@@ -1033,8 +1033,20 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
         if (!f32 && fl_widen(c)) return -3;
         const LK3Set l1 = lk3_set(c, kChansPA, 2, false, false, f32), l2 = lk3_set(c, kChansT, 1, false, true, f32);
         kernel_begin(c);
-        if (lk3_gemm(c, kChansPA, 2, l1) != hipSuccess || lk3_gemm(c, kChansT, 1, l2) != hipSuccess)
+        if (f32 && c->NV3 <= 64) {   // momentum-batched, the operands of one Ω on chip (same results)
+            const K3Geom g = k3geom(c);
+            const size_t npg = cdiv(c->NP, LK3P_G);
+            if (c->NV3 == 64) {   // the momenta software-pipelined (Nν3 = 64)
+                k_lk3_pb64<<<8 * cdiv(2 * c->NW3 * npg, 8), 256, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
+                k_lk3_pb64<<<8 * cdiv(c->NW3 * npg, 8), 256, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);
+            } else {
+                k_lk3_pbatch<<<8 * cdiv(2 * c->NW3 * npg, 8), 256, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
+                k_lk3_pbatch<<<8 * cdiv(c->NW3 * npg, 8), 256, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);
+            }
+            LAUNCHCHK(c);
+        } else if (lk3_gemm(c, kChansPA, 2, l1) != hipSuccess || lk3_gemm(c, kChansT, 1, l2) != hipSuccess) {
             return fail(c, "BSE_L_K3 GEMM launch failed", -3);
+        }
         kernel_end(c, 9);
         c->fl32 = f32;
         if (lk3_writes_r32(c, false)) c->r32_fresh |= bits;

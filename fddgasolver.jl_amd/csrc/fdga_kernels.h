@@ -930,17 +930,23 @@ struct LK3_Aop {
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
+// B element of BSE_L_K3!, Π0(ω) F0r[ω, νp], without FMA contraction (the momentum-batched kernel
+// forms the same products and must round them alike)
+__device__ __forceinline__ double2 lk3_b(double2 p, double2 f) {
+#pragma clang fp contract(off)
+    return cmk(p.x * f.x - p.y * f.y, p.x * f.y + p.y * f.x);
+}
 struct LK3_Bop {
     K3Geom g; LK3Chan c0, c1, c2;
     struct Bound {
         const double2* pi0; const double2* F0r; int NV;   // pi0[k] = Π0(Ω, ω = k - n3f, P)
         __device__ __forceinline__ double2 operator()(int k, int j) const {
-            return cmul(pi0[k], F0r[(size_t)k + (size_t)NV * j]);
+            return lk3_b(pi0[k], F0r[(size_t)k + (size_t)NV * j]);
         }
         // the two loads held raw by the prefetch (see LoaderRaw)
         struct Raw { double2 p, f; };
         __device__ __forceinline__ Raw raw(int k, int j) const { return Raw{pi0[k], F0r[(size_t)k + (size_t)NV * j]}; }
-        __device__ __forceinline__ double2 combine(const Raw& x) const { return cmul(x.p, x.f); }
+        __device__ __forceinline__ double2 combine(const Raw& x) const { return lk3_b(x.p, x.f); }
     };
     __device__ __forceinline__ Bound one(const LK3Chan& c, int bb) const {
         const int NV = g.NV(), NW = g.NW();
@@ -970,6 +976,29 @@ struct LK3BoundPlain {
         out[o] = cmk(xr, xi);
     }
 };
+// BSE_L_K3! epilogue arithmetic of the right-operand form for one element: FL = T s v (t: then the
+// post-processing with the finished a channel), and the right operand R at the element's R row
+struct LK3RIn { double2 f0, gm, P1, P0, oa; };
+__device__ __forceinline__ void lk3r_elem(double2 v, double scale, const LK3RIn& q, bool hasL, int tpost, double s1,
+                                          double s2, float2& r32, double& xr, double& xi) {
+#pragma clang fp contract(off)
+    xr = v.x * scale;
+    xi = v.y * scale;
+    const double dr = q.P1.x - q.P0.x, di = q.P1.y - q.P0.y;
+    double rr = (dr * q.f0.x - di * q.f0.y) + (q.P1.x * q.gm.x - q.P1.y * q.gm.y);
+    double ri = (dr * q.f0.y + di * q.f0.x) + (q.P1.x * q.gm.y + q.P1.y * q.gm.x);
+    rr *= s1;
+    ri *= s1;
+    if (hasL) {
+        rr += s2 * (q.P1.x * xr - q.P1.y * xi);
+        ri += s2 * (q.P1.x * xi + q.P1.y * xr);
+    }
+    r32 = make_float2((float)rr, (float)ri);
+    if (tpost) {
+        xr = (xr + q.oa.x) * 0.5;
+        xi = (xi + q.oa.y) * 0.5;
+    }
+}
 struct LK3BoundR {
     double2* out; const double2* outa; double scale; int tpost; int NV;
     float2* out32; const float2* outa32;
@@ -986,12 +1015,10 @@ struct LK3BoundR {
         return hasL ? wc + n3f : i;
     }
     __device__ __forceinline__ void operator()(int i, int j, double2 v) const {
-#pragma clang fp contract(off)
-        double xr = v.x * scale, xi = v.y * scale;
         bool hasL;
         const int k = r_row(i, hasL);
         const size_t kj = (size_t)k + (size_t)NV * j;
-        struct { double2 f0, gm, P1, P0, oa; } q;
+        LK3RIn q;
         q.f0 = F0r[kj];
         q.gm = Gam[kj];
         q.P1 = pi[k];
@@ -1003,20 +1030,10 @@ struct LK3BoundR {
         } else {
             q.oa = czero();
         }
-        const double dr = q.P1.x - q.P0.x, di = q.P1.y - q.P0.y;
-        double rr = (dr * q.f0.x - di * q.f0.y) + (q.P1.x * q.gm.x - q.P1.y * q.gm.y);
-        double ri = (dr * q.f0.y + di * q.f0.x) + (q.P1.x * q.gm.y + q.P1.y * q.gm.x);
-        rr *= s1;
-        ri *= s1;
-        if (hasL) {
-            rr += s2 * (q.P1.x * xr - q.P1.y * xi);
-            ri += s2 * (q.P1.x * xi + q.P1.y * xr);
-        }
-        R32[(size_t)k + (size_t)NV * j] = make_float2((float)rr, (float)ri);
-        if (tpost) {
-            xr = (xr + q.oa.x) * 0.5;
-            xi = (xi + q.oa.y) * 0.5;
-        }
+        float2 r32;
+        double xr, xi;
+        lk3r_elem(v, scale, q, hasL, tpost, s1, s2, r32, xr, xi);
+        R32[(size_t)k + (size_t)NV * j] = r32;
         if (out32) out32[(size_t)i + (size_t)NV * j] = make_float2((float)xr, (float)xi);
         else out[(size_t)i + (size_t)NV * j] = cmk(xr, xi);
     }
@@ -1052,6 +1069,351 @@ struct LK3_EpT {
 };
 typedef LK3_EpT<false> LK3_Ep;
 typedef LK3_EpT<true> LK3R_Ep;
+
+// BSE_L_K3! of the FP32 lattice contraction with its momentum-independent operands kept on chip: one
+// workgroup per (channel slot, Ω, LK3P_G momenta).  Γr(Ω), the left operand rounded to FP32, stays in
+// LDS; each thread keeps F0r(Ω) at its output positions in registers, so a momentum reads only its Π0
+// and Π rows (and Γ~(Ω), from L2, in the epilogue) and forms B = Π0 F0r into LDS.  The tile is zgemm_tiles_t's 64 x 64 (four
+// waves of 32 x 32, the same MFMA sequence and K order) and the epilogue is LK3BoundR's, so every
+// element rounds exactly as in the GEMM engine.  FL.K3 in FP32 (out32) and the FP32 right operand
+// (R32) are written; Nν3 <= 64.
+constexpr int LK3P_G = 8;
+__global__ __launch_bounds__(256, 2) void k_lk3_pbatch(K3Geom g, LK3Chan c0, LK3Chan c1, int nslot) {
+    typedef MfmaT<true> MT;
+    typedef MT::Acc Acc;
+    __shared__ float Gs[2][64][65];   // [re/im][k][m] = Γr(m, k) in FP32
+    __shared__ float Bs[2][64][65];   // [re/im][k][n] = B(k, n) of the current momentum
+    const int NV = g.NV(), NW = g.NW(), NP = g.NP, n3f = g.n3f;
+    const int npg = (NP + LK3P_G - 1) / LK3P_G;
+    // the blocks of one XCD (blockIdx % 8) take consecutive tasks, momentum group fastest: they share Ω
+    // and its momentum-independent operands in that XCD's L2
+    const int per = (int)(gridDim.x / 8);
+    const int task = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (task >= nslot * NW * npg) return;   // uniform over the block
+    const int pg = task % npg, sw = task / npg, iW = sw % NW, s = sw / NW;
+    const int m = iW - (g.n3b - 1);
+    const size_t nn = (size_t)NV * NV;
+    const double2* G = (s == 0 ? c0.G : c1.G) + nn * iW;
+    const double2* F0r = (s == 0 ? c0.F0r : c1.F0r) + nn * iW;
+    const double2* Gam = (s == 0 ? c0.Gam : c1.Gam) + nn * iW;
+    const double2* Pi0 = s == 0 ? c0.Pi0 : c1.Pi0;
+    const double2* Pi = s == 0 ? c0.Pi : c1.Pi;
+    float2* out32 = s == 0 ? c0.out32 : c1.out32;
+    const float2* outa32 = s == 0 ? c0.outa32 : c1.outa32;
+    float2* R32 = s == 0 ? c0.R32 : c1.R32;
+    const double scale = s == 0 ? c0.scale : c1.scale, s1 = s == 0 ? c0.s1 : c1.s1, s2 = s == 0 ? c0.s2 : c1.s2;
+    const int tpost = s == 0 ? c0.tpost : c1.tpost, ch = s == 0 ? c0.ch : c1.ch;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave % 2) * 32, wn = (wave / 2) * 32;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int mm = e % 64, k = e / 64;
+        const double2 v = (mm < NV && k < NV) ? G[(size_t)mm + (size_t)NV * k] : czero();
+        Gs[0][k][mm] = (float)v.x;
+        Gs[1][k][mm] = (float)v.y;
+    }
+    for (int e = tid; e < 64 * 65; e += 256) {   // entries outside the box stay zero
+        (&Bs[0][0][0])[e] = 0.f;
+        (&Bs[1][0][0])[e] = 0.f;
+    }
+    // output positions: rows wm + 16 i + (lane & 15), columns wn + 16 j + 4 (lane >> 4) + r; the R row of
+    // row i is its crossing (p: Ω - ω - 1, an involution of the box rows; a, t: the row itself)
+    int row[2], krow[2];
+    bool hasL[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        row[i] = wm + 16 * i + (lane & 15);
+        const int wc = m - (row[i] - n3f) - 1;
+        hasL[i] = ch != PCH || in_f(wc, n3f);
+        krow[i] = (ch == PCH && hasL[i]) ? wc + n3f : row[i];
+    }
+    double2 f0[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = wn + 16 * j + 4 * (lane >> 4) + r;
+                f0[i][j][r] = (row[i] < NV && col < NV) ? F0r[(size_t)krow[i] + (size_t)NV * col] : czero();
+            }
+    const int pbeg = pg * LK3P_G, pend = min(NP, pbeg + LK3P_G);
+    // the Π and Π0 entries of the thread's two R rows, one momentum ahead
+    double2 P1[2], P0[2], P1n[2], P0n[2];
+    auto prow = [&](int p, double2* a, double2* b) {
+        const size_t pr = g.pidx(m, -n3f, p);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            a[i] = row[i] < NV ? Pi[pr + krow[i]] : czero();
+            b[i] = row[i] < NV ? Pi0[pr + krow[i]] : czero();
+        }
+    };
+    prow(pbeg, P1n, P0n);
+    for (int p = pbeg; p < pend; ++p) {
+        const size_t off = nn * ((size_t)iW + (size_t)NW * p);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            P1[i] = P1n[i];
+            P0[i] = P0n[i];
+        }
+        if (p + 1 < pend) prow(p + 1, P1n, P0n);
+        __syncthreads();   // Gs written (first momentum) / Bs read by the previous momentum's MFMAs
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = wn + 16 * j + 4 * (lane >> 4) + r;
+                    if (row[i] < NV && col < NV) {
+                        const double2 b = lk3_b(P0[i], f0[i][j][r]);
+                        Bs[0][krow[i]][col] = (float)b.x;
+                        Bs[1][krow[i]][col] = (float)b.y;
+                    }
+                }
+        __syncthreads();
+        Acc accr[2][2], acci[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                accr[i][j] = Acc{0, 0, 0, 0};
+                acci[i][j] = Acc{0, 0, 0, 0};
+            }
+        // 4-deep steps over the K range padded to GBK, as the GEMM engine runs them; each step's LDS
+        // fragments are read one step ahead so that their latency overlaps the MFMAs
+        const int nst = ((NV + GBK - 1) / GBK) * (GBK / 4), cc = lane & 15;
+        float fa[2][4], fb[2][4];   // [buffer][ar0, ar1, ai0, ai1] / [br0, br1, bi0, bi1]
+        auto frag = [&](int st, float* a, float* b) {
+            const int kr = 4 * st + (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = Gs[0][kr][wm + 16 * i + cc];
+                a[2 + i] = Gs[1][kr][wm + 16 * i + cc];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                b[j] = Bs[0][kr][wn + 16 * j + cc];
+                b[2 + j] = Bs[1][kr][wn + 16 * j + cc];
+            }
+        };
+        auto step = [&](const float* a, const float* b) {
+            const float nai[2] = {-a[2], -a[3]};
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = MT::mfma(b[j], a[i], accr[i][j]);
+                    acci[i][j] = MT::mfma(b[2 + j], a[i], acci[i][j]);
+                }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = MT::mfma(b[2 + j], nai[i], accr[i][j]);
+                    acci[i][j] = MT::mfma(b[j], a[2 + i], acci[i][j]);
+                }
+        };
+        frag(0, fa[0], fb[0]);
+        for (int st = 0; st < nst; st += 2) {   // nst is a multiple of 4
+            frag(st + 1, fa[1], fb[1]);
+            step(fa[0], fb[0]);
+            if (st + 2 < nst) frag(st + 2, fa[0], fb[0]);
+            step(fa[1], fb[1]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = wn + 16 * j + 4 * (lane >> 4) + r;
+                    if (row[i] >= NV || col >= NV) continue;
+                    const size_t o = off + (size_t)row[i] + (size_t)NV * col;
+                    LK3RIn q;
+                    q.f0 = f0[i][j][r];
+                    q.gm = Gam[(size_t)krow[i] + (size_t)NV * col];   // momentum-independent: an L2 hit
+                    q.P1 = P1[i];
+                    q.P0 = P0[i];
+                    if (tpost) { const float2 a = outa32[o]; q.oa = cmk(a.x, a.y); }
+                    else q.oa = czero();
+                    float2 r32;
+                    double xr, xi;
+                    lk3r_elem(cmk((double)accr[i][j][r], (double)acci[i][j][r]), scale, q, hasL[i], tpost, s1, s2,
+                              r32, xr, xi);
+                    R32[off + (size_t)krow[i] + (size_t)NV * col] = r32;
+                    out32[o] = make_float2((float)xr, (float)xi);
+                }
+    }
+}
+
+// Nν3 = 64: the same tile with the momenta software-pipelined inside each workgroup.  While the MFMAs
+// of momentum p run (sixteen 4-deep steps), every step also forms one B element of momentum p + 1
+// (into the second LDS buffer) and finishes one epilogue element of momentum p - 1 (from its saved
+// accumulators), so that their VALU work and stores hide under the matrix core's.  F0r and Γ~ at the
+// thread's 16 positions stay in registers (one wave per SIMD), Π rows and the t post-processing's
+// FL.a entries are loaded one momentum ahead.  Same MFMA order and epilogue arithmetic as
+// k_lk3_pbatch and the GEMM engine: the results are equal bit for bit.
+__global__ __launch_bounds__(256, 1) void k_lk3_pb64(K3Geom g, LK3Chan c0, LK3Chan c1, int nslot) {
+    typedef MfmaT<true> MT;
+    typedef MT::Acc Acc;
+    constexpr int NV = 64, NST = 16;
+    __shared__ float Gs[2][64][65];      // [re/im][k][m] = Γr(m, k) in FP32
+    __shared__ float Bs[2][2][64][65];   // [buffer][re/im][k][n]
+    const int NW = g.NW(), NP = g.NP, n3f = g.n3f;
+    const int npg = (NP + LK3P_G - 1) / LK3P_G;
+    const int per = (int)(gridDim.x / 8);
+    const int task = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (task >= nslot * NW * npg) return;   // uniform over the block
+    const int pg = task % npg, sw = task / npg, iW = sw % NW, s = sw / NW;
+    const int m = iW - (g.n3b - 1);
+    const size_t nn = (size_t)NV * NV;
+    const double2* G = (s == 0 ? c0.G : c1.G) + nn * iW;
+    const double2* F0r = (s == 0 ? c0.F0r : c1.F0r) + nn * iW;
+    const double2* Gam = (s == 0 ? c0.Gam : c1.Gam) + nn * iW;
+    const double2* Pi0 = s == 0 ? c0.Pi0 : c1.Pi0;
+    const double2* Pi = s == 0 ? c0.Pi : c1.Pi;
+    float2* out32 = s == 0 ? c0.out32 : c1.out32;
+    const float2* outa32 = s == 0 ? c0.outa32 : c1.outa32;
+    float2* R32 = s == 0 ? c0.R32 : c1.R32;
+    const double scale = s == 0 ? c0.scale : c1.scale, s1 = s == 0 ? c0.s1 : c1.s1, s2 = s == 0 ? c0.s2 : c1.s2;
+    const int tpost = s == 0 ? c0.tpost : c1.tpost, ch = s == 0 ? c0.ch : c1.ch;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave % 2) * 32, wn = (wave / 2) * 32, cc = lane & 15;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const double2 v = G[e];   // e = m + 64 k
+        Gs[0][e / 64][e % 64] = (float)v.x;
+        Gs[1][e / 64][e % 64] = (float)v.y;
+    }
+    int row[2], krow[2];
+    bool hasL[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        row[i] = wm + 16 * i + (lane & 15);
+        const int wc = m - (row[i] - n3f) - 1;
+        hasL[i] = ch != PCH || in_f(wc, n3f);
+        krow[i] = (ch == PCH && hasL[i]) ? wc + n3f : row[i];
+    }
+    // element e of the thread: i = e / 8, j = (e / 4) % 2, r = e % 4
+    auto ecol = [&](int e) { return wn + 16 * ((e >> 2) & 1) + 4 * (lane >> 4) + (e & 3); };
+    double2 f0[16], gm[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const size_t kj = (size_t)krow[e >> 3] + (size_t)NV * ecol(e);
+        f0[e] = F0r[kj];
+        gm[e] = Gam[kj];
+    }
+    const int pbeg = pg * LK3P_G, n = min(NP, pbeg + LK3P_G) - pbeg;
+    // Π1, Π0 of the two R rows per momentum: p - 1 (epilogue), p, p + 1 (B staging), p + 2 (in flight)
+    double2 Pp1[2], Pp0[2], Pc1[2], Pc0[2], Pn1[2], Pn0[2], Pq1[2], Pq0[2];
+    auto prow = [&](int p, double2* a, double2* b) {
+        const size_t pr = g.pidx(m, -n3f, p);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            a[i] = Pi[pr + krow[i]];
+            b[i] = Pi0[pr + krow[i]];
+        }
+    };
+    prow(pbeg, Pc1, Pc0);
+    if (n > 1) prow(pbeg + 1, Pn1, Pn0);
+    auto stage = [&](int buf, int e, const double2* P0) {
+        const double2 b = lk3_b(P0[e >> 3], f0[e]);
+        Bs[buf][0][krow[e >> 3]][ecol(e)] = (float)b.x;
+        Bs[buf][1][krow[e >> 3]][ecol(e)] = (float)b.y;
+    };
+#pragma unroll
+    for (int e = 0; e < 16; ++e) stage(0, e, Pc0);
+    float2 oac[16], oan[16];   // FL.a entries of the epilogue's momentum / of the current one (t only)
+    Acc accr[2][2], acci[2][2], par[2][2], pai[2][2];
+    auto epilogue = [&](int e, size_t off, const double2* P1, const double2* P0) {
+        const int i = e >> 3, j = (e >> 2) & 1, r = e & 3, col = ecol(e);
+        LK3RIn q;
+        q.f0 = f0[e];
+        q.gm = gm[e];
+        q.P1 = P1[i];
+        q.P0 = P0[i];
+        q.oa = tpost ? cmk(oac[e].x, oac[e].y) : czero();
+        float2 r32;
+        double xr, xi;
+        lk3r_elem(cmk((double)par[i][j][r], (double)pai[i][j][r]), scale, q, hasL[i], tpost, s1, s2, r32, xr, xi);
+        R32[off + (size_t)krow[i] + (size_t)NV * col] = r32;
+        out32[off + (size_t)row[i] + (size_t)NV * col] = make_float2((float)xr, (float)xi);
+    };
+    __syncthreads();
+    for (int t = 0; t < n; ++t) {
+        const int p = pbeg + t, cur = t & 1;
+        const size_t offc = nn * ((size_t)iW + (size_t)NW * p), offp = offc - nn * (size_t)NW;
+        if (t + 2 < n) prow(p + 2, Pq1, Pq0);
+        if (tpost) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) oan[e] = outa32[offc + (size_t)row[e >> 3] + (size_t)NV * ecol(e)];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                accr[i][j] = Acc{0, 0, 0, 0};
+                acci[i][j] = Acc{0, 0, 0, 0};
+            }
+        float fa[2][4], fb[2][4];
+        auto frag = [&](int st, float* a, float* b) {
+            const int kr = 4 * st + (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = Gs[0][kr][wm + 16 * i + cc];
+                a[2 + i] = Gs[1][kr][wm + 16 * i + cc];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                b[j] = Bs[cur][0][kr][wn + 16 * j + cc];
+                b[2 + j] = Bs[cur][1][kr][wn + 16 * j + cc];
+            }
+        };
+        frag(0, fa[0], fb[0]);
+#pragma unroll
+        for (int st = 0; st < NST; ++st) {
+            if (st + 1 < NST) frag(st + 1, fa[(st + 1) & 1], fb[(st + 1) & 1]);
+            const float* a = fa[st & 1];
+            const float* b = fb[st & 1];
+            const float nai[2] = {-a[2], -a[3]};
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = MT::mfma(b[j], a[i], accr[i][j]);
+                    acci[i][j] = MT::mfma(b[2 + j], a[i], acci[i][j]);
+                }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = MT::mfma(b[2 + j], nai[i], accr[i][j]);
+                    acci[i][j] = MT::mfma(b[j], a[2 + i], acci[i][j]);
+                }
+            if (t + 1 < n) stage(cur ^ 1, st, Pn0);
+            if (t > 0) epilogue(st, offp, Pp1, Pp0);
+        }
+        // rotate: this momentum's accumulators and Π rows become the next epilogue's
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                par[i][j] = accr[i][j];
+                pai[i][j] = acci[i][j];
+            }
+            Pp1[i] = Pc1[i]; Pp0[i] = Pc0[i];
+            Pc1[i] = Pn1[i]; Pc0[i] = Pn0[i];
+            Pn1[i] = Pq1[i]; Pn0[i] = Pq0[i];
+        }
+        if (tpost) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) oac[e] = oan[e];
+        }
+        __syncthreads();   // B of p + 1 complete; the reads of Bs[cur] done before it is restaged
+    }
+    const size_t offl = nn * ((size_t)iW + (size_t)NW * (pbeg + n - 1));
+#pragma unroll
+    for (int e = 0; e < 16; ++e) epilogue(e, offl, Pp1, Pp0);
+}
 
 // --- BSE_K3!:
 //   A(ν, ω)  = cache_Fr = Fbase_r[ν, ω, Ω] + K2r(Ω, ν, P) + K3r(Ω, ν, ω, P)   (t: 2·(t) - (a))

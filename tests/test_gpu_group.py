@@ -77,6 +77,28 @@ def test_group_every_tile_shape_and_grid(precision):
         G.close()
 
 
+def test_group_matches_members_at_64_frequencies():
+    """FP32 lattice members whose K3 box has Nν3 = 64, the headline's: the single solver's
+    momentum-pipelined BSE_L_K3! kernel gives the bits of the grouped GEMM engine (non-zero FL: a local
+    reference vertex with RefVertex tables)."""
+    import fdga
+    box = dict(nG=48, nK1=32, nK2=(4, 32), nK3=(2, 32), mPinu=1, T=0.3, U=1.3)
+    mem = [("nl_L16_ref", box), ("nl_L16_ref", dict(box, nK3=(3, 32)))]
+    solo, grp = _pairs(mem, seed=31, precision="fp32")
+    G = fdga.SolverGroup(grp)
+    try:
+        for it in range(2):
+            for S in solo:
+                fdga.iterate_solver(S, "fdPA")
+            G.iterate("fdPA")
+            G.sync()
+            _assert_same(solo, grp, f"Nν3 = 64 iteration {it}")
+        for S in solo:
+            assert max(np.abs(S.vertex(2, ch, 3)).max() for ch in range(3)) > 1e-6
+    finally:
+        G.close()
+
+
 def test_group_graph_replay_and_autotune():
     """Eager first iteration (caches the reference SDE), then hipGraph replays with eager work in
     between; then the autotuner (which must leave the state alone) and more replays."""
