@@ -1037,8 +1037,8 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
             const K3Geom g = k3geom(c);
             const size_t npg = cdiv(c->NP, LK3P_G);
             if (c->NV3 == 64) {   // the momenta software-pipelined (Nν3 = 64)
-                k_lk3_pb64<<<8 * cdiv(2 * c->NW3 * npg, 8), 256, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
-                k_lk3_pb64<<<8 * cdiv(c->NW3 * npg, 8), 256, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);
+                k_lk3_pb64<false><<<8 * cdiv(2 * c->NW3 * npg, 8), 512, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
+                k_lk3_pb64<true><<<8 * cdiv(c->NW3 * npg, 8), 512, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);
             } else {
                 k_lk3_pbatch<<<8 * cdiv(2 * c->NW3 * npg, 8), 256, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
                 k_lk3_pbatch<<<8 * cdiv(c->NW3 * npg, 8), 256, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);

@@ -1247,16 +1247,18 @@ __global__ __launch_bounds__(256, 2) void k_lk3_pbatch(K3Geom g, LK3Chan c0, LK3
 }
 
 // Nν3 = 64: the same tile with the momenta software-pipelined inside each workgroup.  While the MFMAs
-// of momentum p run (sixteen 4-deep steps), every step also forms one B element of momentum p + 1
-// (into the second LDS buffer) and finishes one epilogue element of momentum p - 1 (from its saved
-// accumulators), so that their VALU work and stores hide under the matrix core's.  F0r and Γ~ at the
-// thread's 16 positions stay in registers (one wave per SIMD), Π rows and the t post-processing's
-// FL.a entries are loaded one momentum ahead.  Same MFMA order and epilogue arithmetic as
-// k_lk3_pbatch and the GEMM engine: the results are equal bit for bit.
-__global__ __launch_bounds__(256, 1) void k_lk3_pb64(K3Geom g, LK3Chan c0, LK3Chan c1, int nslot) {
+// of momentum p run (sixteen 4-deep steps), every step also forms B elements of momentum p + 1 (into
+// the second LDS buffer) and finishes epilogue elements of momentum p - 1 (from its saved
+// accumulators), so that their VALU work and stores overlap the matrix core's.  Eight waves of 16 x 32
+// (two per SIMD); F0r and Γ~ at a thread's 8 positions stay in registers, its Π row and the t
+// post-processing's FL.a entries are loaded one momentum ahead.  Every element sees the MFMA sequence
+// and epilogue arithmetic of k_lk3_pbatch and the GEMM engine: the results are equal bit for bit.
+// TP: the launch of the t channel (post-processing with the finished a channel)
+template <bool TP>
+__global__ __launch_bounds__(512, 1) void k_lk3_pb64(K3Geom g, LK3Chan c0, LK3Chan c1, int nslot) {
     typedef MfmaT<true> MT;
     typedef MT::Acc Acc;
-    constexpr int NV = 64, NST = 16;
+    constexpr int NV = 64, NST = 16, NE = 8;
     __shared__ float Gs[2][64][65];      // [re/im][k][m] = Γr(m, k) in FP32
     __shared__ float Bs[2][2][64][65];   // [buffer][re/im][k][n]
     const int NW = g.NW(), NP = g.NP, n3f = g.n3f;
@@ -1276,92 +1278,85 @@ __global__ __launch_bounds__(256, 1) void k_lk3_pb64(K3Geom g, LK3Chan c0, LK3Ch
     const float2* outa32 = s == 0 ? c0.outa32 : c1.outa32;
     float2* R32 = s == 0 ? c0.R32 : c1.R32;
     const double scale = s == 0 ? c0.scale : c1.scale, s1 = s == 0 ? c0.s1 : c1.s1, s2 = s == 0 ? c0.s2 : c1.s2;
-    const int tpost = s == 0 ? c0.tpost : c1.tpost, ch = s == 0 ? c0.ch : c1.ch;
+    constexpr int tpost = TP;
+    const int ch = s == 0 ? c0.ch : c1.ch;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave % 2) * 32, wn = (wave / 2) * 32, cc = lane & 15;
-    for (int e = tid; e < 64 * 64; e += 256) {
+    const int wm = (wave % 4) * 16, wn = (wave / 4) * 32, cc = lane & 15;
+    for (int e = tid; e < 64 * 64; e += 512) {
         const double2 v = G[e];   // e = m + 64 k
         Gs[0][e / 64][e % 64] = (float)v.x;
         Gs[1][e / 64][e % 64] = (float)v.y;
     }
-    int row[2], krow[2];
-    bool hasL[2];
+    // the thread's row and its R row (p: the crossing Ω - ω - 1, an involution of the box rows)
+    const int row = wm + (lane & 15);
+    const int wcr = m - (row - n3f) - 1;
+    const bool hasL = ch != PCH || in_f(wcr, n3f);
+    const int krow = (ch == PCH && hasL) ? wcr + n3f : row;
+    // element e of the thread: column wn + 16 (e / 4) + 4 (lane / 16) + e % 4
+    auto ecol = [&](int e) { return wn + 16 * (e >> 2) + 4 * (lane >> 4) + (e & 3); };
+    double2 f0[NE], gm[NE];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        row[i] = wm + 16 * i + (lane & 15);
-        const int wc = m - (row[i] - n3f) - 1;
-        hasL[i] = ch != PCH || in_f(wc, n3f);
-        krow[i] = (ch == PCH && hasL[i]) ? wc + n3f : row[i];
-    }
-    // element e of the thread: i = e / 8, j = (e / 4) % 2, r = e % 4
-    auto ecol = [&](int e) { return wn + 16 * ((e >> 2) & 1) + 4 * (lane >> 4) + (e & 3); };
-    double2 f0[16], gm[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const size_t kj = (size_t)krow[e >> 3] + (size_t)NV * ecol(e);
+    for (int e = 0; e < NE; ++e) {
+        const size_t kj = (size_t)krow + (size_t)NV * ecol(e);
         f0[e] = F0r[kj];
         gm[e] = Gam[kj];
     }
     const int pbeg = pg * LK3P_G, n = min(NP, pbeg + LK3P_G) - pbeg;
-    // Π1, Π0 of the two R rows per momentum: p - 1 (epilogue), p, p + 1 (B staging), p + 2 (in flight)
-    double2 Pp1[2], Pp0[2], Pc1[2], Pc0[2], Pn1[2], Pn0[2], Pq1[2], Pq0[2];
-    auto prow = [&](int p, double2* a, double2* b) {
+    // Π1, Π0 of the R row per momentum: p - 1 (epilogue), p, p + 1 (B staging), p + 2 (in flight)
+    double2 Pp1, Pp0, Pc1, Pc0, Pn1, Pn0, Pq1, Pq0;
+    auto prow = [&](int p, double2& a, double2& b) {
         const size_t pr = g.pidx(m, -n3f, p);
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            a[i] = Pi[pr + krow[i]];
-            b[i] = Pi0[pr + krow[i]];
-        }
+        a = Pi[pr + krow];
+        b = Pi0[pr + krow];
     };
     prow(pbeg, Pc1, Pc0);
     if (n > 1) prow(pbeg + 1, Pn1, Pn0);
-    auto stage = [&](int buf, int e, const double2* P0) {
-        const double2 b = lk3_b(P0[e >> 3], f0[e]);
-        Bs[buf][0][krow[e >> 3]][ecol(e)] = (float)b.x;
-        Bs[buf][1][krow[e >> 3]][ecol(e)] = (float)b.y;
+    auto stage = [&](int buf, int e, const double2& P0) {
+        const double2 b = lk3_b(P0, f0[e]);
+        Bs[buf][0][krow][ecol(e)] = (float)b.x;
+        Bs[buf][1][krow][ecol(e)] = (float)b.y;
     };
 #pragma unroll
-    for (int e = 0; e < 16; ++e) stage(0, e, Pc0);
-    float2 oac[16], oan[16];   // FL.a entries of the epilogue's momentum / of the current one (t only)
-    Acc accr[2][2], acci[2][2], par[2][2], pai[2][2];
-    auto epilogue = [&](int e, size_t off, const double2* P1, const double2* P0) {
-        const int i = e >> 3, j = (e >> 2) & 1, r = e & 3, col = ecol(e);
+    for (int e = 0; e < NE; ++e) stage(0, e, Pc0);
+    float2 oac[TP ? NE : 1], oan[TP ? NE : 1];   // FL.a entries of the epilogue's / the current momentum
+    Acc accr[2], acci[2], par[2], pai[2];   // [column block]
+    auto epilogue = [&](int e, size_t off, const double2& P1, const double2& P0) {
+        const int j = e >> 2, r = e & 3, col = ecol(e);
         LK3RIn q;
         q.f0 = f0[e];
         q.gm = gm[e];
-        q.P1 = P1[i];
-        q.P0 = P0[i];
-        q.oa = tpost ? cmk(oac[e].x, oac[e].y) : czero();
+        q.P1 = P1;
+        q.P0 = P0;
+        if constexpr (TP) q.oa = cmk(oac[e].x, oac[e].y);
+        else q.oa = czero();
         float2 r32;
         double xr, xi;
-        lk3r_elem(cmk((double)par[i][j][r], (double)pai[i][j][r]), scale, q, hasL[i], tpost, s1, s2, r32, xr, xi);
-        R32[off + (size_t)krow[i] + (size_t)NV * col] = r32;
-        out32[off + (size_t)row[i] + (size_t)NV * col] = make_float2((float)xr, (float)xi);
+        lk3r_elem(cmk((double)par[j][r], (double)pai[j][r]), scale, q, hasL, tpost, s1, s2, r32, xr, xi);
+        R32[off + (size_t)krow + (size_t)NV * col] = r32;
+        out32[off + (size_t)row + (size_t)NV * col] = make_float2((float)xr, (float)xi);
     };
     __syncthreads();
-    for (int t = 0; t < n; ++t) {
+    // one momentum: its MFMAs, with B of the next (STAGE) and the epilogue of the previous (EPI) in the
+    // same straight-line steps (one element of each every other step)
+    auto momentum = [&](int t, auto stage_c, auto epi_c) {
+        constexpr bool STAGE = decltype(stage_c)::value, EPI = decltype(epi_c)::value;
         const int p = pbeg + t, cur = t & 1;
         const size_t offc = nn * ((size_t)iW + (size_t)NW * p), offp = offc - nn * (size_t)NW;
         if (t + 2 < n) prow(p + 2, Pq1, Pq0);
-        if (tpost) {
+        if constexpr (TP) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) oan[e] = outa32[offc + (size_t)row[e >> 3] + (size_t)NV * ecol(e)];
+            for (int e = 0; e < NE; ++e) oan[e] = outa32[offc + (size_t)row + (size_t)NV * ecol(e)];
         }
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                accr[i][j] = Acc{0, 0, 0, 0};
-                acci[i][j] = Acc{0, 0, 0, 0};
-            }
-        float fa[2][4], fb[2][4];
+        for (int j = 0; j < 2; ++j) {
+            accr[j] = Acc{0, 0, 0, 0};
+            acci[j] = Acc{0, 0, 0, 0};
+        }
+        float fa[2][2], fb[2][4];   // [buffer][ar, ai] / [br0, br1, bi0, bi1]
         auto frag = [&](int st, float* a, float* b) {
             const int kr = 4 * st + (lane >> 4);
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[i] = Gs[0][kr][wm + 16 * i + cc];
-                a[2 + i] = Gs[1][kr][wm + 16 * i + cc];
-            }
+            a[0] = Gs[0][kr][wm + cc];
+            a[1] = Gs[1][kr][wm + cc];
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 b[j] = Bs[cur][0][kr][wn + 16 * j + cc];
@@ -1374,45 +1369,49 @@ __global__ __launch_bounds__(256, 1) void k_lk3_pb64(K3Geom g, LK3Chan c0, LK3Ch
             if (st + 1 < NST) frag(st + 1, fa[(st + 1) & 1], fb[(st + 1) & 1]);
             const float* a = fa[st & 1];
             const float* b = fb[st & 1];
-            const float nai[2] = {-a[2], -a[3]};
+            const float nai = -a[1];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+                accr[j] = MT::mfma(b[j], a[0], accr[j]);
+                acci[j] = MT::mfma(b[2 + j], a[0], acci[j]);
+            }
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    accr[i][j] = MT::mfma(b[j], a[i], accr[i][j]);
-                    acci[i][j] = MT::mfma(b[2 + j], a[i], acci[i][j]);
-                }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    accr[i][j] = MT::mfma(b[2 + j], nai[i], accr[i][j]);
-                    acci[i][j] = MT::mfma(b[j], a[2 + i], acci[i][j]);
-                }
-            if (t + 1 < n) stage(cur ^ 1, st, Pn0);
-            if (t > 0) epilogue(st, offp, Pp1, Pp0);
+            for (int j = 0; j < 2; ++j) {
+                accr[j] = MT::mfma(b[2 + j], nai, accr[j]);
+                acci[j] = MT::mfma(b[j], a[1], acci[j]);
+            }
+            if (st % 2 == 0) {
+                if constexpr (STAGE) stage(cur ^ 1, st / 2, Pn0);
+                if constexpr (EPI) epilogue(st / 2, offp, Pp1, Pp0);
+            }
         }
         // rotate: this momentum's accumulators and Π rows become the next epilogue's
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                par[i][j] = accr[i][j];
-                pai[i][j] = acci[i][j];
-            }
-            Pp1[i] = Pc1[i]; Pp0[i] = Pc0[i];
-            Pc1[i] = Pn1[i]; Pc0[i] = Pn0[i];
-            Pn1[i] = Pq1[i]; Pn0[i] = Pq0[i];
+        for (int j = 0; j < 2; ++j) {
+            par[j] = accr[j];
+            pai[j] = acci[j];
         }
-        if (tpost) {
+        Pp1 = Pc1; Pp0 = Pc0;
+        Pc1 = Pn1; Pc0 = Pn0;
+        Pn1 = Pq1; Pn0 = Pq0;
+        if constexpr (TP) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) oac[e] = oan[e];
+            for (int e = 0; e < NE; ++e) oac[e] = oan[e];
         }
         __syncthreads();   // B of p + 1 complete; the reads of Bs[cur] done before it is restaged
+    };
+    typedef std::integral_constant<bool, true> Yes;
+    typedef std::integral_constant<bool, false> No;
+    if (n == 1) {
+        momentum(0, No{}, No{});
+    } else {
+        momentum(0, Yes{}, No{});
+        for (int t = 1; t < n - 1; ++t) momentum(t, Yes{}, Yes{});
+        momentum(n - 1, No{}, Yes{});
     }
     const size_t offl = nn * ((size_t)iW + (size_t)NW * (pbeg + n - 1));
 #pragma unroll
-    for (int e = 0; e < 16; ++e) epilogue(e, offl, Pp1, Pp0);
+    for (int e = 0; e < NE; ++e) epilogue(e, offl, Pp1, Pp0);
 }
 
 // --- BSE_K3!:
