@@ -259,29 +259,33 @@ def c2_band(w):
     return wb, (nVP + wb + 15) // 16 - (nVP - wb) // 16
 
 
-def tune_gemm_grid(S, grids=(0, 256, 512, 768, 1024), reps=4):
-    """Workgroup count of the contraction GEMMs by whole-iteration time (hipGraph replay): a persistent
+def tune_gemm_grid(S, grids=(0, 256, 512, 768, 1024), lk3_grids=(0, 160, 192, 224), reps=4):
+    """Workgroup counts of the contraction launches by whole-iteration time (hipGraph replay): a persistent
     grid leaves compute units to the iteration's other chain (BSE_L_K2!, weights, BSE_K1!, BSE_K2!),
-    which the launch-by-launch autotuner cannot see.  The state is restored afterwards; results do not
-    depend on the grid."""
+    which the launch-by-launch autotuner cannot see.  First one count for both contractions, then
+    BSE_L_K3!'s alone (the FP32 lattice form runs one workgroup per compute unit: counts below 256 leave
+    some free).  The state is restored afterwards; results do not depend on the grids.  Returns
+    (BSE_L_K3!, BSE_K3!) counts."""
     import fdga
     x = fdga.flatten(S)
     S.set_graph(True)
-    best = None
-    for g in grids:
-        S.set_gemm_grid(True, g)
-        S.set_gemm_grid(False, g)
+
+    def timed(gl, gk):
+        S.set_gemm_grid(True, gl)
+        S.set_gemm_grid(False, gk)
         fdga.unflatten(S, x)
         fdga.iterate_solver(S, "fdPA")   # capture
         S.sync()
-        ms = S.time_iterations("fdPA", reps)
-        if best is None or ms < best[0]:
-            best = (ms, g)
-    S.set_gemm_grid(True, best[1])
-    S.set_gemm_grid(False, best[1])
+        return S.time_iterations("fdPA", reps)
+    best = min((timed(g, g), g) for g in grids)
+    gk = best[1]
+    best = min((timed(g, gk), g) for g in lk3_grids if g != gk or g == 0)
+    gl = best[1]
+    S.set_gemm_grid(True, gl)
+    S.set_gemm_grid(False, gk)
     S.set_graph(False)
     fdga.unflatten(S, x)
-    return best[1]
+    return gl, gk
 
 
 def gemm_roofline(head, P, gemm_peak, traffic, n3P):

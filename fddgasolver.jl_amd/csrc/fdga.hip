@@ -1036,12 +1036,17 @@ static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
         if (f32 && c->NV3 <= 64) {   // momentum-batched, the operands of one Ω on chip (same results)
             const K3Geom g = k3geom(c);
             const size_t npg = cdiv(c->NP, LK3P_G);
+            // one workgroup per task (slot, Ω, momentum group), or the persistent grid set for BSE_L_K3!
+            auto grid = [&](size_t ntask) {
+                const size_t full = 8 * (size_t)cdiv(ntask, 8);
+                return (unsigned)(c->kgrid[0] ? std::min(full, (size_t)c->kgrid[0]) : full);
+            };
             if (c->NV3 == 64) {   // the momenta software-pipelined (Nν3 = 64)
-                k_lk3_pb64<false><<<8 * cdiv(2 * c->NW3 * npg, 8), 512, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
-                k_lk3_pb64<true><<<8 * cdiv(c->NW3 * npg, 8), 512, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);
+                k_lk3_pb64<false><<<grid(2 * c->NW3 * npg), 512, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
+                k_lk3_pb64<true><<<grid(c->NW3 * npg), 512, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);
             } else {
-                k_lk3_pbatch<<<8 * cdiv(2 * c->NW3 * npg, 8), 256, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
-                k_lk3_pbatch<<<8 * cdiv(c->NW3 * npg, 8), 256, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);
+                k_lk3_pbatch<<<grid(2 * c->NW3 * npg), 256, 0, c->run>>>(g, l1.er.c0, l1.er.c1, 2);
+                k_lk3_pbatch<<<grid(c->NW3 * npg), 256, 0, c->run>>>(g, l2.er.c0, l2.er.c0, 1);
             }
             LAUNCHCHK(c);
         } else if (lk3_gemm(c, kChansPA, 2, l1) != hipSuccess || lk3_gemm(c, kChansT, 1, l2) != hipSuccess) {
@@ -2313,7 +2318,7 @@ int fdga_gemm_tile(const fdga_ctx* c, int32_t left) { return c ? c->ktile[left ?
 int fdga_gemm_grid(const fdga_ctx* c, int32_t left) { return c ? c->kgrid[left ? 0 : 1] : -1; }
 int fdga_set_gemm_grid(fdga_ctx* c, int32_t left, int32_t grid) {
     if (!c) return -1;
-    if (grid < 0 || grid % 256) return fail(c, "the workgroup count must be 0 (one per work item) or a multiple of 256");
+    if (grid < 0 || grid % 8) return fail(c, "the workgroup count must be 0 (one per work item) or a multiple of 8");
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->kgrid[left ? 0 : 1] = grid;
     graphs_invalidate(c);

@@ -1084,12 +1084,13 @@ __global__ __launch_bounds__(256, 2) void k_lk3_pbatch(K3Geom g, LK3Chan c0, LK3
     __shared__ float Gs[2][64][65];   // [re/im][k][m] = Γr(m, k) in FP32
     __shared__ float Bs[2][64][65];   // [re/im][k][n] = B(k, n) of the current momentum
     const int NV = g.NV(), NW = g.NW(), NP = g.NP, n3f = g.n3f;
-    const int npg = (NP + LK3P_G - 1) / LK3P_G;
-    // the blocks of one XCD (blockIdx % 8) take consecutive tasks, momentum group fastest: they share Ω
-    // and its momentum-independent operands in that XCD's L2
-    const int per = (int)(gridDim.x / 8);
-    const int task = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (task >= nslot * NW * npg) return;   // uniform over the block
+    const int npg = (NP + LK3P_G - 1) / LK3P_G, ntask = nslot * NW * npg;
+    // tasks in contiguous ranges per XCD (blockIdx % 8), momentum group fastest: the blocks of one XCD
+    // share Ω and its momentum-independent operands in that XCD's L2; a smaller grid loops over them
+    const int perx = (ntask + 7) / 8, nbx = (int)(gridDim.x / 8);
+    for (int kt = (int)(blockIdx.x / 8); kt < perx; kt += nbx) {
+    const int task = (int)(blockIdx.x % 8) * perx + kt;
+    if (task >= ntask) break;   // uniform over the block
     const int pg = task % npg, sw = task / npg, iW = sw % NW, s = sw / NW;
     const int m = iW - (g.n3b - 1);
     const size_t nn = (size_t)NV * NV;
@@ -1244,6 +1245,8 @@ __global__ __launch_bounds__(256, 2) void k_lk3_pbatch(K3Geom g, LK3Chan c0, LK3
                     out32[o] = make_float2((float)xr, (float)xi);
                 }
     }
+    __syncthreads();   // the next task rewrites Gs and Bs
+    }
 }
 
 // Nν3 = 64: the same tile with the momenta software-pipelined inside each workgroup.  While the MFMAs
@@ -1262,10 +1265,14 @@ __global__ __launch_bounds__(512, 1) void k_lk3_pb64(K3Geom g, LK3Chan c0, LK3Ch
     __shared__ float Gs[2][64][65];      // [re/im][k][m] = Γr(m, k) in FP32
     __shared__ float Bs[2][2][64][65];   // [buffer][re/im][k][n]
     const int NW = g.NW(), NP = g.NP, n3f = g.n3f;
-    const int npg = (NP + LK3P_G - 1) / LK3P_G;
-    const int per = (int)(gridDim.x / 8);
-    const int task = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (task >= nslot * NW * npg) return;   // uniform over the block
+    const int npg = (NP + LK3P_G - 1) / LK3P_G, ntask = nslot * NW * npg;
+    // tasks (slot, Ω, momentum group) in contiguous ranges per XCD (blockIdx % 8), momentum group fastest,
+    // so that the blocks of one XCD share Ω and its momentum-independent operands in its L2; a grid of
+    // fewer blocks than tasks loops over them
+    const int perx = (ntask + 7) / 8, nbx = (int)(gridDim.x / 8);
+    for (int k = (int)(blockIdx.x / 8); k < perx; k += nbx) {
+    const int task = (int)(blockIdx.x % 8) * perx + k;
+    if (task >= ntask) break;   // uniform over the block
     const int pg = task % npg, sw = task / npg, iW = sw % NW, s = sw / NW;
     const int m = iW - (g.n3b - 1);
     const size_t nn = (size_t)NV * NV;
@@ -1412,6 +1419,7 @@ __global__ __launch_bounds__(512, 1) void k_lk3_pb64(K3Geom g, LK3Chan c0, LK3Ch
     const size_t offl = nn * ((size_t)iW + (size_t)NW * (pbeg + n - 1));
 #pragma unroll
     for (int e = 0; e < NE; ++e) epilogue(e, offl, Pp1, Pp0);
+    }
 }
 
 // --- BSE_K3!:

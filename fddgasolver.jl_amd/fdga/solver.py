@@ -165,7 +165,7 @@ class Solver:
         return tuple(self.lib.fdga_gemm_grid(self.h, left) for left in (1, 0))
 
     def set_gemm_grid(self, left, grid):
-        """A persistent grid of `grid` workgroups (a multiple of 256) for that launch, or 0; results do
+        """A persistent grid of `grid` workgroups (a multiple of 8) for that launch, or 0; results do
         not depend on it."""
         self._check(self.lib.fdga_set_gemm_grid(self.h, int(left), int(grid)))
 

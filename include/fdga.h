@@ -177,8 +177,9 @@ double  fdga_gemm_flops(const fdga_ctx* ctx, int32_t left);
 int     fdga_gemm_tile(const fdga_ctx* ctx, int32_t left);
 int     fdga_set_gemm_tile(fdga_ctx* ctx, int32_t left, int32_t tile);
 /* workgroup count of that launch: 0 = one workgroup per output tile (the default), otherwise a
- * persistent grid of `grid` workgroups (a multiple of 256, so that every workgroup stays on one XCD)
- * looping over the tiles; a smaller grid leaves compute units to the other chain of the iteration.
+ * persistent grid of `grid` workgroups (a multiple of 8, so that every workgroup stays on one XCD)
+ * looping over the tiles (the FP32 lattice BSE_L_K3!: over its (channel, Ω, momentum group) tasks); a
+ * smaller grid leaves compute units to the other chain of the iteration.
  * Results do not depend on it.  No counterpart in the reference (execution option). */
 int     fdga_gemm_grid(const fdga_ctx* ctx, int32_t left);
 int     fdga_set_gemm_grid(fdga_ctx* ctx, int32_t left, int32_t grid);

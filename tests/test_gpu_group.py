@@ -85,6 +85,7 @@ def test_group_matches_members_at_64_frequencies():
     box = dict(nG=48, nK1=32, nK2=(4, 32), nK3=(2, 32), mPinu=1, T=0.3, U=1.3)
     mem = [("nl_L16_ref", box), ("nl_L16_ref", dict(box, nK3=(3, 32)))]
     solo, grp = _pairs(mem, seed=31, precision="fp32")
+    solo[1].set_gemm_grid(True, 8)   # the batched kernel's workgroups looping over its tasks
     G = fdga.SolverGroup(grp)
     try:
         for it in range(2):

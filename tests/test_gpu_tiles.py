@@ -47,11 +47,11 @@ def test_persistent_gemm_grid_same_iteration(case, precision):
         fdga.iterate_solver(ref, "fdPA")
     want = fdga.flatten(ref)
     S, _ = make_pair(case, seed=7, precision=precision)
-    with pytest.raises(fdga.FdgaError, match="multiple of 256"):
+    with pytest.raises(fdga.FdgaError, match="multiple of 8"):
         S.set_gemm_grid(True, 100)
     for graph in (False, True):
         S.set_graph(graph)
-        for g in (256, 512, 0):
+        for g in (8, 192, 256, 512, 0):
             fdga.unflatten(S, x0)
             S.set_gemm_grid(True, g)
             S.set_gemm_grid(False, g)
