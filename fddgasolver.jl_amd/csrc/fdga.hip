@@ -74,6 +74,9 @@ struct fdga_ctx {
     // FL.K3 of the three channels holds FP32 values in the first half of its bytes (the FP32 lattice
     // iteration's form; fl_widen restores FP64 for other readers)
     int fl32 = 0;
+    // F.K3 and Fbuff.K3 of the three channels hold FP32 values in the same way (the FP32 lattice
+    // iteration's form, taken and left with FL.K3's; fk_widen restores FP64)
+    int fk32 = 0;
     // the stream the stage kernels are launched on: st, or st2 for the second chain of an iteration
     hipStream_t run = nullptr;
     hipStream_t st2 = nullptr;
@@ -227,7 +230,9 @@ ChanV chan_view(const fdga_ctx* c, const DevChan& dc, int n1, int n2b, int n2f, 
     v.A1 = dc.A1; v.A2 = dc.A2; v.A3 = dc.A3;
     v.n1 = n1; v.n2b = n2b; v.n2f = n2f; v.n3b = n3b; v.n3f = n3f;
     v.NP = NP;
-    (void)c;
+    v.k3f = 0;   // K3 in FP32: F and Fbuff under fk32, FL under fl32
+    for (int r = 0; r < 3; ++r)
+        if (((&dc == &c->F[r] || &dc == &c->Fb[r]) && c->fk32) || (&dc == &c->FL[r] && c->fl32)) v.k3f = 1;
     return v;
 }
 ChanV chanF(const fdga_ctx* c, const DevChan& dc) {
@@ -313,6 +318,7 @@ int refresh_avg(fdga_ctx* c) {
         for (int k = 0; k < 3; ++k, ++j) {
             jobs.A[j] = ak[k].first;
             jobs.K[j] = ak[k].second;
+            jobs.Kf[j] = (k == 2 && c->fk32) ? reinterpret_cast<const float2*>(c->F[r].K3) : nullptr;
             jobs.n[j] = nn[k];
             jobs.blk0[j] = blk;
             blk += cdiv(nn[k], 16);
@@ -1019,8 +1025,37 @@ static int fl_narrow(fdga_ctx* c) {
     c->r32_fresh = 0;
     return 0;
 }
-// the FL.K3 form an iteration of this context works in
-static int fl_prepare(fdga_ctx* c) { return fl32_mode(c) ? fl_narrow(c) : fl_widen(c); }
+// The same for F.K3 and Fbuff.K3 (six arrays): the FP32 lattice iteration reads F.K3 (build_K3_cache!,
+// BSE_K3!'s left operand, the K2 finish, the SDE, the momentum averages) and writes Fbuff.K3 in FP32,
+// 2.6 GB less traffic per iteration at L = 16.  Flatten, unflatten and the array accessors convert
+// on the fly; anything else that reads or writes them raw widens first.
+static int fk_convert(fdga_ctx* c, bool narrow) {
+    if ((c->fk32 != 0) == narrow) return 0;
+    const size_t n = (size_t)c->n3 * c->NP;
+    float2* tmp = reinterpret_cast<float2*>(c->R[0]);
+    for (DevChan* v : {c->F, c->Fb})
+        for (int r = 0; r < 3; ++r) {
+            if (narrow) {
+                k_narrow32<<<cdiv(n, 256), 256, 0, c->run>>>(tmp, v[r].K3, n);
+                LAUNCHCHK(c);
+                HIPCHK(c, hipMemcpyAsync(v[r].K3, tmp, n * sizeof(float2), hipMemcpyDeviceToDevice, c->run));
+            } else {
+                HIPCHK(c, hipMemcpyAsync(tmp, v[r].K3, n * sizeof(float2), hipMemcpyDeviceToDevice, c->run));
+                k_widen32<<<cdiv(n, 256), 256, 0, c->run>>>(v[r].K3, tmp, n);
+                LAUNCHCHK(c);
+            }
+        }
+    c->fk32 = narrow;
+    c->r32_fresh = 0;
+    // the momentum averages of the rounded values (as unflatten would form them)
+    return narrow ? refresh_avg(c) : 0;
+}
+static int fk_widen(fdga_ctx* c) { return fk_convert(c, false); }
+// the FL.K3, F.K3 and Fbuff.K3 forms an iteration of this context works in
+static int fl_prepare(fdga_ctx* c) {
+    const bool f32 = fl32_mode(c);
+    return (f32 ? fl_narrow(c) : fl_widen(c)) || fk_convert(c, f32);
+}
 
 static int launch_lk3(fdga_ctx* c, const int* chans, int n) {
     const bool sym = k3_sym(c, chans, n, true);
@@ -1077,7 +1112,7 @@ struct K3Set {
     K3Chan kc[3];
     double T;
     K3_Bop b() const { return K3_Bop{g, T, kc[0], kc[1], kc[2]}; }
-    K3_Ep e() const { return K3_Ep{g, T, kc[0], kc[1], kc[2]}; }
+    template <int FORM = 2> K3_EpT<FORM> e() const { return K3_EpT<FORM>{g, T, kc[0], kc[1], kc[2]}; }
     template <bool FULL> K3_AopT<FULL> a() const { return K3_AopT<FULL>{g, T, kc[0], kc[1], kc[2]}; }
     bool op32() const { return kc[0].A32 != nullptr; }   // FP32 A and R (lattice solver, FP32 contraction)
     bool r32() const { return kc[0].R32 != nullptr; }    // FP32 R (any solver, FP32 contraction)
@@ -1090,6 +1125,7 @@ struct K3Set {
 // fl32: FL.K3 in its FP32 form (-1: as the context holds it now; solver groups fix it per member)
 static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, bool mfrg = false, int fl32 = -1) {
     const bool f32 = fl32 < 0 ? c->fl32 != 0 : fl32 != 0;
+    const bool fk = fl32 < 0 ? c->fk32 != 0 : fl32 != 0;   // F.K3 / Fbuff.K3 in FP32
     K3Set ks{};
     ks.g = k3geom(c);
     ks.T = c->d.T;
@@ -1121,10 +1157,10 @@ static K3Set k3_set(fdga_ctx* c, const int* chans, int n, bool sym, bool t_epi, 
         kc[s].csum = c->csum[ch];
         kc[s].Lr = c->FL[ch].K3;
         kc[s].La = c->FL[ACH].K3;
-        kc[s].Lr32 = f32 ? reinterpret_cast<const float2*>(c->FL[ch].K3) : nullptr;
-        kc[s].La32 = f32 ? reinterpret_cast<const float2*>(c->FL[ACH].K3) : nullptr;
+        kc[s].l32 = f32;
         kc[s].out = (sym && ch == TCH) ? c->k3raw : c->Fb[ch].K3;
         kc[s].outa = c->Fb[ACH].K3;
+        kc[s].k32 = fk;   // (never with symmetry groups: those are the local solver's)
         kc[s].tpost = (ch == TCH && t_epi);
         if (c->d.precision == FDGA_FP32) {
             // FP32 operands in the right-operand buffer: R32 in its first half, and for the lattice
@@ -1176,10 +1212,19 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n, bool sym = false, bool sk
 }
 // the BSE_K3! GEMM launch of `n` channels with the loaders of the solver kind and precision
 static hipError_t k3_gemm_set(fdga_ctx* c, const int* chans, int n, const K3Set& ks) {
-    const K3_Ep e = ks.e();
-    if (ks.kc[0].full)
-        return ks.r32() ? k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b32(), e) : k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b(), e);
-    return ks.op32() ? k3_gemm(c, chans, n, 1, ks.a32(), ks.b32(), e) : k3_gemm(c, chans, n, 1, ks.a<false>(), ks.b(), e);
+    // the FP64 solver's arrays are all FP64; the FP32 lattice iteration's all FP32
+    if (c->d.precision == FDGA_FP64)
+        return ks.kc[0].full ? k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b(), ks.e<0>())
+                             : k3_gemm(c, chans, n, 1, ks.a<false>(), ks.b(), ks.e<0>());
+    bool all32 = true, any32 = false;
+    for (int s = 0; s < n; ++s) {
+        all32 = all32 && ks.kc[s].l32 && ks.kc[s].k32;
+        any32 = any32 || ks.kc[s].l32 || ks.kc[s].k32;
+    }
+    if (ks.kc[0].full)   // the local solver (FP64 forms), or build_K3_cache_mfRG!'s caches
+        return any32 ? k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b32(), ks.e<2>())
+                     : k3_gemm(c, chans, n, 1, ks.a<true>(), ks.b32(), ks.e<0>());
+    return all32 ? k3_gemm(c, chans, n, 1, ks.a32(), ks.b32(), ks.e<1>()) : k3_gemm(c, chans, n, 1, ks.a32(), ks.b32(), ks.e<2>());
 }
 // BSE_K3! for `n` channels: right operands and column sums, then the GEMM launch(es)
 static int launch_k3(fdga_ctx* c, const int* chans, int n, bool mfrg = false) {
@@ -1198,6 +1243,10 @@ static int launch_k3(fdga_ctx* c, const int* chans, int n, bool mfrg = false) {
             return fail(c, "BSE_K3 GEMM launch failed", -3);
         kernel_end(c, 10);
         return 0;
+    }
+    if (c->fk32) {   // a subset of the channels, or symmetry groups: FP64 F.K3 and Fbuff.K3
+        if (fk_widen(c)) return -3;
+        fresh = false;
     }
     const bool t_epi = (n == 1 && chans[0] == TCH && !sym);
     const K3Set ks = k3_set(c, chans, n, sym, t_epi, mfrg);
@@ -1526,6 +1575,12 @@ static void invalidate(fdga_ctx* c, int which) {
     }
 }
 
+// F.K3 or Fbuff.K3 of a channel while they hold FP32 values
+static bool fk_array(const fdga_ctx* c, int which) {
+    const int k = which - FDGA_VERTEX0;
+    return c->fk32 && k >= 0 && k < 32 && (k % 16) < 9 && (k % 16) % 3 == 2;
+}
+
 int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
     if (!c || !host) return -1;
     c->r32_fresh = 0;
@@ -1539,7 +1594,9 @@ int fdga_set(fdga_ctx* c, int32_t which, const double* host) {
     } else {
         if (ensure_stage(c, a.n)) return -2;
         HIPCHK(c, hipMemcpyAsync(c->stage, host, bytes, hipMemcpyHostToDevice, c->st));
-        if (a.kind == 1) k_k3_to_dev<<<cdiv(a.n, 256), 256, 0, c->run>>>(a.p, c->stage, a.NW, a.NV, a.NP);
+        if (a.kind == 1 && fk_array(c, which))
+            k_k3_to_dev32<<<cdiv(a.n, 256), 256, 0, c->run>>>(reinterpret_cast<float2*>(a.p), c->stage, a.NW, a.NV, a.NP);
+        else if (a.kind == 1) k_k3_to_dev<<<cdiv(a.n, 256), 256, 0, c->run>>>(a.p, c->stage, a.NW, a.NV, a.NP);
         else k_pi_to_dev<<<cdiv(a.n, 256), 256, 0, c->run>>>(a.p, c->stage, a.NW, a.NV, a.NP);
         LAUNCHCHK(c);
         if (a.kind == 2 && tail_sums(c, a.p, pi_tail(c, a.p))) return -3;
@@ -1573,7 +1630,9 @@ int fdga_get(fdga_ctx* c, int32_t which, double* host) {
         HIPCHK(c, hipMemcpyAsync(host, a.p, bytes, hipMemcpyDeviceToHost, c->st));
     } else {
         if (ensure_stage(c, a.n)) return -2;
-        if (a.kind == 1) k_k3_to_ref<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, a.NW, a.NV, a.NP);
+        if (a.kind == 1 && fk_array(c, which))
+            k_k3_to_ref32<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, reinterpret_cast<const float2*>(a.p), a.NW, a.NV, a.NP);
+        else if (a.kind == 1) k_k3_to_ref<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, a.NW, a.NV, a.NP);
         else if (a.kind == 2) k_pi_to_ref<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, a.NW, a.NV, a.NP);
         else if (a.kind == 3) k_cache_to_ref<<<cdiv(a.n, 256), 256, 0, c->run>>>(c->stage, a.p, a.NW, a.NV, a.NP);
         else {
@@ -2032,7 +2091,7 @@ int fdga_iterate(fdga_ctx* c, int32_t strategy, int32_t update_sigma, int32_t co
     if (strategy < FDGA_SCPA || strategy > FDGA_FDPA_NEW) return fail(c, "Calculation strategy unknown");
     const bool fd = (strategy == FDGA_FDPA || strategy == FDGA_FDPA_NEW);
     c->caches_mfrg = 0;
-    if (fl_prepare(c)) return -3;   // eagerly: a captured iteration finds FL.K3 in its form   // the iteration's build_K3_cache! (also when replayed from a graph)
+    if (fl_prepare(c)) return -3;   // eagerly: a captured iteration finds FL.K3 and F.K3 in their form
     // graph replay: the fixed-point iteration with Σ update and the Hartree term, stage timing off, and
     // for the fdPA strategies once the reference system's SDE is cached (the first such iteration
     // computes it eagerly)
@@ -2122,7 +2181,10 @@ int unflatten_into(fdga_ctx* c, const double* x, bool sigma, double scale, doubl
         off += c->n1 * c->NP;
         HIPCHK(c, hipMemcpyAsync(c->F[r].K2, buf + off, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
         off += c->n2 * c->NP;
-        k_k3_to_dev<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(c->F[r].K3, buf + off, c->NW3, c->NV3, c->NP);
+        if (c->fk32)
+            k_k3_to_dev32<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(reinterpret_cast<float2*>(c->F[r].K3), buf + off,
+                                                                          c->NW3, c->NV3, c->NP);
+        else k_k3_to_dev<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(c->F[r].K3, buf + off, c->NW3, c->NV3, c->NP);
         LAUNCHCHK(c);
         off += c->n3 * c->NP;
     }
@@ -2143,7 +2205,10 @@ int flatten_from(fdga_ctx* c, double* x, bool sigma, double2* buf = nullptr) {
         off += c->n1 * c->NP;
         HIPCHK(c, hipMemcpyAsync(buf + off, c->F[r].K2, c->n2 * c->NP * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
         off += c->n2 * c->NP;
-        k_k3_to_ref<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(buf + off, c->F[r].K3, c->NW3, c->NV3, c->NP);
+        if (c->fk32)
+            k_k3_to_ref32<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(buf + off, reinterpret_cast<const float2*>(c->F[r].K3),
+                                                                          c->NW3, c->NV3, c->NP);
+        else k_k3_to_ref<<<cdiv(c->n3 * c->NP, 256), 256, 0, c->run>>>(buf + off, c->F[r].K3, c->NW3, c->NV3, c->NP);
         LAUNCHCHK(c);
         off += c->n3 * c->NP;
     }
@@ -2376,11 +2441,7 @@ int fdga_autotune(fdga_ctx* c, int32_t reps) {
     bt = c->ktile[1];
     for (int t = 0; t < TILE_N; ++t) {
         c->ktile[1] = t;
-        const float ms = c->NP == 1
-                             ? (ks.r32() ? timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<true>(), ks.b32(), ks.e()); })
-                                         : timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<true>(), ks.b(), ks.e()); }))
-                         : ks.op32() ? timeit([&] { return k3_gemm(c, all, 3, 1, ks.a32(), ks.b32(), ks.e()); })
-                                     : timeit([&] { return k3_gemm(c, all, 3, 1, ks.a<false>(), ks.b(), ks.e()); });
+        const float ms = timeit([&] { return k3_gemm_set(c, all, 3, ks); });
         if (ms < best) { best = ms; bt = t; }
     }
     c->ktile[1] = bt;
@@ -2900,7 +2961,9 @@ int fdga_group_autotune(fdga_group* g, int32_t reps) {
     if (fdga_group_sync(g) || group_fork(g)) return -2;
     for (fdga_ctx* c : g->m) {
         c->run = c->st;
-        if (fdga_build_k3_cache(c) || k3_rhs(c, k3_set(c, all, 3, false, false), 3)) return member_err(g, c);
+        // the operand forms the group's launches were made for (those of the members' iterations)
+        if (fl_prepare(c) || fdga_build_k3_cache(c) || k3_rhs(c, k3_set(c, all, 3, false, false), 3))
+            return member_err(g, c);
     }
     if (group_join(g) || fdga_group_sync(g)) return -2;
     int dev = 0, ncu = 256;

@@ -73,6 +73,7 @@ struct ChanV {
     const double2* A3;
     int n1, n2b, n2f, n3b, n3f;
     int NP;
+    int k3f;   // K3 holds FP32 values (the FP32 lattice iteration's vertex) in the first half of its bytes
 
     __host__ __device__ __forceinline__ size_t s1() const { return (size_t)(2 * n1 - 1); }
     __host__ __device__ __forceinline__ size_t s2() const { return (size_t)(2 * n2b - 1) * (2 * n2f); }
@@ -84,7 +85,18 @@ struct ChanV {
     }
     __device__ __forceinline__ double2 k1(int m, int p) const { return p < 0 ? A1[i1(m)] : K1[i1(m) + s1() * p]; }
     __device__ __forceinline__ double2 k2(int m, int n, int p) const { return p < 0 ? A2[i2(m, n)] : K2[i2(m, n) + s2() * p]; }
-    __device__ __forceinline__ double2 k3(int m, int n, int np, int p) const { return p < 0 ? A3[i3(m, n, np)] : K3[i3(m, n, np) + s3() * p]; }
+    __device__ __forceinline__ double2 k3(int m, int n, int np, int p) const {
+        if (k3f && p >= 0) return k3p<true>(m, n, np, p);
+        return p < 0 ? A3[i3(m, n, np)] : K3[i3(m, n, np) + s3() * p];   // (one load of a selected address)
+    }
+    // K3 at momentum p >= 0 in the form K32 (== k3f): loops over K3 elements branch on the form once,
+    // outside, and issue one load per element
+    template <bool K32>
+    __device__ __forceinline__ double2 k3p(int m, int n, int np, int p) const {
+        const size_t i = i3(m, n, np) + s3() * p;
+        if (K32) { const float2 v = reinterpret_cast<const float2*>(K3)[i]; return cmk(v.x, v.y); }
+        return K3[i];
+    }
 
     // γ(Ω, ν, νp[, P]) with all classes (src/channel.jl:220-339; src/nonlocal/channel.jl:90-226)
     __device__ __forceinline__ double2 eval(int m, int n, int np, int p) const {

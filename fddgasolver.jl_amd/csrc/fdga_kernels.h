@@ -421,14 +421,19 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __
 // s-wave average over P of one array of `n` elements per P: a block takes 16 consecutive
 // elements, its 16 thread groups sum every 16th momentum, and the groups' sums are added in a
 // fixed order
-__device__ __forceinline__ void pavg_block(double2* __restrict__ A, const double2* __restrict__ K, size_t n, int NP,
-                                           size_t blk) {
+// (Kf: the array held in FP32, read instead of K)
+__device__ __forceinline__ void pavg_block(double2* __restrict__ A, const double2* __restrict__ K, const float2* __restrict__ Kf,
+                                           size_t n, int NP, size_t blk) {
     __shared__ double2 red[16][17];
     const int el = threadIdx.x & 15, grp = threadIdx.x >> 4;
     const size_t i = blk * 16 + el;
     double2 s = czero();
-    if (i < n)
-        for (int p = grp; p < NP; p += 16) s = cadd(s, K[i + n * p]);
+    if (i < n) {
+        if (Kf)
+            for (int p = grp; p < NP; p += 16) { const float2 v = Kf[i + n * p]; s = cadd(s, cmk(v.x, v.y)); }
+        else
+            for (int p = grp; p < NP; p += 16) s = cadd(s, K[i + n * p]);
+    }
     red[grp][el] = s;
     __syncthreads();
     if (grp == 0 && i < n) {
@@ -443,6 +448,7 @@ constexpr int PAVG_JOBS = 9;
 struct PavgJobs {
     double2* A[PAVG_JOBS];
     const double2* K[PAVG_JOBS];
+    const float2* Kf[PAVG_JOBS];
     size_t n[PAVG_JOBS];
     unsigned blk0[PAVG_JOBS + 1];
     int njob;
@@ -450,7 +456,7 @@ struct PavgJobs {
 __global__ __launch_bounds__(256) void k_pavg_multi(PavgJobs jobs, int NP) {
     int j = 0;
     while (j + 1 < jobs.njob && blockIdx.x >= jobs.blk0[j + 1]) ++j;
-    pavg_block(jobs.A[j], jobs.K[j], jobs.n[j], NP, blockIdx.x - jobs.blk0[j]);
+    pavg_block(jobs.A[j], jobs.K[j], jobs.Kf[j], jobs.n[j], NP, blockIdx.x - jobs.blk0[j]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1448,9 +1454,10 @@ struct K3Chan {
     // A32 holds the K2r(ν) term as well (csum null: no rank-1 term in the epilogue), read from
     // K2row[i + Nν3 (iΩ + NΩ3 P)] = K2r(Ω, ν = i - n3f, P) (t: 2 K2t - K2a), written by k_k2_rows
     double2* K2row;
-    // FL.K3 in FP32 (LK3Chan::out32): the self term is read from these when set
-    const float2* Lr32;
-    const float2* La32;
+    // l32: Lr / La hold FP32 values (FL.K3 in its FP32 form, LK3Chan::out32); k32: so do K3r / K3a
+    // and out / outa (the vertex and Fbuff of the FP32 lattice iteration).  Flags rather than second
+    // pointers keep the launch arguments (scalar registers) of the three channels small.
+    int l32, k32;
 };
 // FULL: the cache holds the whole cache_Fr (one momentum, a single load per element)
 template <bool FULL>
@@ -1562,8 +1569,14 @@ __global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, 
             if (ip < pend) {
                 const int p = pl ? pl[ip] : ip;
                 const size_t r = (size_t)o + (size_t)nn * (iW + (size_t)NW * p);
-                k3[u] = c.K3r[r];
-                if (t) ka[u] = c.K3a[r];
+                if (c.k32) {   // the vertex K3 held in FP32
+                    const float2 a = reinterpret_cast<const float2*>(c.K3r)[r];
+                    k3[u] = cmk(a.x, a.y);
+                    if (t) { const float2 b = reinterpret_cast<const float2*>(c.K3a)[r]; ka[u] = cmk(b.x, b.y); }
+                } else {
+                    k3[u] = c.K3r[r];
+                    if (t) ka[u] = c.K3a[r];
+                }
                 kr[u] = c.K2row ? c.K2row[(size_t)i + (size_t)NV * (iW + (size_t)NW * p)] : czero();
             }
         }
@@ -1595,29 +1608,32 @@ __global__ __launch_bounds__(256) void k_k2_rows(K3Geom g, K3Chan c0, K3Chan c1,
     if (c.ch == TCH) v = csub(cscale(v, 2.0), c.K2a[q]);
     c.K2row[r] = v;
 }
-struct K3_Ep {
+// FORM: 0 FL.K3, F.K3 and Fbuff.K3 all FP64; 1 all FP32 (the FP32 lattice iteration); 2 as each
+// channel's l32 / k32 flags say.  The fixed forms compile one load and one store form per array.
+template <int FORM>
+struct K3_EpT {
     K3Geom g; double T; K3Chan c0, c1, c2;
     struct Bound {
         const double2* k2r; const double2* k2a;    // k2r[NW2 * i] = K2r(Ω, ν = i - n3f, P)
-        const double2* csum; const double2* Lr; const double2* La;
-        double2* out; const double2* outa;
-        const float2* Lr32; const float2* La32;
-        double T; int t, tpost, full, NV, NW2;
+        const double2* csum;
+        // FL.K3 of the channel and of a (FP32 values when l32); Fbuff.K3 of the channel and the
+        // finished a channel (FP32 when o32): one address per array whatever its form
+        const void* L; const void* La;
+        void* out; const void* outa;
+        double T; int t, tpost, full, NV, NW2, l32, o32;
+        __device__ __forceinline__ static double2 ld(const void* p, size_t o, int f32) {
+            if (f32) { const float2 v = static_cast<const float2*>(p)[o]; return cmk(v.x, v.y); }
+            return static_cast<const double2*>(p)[o];
+        }
         // the element's loads (FL.K3, and for t FL.a and the finished a channel) issued together,
         // before any arithmetic
         struct Pre { double2 self, oa; };
         __device__ __forceinline__ Pre pre(int i, int j) const {
             const size_t o = (size_t)i + (size_t)NV * j;
             Pre q;
-            if (Lr32) {
-                const float2 l = Lr32[o];
-                q.self = cmk(l.x, l.y);
-                if (t) { const float2 a = La32[o]; q.self = csub(cscale(q.self, 2.0), cmk(a.x, a.y)); }
-            } else {
-                q.self = Lr[o];
-                if (t) q.self = csub(cscale(q.self, 2.0), La[o]);
-            }
-            q.oa = tpost ? outa[o] : czero();
+            q.self = ld(L, o, l32);
+            if (t) q.self = csub(cscale(q.self, 2.0), ld(La, o, l32));
+            q.oa = tpost ? ld(outa, o, o32) : czero();
             return q;
         }
         __device__ __forceinline__ void post(int i, int j, double2 v, const Pre& q) const {
@@ -1629,7 +1645,8 @@ struct K3_Ep {
             const size_t o = (size_t)i + (size_t)NV * j;
             double2 x = cadd(cscale(v, T), q.self);
             if (tpost) x = cscale(cadd(x, q.oa), 0.5);
-            out[o] = x;
+            if (o32) static_cast<float2*>(out)[o] = make_float2((float)x.x, (float)x.y);
+            else static_cast<double2*>(out)[o] = x;
         }
         __device__ __forceinline__ void operator()(int i, int j, double2 v) const { post(i, j, v, pre(i, j)); }
     };
@@ -1646,12 +1663,14 @@ struct K3_Ep {
         r.k2r = cs ? c.K2r + k2o : nullptr;
         r.k2a = (cs && t) ? c.K2a + k2o : nullptr;
         r.csum = cs ? c.csum + (size_t)NV * bb : nullptr;
-        r.Lr = c.Lr + off;
-        r.La = t ? c.La + off : nullptr;
-        r.Lr32 = c.Lr32 ? c.Lr32 + off : nullptr;
-        r.La32 = (c.Lr32 && t) ? c.La32 + off : nullptr;
-        r.out = c.out + off;
-        r.outa = c.tpost ? c.outa + off : nullptr;
+        // element offsets in the array's own form (8-byte FP32 or 16-byte FP64 elements)
+        r.l32 = FORM == 2 ? c.l32 : FORM;
+        r.o32 = FORM == 2 ? c.k32 : FORM;
+        const size_t lo = off * (r.l32 ? 8 : 16), oo = off * (r.o32 ? 8 : 16);
+        r.L = reinterpret_cast<const char*>(c.Lr) + lo;
+        r.La = t ? reinterpret_cast<const char*>(c.La) + lo : nullptr;
+        r.out = reinterpret_cast<char*>(c.out) + oo;
+        r.outa = c.tpost ? reinterpret_cast<const char*>(c.outa) + oo : nullptr;
         r.T = T; r.t = t; r.tpost = c.tpost; r.full = c.full; r.NV = NV; r.NW2 = NW2;
         return r;
     }
@@ -1660,6 +1679,7 @@ struct K3_Ep {
         return s == 0 ? one(c0, bb) : s == 1 ? one(c1, bb) : one(c2, bb);
     }
 };
+typedef K3_EpT<2> K3_Ep;   // the solver groups' (one functor type for every member)
 
 // right operand of BSE_K3! and its column sums; one wave per (νp, Ω, P), lanes over ω
 struct K3RChan {
@@ -2588,14 +2608,19 @@ __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, c
         acc = cadd(acc, cadd(cmul(xf[0], wl), cmul(xf[NV2], wr)));
         acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), wtot));
         if (k3m && in_f(v, g.n3f)) {
-            double2 k3s = czero();
+            // the three channels of F share one K3 form
+            auto wsum = [&](auto form) {
+                constexpr bool K32 = decltype(form)::value;
+                double2 k3s = czero();
 #pragma unroll 8
-            for (int w = -g.n3f; w < g.n3f; ++w) {
-                double2 k3 = g.k3(m, v, w, p);
-                if (CH == TCH) k3 = csub(cscale(k3, 2.0), F.g[ACH].k3(m, v, w, p));
-                k3s = cadd(k3s, cmul(k3, Wr[w]));
-            }
-            acc = cadd(acc, k3s);
+                for (int w = -g.n3f; w < g.n3f; ++w) {
+                    double2 k3 = g.template k3p<K32>(m, v, w, p);
+                    if (CH == TCH) k3 = csub(cscale(k3, 2.0), F.g[ACH].template k3p<K32>(m, v, w, p));
+                    k3s = cadd(k3s, cmul(k3, Wr[w]));
+                }
+                return k3s;
+            };
+            acc = cadd(acc, g.k3f ? wsum(std::true_type{}) : wsum(std::false_type{}));
         }
         const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
         if (raw) { raw[o] = cscale(acc, scale); continue; }
@@ -2722,6 +2747,10 @@ struct ChanBox {
     __device__ double2 k3(int m, int n, int np) const {
         return in_b(m, g.n3b) && in_f(n, g.n3f) && in_f(np, g.n3f) ? g.k3(m, n, np, pc) : czero();
     }
+    template <bool K32>
+    __device__ double2 k3p(int m, int n, int np) const {   // (pc >= 0)
+        return in_b(m, g.n3b) && in_f(n, g.n3f) && in_f(np, g.n3f) ? g.template k3p<K32>(m, n, np, pc) : czero();
+    }
 };
 // the channels of one Vertex level of the lattice SDE: at momentum P (chanNP > 1) or the local
 // vertex (chanNP = 1)
@@ -2788,21 +2817,31 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
             xp = cadd(xp, cadd(cmul(cadd(k1p[l], P.k2(m, v)), spp), k2p[l]));
             xh = cadd(xh, cadd(cmul(cadd(k1h[l], cadd(A.k2(m, v), T.k2(m, v))), sph), k2h[l]));
             const int n3p = L.gp.n3f, n3h = max(L.ga.n3f, L.gt.n3f);
+            // the K3 form is the level's (one vertex): the loops branch on it once
             if (in_b(m, L.gp.n3b) && in_f(v, n3p)) {
-                double2 sp = czero();
+                auto psum = [&](auto form) {
+                    constexpr bool K32 = decltype(form)::value;
+                    double2 sp = czero();
 #pragma unroll 8
-                for (int n = -n3p; n < n3p; ++n) {
-                    const int wp = m - n - 1;
-                    if (inmesh(wp)) sp = cadd(sp, cmul(P.k3(m, n, v), pp[wp]));
-                }
-                xp = cadd(xp, sp);
+                    for (int n = -n3p; n < n3p; ++n) {
+                        const int wp = m - n - 1;
+                        if (inmesh(wp)) sp = cadd(sp, cmul(P.template k3p<K32>(m, n, v), pp[wp]));
+                    }
+                    return sp;
+                };
+                xp = cadd(xp, L.gp.k3f ? psum(std::true_type{}) : psum(std::false_type{}));
             }
             if ((in_b(m, L.ga.n3b) || in_b(m, L.gt.n3b)) && in_f(v, n3h)) {
-                double2 sh = czero();
+                auto hsum = [&](auto form) {
+                    constexpr bool K32 = decltype(form)::value;
+                    double2 sh = czero();
 #pragma unroll 8
-                for (int n = -n3h; n < n3h; ++n)
-                    if (inmesh(n)) sh = cadd(sh, cmul(cadd(A.k3(m, v, n), T.k3(m, v, n)), ph[n]));
-                xh = cadd(xh, sh);
+                    for (int n = -n3h; n < n3h; ++n)
+                        if (inmesh(n))
+                            sh = cadd(sh, cmul(cadd(A.template k3p<K32>(m, v, n), T.template k3p<K32>(m, v, n)), ph[n]));
+                    return sh;
+                };
+                xh = cadd(xh, L.ga.k3f ? hsum(std::true_type{}) : hsum(std::false_type{}));
             }
         }
         const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
@@ -3028,6 +3067,31 @@ __global__ void k_k3_to_dev(double2* __restrict__ out, const double2* __restrict
     int j = (int)(r % NV);
     size_t p = r / NV;
     out[(size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * ((size_t)iW + (size_t)NW * p))] = in[e];
+}
+// the same, for a K3 array held in FP32 on the device
+__global__ void k_k3_to_dev32(float2* __restrict__ out, const double2* __restrict__ in, int NW, int NV, int NP) {
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t tot = (size_t)NW * NV * NV * NP;
+    if (e >= tot) return;
+    int iW = (int)(e % NW);
+    size_t r = e / NW;
+    int i = (int)(r % NV); r /= NV;
+    int j = (int)(r % NV);
+    size_t p = r / NV;
+    const double2 v = in[e];
+    out[(size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * ((size_t)iW + (size_t)NW * p))] = make_float2((float)v.x, (float)v.y);
+}
+__global__ void k_k3_to_ref32(double2* __restrict__ out, const float2* __restrict__ in, int NW, int NV, int NP) {
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t tot = (size_t)NW * NV * NV * NP;
+    if (e >= tot) return;
+    int iW = (int)(e % NW);
+    size_t r = e / NW;
+    int i = (int)(r % NV); r /= NV;
+    int j = (int)(r % NV);
+    size_t p = r / NV;
+    const float2 v = in[(size_t)i + (size_t)NV * ((size_t)j + (size_t)NV * ((size_t)iW + (size_t)NW * p))];
+    out[e] = cmk(v.x, v.y);
 }
 __global__ void k_k3_to_ref(double2* __restrict__ out, const double2* __restrict__ in, int NW, int NV, int NP) {
     size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
