@@ -2758,6 +2758,47 @@ struct SdeLevelNL {
     ChanV gp, gt, ga;
     int chanNP;
 };
+__device__ __forceinline__ double2 shfl_xor2(double2 v, int mask) {
+    return cmk(__shfl_xor(v.x, mask, WAVE), __shfl_xor(v.y, mask, WAVE));
+}
+// The p-channel K3 sum of the lattice SDE for every ν of the level's K3 box (2 n3f <= 64):
+//   S(ν) = Σ_n K3p(Ω, n, ν) Πpp(Ω - n - 1)   (n in the box, Ω - n - 1 on the bubble mesh)
+// with the lanes over n, so that each load reads a contiguous column of K3 (its first index), and the
+// products of four ν at a time reduced across the wave by recursive halving (lanes exchange half
+// their partial sums at distances 32 and 16, then sum at 8, 4, 2 and 1).  Returns S(ν) at the lane
+// iv = ν + n2f (ν = iv - n2f outside the box: zero).
+template <bool K32>
+__device__ __forceinline__ double2 sde_p3_lanes(const ChanV& g, int pc, int m, const double2* pp, int nVP, int n2f,
+                                                int lane) {
+    const int n3 = g.n3f, NV = 2 * n3;
+    const int n = lane - n3, wp = m - n - 1;
+    const bool ln = lane < NV && wp >= -nVP && wp < nVP;
+    const double2 w = ln ? pp[wp] : czero();
+    const bool h5 = lane & 32, h4 = lane & 16;
+    const int vme = lane - n2f + n3;   // this lane's ν, as an index into the box
+    double2 mine = czero();
+    for (int c0 = 0; c0 < NV; c0 += 4) {
+        double2 x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            x[j] = (ln && c0 + j < NV) ? cmul(g.template k3p<K32>(m, n, c0 + j - n3, pc), w) : czero();
+        double2 y[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) y[k] = cadd(h5 ? x[2 + k] : x[k], shfl_xor2(h5 ? x[k] : x[2 + k], 32));
+        double2 u = cadd(h4 ? y[1] : y[0], shfl_xor2(h4 ? y[0] : y[1], 16));
+        u = cadd(u, shfl_xor2(u, 8));
+        u = cadd(u, shfl_xor2(u, 4));
+        u = cadd(u, shfl_xor2(u, 2));
+        u = cadd(u, shfl_xor2(u, 1));
+        // u is S(ν) of box index c0 + ((lane >> 5) & 1) * 2 + ((lane >> 4) & 1)
+        const int jv = (vme - c0) & 3;
+        const int src = ((jv >> 1) & 1) << 5 | (jv & 1) << 4;
+        const double2 s = cmk(__shfl(u.x, src, WAVE), __shfl(u.y, src, WAVE));
+        if (vme >= c0 && vme < c0 + 4) mine = s;
+    }
+    return mine;
+}
+
 // NLEV Vertex levels summed into one pair of L-vertices (the target vertex and the local reference
 // vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch)
 template <int NLEV>
@@ -2806,6 +2847,20 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
         k1p[l] = P.k1(m);
         k1h[l] = cadd(A.k1(m), T.k1(m));
     }
+    // the p-channel K3 sums with the lanes over the summation index (one pass over ν: 2 n2f <= 64)
+    double2 p3[NLEV];
+    bool p3l[NLEV];
+#pragma unroll
+    for (int l = 0; l < NLEV; ++l) {
+        const SdeLevelNL& L = l == 0 ? lv0 : lv1;
+        p3l[l] = NV2 <= WAVE && 2 * L.gp.n3f <= WAVE;
+        p3[l] = czero();
+        if (p3l[l] && in_b(m, L.gp.n3b)) {
+            const int pc = L.chanNP > 1 ? p : 0;
+            p3[l] = L.gp.k3f ? sde_p3_lanes<true>(L.gp, pc, m, pp, nVP, n2f, lane)
+                             : sde_p3_lanes<false>(L.gp, pc, m, pp, nVP, n2f, lane);
+        }
+    }
     for (int iv = lane; iv < NV2; iv += WAVE) {
         const int v = iv - n2f;
         double2 xp = czero(), xh = czero();
@@ -2818,11 +2873,12 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
             xh = cadd(xh, cadd(cmul(cadd(k1h[l], cadd(A.k2(m, v), T.k2(m, v))), sph), k2h[l]));
             const int n3p = L.gp.n3f, n3h = max(L.ga.n3f, L.gt.n3f);
             // the K3 form is the level's (one vertex): the loops branch on it once
-            if (in_b(m, L.gp.n3b) && in_f(v, n3p)) {
-                auto psum = [&](auto form) {
+            if (p3l[l]) {
+                if (in_b(m, L.gp.n3b) && in_f(v, n3p)) xp = cadd(xp, p3[l]);
+            } else if (in_b(m, L.gp.n3b) && in_f(v, n3p)) {
+                auto psum = [&](auto form) {   // (wide boxes)
                     constexpr bool K32 = decltype(form)::value;
                     double2 sp = czero();
-#pragma unroll 8
                     for (int n = -n3p; n < n3p; ++n) {
                         const int wp = m - n - 1;
                         if (inmesh(wp)) sp = cadd(sp, cmul(P.template k3p<K32>(m, n, v), pp[wp]));
