@@ -10,7 +10,8 @@ inputs resident in HBM, replayed from the captured hipGraph of one iteration; `w
 first, then `steps` passes are timed with HIP events on the solver's stream.
 
 Both precision paths run from the same initial state: the FP64 path and the FP32-MFMA form (the BSE
-contractions on FP32 MFMA with FP32 accumulation, everything else FP64).  The max relative error of
+contractions on FP32 MFMA with FP32 accumulation, the K3 arrays of FL, F and Fbuff stored in FP32, the
+arithmetic around them and everything else FP64).  The max relative error of
 Σ, χ_sp(Ω, q), χ_ch(Ω, q) and the three K3 classes of the FP32 path against FP64 is taken after
 `--acc-iters` iterations from that state (the lattice fixed point is not solved for: the host
 Anderson loop over 1.6 GB state vectors is outside the timed path).  `value` is the FP32-MFMA path's
@@ -290,18 +291,19 @@ def tune_gemm_grid(S, grids=(0, 256, 512, 768, 1024), lk3_grids=(0, 160, 192, 22
 
 def gemm_roofline(head, P, gemm_peak, traffic, n3P):
     """Roofline object of the BSE contraction GEMMs.  FP32 path: HBM-bound (their epilogues and FP32 operands
-    move 200 B per K3 element of the three channels per iteration: FL.K3 in FP32 and the FP32 right operand
-    written by BSE_L_K3! (24 + 24), FL.a (FP32) read for its t post-processing (8); BSE_K3! reads A32, R32,
-    FL.K3 (FP32), FL.a (FP32) for the t self term and Fbuff.a for the t post-processing (24 + 24 + 24 + 8 + 16)
-    and writes Fbuff.K3 (48)); the MFMA rate is stated beside.  FP64 path: MFMA-bound (executed flops)."""
+    move 168 B per K3 element of the three channels per iteration, every K3 array in FP32: FL.K3 and the
+    FP32 right operand written by BSE_L_K3! (24 + 24), FL.a read for its t post-processing (8); BSE_K3! reads
+    A32, R32, FL.K3, FL.a for the t self term and Fbuff.a for the t post-processing (24 + 24 + 24 + 8 + 8) and
+    writes Fbuff.K3 (24)); the MFMA rate is stated beside.  FP64 path: MFMA-bound (executed flops)."""
     t = P["t_gemm"]
     mfma = P["f_exec"] / t
-    common = {"kernel": f"zgemm_tiles_t ({'FP32' if head == 'fp32' else 'FP64'} MFMA, BSE_L_K3! + BSE_K3!, "
-                        "4 launches per iteration)",
+    kern = ("k_lk3_pb64 (BSE_L_K3!) + zgemm_tiles_t (BSE_K3!), FP32 MFMA" if head == "fp32"
+            else "zgemm_tiles_t (FP64 MFMA, BSE_L_K3! + BSE_K3!)")
+    common = {"kernel": kern + ", 4 launches per iteration",
               "launches_per_iteration": 4, "ms_per_iteration": t * 1e3, "traffic": traffic,
               "executed_flops_per_iteration": P["f_exec"], "mfma_tflops": mfma / 1e12, "mfma_frac": mfma / gemm_peak}
     if head == "fp32":
-        b = 200.0 * n3P / 4
+        b = 168.0 * n3P / 4
         return dict({"bound": "hbm", "achieved": b / (t / 4) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": b / (t / 4) / HBM_PEAK, "algorithmic_bytes_per_launch": b}, **common)
     return dict({"bound": "mfma", "achieved": mfma / 1e12, "peak": gemm_peak / 1e12, "unit": "TFLOP/s",
@@ -366,7 +368,7 @@ def main_c2(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mfma contraction, f64 elsewhere" if head == "fp32" else "f64",
+        "dtype": "f32-mfma contraction and f32 K3 storage, f64 elsewhere" if head == "fp32" else "f64",
         "data": "synthetic (Hubbard model inputs, local reference system: Gaussian reference K3 seed 0, k-averaged G0)",
         "config": {"workload": "nonlocal 2D Hubbard fdPA (NL_ParquetSolver), 16x16 lattice, t=1, t'=0, U=2, beta=5, "
                                "half filling, nK1=64, nK2=(32,32), nK3=(16,32), mPinu=32, nG=256, local reference system "

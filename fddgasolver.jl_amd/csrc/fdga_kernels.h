@@ -2810,7 +2810,9 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= NW2 * NP) return;
-    const int iW = wid % NW2, p = wid / NW2;
+    // momentum fastest: the waves in flight share Ω, so the local reference vertex's K3(Ω) slab (the
+    // same at every momentum) stays in L2 while they read it
+    const int p = wid % NP, iW = wid / NP;
     const int m = iW - (n2b - 1);
     const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
     const double2* pp = Pipp + (size_t)(2 * nVP) * row + nVP;   // pp[ω], ω in [-nVP, nVP)
