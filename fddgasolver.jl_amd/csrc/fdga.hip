@@ -434,7 +434,13 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc, boo
     const int wlo = band ? (c->nVP - c->wb) / NB : 0;
     const int whi = band ? (c->nVP + c->wb + NB - 1) / NB : (int)grid.x;
     kernel_begin(c);
-    if (L == 16) {
+    if (L == 16 && d.precision == FDGA_FP32) {
+        // the FP32 form: G(R) rounded to FP32 (into tmpG2), the products and the transform in FP32
+        float2* GR32 = reinterpret_cast<float2*>(c->tmpG2);
+        k_narrow32<<<cdiv(c->nGel, 256), 256, 0, c->run>>>(GR32, c->tmpG1, c->nGel);
+        k_bubble_nl<16, 16, 512, float2><<<grid, 512, (size_t)NB * 256 * sizeof(float2), c->run>>>(
+            pp, ph, GR32, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph, wlo, whi);
+    } else if (L == 16) {
         k_bubble_nl<16, 16, 512><<<grid, 512, (size_t)NB * 256 * sizeof(double2), c->run>>>(
             pp, ph, c->tmpG1, c->folds, d.nG, L, d.nK1, c->nVP, d.T, NB, cpp, cph, wlo, whi);
     } else if (L == 24) {

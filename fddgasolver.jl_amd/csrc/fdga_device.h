@@ -36,6 +36,15 @@ __host__ __device__ __forceinline__ double2 cinv(double2 a) {
     double d = a.x * a.x + a.y * a.y;
     return cmk(a.x / d, -a.y / d);
 }
+// the same on float2 (the FP32 form of the lattice bubble)
+__host__ __device__ __forceinline__ float2 cmkf(float r, float i) { float2 z; z.x = r; z.y = i; return z; }
+__host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return cmkf(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return cmkf(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return cmkf(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return cmkf(a.x * s, a.y * s); }
+// a complex number of type CT (double2 or float2) from FP64 parts
+template <class CT>
+__host__ __device__ __forceinline__ CT cmk_as(double r, double i) { CT z; z.x = r; z.y = i; return z; }
 
 // ---------------------------------------------------------------------------------------
 // Matsubara index arithmetic; νInf absorbs

@@ -196,10 +196,13 @@ struct BubbleDft {
 };
 // length 16: radix-4 in two stages, j = 4 j1 + j2, k = k1 + 4 k2:
 //   X[k1 + 4 k2] = Σ_j2 e^{2πi j2 k2/4} [e^{2πi j2 k1/16} Σ_j1 x[4 j1 + j2] e^{2πi j1 k1/4}]
-__device__ __forceinline__ void dft4p(double2& a, double2& b, double2& c, double2& d) {
+template <class CT>
+__device__ __forceinline__ void dft4p(CT& a, CT& b, CT& c, CT& d) {
     // sign +1: X0 = a+b+c+d, X1 = a+ib-c-id, X2 = a-b+c-d, X3 = a-ib-c+id
-    const double2 s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = csub(b, d);
-    const double2 is3 = cmk(-s3.y, s3.x);
+    const CT s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = csub(b, d);
+    CT is3;
+    is3.x = -s3.y;
+    is3.y = s3.x;
     a = cadd(s0, s2);
     b = cadd(s1, is3);
     c = csub(s0, s2);
@@ -207,25 +210,31 @@ __device__ __forceinline__ void dft4p(double2& a, double2& b, double2& c, double
 }
 template <>
 struct BubbleDft<16> {
-    __device__ static void run(double2* x, const double2*, int) {
+    template <class CT>   // double2, or float2 for the FP32 form
+    __device__ static void run(CT* x, const CT*, int) {
         constexpr double C1 = 0.92387953251128675613, S1 = 0.38268343236508977173, C2 = 0.70710678118654752440;
 #pragma unroll
         for (int j2 = 0; j2 < 4; ++j2) dft4p(x[j2], x[4 + j2], x[8 + j2], x[12 + j2]);
         // x[4 k1 + j2] now holds A[j2][k1]; twiddle e^{2πi j2 k1/16}
-        const double2 w1 = cmk(C1, S1), w2 = cmk(C2, C2), w3 = cmk(S1, C1), w6 = cmk(-C2, C2), w9 = cmk(-C1, -S1);
+        const CT w1 = cmk_as<CT>(C1, S1), w2 = cmk_as<CT>(C2, C2), w3 = cmk_as<CT>(S1, C1), w6 = cmk_as<CT>(-C2, C2),
+                 w9 = cmk_as<CT>(-C1, -S1);
         x[4 + 1] = cmul(x[4 + 1], w1);
         x[4 + 2] = cmul(x[4 + 2], w2);
         x[4 + 3] = cmul(x[4 + 3], w3);
         x[8 + 1] = cmul(x[8 + 1], w2);
-        x[8 + 2] = cmk(-x[8 + 2].y, x[8 + 2].x);   // e^{iπ/2}
+        {   // e^{iπ/2}
+            const CT z = x[8 + 2];
+            x[8 + 2].x = -z.y;
+            x[8 + 2].y = z.x;
+        }
         x[8 + 3] = cmul(x[8 + 3], w6);
         x[12 + 1] = cmul(x[12 + 1], w3);
         x[12 + 2] = cmul(x[12 + 2], w6);
         x[12 + 3] = cmul(x[12 + 3], w9);
-        double2 y[16];
+        CT y[16];
 #pragma unroll
         for (int k1 = 0; k1 < 4; ++k1) {
-            double2 a = x[4 * k1], b = x[4 * k1 + 1], c = x[4 * k1 + 2], d = x[4 * k1 + 3];
+            CT a = x[4 * k1], b = x[4 * k1 + 1], c = x[4 * k1 + 2], d = x[4 * k1 + 3];
             dft4p(a, b, c, d);
             y[k1] = a; y[k1 + 4] = b; y[k1 + 8] = c; y[k1 + 12] = d;
         }
@@ -300,10 +309,11 @@ struct BubbleDft<24> {
 };
 
 // Π_R(ω, Ω, R') for one R' (pp: which = 0, ph: 1; see k_bubble_nl)
-__device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, const BubbleFold& f, int nG, double T, int m,
-                                            int n, int which) {
+template <class CT>   // double2, or float2 (the FP32 form: G(R) in FP32, products in FP32)
+__device__ __forceinline__ CT bubble_R(const CT* __restrict__ GR, const BubbleFold& f, int nG, double T, int m, int n,
+                                       int which) {
     const int NG = 2 * nG;
-    double2 acc = czero();
+    CT acc = cmk_as<CT>(0.0, 0.0);
     const int n2nd = which == 0 ? bmf(m, n) : bpf(m, n);
     const int terms[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
@@ -311,11 +321,11 @@ __device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, cons
         const int t = terms[c], wc = fold_wcode(t);
         if (wc == 0) break;
         const bool origin = fold_origin(t);
-        auto gat = [&](int k, int kk) -> double2 {
+        auto gat = [&](int k, int kk) -> CT {
             if (in_f(k, nG)) return GR[(size_t)(k + nG) + (size_t)NG * kk];
-            return origin ? cmk(1.0 / (M_PI * T * (2.0 * k + 1.0)), 0.0) : czero();
+            return cmk_as<CT>(origin ? 1.0 / (M_PI * T * (2.0 * k + 1.0)) : 0.0, 0.0);
         };
-        const double2 x = cmul(gat(n2nd, fold_kp(t)), gat(n, which == 0 ? fold_kp(t) : fold_km(t)));
+        const CT x = cmul(gat(n2nd, fold_kp(t)), gat(n, which == 0 ? fold_kp(t) : fold_km(t)));
         acc = cadd(acc, wc == 1 ? x : cscale(x, wc == 2 ? 0.5 : 0.25));
     }
     return acc;
@@ -328,16 +338,19 @@ __device__ __forceinline__ double2 bubble_R(const double2* __restrict__ GR, cons
 // L <= 32 (direct DFT through private arrays, for small test lattices).  Block: NT threads (512 at
 // L = 16, held to 128 VGPRs: two blocks per CU, twice the waves on the same LDS tiles, to hide the
 // latency of the G(R) gathers).
-template <int LT, int NBT, int NT = 256>
-__global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __restrict__ pp, double2* __restrict__ ph,
-                                                   const double2* __restrict__ GR, const BubbleFold* __restrict__ folds,
+// CT: double2, or float2 for the FP32 form (G(R), the products, the transform and the LDS tiles in
+// FP32; Π stored and the chunk sums accumulated in FP64): 32 KB of LDS per block and at most 80 VGPRs,
+// three blocks per CU.
+template <int LT, int NBT, int NT = 256, class CT = double2>
+__global__ __launch_bounds__(NT, NT >= 512 ? (sizeof(CT) == 8 ? 6 : 4) : 1) void k_bubble_nl(double2* __restrict__ pp, double2* __restrict__ ph,
+                                                   const CT* __restrict__ GR, const BubbleFold* __restrict__ folds,
                                                    int nG, int Lr, int nW, int nV, double T, int NBr,
                                                    double2* __restrict__ cpp, double2* __restrict__ cph, int wlo,
                                                    int whi) {
-    extern __shared__ double2 sm[];   // NB * L * L (+ L twiddles unless L = 16)
+    extern __shared__ double2 sm[];   // NB * L * L (+ L twiddles unless L = 16) of CT
     const int L = LT ? LT : Lr, LL = L * L, NB = NBT ? NBT : NBr;
-    double2* s = sm;
-    double2* tw = sm + NB * LL;
+    CT* s = reinterpret_cast<CT*>(sm);
+    CT* tw = s + NB * LL;
     const int tid = threadIdx.x;
     const int iw0 = blockIdx.x * NB, iW = blockIdx.y;
     const int m = iW - (nW - 1);
@@ -346,7 +359,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __
         for (int k = tid; k < L; k += blockDim.x) {
             double sn, cs;
             sincospi(2.0 * k / L, &sn, &cs);
-            tw[k] = cmk(cs, sn);
+            tw[k] = cmk_as<CT>(cs, sn);
         }
     for (int which = 0; which < 2; ++which) {
         // products (a compile-time trip count when L and NB are fixed, so that the loads of
@@ -358,13 +371,13 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __
                 const int e = tid + NT * q;
                 if (e < NB * LL) {
                     const int wl = e % NB, R = e / NB;
-                    s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : czero();
+                    s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : cmk_as<CT>(0.0, 0.0);
                 }
             }
         } else {
             for (int e = tid; e < NB * LL; e += blockDim.x) {
                 const int wl = e % NB, R = e / NB;
-                s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : czero();
+                s[e] = wl < nb ? bubble_R(GR, folds[R], nG, T, m, iw0 + wl - nV, which) : cmk_as<CT>(0.0, 0.0);
             }
         }
         __syncthreads();
@@ -373,13 +386,13 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __
             for (int e = tid; e < NB * L; e += blockDim.x) {
                 const int wl = e % NB, r = e / NB;
                 const int base = pass == 0 ? L * r : r, step = pass == 0 ? 1 : L;
-                double2 x[LT ? LT : 32];
+                CT x[LT ? LT : 32];
                 for (int j = 0; j < L; ++j) x[j] = s[(base + step * j) * NB + wl];
                 if (LT) BubbleDft<LT ? LT : 1>::run(x, tw, L);
                 else {
-                    double2 y[32];
+                    CT y[32];
                     for (int k = 0; k < L; ++k) {
-                        double2 acc = czero();
+                        CT acc = cmk_as<CT>(0.0, 0.0);
                         for (int j = 0; j < L; ++j) acc = cadd(acc, cmul(x[j], tw[(j * k) % L]));
                         y[k] = acc;
                     }
@@ -394,7 +407,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __
                         double2* out = which == 0 ? pp : ph;
                         const size_t o = (size_t)(iw0 + wl) + (size_t)(2 * nV) * (size_t)iW;
                         const size_t qs = (size_t)(2 * nV) * (size_t)(2 * nW - 1);
-                        for (int k = 0; k < L; ++k) out[o + qs * (size_t)(r + L * k)] = x[k];
+                        for (int k = 0; k < L; ++k) out[o + qs * (size_t)(r + L * k)] = cmk(x[k].x, x[k].y);
                     }
                 }
             }
@@ -409,7 +422,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 4 : 1) void k_bubble_nl(double2* __
                 double sr = 0.0, si = 0.0;
                 for (int j = 0; j < NB; ++j) {
                     const int wl = (j + P) % NB;
-                    if (wl < nb) { const double2 v = s[P * NB + wl]; sr += v.x; si += v.y; }
+                    if (wl < nb) { const CT v = s[P * NB + wl]; sr += v.x; si += v.y; }
                 }
                 cs[(size_t)P + (size_t)LL * (blockIdx.x + (size_t)gridDim.x * iW)] = cmk(sr, si);
             }
