@@ -11,18 +11,20 @@ first, then `steps` passes are timed with HIP events on the solver's stream.
 
 Both precision paths run from the same initial state: the FP64 path and the FP32-MFMA form (the BSE
 contractions on FP32 MFMA with FP32 accumulation, the K3 arrays of FL, F and Fbuff stored in FP32, the
-arithmetic around them and everything else FP64).  The max relative error of
+lattice bubble's products and momentum transform in FP32; Dyson, the Π tail sums, the SDE and
+everything else FP64).  The max relative error of
 Σ, χ_sp(Ω, q), χ_ch(Ω, q) and the three K3 classes of the FP32 path against FP64 is taken after
 `--acc-iters` iterations from that state (the lattice fixed point is not solved for: the host
 Anderson loop over 1.6 GB state vectors is outside the timed path).  `value` is the FP32-MFMA path's
 iterations/s when every error is <= 1e-6, else the FP64 path's (`config.value_path`).
 
-`roofline` is the lattice bubble kernel (k_bubble_nl, bubbles_real_space!, the largest single kernel of
-the iteration): algorithmic bytes per launch B_b = 16 (2 NΩΠ NνΠ NP + NνG NkG) (both bubbles written
-once, G read once) over its launch time measured with HIP events in the per-stage pass; `traffic` is
-its FETCH_SIZE + WRITE_SIZE per launch from rocprofv3 (profiles/c2_bubble_nl_traffic.txt).
-`roofline_bse_gemm` is the BSE contraction (BSE_L_K3! + BSE_K3!, batched complex GEMM on MFMA):
-executed flops 8 BM BN Nν3 per computed tile over the two launches' times.  `cpu_baseline` times the CPU
+`roofline` is the BSE contraction, the iteration's largest kernel family (BSE_L_K3! + BSE_K3!, four
+launches per iteration): on the FP32 path HBM-bound, algorithmic 168 B per K3 element of the three
+channels per iteration (gemm_roofline) over the launches' HIP-event time, with `traffic` the rocprofv3
+FETCH + WRITE bytes per launch (profiles/c2_bse_gemm_traffic.txt); on the FP64 path MFMA-bound
+(executed flops 8 BM BN Nν3 per computed tile).  `roofline_bubble` is the lattice bubble kernel
+(k_bubble_nl, bubbles_real_space!): G(R) read once, Πpp and Πph written on the band chunks and the
+chunk sums, over its HIP-event time; `traffic` from profiles/c2_bubble_nl_traffic.txt.  `cpu_baseline` times the CPU
 restatement of the reference loops (oracle/) on a stated bounded sample of the same workload, at one
 thread and at all cores.
 
@@ -350,10 +352,12 @@ def main_c2(args):
         + ", ".join(f"{k}={v:.2e}" for k, v in errs.items()))
     head = "fp32" if max(errs.values()) <= 1e-6 else "fp64"
     P = paths[head]
-    # k_bubble_nl as built: G(R) read once, Πpp and Πph stored on the band chunks, the chunk sums of
-    # all 16-ω chunks written (the far regions enter the later stages only through those sums)
+    # k_bubble_nl as built: G(R) read once (FP32 on the FP32 path), Πpp and Πph stored on the band
+    # chunks, the chunk sums of all 16-ω chunks written (the far regions enter the later stages only
+    # through those sums)
     wb, nchunk = c2_band(w)
-    b_bubble = 16.0 * (2 * w["nG"] * NP + 2 * nchunk * 16 * NWP * NP + 2 * (NVP // 16) * NWP * NP)
+    b_bubble = ((8.0 if head == "fp32" else 16.0) * 2 * w["nG"] * NP
+                + 16.0 * (2 * nchunk * 16 * NWP * NP + 2 * (NVP // 16) * NWP * NP))
     t_bub = P["st"]["bubble_kernel"] * 1e-3
     gemm_peak = FP32_MFMA_PEAK if head == "fp32" else FP64_MFMA_PEAK
     gemm_traffic = recorded_traffic("c2_bse_gemm_traffic.txt", f"{head}_hbm_bytes_per_launch")
@@ -368,7 +372,8 @@ def main_c2(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mfma contraction and f32 K3 storage, f64 elsewhere" if head == "fp32" else "f64",
+        "dtype": ("f32-mfma contractions, f32 K3 storage and f32 lattice-bubble interior, f64 elsewhere"
+                  if head == "fp32" else "f64"),
         "data": "synthetic (Hubbard model inputs, local reference system: Gaussian reference K3 seed 0, k-averaged G0)",
         "config": {"workload": "nonlocal 2D Hubbard fdPA (NL_ParquetSolver), 16x16 lattice, t=1, t'=0, U=2, beta=5, "
                                "half filling, nK1=64, nK2=(32,32), nK3=(16,32), mPinu=32, nG=256, local reference system "
