@@ -603,11 +603,29 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
         const SdeLevelNL l0{lv.g[0], lv.g[1], lv.g[2], lv.chanNP};
         if (lv2) {
             const SdeLevelNL l1{lv2->g[0], lv2->g[1], lv2->g[2], lv2->chanNP};
+            // a momentum-independent second level (the local reference vertex): its K3 terms as GEMMs
+            // over the momenta after the pass
+            const bool refgemm = lv2->chanNP == 1 && c->NP > 1;
             k_sde_L_nl<2><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f, d.nK1, c->nVP,
-                                                  c->wb, c->NP, TU);
+                                                  c->wb, c->NP, TU, refgemm ? 1 : 0);
+            LAUNCHCHK(c);
+            if (refgemm) {
+                const ChanV& gp = lv2->g[PCH];
+                const int NV = 2 * gp.n3f, NW = 2 * gp.n3b - 1;
+                const SdeRefA<0> ap{gp.K3, nullptr, NV};
+                const SdeRefA<1> ah{lv2->g[ACH].K3, lv2->g[TCH].K3, NV};
+                const SdeRefB<0> bp{Pp, gp.n3b, gp.n3f, d.nK1, c->nVP, 0};
+                const SdeRefB<1> bh{Ph, gp.n3b, gp.n3f, d.nK1, c->nVP, 0};
+                const SdeRefE ep{c->Lpp, TU, d.nK2b, d.nK2f, gp.n3b, gp.n3f, 0};
+                const SdeRefE eh{c->Lph, TU, d.nK2b, d.nK2f, gp.n3b, gp.n3f, 0};
+                const int tm = cdiv(NV, GBM), tn = cdiv(c->NP, GBN);
+                if (zgemm_launch(FDGA_FP64, NW, c->run, NV, c->NP, NV, tm, tn, nullptr, 0, ap, bp, ep) != hipSuccess ||
+                    zgemm_launch(FDGA_FP64, NW, c->run, NV, c->NP, NV, tm, tn, nullptr, 0, ah, bh, eh) != hipSuccess)
+                    return fail(c, "SDE reference-level GEMM launch failed", -3);
+            }
         } else {
             k_sde_L_nl<1><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l0, d.nK2b, d.nK2f, d.nK1, c->nVP,
-                                                  c->wb, c->NP, TU);
+                                                  c->wb, c->NP, TU, 0);
         }
         LAUNCHCHK(c);
         // SGpp2 / SGph2 (SG0pp2 / SG0ph2 for the reference system): members from representatives

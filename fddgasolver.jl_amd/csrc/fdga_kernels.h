@@ -2812,14 +2812,61 @@ __device__ __forceinline__ double2 sde_p3_lanes(const ChanV& g, int pc, int m, c
     return mine;
 }
 
+// The K3 terms of a momentum-independent Vertex level (the local reference vertex) in the lattice SDE
+// L-vertices, as FP64 GEMMs per Ω of its K3 box over all momenta at once (n, ν in that box, the Π
+// rows of (Ω, P)):
+//   Lpp(Ω, ν, P) += T U Σ_n K3p(Ω, n, ν) Πpp(Ω, Ω - n - 1, P)     (SdeRefA<0>, SdeRefB<0>)
+//   Lph(Ω, ν, P) += T U Σ_n [K3a + K3t](Ω, ν, n) Πph(Ω, n, P)      (SdeRefA<1>, SdeRefB<1>)
+// The level's K3 slab is read once per Ω instead of once per (Ω, P).
+template <int PH>
+struct SdeRefA {   // A(i = ν, k = n) of one Ω of the box
+    const double2* K0; const double2* K1; int NV;
+    __device__ SdeRefA bind(int b) const {
+        SdeRefA r = *this;
+        r.K0 += (size_t)NV * NV * b;
+        if (PH) r.K1 += (size_t)NV * NV * b;
+        return r;
+    }
+    __device__ double2 operator()(int i, int k) const {
+        if (PH) { const size_t o = (size_t)i + (size_t)NV * k; return cadd(K0[o], K1[o]); }   // K3a + K3t
+        return K0[(size_t)k + (size_t)NV * i];                                                // K3p(n, ν)
+    }
+};
+template <int PH>
+struct SdeRefB {   // B(k = n, j = P): the Π row of (Ω, P) at ω = Ω - n - 1 (pp) or n (ph), 0 off the mesh
+    const double2* Pi; int n3b, n3f, nWP, nVP, m;
+    __device__ SdeRefB bind(int b) const {
+        SdeRefB r = *this;
+        r.m = b - (n3b - 1);
+        r.Pi += (size_t)(2 * nVP) * (size_t)(r.m + nWP - 1) + nVP;
+        return r;
+    }
+    __device__ double2 operator()(int k, int j) const {
+        const int n = k - n3f, w = PH ? n : m - n - 1;
+        if (w < -nVP || w >= nVP) return czero();
+        return Pi[(size_t)(2 * nVP) * (size_t)(2 * nWP - 1) * j + w];
+    }
+};
+struct SdeRefE {   // L(Ω, ν, P) += T U C(ν, P) inside the L box
+    double2* L; double2 TU; int n2b, n2f, n3b, n3f, m;
+    __device__ SdeRefE bind(int b) const { SdeRefE r = *this; r.m = b - (n3b - 1); return r; }
+    __device__ void operator()(int i, int j, double2 v) const {
+        const int nu = i - n3f;
+        if (m <= -n2b || m >= n2b || nu < -n2f || nu >= n2f) return;
+        const size_t o = (size_t)(m + n2b - 1) + (size_t)(2 * n2b - 1) * ((size_t)(nu + n2f) + (size_t)(2 * n2f) * j);
+        L[o] = cadd(L[o], cmul(v, TU));
+    }
+};
+
 // NLEV Vertex levels summed into one pair of L-vertices (the target vertex and the local reference
-// vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch)
+// vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch).  skip1: the second level's
+// K3 terms are left out (added by the momentum GEMMs SdeRef*, when that level is momentum-independent)
 template <int NLEV>
 __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, double2* __restrict__ Lph,
                                                   const double2* __restrict__ Pipp, const double2* __restrict__ Piph,
                                                   const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
                                                   SdeLevelNL lv0, SdeLevelNL lv1, int n2b, int n2f, int nWP, int nVP,
-                                                  int wb, int NP, double2 TU) {
+                                                  int wb, int NP, double2 TU, int skip1) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= NW2 * NP) return;
@@ -2870,7 +2917,7 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
         const SdeLevelNL& L = l == 0 ? lv0 : lv1;
         p3l[l] = NV2 <= WAVE && 2 * L.gp.n3f <= WAVE;
         p3[l] = czero();
-        if (p3l[l] && in_b(m, L.gp.n3b)) {
+        if (p3l[l] && in_b(m, L.gp.n3b) && !(l == 1 && skip1)) {
             const int pc = L.chanNP > 1 ? p : 0;
             p3[l] = L.gp.k3f ? sde_p3_lanes<true>(L.gp, pc, m, pp, nVP, n2f, lane)
                              : sde_p3_lanes<false>(L.gp, pc, m, pp, nVP, n2f, lane);
@@ -2887,6 +2934,7 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
             xp = cadd(xp, cadd(cmul(cadd(k1p[l], P.k2(m, v)), spp), k2p[l]));
             xh = cadd(xh, cadd(cmul(cadd(k1h[l], cadd(A.k2(m, v), T.k2(m, v))), sph), k2h[l]));
             const int n3p = L.gp.n3f, n3h = max(L.ga.n3f, L.gt.n3f);
+            if (l == 1 && skip1) continue;
             // the K3 form is the level's (one vertex): the loops branch on it once
             if (p3l[l]) {
                 if (in_b(m, L.gp.n3b) && in_f(v, n3p)) xp = cadd(xp, p3[l]);
