@@ -534,8 +534,8 @@ int sde_sigma(fdga_ctx* c, double2* S, const double2* G, double scale) {
     size_t nrow = (size_t)c->NW2 * c->NV2;
     if (dft2(c, c->tmpL1, nrow, d.L, -1, 1.0 / ((double)d.L * d.L))) return -3;
     if (dft2(c, c->tmpL2, nrow, d.L, -1, 1.0 / ((double)d.L * d.L))) return -3;
-    dim3 g(cdiv(c->NG, 128), c->NkG);
-    k_sde_sigma_nl_R<<<g, 128, 0, c->run>>>(c->tmpG2, c->tmpG1, c->tmpL1, c->tmpL2, d.nG, d.nK2b, d.nK2f, d.LG, d.L, d.T);
+    k_sde_sigma_nl_R<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->tmpG2, c->tmpG1, c->tmpL1, c->tmpL2,
+                                                                                 d.nG, d.nK2b, d.nK2f, d.LG, d.L, d.T);
     LAUNCHCHK(c);
     if (dft2(c, c->tmpG2, c->NG, d.LG, +1, 1.0)) return -3;
     if (sym_fill(c, c->tmpG2, c->sg[SG_S].repop, c->nGel)) return -3;   // SGΣ(Σ) (src/nonlocal/SDE.jl:280)

@@ -3008,21 +3008,35 @@ __global__ __launch_bounds__(256) void k_sde_sigma_local(double2* __restrict__ S
 // NL SDE_compute_inner! real-space step (src/nonlocal/SDE.jl:209-276): for output column q of
 // Σ_R[ν, q] (q on the LG mesh) collect Σ_R( R) += G(-R) Lpp(R) and Σ_R(-R) += G(-R) Lph(R)
 // over R in [-L/2, L/2]^2 (half weights at |R_i| = L/2 for even L); the result is T Σ_R.
-__global__ void k_sde_sigma_nl_R(double2* __restrict__ SR, const double2* __restrict__ GR, const double2* __restrict__ LppR,
-                                 const double2* __restrict__ LphR, int nG, int n2b, int n2f, int LG, int L, double T) {
+// One wave per (ν, q), the lanes over Ω (the L-vertices' Ω index is their fastest: coalesced rows)
+// and a wave sum; ν outside the K2 box write zero.
+__global__ __launch_bounds__(256) void k_sde_sigma_nl_R(double2* __restrict__ SR, const double2* __restrict__ GR,
+                                                        const double2* __restrict__ LppR, const double2* __restrict__ LphR,
+                                                        int nG, int n2b, int n2f, int LG, int L, double T) {
     const int NG = 2 * nG;
-    int iv = blockIdx.x * blockDim.x + threadIdx.x;
-    int q = blockIdx.y;
-    if (iv >= NG) return;
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid >= NG * LG * LG) return;
+    const int iv = wid % NG, q = wid / NG;
     const int v = iv - nG;
-    double2 s = czero();
+    double sr = 0.0, si = 0.0;
     if (in_f(v, n2f)) {
         const int q1 = q % LG, q2 = q / LG, h = L / 2;
         const int NW2 = 2 * n2b - 1;
         const size_t nrow = (size_t)NW2 * (2 * n2f);
         auto md = [](int a, int n) { return ((a % n) + n) % n; };
-        for (int R2 = -h; R2 <= h; ++R2)
-            for (int R1 = -h; R1 <= h; ++R1) {
+        // the R_i in [-h, h] that fold onto q_i or -q_i, in ascending order (at most four per axis when
+        // L <= LG; otherwise every R_i is visited)
+        int ra[4], rb[4], na = 0, nb = 0;
+        const bool lists = L <= LG;
+        if (lists)
+            for (int R = -h; R <= h; ++R) {
+                if ((md(R, LG) == q1 || md(-R, LG) == q1) && na < 4) ra[na++] = R;
+                if ((md(R, LG) == q2 || md(-R, LG) == q2) && nb < 4) rb[nb++] = R;
+            }
+        const int n1 = lists ? na : 2 * h + 1, n2 = lists ? nb : 2 * h + 1;
+        for (int i2 = 0; i2 < n2; ++i2)
+            for (int i1 = 0; i1 < n1; ++i1) {
+                const int R1 = lists ? ra[i1] : i1 - h, R2 = lists ? rb[i2] : i2 - h;
                 const bool tpp = md(R1, LG) == q1 && md(R2, LG) == q2;
                 const bool tph = md(-R1, LG) == q1 && md(-R2, LG) == q2;
                 if (!tpp && !tph) continue;
@@ -3033,9 +3047,10 @@ __global__ void k_sde_sigma_nl_R(double2* __restrict__ SR, const double2* __rest
                 }
                 const size_t iRL = (size_t)(md(R1, L) + L * md(R2, L));
                 const size_t imRG = (size_t)(md(-R1, LG) + LG * md(-R2, LG));
-                for (int iW = 0; iW < NW2; ++iW) {
+                for (int iW = lane; iW < NW2; iW += WAVE) {
                     const int m = iW - (n2b - 1);
                     const size_t k = (size_t)iW + (size_t)NW2 * (v + n2f) + nrow * iRL;
+                    double2 s = czero();
                     if (tpp) {
                         const int a = bmf(m, v);
                         if (in_f(a, nG)) s = cadd(s, cscale(cmul(GR[(size_t)(a + nG) + (size_t)NG * imRG], LppR[k]), w));
@@ -3044,10 +3059,14 @@ __global__ void k_sde_sigma_nl_R(double2* __restrict__ SR, const double2* __rest
                         const int b = bpf(m, v);
                         if (in_f(b, nG)) s = cadd(s, cscale(cmul(GR[(size_t)(b + nG) + (size_t)NG * imRG], LphR[k]), w));
                     }
+                    sr += s.x;
+                    si += s.y;
                 }
             }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
     }
-    SR[(size_t)iv + (size_t)NG * q] = cscale(s, T);
+    if (lane == 0) SR[(size_t)iv + (size_t)NG * q] = cscale(cmk(sr, si), T);
 }
 
 // SDE_U2_using_G (src/SDE.jl:328-364) via C(m) = Σ_n G(n) G(n+m):
