@@ -561,9 +561,14 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G, bool c_ready = false) {
     HIPCHK(c, hipMemcpyAsync(c->tmpG2, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
     if (dft2(c, c->tmpG1, c->NG, LG, -1, 1.0 / ((double)LG * LG))) return -3;   // G(+R)
     if (dft2(c, c->tmpG2, c->NG, LG, +1, 1.0 / ((double)LG * LG))) return -3;   // G(-R)
-    k_u2_C<<<cdiv((size_t)NB * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
-    k_u2_S<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac, 0,
-                                                                         SymCls{nullptr, nullptr, 0});
+    // four outputs per wave (NG = 2 nG is even; the last C wave may hold fewer than four m)
+    k_u2_C4<<<cdiv((size_t)cdiv(NB, 4) * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
+    if (c->NG % 4 == 0)
+        k_u2_S4<<<cdiv((size_t)(c->NG / 4) * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG,
+                                                                                 fac);
+    else
+        k_u2_S<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->tmpG3, c->u2C, c->tmpG1, d.nG, c->NkG, fac,
+                                                                             0, SymCls{nullptr, nullptr, 0});
     LAUNCHCHK(c);
     if (dft2(c, c->tmpG3, c->NG, LG, +1, 1.0)) return -3;
     if (sym_fill(c, c->tmpG3, c->sg[SG_S].repop, c->nGel)) return -3;   // SGΣ(Σ_U²) (src/nonlocal/SDE.jl:444)

@@ -3093,6 +3093,33 @@ __global__ __launch_bounds__(256) void k_u2_C(double2* __restrict__ C, const dou
                                               const double2* __restrict__ Gp, int nG, int nR) {
     u2_C_wave((blockIdx.x * blockDim.x + threadIdx.x) / WAVE, threadIdx.x & (WAVE - 1), C, Gm, Gp, nG, nR);
 }
+// The lattice solver's C(m) for every R column (nR = NkG): each wave four consecutive m of one column
+// (Gp loaded once for the four; every output's lane partial sums and wave sum as in u2_C_wave)
+__global__ __launch_bounds__(256) void k_u2_C4(double2* __restrict__ C, const double2* __restrict__ Gm,
+                                               const double2* __restrict__ Gp, int nG, int nR) {
+    const int NG = 2 * nG, NB = 4 * nG - 1, NB4 = (NB + 3) / 4;
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid >= NB4 * nR) return;
+    const int im0 = 4 * (wid % NB4), r = wid / NB4;
+    double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i2 = lane; i2 < NG; i2 += WAVE) {
+        const int n2 = i2 - nG;
+        const double2 gp = Gp[(size_t)i2 + (size_t)NG * r];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int m = im0 + u - (2 * nG - 1), n1 = n2 + m;
+            if (im0 + u < NB && in_f(n1, nG)) {
+                const double2 x = cmul(Gm[(size_t)(n1 + nG) + (size_t)NG * r], gp);
+                sr[u] += x.x; si[u] += x.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const double a = wave_sum(sr[u]), b = wave_sum(si[u]);
+        if (lane == 0 && im0 + u < NB) C[(size_t)(im0 + u) + (size_t)NB * r] = cmk(a, b);
+    }
+}
 // local solver, first SDE level: the Σ sum over the L-vertices (waves [0, nsig)) and the U² term's
 // C(m) (the waves after them) in one launch; they read different arrays and write S and C
 __global__ __launch_bounds__(256) void k_sigma_u2c(double2* __restrict__ S, const double2* __restrict__ G,
@@ -3128,6 +3155,34 @@ __global__ __launch_bounds__(256) void k_u2_S(double2* __restrict__ S, const dou
         const int code = sym.member(wid, k), o = code >> 2;
         const double2 s = sym_apply(code & 3, raw);
         S[o] = accumulate ? cadd(S[o], s) : s;
+    }
+}
+
+// The lattice solver's Σ_U² for every R column (no symmetry classes): each wave four consecutive ν of
+// one column (C loaded once for the four; every output's sums as in k_u2_S)
+__global__ __launch_bounds__(256) void k_u2_S4(double2* __restrict__ S, const double2* __restrict__ C,
+                                               const double2* __restrict__ Gp, int nG, int nR, double2 fac) {
+    const int NG = 2 * nG, NB = 4 * nG - 1, NG4 = NG / 4;
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (wid >= NG4 * nR) return;
+    const int iv0 = 4 * (wid % NG4), r = wid / NG4;
+    double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int im = lane; im < NB; im += WAVE) {
+        const int m = im - (2 * nG - 1);
+        const double2 cm = C[(size_t)im + (size_t)NB * r];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int x = m + iv0 + u - nG;
+            if (in_f(x, nG)) {
+                const double2 y = cmul(cm, Gp[(size_t)(x + nG) + (size_t)NG * r]);
+                sr[u] += y.x; si[u] += y.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const double a = wave_sum(sr[u]), b = wave_sum(si[u]);
+        if (lane == 0) S[(size_t)(iv0 + u) + (size_t)NG * r] = cmul(cmk(a, b), fac);
     }
 }
 
