@@ -611,8 +611,16 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
             // a momentum-independent second level (the local reference vertex): its K3 terms as GEMMs
             // over the momenta after the pass
             const bool refgemm = lv2->chanNP == 1 && c->NP > 1;
-            k_sde_L_nl<2><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f, d.nK1, c->nVP,
-                                                  c->wb, c->NP, TU, refgemm ? 1 : 0);
+            // (the target level's K3 form fixed at compile time: one load form in the loops)
+            if (refgemm && l0.gp.k3f && l0.ga.k3f && l0.gt.k3f)
+                k_sde_L_nl<2, true, 1><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f,
+                                                               d.nK1, c->nVP, c->wb, c->NP, TU);
+            else if (refgemm && !l0.gp.k3f && !l0.ga.k3f && !l0.gt.k3f)
+                k_sde_L_nl<2, true, 0><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f,
+                                                               d.nK1, c->nVP, c->wb, c->NP, TU);
+            else
+                k_sde_L_nl<2><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f, d.nK1,
+                                                      c->nVP, c->wb, c->NP, TU);
             LAUNCHCHK(c);
             if (refgemm) {
                 const ChanV& gp = lv2->g[PCH];
@@ -630,7 +638,7 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
             }
         } else {
             k_sde_L_nl<1><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l0, d.nK2b, d.nK2f, d.nK1, c->nVP,
-                                                  c->wb, c->NP, TU, 0);
+                                                  c->wb, c->NP, TU);
         }
         LAUNCHCHK(c);
         // SGpp2 / SGph2 (SG0pp2 / SG0ph2 for the reference system): members from representatives
@@ -1233,8 +1241,14 @@ static int k3_rhs(fdga_ctx* c, const K3Set& ks, int n, bool sym = false, bool sk
     if (ks.op32()) {
         k_k2_rows<<<cdiv((size_t)c->NV3 * c->NW3 * c->NP * n, 256), 256, 0, c->run>>>(ks.g, ks.kc[0], ks.kc[1], ks.kc[2], n);
         LAUNCHCHK(c);
-        k_k3_A32<<<dim3((unsigned)(c->NW3 * cdiv(npl, K3_PCH) * n), cdiv((size_t)c->NV3 * c->NV3, 256)), 256, 0, c->run>>>(
-            ks.g, ks.kc[0], ks.kc[1], ks.kc[2], pl, npl);
+        // the three slots (p, a, t) in one pass when the launch holds them in that order
+        const K3Chan* kc = ks.kc;
+        const bool pat = n == 3 && kc[0].ch == PCH && kc[1].ch == ACH && kc[2].ch == TCH && kc[0].k32 == kc[1].k32 &&
+                         kc[1].k32 == kc[2].k32;
+        const dim3 grid((unsigned)(c->NW3 * cdiv(npl, K3_PCH) * (pat ? 1 : n)), cdiv((size_t)c->NV3 * c->NV3, 256));
+        if (pat && kc[0].k32) k_k3_A32_pat<true><<<grid, 256, 0, c->run>>>(ks.g, kc[0], kc[1], kc[2], pl, npl);
+        else if (pat) k_k3_A32_pat<false><<<grid, 256, 0, c->run>>>(ks.g, kc[0], kc[1], kc[2], pl, npl);
+        else k_k3_A32<<<grid, 256, 0, c->run>>>(ks.g, kc[0], kc[1], kc[2], pl, npl);
         LAUNCHCHK(c);
     }
     return 0;

@@ -1605,6 +1605,59 @@ __global__ __launch_bounds__(256) void k_k3_A32(K3Geom g, K3Chan c0, K3Chan c1, 
         }
     }
 }
+// The same for the three slots (p, a, t) of one launch in one pass: a thread builds the element of
+// all three, so the a channel's K3 is read once for its own A and for t's 2 K3t - K3a (each element
+// computed exactly as above).  K32: the vertex K3 held in FP32.
+template <bool K32>
+__global__ __launch_bounds__(256) void k_k3_A32_pat(K3Geom g, K3Chan cp, K3Chan ca, K3Chan ct, const int* __restrict__ pl,
+                                                    int npl) {
+    typedef typename std::conditional<K32, float2, double2>::type KT;
+    const int NV = g.NV(), NW = g.NW();
+    const int nn = NV * NV;
+    const int o = blockIdx.y * 256 + threadIdx.x;
+    if (o >= nn) return;
+    const int iW = (int)blockIdx.x % NW, p0 = ((int)blockIdx.x / NW) * K3_PCH;
+    const size_t fo = (size_t)o + (size_t)nn * iW;
+    const double2 xp = cp.Fbase[fo], xa = ca.Fbase[fo], xt = ct.Fbase[fo];
+    const KT* kp = reinterpret_cast<const KT*>(cp.K3r);
+    const KT* ka = reinterpret_cast<const KT*>(ca.K3r);
+    const KT* kt = reinterpret_cast<const KT*>(ct.K3r);
+    const int pend = min(p0 + K3_PCH, npl);
+    const int i = o % NV;   // the element's row ν = i - n3f
+    for (int pq = p0; pq < pend; pq += 2) {
+        KT lp[2], la[2], lt[2];
+        double2 rp[2], ra[2], rt[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (pq + u < pend) {
+                const int p = pl ? pl[pq + u] : pq + u;
+                const size_t r = (size_t)o + (size_t)nn * (iW + (size_t)NW * p);
+                lp[u] = kp[r];
+                la[u] = ka[r];
+                lt[u] = kt[r];
+                const size_t q = (size_t)i + (size_t)NV * (iW + (size_t)NW * p);
+                rp[u] = cp.K2row ? cp.K2row[q] : czero();
+                ra[u] = ca.K2row ? ca.K2row[q] : czero();
+                rt[u] = ct.K2row ? ct.K2row[q] : czero();
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (pq + u < pend) {
+                const int p = pl ? pl[pq + u] : pq + u;
+                const size_t w = (size_t)o + (size_t)nn * (iW + (size_t)NW * p);
+                const double2 vp0 = cmk(lp[u].x, lp[u].y), va0 = cmk(la[u].x, la[u].y), vt0 = cmk(lt[u].x, lt[u].y);
+                double2 vp = cadd(xp, vp0), va = cadd(xa, va0), vt = cadd(xt, csub(cscale(vt0, 2.0), va0));
+                if (cp.K2row) vp = cadd(vp, rp[u]);
+                if (ca.K2row) va = cadd(va, ra[u]);
+                if (ct.K2row) vt = cadd(vt, rt[u]);
+                cp.A32[w] = make_float2((float)vp.x, (float)vp.y);
+                ca.A32[w] = make_float2((float)va.x, (float)va.y);
+                ct.A32[w] = make_float2((float)vt.x, (float)vt.y);
+            }
+        }
+    }
+}
 // K2row of the FP32 lattice A operand (K3Chan::K2row) for every channel slot: one thread per
 // (ν, Ω, P) of the K3 box, coalesced writes along ν
 __global__ __launch_bounds__(256) void k_k2_rows(K3Geom g, K3Chan c0, K3Chan c1, K3Chan c2, int nchan) {
@@ -2859,14 +2912,15 @@ struct SdeRefE {   // L(Ω, ν, P) += T U C(ν, P) inside the L box
 };
 
 // NLEV Vertex levels summed into one pair of L-vertices (the target vertex and the local reference
-// vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch).  skip1: the second level's
-// K3 terms are left out (added by the momentum GEMMs SdeRef*, when that level is momentum-independent)
-template <int NLEV>
+// vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch).  SKIP1: the second level's
+// K3 terms are left out (added by the momentum GEMMs SdeRef*, when that level is momentum-independent).
+// F0: the first level's K3 form fixed at compile time (0 FP64, 1 FP32; -1: read from the level)
+template <int NLEV, bool SKIP1 = false, int F0 = -1>
 __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, double2* __restrict__ Lph,
                                                   const double2* __restrict__ Pipp, const double2* __restrict__ Piph,
                                                   const double2* __restrict__ Tpp, const double2* __restrict__ Tph,
                                                   SdeLevelNL lv0, SdeLevelNL lv1, int n2b, int n2f, int nWP, int nVP,
-                                                  int wb, int NP, double2 TU, int skip1) {
+                                                  int wb, int NP, double2 TU) {
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
     if (wid >= NW2 * NP) return;
@@ -2917,9 +2971,10 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
         const SdeLevelNL& L = l == 0 ? lv0 : lv1;
         p3l[l] = NV2 <= WAVE && 2 * L.gp.n3f <= WAVE;
         p3[l] = czero();
-        if (p3l[l] && in_b(m, L.gp.n3b) && !(l == 1 && skip1)) {
+        if (p3l[l] && in_b(m, L.gp.n3b) && !(l == 1 && SKIP1)) {
             const int pc = L.chanNP > 1 ? p : 0;
-            p3[l] = L.gp.k3f ? sde_p3_lanes<true>(L.gp, pc, m, pp, nVP, n2f, lane)
+            const bool f32 = (l == 0 && F0 >= 0) ? F0 == 1 : L.gp.k3f != 0;
+            p3[l] = f32 ? sde_p3_lanes<true>(L.gp, pc, m, pp, nVP, n2f, lane)
                              : sde_p3_lanes<false>(L.gp, pc, m, pp, nVP, n2f, lane);
         }
     }
@@ -2934,7 +2989,7 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
             xp = cadd(xp, cadd(cmul(cadd(k1p[l], P.k2(m, v)), spp), k2p[l]));
             xh = cadd(xh, cadd(cmul(cadd(k1h[l], cadd(A.k2(m, v), T.k2(m, v))), sph), k2h[l]));
             const int n3p = L.gp.n3f, n3h = max(L.ga.n3f, L.gt.n3f);
-            if (l == 1 && skip1) continue;
+            if (l == 1 && SKIP1) continue;
             // the K3 form is the level's (one vertex): the loops branch on it once
             if (p3l[l]) {
                 if (in_b(m, L.gp.n3b) && in_f(v, n3p)) xp = cadd(xp, p3[l]);
@@ -2948,7 +3003,8 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
                     }
                     return sp;
                 };
-                xp = cadd(xp, L.gp.k3f ? psum(std::true_type{}) : psum(std::false_type{}));
+                const bool f32 = (l == 0 && F0 >= 0) ? F0 == 1 : L.gp.k3f != 0;
+                xp = cadd(xp, f32 ? psum(std::true_type{}) : psum(std::false_type{}));
             }
             if ((in_b(m, L.ga.n3b) || in_b(m, L.gt.n3b)) && in_f(v, n3h)) {
                 auto hsum = [&](auto form) {
@@ -2960,7 +3016,8 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
                             sh = cadd(sh, cmul(cadd(A.template k3p<K32>(m, v, n), T.template k3p<K32>(m, v, n)), ph[n]));
                     return sh;
                 };
-                xh = cadd(xh, L.ga.k3f ? hsum(std::true_type{}) : hsum(std::false_type{}));
+                const bool f32 = (l == 0 && F0 >= 0) ? F0 == 1 : L.ga.k3f != 0;
+                xh = cadd(xh, f32 ? hsum(std::true_type{}) : hsum(std::false_type{}));
             }
         }
         const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
