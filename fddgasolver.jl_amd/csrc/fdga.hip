@@ -612,13 +612,16 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
             // over the momenta after the pass
             const bool refgemm = lv2->chanNP == 1 && c->NP > 1;
             // (the target level's K3 form fixed at compile time: one load form in the loops)
-            if (refgemm && l0.gp.k3f && l0.ga.k3f && l0.gt.k3f)
+            // (with it fixed, the target level's a/t K3 sum runs in k_sde_h3 after the pass)
+            if (refgemm && l0.gp.k3f && l0.ga.k3f && l0.gt.k3f) {
                 k_sde_L_nl<2, true, 1><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f,
                                                                d.nK1, c->nVP, c->wb, c->NP, TU);
-            else if (refgemm && !l0.gp.k3f && !l0.ga.k3f && !l0.gt.k3f)
+                k_sde_h3<true><<<nb, 256, 0, c->run>>>(c->Lph, Ph, l0.ga, l0.gt, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, TU);
+            } else if (refgemm && !l0.gp.k3f && !l0.ga.k3f && !l0.gt.k3f) {
                 k_sde_L_nl<2, true, 0><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f,
                                                                d.nK1, c->nVP, c->wb, c->NP, TU);
-            else
+                k_sde_h3<false><<<nb, 256, 0, c->run>>>(c->Lph, Ph, l0.ga, l0.gt, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, TU);
+            } else
                 k_sde_L_nl<2><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f, d.nK1,
                                                       c->nVP, c->wb, c->NP, TU);
             LAUNCHCHK(c);

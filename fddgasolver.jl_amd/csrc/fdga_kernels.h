@@ -2914,7 +2914,8 @@ struct SdeRefE {   // L(Ω, ν, P) += T U C(ν, P) inside the L box
 // NLEV Vertex levels summed into one pair of L-vertices (the target vertex and the local reference
 // vertex of the fdPA iteration share ΣΠ, the bubble rows and the launch).  SKIP1: the second level's
 // K3 terms are left out (added by the momentum GEMMs SdeRef*, when that level is momentum-independent).
-// F0: the first level's K3 form fixed at compile time (0 FP64, 1 FP32; -1: read from the level)
+// F0: the first level's K3 form fixed at compile time (0 FP64, 1 FP32; -1: read from the level); with
+// a fixed form the first level's a/t K3 sum is left to k_sde_h3 (a lean kernel of its own)
 template <int NLEV, bool SKIP1 = false, int F0 = -1>
 __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, double2* __restrict__ Lph,
                                                   const double2* __restrict__ Pipp, const double2* __restrict__ Piph,
@@ -3006,7 +3007,7 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
                 const bool f32 = (l == 0 && F0 >= 0) ? F0 == 1 : L.gp.k3f != 0;
                 xp = cadd(xp, f32 ? psum(std::true_type{}) : psum(std::false_type{}));
             }
-            if ((in_b(m, L.ga.n3b) || in_b(m, L.gt.n3b)) && in_f(v, n3h)) {
+            if (!(l == 0 && F0 >= 0) && (in_b(m, L.ga.n3b) || in_b(m, L.gt.n3b)) && in_f(v, n3h)) {
                 auto hsum = [&](auto form) {
                     constexpr bool K32 = decltype(form)::value;
                     double2 sh = czero();
@@ -3023,6 +3024,35 @@ __global__ __launch_bounds__(256) void k_sde_L_nl(double2* __restrict__ Lpp, dou
         const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
         Lpp[o] = cmul(xp, TU);
         Lph[o] = cmul(xh, TU);
+    }
+}
+
+// The a/t K3 sum of a lattice Vertex level at momentum P, added to Lph (the part k_sde_L_nl<.., F0>
+// leaves out): Lph(Ω, ν, P) += T U Σ_n [K3a + K3t](Ω, ν, n, P) Πph(Ω, n, P), the sum in k_sde_L_nl's
+// order.  One wave per (Ω, P), momentum fastest, lanes over ν; few registers, so many waves in flight.
+template <bool K32>
+__global__ __launch_bounds__(256) void k_sde_h3(double2* __restrict__ Lph, const double2* __restrict__ Piph, ChanV ga,
+                                                ChanV gt, int n2b, int n2f, int nWP, int nVP, int NP, double2 TU) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
+    if (wid >= NW2 * NP) return;
+    const int p = wid % NP, iW = wid / NP;
+    const int m = iW - (n2b - 1);
+    if (!in_b(m, ga.n3b) && !in_b(m, gt.n3b)) return;
+    const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
+    const double2* ph = Piph + (size_t)(2 * nVP) * row + nVP;
+    const ChanBox A{ga, p}, T{gt, p};
+    const int n3h = max(ga.n3f, gt.n3f);
+    for (int iv = lane; iv < NV2; iv += WAVE) {
+        const int v = iv - n2f;
+        if (!in_f(v, n3h)) continue;
+        double2 sh = czero();
+#pragma unroll 8
+        for (int n = -n3h; n < n3h; ++n)
+            if (n >= -nVP && n < nVP)
+                sh = cadd(sh, cmul(cadd(A.template k3p<K32>(m, v, n), T.template k3p<K32>(m, v, n)), ph[n]));
+        const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
+        Lph[o] = cadd(Lph[o], cmul(sh, TU));
     }
 }
 
