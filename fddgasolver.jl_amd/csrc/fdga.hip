@@ -621,9 +621,13 @@ int sde_level_L(fdga_ctx* c, const Level& lv, const double2* Pp, const double2* 
                 k_sde_L_nl<2, true, 0><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f,
                                                                d.nK1, c->nVP, c->wb, c->NP, TU);
                 k_sde_h3<false><<<nb, 256, 0, c->run>>>(c->Lph, Ph, l0.ga, l0.gt, d.nK2b, d.nK2f, d.nK1, c->nVP, c->NP, TU);
-            } else
+            } else if (refgemm) {   // (the reference level's K3 terms still in the GEMMs below)
+                k_sde_L_nl<2, true><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f,
+                                                            d.nK1, c->nVP, c->wb, c->NP, TU);
+            } else {
                 k_sde_L_nl<2><<<nb, 256, 0, c->run>>>(c->Lpp, c->Lph, Pp, Ph, Tpp, Tph, l0, l1, d.nK2b, d.nK2f, d.nK1,
                                                       c->nVP, c->wb, c->NP, TU);
+            }
             LAUNCHCHK(c);
             if (refgemm) {
                 const ChanV& gp = lv2->g[PCH];
