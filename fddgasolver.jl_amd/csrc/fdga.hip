@@ -850,6 +850,17 @@ static int launch_k2_nl(fdga_ctx* c) {
         c->Fb[CH].K2, c->Fb[ACH].K2, c->k2y, c->k2xf, c->Wc[CH], wtail(c, CH), vtx_F(c), chanF(c, c->FL[CH]),
         chanF(c, c->FL[ACH]), c->d.nK2b, c->d.nK2f, c->d.nK1, c->wb, c->NP, c->d.T * kSign[CH], raw);
     LAUNCHCHK(c);
+    {   // the own-K3 product term (T s Σ_ω K3 W; halved for a finished t, whose post-processing halves)
+        const VtxV F = vtx_F(c);
+        double2* out = raw ? raw : c->Fb[CH].K2;
+        const double f = c->d.T * kSign[CH] * ((CH == TCH && !raw) ? 0.5 : 1.0);
+        const unsigned nb = (unsigned)cdiv((size_t)NW2 * c->NP * WAVE, 256);
+        if (F.g[CH].k3f)
+            k_k2nl_k3<CH, true><<<nb, 256, 0, c->run>>>(out, c->Wc[CH], F, c->d.nK2b, c->d.nK2f, c->d.nK1, c->wb, c->NP, f);
+        else
+            k_k2nl_k3<CH, false><<<nb, 256, 0, c->run>>>(out, c->Wc[CH], F, c->d.nK2b, c->d.nK2f, c->d.nK1, c->wb, c->NP, f);
+        LAUNCHCHK(c);
+    }
     if (rp) {
         k_k2nl_symfinish<CH><<<cdiv(n2, 256), 256, 0, c->run>>>(c->Fb[CH].K2, c->Fb[ACH].K2, raw, rp, c->FL[CH].K2,
                                                               c->FL[ACH].K2, n2);

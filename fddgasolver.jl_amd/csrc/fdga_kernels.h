@@ -2646,7 +2646,9 @@ __device__ __forceinline__ double2 k2_add_fl(double2 val, double2 lr, double2 la
 // BSE_K2! finish of the lattice solver, one wave per (Ω, P):
 //   K2 = T s [Y + Xf_L Wt_L + Xf_R Wt_R + K2r(Ω, ν, P) (Σ_band W + Wt_L + Wt_R) + Σ_ω K3r(Ω, ν, ω, P) W(ω)]
 //        + FL.K2 (t: (... + 2 FL.t.K2 - FL.a.K2 + K2a) / 2)
-// with Wt_L, Wt_R the sums of W beyond the band and K3r the own K3 (t: 2 K3t - K3a).
+// with Wt_L, Wt_R the sums of W beyond the band and K3r the own K3 (t: 2 K3t - K3a).  The K3 product
+// term is added afterwards by k_k2nl_k3 (a lean kernel of its own; K2 is linear in it, with weight
+// T s, halved for t).
 template <int CH>
 __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, const double2* __restrict__ K2a,
                                                      const double2* __restrict__ Y, const double2* __restrict__ Xf,
@@ -2665,32 +2667,45 @@ __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, c
     sr = wave_sum(sr); si = wave_sum(si);
     const double2 wl = Wtail.at(row, 0, wb), wr = Wtail.at(row, (wb + 1) + wb, wb);
     const double2 wtot = cadd(cmk(sr, si), cadd(wl, wr));
-    const ChanV& g = F.g[CH];
-    const bool k3m = in_b(m, g.n3b);
     for (int iv = lane; iv < NV2; iv += WAVE) {
         const int v = iv - n2f;
         double2 acc = Y[(size_t)iv + (size_t)NV2 * (iW + (size_t)NW2 * p)];
         const double2* xf = Xf + (size_t)iv + (size_t)NV2 * 2 * iW;
         acc = cadd(acc, cadd(cmul(xf[0], wl), cmul(xf[NV2], wr)));
         acc = cadd(acc, cmul(k2d_rank1<CH>(F, m, v, p), wtot));
-        if (k3m && in_f(v, g.n3f)) {
-            // the three channels of F share one K3 form
-            auto wsum = [&](auto form) {
-                constexpr bool K32 = decltype(form)::value;
-                double2 k3s = czero();
-#pragma unroll 8
-                for (int w = -g.n3f; w < g.n3f; ++w) {
-                    double2 k3 = g.template k3p<K32>(m, v, w, p);
-                    if (CH == TCH) k3 = csub(cscale(k3, 2.0), F.g[ACH].template k3p<K32>(m, v, w, p));
-                    k3s = cadd(k3s, cmul(k3, Wr[w]));
-                }
-                return k3s;
-            };
-            acc = cadd(acc, g.k3f ? wsum(std::true_type{}) : wsum(std::false_type{}));
-        }
         const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
         if (raw) { raw[o] = cscale(acc, scale); continue; }
         K2[o] = k2_add_fl<CH>(cscale(acc, scale), Lr.K2[o], La.K2[o], K2a[o]);
+    }
+}
+
+// The K3 product term of the lattice BSE_K2! finish, added to its output (K2, or the raw values under a
+// symmetry group): out += f Σ_ω K3r(Ω, ν, ω, P) W(ω) with f = T s (halved for t; t: K3r = 2 K3t - K3a).
+// One wave per (Ω, P) as the finish, lanes over ν; K32: F.K3 held in FP32.
+template <int CH, bool K32>
+__global__ __launch_bounds__(256) void k_k2nl_k3(double2* __restrict__ out, const double2* __restrict__ W, VtxV F,
+                                                 int n2b, int n2f, int nWP, int wb, int NP, double f) {
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int NW2 = 2 * n2b - 1, NV2 = 2 * n2f;
+    if (wid >= NW2 * NP) return;
+    const int iW = wid % NW2, p = wid / NW2;
+    const int m = iW - (n2b - 1);
+    const ChanV& g = F.g[CH];
+    if (!in_b(m, g.n3b)) return;
+    const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
+    const double2* Wr = W + (size_t)(2 * wb) * row + wb;   // Wr[ω], ω in [-wb, wb)
+    for (int iv = lane; iv < NV2; iv += WAVE) {
+        const int v = iv - n2f;
+        if (!in_f(v, g.n3f)) continue;
+        double2 k3s = czero();
+#pragma unroll 8
+        for (int w = -g.n3f; w < g.n3f; ++w) {
+            double2 k3 = g.template k3p<K32>(m, v, w, p);
+            if (CH == TCH) k3 = csub(cscale(k3, 2.0), F.g[ACH].template k3p<K32>(m, v, w, p));
+            k3s = cadd(k3s, cmul(k3, Wr[w]));
+        }
+        const size_t o = (size_t)iW + (size_t)NW2 * ((size_t)iv + (size_t)NV2 * p);
+        out[o] = cadd(out[o], cscale(k3s, f));
     }
 }
 
