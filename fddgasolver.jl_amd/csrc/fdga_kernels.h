@@ -362,6 +362,43 @@ __global__ __launch_bounds__(NT, NT >= 512 ? (sizeof(CT) == 8 ? 6 : 4) : 1) void
             tw[k] = cmk_as<CT>(cs, sn);
         }
     for (int which = 0; which < 2; ++which) {
+        // Far from ω = 0 one of each product's two frequencies lies outside G's mesh for every ω of
+        // the block: then every product vanishes except the origin's (R = 0, the asymptotic G on both
+        // sides), which folds onto R' = 0 alone, and the transform of that delta is the constant
+        // Π_R(ω, R' = 0) at every momentum (exactly: the radix-4 steps add zeros and multiply zeros).
+        // Such blocks store that constant and its chunk sums without the products and transforms:
+        // the same values (up to the sign of zeros).  (L = 16, the radix-4 form.)
+        bool far = LT == 16;
+        if (LT == 16)
+            for (int wl = 0; wl < nb; ++wl) {
+                const int n = iw0 + wl - nV, n2nd = which == 0 ? bmf(m, n) : bpf(m, n);
+                far = far && (!in_f(n, nG) || !in_f(n2nd, nG));
+            }
+        if (far) {
+            if (tid < NB)
+                s[tid] = tid < nb ? bubble_R(GR, folds[0], nG, T, m, iw0 + tid - nV, which) : cmk_as<CT>(0.0, 0.0);
+            __syncthreads();
+            if ((int)blockIdx.x >= wlo && (int)blockIdx.x < whi) {
+                double2* out = which == 0 ? pp : ph;
+                const size_t qs = (size_t)(2 * nV) * (size_t)(2 * nW - 1);
+                for (int e = tid; e < NB * LL; e += blockDim.x) {
+                    const int wl = e % NB, P = e / NB;
+                    if (wl < nb) out[(size_t)(iw0 + wl) + (size_t)(2 * nV) * (size_t)iW + qs * (size_t)P] = cmk(s[wl].x, s[wl].y);
+                }
+            }
+            double2* cs = which == 0 ? cpp : cph;
+            if (cs)
+                for (int P = tid; P < LL; P += blockDim.x) {
+                    double sr = 0.0, si = 0.0;
+                    for (int j = 0; j < NB; ++j) {   // the order of the transformed block's sums below
+                        const int wl = (j + P) % NB;
+                        if (wl < nb) { const CT v = s[wl]; sr += v.x; si += v.y; }
+                    }
+                    cs[(size_t)P + (size_t)LL * (blockIdx.x + (size_t)gridDim.x * iW)] = cmk(sr, si);
+                }
+            __syncthreads();   // s is reused by the next products
+            continue;
+        }
         // products (a compile-time trip count when L and NB are fixed, so that the loads of
         // several products are in flight together)
         if (LT && NBT) {
