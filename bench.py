@@ -180,7 +180,10 @@ def max_relerr(a, b):
 
 def time_path(S, args, names):
     """warmup + timed iterations (HIP events on the solver's stream; replayed from the captured
-    hipGraph of one iteration unless --eager), then a per-stage pass (eager, one stage at a time)."""
+    hipGraph of one iteration unless --eager), then a per-stage pass (eager, one stage at a time).
+    The per-stage pass runs the contractions with one workgroup per work item: a persistent grid
+    tuned to leave compute units to the iteration's other chain has no other chain to serve there,
+    and the kernels alone are what the stage times and the roofline describe."""
     import fdga
     S.set_graph(not args.eager)
     for _ in range(args.warmup):
@@ -188,6 +191,9 @@ def time_path(S, args, names):
     S.sync()
     ms = S.time_iterations("fdPA", args.steps)
     S.set_graph(False)
+    grids = S.gemm_grids()
+    S.set_gemm_grid(True, 0)
+    S.set_gemm_grid(False, 0)
     nst = max(1, min(args.steps, 5))
     S.set_stage_timing(True)
     for _ in range(nst):
@@ -195,6 +201,8 @@ def time_path(S, args, names):
     S.sync()
     st = [x / nst for x in S.stage_ms()]
     S.set_stage_timing(False)
+    S.set_gemm_grid(True, grids[0])
+    S.set_gemm_grid(False, grids[1])
     return ms / args.steps, dict(zip(names, st))
 
 
