@@ -141,6 +141,22 @@ def test_full_iteration_band_narrower_than_bubble_mesh(case, strategy, fdga_mod)
         assert relerr(x, y) < TOL, f"{case} {strategy} iteration {it}: {relerr(x, y):.3e}"
 
 
+@pytest.mark.parametrize("nK1, mPinu", [(64, 4), (100, 2)], ids=["band131", "band200"])
+def test_full_iteration_wide_band(nK1, mPinu, fdga_mod):
+    """Band half-widths beyond one wave: 131 entries per side (three 64-entry scan steps held in
+    registers, cut chunks on both sides) and 200 (the whole bubble mesh, scanned step by step from
+    memory); the lattice tail sums feed every K1/K2 gather of the iteration."""
+    box = dict(BOX, nK1=nK1, mPinu=mPinu, nG=40)
+    S, O = make_pair("nl_L2", seed=9, box=box)
+    for it in range(2):
+        fdga_mod.iterate_solver(S, "fdPA")
+        O.iterate(po.FDPA)
+        x = fdga_mod.flatten(S)
+        y = O.flatten()
+        assert relerr(x, y) < TOL, f"nK1={nK1} mPinu={mPinu} iteration {it}: {relerr(x, y):.3e}"
+    S.close()
+
+
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
 def test_band_stored_bubble_read_back(graph, fdga_mod):
     """The iteration's lattice bubble stores Πpp/Πph only on the 16-ω chunks that hold the band
