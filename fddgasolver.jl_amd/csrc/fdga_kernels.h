@@ -2699,10 +2699,10 @@ __global__ __launch_bounds__(256) void k_k2nl_finish(double2* __restrict__ K2, c
     const int m = iW - (n2b - 1);
     const size_t row = (size_t)(m + nWP - 1) + (size_t)(2 * nWP - 1) * p;
     const double2* Wr = W + (size_t)(2 * wb) * row + wb;   // Wr[ω], ω in [-wb, wb)
+    const double2 wl = Wtail.at(row, 0, wb), wr = Wtail.at(row, (wb + 1) + wb, wb);   // issued first
     double sr = 0.0, si = 0.0;
     for (int k = lane; k < 2 * wb; k += WAVE) { const double2 x = Wr[k - wb]; sr += x.x; si += x.y; }
     sr = wave_sum(sr); si = wave_sum(si);
-    const double2 wl = Wtail.at(row, 0, wb), wr = Wtail.at(row, (wb + 1) + wb, wb);
     const double2 wtot = cadd(cmk(sr, si), cadd(wl, wr));
     for (int iv = lane; iv < NV2; iv += WAVE) {
         const int v = iv - n2f;
