@@ -2044,13 +2044,12 @@ __device__ __forceinline__ void bse_w_elem(size_t e, const WArgs& a, const VtxV&
         const double2 pi = Pi[o];
         W[(size_t)t + (size_t)(2 * wb) * row] = cadd(cmul(csub(pi, Pi0[o]), f0r), cmul(pi, flr));
     }
-    if (t == 0)
-        for (int side = 0; side < 2; ++side) {
-            const int wf = side ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
-            const int wc = crossing(m, wf, ch);
-            a.far[4 * row + 2 * side] = f0.eval(m, wc, FDGA_INF, ch, sp);
-            a.far[4 * row + 2 * side + 1] = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
-        }
+    for (int side = t; side < 2; side += per) {   // the row's far factors, one side per thread
+        const int wf = side ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
+        const int wc = crossing(m, wf, ch);
+        a.far[4 * row + 2 * side] = f0.eval(m, wc, FDGA_INF, ch, sp);
+        a.far[4 * row + 2 * side + 1] = FL.eval(m, wc, FDGA_INF, p, ch, sp, FL_ALL);
+    }
 }
 template <int CH>
 __global__ __launch_bounds__(256) void k_bse_w(WArgs a, VtxV FL, F0V f0, int nWP, int nVP, int wb, int NK, size_t nrows) {
@@ -2085,17 +2084,17 @@ __device__ __forceinline__ void bse_w_pchunk(size_t e, const WArgs& a, const Vtx
         const double2 pi = Pi[o];
         W[(size_t)t + (size_t)(2 * wb) * row] = cadd(cmul(csub(pi, Pi0[o]), f0r), cmul(pi, flr));
     }
-    if (t == 0)
-        for (int side = 0; side < 2; ++side) {
-            const int wf = side ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
-            const int wcf = crossing(m, wf, ch);
-            const double2 f0f = f0.eval(m, wcf, FDGA_INF, ch, sp);
-            for (int p = p0; p < p1; ++p) {
-                const size_t row = (size_t)iW + (size_t)NWP * p;
-                a.far[4 * row + 2 * side] = f0f;
-                a.far[4 * row + 2 * side + 1] = FL.eval(m, wcf, FDGA_INF, p, ch, sp, FL_ALL);
-            }
-        }
+    // the rows' far factors (F0 and FL at the first ω beyond every box, on each side): one (side,
+    // momentum) pair per thread of the column, so that no thread evaluates them in a chain
+    for (int j = t; j < 2 * W_PCH; j += per) {
+        const int side = j / W_PCH, p = p0 + j % W_PCH;
+        if (p >= p1) continue;
+        const int wf = side ? NK + abs(m) + 3 : -(NK + abs(m) + 3);
+        const int wcf = crossing(m, wf, ch);
+        const size_t row = (size_t)iW + (size_t)NWP * p;
+        a.far[4 * row + 2 * side] = f0.eval(m, wcf, FDGA_INF, ch, sp);
+        a.far[4 * row + 2 * side + 1] = FL.eval(m, wcf, FDGA_INF, p, ch, sp, FL_ALL);
+    }
 }
 // the three channels' weights in one launch (nb blocks each, in the order p, t, a); pchunk: one
 // thread per (band entry, Ω, chunk of W_PCH momenta)
