@@ -351,6 +351,7 @@ int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale
     }
     const int LL = L * L;
     int RB = std::max(1, std::min(16, 1024 / LL));
+    while (RB > 1 && cdiv(nrows, (size_t)RB) < 512) RB /= 2;   // at least two workgroups per compute unit
     size_t smem = (size_t)(2 * RB * LL + L) * sizeof(double2);
     k_dft2_rows<<<cdiv(nrows, RB), 256, smem, c->run>>>(data, nrows, L, RB, sign, scale);
     LAUNCHCHK(c);

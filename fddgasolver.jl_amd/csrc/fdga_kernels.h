@@ -131,7 +131,9 @@ __global__ void k_bubble_local(double2* __restrict__ pp, double2* __restrict__ p
 
 // ---------------------------------------------------------------------------------------
 // 2D DFT over the trailing L x L plane of rows: data[r + nrows*(j1 + L*j2)], in place.
-// One workgroup per block of RB rows, two LDS planes (in/out) plus the L twiddles;
+// One workgroup per block of RB rows (fewer rows per block when that leaves the GPU short of
+// workgroups; every output is the same sum in the same order either way), two LDS planes (in/out)
+// plus the L twiddles;
 // sign -1 = fft, +1 = bfft (FFTW conventions of src/nonlocal/bubble.jl:110,154); `scale`
 // multiplies the result.
 __global__ void k_dft2_rows(double2* __restrict__ data, size_t nrows, int L, int RB, int sign, double scale) {
@@ -147,9 +149,16 @@ __global__ void k_dft2_rows(double2* __restrict__ data, size_t nrows, int L, int
         sincospi(sign * 2.0 * k / L, &sn, &cs);
         tw[k] = cmk(cs, sn);
     }
-    for (int e = threadIdx.x; e < RB * LL; e += blockDim.x) {
-        int r = e % RB, j = e / RB;
-        s0[e] = (r < nrb) ? data[r0 + r + nrows * (size_t)j] : czero();
+    for (int e0 = threadIdx.x; e0 < RB * LL; e0 += 4 * blockDim.x) {   // four loads in flight per thread
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * blockDim.x, r = e % RB, j = e / RB;
+            v[u] = (e < RB * LL && r < nrb) ? data[r0 + r + nrows * (size_t)j] : czero();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * blockDim.x < RB * LL) s0[e0 + u * blockDim.x] = v[u];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < RB * LL; e += blockDim.x) {   // along j1
