@@ -341,8 +341,12 @@ int ensure_stage(fdga_ctx* c, size_t n) {
     return 0;
 }
 
-int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale) {
+// the 2-D transform of src's rows into data (src = nullptr: in place)
+int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale, const double2* src = nullptr) {
+    if (!src) src = data;
     if (L == 1) {
+        if (src != data)
+            HIPCHK(c, hipMemcpyAsync(data, src, nrows * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
         if (scale != 1.0) {
             k_axpby<<<cdiv(nrows, 256), 256, 0, c->run>>>(data, data, scale, 0.0, nrows);
             LAUNCHCHK(c);
@@ -353,7 +357,7 @@ int dft2(fdga_ctx* c, double2* data, size_t nrows, int L, int sign, double scale
     int RB = std::max(1, std::min(16, 1024 / LL));
     while (RB > 1 && cdiv(nrows, (size_t)RB) < 512) RB /= 2;   // at least two workgroups per compute unit
     size_t smem = (size_t)(2 * RB * LL + L) * sizeof(double2);
-    k_dft2_rows<<<cdiv(nrows, RB), 256, smem, c->run>>>(data, nrows, L, RB, sign, scale);
+    k_dft2_rows<<<cdiv(nrows, RB), 256, smem, c->run>>>(data, src, nrows, L, RB, sign, scale);
     LAUNCHCHK(c);
     return 0;
 }
@@ -425,8 +429,7 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc, boo
         return 0;
     }
     // G(R) = fft(G)/LG^2 (src/nonlocal/bubble.jl:110)
-    HIPCHK(c, hipMemcpyAsync(c->tmpG1, Gsrc, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
-    if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG))) return -3;
+    if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG), Gsrc)) return -3;
     // the Π_R products and the backward DFT over R' (src/nonlocal/bubble.jl:110-155) in one pass
     const int L = d.L, NB = bubble_nb(L);
     const dim3 grid(cdiv(c->NVP, NB), c->NWP);
@@ -528,13 +531,10 @@ int sde_sigma(fdga_ctx* c, double2* S, const double2* G, double scale) {
         return 0;
     }
     // real-space evaluation (src/nonlocal/SDE.jl:209-280)
-    HIPCHK(c, hipMemcpyAsync(c->tmpG1, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
-    if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG))) return -3;
-    HIPCHK(c, hipMemcpyAsync(c->tmpL1, c->Lpp, c->nL * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
-    HIPCHK(c, hipMemcpyAsync(c->tmpL2, c->Lph, c->nL * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
+    if (dft2(c, c->tmpG1, c->NG, d.LG, -1, 1.0 / ((double)d.LG * d.LG), G)) return -3;
     size_t nrow = (size_t)c->NW2 * c->NV2;
-    if (dft2(c, c->tmpL1, nrow, d.L, -1, 1.0 / ((double)d.L * d.L))) return -3;
-    if (dft2(c, c->tmpL2, nrow, d.L, -1, 1.0 / ((double)d.L * d.L))) return -3;
+    if (dft2(c, c->tmpL1, nrow, d.L, -1, 1.0 / ((double)d.L * d.L), c->Lpp)) return -3;
+    if (dft2(c, c->tmpL2, nrow, d.L, -1, 1.0 / ((double)d.L * d.L), c->Lph)) return -3;
     k_sde_sigma_nl_R<<<cdiv((size_t)c->NG * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->tmpG2, c->tmpG1, c->tmpL1, c->tmpL2,
                                                                                  d.nG, d.nK2b, d.nK2f, d.LG, d.L, d.T);
     LAUNCHCHK(c);
@@ -558,10 +558,8 @@ int sde_u2(fdga_ctx* c, double2* S, const double2* G, bool c_ready = false) {
         return 0;
     }
     const int LG = d.LG;
-    HIPCHK(c, hipMemcpyAsync(c->tmpG1, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
-    HIPCHK(c, hipMemcpyAsync(c->tmpG2, G, c->nGel * sizeof(double2), hipMemcpyDeviceToDevice, c->run));
-    if (dft2(c, c->tmpG1, c->NG, LG, -1, 1.0 / ((double)LG * LG))) return -3;   // G(+R)
-    if (dft2(c, c->tmpG2, c->NG, LG, +1, 1.0 / ((double)LG * LG))) return -3;   // G(-R)
+    if (dft2(c, c->tmpG1, c->NG, LG, -1, 1.0 / ((double)LG * LG), G)) return -3;   // G(+R)
+    if (dft2(c, c->tmpG2, c->NG, LG, +1, 1.0 / ((double)LG * LG), G)) return -3;   // G(-R)
     // four outputs per wave (NG = 2 nG is even; the last C wave may hold fewer than four m)
     k_u2_C4<<<cdiv((size_t)cdiv(NB, 4) * c->NkG * WAVE, 256), 256, 0, c->run>>>(c->u2C, c->tmpG2, c->tmpG1, d.nG, c->NkG);
     if (c->NG % 4 == 0)

@@ -133,10 +133,11 @@ __global__ void k_bubble_local(double2* __restrict__ pp, double2* __restrict__ p
 // 2D DFT over the trailing L x L plane of rows: data[r + nrows*(j1 + L*j2)], in place.
 // One workgroup per block of RB rows (fewer rows per block when that leaves the GPU short of
 // workgroups; every output is the same sum in the same order either way), two LDS planes (in/out)
-// plus the L twiddles;
-// sign -1 = fft, +1 = bfft (FFTW conventions of src/nonlocal/bubble.jl:110,154); `scale`
-// multiplies the result.
-__global__ void k_dft2_rows(double2* __restrict__ data, size_t nrows, int L, int RB, int sign, double scale) {
+// plus the L twiddles; sign -1 = fft, +1 = bfft (FFTW conventions of src/nonlocal/bubble.jl:110,154);
+// `scale` multiplies the result.  The input rows are read from src, which is data itself (in place)
+// or another array of the same shape: a workgroup reads all of its rows into LDS before it writes
+// them, and no two workgroups share a row.
+__global__ void k_dft2_rows(double2* data, const double2* src, size_t nrows, int L, int RB, int sign, double scale) {
     extern __shared__ double2 sm[];   // 2 * RB * L * L + L
     const int LL = L * L;
     double2* s0 = sm;
@@ -154,7 +155,7 @@ __global__ void k_dft2_rows(double2* __restrict__ data, size_t nrows, int L, int
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int e = e0 + u * blockDim.x, r = e % RB, j = e / RB;
-            v[u] = (e < RB * LL && r < nrb) ? data[r0 + r + nrows * (size_t)j] : czero();
+            v[u] = (e < RB * LL && r < nrb) ? src[r0 + r + nrows * (size_t)j] : czero();
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
