@@ -462,8 +462,7 @@ int bubbles_into(fdga_ctx* c, double2* pp, double2* ph, const double2* Gsrc, boo
     LAUNCHCHK(c);
     // tail sums from the bubble's chunk sums
     const unsigned nblk = (unsigned)c->NWP * (unsigned)cdiv(c->NP, TAIL_NPB);
-    for (int k = 0; k < 2; ++k)
-        k_tail_sums_chunked<<<nblk, 256, 0, c->run>>>(k ? ph : pp, k ? cph : cpp, pi_tail(c, k ? ph : pp), c->nVP, c->wb,
+    k_tail_sums_chunked<<<2 * nblk, 256, 0, c->run>>>(pp, cpp, pi_tail(c, pp), ph, cph, pi_tail(c, ph), nblk, c->nVP, c->wb,
                                                       NB, (int)grid.x, c->NWP, c->NP);
     LAUNCHCHK(c);
     return 0;

@@ -2212,13 +2212,21 @@ __global__ __launch_bounds__(256) void k_tail_sums2(const double2* __restrict__ 
 // line), the 32 thread groups' partial sums are added in a fixed order, and each wave then scans
 // the band of two rows.
 constexpr int TAIL_NPB = 8;
-__global__ __launch_bounds__(256) void k_tail_sums_chunked(const double2* __restrict__ X, const double2* __restrict__ C,
-                                                           double2* __restrict__ T, int nVP, int wb, int NB, int NC,
-                                                           int NWP, int NP) {
+// Both arrays of a bubble (Πpp, Πph) in one launch: blocks [0, nblk) the first, the rest the second.
+__global__ __launch_bounds__(256) void k_tail_sums_chunked(const double2* __restrict__ X0, const double2* __restrict__ C0,
+                                                           double2* __restrict__ T0, const double2* __restrict__ X1,
+                                                           const double2* __restrict__ C1, double2* __restrict__ T1,
+                                                           unsigned nblk, int nVP, int wb, int NB, int NC, int NWP,
+                                                           int NP) {
     __shared__ double2 red[2][32][TAIL_NPB + 1];
     __shared__ double2 far[2][TAIL_NPB];
+    const bool second = blockIdx.x >= nblk;
+    const double2* __restrict__ X = second ? X1 : X0;
+    const double2* __restrict__ C = second ? C1 : C0;
+    double2* __restrict__ T = second ? T1 : T0;
+    const unsigned bid = blockIdx.x - (second ? nblk : 0u);
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-    const int iW = (int)(blockIdx.x % NWP), p0 = (int)(blockIdx.x / NWP) * TAIL_NPB;
+    const int iW = (int)(bid % NWP), p0 = (int)(bid / NWP) * TAIL_NPB;
     const int nout = nVP - wb;                 // far left: ω index < nout; far right: >= nVP + wb
     const int cl = nout / NB;                  // chunks [0, cl) lie wholly left of the band
     const int cr = (nVP + wb + NB - 1) / NB;   // chunks [cr, NC) lie wholly right of it
